@@ -1,0 +1,202 @@
+"""MI355X-native Box2D v3 world step (forge2d ``World.step``) — Python host binding.
+
+The product is ``libforge2d_amd.so`` (HIP kernels for gfx950 behind a plain
+C-ABI, ``include/forge2d_amd.h``). This module loads it with ctypes and offers
+a thin batch wrapper used by the tests, the benchmark and the serving loop.
+There is no CPU fallback: if the library is missing or cannot load, every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import scenes, snapshot  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libforge2d_amd.so")
+
+PH_PAIRS, PH_COLLIDE, PH_STATE, PH_SOLVE, PH_FINALIZE = 1, 2, 4, 8, 16
+PH_ALL = 31
+
+_lib = None
+
+# (name, restype, argtypes) of every exported symbol declared in include/forge2d_amd.h
+_VP, _I, _L, _F, _U = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_float, ctypes.c_uint32
+_PI = ctypes.POINTER(ctypes.c_int32)
+_PF = ctypes.POINTER(ctypes.c_float)
+SIGNATURES = [
+    ("f2dg_create_from_spec", _VP, [_VP, _L]),
+    ("f2dg_destroy", None, [_VP]),
+    ("f2dg_world_count", _I, [_VP]),
+    ("f2dg_body_count", _I, [_VP, _I]),
+    ("f2dg_shape_count", _I, [_VP, _I]),
+    ("f2dg_awake_body_count", _L, [_VP]),
+    ("f2dg_set_contact_capacity_factor", _I, [_VP, _F]),
+    ("f2dg_set_stream", _I, [_VP, _VP]),
+    ("f2dg_upload", _I, [_VP]),
+    ("f2dg_device_bytes", _L, [_VP]),
+    ("f2dg_step", _I, [_VP, _F, _I, _I, _I]),
+    ("f2dg_step_worlds", _I, [_VP, _PI, _I, _F, _I, _I]),
+    ("f2dg_step_each", _I, [_VP, _F, _I, _I]),
+    ("f2dg_step_timed", _I, [_VP, _F, _I, _I, _PF]),
+    ("f2dg_synchronize", _I, [_VP]),
+    ("f2dg_sync", _I, [_VP]),
+    ("f2dg_get_bodies", _I, [_VP, _I, _PF]),
+    ("f2dg_export_world", _L, [_VP, _I, _VP, _L]),
+    ("f2dg_import_world", _I, [_VP, _I, _VP, _L]),
+    ("f2dg_last_error", ctypes.c_char_p, [_VP]),
+    ("f2dg_world_counters", _I, [_VP, _I, _PI]),
+    ("f2d_create_world", _U, [_F, _F, _F, _F, _F, _F, _F, _F, _I, _I]),
+    ("f2d_destroy_world", None, [_U]),
+    ("f2d_world_is_valid", _I, [_U]),
+    ("f2d_world_step", None, [_U, _F, _I]),
+    ("f2d_world_set_gravity", None, [_U, _F, _F]),
+    ("f2d_world_get_gravity", None, [_U, _PF]),
+    ("f2d_world_enable_sleeping", None, [_U, _I]),
+    ("f2d_world_is_sleeping_enabled", _I, [_U]),
+    ("f2d_world_enable_continuous", None, [_U, _I]),
+    ("f2d_world_is_continuous_enabled", _I, [_U]),
+    ("f2d_create_body", None, [_U, _I, _F, _F, _F, _F, _F, _F, _F, _F, _F, _F, _F, ctypes.c_char_p, _I, _I, _I, _I,
+                               _I, _I, _PI]),
+    ("f2d_body_is_valid", _I, [ctypes.c_int32, _U]),
+    ("f2d_body_get_position", None, [ctypes.c_int32, _U, _PF]),
+    ("f2d_body_get_rotation", None, [ctypes.c_int32, _U, _PF]),
+    ("f2d_body_get_linear_velocity", None, [ctypes.c_int32, _U, _PF]),
+    ("f2d_body_get_angular_velocity", _F, [ctypes.c_int32, _U]),
+    ("f2d_body_set_linear_velocity", None, [ctypes.c_int32, _U, _F, _F]),
+    ("f2d_body_apply_force_to_center", None, [ctypes.c_int32, _U, _F, _F, _I]),
+    ("f2d_body_apply_linear_impulse_to_center", None, [ctypes.c_int32, _U, _F, _F, _I]),
+    ("f2d_body_get_world_center", None, [ctypes.c_int32, _U, _PF]),
+    ("f2d_body_get_local_center", None, [ctypes.c_int32, _U, _PF]),
+    ("f2d_body_get_mass", _F, [ctypes.c_int32, _U]),
+    ("f2d_body_get_rotational_inertia", _F, [ctypes.c_int32, _U]),
+    ("f2d_body_get_type", _I, [ctypes.c_int32, _U]),
+    ("f2d_create_circle_shape", None, [ctypes.c_int32, _U, _VP, _F, _F, _F, _PI]),
+    ("f2d_create_capsule_shape", None, [ctypes.c_int32, _U, _VP, _F, _F, _F, _F, _F, _PI]),
+    ("f2d_create_segment_shape", None, [ctypes.c_int32, _U, _VP, _F, _F, _F, _F, _PI]),
+    ("f2d_create_box_shape", None, [ctypes.c_int32, _U, _VP, _F, _F, _F, _F, _F, _F, _F, _PI]),
+    ("f2d_create_polygon_shape", _I, [ctypes.c_int32, _U, _VP, _PF, _I, _F, _PI]),
+    ("f2d_create_revolute_joint", None, [_U, ctypes.c_int32, _U, ctypes.c_int32, _U, _F, _F, _F, _F, _F, _F, _I, _F,
+                                         _F, _I, _F, _F, _I, _F, _F, _F, _I, _PI]),
+    ("f2d_world_batch", _VP, [_U]),
+]
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the HIP library once; raise loudly when it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(f"{LIB_PATH} is not built; run `python -m forge2d_amd.build` (hipcc, gfx950)")
+    try:
+        l = ctypes.CDLL(LIB_PATH)
+    except OSError as e:  # pragma: no cover - environment specific
+        raise LibraryMissing(f"cannot load {LIB_PATH}: {e}") from e
+    for name, res, args in SIGNATURES:
+        fn = getattr(l, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = l
+    return l
+
+
+class Batch:
+    """A set of independent worlds on one GPU, built from a scene spec."""
+
+    def __init__(self, spec: bytes | list, capacity_factor: float | None = None):
+        if not isinstance(spec, (bytes, bytearray)):
+            spec = scenes.to_bytes(spec)
+        self._spec = bytes(spec)
+        L = lib()
+        buf = ctypes.create_string_buffer(self._spec, len(self._spec))
+        self.h = L.f2dg_create_from_spec(ctypes.cast(buf, ctypes.c_void_p), len(self._spec))
+        if not self.h:
+            raise ValueError("scene spec rejected by the library")
+        if capacity_factor is not None:
+            L.f2dg_set_contact_capacity_factor(self.h, capacity_factor)
+        self.world_count = L.f2dg_world_count(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().f2dg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            err = lib().f2dg_last_error(self.h)
+            raise RuntimeError(f"{what} failed ({rc}): {err.decode() if err else ''}")
+
+    def upload(self):
+        self._check(lib().f2dg_upload(self.h), "upload")
+
+    def step(self, dt=1.0 / 60.0, substeps=4, steps=1, phases=PH_ALL):
+        self._check(lib().f2dg_step(self.h, dt, substeps, steps, phases), "step")
+
+    def step_worlds(self, worlds, dt=1.0 / 60.0, substeps=4, steps=1):
+        ids = np.ascontiguousarray(worlds, dtype=np.int32)
+        self._check(lib().f2dg_step_worlds(self.h, ids.ctypes.data_as(_PI), len(ids), dt, substeps, steps),
+                    "step_worlds")
+
+    def step_each(self, dt=1.0 / 60.0, substeps=4, steps=1):
+        self._check(lib().f2dg_step_each(self.h, dt, substeps, steps), "step_each")
+
+    def step_timed(self, dt=1.0 / 60.0, substeps=4, steps=1) -> float:
+        ms = ctypes.c_float(0.0)
+        self._check(lib().f2dg_step_timed(self.h, dt, substeps, steps, ctypes.byref(ms)), "step_timed")
+        return ms.value
+
+    def synchronize(self):
+        self._check(lib().f2dg_synchronize(self.h), "synchronize")
+
+    def sync(self) -> int:
+        rc = lib().f2dg_sync(self.h)
+        if rc < 0:
+            raise RuntimeError("sync failed")
+        return rc
+
+    def bodies(self, world: int) -> np.ndarray:
+        n = lib().f2dg_body_count(self.h, world)
+        out = np.zeros((n, 7), np.float32)
+        lib().f2dg_get_bodies(self.h, world, out.ctypes.data_as(_PF))
+        return out
+
+    def export(self, world: int) -> bytes:
+        n = lib().f2dg_export_world(self.h, world, None, 0)
+        if n < 0:
+            raise RuntimeError("export failed")
+        buf = ctypes.create_string_buffer(n)
+        lib().f2dg_export_world(self.h, world, buf, n)
+        return buf.raw
+
+    def snapshot(self, world: int) -> snapshot.Snapshot:
+        return snapshot.parse_one(self.export(world))
+
+    def import_state(self, world: int, snap_bytes: bytes):
+        buf = ctypes.create_string_buffer(snap_bytes, len(snap_bytes))
+        self._check(lib().f2dg_import_world(self.h, world, buf, len(snap_bytes)), "import")
+
+    def counters(self, world: int) -> dict:
+        out = (ctypes.c_int32 * 8)()
+        lib().f2dg_world_counters(self.h, world, out)
+        keys = ["bodies", "shapes", "contacts", "touching", "overflow", "active_colors", "step", "errors"]
+        return dict(zip(keys, list(out)))
+
+    def device_bytes(self) -> int:
+        return lib().f2dg_device_bytes(self.h)
+
+    def awake_bodies(self) -> int:
+        return lib().f2dg_awake_body_count(self.h)

@@ -1,0 +1,1688 @@
+// C-ABI of the MI355X world step.
+//
+// Two entry families, both plain C (scalars, pointers, sizes):
+//  * f2d_*  : the flat per-world ABI of the reference's own C shim
+//             (F/native/wasm/f2d_shim.c), same names, argument meaning and id
+//             packing, for the subset the step path needs;
+//  * f2dg_* : the batch ABI: many independent worlds packed into one device
+//             batch and stepped by one launch per step.
+// Worlds are built on the host exactly as the reference builds them
+// (S/body.c:177-330, S/shape.c:36-210, S/joint.c:146-280), committed to the
+// device once, stepped there, and mirrored back lazily for getters.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "manifold.hpp"
+#include "world_store.hpp"
+
+extern "C" int f2d_launch_step(const f2d::BatchView* b, const f2d::StepParams* sp, const int* worldList, int nWorlds,
+                               hipStream_t stream);
+
+namespace f2d {
+
+#define F2D_HIP(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      std::fprintf(stderr, "forge2d_amd: %s failed: %s\n", #expr, hipGetErrorString(e_));    \
+      return -(int)e_ - 1000;                                                                \
+    }                                                                                        \
+  } while (0)
+
+// ---------------------------------------------------------------------------- host construction
+
+void HostWorld::ensureColorWords() {
+  int words = ((int)sims.size() + 31) / 32;
+  if (words < 1) words = 1;
+  if (words == colorWords && (int)colorBits.size() == kOverflowIndex * words) return;
+  std::vector<uint32_t> nb((size_t)kOverflowIndex * words, 0u);
+  for (int c = 0; c < kOverflowIndex; ++c)
+    for (int k = 0; k < colorWords && k < words; ++k) nb[(size_t)c * words + k] = colorBits[(size_t)c * colorWords + k];
+  colorBits.swap(nb);
+  colorWords = words;
+}
+
+// b2CreateBody, S/body.c:177-330
+int HostWorld::createBody(int type, Vec2 p, Rot q, Vec2 v, float w, float linDamp, float angDamp, float gravityScale,
+                          float sleepThreshold, int flags) {
+  BodySim s = {};
+  s.xf.p = p;
+  s.xf.q = q;
+  s.center = p;
+  s.rotation0 = q;
+  s.center0 = p;
+  s.minExtent = kHuge;
+  s.maxExtent = 0.0f;
+  s.linearDamping = linDamp;
+  s.angularDamping = angDamp;
+  s.gravityScale = gravityScale;
+  s.type = type;
+  s.flags = flags;
+  s.headShape = -1;
+  s.headContactKey = -1;
+  s.contactCount = 0;
+  s.headJointKey = -1;
+  s.jointCount = 0;
+  s.sleepThreshold = sleepThreshold;
+  s.sleepTime = 0.0f;
+  BodyState st = {};
+  st.v = v;
+  st.w = w;
+  st.dq = {1.0f, 0.0f};
+  if (type == kStatic) {
+    st.v = {0.0f, 0.0f};
+    st.w = 0.0f;
+  }
+  sims.push_back(s);
+  states.push_back(st);
+  bodyMass.push_back(0.0f);
+  bodyInertia.push_back(0.0f);
+  bodyGen.push_back(1);
+  bodyFixedRotation.push_back((flags & BF_FIXED_ROTATION) ? 1 : 0);
+  ensureColorWords();
+  return (int)sims.size() - 1;
+}
+
+// b2UpdateBodyMassData, S/body.c:528-631
+void HostWorld::updateBodyMass(int bi) {
+  BodySim& sim = sims[bi];
+  bodyMass[bi] = 0.0f;
+  bodyInertia[bi] = 0.0f;
+  sim.invMass = 0.0f;
+  sim.invI = 0.0f;
+  sim.localCenter = {0.0f, 0.0f};
+  sim.minExtent = kHuge;
+  sim.maxExtent = 0.0f;
+  if (sim.type != kDynamic) {
+    sim.center = sim.xf.p;
+    if (sim.type == kKinematic) {
+      for (int s = sim.headShape; s >= 0; s = shapes[s].next) {
+        float mn, mx;
+        shapeExtent(geoms[s], {0.0f, 0.0f}, &mn, &mx);
+        sim.minExtent = minf(sim.minExtent, mn);
+        sim.maxExtent = maxf(sim.maxExtent, mx);
+      }
+    }
+    return;
+  }
+  Vec2 localCenter = {0.0f, 0.0f};
+  for (int s = sim.headShape; s >= 0; s = shapes[s].next) {
+    if (shapeDensity[s] == 0.0f) continue;
+    MassData md = shapeMass(geoms[s], shapeDensity[s]);
+    bodyMass[bi] += md.mass;
+    localCenter = mulAdd(localCenter, md.mass, md.center);
+    bodyInertia[bi] += md.rotationalInertia;
+  }
+  if (bodyMass[bi] > 0.0f) {
+    sim.invMass = 1.0f / bodyMass[bi];
+    localCenter = mulSV(sim.invMass, localCenter);
+  }
+  if (bodyInertia[bi] > 0.0f && bodyFixedRotation[bi] == 0) {
+    bodyInertia[bi] -= bodyMass[bi] * dot(localCenter, localCenter);
+    sim.invI = 1.0f / bodyInertia[bi];
+  } else {
+    bodyInertia[bi] = 0.0f;
+    sim.invI = 0.0f;
+  }
+  Vec2 oldCenter = sim.center;
+  sim.localCenter = localCenter;
+  sim.center = transformPoint(sim.xf, sim.localCenter);
+  sim.center0 = sim.center;
+  BodyState& st = states[bi];
+  Vec2 deltaLinear = crossSV(st.w, sub(sim.center, oldCenter));
+  st.v = add(st.v, deltaLinear);
+  for (int s = sim.headShape; s >= 0; s = shapes[s].next) {
+    float mn, mx;
+    shapeExtent(geoms[s], localCenter, &mn, &mx);
+    sim.minExtent = minf(sim.minExtent, mn);
+    sim.maxExtent = maxf(sim.maxExtent, mx);
+  }
+}
+
+// b2CreateShapeInternal + b2CreateShapeProxy, S/shape.c:59-161, 901-910
+int HostWorld::createShape(int body, const ShapeGeom& g, float density, float friction, float restitution,
+                           float rolling, float tangentSpeed, uint64_t cat, uint64_t mask, int group, int flags,
+                           bool invokeContact, bool updateMass) {
+  Shape sh = {};
+  sh.body = body;
+  sh.friction = friction;
+  sh.restitution = restitution;
+  sh.rollingResistance = rolling;
+  sh.tangentSpeed = tangentSpeed;
+  sh.catLo = (uint32_t)cat;
+  sh.catHi = (uint32_t)(cat >> 32);
+  sh.maskLo = (uint32_t)mask;
+  sh.maskHi = (uint32_t)(mask >> 32);
+  sh.group = group;
+  sh.flags = flags & (SF_CONTACT_EVENTS | SF_HIT_EVENTS | SF_PRESOLVE | SF_SENSOR);
+  BodySim& sim = sims[body];
+  int type = sim.type;
+  // b2UpdateShapeAABBs, S/shape.c:36-57
+  AABB aabb = computeShapeAABB(g, sim.xf);
+  aabb.lo.x -= kSpeculativeDistance;
+  aabb.lo.y -= kSpeculativeDistance;
+  aabb.hi.x += kSpeculativeDistance;
+  aabb.hi.y += kSpeculativeDistance;
+  sh.aabb = aabb;
+  float margin = type == kStatic ? kSpeculativeDistance : kAabbMargin;
+  sh.fat.lo.x = aabb.lo.x - margin;
+  sh.fat.lo.y = aabb.lo.y - margin;
+  sh.fat.hi.x = aabb.hi.x + margin;
+  sh.fat.hi.y = aabb.hi.y + margin;
+  // proxy id of a fresh leaf: the tree allocates the leaf, then a new parent
+  // (S/dynamic_tree.c:122-158, 625-660, 770-790)
+  int proxyId = treeNodes[type];
+  treeNodes[type] += treeNodes[type] == 0 ? 1 : 2;
+  sh.proxyKey = (proxyId << 2) | type;
+  if (type != kStatic || invokeContact || (flags & SF_SENSOR)) {
+    sh.flags |= SF_MOVED;
+    sh.moveRank = (1 << 30) + createdSeq++;
+  }
+  sh.next = sim.headShape;
+  int id = (int)shapes.size();
+  shapes.push_back(sh);
+  geoms.push_back(g);
+  shapeDensity.push_back(density);
+  sims[body].headShape = id;
+  if (updateMass) updateBodyMass(body);
+  return id;
+}
+
+// b2CreateJoint + b2CreateRevoluteJoint + b2AssignJointColor, S/joint.c:146-280, 507-560,
+// S/constraint_graph.c:214-290
+int HostWorld::createRevolute(int bodyA, int bodyB, Vec2 la, Vec2 lb, float refAngle, float targetAngle,
+                              bool enableSpring, float hertz, float damping, bool enableLimit, float lower, float upper,
+                              bool enableMotor, float maxMotorTorque, float motorSpeed, bool collideConnected) {
+  Joint j = {};
+  int id = (int)joints.size();
+  j.bodyA = bodyA;
+  j.bodyB = bodyB;
+  j.collideConnected = collideConnected ? 1 : 0;
+  j.prevA = -1;
+  j.nextA = sims[bodyA].headJointKey;
+  int keyA = (id << 1) | 0;
+  if (sims[bodyA].headJointKey >= 0) {
+    Joint& hj = joints[sims[bodyA].headJointKey >> 1];
+    if ((sims[bodyA].headJointKey & 1) == 0) hj.prevA = keyA; else hj.prevB = keyA;
+  }
+  sims[bodyA].headJointKey = keyA;
+  sims[bodyA].jointCount += 1;
+  j.prevB = -1;
+  j.nextB = sims[bodyB].headJointKey;
+  int keyB = (id << 1) | 1;
+  if (sims[bodyB].headJointKey >= 0) {
+    Joint& hj = joints[sims[bodyB].headJointKey >> 1];
+    if ((sims[bodyB].headJointKey & 1) == 0) hj.prevA = keyB; else hj.prevB = keyB;
+  }
+  sims[bodyB].headJointKey = keyB;
+  sims[bodyB].jointCount += 1;
+  j.localOriginAnchorA = la;
+  j.localOriginAnchorB = lb;
+  j.referenceAngle = clampf(refAngle, -kPi, kPi);
+  j.targetAngle = clampf(targetAngle, -kPi, kPi);
+  j.hertz = hertz;
+  j.dampingRatio = damping;
+  j.lowerAngle = lower;
+  j.upperAngle = upper;
+  j.maxMotorTorque = maxMotorTorque;
+  j.motorSpeed = motorSpeed;
+  j.enableSpring = enableSpring;
+  j.enableLimit = enableLimit;
+  j.enableMotor = enableMotor;
+  j.constraintHertz = 60.0f;        // B2_JOINT_CONSTRAINT_HERTZ
+  j.constraintDampingRatio = 2.0f;  // B2_JOINT_CONSTRAINT_DAMPING_RATIO
+  j.constraintSoftness = {0.0f, 1.0f, 0.0f};
+  j.color = -1;
+  j.localIndex = -1;
+  bool staticA = sims[bodyA].type == kStatic, staticB = sims[bodyB].type == kStatic;
+  if (!(staticA && staticB)) {
+    ensureColorWords();
+    int color = kOverflowIndex;
+    auto bit = [&](int c, int b) -> bool { return (colorBits[(size_t)c * colorWords + (b >> 5)] >> (b & 31)) & 1u; };
+    auto set = [&](int c, int b) { colorBits[(size_t)c * colorWords + (b >> 5)] |= 1u << (b & 31); };
+    if (!staticA && !staticB) {
+      for (int i = 0; i < kOverflowIndex; ++i) {
+        if (bit(i, bodyA) || bit(i, bodyB)) continue;
+        set(i, bodyA);
+        set(i, bodyB);
+        color = i;
+        break;
+      }
+    } else if (!staticA) {
+      for (int i = 0; i < kOverflowIndex; ++i) {
+        if (bit(i, bodyA)) continue;
+        set(i, bodyA);
+        color = i;
+        break;
+      }
+    } else {
+      for (int i = 0; i < kOverflowIndex; ++i) {
+        if (bit(i, bodyB)) continue;
+        set(i, bodyB);
+        color = i;
+        break;
+      }
+    }
+    j.color = color;
+  }
+  joints.push_back(j);
+  return id;
+}
+
+// ---------------------------------------------------------------------------- scene spec
+
+#pragma pack(push, 1)
+struct SpecWorld {
+  int32_t body_count, shape_count, joint_count, flags;
+  float gx, gy, restitution_threshold, hit_event_threshold, contact_hertz, contact_damping_ratio,
+      max_contact_push_speed, maximum_linear_speed;
+};
+struct SpecBody {
+  int32_t type;
+  float px, py, qc, qs, vx, vy, w, lin_damp, ang_damp, gravity_scale, sleep_threshold;
+  int32_t flags;
+};
+struct SpecShape {
+  int32_t body, kind;
+  float friction, restitution, rolling, tangent_speed, density;
+  uint32_t cat_lo, cat_hi, mask_lo, mask_hi;
+  int32_t group, flags, npts;
+  float radius;
+  float data[16];
+};
+struct SpecJoint {
+  int32_t body_a, body_b;
+  float lax, lay, lbx, lby, ref_angle;
+  int32_t enable_limit;
+  float lower, upper;
+  int32_t enable_motor;
+  float motor_speed, max_motor_torque;
+  int32_t enable_spring;
+  float hertz, damping;
+  int32_t collide_connected;
+  float draw_size;
+};
+#pragma pack(pop)
+static_assert(sizeof(SpecWorld) == 48 && sizeof(SpecBody) == 52, "spec layout");
+static_assert(sizeof(SpecShape) == 124 && sizeof(SpecJoint) == 72, "spec layout");
+
+static void initWorldHeader(HostWorld& hw, const SpecWorld& sw) {
+  WorldHeader& h = hw.hdr;
+  h.flags = sw.flags;
+  h.gravity = {sw.gx, sw.gy};
+  h.restitutionThreshold = sw.restitution_threshold;
+  h.maxLinearSpeed = sw.maximum_linear_speed;
+  h.contactHertz = sw.contact_hertz;
+  h.contactDampingRatio = sw.contact_damping_ratio;
+  h.maxContactPushSpeed = sw.max_contact_push_speed;
+  h.jointHertz = 60.0f;
+  h.jointDampingRatio = 2.0f;
+}
+
+// Shape kinds of the spec: 0 circle, 1 capsule, 2 segment, 3 polygon (hull of points), 4 box
+static int buildSpecShape(HostWorld& hw, const SpecShape& ss, std::string& err) {
+  ShapeGeom g = {};
+  const float* d = ss.data;
+  switch (ss.kind) {
+    case 0:
+      g.type = kCircle;
+      g.v[0] = {d[0], d[1]};
+      g.radius = ss.radius;
+      break;
+    case 1: {
+      Vec2 c1 = {d[0], d[1]}, c2 = {d[2], d[3]};
+      // b2CreateCapsuleShape, S/shape.c:192-201: a degenerate capsule becomes a circle
+      if (distanceSq(c1, c2) <= kLinearSlop * kLinearSlop) {
+        g.type = kCircle;
+        g.v[0] = lerp(c1, c2, 0.5f);
+      } else {
+        g.type = kCapsule;
+        g.v[0] = c1;
+        g.v[1] = c2;
+      }
+      g.radius = ss.radius;
+      break;
+    }
+    case 2:
+      g.type = kSegment;
+      g.v[0] = {d[0], d[1]};
+      g.v[1] = {d[2], d[3]};
+      break;
+    case 3: {
+      Hull hull = computeHull(reinterpret_cast<const Vec2*>(d), ss.npts);
+      if (hull.count == 0) {
+        err = "degenerate hull";
+        return -1;
+      }
+      g = makePolygon(hull, ss.radius);
+      break;
+    }
+    case 4:
+      g = makeOffsetRoundedBox(d[0], d[1], {d[2], d[3]}, {d[4], d[5]}, ss.radius);
+      break;
+    default:
+      err = "bad shape kind";
+      return -1;
+  }
+  if (g.type == kPolygon && g.count < 3) {
+    err = "bad polygon";
+    return -1;
+  }
+  int sflags = 0;
+  if (ss.flags & 1) sflags |= SF_CONTACT_EVENTS;
+  if (ss.flags & 2) sflags |= SF_HIT_EVENTS;
+  if (ss.flags & 8) sflags |= SF_PRESOLVE;
+  if (ss.flags & 32) sflags |= SF_SENSOR;
+  uint64_t cat = (uint64_t)ss.cat_lo | ((uint64_t)ss.cat_hi << 32);
+  uint64_t mask = (uint64_t)ss.mask_lo | ((uint64_t)ss.mask_hi << 32);
+  return hw.createShape(ss.body, g, ss.density, ss.friction, ss.restitution, ss.rolling, ss.tangent_speed, cat, mask,
+                        ss.group, sflags, (ss.flags & 4) != 0, (ss.flags & 16) != 0);
+}
+
+static int parseSpec(Batch& b, const unsigned char* buf, long bytes) {
+  if (bytes < 16 || std::memcmp(buf, "F2DSCN01", 8) != 0) {
+    b.lastError = "bad scene spec magic";
+    return -1;
+  }
+  int32_t count;
+  std::memcpy(&count, buf + 8, 4);
+  long off = 16;
+  b.worlds.resize(count);
+  for (int i = 0; i < count; ++i) {
+    if (off + (long)sizeof(SpecWorld) > bytes) {
+      b.lastError = "truncated scene spec";
+      return -1;
+    }
+    SpecWorld sw;
+    std::memcpy(&sw, buf + off, sizeof(sw));
+    off += sizeof(sw);
+    long need = (long)sw.body_count * sizeof(SpecBody) + (long)sw.shape_count * sizeof(SpecShape) +
+                (long)sw.joint_count * sizeof(SpecJoint);
+    if (off + need > bytes) {
+      b.lastError = "truncated scene spec";
+      return -1;
+    }
+    HostWorld& hw = b.worlds[i];
+    initWorldHeader(hw, sw);
+    hw.sims.reserve(sw.body_count);
+    for (int k = 0; k < sw.body_count; ++k) {
+      SpecBody sb;
+      std::memcpy(&sb, buf + off, sizeof(sb));
+      off += sizeof(sb);
+      int flags = 0;
+      if (sb.flags & 1) flags |= BF_BULLET;
+      if (sb.flags & 2) flags |= BF_FIXED_ROTATION;
+      if (sb.flags & 4) flags |= BF_ALLOW_FAST_ROTATION;
+      if (sb.flags & 8) flags |= BF_ENABLE_SLEEP;
+      hw.createBody(sb.type, {sb.px, sb.py}, {sb.qc, sb.qs}, {sb.vx, sb.vy}, sb.w, sb.lin_damp, sb.ang_damp,
+                    sb.gravity_scale, sb.sleep_threshold, flags);
+    }
+    for (int k = 0; k < sw.shape_count; ++k) {
+      SpecShape ss;
+      std::memcpy(&ss, buf + off, sizeof(ss));
+      off += sizeof(ss);
+      if (ss.body < 0 || ss.body >= (int)hw.sims.size()) {
+        b.lastError = "shape body out of range";
+        return -1;
+      }
+      if (buildSpecShape(hw, ss, b.lastError) < 0) return -1;
+    }
+    for (int k = 0; k < sw.joint_count; ++k) {
+      SpecJoint sj;
+      std::memcpy(&sj, buf + off, sizeof(sj));
+      off += sizeof(sj);
+      hw.createRevolute(sj.body_a, sj.body_b, {sj.lax, sj.lay}, {sj.lbx, sj.lby}, sj.ref_angle, 0.0f,
+                        sj.enable_spring != 0, sj.hertz, sj.damping, sj.enable_limit != 0, sj.lower, sj.upper,
+                        sj.enable_motor != 0, sj.max_motor_torque, sj.motor_speed, sj.collide_connected != 0);
+    }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------- device commit
+
+static int nextPow2(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+template <typename T>
+static int devAlloc(DeviceBatch& d, T** ptr, size_t count) {
+  size_t bytes = sizeof(T) * (count > 0 ? count : 1);
+  void* p = nullptr;
+  F2D_HIP(hipMalloc(&p, bytes));
+  d.allocations.push_back(p);
+  d.bytes += bytes;
+  *ptr = reinterpret_cast<T*>(p);
+  return 0;
+}
+
+static void freeDevice(Batch& b) {
+  for (void* p : b.dev.allocations) (void)hipFree(p);
+  b.dev.allocations.clear();
+  b.dev.bytes = 0;
+  b.dev.view = BatchView{};
+  if (b.dWorldList) (void)hipFree(b.dWorldList);
+  b.dWorldList = nullptr;
+  b.dWorldListCap = 0;
+  b.committed = false;
+}
+
+// Lay out all worlds in flat arrays and allocate the device batch.
+static int commit(Batch& b) {
+  freeDevice(b);
+  long nb = 0, ns = 0, nc = 0, nj = 0, ncol = 0, nscratch = 0;
+  for (HostWorld& hw : b.worlds) {
+    hw.ensureColorWords();
+    WorldHeader& h = hw.hdr;
+    h.bodyOffset = (int)nb;
+    h.bodyCount = (int)hw.sims.size();
+    h.shapeOffset = (int)ns;
+    h.shapeCount = (int)hw.shapes.size();
+    int cap = (int)std::ceil(b.contactCapFactor * h.shapeCount) + 64;
+    cap = std::max(cap, (int)hw.contacts.size());
+    cap = std::max(cap, hw.contactCount + 64);
+    h.contactCap = cap;
+    h.contactOffset = (int)nc;
+    h.freeOffset = (int)nc;
+    h.pairOffset = (int)nc;
+    h.pairCap = cap;
+    h.overflowOffset = (int)nc;
+    h.constraintOffset = (int)nc;
+    h.jointOffset = (int)nj;
+    h.jointCount = (int)hw.joints.size();
+    h.colorOffset = (int)ncol;
+    h.colorWords = hw.colorWords;
+    h.scratchOffset = (int)nscratch;
+    int np = nextPow2(std::max(h.shapeCount, 1));
+    int pp = nextPow2(cap);
+    long sc = std::max<long>({4L * np + 16, 3L * pp + 16, (long)cap + 16, (long)h.jointCount + 16});
+    sc = (sc + 3) & ~3L;  // 16-byte aligned world regions (64-bit sort keys live here)
+    h.scratchCap = (int)sc;
+    h.contactCount = hw.contactCount;
+    h.freeCount = (int)hw.freeIds.size();
+    h.overflowCount = (int)hw.overflow.size();
+    h.stepIndex = hw.stepIndex;
+    h.error = 0;
+    nb += h.bodyCount;
+    ns += h.shapeCount;
+    nc += cap;
+    nj += h.jointCount;
+    ncol += (long)kOverflowIndex * hw.colorWords;
+    nscratch += sc;
+    hw.contacts.resize(cap);
+    for (int k = hw.contactCount; k < cap; ++k) {
+      hw.contacts[k] = Contact{};
+      hw.contacts[k].shapeA = -1;
+      hw.contacts[k].shapeB = -1;
+      hw.contacts[k].color = -1;
+    }
+  }
+  if (nc > (1L << 31) / CN_FIELD_COUNT) {
+    b.lastError = "batch too large for 32-bit constraint indexing";
+    return -1;
+  }
+  DeviceBatch& d = b.dev;
+  BatchView& v = d.view;
+  int W = (int)b.worlds.size();
+  if (devAlloc(d, &v.worlds, W) || devAlloc(d, &v.sims, nb) || devAlloc(d, &v.states, nb) ||
+      devAlloc(d, &v.shapes, ns) || devAlloc(d, &v.geoms, ns) || devAlloc(d, &v.sapBoxes, ns) ||
+      devAlloc(d, &v.contacts, nc) || devAlloc(d, &v.freeIds, nc) || devAlloc(d, &v.overflow, nc) ||
+      devAlloc(d, &v.pairs, nc) || devAlloc(d, &v.pairShapes, nc) || devAlloc(d, &v.joints, nj) ||
+      devAlloc(d, &v.colorBits, ncol) || devAlloc(d, &v.cons, (size_t)nc * CN_FIELD_COUNT) ||
+      devAlloc(d, &v.scratch, nscratch)) {
+    b.lastError = "device allocation failed";
+    freeDevice(b);
+    return -1;
+  }
+  v.consStride = nc;
+  v.worldCount = W;
+  b.committed = true;
+  return 0;
+}
+
+static int upload(Batch& b) {
+  if (!b.committed) {
+    if (commit(b) != 0) return -1;
+  }
+  BatchView& v = b.dev.view;
+  std::vector<WorldHeader> headers(b.worlds.size());
+  for (size_t i = 0; i < b.worlds.size(); ++i) {
+    HostWorld& hw = b.worlds[i];
+    WorldHeader& h = hw.hdr;
+    h.contactCount = hw.contactCount;
+    h.freeCount = (int)hw.freeIds.size();
+    h.overflowCount = (int)hw.overflow.size();
+    h.stepIndex = hw.stepIndex;
+    h.jointCount = (int)hw.joints.size();
+    headers[i] = h;
+    hipStream_t s = b.stream;
+    F2D_HIP(hipMemcpyAsync(v.sims + h.bodyOffset, hw.sims.data(), sizeof(BodySim) * h.bodyCount, hipMemcpyHostToDevice, s));
+    F2D_HIP(hipMemcpyAsync(v.states + h.bodyOffset, hw.states.data(), sizeof(BodyState) * h.bodyCount, hipMemcpyHostToDevice, s));
+    F2D_HIP(hipMemcpyAsync(v.shapes + h.shapeOffset, hw.shapes.data(), sizeof(Shape) * h.shapeCount, hipMemcpyHostToDevice, s));
+    F2D_HIP(hipMemcpyAsync(v.geoms + h.shapeOffset, hw.geoms.data(), sizeof(ShapeGeom) * h.shapeCount, hipMemcpyHostToDevice, s));
+    F2D_HIP(hipMemcpyAsync(v.contacts + h.contactOffset, hw.contacts.data(), sizeof(Contact) * h.contactCap, hipMemcpyHostToDevice, s));
+    if (!hw.freeIds.empty())
+      F2D_HIP(hipMemcpyAsync(v.freeIds + h.freeOffset, hw.freeIds.data(), sizeof(int) * hw.freeIds.size(), hipMemcpyHostToDevice, s));
+    if (!hw.overflow.empty())
+      F2D_HIP(hipMemcpyAsync(v.overflow + h.overflowOffset, hw.overflow.data(), sizeof(int) * hw.overflow.size(), hipMemcpyHostToDevice, s));
+    if (!hw.joints.empty())
+      F2D_HIP(hipMemcpyAsync(v.joints + h.jointOffset, hw.joints.data(), sizeof(Joint) * hw.joints.size(), hipMemcpyHostToDevice, s));
+    F2D_HIP(hipMemcpyAsync(v.colorBits + h.colorOffset, hw.colorBits.data(), sizeof(uint32_t) * hw.colorBits.size(), hipMemcpyHostToDevice, s));
+  }
+  F2D_HIP(hipMemcpyAsync(v.worlds, headers.data(), sizeof(WorldHeader) * headers.size(), hipMemcpyHostToDevice, b.stream));
+  F2D_HIP(hipStreamSynchronize(b.stream));
+  b.hostDirty = false;
+  return 0;
+}
+
+static int download(Batch& b) {
+  if (!b.committed || !b.deviceNewer) return 0;
+  BatchView& v = b.dev.view;
+  std::vector<WorldHeader> headers(b.worlds.size());
+  F2D_HIP(hipMemcpyAsync(headers.data(), v.worlds, sizeof(WorldHeader) * headers.size(), hipMemcpyDeviceToHost, b.stream));
+  F2D_HIP(hipStreamSynchronize(b.stream));
+  for (size_t i = 0; i < b.worlds.size(); ++i) {
+    HostWorld& hw = b.worlds[i];
+    const WorldHeader& h = headers[i];
+    hipStream_t s = b.stream;
+    F2D_HIP(hipMemcpyAsync(hw.sims.data(), v.sims + h.bodyOffset, sizeof(BodySim) * h.bodyCount, hipMemcpyDeviceToHost, s));
+    F2D_HIP(hipMemcpyAsync(hw.states.data(), v.states + h.bodyOffset, sizeof(BodyState) * h.bodyCount, hipMemcpyDeviceToHost, s));
+    F2D_HIP(hipMemcpyAsync(hw.shapes.data(), v.shapes + h.shapeOffset, sizeof(Shape) * h.shapeCount, hipMemcpyDeviceToHost, s));
+    hw.contacts.resize(h.contactCap);
+    F2D_HIP(hipMemcpyAsync(hw.contacts.data(), v.contacts + h.contactOffset, sizeof(Contact) * h.contactCap, hipMemcpyDeviceToHost, s));
+    hw.freeIds.resize(h.freeCount);
+    if (h.freeCount)
+      F2D_HIP(hipMemcpyAsync(hw.freeIds.data(), v.freeIds + h.freeOffset, sizeof(int) * h.freeCount, hipMemcpyDeviceToHost, s));
+    hw.overflow.resize(h.overflowCount);
+    if (h.overflowCount)
+      F2D_HIP(hipMemcpyAsync(hw.overflow.data(), v.overflow + h.overflowOffset, sizeof(int) * h.overflowCount, hipMemcpyDeviceToHost, s));
+    if (!hw.joints.empty())
+      F2D_HIP(hipMemcpyAsync(hw.joints.data(), v.joints + h.jointOffset, sizeof(Joint) * hw.joints.size(), hipMemcpyDeviceToHost, s));
+    F2D_HIP(hipMemcpyAsync(hw.colorBits.data(), v.colorBits + h.colorOffset, sizeof(uint32_t) * hw.colorBits.size(), hipMemcpyDeviceToHost, s));
+    F2D_HIP(hipStreamSynchronize(s));
+    WorldHeader keep = hw.hdr;
+    hw.hdr = h;
+    (void)keep;
+    hw.contactCount = h.contactCount;
+    hw.stepIndex = h.stepIndex;
+    hw.contactSpeed = h.contactSpeed;
+    hw.inv_h = h.inv_h;
+  }
+  b.deviceNewer = false;
+  return 0;
+}
+
+static int ensureWorldList(Batch& b, int n) {
+  if (n <= b.dWorldListCap) return 0;
+  if (b.dWorldList) (void)hipFree(b.dWorldList);
+  F2D_HIP(hipMalloc(&b.dWorldList, sizeof(int) * n));
+  b.dWorldListCap = n;
+  return 0;
+}
+
+static StepParams makeParams(float dt, int subSteps, int phases) {
+  // S/world.c:743-757
+  StepParams sp;
+  sp.subSteps = subSteps < 1 ? 1 : subSteps;
+  sp.dt = dt;
+  if (dt > 0.0f) {
+    sp.inv_dt = 1.0f / dt;
+    sp.h = dt / sp.subSteps;
+    sp.inv_h = sp.subSteps * sp.inv_dt;
+  } else {
+    sp.inv_dt = 0.0f;
+    sp.h = 0.0f;
+    sp.inv_h = 0.0f;
+  }
+  sp.phases = phases;
+  return sp;
+}
+
+static int prepareForStep(Batch& b) {
+  if (b.hostDirty) {
+    if (b.deviceNewer) {
+      b.lastError = "host and device both modified";
+      return -1;
+    }
+    if (upload(b) != 0) return -1;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------- snapshots
+
+#pragma pack(push, 1)
+struct SnapHeader {
+  int32_t magic, version, step, body_count, shape_count, contact_cap, joint_count, overflow_count, move_count,
+      free_id_count, color_total, reserved0;
+  float contact_speed, inv_h, reserved1, reserved2;
+};
+struct SnapBody {
+  int32_t type, set_index, local_index, flags;
+  float px, py, qc, qs, cx, cy, c0x, c0y, q0c, q0s, lcx, lcy, vx, vy, w, fx, fy, torque, inv_mass, inv_i, min_ext,
+      max_ext, sleep_time, reserved;
+};
+struct SnapShape {
+  float aabb[4];
+  float fat[4];
+  int32_t enlarged, proxy_key, body, reserved;
+};
+struct SnapPoint {
+  float px, py, ax, ay, bx, by, sep, ni, ti, tni, nv;
+  int32_t id, persisted;
+};
+struct SnapContact {
+  int32_t id, shape_a, shape_b, flags, color, local_index, sim_flags, point_count;
+  float friction, restitution, rolling, tangent_speed, nx, ny, rolling_impulse, reserved;
+  SnapPoint pts[2];
+};
+struct SnapJoint {
+  int32_t id, color, local_index, body_a, body_b, enable_limit, enable_motor, enable_spring;
+  float linear_impulse_x, linear_impulse_y, spring_impulse, motor_impulse, lower_impulse, upper_impulse, ref_angle,
+      reserved;
+};
+#pragma pack(pop)
+static_assert(sizeof(SnapHeader) == 64 && sizeof(SnapBody) == 112 && sizeof(SnapShape) == 48, "snap layout");
+static_assert(sizeof(SnapContact) == 168 && sizeof(SnapJoint) == 64, "snap layout");
+
+template <typename T>
+static void put(std::vector<unsigned char>& out, const T& v) {
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+  out.insert(out.end(), p, p + sizeof(T));
+}
+
+static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
+  SnapHeader h = {};
+  h.magic = 0x50414E53;
+  h.version = 1;
+  h.step = hw.stepIndex;
+  h.body_count = (int)hw.sims.size();
+  h.shape_count = (int)hw.shapes.size();
+  h.contact_cap = hw.contactCount;
+  h.joint_count = (int)hw.joints.size();
+  h.overflow_count = (int)hw.overflow.size();
+  std::vector<int> colorLists[kGraphColorCount];
+  for (int i = 0; i < hw.contactCount; ++i) {
+    const Contact& c = hw.contacts[i];
+    if (c.shapeA >= 0 && c.color >= 0 && c.color < kOverflowIndex) colorLists[c.color].push_back(i);
+  }
+  colorLists[kOverflowIndex] = hw.overflow;
+  int total = 0;
+  for (auto& l : colorLists) total += (int)l.size();
+  h.color_total = total;
+  std::vector<std::pair<int, int>> moves;
+  for (size_t s = 0; s < hw.shapes.size(); ++s)
+    if (hw.shapes[s].flags & SF_MOVED) moves.push_back({hw.shapes[s].moveRank, hw.shapes[s].proxyKey});
+  std::sort(moves.begin(), moves.end());
+  h.move_count = (int)moves.size();
+  h.free_id_count = (int)hw.freeIds.size();
+  h.contact_speed = hw.contactSpeed;
+  h.inv_h = hw.inv_h;
+  put(out, h);
+  for (size_t i = 0; i < hw.sims.size(); ++i) {
+    const BodySim& s = hw.sims[i];
+    const BodyState& st = hw.states[i];
+    SnapBody sb = {};
+    sb.type = s.type;
+    sb.set_index = s.type == kStatic ? 0 : 2;
+    sb.local_index = (int)i;
+    sb.flags = ((s.flags & BF_IS_FAST) ? 1 : 0) | ((s.flags & BF_BULLET) ? 2 : 0) | ((s.flags & BF_SPEED_CAPPED) ? 4 : 0);
+    sb.px = s.xf.p.x;
+    sb.py = s.xf.p.y;
+    sb.qc = s.xf.q.c;
+    sb.qs = s.xf.q.s;
+    sb.cx = s.center.x;
+    sb.cy = s.center.y;
+    sb.c0x = s.center0.x;
+    sb.c0y = s.center0.y;
+    sb.q0c = s.rotation0.c;
+    sb.q0s = s.rotation0.s;
+    sb.lcx = s.localCenter.x;
+    sb.lcy = s.localCenter.y;
+    if (s.type != kStatic) {
+      sb.vx = st.v.x;
+      sb.vy = st.v.y;
+      sb.w = st.w;
+    }
+    sb.fx = s.force.x;
+    sb.fy = s.force.y;
+    sb.torque = s.torque;
+    sb.inv_mass = s.invMass;
+    sb.inv_i = s.invI;
+    sb.min_ext = s.minExtent;
+    sb.max_ext = s.maxExtent;
+    sb.sleep_time = s.sleepTime;
+    put(out, sb);
+  }
+  for (const Shape& s : hw.shapes) {
+    SnapShape ss = {};
+    ss.aabb[0] = s.aabb.lo.x;
+    ss.aabb[1] = s.aabb.lo.y;
+    ss.aabb[2] = s.aabb.hi.x;
+    ss.aabb[3] = s.aabb.hi.y;
+    ss.fat[0] = s.fat.lo.x;
+    ss.fat[1] = s.fat.lo.y;
+    ss.fat[2] = s.fat.hi.x;
+    ss.fat[3] = s.fat.hi.y;
+    ss.enlarged = 0;
+    ss.proxy_key = s.proxyKey;
+    ss.body = s.body;
+    put(out, ss);
+  }
+  for (int i = 0; i < hw.contactCount; ++i) {
+    const Contact& c = hw.contacts[i];
+    SnapContact sc = {};
+    sc.id = c.shapeA >= 0 ? i : -1;
+    if (c.shapeA >= 0) {
+      sc.shape_a = c.shapeA;
+      sc.shape_b = c.shapeB;
+      sc.flags = c.flags;
+      sc.color = c.color;
+      sc.local_index = c.localIndex;
+      sc.sim_flags = c.simFlags;
+      sc.point_count = c.m.pointCount;
+      sc.friction = c.friction;
+      sc.restitution = c.restitution;
+      sc.rolling = c.rollingResistance;
+      sc.tangent_speed = c.tangentSpeed;
+      sc.nx = c.m.normal.x;
+      sc.ny = c.m.normal.y;
+      sc.rolling_impulse = c.m.rollingImpulse;
+      for (int k = 0; k < 2; ++k) {
+        const ManifoldPoint& mp = c.m.points[k];
+        SnapPoint& sp = sc.pts[k];
+        sp.px = mp.point.x;
+        sp.py = mp.point.y;
+        sp.ax = mp.anchorA.x;
+        sp.ay = mp.anchorA.y;
+        sp.bx = mp.anchorB.x;
+        sp.by = mp.anchorB.y;
+        sp.sep = mp.separation;
+        sp.ni = mp.normalImpulse;
+        sp.ti = mp.tangentImpulse;
+        sp.tni = mp.totalNormalImpulse;
+        sp.nv = mp.normalVelocity;
+        sp.id = mp.id & 0xFFFF;
+        sp.persisted = (mp.id >> 16) & 1;
+      }
+    }
+    put(out, sc);
+  }
+  for (size_t i = 0; i < hw.joints.size(); ++i) {
+    const Joint& j = hw.joints[i];
+    SnapJoint sj = {};
+    sj.id = (int)i;
+    sj.color = j.color;
+    sj.local_index = j.localIndex;
+    sj.body_a = j.bodyA;
+    sj.body_b = j.bodyB;
+    sj.enable_limit = j.enableLimit;
+    sj.enable_motor = j.enableMotor;
+    sj.enable_spring = j.enableSpring;
+    sj.linear_impulse_x = j.linearImpulse.x;
+    sj.linear_impulse_y = j.linearImpulse.y;
+    sj.spring_impulse = j.springImpulse;
+    sj.motor_impulse = j.motorImpulse;
+    sj.lower_impulse = j.lowerImpulse;
+    sj.upper_impulse = j.upperImpulse;
+    sj.ref_angle = j.referenceAngle;
+    put(out, sj);
+  }
+  for (int c = 0; c < kGraphColorCount; ++c) put(out, (int32_t)colorLists[c].size());
+  for (int c = 0; c < kGraphColorCount; ++c)
+    for (int id : colorLists[c]) put(out, (int32_t)id);
+  for (auto& m : moves) put(out, (int32_t)m.second);
+  for (int id : hw.freeIds) put(out, (int32_t)id);
+}
+
+// Overlay a snapshot onto a world built from the same scene spec.
+static int importWorld(HostWorld& hw, const unsigned char* buf, long bytes, std::string& err) {
+  if (bytes < (long)sizeof(SnapHeader)) {
+    err = "snapshot too short";
+    return -1;
+  }
+  SnapHeader h;
+  std::memcpy(&h, buf, sizeof(h));
+  if (h.magic != 0x50414E53) {
+    err = "bad snapshot magic";
+    return -1;
+  }
+  if (h.body_count != (int)hw.sims.size() || h.shape_count != (int)hw.shapes.size() ||
+      h.joint_count != (int)hw.joints.size()) {
+    err = "snapshot does not match the world's bodies/shapes/joints";
+    return -1;
+  }
+  long off = sizeof(h);
+  long need = (long)h.body_count * sizeof(SnapBody) + (long)h.shape_count * sizeof(SnapShape) +
+              (long)h.contact_cap * sizeof(SnapContact) + (long)h.joint_count * sizeof(SnapJoint) + 4L * 12;
+  if (off + need > bytes) {
+    err = "snapshot truncated";
+    return -1;
+  }
+  for (int i = 0; i < h.body_count; ++i) {
+    SnapBody sb;
+    std::memcpy(&sb, buf + off, sizeof(sb));
+    off += sizeof(sb);
+    BodySim& s = hw.sims[i];
+    BodyState& st = hw.states[i];
+    s.xf.p = {sb.px, sb.py};
+    s.xf.q = {sb.qc, sb.qs};
+    s.center = {sb.cx, sb.cy};
+    s.center0 = {sb.c0x, sb.c0y};
+    s.rotation0 = {sb.q0c, sb.q0s};
+    s.localCenter = {sb.lcx, sb.lcy};
+    s.force = {sb.fx, sb.fy};
+    s.torque = sb.torque;
+    s.invMass = sb.inv_mass;
+    s.invI = sb.inv_i;
+    s.minExtent = sb.min_ext;
+    s.maxExtent = sb.max_ext;
+    s.sleepTime = sb.sleep_time;
+    s.headContactKey = -1;
+    s.contactCount = 0;
+    st.v = {sb.vx, sb.vy};
+    st.w = sb.w;
+    st.dp = {0.0f, 0.0f};
+    st.dq = {1.0f, 0.0f};
+  }
+  for (int i = 0; i < h.shape_count; ++i) {
+    SnapShape ss;
+    std::memcpy(&ss, buf + off, sizeof(ss));
+    off += sizeof(ss);
+    Shape& s = hw.shapes[i];
+    s.aabb = {{ss.aabb[0], ss.aabb[1]}, {ss.aabb[2], ss.aabb[3]}};
+    s.fat = {{ss.fat[0], ss.fat[1]}, {ss.fat[2], ss.fat[3]}};
+    if (ss.proxy_key != s.proxyKey) {
+      err = "snapshot proxy keys differ from this build's proxy-key mirror";
+      return -1;
+    }
+    s.flags &= ~SF_MOVED;
+  }
+  hw.contacts.assign(h.contact_cap, Contact{});
+  hw.contactCount = h.contact_cap;
+  std::vector<int> live;
+  for (int i = 0; i < h.contact_cap; ++i) {
+    SnapContact sc;
+    std::memcpy(&sc, buf + off, sizeof(sc));
+    off += sizeof(sc);
+    Contact& c = hw.contacts[i];
+    c = Contact{};
+    c.shapeA = -1;
+    c.shapeB = -1;
+    c.color = -1;
+    c.localIndex = -1;
+    if (sc.id < 0) continue;
+    c.shapeA = sc.shape_a;
+    c.shapeB = sc.shape_b;
+    c.flags = sc.flags & (CF_TOUCHING | CF_CONTACT_EVENTS);
+    c.simFlags = sc.sim_flags & (SIM_TOUCHING | SIM_HIT_EVENT | SIM_PRESOLVE);
+    c.color = sc.color;
+    c.friction = sc.friction;
+    c.restitution = sc.restitution;
+    c.rollingResistance = sc.rolling;
+    c.tangentSpeed = sc.tangent_speed;
+    c.m.normal = {sc.nx, sc.ny};
+    c.m.rollingImpulse = sc.rolling_impulse;
+    c.m.pointCount = sc.point_count;
+    for (int k = 0; k < 2; ++k) {
+      const SnapPoint& sp = sc.pts[k];
+      ManifoldPoint& mp = c.m.points[k];
+      mp.point = {sp.px, sp.py};
+      mp.anchorA = {sp.ax, sp.ay};
+      mp.anchorB = {sp.bx, sp.by};
+      mp.separation = sp.sep;
+      mp.normalImpulse = sp.ni;
+      mp.tangentImpulse = sp.ti;
+      mp.totalNormalImpulse = sp.tni;
+      mp.normalVelocity = sp.nv;
+      mp.id = (sp.id & 0xFFFF) | (sp.persisted ? 0x10000 : 0);
+    }
+    live.push_back(i);
+  }
+  // body contact lists, built in ascending contact id (list order does not affect results)
+  for (int id : live) {
+    Contact& c = hw.contacts[id];
+    int bodyA = hw.shapes[c.shapeA].body, bodyB = hw.shapes[c.shapeB].body;
+    c.bodyA = bodyA;
+    c.bodyB = bodyB;
+    BodySim& ba = hw.sims[bodyA];
+    c.prevA = -1;
+    c.nextA = ba.headContactKey;
+    if (ba.headContactKey >= 0) {
+      Contact& hc = hw.contacts[ba.headContactKey >> 1];
+      if ((ba.headContactKey & 1) == 0) hc.prevA = (id << 1); else hc.prevB = (id << 1);
+    }
+    ba.headContactKey = id << 1;
+    ba.contactCount++;
+    BodySim& bb = hw.sims[bodyB];
+    c.prevB = -1;
+    c.nextB = bb.headContactKey;
+    if (bb.headContactKey >= 0) {
+      Contact& hc = hw.contacts[bb.headContactKey >> 1];
+      if ((bb.headContactKey & 1) == 0) hc.prevA = (id << 1) | 1; else hc.prevB = (id << 1) | 1;
+    }
+    bb.headContactKey = (id << 1) | 1;
+    bb.contactCount++;
+  }
+  for (int i = 0; i < h.joint_count; ++i) {
+    SnapJoint sj;
+    std::memcpy(&sj, buf + off, sizeof(sj));
+    off += sizeof(sj);
+    Joint& j = hw.joints[i];
+    j.color = sj.color;
+    j.localIndex = sj.local_index;
+    j.linearImpulse = {sj.linear_impulse_x, sj.linear_impulse_y};
+    j.springImpulse = sj.spring_impulse;
+    j.motorImpulse = sj.motor_impulse;
+    j.lowerImpulse = sj.lower_impulse;
+    j.upperImpulse = sj.upper_impulse;
+  }
+  int counts[kGraphColorCount];
+  std::memcpy(counts, buf + off, sizeof(counts));
+  off += sizeof(counts);
+  long idsTotal = 0;
+  for (int c = 0; c < kGraphColorCount; ++c) idsTotal += counts[c];
+  if (off + 4 * (idsTotal + h.move_count + h.free_id_count) > bytes) {
+    err = "snapshot truncated";
+    return -1;
+  }
+  // colour bitsets from contacts and joints, overflow list in array order
+  hw.ensureColorWords();
+  std::fill(hw.colorBits.begin(), hw.colorBits.end(), 0u);
+  auto setBit = [&](int c, int b) { hw.colorBits[(size_t)c * hw.colorWords + (b >> 5)] |= 1u << (b & 31); };
+  hw.overflow.clear();
+  for (int c = 0; c < kGraphColorCount; ++c) {
+    for (int k = 0; k < counts[c]; ++k) {
+      int32_t id;
+      std::memcpy(&id, buf + off, 4);
+      off += 4;
+      Contact& ct = hw.contacts[id];
+      if (c == kOverflowIndex) {
+        ct.localIndex = (int)hw.overflow.size();
+        hw.overflow.push_back(id);
+      } else {
+        ct.localIndex = -1;
+        if (hw.sims[ct.bodyA].type != kStatic) setBit(c, ct.bodyA);
+        if (hw.sims[ct.bodyB].type != kStatic) setBit(c, ct.bodyB);
+      }
+    }
+  }
+  for (const Joint& j : hw.joints) {
+    if (j.color >= 0 && j.color < kOverflowIndex) {
+      if (hw.sims[j.bodyA].type != kStatic) setBit(j.color, j.bodyA);
+      if (hw.sims[j.bodyB].type != kStatic) setBit(j.color, j.bodyB);
+    }
+  }
+  for (int k = 0; k < h.move_count; ++k) {
+    int32_t key;
+    std::memcpy(&key, buf + off, 4);
+    off += 4;
+    for (Shape& s : hw.shapes) {
+      if (s.proxyKey == key) {
+        s.flags |= SF_MOVED;
+        s.moveRank = k;
+        break;
+      }
+    }
+  }
+  hw.freeIds.resize(h.free_id_count);
+  for (int k = 0; k < h.free_id_count; ++k) {
+    std::memcpy(&hw.freeIds[k], buf + off, 4);
+    off += 4;
+  }
+  hw.stepIndex = h.step;
+  hw.contactSpeed = h.contact_speed;
+  hw.inv_h = h.inv_h;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------- handle tables
+
+static std::mutex g_lock;
+struct WorldSlot {
+  Batch* batch = nullptr;
+  uint16_t generation = 0;
+};
+static std::vector<WorldSlot> g_worlds;  // f2d_* single-world handles
+
+static Batch* batchOf(uint32_t w) {
+  uint32_t index1 = w & 0xFFFF, gen = w >> 16;
+  if (index1 == 0 || index1 > g_worlds.size()) return nullptr;
+  WorldSlot& s = g_worlds[index1 - 1];
+  if (s.batch == nullptr || s.generation != gen) return nullptr;
+  return s.batch;
+}
+
+static HostWorld* bodyWorld(int32_t index1, uint32_t wg, int* bodyId) {
+  uint32_t world0 = wg & 0xFFFF, gen = wg >> 16;
+  if (world0 >= g_worlds.size()) return nullptr;
+  WorldSlot& s = g_worlds[world0];
+  if (!s.batch) return nullptr;
+  if (s.batch->deviceNewer && download(*s.batch) != 0) return nullptr;
+  HostWorld& hw = s.batch->worlds[0];
+  int id = index1 - 1;
+  if (id < 0 || id >= (int)hw.sims.size() || hw.bodyGen[id] != gen) return nullptr;
+  *bodyId = id;
+  return &hw;
+}
+
+static Batch* bodyBatch(uint32_t wg) {
+  uint32_t world0 = wg & 0xFFFF;
+  if (world0 >= g_worlds.size()) return nullptr;
+  return g_worlds[world0].batch;
+}
+
+}  // namespace f2d
+
+using namespace f2d;
+
+// ============================================================================ C-ABI
+
+extern "C" {
+
+// ---- batch ABI --------------------------------------------------------------
+
+void* f2dg_create_from_spec(const void* spec, long bytes) {
+  Batch* b = new Batch();
+  if (parseSpec(*b, static_cast<const unsigned char*>(spec), bytes) != 0) {
+    std::fprintf(stderr, "forge2d_amd: %s\n", b->lastError.c_str());
+    delete b;
+    return nullptr;
+  }
+  return b;
+}
+
+void f2dg_destroy(void* h) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return;
+  freeDevice(*b);
+  if (b->ownsStream && b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int f2dg_world_count(void* h) { return h ? (int)static_cast<Batch*>(h)->worlds.size() : -1; }
+
+int f2dg_body_count(void* h, int world) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || world < 0 || world >= (int)b->worlds.size()) return -1;
+  return (int)b->worlds[world].sims.size();
+}
+
+int f2dg_shape_count(void* h, int world) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || world < 0 || world >= (int)b->worlds.size()) return -1;
+  return (int)b->worlds[world].shapes.size();
+}
+
+long f2dg_awake_body_count(void* h) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return -1;
+  long n = 0;
+  for (auto& w : b->worlds)
+    for (auto& s : w.sims) n += s.type != kStatic ? 1 : 0;
+  return n;
+}
+
+int f2dg_set_contact_capacity_factor(void* h, float factor) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || b->committed || factor <= 0.0f) return -1;
+  b->contactCapFactor = factor;
+  return 0;
+}
+
+int f2dg_set_stream(void* h, void* stream) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return -1;
+  b->stream = static_cast<hipStream_t>(stream);
+  return 0;
+}
+
+int f2dg_upload(void* h) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return -1;
+  return upload(*b);
+}
+
+long f2dg_device_bytes(void* h) {
+  Batch* b = static_cast<Batch*>(h);
+  return b ? (long)b->dev.bytes : -1;
+}
+
+// Step every world of the batch `steps` times: one launch per step, one workgroup per world.
+int f2dg_step(void* h, float dt, int subSteps, int steps, int phases) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return -1;
+  if (prepareForStep(*b) != 0) return -1;
+  if (dt == 0.0f) return 0;
+  StepParams sp = makeParams(dt, subSteps, phases == 0 ? PH_ALL : phases);
+  int W = (int)b->worlds.size();
+  for (int s = 0; s < steps; ++s) {
+    int rc = f2d_launch_step(&b->dev.view, &sp, nullptr, W, b->stream);
+    if (rc != 0) {
+      std::fprintf(stderr, "forge2d_amd: step launch failed (%d)\n", rc);
+      return -1;
+    }
+  }
+  b->deviceNewer = true;
+  return 0;
+}
+
+// Step a subset of worlds (serving: the worlds that have pending requests).
+int f2dg_step_worlds(void* h, const int* worldIds, int n, float dt, int subSteps, int steps) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || n < 0) return -1;
+  if (prepareForStep(*b) != 0) return -1;
+  if (n == 0 || dt == 0.0f) return 0;
+  for (int i = 0; i < n; ++i) {
+    if (worldIds[i] < 0 || worldIds[i] >= (int)b->worlds.size()) return -1;
+  }
+  if (ensureWorldList(*b, n) != 0) return -1;
+  F2D_HIP(hipMemcpyAsync(b->dWorldList, worldIds, sizeof(int) * n, hipMemcpyHostToDevice, b->stream));
+  StepParams sp = makeParams(dt, subSteps, PH_ALL);
+  for (int s = 0; s < steps; ++s) {
+    if (f2d_launch_step(&b->dev.view, &sp, b->dWorldList, n, b->stream) != 0) return -1;
+  }
+  b->deviceNewer = true;
+  return 0;
+}
+
+// The one-world-at-a-time GPU path: each world gets its own launch per step.
+int f2dg_step_each(void* h, float dt, int subSteps, int steps) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return -1;
+  if (prepareForStep(*b) != 0) return -1;
+  int W = (int)b->worlds.size();
+  if (ensureWorldList(*b, W) != 0) return -1;
+  std::vector<int> ids(W);
+  for (int i = 0; i < W; ++i) ids[i] = i;
+  F2D_HIP(hipMemcpyAsync(b->dWorldList, ids.data(), sizeof(int) * W, hipMemcpyHostToDevice, b->stream));
+  StepParams sp = makeParams(dt, subSteps, PH_ALL);
+  for (int i = 0; i < W; ++i) {
+    for (int s = 0; s < steps; ++s) {
+      if (f2d_launch_step(&b->dev.view, &sp, b->dWorldList + i, 1, b->stream) != 0) return -1;
+    }
+  }
+  b->deviceNewer = true;
+  return 0;
+}
+
+// Time `steps` batch steps with HIP events on the batch's stream; writes the total ms.
+int f2dg_step_timed(void* h, float dt, int subSteps, int steps, float* msOut) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return -1;
+  if (prepareForStep(*b) != 0) return -1;
+  hipEvent_t e0, e1;
+  F2D_HIP(hipEventCreate(&e0));
+  F2D_HIP(hipEventCreate(&e1));
+  StepParams sp = makeParams(dt, subSteps, PH_ALL);
+  int W = (int)b->worlds.size();
+  F2D_HIP(hipEventRecord(e0, b->stream));
+  for (int s = 0; s < steps; ++s) {
+    if (f2d_launch_step(&b->dev.view, &sp, nullptr, W, b->stream) != 0) return -1;
+  }
+  F2D_HIP(hipEventRecord(e1, b->stream));
+  F2D_HIP(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  F2D_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *msOut = ms;
+  b->deviceNewer = true;
+  return 0;
+}
+
+int f2dg_synchronize(void* h) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return -1;
+  F2D_HIP(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+// Copy device state back to the host mirror; returns the OR of the worlds' error bits.
+int f2dg_sync(void* h) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return -1;
+  if (download(*b) != 0) return -1;
+  int err = 0;
+  for (auto& w : b->worlds) err |= w.hdr.error;
+  return err;
+}
+
+// Per-body [px, py, qc, qs, vx, vy, w] of one world (after f2dg_sync).
+int f2dg_get_bodies(void* h, int world, float* out) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || world < 0 || world >= (int)b->worlds.size()) return -1;
+  if (download(*b) != 0) return -1;
+  const HostWorld& hw = b->worlds[world];
+  for (size_t i = 0; i < hw.sims.size(); ++i) {
+    const BodySim& s = hw.sims[i];
+    const BodyState& st = hw.states[i];
+    float* o = out + 7 * i;
+    o[0] = s.xf.p.x;
+    o[1] = s.xf.p.y;
+    o[2] = s.xf.q.c;
+    o[3] = s.xf.q.s;
+    o[4] = s.type == kStatic ? 0.0f : st.v.x;
+    o[5] = s.type == kStatic ? 0.0f : st.v.y;
+    o[6] = s.type == kStatic ? 0.0f : st.w;
+  }
+  return (int)hw.sims.size();
+}
+
+// Serialize one world's state (snapshot layout). Returns the byte count; writes
+// only when cap is large enough.
+long f2dg_export_world(void* h, int world, void* out, long cap) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || world < 0 || world >= (int)b->worlds.size()) return -1;
+  if (download(*b) != 0) return -1;
+  std::vector<unsigned char> buf;
+  exportWorld(b->worlds[world], buf);
+  if (out && cap >= (long)buf.size()) std::memcpy(out, buf.data(), buf.size());
+  return (long)buf.size();
+}
+
+int f2dg_import_world(void* h, int world, const void* snap, long bytes) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || world < 0 || world >= (int)b->worlds.size()) return -1;
+  if (download(*b) != 0) return -1;
+  if (importWorld(b->worlds[world], static_cast<const unsigned char*>(snap), bytes, b->lastError) != 0) {
+    std::fprintf(stderr, "forge2d_amd: import: %s\n", b->lastError.c_str());
+    return -1;
+  }
+  b->hostDirty = true;
+  if (b->committed) {
+    // capacities may need to grow; re-lay out the batch
+    freeDevice(*b);
+  }
+  return 0;
+}
+
+const char* f2dg_last_error(void* h) {
+  Batch* b = static_cast<Batch*>(h);
+  return b ? b->lastError.c_str() : "null batch";
+}
+
+int f2dg_world_counters(void* h, int world, int* out) {
+  // [bodies, shapes, live contacts, touching contacts, overflow, active colours, step index, errors]
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || world < 0 || world >= (int)b->worlds.size()) return -1;
+  if (download(*b) != 0) return -1;
+  const HostWorld& hw = b->worlds[world];
+  int live = 0, touching = 0;
+  for (int i = 0; i < hw.contactCount; ++i) {
+    if (hw.contacts[i].shapeA >= 0) {
+      live++;
+      touching += (hw.contacts[i].flags & CF_TOUCHING) ? 1 : 0;
+    }
+  }
+  out[0] = (int)hw.sims.size();
+  out[1] = (int)hw.shapes.size();
+  out[2] = live;
+  out[3] = touching;
+  out[4] = (int)hw.overflow.size();
+  out[5] = hw.hdr.activeColors;
+  out[6] = hw.stepIndex;
+  out[7] = hw.hdr.error;
+  return 0;
+}
+
+// ---- flat per-world ABI (F/native/wasm/f2d_shim.c) ---------------------------
+
+// f2d_shim.c:123-139
+uint32_t f2d_create_world(float gravity_x, float gravity_y, float restitution_threshold, float hit_event_threshold,
+                          float contact_hertz, float contact_damping_ratio, float max_contact_push_speed,
+                          float maximum_linear_speed, int enable_sleep, int enable_continuous) {
+  std::lock_guard<std::mutex> g(g_lock);
+  Batch* b = new Batch();
+  b->worlds.resize(1);
+  SpecWorld sw = {};
+  sw.flags = (enable_sleep ? 1 : 0) | (enable_continuous ? 2 : 0) | 4;
+  sw.gx = gravity_x;
+  sw.gy = gravity_y;
+  sw.restitution_threshold = restitution_threshold;
+  sw.hit_event_threshold = hit_event_threshold;
+  sw.contact_hertz = contact_hertz;
+  sw.contact_damping_ratio = contact_damping_ratio;
+  sw.max_contact_push_speed = max_contact_push_speed;
+  sw.maximum_linear_speed = maximum_linear_speed;
+  initWorldHeader(b->worlds[0], sw);
+  size_t slot = g_worlds.size();
+  for (size_t i = 0; i < g_worlds.size(); ++i) {
+    if (g_worlds[i].batch == nullptr) {
+      slot = i;
+      break;
+    }
+  }
+  if (slot == g_worlds.size()) g_worlds.push_back(WorldSlot{});
+  g_worlds[slot].batch = b;
+  g_worlds[slot].generation += 1;
+  return (uint32_t)(slot + 1) | ((uint32_t)g_worlds[slot].generation << 16);
+}
+
+void f2d_destroy_world(uint32_t w) {
+  std::lock_guard<std::mutex> g(g_lock);
+  Batch* b = batchOf(w);
+  if (!b) return;
+  f2dg_destroy(b);
+  g_worlds[(w & 0xFFFF) - 1].batch = nullptr;
+}
+
+int f2d_world_is_valid(uint32_t w) { return batchOf(w) != nullptr; }
+
+// f2d_shim.c:147-149 -> b2World_Step
+void f2d_world_step(uint32_t w, float time_step, int sub_steps) {
+  Batch* b = batchOf(w);
+  if (!b) return;
+  if (time_step == 0.0f) return;
+  if (f2dg_step(b, time_step, sub_steps, 1, PH_ALL) != 0) {
+    std::fprintf(stderr, "forge2d_amd: world step failed\n");
+    std::abort();
+  }
+  if (f2dg_synchronize(b) != 0) std::abort();
+}
+
+void f2d_world_set_gravity(uint32_t w, float x, float y) {
+  Batch* b = batchOf(w);
+  if (!b) return;
+  download(*b);
+  b->worlds[0].hdr.gravity = {x, y};
+  b->hostDirty = true;
+}
+
+void f2d_world_get_gravity(uint32_t w, float* out) {
+  Batch* b = batchOf(w);
+  if (!b) return;
+  out[0] = b->worlds[0].hdr.gravity.x;
+  out[1] = b->worlds[0].hdr.gravity.y;
+}
+
+void f2d_world_enable_sleeping(uint32_t w, int enabled) {
+  Batch* b = batchOf(w);
+  if (!b) return;
+  download(*b);
+  WorldHeader& h = b->worlds[0].hdr;
+  h.flags = enabled ? (h.flags | WF_SLEEP) : (h.flags & ~WF_SLEEP);
+  b->hostDirty = true;
+}
+
+int f2d_world_is_sleeping_enabled(uint32_t w) {
+  Batch* b = batchOf(w);
+  return b ? ((b->worlds[0].hdr.flags & WF_SLEEP) != 0) : 0;
+}
+
+void f2d_world_enable_continuous(uint32_t w, int enabled) {
+  Batch* b = batchOf(w);
+  if (!b) return;
+  download(*b);
+  WorldHeader& h = b->worlds[0].hdr;
+  h.flags = enabled ? (h.flags | WF_CONTINUOUS) : (h.flags & ~WF_CONTINUOUS);
+  b->hostDirty = true;
+}
+
+int f2d_world_is_continuous_enabled(uint32_t w) {
+  Batch* b = batchOf(w);
+  return b ? ((b->worlds[0].hdr.flags & WF_CONTINUOUS) != 0) : 0;
+}
+
+// f2d_shim.c:192-218
+void f2d_create_body(uint32_t w, int type, float px, float py, float qc, float qs, float lvx, float lvy,
+                     float angular_velocity, float linear_damping, float angular_damping, float gravity_scale,
+                     float sleep_threshold, const char* name, int enable_sleep, int is_awake, int fixed_rotation,
+                     int is_bullet, int is_enabled, int allow_fast_rotation, int32_t* out) {
+  (void)name;
+  (void)is_awake;
+  out[0] = 0;
+  out[1] = 0;
+  Batch* b = batchOf(w);
+  if (!b || !is_enabled) return;
+  download(*b);
+  if (b->committed) freeDevice(*b);  // new bodies change the batch layout
+  int flags = 0;
+  if (is_bullet) flags |= BF_BULLET;
+  if (fixed_rotation) flags |= BF_FIXED_ROTATION;
+  if (allow_fast_rotation) flags |= BF_ALLOW_FAST_ROTATION;
+  if (enable_sleep) flags |= BF_ENABLE_SLEEP;
+  HostWorld& hw = b->worlds[0];
+  int id = hw.createBody(type, {px, py}, {qc, qs}, {lvx, lvy}, angular_velocity, linear_damping, angular_damping,
+                         gravity_scale, sleep_threshold, flags);
+  b->hostDirty = true;
+  out[0] = id + 1;
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | ((uint32_t)hw.bodyGen[id] << 16));
+}
+
+int f2d_body_is_valid(int32_t index1, uint32_t wg) {
+  int id;
+  return bodyWorld(index1, wg, &id) != nullptr;
+}
+
+void f2d_body_get_position(int32_t index1, uint32_t wg, float* out) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  out[0] = hw->sims[id].xf.p.x;
+  out[1] = hw->sims[id].xf.p.y;
+}
+
+void f2d_body_get_rotation(int32_t index1, uint32_t wg, float* out) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  out[0] = hw->sims[id].xf.q.c;
+  out[1] = hw->sims[id].xf.q.s;
+}
+
+void f2d_body_get_linear_velocity(int32_t index1, uint32_t wg, float* out) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  out[0] = hw->states[id].v.x;
+  out[1] = hw->states[id].v.y;
+}
+
+float f2d_body_get_angular_velocity(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  return hw ? hw->states[id].w : 0.0f;
+}
+
+void f2d_body_set_linear_velocity(int32_t index1, uint32_t wg, float x, float y) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw || hw->sims[id].type == kStatic) return;
+  hw->states[id].v = {x, y};
+  bodyBatch(wg)->hostDirty = true;
+}
+
+void f2d_body_apply_force_to_center(int32_t index1, uint32_t wg, float fx, float fy, int wake) {
+  (void)wake;
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw || hw->sims[id].type != kDynamic) return;
+  hw->sims[id].force = add(hw->sims[id].force, {fx, fy});
+  bodyBatch(wg)->hostDirty = true;
+}
+
+// b2Body_ApplyLinearImpulseToCenter, S/body.c: v += invMass * impulse
+void f2d_body_apply_linear_impulse_to_center(int32_t index1, uint32_t wg, float ix, float iy, int wake) {
+  (void)wake;
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw || hw->sims[id].type != kDynamic) return;
+  hw->states[id].v = mulAdd(hw->states[id].v, hw->sims[id].invMass, {ix, iy});
+  bodyBatch(wg)->hostDirty = true;
+}
+
+void f2d_body_get_world_center(int32_t index1, uint32_t wg, float* out) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  out[0] = hw->sims[id].center.x;
+  out[1] = hw->sims[id].center.y;
+}
+
+void f2d_body_get_local_center(int32_t index1, uint32_t wg, float* out) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  out[0] = hw->sims[id].localCenter.x;
+  out[1] = hw->sims[id].localCenter.y;
+}
+
+float f2d_body_get_mass(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  return hw ? hw->bodyMass[id] : 0.0f;
+}
+
+float f2d_body_get_rotational_inertia(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  return hw ? hw->bodyInertia[id] : 0.0f;
+}
+
+int f2d_body_get_type(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  return hw ? hw->sims[id].type : -1;
+}
+
+// f2d_shim.c:412-431
+typedef struct f2dShapeDef {
+  float friction;
+  float restitution;
+  float rolling_resistance;
+  float tangent_speed;
+  int32_t user_material_id;
+  uint32_t custom_color;
+  float density;
+  uint32_t category_lo;
+  uint32_t category_hi;
+  uint32_t mask_lo;
+  uint32_t mask_hi;
+  int32_t group_index;
+  int32_t is_sensor;
+  int32_t enable_sensor_events;
+  int32_t enable_contact_events;
+  int32_t enable_hit_events;
+  int32_t enable_pre_solve_events;
+  int32_t invoke_contact_creation;
+  int32_t update_body_mass;
+} f2dShapeDef;
+
+static void createShapeFlat(int32_t index1, uint32_t wg, const f2dShapeDef* def, const ShapeGeom& g, int32_t* out) {
+  out[0] = 0;
+  out[1] = 0;
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  Batch* b = bodyBatch(wg);
+  if (b->committed) freeDevice(*b);
+  int flags = 0;
+  if (def->enable_contact_events) flags |= SF_CONTACT_EVENTS;
+  if (def->enable_hit_events) flags |= SF_HIT_EVENTS;
+  if (def->enable_pre_solve_events) flags |= SF_PRESOLVE;
+  if (def->is_sensor) flags |= SF_SENSOR;
+  uint64_t cat = (uint64_t)def->category_lo | ((uint64_t)def->category_hi << 32);
+  uint64_t mask = (uint64_t)def->mask_lo | ((uint64_t)def->mask_hi << 32);
+  int sid = hw->createShape(id, g, def->density, def->friction, def->restitution, def->rolling_resistance,
+                            def->tangent_speed, cat, mask, def->group_index, flags, def->invoke_contact_creation != 0,
+                            def->update_body_mass != 0);
+  b->hostDirty = true;
+  out[0] = sid + 1;
+  out[1] = (int32_t)((wg & 0xFFFF) | (1u << 16));
+}
+
+// f2d_shim.c:459-466
+void f2d_create_circle_shape(int32_t index1, uint32_t wg, const f2dShapeDef* def, float cx, float cy, float radius,
+                             int32_t* out) {
+  ShapeGeom g = {};
+  g.type = kCircle;
+  g.v[0] = {cx, cy};
+  g.radius = radius;
+  createShapeFlat(index1, wg, def, g, out);
+}
+
+// f2d_shim.c:468-476, b2CreateCapsuleShape S/shape.c:192-201
+void f2d_create_capsule_shape(int32_t index1, uint32_t wg, const f2dShapeDef* def, float c1x, float c1y, float c2x,
+                              float c2y, float radius, int32_t* out) {
+  ShapeGeom g = {};
+  Vec2 c1 = {c1x, c1y}, c2 = {c2x, c2y};
+  if (distanceSq(c1, c2) <= kLinearSlop * kLinearSlop) {
+    g.type = kCircle;
+    g.v[0] = lerp(c1, c2, 0.5f);
+  } else {
+    g.type = kCapsule;
+    g.v[0] = c1;
+    g.v[1] = c2;
+  }
+  g.radius = radius;
+  createShapeFlat(index1, wg, def, g, out);
+}
+
+// f2d_shim.c:478-486
+void f2d_create_segment_shape(int32_t index1, uint32_t wg, const f2dShapeDef* def, float p1x, float p1y, float p2x,
+                              float p2y, int32_t* out) {
+  ShapeGeom g = {};
+  g.type = kSegment;
+  g.v[0] = {p1x, p1y};
+  g.v[1] = {p2x, p2y};
+  createShapeFlat(index1, wg, def, g, out);
+}
+
+// f2d_shim.c:488-500
+void f2d_create_box_shape(int32_t index1, uint32_t wg, const f2dShapeDef* def, float half_width, float half_height,
+                          float cx, float cy, float qc, float qs, float radius, int32_t* out) {
+  ShapeGeom g = makeOffsetRoundedBox(half_width, half_height, {cx, cy}, {qc, qs}, radius);
+  createShapeFlat(index1, wg, def, g, out);
+}
+
+// f2d_shim.c:502-515: returns 0 when the hull is degenerate
+int f2d_create_polygon_shape(int32_t index1, uint32_t wg, const f2dShapeDef* def, const float* points, int count,
+                             float radius, int32_t* out) {
+  Hull hull = computeHull(reinterpret_cast<const Vec2*>(points), count);
+  if (hull.count == 0) {
+    out[0] = 0;
+    out[1] = 0;
+    return 0;
+  }
+  ShapeGeom g = makePolygon(hull, radius);
+  createShapeFlat(index1, wg, def, g, out);
+  return 1;
+}
+
+// f2d_shim.c:816-843 (revolute subset of the def)
+void f2d_create_revolute_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t b_index1, uint32_t b_wg,
+                               float lax, float lay, float lbx, float lby, float reference_angle, float target_angle,
+                               int enable_spring, float hertz, float damping_ratio, int enable_limit,
+                               float lower_angle, float upper_angle, int enable_motor, float max_motor_torque,
+                               float motor_speed, float draw_size, int collide_connected, int32_t* out) {
+  (void)draw_size;
+  (void)b_wg;
+  out[0] = 0;
+  out[1] = 0;
+  Batch* b = batchOf(w);
+  int ia, ib;
+  HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
+  if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (b->committed) freeDevice(*b);
+  int id = hw->createRevolute(ia, ib, {lax, lay}, {lbx, lby}, reference_angle, target_angle, enable_spring != 0, hertz,
+                              damping_ratio, enable_limit != 0, lower_angle, upper_angle, enable_motor != 0,
+                              max_motor_torque, motor_speed, collide_connected != 0);
+  b->hostDirty = true;
+  out[0] = id + 1;
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+}
+
+// Batch handle behind a single world, for the batch ABI (export, counters).
+void* f2d_world_batch(uint32_t w) { return batchOf(w); }
+
+}  // extern "C"

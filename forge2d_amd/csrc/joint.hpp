@@ -1,0 +1,205 @@
+// Revolute joint stages, one joint per lane, in the reference's operation order:
+// prepare S/joint.c:1348-1355 + S/revolute_joint.c:202-259, warm start :261-287,
+// solve :289-465. Static bodies read the identity body state as in the reference.
+#pragma once
+
+#include "solver.hpp"
+
+namespace f2d {
+
+// b2Atan2, S/math_functions.c:61-101 (deterministic minimax approximation)
+F2D_HD float atan2Approx(float y, float x) {
+  if (x == 0.0f && y == 0.0f) return 0.0f;
+  float ax = absf(x);
+  float ay = absf(y);
+  float mx = maxf(ay, ax);
+  float mn = minf(ay, ax);
+  float a = mn / mx;
+  float s = a * a;
+  float c = s * a;
+  float q = s * s;
+  float r = 0.024840285f * q + 0.18681418f;
+  float t = -0.094097948f * q - 0.33213072f;
+  r = r * s + t;
+  r = r * c + a;
+  if (ay > ax) r = 1.57079637f - r;
+  if (x < 0) r = 3.14159274f - r;
+  if (y < 0) r = -r;
+  return r;
+}
+
+// b2RelativeAngle, I/math_functions.h:472-479
+F2D_HD float relativeAngle(Rot b, Rot a) {
+  float s = b.s * a.c - b.c * a.s;
+  float c = b.c * a.c + b.s * a.s;
+  return atan2Approx(s, c);
+}
+
+// b2UnwindAngle, I/math_functions.h:481-485
+F2D_HD float unwindAngle(float radians) { return remainderf(radians, 2.0f * kPi); }
+
+// b2Solve22, I/math_functions.h:569-579
+F2D_HD Vec2 solve22(float a11, float a12, float a21, float a22, Vec2 b) {
+  float det = a11 * a22 - a12 * a21;
+  if (det != 0.0f) det = 1.0f / det;
+  return {det * (a22 * b.x - a12 * b.y), det * (a11 * b.y - a21 * b.x)};
+}
+
+F2D_HD void prepareRevolute(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+  // constraint softness, S/joint.c:1350-1352
+  float hertz = minf(j.constraintHertz, 0.25f * inv_h);
+  j.constraintSoftness = makeSoft(hertz, j.constraintDampingRatio, h);
+  const BodySim& a = sims[j.bodyA];
+  const BodySim& b = sims[j.bodyB];
+  j.invMassA = a.invMass;
+  j.invMassB = b.invMass;
+  j.invIA = a.invI;
+  j.invIB = b.invI;
+  j.anchorA = rotate(a.xf.q, sub(j.localOriginAnchorA, a.localCenter));
+  j.anchorB = rotate(b.xf.q, sub(j.localOriginAnchorB, b.localCenter));
+  j.deltaCenter = sub(b.center, a.center);
+  j.deltaAngle = relativeAngle(b.xf.q, a.xf.q);
+  float k = j.invIA + j.invIB;
+  j.axialMass = k > 0.0f ? 1.0f / k : 0.0f;
+  j.springSoftness = makeSoft(j.hertz, j.dampingRatio, h);
+  if (warmStartScale == 0.0f) {
+    j.linearImpulse = {0.0f, 0.0f};
+    j.springImpulse = 0.0f;
+    j.motorImpulse = 0.0f;
+    j.lowerImpulse = 0.0f;
+    j.upperImpulse = 0.0f;
+  }
+}
+
+// The body index of a joint side: -1 for static bodies (they have no solver state).
+F2D_HD int jointIndex(const BodySim* sims, int body) { return sims[body].type == kStatic ? -1 : body; }
+
+F2D_HD void warmStartRevolute(Joint& j, BodyState* states, const BodySim* sims) {
+  float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+  BodyState dummyA = identityState(), dummyB = identityState();
+  int ia = jointIndex(sims, j.bodyA), ib = jointIndex(sims, j.bodyB);
+  BodyState* sA = ia < 0 ? &dummyA : states + ia;
+  BodyState* sB = ib < 0 ? &dummyB : states + ib;
+  Vec2 rA = rotate(sA->dq, j.anchorA);
+  Vec2 rB = rotate(sB->dq, j.anchorB);
+  float axialImpulse = j.springImpulse + j.motorImpulse + j.lowerImpulse - j.upperImpulse;
+  sA->v = mulSub(sA->v, mA, j.linearImpulse);
+  sA->w -= iA * (cross(rA, j.linearImpulse) + axialImpulse);
+  sB->v = mulAdd(sB->v, mB, j.linearImpulse);
+  sB->w += iB * (cross(rB, j.linearImpulse) + axialImpulse);
+}
+
+F2D_HD void solveRevolute(Joint& j, BodyState* states, const BodySim* sims, float h, float inv_h, bool useBias) {
+  float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+  BodyState dummyA = identityState(), dummyB = identityState();
+  int ia = jointIndex(sims, j.bodyA), ib = jointIndex(sims, j.bodyB);
+  BodyState* sA = ia < 0 ? &dummyA : states + ia;
+  BodyState* sB = ib < 0 ? &dummyB : states + ib;
+  Vec2 vA = sA->v;
+  float wA = sA->w;
+  Vec2 vB = sB->v;
+  float wB = sB->w;
+  const Rot dqA = sA->dq;
+  const Rot dqB = sB->dq;
+  bool fixedRotation = (iA + iB == 0.0f);
+  if (j.enableSpring && fixedRotation == false) {
+    float jointAngle = relativeAngle(sB->dq, sA->dq) + j.deltaAngle;
+    float jointAngleDelta = unwindAngle(jointAngle - j.targetAngle);
+    float C = jointAngleDelta;
+    float bias = j.springSoftness.biasRate * C;
+    float massScale = j.springSoftness.massScale;
+    float impulseScale = j.springSoftness.impulseScale;
+    float Cdot = wB - wA;
+    float impulse = -massScale * j.axialMass * (Cdot + bias) - impulseScale * j.springImpulse;
+    j.springImpulse += impulse;
+    wA -= iA * impulse;
+    wB += iB * impulse;
+  }
+  if (j.enableMotor && fixedRotation == false) {
+    float Cdot = wB - wA - j.motorSpeed;
+    float impulse = -j.axialMass * Cdot;
+    float oldImpulse = j.motorImpulse;
+    float maxImpulse = h * j.maxMotorTorque;
+    j.motorImpulse = clampf(j.motorImpulse + impulse, -maxImpulse, maxImpulse);
+    impulse = j.motorImpulse - oldImpulse;
+    wA -= iA * impulse;
+    wB += iB * impulse;
+  }
+  if (j.enableLimit && fixedRotation == false) {
+    float jointAngle = relativeAngle(dqB, dqA) + j.deltaAngle - j.referenceAngle;
+    jointAngle = unwindAngle(jointAngle);
+    {
+      float C = jointAngle - j.lowerAngle;
+      float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
+      if (C > 0.0f) {
+        bias = C * inv_h;
+      } else if (useBias) {
+        bias = j.constraintSoftness.biasRate * C;
+        massScale = j.constraintSoftness.massScale;
+        impulseScale = j.constraintSoftness.impulseScale;
+      }
+      float Cdot = wB - wA;
+      float oldImpulse = j.lowerImpulse;
+      float impulse = -massScale * j.axialMass * (Cdot + bias) - impulseScale * oldImpulse;
+      j.lowerImpulse = maxf(oldImpulse + impulse, 0.0f);
+      impulse = j.lowerImpulse - oldImpulse;
+      wA -= iA * impulse;
+      wB += iB * impulse;
+    }
+    {
+      float C = j.upperAngle - jointAngle;
+      float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
+      if (C > 0.0f) {
+        bias = C * inv_h;
+      } else if (useBias) {
+        bias = j.constraintSoftness.biasRate * C;
+        massScale = j.constraintSoftness.massScale;
+        impulseScale = j.constraintSoftness.impulseScale;
+      }
+      float Cdot = wA - wB;
+      float oldImpulse = j.upperImpulse;
+      float impulse = -massScale * j.axialMass * (Cdot + bias) - impulseScale * oldImpulse;
+      j.upperImpulse = maxf(oldImpulse + impulse, 0.0f);
+      impulse = j.upperImpulse - oldImpulse;
+      wA += iA * impulse;
+      wB -= iB * impulse;
+    }
+  }
+  {
+    Vec2 rA = rotate(sA->dq, j.anchorA);
+    Vec2 rB = rotate(sB->dq, j.anchorB);
+    Vec2 Cdot = sub(add(vB, crossSV(wB, rB)), add(vA, crossSV(wA, rA)));
+    Vec2 bias = {0.0f, 0.0f};
+    float massScale = 1.0f;
+    float impulseScale = 0.0f;
+    if (useBias) {
+      Vec2 dcA = sA->dp;
+      Vec2 dcB = sB->dp;
+      Vec2 separation = add(add(sub(dcB, dcA), sub(rB, rA)), j.deltaCenter);
+      bias = mulSV(j.constraintSoftness.biasRate, separation);
+      massScale = j.constraintSoftness.massScale;
+      impulseScale = j.constraintSoftness.impulseScale;
+    }
+    float k11 = mA + mB + rA.y * rA.y * iA + rB.y * rB.y * iB;
+    float k12 = -rA.y * rA.x * iA - rB.y * rB.x * iB;
+    float k21 = k12;
+    float k22 = mA + mB + rA.x * rA.x * iA + rB.x * rB.x * iB;
+    // b2Mat22 {cx = (k11, k21), cy = (k12, k22)}: a11 = cx.x, a12 = cy.x, a21 = cx.y, a22 = cy.y
+    Vec2 b = solve22(k11, k12, k21, k22, add(Cdot, bias));
+    Vec2 impulse;
+    impulse.x = -massScale * b.x - impulseScale * j.linearImpulse.x;
+    impulse.y = -massScale * b.y - impulseScale * j.linearImpulse.y;
+    j.linearImpulse.x += impulse.x;
+    j.linearImpulse.y += impulse.y;
+    vA = mulSub(vA, mA, impulse);
+    wA -= iA * cross(rA, impulse);
+    vB = mulAdd(vB, mB, impulse);
+    wB += iB * cross(rB, impulse);
+  }
+  sA->v = vA;
+  sA->w = wA;
+  sB->v = vB;
+  sB->w = wB;
+}
+
+}  // namespace f2d

@@ -1,0 +1,240 @@
+// Device data layout of a batch of worlds.
+//
+// All worlds of a batch live in shared flat arrays; a WorldHeader holds each
+// world's offsets into them. Bodies and shapes are array-of-structs (the solver
+// gathers whole body states by index; per-body passes read a whole record).
+// Contact constraints built each step are struct-of-arrays (ConstraintSoA), so
+// the lanes of a colour stage read each field with one coalesced access.
+//
+// Ids are world-local: body id, shape id and contact id index the world's slice
+// of the flat arrays, exactly as bodyId/shapeId/contactId index the reference's
+// b2World arrays (S/world.h).
+#pragma once
+
+#include "b2math.hpp"
+
+namespace f2d {
+
+enum BodyType : int { kStatic = 0, kKinematic = 1, kDynamic = 2 };
+enum ShapeType : int { kCircle = 0, kCapsule = 1, kSegment = 2, kPolygon = 3 };
+
+// body flags
+enum : int {
+  BF_BULLET = 1,
+  BF_FIXED_ROTATION = 2,
+  BF_ALLOW_FAST_ROTATION = 4,
+  BF_ENABLE_SLEEP = 8,
+  BF_IS_FAST = 16,
+  BF_SPEED_CAPPED = 32,
+  BF_ENLARGED = 64,
+};
+// shape flags
+enum : int {
+  SF_CONTACT_EVENTS = 1,
+  SF_HIT_EVENTS = 2,
+  SF_ENLARGED = 4,   // fat AABB grew in the last finalize (enlargedAABB)
+  SF_MOVED = 8,      // proxy is in the move buffer for the next pair update
+  SF_PRESOLVE = 16,
+  SF_SENSOR = 32,
+};
+// contact flags (b2ContactFlags, S/contact.h:17-40) and sim flags (:66-86)
+enum : int {
+  CF_TOUCHING = 0x1,
+  CF_CONTACT_EVENTS = 0x2,
+  SIM_TOUCHING = 0x00010000,
+  SIM_DISJOINT = 0x00020000,
+  SIM_STARTED = 0x00040000,
+  SIM_STOPPED = 0x00080000,
+  SIM_HIT_EVENT = 0x00100000,
+  SIM_PRESOLVE = 0x00200000,
+};
+// world flags
+enum : int { WF_SLEEP = 1, WF_CONTINUOUS = 2, WF_WARM = 4 };
+// error bits reported back to the host
+enum : int {
+  ERR_CONTACT_CAP = 1,
+  ERR_PAIR_CAP = 2,
+  ERR_OVERFLOW_CAP = 4,
+  ERR_SCRATCH = 8,
+};
+
+// b2BodyState, S/body.h:100-113 (32 bytes, identical field order)
+struct alignas(16) BodyState {
+  Vec2 v;
+  float w;
+  int flags;
+  Vec2 dp;
+  Rot dq;
+};
+
+// b2BodySim (S/body.h:118-159) plus the b2Body fields the step reads.
+struct alignas(16) BodySim {
+  Xf xf;            // transform of the body origin
+  Vec2 center;      // world centre of mass
+  Vec2 center0;     // centre of mass at the start of the step (TOI)
+  Rot rotation0;
+  Vec2 localCenter;
+  Vec2 force;
+  float torque;
+  float invMass, invI;
+  float minExtent, maxExtent;
+  float linearDamping, angularDamping, gravityScale;
+  int type;
+  int flags;
+  int headShape;       // first shape of the body's shape list (b2Body::headShapeId)
+  int headContactKey;  // b2Body::headContactKey
+  int contactCount;
+  int headJointKey;
+  int jointCount;
+  float sleepThreshold;
+  float sleepTime;
+  int pad[2];
+};
+
+struct alignas(16) ShapeGeom {
+  int type;
+  int count;     // polygon vertex count
+  float radius;  // circle/capsule/polygon radius
+  float pad;
+  Vec2 centroid;
+  Vec2 v[kMaxPolygonVertices];  // circle: v[0] centre; capsule/segment: v[0], v[1]
+  Vec2 n[kMaxPolygonVertices];
+};
+
+struct alignas(16) Shape {
+  AABB aabb;
+  AABB fat;
+  int body;
+  int next;       // next shape on the same body (b2Shape::nextShapeId)
+  int proxyKey;   // (proxyId << 2) | bodyType, S/broad_phase.h:18-21
+  int flags;
+  float friction, restitution, rollingResistance, tangentSpeed;
+  uint32_t catLo, catHi, maskLo, maskHi;
+  int group;
+  int moveRank;   // order of this proxy in the move buffer when it moves
+  int pad[2];
+};
+
+// b2ManifoldPoint, I/collision.h:498-536
+struct ManifoldPoint {
+  Vec2 point, anchorA, anchorB;
+  float separation, normalImpulse, tangentImpulse, totalNormalImpulse, normalVelocity;
+  int id;  // uint16 id in the low bits, persisted flag in bit 16
+};
+
+struct Manifold {
+  Vec2 normal;
+  float rollingImpulse;
+  int pointCount;
+  ManifoldPoint points[2];
+};
+
+// One slot per contact id: b2Contact + b2ContactSim (S/contact.h:42-131).
+struct alignas(16) Contact {
+  int shapeA, shapeB;   // -1 when the slot is free
+  int flags;            // CF_*
+  int simFlags;         // SIM_*
+  int color;            // constraint-graph colour or -1
+  int localIndex;       // index within the overflow list when color == overflow
+  int bodyA, prevA, nextA;  // edge 0 (S/contact.h b2ContactEdge)
+  int bodyB, prevB, nextB;  // edge 1
+  float friction, restitution, rollingResistance, tangentSpeed;
+  Manifold m;
+};
+
+// Revolute joint slot (b2Joint + b2JointSim + b2RevoluteJoint, S/joint.h:27-60, 162-190, 247-281)
+struct alignas(16) Joint {
+  int bodyA, bodyB;
+  int color, localIndex;
+  int prevA, nextA, prevB, nextB;  // body joint list edges
+  int collideConnected;
+  int enableSpring, enableMotor, enableLimit;
+  Vec2 localOriginAnchorA, localOriginAnchorB;
+  float referenceAngle, targetAngle, hertz, dampingRatio, maxMotorTorque, motorSpeed, lowerAngle, upperAngle;
+  float constraintHertz, constraintDampingRatio;
+  Vec2 linearImpulse;
+  float springImpulse, motorImpulse, lowerImpulse, upperImpulse;
+  // per-step prepared data
+  float invMassA, invMassB, invIA, invIB;
+  Softness constraintSoftness, springSoftness;
+  Vec2 anchorA, anchorB, deltaCenter;
+  float deltaAngle, axialMass;
+  int pad;
+};
+
+// Per-world header. Offsets index the batch's flat arrays.
+struct alignas(16) WorldHeader {
+  int bodyOffset, bodyCount;
+  int shapeOffset, shapeCount;
+  int contactOffset, contactCap;
+  int contactCount;      // slots in use = next fresh id (contact array count)
+  int freeCount;         // LIFO free-id stack depth (stack at freeOffset)
+  int freeOffset;
+  int jointOffset, jointCount;
+  int pairOffset, pairCap;
+  int colorOffset, colorWords;   // (kOverflowIndex) bitsets of colorWords uint32 each
+  int overflowOffset, overflowCount;
+  int constraintOffset;          // constraint SoA region, contactCap lanes
+  int scratchOffset, scratchCap; // int scratch for large-world sorts
+  int flags;
+  int error;
+  int stepIndex;
+  int activeColors;              // colours used in the last solve (statistics)
+  Vec2 gravity;
+  float restitutionThreshold, maxLinearSpeed, contactHertz, contactDampingRatio, maxContactPushSpeed;
+  float contactSpeed;            // world->contactSpeed, set each step
+  float inv_h;                   // world->inv_h
+  float jointHertz, jointDampingRatio;
+  int pad[1];
+};
+
+// Per-step scalars, S/world.c:743-771
+struct StepParams {
+  float dt, inv_dt, h, inv_h;
+  int subSteps;
+  int phases;
+};
+enum : int {
+  PH_PAIRS = 1,
+  PH_COLLIDE = 2,
+  PH_STATE = 4,
+  PH_SOLVE = 8,
+  PH_FINALIZE = 16,
+  PH_ALL = 31,
+};
+
+// Contact constraint lanes (one constraint per lane), S/contact_solver.c:1034-1064.
+enum ConstraintField : int {
+  CN_INDEX_A = 0, CN_INDEX_B, CN_SLOT,
+  CN_INV_MASS_A, CN_INV_MASS_B, CN_INV_I_A, CN_INV_I_B,
+  CN_NX, CN_NY, CN_FRICTION, CN_TANGENT_SPEED, CN_ROLLING_RES, CN_ROLLING_MASS, CN_ROLLING_IMPULSE,
+  CN_BIAS_RATE, CN_MASS_SCALE, CN_IMPULSE_SCALE,
+  CN_A1X, CN_A1Y, CN_B1X, CN_B1Y, CN_BASE_SEP1, CN_NI1, CN_TNI1, CN_TI1, CN_NM1, CN_TM1,
+  CN_A2X, CN_A2Y, CN_B2X, CN_B2Y, CN_BASE_SEP2, CN_NI2, CN_TNI2, CN_TI2, CN_NM2, CN_TM2,
+  CN_RESTITUTION, CN_REL_VEL1, CN_REL_VEL2, CN_POINT_COUNT,
+  CN_FIELD_COUNT
+};
+
+// Device pointers of a batch.
+struct BatchView {
+  WorldHeader* worlds;
+  BodySim* sims;
+  BodyState* states;
+  Shape* shapes;
+  ShapeGeom* geoms;
+  Contact* contacts;
+  int* freeIds;
+  Joint* joints;
+  uint64_t* pairs;      // new-pair scratch: sort key (hi) | local pair index (lo)
+  int2* pairShapes;     // new-pair shapes (query order, unoriented)
+  uint32_t* colorBits;
+  int* overflow;        // overflow-colour contact ids in array order
+  float* cons;          // constraint SoA: field f of lane i at cons[f * consStride + i]
+  long consStride;
+  int* scratch;         // large-world sort scratch
+  float4* sapBoxes;     // large-world sorted fat AABBs
+  int* events;          // reserved
+  int worldCount;
+};
+
+}  // namespace f2d

@@ -1,0 +1,997 @@
+// Fused world step for gfx950: one workgroup advances one world through a whole
+// b2World_Step (S/world.c:695-812) with workgroup barriers between the stages.
+//
+//   pairs     sort-and-sweep over fat AABBs (LDS bitonic sort of lower-x keys),
+//             new pairs emitted with wavefront atomics, ordered like the
+//             reference's move buffer, then contacts created serially
+//             (S/broad_phase.c:160-474, S/contact.c:186-328)
+//   collide   one contact per lane: overlap test, manifold, warm-start id
+//             matching (S/world.c:357-445, S/contact.c:472-633)
+//   state     contact begin/end/destroy in ascending contact id with persistent
+//             greedy colouring (S/world.c:587-690, S/constraint_graph.c:68-212)
+//   solve     prepare, then substeps x (integrate velocities, warm start, solve,
+//             integrate positions, relax), restitution, store; overflow colour
+//             first in each stage, then one colour per barrier, one constraint
+//             per lane, body states resident in LDS (S/solver.c:929-1106)
+//   finalize  one body per lane: transforms, AABBs, fat-AABB enlarge and the
+//             next move buffer (S/solver.c:543-726, 1819-1911)
+//
+// Thousands of independent worlds are stepped by one launch (one workgroup
+// each); a single world is the same launch with one workgroup.
+#include <hip/hip_runtime.h>
+
+#include "manifold.hpp"
+#include "solver.hpp"
+#include "joint.hpp"
+
+namespace f2d {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kLdsShapes = 1024;   // sort-and-sweep stays in LDS up to this many shapes
+constexpr int kLdsBodies = 1024;   // body states stay in LDS up to this many bodies
+constexpr int kLdsPairs = 1024;    // new-pair sort in LDS up to this many pairs
+constexpr int kLdsBytes = 32768;
+
+struct SharedState {
+  int colorCount[kGraphColorCount];
+  int colorStart[kGraphColorCount + 1];
+  int colorFill[kGraphColorCount];
+  int jointColorCount[kGraphColorCount];
+  int jointColorStart[kGraphColorCount + 1];
+  int jointColorFill[kGraphColorCount];
+  int waveSums[kWaves];
+  int counter;
+  int changedCount;
+  int error;
+  int pairCount;
+};
+
+// ---------------------------------------------------------------------------- block helpers
+
+// Ordered block-wide compaction step: returns this thread's slot among flagged
+// threads of the block and writes the block total.
+__device__ __forceinline__ int blockCompact(bool flag, int* waveSums, int* total) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  unsigned long long mask = __ballot(flag);
+  unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int prefix = __popcll(mask & lt);
+  if (lane == 0) waveSums[wave] = __popcll(mask);
+  __syncthreads();
+  int base = 0, sum = 0;
+  for (int w = 0; w < kWaves; ++w) {
+    int s = waveSums[w];
+    if (w < wave) base += s;
+    sum += s;
+  }
+  __syncthreads();
+  *total = sum;
+  return base + prefix;
+}
+
+template <typename K, typename V>
+__device__ void bitonicSort(K* key, V* val, int n) {
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < n; i += kBlock) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          bool up = (i & k) == 0;
+          K a = key[i], b = key[ixj];
+          if ((a > b) == up) {
+            key[i] = b;
+            key[ixj] = a;
+            V t = val[i];
+            val[i] = val[ixj];
+            val[ixj] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t orderedKey(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ int nextPow2(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+// b2ShouldShapesCollide, S/shape.h:122-130
+__device__ __forceinline__ bool shouldShapesCollide(const Shape& a, const Shape& b) {
+  if (a.group == b.group && a.group != 0) return a.group > 0;
+  uint64_t maskA = (uint64_t)a.maskLo | ((uint64_t)a.maskHi << 32);
+  uint64_t catA = (uint64_t)a.catLo | ((uint64_t)a.catHi << 32);
+  uint64_t maskB = (uint64_t)b.maskLo | ((uint64_t)b.maskHi << 32);
+  uint64_t catB = (uint64_t)b.catLo | ((uint64_t)b.catHi << 32);
+  return (maskA & catB) != 0 && (catA & maskB) != 0;
+}
+
+// b2ShouldBodiesCollide, S/body.c:1861-1897 (joint part; the type test is done by the caller)
+__device__ __forceinline__ bool shouldBodiesCollide(const BodySim* sims, const Joint* joints, int bodyA, int bodyB) {
+  int jointKey, other;
+  if (sims[bodyA].jointCount < sims[bodyB].jointCount) {
+    jointKey = sims[bodyA].headJointKey;
+    other = bodyB;
+  } else {
+    jointKey = sims[bodyB].headJointKey;
+    other = bodyA;
+  }
+  for (int guard = sims[bodyA].jointCount + sims[bodyB].jointCount; jointKey >= 0 && guard >= 0; --guard) {
+    int jointId = jointKey >> 1;
+    int edge = jointKey & 1;
+    const Joint& j = joints[jointId];
+    int otherBody = edge == 0 ? j.bodyB : j.bodyA;
+    if (j.collideConnected == 0 && otherBody == other) return false;
+    jointKey = edge == 0 ? j.nextA : j.nextB;
+  }
+  return true;
+}
+
+// Does a contact between shapes s and t exist? Walks the shorter body contact list
+// (the reference keeps the same information in its pair set, S/broad_phase.c:221-226).
+__device__ __forceinline__ bool contactExists(const BodySim* sims, const Contact* contacts, int bodyS, int bodyT,
+                                              int s, int t) {
+  int body = sims[bodyS].contactCount <= sims[bodyT].contactCount ? bodyS : bodyT;
+  int key = sims[body].headContactKey;
+  // the walk is bounded by the list length; a longer walk means a corrupted list
+  for (int guard = sims[body].contactCount; key >= 0 && guard >= 0; --guard) {
+    const Contact& c = contacts[key >> 1];
+    if ((c.shapeA == s && c.shapeB == t) || (c.shapeA == t && c.shapeB == s)) return true;
+    key = (key & 1) == 0 ? c.nextA : c.nextB;
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------- phase: pairs
+
+struct WorldPtrs {
+  WorldHeader* hdr;
+  BodySim* sims;
+  BodyState* states;
+  Shape* shapes;
+  ShapeGeom* geoms;
+  Contact* contacts;
+  int* freeIds;
+  Joint* joints;
+  uint64_t* pairKeys;
+  int2* pairShapes;
+  uint32_t* colorBits;
+  int* overflow;
+  Cons cons;
+  int* scratch;
+  float4* sapBoxes;
+};
+
+__device__ WorldPtrs worldPtrs(const BatchView& b, WorldHeader* h) {
+  WorldPtrs w;
+  w.hdr = h;
+  w.sims = b.sims + h->bodyOffset;
+  w.states = b.states + h->bodyOffset;
+  w.shapes = b.shapes + h->shapeOffset;
+  w.geoms = b.geoms + h->shapeOffset;
+  w.contacts = b.contacts + h->contactOffset;
+  w.freeIds = b.freeIds + h->freeOffset;
+  w.joints = b.joints + h->jointOffset;
+  w.pairKeys = b.pairs + h->pairOffset;
+  w.pairShapes = b.pairShapes + h->pairOffset;
+  w.colorBits = b.colorBits + h->colorOffset;
+  w.overflow = b.overflow + h->overflowOffset;
+  w.cons.f = b.cons + h->constraintOffset;
+  w.cons.stride = b.consStride;
+  w.scratch = b.scratch + h->scratchOffset;
+  w.sapBoxes = b.sapBoxes + h->shapeOffset;
+  return w;
+}
+
+// Evaluate one overlapping candidate (s, t) of the sweep. Emits a new pair when the
+// reference's pair query would: at least one proxy moved, tree rules, filters, no
+// existing contact. The emitter is the proxy whose query reports the pair.
+__device__ void considerPair(const WorldPtrs& w, SharedState& sh, int s, int t) {
+  const Shape& S = w.shapes[s];
+  const Shape& T = w.shapes[t];
+  int bs = S.body, bt = T.body;
+  if (bs == bt) return;
+  int typeS = w.sims[bs].type, typeT = w.sims[bt].type;
+  if (typeS != kDynamic && typeT != kDynamic) return;
+  bool movedS = (S.flags & SF_MOVED) != 0, movedT = (T.flags & SF_MOVED) != 0;
+  int emitter, partner;
+  if (typeS == kDynamic && typeT == kDynamic) {
+    if (movedS && movedT) {
+      emitter = S.proxyKey < T.proxyKey ? s : t;
+    } else if (movedS) {
+      emitter = s;
+    } else if (movedT) {
+      emitter = t;
+    } else {
+      return;
+    }
+  } else {
+    int d = typeS == kDynamic ? s : t;
+    int x = typeS == kDynamic ? t : s;
+    bool movedD = typeS == kDynamic ? movedS : movedT;
+    bool movedX = typeS == kDynamic ? movedT : movedS;
+    if (movedD) {
+      emitter = d;
+    } else if (movedX) {
+      emitter = x;
+    } else {
+      return;
+    }
+  }
+  partner = emitter == s ? t : s;
+  if ((S.flags | T.flags) & SF_SENSOR) return;
+  if (!shouldShapesCollide(S, T)) return;
+  if (!shouldBodiesCollide(w.sims, w.joints, bs, bt)) return;
+  if (contactExists(w.sims, w.contacts, bs, bt, s, t)) return;
+  const Shape& E = w.shapes[emitter];
+  const Shape& P = w.shapes[partner];
+  // Order within one move result: dynamic-tree partners, then static, then kinematic
+  // (the query order reversed by the list prepend, S/broad_phase.c:345-371, 285-287).
+  int ptype = P.proxyKey & 3;
+  int treeRank = ptype == kDynamic ? 0 : (ptype == kStatic ? 1 : 2);
+  uint32_t partnerOrder = 0x3FFFFFFFu - (uint32_t)(P.proxyKey >> 2);
+  uint64_t key = ((uint64_t)(uint32_t)E.moveRank << 32) | ((uint64_t)treeRank << 30) | partnerOrder;
+  int idx = atomicAdd(&sh.pairCount, 1);
+  if (idx < w.hdr->pairCap) {
+    w.pairKeys[idx] = key;
+    w.pairShapes[idx] = make_int2(emitter, partner);
+  }
+}
+
+// b2CreateContact, S/contact.c:186-328 (serial, thread 0)
+__device__ void createContact(const WorldPtrs& w, int shapeIdA, int shapeIdB) {
+  int ta = w.geoms[shapeIdA].type, tb = w.geoms[shapeIdB].type;
+  if (!hasManifoldFcn(ta, tb)) return;
+  if (!isPrimary(ta, tb)) {
+    int tmp = shapeIdA;
+    shapeIdA = shapeIdB;
+    shapeIdB = tmp;
+  }
+  WorldHeader* h = w.hdr;
+  int id;
+  if (h->freeCount > 0) {
+    id = w.freeIds[--h->freeCount];
+  } else {
+    if (h->contactCount >= h->contactCap) {
+      h->error |= ERR_CONTACT_CAP;
+      return;
+    }
+    id = h->contactCount++;
+  }
+  const Shape& A = w.shapes[shapeIdA];
+  const Shape& B = w.shapes[shapeIdB];
+  Contact& c = w.contacts[id];
+  c.shapeA = shapeIdA;
+  c.shapeB = shapeIdB;
+  c.flags = ((A.flags | B.flags) & SF_CONTACT_EVENTS) ? CF_CONTACT_EVENTS : 0;
+  c.simFlags = ((A.flags | B.flags) & SF_PRESOLVE) ? SIM_PRESOLVE : 0;
+  c.color = -1;
+  c.localIndex = -1;
+  int bodyA = A.body, bodyB = B.body;
+  {
+    BodySim& ba = w.sims[bodyA];
+    c.bodyA = bodyA;
+    c.prevA = -1;
+    c.nextA = ba.headContactKey;
+    int keyA = (id << 1) | 0;
+    int head = ba.headContactKey;
+    if (head >= 0) {
+      Contact& hc = w.contacts[head >> 1];
+      if ((head & 1) == 0) hc.prevA = keyA; else hc.prevB = keyA;
+    }
+    ba.headContactKey = keyA;
+    ba.contactCount += 1;
+  }
+  {
+    BodySim& bb = w.sims[bodyB];
+    c.bodyB = bodyB;
+    c.prevB = -1;
+    c.nextB = bb.headContactKey;
+    int keyB = (id << 1) | 1;
+    int head = bb.headContactKey;
+    if (head >= 0) {
+      Contact& hc = w.contacts[head >> 1];
+      if ((head & 1) == 0) hc.prevA = keyB; else hc.prevB = keyB;
+    }
+    bb.headContactKey = keyB;
+    bb.contactCount += 1;
+  }
+  zeroManifold(c.m);
+  c.friction = sqrtf(A.friction * B.friction);
+  c.restitution = maxf(A.restitution, B.restitution);
+  c.rollingResistance = 0.0f;
+  c.tangentSpeed = 0.0f;
+}
+
+__device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds) {
+  WorldHeader* h = w.hdr;
+  const int n = h->shapeCount;
+  if (threadIdx.x == 0) sh.pairCount = 0;
+  // any moved proxy at all?
+  int anyMoved = 0;
+  for (int i = threadIdx.x; i < n; i += kBlock) anyMoved |= (w.shapes[i].flags & SF_MOVED) ? 1 : 0;
+  anyMoved = __syncthreads_or(anyMoved);
+  if (!anyMoved) return;
+
+  const int np = nextPow2(n);
+  const bool inLds = np <= kLdsShapes;
+  uint64_t* keys = inLds ? reinterpret_cast<uint64_t*>(lds) : reinterpret_cast<uint64_t*>(w.scratch);
+  int* dummyVals = nullptr;
+  (void)dummyVals;
+  for (int i = threadIdx.x; i < np; i += kBlock) {
+    uint64_t k = ~0ull;
+    if (i < n) k = ((uint64_t)orderedKey(w.shapes[i].fat.lo.x) << 32) | (uint32_t)i;
+    keys[i] = k;
+  }
+  __syncthreads();
+  // sort keys only (index rides in the low word)
+  for (int k = 2; k <= np; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < np; i += kBlock) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          bool up = (i & k) == 0;
+          uint64_t a = keys[i], b = keys[ixj];
+          if ((a > b) == up) {
+            keys[i] = b;
+            keys[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // sorted boxes and indices
+  float4* boxes = inLds ? reinterpret_cast<float4*>(lds + kLdsShapes * 8) : w.sapBoxes;
+  int* order = inLds ? reinterpret_cast<int*>(lds + kLdsShapes * 8 + kLdsShapes * 16) : (w.scratch + 2 * np);
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    int idx = (int)(uint32_t)keys[i];
+    AABB f = w.shapes[idx].fat;
+    boxes[i] = make_float4(f.lo.x, f.lo.y, f.hi.x, f.hi.y);
+    order[i] = idx;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    float4 a = boxes[i];
+    int s = order[i];
+    for (int j = i + 1; j < n; ++j) {
+      float4 bx = boxes[j];
+      if (bx.x > a.z) break;
+      // b2AABB_Overlaps on the two fat boxes (x already known to overlap on one side)
+      bool overlap = !(bx.x > a.z || bx.y > a.w || a.x > bx.z || a.y > bx.w);
+      if (overlap) considerPair(w, sh, s, order[j]);
+    }
+  }
+  __syncthreads();
+  int count = sh.pairCount;
+  if (count > h->pairCap) {
+    if (threadIdx.x == 0) h->error |= ERR_PAIR_CAP;
+    count = h->pairCap;
+  }
+  if (count > 0) {
+    // sort by move order key; values are pair indices
+    const int pp = nextPow2(count);
+    const bool pairsInLds = pp <= kLdsPairs;
+    uint64_t* pk = pairsInLds ? reinterpret_cast<uint64_t*>(lds) : reinterpret_cast<uint64_t*>(w.scratch);
+    int* pv = pairsInLds ? reinterpret_cast<int*>(lds + kLdsPairs * 8) : (w.scratch + 2 * pp);
+    __syncthreads();
+    for (int i = threadIdx.x; i < pp; i += kBlock) {
+      pk[i] = i < count ? w.pairKeys[i] : ~0ull;
+      pv[i] = i;
+    }
+    __syncthreads();
+    bitonicSort(pk, pv, pp);
+    if (threadIdx.x == 0) {
+      for (int i = 0; i < count; ++i) {
+        int2 pr = w.pairShapes[pv[i]];
+        int e = pr.x, p = pr.y;
+        // orientation by proxy key, S/broad_phase.c:228-237
+        int a, bsh;
+        if (w.shapes[p].proxyKey < w.shapes[e].proxyKey) {
+          a = p;
+          bsh = e;
+        } else {
+          a = e;
+          bsh = p;
+        }
+        createContact(w, a, bsh);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kBlock) w.shapes[i].flags &= ~SF_MOVED;
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------- phase: collide
+
+// b2UpdateContact, S/contact.c:472-633 (no pre-solve callback on the device)
+__device__ void updateContact(const WorldPtrs& w, Contact& c) {
+  const Shape& A = w.shapes[c.shapeA];
+  const Shape& B = w.shapes[c.shapeB];
+  const BodySim& simA = w.sims[A.body];
+  const BodySim& simB = w.sims[B.body];
+  Xf xfA = simA.xf, xfB = simB.xf;
+  Vec2 centerOffsetA = rotate(xfA.q, simA.localCenter);
+  Vec2 centerOffsetB = rotate(xfB.q, simB.localCenter);
+  bool wasTouching = (c.simFlags & SIM_TOUCHING) != 0;
+  Manifold old = c.m;
+  Manifold m;
+  computeManifold(m, w.geoms[c.shapeA], xfA, w.geoms[c.shapeB], xfB);
+  c.friction = sqrtf(A.friction * B.friction);
+  c.restitution = maxf(A.restitution, B.restitution);
+  if (A.rollingResistance > 0.0f || B.rollingResistance > 0.0f) {
+    float radiusA = shapeRadius(w.geoms[c.shapeA]);
+    float radiusB = shapeRadius(w.geoms[c.shapeB]);
+    float maxRadius = maxf(radiusA, radiusB);
+    c.rollingResistance = maxf(A.rollingResistance, B.rollingResistance) * maxRadius;
+  } else {
+    c.rollingResistance = 0.0f;
+  }
+  c.tangentSpeed = A.tangentSpeed + B.tangentSpeed;
+  int pointCount = m.pointCount;
+  bool touching = pointCount > 0;
+  if (touching && ((A.flags | B.flags) & SF_HIT_EVENTS)) {
+    c.simFlags |= SIM_HIT_EVENT;
+  } else {
+    c.simFlags &= ~SIM_HIT_EVENT;
+  }
+  if (pointCount > 0) m.rollingImpulse = old.rollingImpulse;
+  for (int i = 0; i < pointCount; ++i) {
+    ManifoldPoint& mp2 = m.points[i];
+    mp2.anchorA = sub(mp2.anchorA, centerOffsetA);
+    mp2.anchorB = sub(mp2.anchorB, centerOffsetB);
+    mp2.normalImpulse = 0.0f;
+    mp2.tangentImpulse = 0.0f;
+    mp2.totalNormalImpulse = 0.0f;
+    mp2.normalVelocity = 0.0f;
+    int id2 = mp2.id & 0xFFFF;
+    mp2.id = id2;  // persisted = false
+    for (int j = 0; j < old.pointCount; ++j) {
+      ManifoldPoint& mp1 = old.points[j];
+      if ((mp1.id & 0xFFFF) == id2) {
+        mp2.normalImpulse = mp1.normalImpulse;
+        mp2.tangentImpulse = mp1.tangentImpulse;
+        mp2.id = id2 | 0x10000;
+        mp1.normalImpulse = 0.0f;
+        mp1.tangentImpulse = 0.0f;
+        break;
+      }
+    }
+  }
+  if (touching) {
+    c.simFlags |= SIM_TOUCHING;
+  } else {
+    c.simFlags &= ~SIM_TOUCHING;
+  }
+  if (touching && !wasTouching) {
+    c.simFlags |= SIM_STARTED;
+  } else if (!touching && wasTouching) {
+    c.simFlags |= SIM_STOPPED;
+  }
+  c.m = m;
+}
+
+__device__ void collidePhase(const WorldPtrs& w, SharedState& sh) {
+  WorldHeader* h = w.hdr;
+  const int n = h->contactCount;
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    Contact& c = w.contacts[i];
+    if (c.shapeA < 0) continue;
+    AABB fa = w.shapes[c.shapeA].fat, fb = w.shapes[c.shapeB].fat;
+    if (!aabbOverlaps(fa, fb)) {
+      c.simFlags |= SIM_DISJOINT;
+      c.simFlags &= ~SIM_TOUCHING;
+    } else {
+      updateContact(w, c);
+    }
+  }
+  __syncthreads();
+  // ordered list of contacts whose state changed (ascending contact id)
+  if (threadIdx.x == 0) sh.changedCount = 0;
+  __syncthreads();
+  int* list = w.scratch;
+  for (int base = 0; base < n; base += kBlock) {
+    int i = base + threadIdx.x;
+    bool f = false;
+    if (i < n) {
+      const Contact& c = w.contacts[i];
+      f = c.shapeA >= 0 && (c.simFlags & (SIM_DISJOINT | SIM_STARTED | SIM_STOPPED)) != 0;
+    }
+    int total;
+    int slot = blockCompact(f, sh.waveSums, &total);
+    if (f) list[sh.changedCount + slot] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) sh.changedCount += total;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------- phase: contact state + colouring
+
+__device__ __forceinline__ uint32_t* colorSet(const WorldPtrs& w, int color) {
+  return w.colorBits + (long)color * w.hdr->colorWords;
+}
+__device__ __forceinline__ bool getBit(const uint32_t* set, int i) { return (set[i >> 5] >> (i & 31)) & 1u; }
+__device__ __forceinline__ void setBit(uint32_t* set, int i) { set[i >> 5] |= 1u << (i & 31); }
+__device__ __forceinline__ void clearBit(uint32_t* set, int i) { set[i >> 5] &= ~(1u << (i & 31)); }
+
+// b2AddContactToGraph colour choice, S/constraint_graph.c:68-135
+__device__ void addContactToGraph(const WorldPtrs& w, int id) {
+  Contact& c = w.contacts[id];
+  int color = kOverflowIndex;
+  int bodyA = c.bodyA, bodyB = c.bodyB;
+  bool staticA = w.sims[bodyA].type == kStatic;
+  bool staticB = w.sims[bodyB].type == kStatic;
+  if (!staticA && !staticB) {
+    for (int i = 0; i < kOverflowIndex; ++i) {
+      uint32_t* set = colorSet(w, i);
+      if (getBit(set, bodyA) || getBit(set, bodyB)) continue;
+      setBit(set, bodyA);
+      setBit(set, bodyB);
+      color = i;
+      break;
+    }
+  } else if (!staticA) {
+    for (int i = 1; i < kOverflowIndex; ++i) {
+      uint32_t* set = colorSet(w, i);
+      if (getBit(set, bodyA)) continue;
+      setBit(set, bodyA);
+      color = i;
+      break;
+    }
+  } else if (!staticB) {
+    for (int i = 1; i < kOverflowIndex; ++i) {
+      uint32_t* set = colorSet(w, i);
+      if (getBit(set, bodyB)) continue;
+      setBit(set, bodyB);
+      color = i;
+      break;
+    }
+  }
+  c.color = color;
+  if (color == kOverflowIndex) {
+    WorldHeader* h = w.hdr;
+    if (h->overflowCount >= h->contactCap) {
+      h->error |= ERR_OVERFLOW_CAP;
+      c.localIndex = -1;
+      return;
+    }
+    c.localIndex = h->overflowCount;
+    w.overflow[h->overflowCount++] = id;
+  } else {
+    c.localIndex = -1;
+  }
+}
+
+// b2RemoveContactFromGraph, S/constraint_graph.c:184-212
+__device__ void removeContactFromGraph(const WorldPtrs& w, int id) {
+  Contact& c = w.contacts[id];
+  int color = c.color;
+  if (color < 0) return;
+  if (color != kOverflowIndex) {
+    uint32_t* set = colorSet(w, color);
+    clearBit(set, c.bodyA);
+    clearBit(set, c.bodyB);
+  } else {
+    WorldHeader* h = w.hdr;
+    int idx = c.localIndex;
+    int last = --h->overflowCount;
+    if (idx != last && idx >= 0) {
+      int moved = w.overflow[last];
+      w.overflow[idx] = moved;
+      w.contacts[moved].localIndex = idx;
+    }
+  }
+  c.color = -1;
+  c.localIndex = -1;
+}
+
+// b2DestroyContact, S/contact.c:337-454 (no island or event bookkeeping on the device)
+__device__ void destroyContact(const WorldPtrs& w, int id) {
+  Contact& c = w.contacts[id];
+  // remove from body A list
+  {
+    if (c.prevA >= 0) {
+      Contact& pc = w.contacts[c.prevA >> 1];
+      if ((c.prevA & 1) == 0) pc.nextA = c.nextA; else pc.nextB = c.nextA;
+    }
+    if (c.nextA >= 0) {
+      Contact& nc = w.contacts[c.nextA >> 1];
+      if ((c.nextA & 1) == 0) nc.prevA = c.prevA; else nc.prevB = c.prevA;
+    }
+    BodySim& ba = w.sims[c.bodyA];
+    if (ba.headContactKey == ((id << 1) | 0)) ba.headContactKey = c.nextA;
+    ba.contactCount -= 1;
+  }
+  {
+    if (c.prevB >= 0) {
+      Contact& pc = w.contacts[c.prevB >> 1];
+      if ((c.prevB & 1) == 0) pc.nextA = c.nextB; else pc.nextB = c.nextB;
+    }
+    if (c.nextB >= 0) {
+      Contact& nc = w.contacts[c.nextB >> 1];
+      if ((c.nextB & 1) == 0) nc.prevA = c.prevB; else nc.prevB = c.prevB;
+    }
+    BodySim& bb = w.sims[c.bodyB];
+    if (bb.headContactKey == ((id << 1) | 1)) bb.headContactKey = c.nextB;
+    bb.contactCount -= 1;
+  }
+  if (c.color >= 0) removeContactFromGraph(w, id);
+  c.shapeA = -1;
+  c.shapeB = -1;
+  c.flags = 0;
+  c.simFlags = 0;
+  WorldHeader* h = w.hdr;
+  w.freeIds[h->freeCount++] = id;
+}
+
+// Serial contact state loop, S/world.c:587-690
+__device__ void statePhase(const WorldPtrs& w, SharedState& sh) {
+  if (threadIdx.x == 0) {
+    const int* list = w.scratch;
+    const int n = sh.changedCount;
+    for (int k = 0; k < n; ++k) {
+      int id = list[k];
+      Contact& c = w.contacts[id];
+      int simFlags = c.simFlags;
+      if (simFlags & SIM_DISJOINT) {
+        destroyContact(w, id);
+      } else if (simFlags & SIM_STARTED) {
+        c.flags |= CF_TOUCHING;
+        c.simFlags &= ~SIM_STARTED;
+        addContactToGraph(w, id);
+      } else if (simFlags & SIM_STOPPED) {
+        c.simFlags &= ~SIM_STOPPED;
+        c.flags &= ~CF_TOUCHING;
+        removeContactFromGraph(w, id);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------- phase: solve + finalize
+
+struct SolveCtx {
+  float h, inv_h, dt, inv_dt;
+  int subSteps;
+  Softness contactSoftness, staticSoftness;
+  float contactSpeed, maxContactPushSpeed, restitutionThreshold, maxLinearSpeed, maxAngularSpeed;
+  float warmStartScale;
+  Vec2 gravity;
+  bool enableContinuous, enableSleep;
+};
+
+template <bool kLds>
+__device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState* ldsStates, const SolveCtx& ctx,
+                                 int phases) {
+  WorldHeader* h = w.hdr;
+  const int nb = h->bodyCount;
+  const int nc = h->contactCount;
+  BodyState* states = kLds ? ldsStates : w.states;
+  const Cons& cons = w.cons;
+
+  if (phases & PH_SOLVE) {
+    // -- bucket touching constraints by colour (overflow keeps its list order)
+    if (threadIdx.x < kGraphColorCount) {
+      sh.colorCount[threadIdx.x] = 0;
+      sh.colorFill[threadIdx.x] = 0;
+      sh.jointColorCount[threadIdx.x] = 0;
+      sh.jointColorFill[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nc; i += kBlock) {
+      const Contact& c = w.contacts[i];
+      if (c.shapeA >= 0 && c.color >= 0 && c.color < kOverflowIndex) atomicAdd(&sh.colorCount[c.color], 1);
+    }
+    for (int i = threadIdx.x; i < h->jointCount; i += kBlock) {
+      const Joint& j = w.joints[i];
+      if (j.color >= 0 && j.color < kOverflowIndex) atomicAdd(&sh.jointColorCount[j.color], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int s = 0, sj = 0, active = 0;
+      for (int k = 0; k < kOverflowIndex; ++k) {
+        sh.colorStart[k] = s;
+        s += sh.colorCount[k];
+        sh.jointColorStart[k] = sj;
+        sj += sh.jointColorCount[k];
+        active += (sh.colorCount[k] + sh.jointColorCount[k]) > 0 ? 1 : 0;
+      }
+      sh.colorStart[kOverflowIndex] = s;
+      sh.colorCount[kOverflowIndex] = h->overflowCount;
+      sh.colorStart[kGraphColorCount] = s + h->overflowCount;
+      sh.jointColorStart[kOverflowIndex] = sj;
+      h->activeColors = active;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nc; i += kBlock) {
+      const Contact& c = w.contacts[i];
+      if (c.shapeA >= 0 && c.color >= 0 && c.color < kOverflowIndex) {
+        int lane = sh.colorStart[c.color] + atomicAdd(&sh.colorFill[c.color], 1);
+        cons.seti(CN_SLOT, lane, i);
+      }
+    }
+    for (int i = threadIdx.x; i < h->overflowCount; i += kBlock) {
+      cons.seti(CN_SLOT, sh.colorStart[kOverflowIndex] + i, w.overflow[i]);
+    }
+    int* jointLanes = w.scratch;  // joint ids by colour
+    for (int i = threadIdx.x; i < h->jointCount; i += kBlock) {
+      const Joint& j = w.joints[i];
+      if (j.color >= 0 && j.color < kOverflowIndex) {
+        jointLanes[sh.jointColorStart[j.color] + atomicAdd(&sh.jointColorFill[j.color], 1)] = i;
+      }
+    }
+    // body states into LDS
+    if (kLds) {
+      for (int i = threadIdx.x; i < nb; i += kBlock) ldsStates[i] = w.states[i];
+    }
+    __syncthreads();
+    const int totalLanes = sh.colorStart[kGraphColorCount];
+    const int overflowStart = sh.colorStart[kOverflowIndex];
+    const int overflowCount = sh.colorCount[kOverflowIndex];
+    const int coloredJoints = sh.jointColorStart[kOverflowIndex];
+
+    // -- prepare joints (all), S/solver.c:967-987 -> S/joint.c:1348-1402
+    for (int i = threadIdx.x; i < h->jointCount; i += kBlock) {
+      prepareRevolute(w.joints[i], w.sims, ctx.h, ctx.inv_h, ctx.warmStartScale);
+    }
+    // -- prepare contacts (all lanes), S/contact_solver.c:1438-1669
+    for (int lane = threadIdx.x; lane < totalLanes; lane += kBlock) {
+      int slot = cons.geti(CN_SLOT, lane);
+      const Contact& c = w.contacts[slot];
+      const BodySim& sa = w.sims[c.bodyA];
+      const BodySim& sb = w.sims[c.bodyB];
+      int ia = sa.type == kStatic ? -1 : c.bodyA;
+      int ib = sb.type == kStatic ? -1 : c.bodyB;
+      float mA = ia < 0 ? 0.0f : sa.invMass, iA = ia < 0 ? 0.0f : sa.invI;
+      float mB = ib < 0 ? 0.0f : sb.invMass, iB = ib < 0 ? 0.0f : sb.invI;
+      prepareContactLane(cons, lane, c, slot, ia, ib, mA, iA, mB, iB, states, ctx.contactSoftness,
+                         ctx.staticSoftness, ctx.warmStartScale);
+    }
+    __syncthreads();
+
+    BodyState dummy = identityState();
+    for (int sub = 0; sub < ctx.subSteps; ++sub) {
+      // integrate velocities
+      for (int i = threadIdx.x; i < nb; i += kBlock) {
+        BodySim& sim = w.sims[i];
+        if (sim.type == kStatic) continue;
+        BodyState st = states[i];
+        integrateVelocity(sim, st, ctx.gravity, ctx.h, ctx.maxLinearSpeed, ctx.maxAngularSpeed);
+        states[i] = st;
+      }
+      __syncthreads();
+      // warm start: overflow joints, overflow contacts, then colours
+      if (threadIdx.x == 0) {
+        for (int j = 0; j < h->jointCount; ++j)
+          if (w.joints[j].color == kOverflowIndex) warmStartRevolute(w.joints[j], states, w.sims);
+        dummy = identityState();
+        for (int k = 0; k < overflowCount; ++k) overflowWarmStart(cons, overflowStart + k, states, dummy);
+      }
+      __syncthreads();
+      for (int col = 0; col < kOverflowIndex; ++col) {
+        int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
+        if (cnt + jcnt == 0) continue;
+        for (int k = threadIdx.x; k < jcnt; k += kBlock)
+          warmStartRevolute(w.joints[jointLanes[sh.jointColorStart[col] + k]], states, w.sims);
+        for (int k = threadIdx.x; k < cnt; k += kBlock) warmStartLane(cons, sh.colorStart[col] + k, states);
+        __syncthreads();
+      }
+      // solve
+      if (threadIdx.x == 0) {
+        for (int j = 0; j < h->jointCount; ++j)
+          if (w.joints[j].color == kOverflowIndex) solveRevolute(w.joints[j], states, w.sims, ctx.h, ctx.inv_h, true);
+        dummy = identityState();
+        for (int k = 0; k < overflowCount; ++k)
+          overflowSolve(cons, overflowStart + k, states, dummy, ctx.inv_h, ctx.maxContactPushSpeed, true);
+      }
+      __syncthreads();
+      for (int col = 0; col < kOverflowIndex; ++col) {
+        int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
+        if (cnt + jcnt == 0) continue;
+        for (int k = threadIdx.x; k < jcnt; k += kBlock)
+          solveRevolute(w.joints[jointLanes[sh.jointColorStart[col] + k]], states, w.sims, ctx.h, ctx.inv_h, true);
+        for (int k = threadIdx.x; k < cnt; k += kBlock)
+          solveLane(cons, sh.colorStart[col] + k, states, ctx.inv_h, -ctx.contactSpeed, true);
+        __syncthreads();
+      }
+      // integrate positions
+      for (int i = threadIdx.x; i < nb; i += kBlock) {
+        if (w.sims[i].type == kStatic) continue;
+        BodyState st = states[i];
+        integratePosition(st, ctx.h);
+        states[i] = st;
+      }
+      __syncthreads();
+      // relax
+      if (threadIdx.x == 0) {
+        for (int j = 0; j < h->jointCount; ++j)
+          if (w.joints[j].color == kOverflowIndex) solveRevolute(w.joints[j], states, w.sims, ctx.h, ctx.inv_h, false);
+        dummy = identityState();
+        for (int k = 0; k < overflowCount; ++k)
+          overflowSolve(cons, overflowStart + k, states, dummy, ctx.inv_h, ctx.maxContactPushSpeed, false);
+      }
+      __syncthreads();
+      for (int col = 0; col < kOverflowIndex; ++col) {
+        int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
+        if (cnt + jcnt == 0) continue;
+        for (int k = threadIdx.x; k < jcnt; k += kBlock)
+          solveRevolute(w.joints[jointLanes[sh.jointColorStart[col] + k]], states, w.sims, ctx.h, ctx.inv_h, false);
+        for (int k = threadIdx.x; k < cnt; k += kBlock)
+          solveLane(cons, sh.colorStart[col] + k, states, ctx.inv_h, -ctx.contactSpeed, false);
+        __syncthreads();
+      }
+    }
+    // restitution: overflow then colours
+    if (threadIdx.x == 0) {
+      dummy = identityState();
+      for (int k = 0; k < overflowCount; ++k)
+        overflowRestitution(cons, overflowStart + k, states, dummy, ctx.restitutionThreshold);
+    }
+    __syncthreads();
+    for (int col = 0; col < kOverflowIndex; ++col) {
+      int cnt = sh.colorCount[col];
+      if (cnt == 0) continue;
+      for (int k = threadIdx.x; k < cnt; k += kBlock)
+        restitutionLane(cons, sh.colorStart[col] + k, states, ctx.restitutionThreshold);
+      __syncthreads();
+    }
+    // store impulses
+    for (int lane = threadIdx.x; lane < totalLanes; lane += kBlock) {
+      int slot = cons.geti(CN_SLOT, lane);
+      if (lane >= overflowStart) {
+        overflowStore(cons, lane, w.contacts[slot]);
+      } else {
+        storeLane(cons, lane, w.contacts[slot]);
+      }
+    }
+    (void)coloredJoints;
+    __syncthreads();
+  } else if (kLds) {
+    for (int i = threadIdx.x; i < nb; i += kBlock) ldsStates[i] = w.states[i];
+    __syncthreads();
+  }
+
+  if (phases & PH_FINALIZE) {
+    // b2FinalizeBodiesTask, S/solver.c:543-726
+    for (int i = threadIdx.x; i < nb; i += kBlock) {
+      BodySim& sim = w.sims[i];
+      if (sim.type == kStatic) continue;
+      BodyState st = states[i];
+      Vec2 v = st.v;
+      float wv = st.w;
+      sim.center = add(sim.center, st.dp);
+      sim.xf.q = normalizeRot(mulRot(st.dq, sim.xf.q));
+      float maxVelocity = length(v) + absf(wv) * sim.maxExtent;
+      float maxDeltaPosition = length(st.dp) + absf(st.dq.s) * sim.maxExtent;
+      float positionSleepFactor = 0.5f;
+      float sleepVelocity = maxf(maxVelocity, positionSleepFactor * ctx.inv_dt * maxDeltaPosition);
+      st.dp = {0.0f, 0.0f};
+      st.dq = {1.0f, 0.0f};
+      sim.xf.p = sub(sim.center, rotate(sim.xf.q, sim.localCenter));
+      sim.force = {0.0f, 0.0f};
+      sim.torque = 0.0f;
+      int flags = sim.flags & ~(BF_IS_FAST | BF_ENLARGED);
+      // isSpeedCapped is reported for this step and cleared for the next
+      bool capped = (sim.flags & BF_SPEED_CAPPED) != 0;
+      flags &= ~BF_SPEED_CAPPED;
+      (void)capped;
+      bool isFast = false;
+      if (!ctx.enableSleep || (sim.flags & BF_ENABLE_SLEEP) == 0 || sleepVelocity > sim.sleepThreshold) {
+        sim.sleepTime = 0.0f;
+        if (sim.type == kDynamic && ctx.enableContinuous && maxVelocity * ctx.dt > 0.5f * sim.minExtent) {
+          // continuous collision is not run on the device; the body advances like a slow one
+          isFast = false;
+          sim.center0 = sim.center;
+          sim.rotation0 = sim.xf.q;
+        } else {
+          sim.center0 = sim.center;
+          sim.rotation0 = sim.xf.q;
+        }
+      } else {
+        sim.center0 = sim.center;
+        sim.rotation0 = sim.xf.q;
+        sim.sleepTime += ctx.dt;
+      }
+      Xf xf = sim.xf;
+      int rank = 0;
+      bool enlargedBody = false;
+      for (int s = sim.headShape; s >= 0 && rank < h->shapeCount; s = w.shapes[s].next, ++rank) {
+        Shape& shape = w.shapes[s];
+        AABB aabb = computeShapeAABB(w.geoms[s], xf);
+        aabb.lo.x -= kSpeculativeDistance;
+        aabb.lo.y -= kSpeculativeDistance;
+        aabb.hi.x += kSpeculativeDistance;
+        aabb.hi.y += kSpeculativeDistance;
+        shape.aabb = aabb;
+        if (!aabbContains(shape.fat, aabb)) {
+          AABB fat;
+          fat.lo.x = aabb.lo.x - kAabbMargin;
+          fat.lo.y = aabb.lo.y - kAabbMargin;
+          fat.hi.x = aabb.hi.x + kAabbMargin;
+          fat.hi.y = aabb.hi.y + kAabbMargin;
+          shape.fat = fat;
+          // enlarged proxy joins the move buffer in sim order (S/solver.c:1848-1903)
+          shape.flags |= SF_MOVED;
+          shape.moveRank = (i << 8) | (rank < 255 ? rank : 255);
+          enlargedBody = true;
+        }
+      }
+      if (enlargedBody) flags |= BF_ENLARGED;
+      if (isFast) flags |= BF_IS_FAST;
+      sim.flags = flags;
+      w.states[i] = st;
+    }
+    __syncthreads();
+  } else if (kLds && (phases & PH_SOLVE)) {
+    for (int i = threadIdx.x; i < nb; i += kBlock) w.states[i] = ldsStates[i];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------- kernel
+
+__global__ __launch_bounds__(kBlock) void stepWorldsKernel(BatchView b, StepParams sp, const int* worldList) {
+  __shared__ SharedState sh;
+  __shared__ __align__(16) char lds[kLdsBytes];
+  const int wi = worldList ? worldList[blockIdx.x] : blockIdx.x;
+  WorldHeader* h = b.worlds + wi;
+  WorldPtrs w = worldPtrs(b, h);
+  if (threadIdx.x == 0) sh.error = 0;
+
+  SolveCtx ctx;
+  ctx.dt = sp.dt;
+  ctx.inv_dt = sp.inv_dt;
+  ctx.h = sp.h;
+  ctx.inv_h = sp.inv_h;
+  ctx.subSteps = sp.subSteps;
+  // S/world.c:760-771
+  float contactHertz = minf(h->contactHertz, 0.125f * sp.inv_h);
+  ctx.contactSoftness = makeSoft(contactHertz, h->contactDampingRatio, sp.h);
+  ctx.staticSoftness = makeSoft(2.0f * contactHertz, h->contactDampingRatio, sp.h);
+  ctx.contactSpeed = h->maxContactPushSpeed / ctx.staticSoftness.massScale;
+  ctx.maxContactPushSpeed = h->maxContactPushSpeed;
+  ctx.restitutionThreshold = h->restitutionThreshold;
+  ctx.maxLinearSpeed = h->maxLinearSpeed;
+  ctx.maxAngularSpeed = kMaxRotation * sp.inv_dt;
+  ctx.warmStartScale = (h->flags & WF_WARM) ? 1.0f : 0.0f;
+  ctx.gravity = h->gravity;
+  ctx.enableContinuous = (h->flags & WF_CONTINUOUS) != 0;
+  ctx.enableSleep = (h->flags & WF_SLEEP) != 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    h->inv_h = sp.inv_h;
+    h->contactSpeed = ctx.contactSpeed;
+  }
+
+  if (sp.phases & PH_PAIRS) pairsPhase(w, sh, lds);
+  if (sp.phases & PH_COLLIDE) collidePhase(w, sh);
+  if (sp.phases & PH_STATE) statePhase(w, sh);
+  if (sp.phases & (PH_SOLVE | PH_FINALIZE)) {
+    if (h->bodyCount <= kLdsBodies) {
+      solveAndFinalize<true>(w, sh, reinterpret_cast<BodyState*>(lds), ctx, sp.phases);
+    } else {
+      solveAndFinalize<false>(w, sh, nullptr, ctx, sp.phases);
+    }
+  }
+  if (threadIdx.x == 0) h->stepIndex += 1;
+}
+
+}  // namespace f2d
+
+extern "C" int f2d_launch_step(const f2d::BatchView* b, const f2d::StepParams* sp, const int* worldList, int nWorlds,
+                               hipStream_t stream) {
+  if (nWorlds <= 0) return 0;
+  hipLaunchKernelGGL(f2d::stepWorldsKernel, dim3(nWorlds), dim3(f2d::kBlock), 0, stream, *b, *sp, worldList);
+  return (int)hipGetLastError();
+}

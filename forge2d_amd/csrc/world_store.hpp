@@ -1,0 +1,69 @@
+// Host world store: the authoritative description of worlds while they are built,
+// and the host mirror of device state after a step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "geometry.hpp"
+
+namespace f2d {
+
+struct HostWorld {
+  WorldHeader hdr = {};
+  std::vector<BodySim> sims;
+  std::vector<BodyState> states;
+  std::vector<Shape> shapes;
+  std::vector<ShapeGeom> geoms;
+  std::vector<Joint> joints;
+  std::vector<Contact> contacts;   // size = contact capacity after commit
+  std::vector<int> freeIds;        // LIFO stack
+  std::vector<int> overflow;
+  std::vector<uint32_t> colorBits; // kOverflowIndex x colorWords
+  std::vector<float> bodyMass, bodyInertia, shapeDensity;
+  std::vector<uint16_t> bodyGen;
+  std::vector<int> bodyFixedRotation;
+  int treeNodes[3] = {0, 0, 0};    // dynamic-tree node allocation mirror per body type
+  int createdSeq = 0;              // proxies created since the last step
+  int contactCount = 0;
+  int stepIndex = 0;
+  float contactSpeed = 0.0f, inv_h = 0.0f;
+  int colorWords = 0;
+  bool hasCallbacks = false;
+
+  void ensureColorWords();
+  int createBody(int type, Vec2 p, Rot q, Vec2 v, float w, float linDamp, float angDamp, float gravityScale,
+                 float sleepThreshold, int flags);
+  int createShape(int body, const ShapeGeom& g, float density, float friction, float restitution, float rolling,
+                  float tangentSpeed, uint64_t cat, uint64_t mask, int group, int flags, bool invokeContact,
+                  bool updateMass);
+  int createRevolute(int bodyA, int bodyB, Vec2 la, Vec2 lb, float refAngle, float targetAngle, bool enableSpring,
+                     float hertz, float damping, bool enableLimit, float lower, float upper, bool enableMotor,
+                     float maxMotorTorque, float motorSpeed, bool collideConnected);
+  void updateBodyMass(int body);
+};
+
+struct DeviceBatch {
+  BatchView view = {};
+  std::vector<void*> allocations;
+  size_t bytes = 0;
+};
+
+struct Batch {
+  std::vector<HostWorld> worlds;
+  DeviceBatch dev;
+  bool committed = false;
+  bool hostDirty = true;     // host holds changes the device lacks
+  bool deviceNewer = false;  // device holds a step the host mirror lacks
+  hipStream_t stream = nullptr;
+  bool ownsStream = false;
+  int* dWorldList = nullptr;
+  int dWorldListCap = 0;
+  std::string lastError;
+  float contactCapFactor = 6.0f;
+};
+
+}  // namespace f2d

@@ -1,0 +1,159 @@
+"""World-state snapshot format (checkpoint / parity fixtures).
+
+One snapshot holds the complete persistent state of one world after a step:
+bodies, shapes (AABB, fat AABB, proxy key), contacts by contact id (manifold,
+colour, flags), revolute joints, the constraint-graph colour arrays in array
+order, the broad-phase move array and the contact-id free list. The engine's
+``f2dg_export_world`` writes it and ``f2dg_import_world`` reads it; the
+reference fixture harness (``oracle/ref_harness.c``) writes the same layout
+from the reference's internal structures.
+
+Records (little-endian, packed):
+  HEADER 64 B, BODY 112 B, SHAPE 48 B, CONTACT 168 B, JOINT 64 B, then
+  int32 colour counts[12], int32 colour ids[sum], int32 move keys[n],
+  int32 free contact ids[n].
+"""
+from __future__ import annotations
+
+import numpy as np
+
+SNAP_MAGIC = 0x50414E53
+COLOR_COUNT = 12
+OVERFLOW = 11
+
+HEADER_DT = np.dtype([
+    ("magic", "<i4"), ("version", "<i4"), ("step", "<i4"), ("body_count", "<i4"), ("shape_count", "<i4"),
+    ("contact_cap", "<i4"), ("joint_count", "<i4"), ("overflow_count", "<i4"), ("move_count", "<i4"),
+    ("free_id_count", "<i4"), ("color_total", "<i4"), ("reserved0", "<i4"),
+    ("contact_speed", "<f4"), ("inv_h", "<f4"), ("reserved1", "<f4"), ("reserved2", "<f4"),
+])
+BODY_DT = np.dtype([
+    ("type", "<i4"), ("set_index", "<i4"), ("local_index", "<i4"), ("flags", "<i4"),
+    ("px", "<f4"), ("py", "<f4"), ("qc", "<f4"), ("qs", "<f4"), ("cx", "<f4"), ("cy", "<f4"),
+    ("c0x", "<f4"), ("c0y", "<f4"), ("q0c", "<f4"), ("q0s", "<f4"), ("lcx", "<f4"), ("lcy", "<f4"),
+    ("vx", "<f4"), ("vy", "<f4"), ("w", "<f4"), ("fx", "<f4"), ("fy", "<f4"), ("torque", "<f4"),
+    ("inv_mass", "<f4"), ("inv_i", "<f4"), ("min_ext", "<f4"), ("max_ext", "<f4"), ("sleep_time", "<f4"),
+    ("reserved", "<f4"),
+])
+SHAPE_DT = np.dtype([
+    ("aabb", "<f4", (4,)), ("fat", "<f4", (4,)), ("enlarged", "<i4"), ("proxy_key", "<i4"), ("body", "<i4"),
+    ("reserved", "<i4"),
+])
+POINT_DT = np.dtype([
+    ("px", "<f4"), ("py", "<f4"), ("ax", "<f4"), ("ay", "<f4"), ("bx", "<f4"), ("by", "<f4"),
+    ("sep", "<f4"), ("ni", "<f4"), ("ti", "<f4"), ("tni", "<f4"), ("nv", "<f4"), ("id", "<i4"),
+    ("persisted", "<i4"),
+])
+CONTACT_DT = np.dtype([
+    ("id", "<i4"), ("shape_a", "<i4"), ("shape_b", "<i4"), ("flags", "<i4"), ("color", "<i4"),
+    ("local_index", "<i4"), ("sim_flags", "<i4"), ("point_count", "<i4"), ("friction", "<f4"),
+    ("restitution", "<f4"), ("rolling", "<f4"), ("tangent_speed", "<f4"), ("nx", "<f4"), ("ny", "<f4"),
+    ("rolling_impulse", "<f4"), ("reserved", "<f4"), ("pts", POINT_DT, (2,)),
+])
+JOINT_DT = np.dtype([
+    ("id", "<i4"), ("color", "<i4"), ("local_index", "<i4"), ("body_a", "<i4"), ("body_b", "<i4"),
+    ("enable_limit", "<i4"), ("enable_motor", "<i4"), ("enable_spring", "<i4"),
+    ("linear_impulse_x", "<f4"), ("linear_impulse_y", "<f4"), ("spring_impulse", "<f4"),
+    ("motor_impulse", "<f4"), ("lower_impulse", "<f4"), ("upper_impulse", "<f4"), ("ref_angle", "<f4"),
+    ("reserved", "<f4"),
+])
+assert HEADER_DT.itemsize == 64 and BODY_DT.itemsize == 112 and SHAPE_DT.itemsize == 48
+assert POINT_DT.itemsize == 52 and CONTACT_DT.itemsize == 168 and JOINT_DT.itemsize == 64
+
+# b2ContactFlags / b2ContactSimFlags (S/contact.h)
+CONTACT_TOUCHING = 0x1
+SIM_TOUCHING = 0x00010000
+
+
+class Snapshot:
+    __slots__ = ("header", "bodies", "shapes", "contacts", "joints", "colors", "move", "free_ids", "nbytes")
+
+    @property
+    def step(self) -> int:
+        return int(self.header["step"])
+
+    def live_contacts(self) -> np.ndarray:
+        return self.contacts[self.contacts["id"] >= 0]
+
+    def pair_keys(self) -> np.ndarray:
+        """(shape_a, shape_b) of every live contact as int64 keys a<<32|b, sorted."""
+        c = self.live_contacts()
+        return np.sort((c["shape_a"].astype(np.int64) << 32) | c["shape_b"].astype(np.int64))
+
+    def contact_by_pair(self) -> dict:
+        c = self.live_contacts()
+        return {(int(a), int(b)): rec for a, b, rec in zip(c["shape_a"], c["shape_b"], c)}
+
+
+def parse_one(buf, off: int = 0) -> Snapshot:
+    s = Snapshot()
+    h = np.frombuffer(buf, HEADER_DT, 1, off)[0]
+    if int(h["magic"]) != SNAP_MAGIC:
+        raise ValueError("bad snapshot magic at offset %d" % off)
+    o = off + HEADER_DT.itemsize
+    s.header = h
+    s.bodies = np.frombuffer(buf, BODY_DT, int(h["body_count"]), o)
+    o += s.bodies.nbytes
+    s.shapes = np.frombuffer(buf, SHAPE_DT, int(h["shape_count"]), o)
+    o += s.shapes.nbytes
+    s.contacts = np.frombuffer(buf, CONTACT_DT, int(h["contact_cap"]), o)
+    o += s.contacts.nbytes
+    s.joints = np.frombuffer(buf, JOINT_DT, int(h["joint_count"]), o)
+    o += s.joints.nbytes
+    counts = np.frombuffer(buf, "<i4", COLOR_COUNT, o)
+    o += 4 * COLOR_COUNT
+    ids = np.frombuffer(buf, "<i4", int(counts.sum()), o)
+    o += ids.nbytes
+    s.colors = []
+    k = 0
+    for c in range(COLOR_COUNT):
+        s.colors.append(ids[k:k + int(counts[c])])
+        k += int(counts[c])
+    s.move = np.frombuffer(buf, "<i4", int(h["move_count"]), o)
+    o += s.move.nbytes
+    s.free_ids = np.frombuffer(buf, "<i4", int(h["free_id_count"]), o)
+    o += s.free_ids.nbytes
+    s.nbytes = o - off
+    return s
+
+
+def parse_all(buf) -> list[Snapshot]:
+    out = []
+    off = 0
+    while off < len(buf):
+        s = parse_one(buf, off)
+        out.append(s)
+        off += s.nbytes
+    return out
+
+
+def load(path: str) -> list[Snapshot]:
+    with open(path, "rb") as f:
+        return parse_all(f.read())
+
+
+def created_pairs(before: Snapshot, after: Snapshot) -> list[tuple[int, int]]:
+    """Contacts alive in ``after`` but not in ``before`` (matched by id and shape
+    pair), in creation order. Creation pops the LIFO contact-id pool
+    (S/id_pool.c:19-37): first the free list from its top, then fresh ids."""
+    old = {}
+    for c in before.live_contacts():
+        old[int(c["id"])] = (int(c["shape_a"]), int(c["shape_b"]))
+    new = {}
+    for c in after.live_contacts():
+        key = (int(c["shape_a"]), int(c["shape_b"]))
+        if old.get(int(c["id"])) != key:
+            new[int(c["id"])] = key
+    order = [int(i) for i in before.free_ids[::-1]]
+    nxt = len(before.contacts)
+    out = []
+    pool = list(order)
+    # ids are handed out in pop order; contacts destroyed in the same step free
+    # ids only after creation, so the pop sequence is exactly pool then fresh.
+    seq = pool + list(range(nxt, nxt + len(new) + 1))
+    for cid in seq:
+        if cid in new:
+            out.append(new[cid])
+            if len(out) == len(new):
+                break
+    return out
