@@ -1,0 +1,235 @@
+"""Benchmark: body-substeps/s of the batched MI355X world step, beside the
+one-world-at-a-time GPU path and the reference CPU step; p50/p99 step latency.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config3|config4|config1|config2]
+
+A step is one b2World_Step of every world of the workload (dt = 1/60, 4 substeps),
+launched as one kernel (one workgroup per world) on inputs already resident in
+HBM. Prints one JSON line (rank 0, N = 1).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import forge2d_amd as F  # noqa: E402
+from forge2d_amd import scenes  # noqa: E402
+
+METRIC = "body-substeps/s: batched vs per-world GPU vs host CPU; p50/p99 step latency"
+HBM_PEAK_GBS = 8000.0
+SUBSTEPS = 4
+DT = 1.0 / 60.0
+
+WORKLOADS = {
+    "config3": ("4,096 independent worlds, each a 50x1 ground box plus a pyramid of U{5..30} rows of 0.5 "
+                "half-extent boxes; FP32, dt=1/60, 4 substeps, sleep off", lambda n: scenes.config3(worlds=n or 4096)),
+    "config4": ("16,384 independent worlds, body counts log-uniform in [10,500]: circles, capsules, 3-8 vertex "
+                "polygons and 0-8 revolute chains in a segment container; FP32, dt=1/60, 4 substeps, sleep off",
+                lambda n: scenes.config4(worlds=n or 16384)),
+    "config1": ("single world: 50x1 ground box and a 40-row pyramid of 820 boxes; FP32, dt=1/60, 4 substeps, "
+                "sleep off", lambda n: scenes.config1()),
+    "config2": ("single world: 200x200 segment container with 20,000 mixed bodies; FP32, dt=1/60, 4 substeps, "
+                "sleep off", lambda n: scenes.config2()),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(st: dict) -> float:
+    """Minimal FP32 field traffic of one step when every stage is a full pass over
+    HBM: per substep 88 B/body (integrate velocities + positions), 380 + 128 p per
+    contact (warm start, solve, relax), 576 per joint; per step 136 B/body and
+    228 B/shape (collide inputs, finalize, broadphase), 424 + 162 p per contact
+    (prepare, store), 148 + 52 p per contact with restitution, 120 per joint."""
+    Nb, Ns, C, P = st["bodies"], st["shapes"], st["contacts"], st["points"]
+    Cr, Pr, Nj = st["restitution_contacts"], st["restitution_points"], st["joints"]
+    per_sub = 88 * Nb + 380 * C + 128 * P + 576 * Nj
+    per_step = 136 * Nb + 228 * Ns + 424 * C + 162 * P + 148 * Cr + 52 * Pr + 120 * Nj
+    return float(SUBSTEPS * per_sub + per_step)
+
+
+def load_pmc(workload: str):
+    path = os.path.join(ROOT, "profiles", f"pmc_{workload}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def cpu_baseline(spec: bytes, worlds: int, budget_s: float):
+    """Time the reference Box2D (built from its sources by oracle/Makefile, hook
+    flags) on this host on a bounded sample of the same worlds."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(harness):
+        return None
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "scene.bin")
+        with open(path, "wb") as f:
+            f.write(spec)
+        # calibrate on a small sample, then size the sample to the time budget
+        sample_worlds = min(worlds, threads * 2)
+        steps = 16
+        out = subprocess.run([harness, "time", path, str(steps), "0", str(threads), "0", str(sample_worlds)],
+                             capture_output=True, text=True, timeout=600, check=True)
+        r = json.loads(out.stdout.strip().splitlines()[-1])
+        per = r["wall_s"] / max(1, sample_worlds * steps)
+        target = max(1, int(budget_s / max(per, 1e-6)))
+        n = min(worlds, max(sample_worlds, target // steps))
+        if n > sample_worlds:
+            out = subprocess.run([harness, "time", path, str(steps), "0", str(threads), "0", str(n)],
+                                 capture_output=True, text=True, timeout=600, check=True)
+            r = json.loads(out.stdout.strip().splitlines()[-1])
+        return {"value": r["body_substeps_per_s"], "unit": "body-substeps/s", "cores": threads, "kind": "reference",
+                "sample": f"first {r['worlds']} worlds of the workload ({int(r['bodies'])} bodies), {steps} steps "
+                          f"from the initial state, {threads} threads stepping disjoint worlds, reference Box2D "
+                          f"v3.1.1 built with the forge2d hook flags (-std=c17 -O2, SSE2)",
+                "step_ms_p50": r["step_ms_p50"], "step_ms_p99": r["step_ms_p99"], "wall_s": r["wall_s"]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--workload", default="config3", choices=sorted(WORKLOADS))
+    ap.add_argument("--worlds", type=int, default=0, help="override the world count (testing)")
+    ap.add_argument("--per-world-sample", type=int, default=64)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: every world of a batch lives on a single device and worlds never "
+              f"exchange data; --gpus {args.gpus} is not supported (shard worlds across processes instead)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    desc, make = WORKLOADS[args.workload]
+    t0 = time.time()
+    worlds = make(args.worlds)
+    spec = scenes.to_bytes(worlds)
+    del worlds
+    log(f"[bench] {args.workload}: spec {len(spec) / 1e6:.1f} MB built in {time.time() - t0:.1f}s")
+
+    batch = F.Batch(spec)
+    W = batch.world_count
+    awake = batch.awake_bodies()
+    batch.upload()
+    log(f"[bench] uploaded {W} worlds, {awake} awake bodies, {batch.device_bytes() / 2**30:.2f} GiB device")
+
+    batch.step(DT, SUBSTEPS, steps=args.warmup)
+    batch.synchronize()
+    if batch.sync() != 0:
+        raise RuntimeError("device reported a capacity error after warmup")
+    st = batch.stats()
+    log(f"[bench] after warmup: {st}")
+
+    # timed region: K steps, synchronized on both sides
+    batch.synchronize()
+    t1 = time.perf_counter()
+    batch.step(DT, SUBSTEPS, steps=args.steps)
+    batch.synchronize()
+    wall = time.perf_counter() - t1
+    ms_per_step = 1e3 * wall / args.steps
+    value = awake * SUBSTEPS * args.steps / wall
+    log(f"[bench] batched: {ms_per_step:.3f} ms/step, {value:.4e} body-substeps/s")
+
+    # kernel duration with HIP events on the launch stream (one launch per step)
+    ev_steps = max(4, min(args.steps, 32))
+    ev_ms = batch.step_timed(DT, SUBSTEPS, steps=ev_steps)
+    launch_ms = ev_ms / ev_steps
+    bytes_step = algorithmic_bytes(st)
+    achieved = bytes_step / (launch_ms * 1e-3) / 1e9
+    pmc = load_pmc(args.workload)
+    traffic = pmc.get("hbm_bytes_per_launch") if pmc else None
+
+    # per-step latency percentiles (each step synchronized on its own)
+    lat = []
+    for _ in range(min(args.steps, 32)):
+        a = time.perf_counter()
+        batch.step(DT, SUBSTEPS, steps=1)
+        batch.synchronize()
+        lat.append(1e3 * (time.perf_counter() - a))
+    p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+    if batch.sync() != 0:
+        raise RuntimeError("device reported a capacity error")
+
+    # one-world-at-a-time GPU path on a sample of the same worlds
+    per_world = None
+    if W > 1:
+        n = min(W, args.per_world_sample)
+        sub = F.Batch(spec_subset(spec, n))
+        sub_awake = sub.awake_bodies()
+        sub.upload()
+        sub.step_each(DT, SUBSTEPS, steps=1)
+        sub.synchronize()
+        k = 4
+        a = time.perf_counter()
+        sub.step_each(DT, SUBSTEPS, steps=k)
+        sub.synchronize()
+        t = time.perf_counter() - a
+        per_world = {"value": sub_awake * SUBSTEPS * k / t, "unit": "body-substeps/s",
+                     "sample": f"first {n} worlds, {k} steps each, one launch per world per step",
+                     "ms_per_world_step": 1e3 * t / (n * k)}
+        sub.close()
+        log(f"[bench] per-world GPU path: {per_world}")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(spec, W, args.cpu_budget)
+            log(f"[bench] cpu baseline: {cpu}")
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] cpu baseline failed: {e}")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "body-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded scene generator (forge2d_amd/scenes.py), no sleep",
+        "config": {"workload": f"{args.workload}: {desc}", "worlds": W, "awake_bodies": awake,
+                   "substeps": SUBSTEPS, "dt": DT, "contacts": st["contacts"], "manifold_points": st["points"],
+                   "joints": st["joints"], "parallelism": "one workgroup per world, one launch per step"},
+        "latency_ms": {"p50": p50, "p99": p99, "samples": len(lat)},
+        "per_world_gpu": per_world,
+        "batched_vs_per_world": (value / per_world["value"]) if per_world else None,
+        "batched_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "stepWorldsKernel", "launch_ms": launch_ms,
+                     "algorithmic_bytes_per_launch": bytes_step},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def spec_subset(spec: bytes, n: int) -> bytes:
+    worlds = scenes.parse(spec)[:n]
+    parts = [scenes.MAGIC, np.array([n, 0], "<i4").tobytes()]
+    for h, b, s, j in worlds:
+        parts += [h.tobytes(), b.tobytes(), s.tobytes(), j.tobytes()]
+    return b"".join(parts)
+
+
+if __name__ == "__main__":
+    main()

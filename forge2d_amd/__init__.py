@@ -49,6 +49,7 @@ SIGNATURES = [
     ("f2dg_import_world", _I, [_VP, _I, _VP, _L]),
     ("f2dg_last_error", ctypes.c_char_p, [_VP]),
     ("f2dg_world_counters", _I, [_VP, _I, _PI]),
+    ("f2dg_batch_stats", _I, [_VP, ctypes.POINTER(ctypes.c_long)]),
     ("f2d_create_world", _U, [_F, _F, _F, _F, _F, _F, _F, _F, _I, _I]),
     ("f2d_destroy_world", None, [_U]),
     ("f2d_world_is_valid", _I, [_U]),
@@ -193,6 +194,13 @@ class Batch:
         out = (ctypes.c_int32 * 8)()
         lib().f2dg_world_counters(self.h, world, out)
         keys = ["bodies", "shapes", "contacts", "touching", "overflow", "active_colors", "step", "errors"]
+        return dict(zip(keys, list(out)))
+
+    def stats(self) -> dict:
+        out = (ctypes.c_long * 8)()
+        lib().f2dg_batch_stats(self.h, out)
+        keys = ["bodies", "shapes", "contacts", "points", "restitution_contacts", "restitution_points", "joints",
+                "overflow"]
         return dict(zip(keys, list(out)))
 
     def device_bytes(self) -> int:

@@ -1335,6 +1335,34 @@ int f2dg_world_counters(void* h, int world, int* out) {
   return 0;
 }
 
+// Aggregate counts for the algorithmic byte model (after f2dg_sync):
+// [awake bodies, shapes, touching contacts, touching manifold points,
+//  touching contacts with restitution, their points, joints, overflow contacts]
+int f2dg_batch_stats(void* h, long* out) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return -1;
+  if (download(*b) != 0) return -1;
+  long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (const HostWorld& hw : b->worlds) {
+    for (const BodySim& s : hw.sims) v[0] += s.type != kStatic ? 1 : 0;
+    v[1] += (long)hw.shapes.size();
+    for (int i = 0; i < hw.contactCount && i < (int)hw.contacts.size(); ++i) {
+      const Contact& c = hw.contacts[i];
+      if (c.shapeA < 0 || (c.flags & CF_TOUCHING) == 0) continue;
+      v[2] += 1;
+      v[3] += c.m.pointCount;
+      if (c.restitution > 0.0f) {
+        v[4] += 1;
+        v[5] += c.m.pointCount;
+      }
+    }
+    v[6] += (long)hw.joints.size();
+    v[7] += (long)hw.overflow.size();
+  }
+  for (int k = 0; k < 8; ++k) out[k] = v[k];
+  return 0;
+}
+
 // ---- flat per-world ABI (F/native/wasm/f2d_shim.c) ---------------------------
 
 // f2d_shim.c:123-139

@@ -104,31 +104,16 @@ class World:
 
     def add_body(self, type_, px, py, angle=0.0, vx=0.0, vy=0.0, w=0.0, flags=BF_ENABLE_SLEEP | BF_AWAKE,
                  lin_damp=0.0, ang_damp=0.0, gravity_scale=1.0, sleep_threshold=0.05):
-        rec = np.zeros((), BODY_DT)
-        rec["type"] = type_
-        rec["px"], rec["py"] = px, py
-        rec["qc"], rec["qs"] = rot_of(angle)
-        rec["vx"], rec["vy"], rec["w"] = vx, vy, w
-        rec["lin_damp"], rec["ang_damp"] = lin_damp, ang_damp
-        rec["gravity_scale"], rec["sleep_threshold"] = gravity_scale, sleep_threshold
-        rec["flags"] = flags
-        self.bodies.append(rec)
+        qc, qs = rot_of(angle)
+        self.bodies.append((type_, px, py, qc, qs, vx, vy, w, lin_damp, ang_damp, gravity_scale, sleep_threshold,
+                            flags))
         return len(self.bodies) - 1
 
     def add_shape(self, body, kind, data, npts=0, radius=0.0, friction=0.6, restitution=0.0, density=1.0,
                   flags=SF_INVOKE_CONTACT | SF_UPDATE_MASS, group=0, rolling=0.0, tangent_speed=0.0):
-        rec = np.zeros((), SHAPE_DT)
-        rec["body"], rec["kind"] = body, kind
-        rec["friction"], rec["restitution"], rec["density"] = friction, restitution, density
-        rec["rolling"], rec["tangent_speed"] = rolling, tangent_speed
-        rec["cat_lo"], rec["cat_hi"] = 1, 0
-        rec["mask_lo"], rec["mask_hi"] = 0xFFFFFFFF, 0xFFFFFFFF
-        rec["group"], rec["flags"] = group, flags
-        rec["npts"], rec["radius"] = npts, radius
-        d = np.zeros(16, np.float32)
-        d[: len(data)] = data
-        rec["data"] = d
-        self.shapes.append(rec)
+        d = list(data) + [0.0] * (16 - len(data))
+        self.shapes.append((body, kind, friction, restitution, rolling, tangent_speed, density, 1, 0, 0xFFFFFFFF,
+                            0xFFFFFFFF, group, flags, npts, radius, d))
         return len(self.shapes) - 1
 
     def add_box(self, body, hx, hy, cx=0.0, cy=0.0, angle=0.0, radius=0.0, **kw):
@@ -151,16 +136,9 @@ class World:
     def add_revolute(self, a, b, la, lb, ref_angle=0.0, enable_limit=False, lower=0.0, upper=0.0,
                      enable_motor=False, motor_speed=0.0, max_motor_torque=0.0, enable_spring=False,
                      hertz=0.0, damping=0.0, collide_connected=False):
-        rec = np.zeros((), JOINT_DT)
-        rec["body_a"], rec["body_b"] = a, b
-        rec["lax"], rec["lay"], rec["lbx"], rec["lby"] = la[0], la[1], lb[0], lb[1]
-        rec["ref_angle"] = ref_angle
-        rec["enable_limit"], rec["lower"], rec["upper"] = int(enable_limit), lower, upper
-        rec["enable_motor"], rec["motor_speed"], rec["max_motor_torque"] = int(enable_motor), motor_speed, max_motor_torque
-        rec["enable_spring"], rec["hertz"], rec["damping"] = int(enable_spring), hertz, damping
-        rec["collide_connected"] = int(collide_connected)
-        rec["draw_size"] = 0.25
-        self.joints.append(rec)
+        self.joints.append((a, b, la[0], la[1], lb[0], lb[1], ref_angle, int(enable_limit), lower, upper,
+                            int(enable_motor), motor_speed, max_motor_torque, int(enable_spring), hertz, damping,
+                            int(collide_connected), 0.25))
         return len(self.joints) - 1
 
     def arrays(self):

@@ -157,3 +157,90 @@ def created_pairs(before: Snapshot, after: Snapshot) -> list[tuple[int, int]]:
             if len(out) == len(new):
                 break
     return out
+
+
+def to_bytes(s: Snapshot) -> bytes:
+    """Inverse of ``parse_one``."""
+    h = np.array(s.header, HEADER_DT).copy()
+    h["body_count"] = len(s.bodies)
+    h["shape_count"] = len(s.shapes)
+    h["contact_cap"] = len(s.contacts)
+    h["joint_count"] = len(s.joints)
+    h["overflow_count"] = len(s.colors[OVERFLOW])
+    h["move_count"] = len(s.move)
+    h["free_id_count"] = len(s.free_ids)
+    h["color_total"] = int(sum(len(c) for c in s.colors))
+    parts = [h.tobytes(), np.ascontiguousarray(s.bodies).tobytes(), np.ascontiguousarray(s.shapes).tobytes(),
+             np.ascontiguousarray(s.contacts).tobytes(), np.ascontiguousarray(s.joints).tobytes(),
+             np.array([len(c) for c in s.colors], "<i4").tobytes()]
+    parts += [np.asarray(c, "<i4").tobytes() for c in s.colors]
+    parts += [np.asarray(s.move, "<i4").tobytes(), np.asarray(s.free_ids, "<i4").tobytes()]
+    return b"".join(parts)
+
+
+def copy(s: Snapshot) -> Snapshot:
+    o = Snapshot()
+    o.header = np.array(s.header, HEADER_DT).copy()
+    o.bodies = s.bodies.copy()
+    o.shapes = s.shapes.copy()
+    o.contacts = s.contacts.copy()
+    o.joints = s.joints.copy()
+    o.colors = [np.array(c, "<i4") for c in s.colors]
+    o.move = np.array(s.move, "<i4")
+    o.free_ids = np.array(s.free_ids, "<i4")
+    o.nbytes = s.nbytes
+    return o
+
+
+def warm_start_matched(prev: Snapshot, cur: Snapshot) -> np.ndarray:
+    """Contacts of ``cur`` with the impulses they carried into the solver of the
+    step that produced ``cur``: manifold geometry of ``cur``, warm-start impulses
+    matched by feature id from the manifold of ``prev`` (b2UpdateContact,
+    S/contact.c:555-600)."""
+    out = cur.contacts.copy()
+    old = prev.contact_by_pair()
+    for i in range(len(out)):
+        rec = out[i]
+        if rec["id"] < 0:
+            continue
+        key = (int(rec["shape_a"]), int(rec["shape_b"]))
+        n = int(rec["point_count"])
+        o = old.get(key)
+        if o is not None:
+            o = o.copy()
+        if n > 0:
+            rec["rolling_impulse"] = o["rolling_impulse"] if o is not None else 0.0
+        for p in range(n):
+            pt = rec["pts"][p]
+            pt["ni"] = 0.0
+            pt["ti"] = 0.0
+            pt["tni"] = 0.0
+            pt["nv"] = 0.0
+            pt["persisted"] = 0
+            if o is None:
+                continue
+            for q in range(int(o["point_count"])):
+                if int(o["pts"][q]["id"]) == int(pt["id"]):
+                    pt["ni"] = o["pts"][q]["ni"]
+                    pt["ti"] = o["pts"][q]["ti"]
+                    pt["persisted"] = 1
+                    o["pts"][q]["ni"] = 0.0
+                    o["pts"][q]["ti"] = 0.0
+                    break
+        out[i] = rec
+    return out
+
+
+def post_collide(prev: Snapshot, cur: Snapshot) -> Snapshot:
+    """The world as the solver of step ``cur.step`` saw it: bodies and shapes from
+    ``prev`` (finalize of the previous step), contacts, colours and the overflow
+    order from ``cur`` with warm-start impulses matched from ``prev``."""
+    s = copy(cur)
+    s.bodies = prev.bodies.copy()
+    s.shapes = prev.shapes.copy()
+    s.contacts = warm_start_matched(prev, cur)
+    s.joints = prev.joints.copy()
+    s.joints["color"] = cur.joints["color"]
+    s.move = np.zeros(0, "<i4")
+    s.header["step"] = prev.header["step"]
+    return s

@@ -1,0 +1,180 @@
+/*
+ * forge2d_amd — MI355X-native Box2D v3.1 world step behind a plain C-ABI.
+ *
+ * Library: forge2d_amd/libforge2d_amd.so (HIP for gfx950).
+ *
+ * Two families of entry points, both scalars/pointers/sizes only:
+ *
+ *  f2d_*   The flat per-world ABI of the reference's own C shim
+ *          (packages/forge2d/native/wasm/f2d_shim.c, "F/native/wasm/f2d_shim.c"
+ *          below): same names, argument order and meaning, same id packing
+ *          (world ids = index1 | generation << 16; body/shape/joint ids =
+ *          (int32 index1, uint32 world0 | generation << 16)). The step-path
+ *          subset is implemented; World.step maps to f2d_world_step, which
+ *          replaces b2World_Step (packages/forge2d/third_party/box2d/src/world.c:695-812)
+ *          as bound by RawBox2DFfi.worldStep (lib/src/backend/raw_box2d_ffi.dart:103-104).
+ *
+ *  f2dg_*  The batch ABI: many independent worlds packed in one device batch,
+ *          stepped by one kernel launch per step (one workgroup per world).
+ *          It has no counterpart in the reference, which steps one world per
+ *          call.
+ *
+ * Errors: functions returning int return 0 on success and a negative value on
+ * failure (f2dg_sync returns the OR of per-world error bits: 1 contact capacity,
+ * 2 pair capacity, 4 overflow capacity). f2d_world_step aborts on a device error,
+ * as the reference asserts (include/box2d/base.h B2_ASSERT).
+ */
+#ifndef FORGE2D_AMD_H
+#define FORGE2D_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- flat per-world ABI (F/native/wasm/f2d_shim.c) ------------------------ */
+
+/* Shape definition record, F/native/wasm/f2d_shim.c:412-431 */
+typedef struct f2dShapeDef {
+  float friction;
+  float restitution;
+  float rolling_resistance;
+  float tangent_speed;
+  int32_t user_material_id;
+  uint32_t custom_color;
+  float density;
+  uint32_t category_lo;
+  uint32_t category_hi;
+  uint32_t mask_lo;
+  uint32_t mask_hi;
+  int32_t group_index;
+  int32_t is_sensor;
+  int32_t enable_sensor_events;
+  int32_t enable_contact_events;
+  int32_t enable_hit_events;
+  int32_t enable_pre_solve_events;
+  int32_t invoke_contact_creation;
+  int32_t update_body_mass;
+} f2dShapeDef;
+
+/* F/native/wasm/f2d_shim.c:123-139 (b2CreateWorld) */
+uint32_t f2d_create_world(float gravity_x, float gravity_y, float restitution_threshold, float hit_event_threshold,
+                          float contact_hertz, float contact_damping_ratio, float max_contact_push_speed,
+                          float maximum_linear_speed, int enable_sleep, int enable_continuous);
+/* F/native/wasm/f2d_shim.c:141 (b2DestroyWorld) */
+void f2d_destroy_world(uint32_t w);
+/* F/native/wasm/f2d_shim.c:143-145 (b2World_IsValid) */
+int f2d_world_is_valid(uint32_t w);
+/* F/native/wasm/f2d_shim.c:147-149 (b2World_Step) — the hot path */
+void f2d_world_step(uint32_t w, float time_step, int sub_steps);
+/* F/native/wasm/f2d_shim.c:151-159 (b2World_SetGravity / GetGravity) */
+void f2d_world_set_gravity(uint32_t w, float x, float y);
+void f2d_world_get_gravity(uint32_t w, float* out);
+/* F/native/wasm/f2d_shim.c:161-175 (b2World_EnableSleeping / EnableContinuous) */
+void f2d_world_enable_sleeping(uint32_t w, int enabled);
+int f2d_world_is_sleeping_enabled(uint32_t w);
+void f2d_world_enable_continuous(uint32_t w, int enabled);
+int f2d_world_is_continuous_enabled(uint32_t w);
+
+/* F/native/wasm/f2d_shim.c:192-218 (b2CreateBody); out = [index1, world_and_generation] */
+void f2d_create_body(uint32_t w, int type, float px, float py, float qc, float qs, float lvx, float lvy,
+                     float angular_velocity, float linear_damping, float angular_damping, float gravity_scale,
+                     float sleep_threshold, const char* name, int enable_sleep, int is_awake, int fixed_rotation,
+                     int is_bullet, int is_enabled, int allow_fast_rotation, int32_t* out);
+/* F/native/wasm/f2d_shim.c:224-226 (b2Body_IsValid) */
+int f2d_body_is_valid(int32_t index1, uint32_t world_and_generation);
+/* F/native/wasm/f2d_shim.c:228-238 (b2Body_GetPosition / GetRotation) */
+void f2d_body_get_position(int32_t index1, uint32_t world_and_generation, float* out);
+void f2d_body_get_rotation(int32_t index1, uint32_t world_and_generation, float* out);
+/* F/native/wasm/f2d_shim.c:245-255 (b2Body_GetLinearVelocity / SetLinearVelocity) */
+void f2d_body_get_linear_velocity(int32_t index1, uint32_t world_and_generation, float* out);
+void f2d_body_set_linear_velocity(int32_t index1, uint32_t world_and_generation, float x, float y);
+/* F/native/wasm/f2d_shim.c (F2D_BODY_GET_FLOAT b2Body_GetAngularVelocity) */
+float f2d_body_get_angular_velocity(int32_t index1, uint32_t world_and_generation);
+/* F/native/wasm/f2d_shim.c:263-266 (b2Body_ApplyForceToCenter) */
+void f2d_body_apply_force_to_center(int32_t index1, uint32_t world_and_generation, float fx, float fy, int wake);
+/* F/native/wasm/f2d_shim.c:280-284 (b2Body_ApplyLinearImpulseToCenter) */
+void f2d_body_apply_linear_impulse_to_center(int32_t index1, uint32_t world_and_generation, float ix, float iy,
+                                             int wake);
+/* F/native/wasm/f2d_shim.c:291-303 (b2Body_GetLocalCenterOfMass / GetWorldCenterOfMass) */
+void f2d_body_get_world_center(int32_t index1, uint32_t world_and_generation, float* out);
+void f2d_body_get_local_center(int32_t index1, uint32_t world_and_generation, float* out);
+/* F/native/wasm/f2d_shim.c (F2D_BODY_GET_FLOAT b2Body_GetMass / GetRotationalInertia / GetType) */
+float f2d_body_get_mass(int32_t index1, uint32_t world_and_generation);
+float f2d_body_get_rotational_inertia(int32_t index1, uint32_t world_and_generation);
+int f2d_body_get_type(int32_t index1, uint32_t world_and_generation);
+
+/* F/native/wasm/f2d_shim.c:459-466 (b2CreateCircleShape) */
+void f2d_create_circle_shape(int32_t index1, uint32_t world_and_generation, const f2dShapeDef* def, float cx,
+                             float cy, float radius, int32_t* out);
+/* F/native/wasm/f2d_shim.c:468-476 (b2CreateCapsuleShape) */
+void f2d_create_capsule_shape(int32_t index1, uint32_t world_and_generation, const f2dShapeDef* def, float c1x,
+                              float c1y, float c2x, float c2y, float radius, int32_t* out);
+/* F/native/wasm/f2d_shim.c:478-486 (b2CreateSegmentShape) */
+void f2d_create_segment_shape(int32_t index1, uint32_t world_and_generation, const f2dShapeDef* def, float p1x,
+                              float p1y, float p2x, float p2y, int32_t* out);
+/* F/native/wasm/f2d_shim.c:488-500 (b2MakeOffsetRoundedBox + b2CreatePolygonShape) */
+void f2d_create_box_shape(int32_t index1, uint32_t world_and_generation, const f2dShapeDef* def, float half_width,
+                          float half_height, float cx, float cy, float qc, float qs, float radius, int32_t* out);
+/* F/native/wasm/f2d_shim.c:502-515 (b2ComputeHull + b2MakePolygon + b2CreatePolygonShape); 0 = degenerate */
+int f2d_create_polygon_shape(int32_t index1, uint32_t world_and_generation, const f2dShapeDef* def,
+                             const float* points, int count, float radius, int32_t* out);
+/* F/native/wasm/f2d_shim.c:816-843 (b2CreateRevoluteJoint) */
+void f2d_create_revolute_joint(uint32_t w, int32_t a_index1, uint32_t a_world_and_generation, int32_t b_index1,
+                               uint32_t b_world_and_generation, float anchor_ax, float anchor_ay, float anchor_bx,
+                               float anchor_by, float reference_angle, float target_angle, int enable_spring,
+                               float hertz, float damping_ratio, int enable_limit, float lower_angle,
+                               float upper_angle, int enable_motor, float max_motor_torque, float motor_speed,
+                               float draw_size, int collide_connected, int32_t* out);
+
+/* The batch handle behind a single f2d world (no counterpart in the reference):
+ * lets the f2dg_* export/counter calls read a world created through f2d_*. */
+void* f2d_world_batch(uint32_t w);
+
+/* ---- batch ABI (no counterpart in the reference) --------------------------- */
+
+/* Build worlds from a scene spec buffer (forge2d_amd/scenes.py layout). NULL on error. */
+void* f2dg_create_from_spec(const void* spec, long bytes);
+void f2dg_destroy(void* batch);
+int f2dg_world_count(void* batch);
+int f2dg_body_count(void* batch, int world);
+int f2dg_shape_count(void* batch, int world);
+long f2dg_awake_body_count(void* batch);
+/* contact slots per shape reserved on the device (default 6, before upload) */
+int f2dg_set_contact_capacity_factor(void* batch, float factor);
+/* hipStream_t to launch on (NULL = default stream) */
+int f2dg_set_stream(void* batch, void* stream);
+/* lay out and copy the batch to the device (done implicitly by the first step) */
+int f2dg_upload(void* batch);
+long f2dg_device_bytes(void* batch);
+/* step every world `steps` times; phases bitmask: 1 pairs, 2 collide, 4 contact state,
+ * 8 solve, 16 finalize (0 = all) */
+int f2dg_step(void* batch, float dt, int sub_steps, int steps, int phases);
+/* step only the listed worlds (continuous-batching serving) */
+int f2dg_step_worlds(void* batch, const int32_t* world_ids, int n, float dt, int sub_steps, int steps);
+/* the one-world-at-a-time GPU path: a launch per world per step */
+int f2dg_step_each(void* batch, float dt, int sub_steps, int steps);
+/* step and time with HIP events on the batch stream; *ms_out = total milliseconds */
+int f2dg_step_timed(void* batch, float dt, int sub_steps, int steps, float* ms_out);
+int f2dg_synchronize(void* batch);
+/* copy device state to the host mirror; returns OR of world error bits, <0 on failure */
+int f2dg_sync(void* batch);
+/* per body: px, py, qc, qs, vx, vy, w */
+int f2dg_get_bodies(void* batch, int world, float* out);
+/* world snapshot (forge2d_amd/snapshot.py layout); returns the size, writes when cap suffices */
+long f2dg_export_world(void* batch, int world, void* out, long cap);
+/* overlay a snapshot onto a world built from the same scene spec */
+int f2dg_import_world(void* batch, int world, const void* snapshot, long bytes);
+const char* f2dg_last_error(void* batch);
+/* aggregate counts after f2dg_sync: [awake bodies, shapes, touching contacts, touching points,
+ * restitution contacts, restitution points, joints, overflow contacts] */
+int f2dg_batch_stats(void* batch, long* out);
+/* [bodies, shapes, live contacts, touching, overflow, active colours, step, error bits] */
+int f2dg_world_counters(void* batch, int world, int32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FORGE2D_AMD_H */

@@ -1,0 +1,102 @@
+"""Regenerate the golden fixtures from the reference Box2D (test infrastructure).
+
+Runs only in the build container, where /root/reference exists: it needs the
+reference build of ``oracle/Makefile`` (``make -C oracle ref``). For each
+fixture scene it writes the scene spec and the reference's own world snapshots
+after the listed steps, compressed, to ``tests/golden/<name>.npz``:
+
+    spec   uint8   the scene spec (forge2d_amd/scenes.py layout)
+    snaps  uint8   concatenated snapshots (forge2d_amd/snapshot.py layout)
+    steps  int32   the step index of each snapshot
+
+Usage: python tests/golden/make_golden.py [name ...]
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from forge2d_amd import scenes  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def overflow_world():
+    """A wide dynamic plank under many small circles: every circle contact shares
+    the plank's body, so after ten colours the rest go to the overflow colour
+    (S/constraint_graph.c:68-135)."""
+    w = scenes.World(scenes.world_header(sleep=False, continuous=False))
+    g = w.add_body(scenes.STATIC, 0.0, -1.0)
+    w.add_box(g, 30.0, 1.0)
+    plank = w.add_body(scenes.DYNAMIC, 0.0, 0.5)
+    w.add_box(plank, 12.0, 0.5, density=2.0)
+    for i in range(36):
+        b = w.add_body(scenes.DYNAMIC, -11.0 + 0.62 * i, 1.31 + 0.02 * (i % 3))
+        w.add_circle(b, 0.3, friction=0.4, restitution=0.0, rolling=0.05 if i % 4 == 0 else 0.0)
+    return w
+
+
+def bounce_world():
+    """Circles, capsules and boxes with restitution dropped on a ground box, so the
+    restitution stage (S/contact_solver.c:1969-2076) applies impulses."""
+    rng = scenes.SplitMix64(77)
+    w = scenes.World(scenes.world_header(sleep=False, continuous=False))
+    g = w.add_body(scenes.STATIC, 0.0, -1.0)
+    w.add_box(g, 20.0, 1.0, restitution=0.2)
+    for i in range(24):
+        x = -9.0 + 0.8 * i
+        b = w.add_body(scenes.DYNAMIC, x, 2.0 + rng.uniform(0.0, 2.0), angle=rng.uniform(-1.0, 1.0))
+        k = i % 3
+        e = rng.uniform(0.3, 0.8)
+        if k == 0:
+            w.add_circle(b, 0.3, restitution=e)
+        elif k == 1:
+            w.add_capsule(b, (-0.2, 0.0), (0.2, 0.0), 0.15, restitution=e)
+        else:
+            w.add_box(b, 0.25, 0.25, restitution=e)
+    return w
+
+
+FIXTURES = {
+    # name: (world factory, dumped steps)
+    "pyramid12": (lambda: scenes.pyramid_world(12), [0, 1, 2, 3, 4, 5, 60, 61, 62]),
+    "pyramid40": (lambda: scenes.pyramid_world(40), [0, 1, 2, 3]),
+    "mixed": (lambda: scenes.container_world(150, 12.0, 24.0, 20251202, continuous=False, chains=2,
+                                             drop_band=(0.1, 0.6)),
+              [0, 1, 2, 3, 4, 5, 6, 30, 31, 32, 60, 61, 62, 120, 121, 122]),
+    "overflow": (overflow_world, [0, 1, 2, 3, 4, 20, 21, 22]),
+    "bounce": (bounce_world, [0, 40, 41, 42, 45, 46, 47, 50, 51, 52, 60, 61, 62, 80, 81, 82]),
+}
+
+
+def make(name: str):
+    factory, steps = FIXTURES[name]
+    world = factory()
+    spec = scenes.to_bytes([world])
+    with tempfile.TemporaryDirectory() as td:
+        sp = os.path.join(td, "scene.bin")
+        out = os.path.join(td, "snaps.bin")
+        with open(sp, "wb") as f:
+            f.write(spec)
+        subprocess.run([HARNESS, "dump", sp, out, "0", str(max(steps)), ",".join(map(str, steps))], check=True)
+        with open(out, "rb") as f:
+            snaps = f.read()
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, spec=np.frombuffer(spec, np.uint8), snaps=np.frombuffer(snaps, np.uint8),
+                        steps=np.array(steps, np.int32))
+    print(f"{name}: {len(spec)} B spec, {len(snaps)} B snapshots -> {os.path.getsize(path)} B")
+
+
+if __name__ == "__main__":
+    if not os.path.exists(HARNESS):
+        sys.exit("build the reference harness first: make -C oracle ref")
+    for n in sys.argv[1:] or FIXTURES:
+        make(n)

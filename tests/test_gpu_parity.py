@@ -1,0 +1,104 @@
+"""GPU parity against the reference's own step (golden fixtures written by the
+reference Box2D through oracle/ref_harness.c).
+
+For every pair of consecutive dumped steps (k-1, k) of every fixture:
+  * full step: import the reference state after step k-1, run one device step and
+    compare with the reference after step k: the contact pair set and every
+    manifold (point count, feature ids, anchors, separation, normal) bit-exact;
+    colouring valid; persisting contacts keep their colour; when the colouring
+    equals the reference's, body position, rotation and velocity within 1e-5
+    relative (floor 1: absolute below unit magnitude) — in practice bit-exact.
+  * solver + finalize: the reference's post-collide state of step k (colours,
+    manifolds, impulses matched from k-1, bodies from k-1) stepped through the
+    solver stages and finalize reproduces bodies, stored impulses, fat AABBs and
+    the next move buffer of step k, within 1e-5 (in practice bit-exact).
+"""
+import numpy as np
+import pytest
+
+import forge2d_amd as F
+from forge2d_amd import snapshot as S
+from tests import parity_util as P
+from tests.conftest import FIXTURE_NAMES, load_fixture
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def consecutive_pairs(name):
+    spec, snaps, raws = load_fixture(name)
+    out = []
+    for i in range(1, len(snaps)):
+        if snaps[i].step == snaps[i - 1].step + 1:
+            out.append((name, i))
+    return out
+
+
+CASES = [c for n in FIXTURE_NAMES for c in consecutive_pairs(n)]
+
+
+@pytest.mark.parametrize("name,i", CASES, ids=[f"{n}-{load_fixture(n)[1][i].step}" for n, i in CASES])
+def test_full_step_from_reference_state(gpu, name, i):
+    spec, snaps, raws = load_fixture(name)
+    prev, ref = snaps[i - 1], snaps[i]
+    print(f"[parity] full step {name} {prev.step}->{ref.step}", flush=True)
+    b = F.Batch(spec)
+    b.import_state(0, raws[i - 1])
+    b.step(steps=1)
+    assert b.sync() == 0
+    mine = b.snapshot(0)
+    assert np.array_equal(np.sort(mine.pair_keys()), np.sort(ref.pair_keys())), "contact pair set differs"
+    rep = P.manifold_report(mine, ref)
+    assert rep["point_count"] == 0 and rep["ids"] == 0, rep
+    assert rep["values"] == 0 and rep["normals"] == 0, rep
+    assert P.coloring_violations(mine) == 0
+    cm, cr = P.color_map(mine), P.color_map(ref)
+    persisting = [k for k in P.color_map(prev) if k in cr]
+    assert all(cm[k] == cr[k] for k in persisting)
+    if cm == cr:
+        errs = P.body_errors(mine, ref)
+        assert max(errs.values()) <= TOL, errs
+        rep = P.manifold_report(mine, ref, with_impulses=True)
+        assert rep["impulses"] == 0, rep
+
+
+SOLVE_CASES = [c for c in CASES if load_fixture(c[0])[1][c[1]].live_contacts().size > 0]
+
+
+@pytest.mark.parametrize("name,i", SOLVE_CASES, ids=[f"{n}-{load_fixture(n)[1][i].step}" for n, i in SOLVE_CASES])
+def test_solver_and_finalize_from_post_collide_state(gpu, name, i):
+    spec, snaps, raws = load_fixture(name)
+    prev, ref = snaps[i - 1], snaps[i]
+    print(f"[parity] solve {name} {prev.step}->{ref.step}", flush=True)
+    composed = S.post_collide(prev, ref)
+    b = F.Batch(spec)
+    b.import_state(0, S.to_bytes(composed))
+    b.step(steps=1, phases=F.PH_SOLVE | F.PH_FINALIZE)
+    assert b.sync() == 0
+    mine = b.snapshot(0)
+    errs = P.body_errors(mine, ref, P.BODY_FIELDS + ["c0x", "c0y", "q0c", "q0s"])
+    assert max(errs.values()) <= TOL, errs
+    rep = P.manifold_report(mine, ref, value_fields=(), with_impulses=True)
+    assert rep["impulses"] == 0, rep
+    assert np.array_equal(mine.shapes["fat"], ref.shapes["fat"])
+    assert np.array_equal(mine.move, ref.move)
+    for f in ("linear_impulse_x", "linear_impulse_y", "lower_impulse", "upper_impulse"):
+        assert np.array_equal(mine.joints[f], ref.joints[f]), f
+
+
+@pytest.mark.parametrize("name", FIXTURE_NAMES)
+def test_first_step_pairs_and_manifolds(gpu, name):
+    """Stepping the freshly built scene: the created pair set and every manifold
+    equal the reference's step-1 result bit for bit."""
+    spec, snaps, _ = load_fixture(name)
+    if snaps[1].step != 1:
+        pytest.skip("fixture has no step 1")
+    b = F.Batch(spec)
+    b.step(steps=1)
+    assert b.sync() == 0
+    mine = b.snapshot(0)
+    ref = snaps[1]
+    assert np.array_equal(np.sort(mine.pair_keys()), np.sort(ref.pair_keys()))
+    rep = P.manifold_report(mine, ref)
+    assert rep["point_count"] == rep["ids"] == rep["values"] == rep["normals"] == 0, rep
+    assert P.coloring_violations(mine) == 0
