@@ -135,6 +135,9 @@ def main():
         raise RuntimeError("device reported a capacity error after warmup")
     st = batch.stats()
     log(f"[bench] after warmup: {st}")
+    block, tune_ms = batch.autotune(DT, SUBSTEPS, steps=4)
+    batch.synchronize()
+    log(f"[bench] autotuned workgroup size: {block} lanes per world (ms per 4 steps: {tune_ms})")
 
     # timed region: K steps, synchronized on both sides
     batch.synchronize()
@@ -209,14 +212,15 @@ def main():
         "data": "synthetic: seeded scene generator (forge2d_amd/scenes.py), no sleep",
         "config": {"workload": f"{args.workload}: {desc}", "worlds": W, "awake_bodies": awake,
                    "substeps": SUBSTEPS, "dt": DT, "contacts": st["contacts"], "manifold_points": st["points"],
-                   "joints": st["joints"], "parallelism": "one workgroup per world, one launch per step"},
+                   "joints": st["joints"], "workgroup_lanes": block, "autotune_ms_per_4_steps": tune_ms,
+                   "parallelism": "one workgroup per world; a collide launch and a solve launch per step"},
         "latency_ms": {"p50": p50, "p99": p99, "samples": len(lat)},
         "per_world_gpu": per_world,
         "batched_vs_per_world": (value / per_world["value"]) if per_world else None,
         "batched_vs_cpu": (value / cpu["value"]) if cpu else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "stepWorldsKernel", "launch_ms": launch_ms,
+                     "kernel": "collideKernel + solveKernel (one step)", "launch_ms": launch_ms,
                      "algorithmic_bytes_per_launch": bytes_step},
         "cpu_baseline": cpu,
     }

@@ -50,6 +50,9 @@ SIGNATURES = [
     ("f2dg_last_error", ctypes.c_char_p, [_VP]),
     ("f2dg_world_counters", _I, [_VP, _I, _PI]),
     ("f2dg_batch_stats", _I, [_VP, ctypes.POINTER(ctypes.c_long)]),
+    ("f2dg_autotune", _I, [_VP, _F, _I, _I, _PF]),
+    ("f2dg_set_block", _I, [_VP, _I]),
+    ("f2dg_get_block", _I, [_VP]),
     ("f2d_create_world", _U, [_F, _F, _F, _F, _F, _F, _F, _F, _I, _I]),
     ("f2d_destroy_world", None, [_U]),
     ("f2d_world_is_valid", _I, [_U]),
@@ -195,6 +198,20 @@ class Batch:
         lib().f2dg_world_counters(self.h, world, out)
         keys = ["bodies", "shapes", "contacts", "touching", "overflow", "active_colors", "step", "errors"]
         return dict(zip(keys, list(out)))
+
+    def autotune(self, dt=1.0 / 60.0, substeps=4, steps=4):
+        ms = (ctypes.c_float * 3)()
+        blk = lib().f2dg_autotune(self.h, dt, substeps, steps, ms)
+        if blk < 0:
+            raise RuntimeError("autotune failed")
+        return blk, {64: ms[0], 128: ms[1], 256: ms[2]}
+
+    def set_block(self, block: int):
+        self._check(lib().f2dg_set_block(self.h, block), "set_block")
+
+    @property
+    def block(self) -> int:
+        return lib().f2dg_get_block(self.h)
 
     def stats(self) -> dict:
         out = (ctypes.c_long * 8)()

@@ -23,7 +23,7 @@
 #include "world_store.hpp"
 
 extern "C" int f2d_launch_step(const f2d::BatchView* b, const f2d::StepParams* sp, const int* worldList, int nWorlds,
-                               hipStream_t stream);
+                               hipStream_t stream, const f2d::LaunchConfig* cfg);
 
 namespace f2d {
 
@@ -545,6 +545,22 @@ static int commit(Batch& b) {
   }
   v.consStride = nc;
   v.worldCount = W;
+  // dynamic LDS per kernel: sort-and-sweep (28 B per shape, power-of-two padded)
+  // and new-pair sort (12 B per pair), body states (32 B per body); worlds that do
+  // not fit fall back to their global scratch inside the kernel
+  int maxShapes = 1, maxBodies = 1;
+  for (HostWorld& hw : b.worlds) {
+    maxShapes = std::max(maxShapes, (int)hw.shapes.size());
+    maxBodies = std::max(maxBodies, (int)hw.sims.size());
+  }
+  const int kLdsLimit = 65536;
+  int np = nextPow2(maxShapes);
+  int collide = 28 * np <= kLdsLimit ? 28 * np : 0;
+  collide = std::max(collide, 12 * 256);
+  b.launch.collideLds = collide;
+  int solve = maxBodies * (int)sizeof(BodyState);
+  b.launch.solveLds = solve <= kLdsLimit ? ((solve + 255) & ~255) : 0;
+  if (b.launch.block == 0) b.launch.block = maxShapes <= 640 ? 64 : 256;
   b.committed = true;
   return 0;
 }
@@ -1166,7 +1182,7 @@ int f2dg_step(void* h, float dt, int subSteps, int steps, int phases) {
   StepParams sp = makeParams(dt, subSteps, phases == 0 ? PH_ALL : phases);
   int W = (int)b->worlds.size();
   for (int s = 0; s < steps; ++s) {
-    int rc = f2d_launch_step(&b->dev.view, &sp, nullptr, W, b->stream);
+    int rc = f2d_launch_step(&b->dev.view, &sp, nullptr, W, b->stream, &b->launch);
     if (rc != 0) {
       std::fprintf(stderr, "forge2d_amd: step launch failed (%d)\n", rc);
       return -1;
@@ -1189,7 +1205,7 @@ int f2dg_step_worlds(void* h, const int* worldIds, int n, float dt, int subSteps
   F2D_HIP(hipMemcpyAsync(b->dWorldList, worldIds, sizeof(int) * n, hipMemcpyHostToDevice, b->stream));
   StepParams sp = makeParams(dt, subSteps, PH_ALL);
   for (int s = 0; s < steps; ++s) {
-    if (f2d_launch_step(&b->dev.view, &sp, b->dWorldList, n, b->stream) != 0) return -1;
+    if (f2d_launch_step(&b->dev.view, &sp, b->dWorldList, n, b->stream, &b->launch) != 0) return -1;
   }
   b->deviceNewer = true;
   return 0;
@@ -1208,7 +1224,7 @@ int f2dg_step_each(void* h, float dt, int subSteps, int steps) {
   StepParams sp = makeParams(dt, subSteps, PH_ALL);
   for (int i = 0; i < W; ++i) {
     for (int s = 0; s < steps; ++s) {
-      if (f2d_launch_step(&b->dev.view, &sp, b->dWorldList + i, 1, b->stream) != 0) return -1;
+      if (f2d_launch_step(&b->dev.view, &sp, b->dWorldList + i, 1, b->stream, &b->launch) != 0) return -1;
     }
   }
   b->deviceNewer = true;
@@ -1227,7 +1243,7 @@ int f2dg_step_timed(void* h, float dt, int subSteps, int steps, float* msOut) {
   int W = (int)b->worlds.size();
   F2D_HIP(hipEventRecord(e0, b->stream));
   for (int s = 0; s < steps; ++s) {
-    if (f2d_launch_step(&b->dev.view, &sp, nullptr, W, b->stream) != 0) return -1;
+    if (f2d_launch_step(&b->dev.view, &sp, nullptr, W, b->stream, &b->launch) != 0) return -1;
   }
   F2D_HIP(hipEventRecord(e1, b->stream));
   F2D_HIP(hipEventSynchronize(e1));
@@ -1333,6 +1349,42 @@ int f2dg_world_counters(void* h, int world, int* out) {
   out[6] = hw.stepIndex;
   out[7] = hw.hdr.error;
   return 0;
+}
+
+// Workgroup-size autotuner: time `steps` steps of the live batch with each
+// candidate size (64, 128, 256 lanes per world) and keep the fastest. The trial
+// steps advance the batch; results do not depend on the workgroup size.
+int f2dg_autotune(void* h, float dt, int subSteps, int steps, float* msOut) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return -1;
+  if (prepareForStep(*b) != 0) return -1;
+  const int cands[3] = {64, 128, 256};
+  float best = 1e30f;
+  int bestBlock = b->launch.block;
+  for (int k = 0; k < 3; ++k) {
+    b->launch.block = cands[k];
+    float ms = 0.0f;
+    if (f2dg_step_timed(h, dt, subSteps, steps, &ms) != 0) return -1;
+    if (msOut) msOut[k] = ms;
+    if (ms < best) {
+      best = ms;
+      bestBlock = cands[k];
+    }
+  }
+  b->launch.block = bestBlock;
+  return bestBlock;
+}
+
+int f2dg_set_block(void* h, int block) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || (block != 64 && block != 128 && block != 256)) return -1;
+  b->launch.block = block;
+  return 0;
+}
+
+int f2dg_get_block(void* h) {
+  Batch* b = static_cast<Batch*>(h);
+  return b ? b->launch.block : -1;
 }
 
 // Aggregate counts for the algorithmic byte model (after f2dg_sync):

@@ -76,34 +76,34 @@ F2D_HD int jointIndex(const BodySim* sims, int body) { return sims[body].type ==
 
 F2D_HD void warmStartRevolute(Joint& j, BodyState* states, const BodySim* sims) {
   float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
-  BodyState dummyA = identityState(), dummyB = identityState();
   int ia = jointIndex(sims, j.bodyA), ib = jointIndex(sims, j.bodyB);
-  BodyState* sA = ia < 0 ? &dummyA : states + ia;
-  BodyState* sB = ib < 0 ? &dummyB : states + ib;
-  Vec2 rA = rotate(sA->dq, j.anchorA);
-  Vec2 rB = rotate(sB->dq, j.anchorB);
+  BodyState sA = loadState(states, ia);
+  BodyState sB = loadState(states, ib);
+  Vec2 rA = rotate(sA.dq, j.anchorA);
+  Vec2 rB = rotate(sB.dq, j.anchorB);
   float axialImpulse = j.springImpulse + j.motorImpulse + j.lowerImpulse - j.upperImpulse;
-  sA->v = mulSub(sA->v, mA, j.linearImpulse);
-  sA->w -= iA * (cross(rA, j.linearImpulse) + axialImpulse);
-  sB->v = mulAdd(sB->v, mB, j.linearImpulse);
-  sB->w += iB * (cross(rB, j.linearImpulse) + axialImpulse);
+  sA.v = mulSub(sA.v, mA, j.linearImpulse);
+  sA.w -= iA * (cross(rA, j.linearImpulse) + axialImpulse);
+  sB.v = mulAdd(sB.v, mB, j.linearImpulse);
+  sB.w += iB * (cross(rB, j.linearImpulse) + axialImpulse);
+  storeVel(states, ia, sA.v, sA.w);
+  storeVel(states, ib, sB.v, sB.w);
 }
 
 F2D_HD void solveRevolute(Joint& j, BodyState* states, const BodySim* sims, float h, float inv_h, bool useBias) {
   float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
-  BodyState dummyA = identityState(), dummyB = identityState();
   int ia = jointIndex(sims, j.bodyA), ib = jointIndex(sims, j.bodyB);
-  BodyState* sA = ia < 0 ? &dummyA : states + ia;
-  BodyState* sB = ib < 0 ? &dummyB : states + ib;
-  Vec2 vA = sA->v;
-  float wA = sA->w;
-  Vec2 vB = sB->v;
-  float wB = sB->w;
-  const Rot dqA = sA->dq;
-  const Rot dqB = sB->dq;
+  const BodyState sA = loadState(states, ia);
+  const BodyState sB = loadState(states, ib);
+  Vec2 vA = sA.v;
+  float wA = sA.w;
+  Vec2 vB = sB.v;
+  float wB = sB.w;
+  const Rot dqA = sA.dq;
+  const Rot dqB = sB.dq;
   bool fixedRotation = (iA + iB == 0.0f);
   if (j.enableSpring && fixedRotation == false) {
-    float jointAngle = relativeAngle(sB->dq, sA->dq) + j.deltaAngle;
+    float jointAngle = relativeAngle(sB.dq, sA.dq) + j.deltaAngle;
     float jointAngleDelta = unwindAngle(jointAngle - j.targetAngle);
     float C = jointAngleDelta;
     float bias = j.springSoftness.biasRate * C;
@@ -166,15 +166,15 @@ F2D_HD void solveRevolute(Joint& j, BodyState* states, const BodySim* sims, floa
     }
   }
   {
-    Vec2 rA = rotate(sA->dq, j.anchorA);
-    Vec2 rB = rotate(sB->dq, j.anchorB);
+    Vec2 rA = rotate(sA.dq, j.anchorA);
+    Vec2 rB = rotate(sB.dq, j.anchorB);
     Vec2 Cdot = sub(add(vB, crossSV(wB, rB)), add(vA, crossSV(wA, rA)));
     Vec2 bias = {0.0f, 0.0f};
     float massScale = 1.0f;
     float impulseScale = 0.0f;
     if (useBias) {
-      Vec2 dcA = sA->dp;
-      Vec2 dcB = sB->dp;
+      Vec2 dcA = sA.dp;
+      Vec2 dcB = sB.dp;
       Vec2 separation = add(add(sub(dcB, dcA), sub(rB, rA)), j.deltaCenter);
       bias = mulSV(j.constraintSoftness.biasRate, separation);
       massScale = j.constraintSoftness.massScale;
@@ -196,10 +196,8 @@ F2D_HD void solveRevolute(Joint& j, BodyState* states, const BodySim* sims, floa
     vB = mulAdd(vB, mB, impulse);
     wB += iB * cross(rB, impulse);
   }
-  sA->v = vA;
-  sA->w = wA;
-  sB->v = vB;
-  sB->w = wB;
+  storeVel(states, ia, vA, wA);
+  storeVel(states, ib, vB, wB);
 }
 
 }  // namespace f2d

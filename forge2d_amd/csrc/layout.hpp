@@ -203,6 +203,13 @@ enum : int {
   PH_ALL = 31,
 };
 
+// Launch shape of a batch: workgroup size and dynamic LDS bytes per kernel.
+struct LaunchConfig {
+  int block;       // 64, 128 or 256 lanes per world
+  int collideLds;  // sort-and-sweep + pair-sort LDS
+  int solveLds;    // body-state LDS
+};
+
 // Contact constraint lanes (one constraint per lane), S/contact_solver.c:1034-1064.
 enum ConstraintField : int {
   CN_INDEX_A = 0, CN_INDEX_B, CN_SLOT,

@@ -336,15 +336,28 @@ F2D_HD void storeLane(const Cons& c, long lane, Contact& cs) {
 
 // ---------------------------------------------------------------------------- overflow (scalar order)
 
+// Static bodies have no solver state: the scalar solver reads and writes a dummy
+// identity state for them (S/contact_solver.c:175-176). Their inverse mass and
+// inertia are zero, so for finite impulses the dummy's velocities stay exactly +0
+// and its deltas the identity; reading the identity and dropping the write is
+// therefore the same computation.
+F2D_HD BodyState loadState(const BodyState* states, int idx) { return idx < 0 ? identityState() : states[idx]; }
+F2D_HD void storeVel(BodyState* states, int idx, Vec2 v, float w) {
+  if (idx >= 0) {
+    states[idx].v = v;
+    states[idx].w = w;
+  }
+}
+
 // b2WarmStartOverflowContacts body, S/contact_solver.c:160-225
-F2D_HD void overflowWarmStart(const Cons& c, long lane, BodyState* states, BodyState& dummy) {
+F2D_HD void overflowWarmStart(const Cons& c, long lane, BodyState* states) {
   int ia = c.geti(CN_INDEX_A, lane), ib = c.geti(CN_INDEX_B, lane);
-  BodyState* sA = ia < 0 ? &dummy : states + ia;
-  BodyState* sB = ib < 0 ? &dummy : states + ib;
-  Vec2 vA = sA->v;
-  float wA = sA->w;
-  Vec2 vB = sB->v;
-  float wB = sB->w;
+  BodyState sA = loadState(states, ia);
+  BodyState sB = loadState(states, ib);
+  Vec2 vA = sA.v;
+  float wA = sA.w;
+  Vec2 vB = sB.v;
+  float wB = sB.w;
   float mA = c(CN_INV_MASS_A, lane), iA = c(CN_INV_I_A, lane);
   float mB = c(CN_INV_MASS_B, lane), iB = c(CN_INV_I_B, lane);
   Vec2 normal = {c(CN_NX, lane), c(CN_NY, lane)};
@@ -363,27 +376,24 @@ F2D_HD void overflowWarmStart(const Cons& c, long lane, BodyState* states, BodyS
   float ri = c(CN_ROLLING_IMPULSE, lane);
   wA -= iA * ri;
   wB += iB * ri;
-  sA->v = vA;
-  sA->w = wA;
-  sB->v = vB;
-  sB->w = wB;
+  storeVel(states, ia, vA, wA);
+  storeVel(states, ib, vB, wB);
 }
 
 // b2SolveOverflowContacts body, S/contact_solver.c:227-380
-F2D_HD void overflowSolve(const Cons& c, long lane, BodyState* states, BodyState& dummy, float inv_h, float pushout,
-                          bool useBias) {
+F2D_HD void overflowSolve(const Cons& c, long lane, BodyState* states, float inv_h, float pushout, bool useBias) {
   float mA = c(CN_INV_MASS_A, lane), iA = c(CN_INV_I_A, lane);
   float mB = c(CN_INV_MASS_B, lane), iB = c(CN_INV_I_B, lane);
   int ia = c.geti(CN_INDEX_A, lane), ib = c.geti(CN_INDEX_B, lane);
-  BodyState* sA = ia < 0 ? &dummy : states + ia;
-  Vec2 vA = sA->v;
-  float wA = sA->w;
-  Rot dqA = sA->dq;
-  BodyState* sB = ib < 0 ? &dummy : states + ib;
-  Vec2 vB = sB->v;
-  float wB = sB->w;
-  Rot dqB = sB->dq;
-  Vec2 dp = sub(sB->dp, sA->dp);
+  BodyState sA = loadState(states, ia);
+  Vec2 vA = sA.v;
+  float wA = sA.w;
+  Rot dqA = sA.dq;
+  BodyState sB = loadState(states, ib);
+  Vec2 vB = sB.v;
+  float wB = sB.w;
+  Rot dqB = sB.dq;
+  Vec2 dp = sub(sB.dp, sA.dp);
   Vec2 normal = {c(CN_NX, lane), c(CN_NY, lane)};
   Vec2 tangent = rightPerp(normal);
   float friction = c(CN_FRICTION, lane);
@@ -452,25 +462,23 @@ F2D_HD void overflowSolve(const Cons& c, long lane, BodyState* states, BodyState
     wA -= iA * deltaLambda;
     wB += iB * deltaLambda;
   }
-  sA->v = vA;
-  sA->w = wA;
-  sB->v = vB;
-  sB->w = wB;
+  storeVel(states, ia, vA, wA);
+  storeVel(states, ib, vB, wB);
 }
 
 // b2ApplyOverflowRestitution body, S/contact_solver.c:382-478
-F2D_HD void overflowRestitution(const Cons& c, long lane, BodyState* states, BodyState& dummy, float threshold) {
+F2D_HD void overflowRestitution(const Cons& c, long lane, BodyState* states, float threshold) {
   float restitution = c(CN_RESTITUTION, lane);
   if (restitution == 0.0f) return;
   float mA = c(CN_INV_MASS_A, lane), iA = c(CN_INV_I_A, lane);
   float mB = c(CN_INV_MASS_B, lane), iB = c(CN_INV_I_B, lane);
   int ia = c.geti(CN_INDEX_A, lane), ib = c.geti(CN_INDEX_B, lane);
-  BodyState* sA = ia < 0 ? &dummy : states + ia;
-  Vec2 vA = sA->v;
-  float wA = sA->w;
-  BodyState* sB = ib < 0 ? &dummy : states + ib;
-  Vec2 vB = sB->v;
-  float wB = sB->w;
+  BodyState sA = loadState(states, ia);
+  Vec2 vA = sA.v;
+  float wA = sA.w;
+  BodyState sB = loadState(states, ib);
+  Vec2 vB = sB.v;
+  float wB = sB.w;
   Vec2 normal = {c(CN_NX, lane), c(CN_NY, lane)};
   int pointCount = c.geti(CN_POINT_COUNT, lane);
   for (int p = 0; p < pointCount; ++p) {
@@ -495,10 +503,8 @@ F2D_HD void overflowRestitution(const Cons& c, long lane, BodyState* states, Bod
     vB = mulAdd(vB, mB, P);
     wB += iB * cross(rB, P);
   }
-  sA->v = vA;
-  sA->w = wA;
-  sB->v = vB;
-  sB->w = wB;
+  storeVel(states, ia, vA, wA);
+  storeVel(states, ib, vB, wB);
 }
 
 // b2StoreOverflowImpulses body, S/contact_solver.c:480-509 (writes pointCount points)

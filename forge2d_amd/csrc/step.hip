@@ -26,12 +26,7 @@
 
 namespace f2d {
 
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
-constexpr int kLdsShapes = 1024;   // sort-and-sweep stays in LDS up to this many shapes
-constexpr int kLdsBodies = 1024;   // body states stay in LDS up to this many bodies
-constexpr int kLdsPairs = 1024;    // new-pair sort in LDS up to this many pairs
-constexpr int kLdsBytes = 32768;
+constexpr int kMaxWaves = 4;  // largest workgroup: 256 lanes
 
 struct SharedState {
   int colorCount[kGraphColorCount];
@@ -40,7 +35,7 @@ struct SharedState {
   int jointColorCount[kGraphColorCount];
   int jointColorStart[kGraphColorCount + 1];
   int jointColorFill[kGraphColorCount];
-  int waveSums[kWaves];
+  int waveSums[kMaxWaves];
   int counter;
   int changedCount;
   int error;
@@ -51,7 +46,9 @@ struct SharedState {
 
 // Ordered block-wide compaction step: returns this thread's slot among flagged
 // threads of the block and writes the block total.
+template <int BLOCK>
 __device__ __forceinline__ int blockCompact(bool flag, int* waveSums, int* total) {
+  constexpr int kWaves = BLOCK / 64;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   unsigned long long mask = __ballot(flag);
@@ -70,11 +67,11 @@ __device__ __forceinline__ int blockCompact(bool flag, int* waveSums, int* total
   return base + prefix;
 }
 
-template <typename K, typename V>
+template <int BLOCK, typename K, typename V>
 __device__ void bitonicSort(K* key, V* val, int n) {
   for (int k = 2; k <= n; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < n; i += kBlock) {
+      for (int i = threadIdx.x; i < n; i += BLOCK) {
         int ixj = i ^ j;
         if (ixj > i) {
           bool up = (i & k) == 0;
@@ -311,22 +308,23 @@ __device__ void createContact(const WorldPtrs& w, int shapeIdA, int shapeIdB) {
   c.tangentSpeed = 0.0f;
 }
 
-__device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds) {
+template <int BLOCK>
+__device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int ldsShapes, int ldsPairs) {
   WorldHeader* h = w.hdr;
   const int n = h->shapeCount;
   if (threadIdx.x == 0) sh.pairCount = 0;
   // any moved proxy at all?
   int anyMoved = 0;
-  for (int i = threadIdx.x; i < n; i += kBlock) anyMoved |= (w.shapes[i].flags & SF_MOVED) ? 1 : 0;
+  for (int i = threadIdx.x; i < n; i += BLOCK) anyMoved |= (w.shapes[i].flags & SF_MOVED) ? 1 : 0;
   anyMoved = __syncthreads_or(anyMoved);
   if (!anyMoved) return;
 
   const int np = nextPow2(n);
-  const bool inLds = np <= kLdsShapes;
+  const bool inLds = np <= ldsShapes;
   uint64_t* keys = inLds ? reinterpret_cast<uint64_t*>(lds) : reinterpret_cast<uint64_t*>(w.scratch);
   int* dummyVals = nullptr;
   (void)dummyVals;
-  for (int i = threadIdx.x; i < np; i += kBlock) {
+  for (int i = threadIdx.x; i < np; i += BLOCK) {
     uint64_t k = ~0ull;
     if (i < n) k = ((uint64_t)orderedKey(w.shapes[i].fat.lo.x) << 32) | (uint32_t)i;
     keys[i] = k;
@@ -335,7 +333,7 @@ __device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds) {
   // sort keys only (index rides in the low word)
   for (int k = 2; k <= np; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < np; i += kBlock) {
+      for (int i = threadIdx.x; i < np; i += BLOCK) {
         int ixj = i ^ j;
         if (ixj > i) {
           bool up = (i & k) == 0;
@@ -350,16 +348,16 @@ __device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds) {
     }
   }
   // sorted boxes and indices
-  float4* boxes = inLds ? reinterpret_cast<float4*>(lds + kLdsShapes * 8) : w.sapBoxes;
-  int* order = inLds ? reinterpret_cast<int*>(lds + kLdsShapes * 8 + kLdsShapes * 16) : (w.scratch + 2 * np);
-  for (int i = threadIdx.x; i < n; i += kBlock) {
+  float4* boxes = inLds ? reinterpret_cast<float4*>(lds + ldsShapes * 8) : w.sapBoxes;
+  int* order = inLds ? reinterpret_cast<int*>(lds + ldsShapes * 8 + ldsShapes * 16) : (w.scratch + 2 * np);
+  for (int i = threadIdx.x; i < n; i += BLOCK) {
     int idx = (int)(uint32_t)keys[i];
     AABB f = w.shapes[idx].fat;
     boxes[i] = make_float4(f.lo.x, f.lo.y, f.hi.x, f.hi.y);
     order[i] = idx;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += kBlock) {
+  for (int i = threadIdx.x; i < n; i += BLOCK) {
     float4 a = boxes[i];
     int s = order[i];
     for (int j = i + 1; j < n; ++j) {
@@ -379,16 +377,16 @@ __device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds) {
   if (count > 0) {
     // sort by move order key; values are pair indices
     const int pp = nextPow2(count);
-    const bool pairsInLds = pp <= kLdsPairs;
+    const bool pairsInLds = pp <= ldsPairs;
     uint64_t* pk = pairsInLds ? reinterpret_cast<uint64_t*>(lds) : reinterpret_cast<uint64_t*>(w.scratch);
-    int* pv = pairsInLds ? reinterpret_cast<int*>(lds + kLdsPairs * 8) : (w.scratch + 2 * pp);
+    int* pv = pairsInLds ? reinterpret_cast<int*>(lds + ldsPairs * 8) : (w.scratch + 2 * pp);
     __syncthreads();
-    for (int i = threadIdx.x; i < pp; i += kBlock) {
+    for (int i = threadIdx.x; i < pp; i += BLOCK) {
       pk[i] = i < count ? w.pairKeys[i] : ~0ull;
       pv[i] = i;
     }
     __syncthreads();
-    bitonicSort(pk, pv, pp);
+    bitonicSort<BLOCK>(pk, pv, pp);
     if (threadIdx.x == 0) {
       for (int i = 0; i < count; ++i) {
         int2 pr = w.pairShapes[pv[i]];
@@ -407,7 +405,7 @@ __device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds) {
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += kBlock) w.shapes[i].flags &= ~SF_MOVED;
+  for (int i = threadIdx.x; i < n; i += BLOCK) w.shapes[i].flags &= ~SF_MOVED;
   __syncthreads();
 }
 
@@ -480,10 +478,11 @@ __device__ void updateContact(const WorldPtrs& w, Contact& c) {
   c.m = m;
 }
 
+template <int BLOCK>
 __device__ void collidePhase(const WorldPtrs& w, SharedState& sh) {
   WorldHeader* h = w.hdr;
   const int n = h->contactCount;
-  for (int i = threadIdx.x; i < n; i += kBlock) {
+  for (int i = threadIdx.x; i < n; i += BLOCK) {
     Contact& c = w.contacts[i];
     if (c.shapeA < 0) continue;
     AABB fa = w.shapes[c.shapeA].fat, fb = w.shapes[c.shapeB].fat;
@@ -499,7 +498,7 @@ __device__ void collidePhase(const WorldPtrs& w, SharedState& sh) {
   if (threadIdx.x == 0) sh.changedCount = 0;
   __syncthreads();
   int* list = w.scratch;
-  for (int base = 0; base < n; base += kBlock) {
+  for (int base = 0; base < n; base += BLOCK) {
     int i = base + threadIdx.x;
     bool f = false;
     if (i < n) {
@@ -507,7 +506,7 @@ __device__ void collidePhase(const WorldPtrs& w, SharedState& sh) {
       f = c.shapeA >= 0 && (c.simFlags & (SIM_DISJOINT | SIM_STARTED | SIM_STOPPED)) != 0;
     }
     int total;
-    int slot = blockCompact(f, sh.waveSums, &total);
+    int slot = blockCompact<BLOCK>(f, sh.waveSums, &total);
     if (f) list[sh.changedCount + slot] = i;
     __syncthreads();
     if (threadIdx.x == 0) sh.changedCount += total;
@@ -671,7 +670,7 @@ struct SolveCtx {
   bool enableContinuous, enableSleep;
 };
 
-template <bool kLds>
+template <int BLOCK, bool kLds>
 __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState* ldsStates, const SolveCtx& ctx,
                                  int phases) {
   WorldHeader* h = w.hdr;
@@ -689,11 +688,11 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       sh.jointColorFill[threadIdx.x] = 0;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nc; i += kBlock) {
+    for (int i = threadIdx.x; i < nc; i += BLOCK) {
       const Contact& c = w.contacts[i];
       if (c.shapeA >= 0 && c.color >= 0 && c.color < kOverflowIndex) atomicAdd(&sh.colorCount[c.color], 1);
     }
-    for (int i = threadIdx.x; i < h->jointCount; i += kBlock) {
+    for (int i = threadIdx.x; i < h->jointCount; i += BLOCK) {
       const Joint& j = w.joints[i];
       if (j.color >= 0 && j.color < kOverflowIndex) atomicAdd(&sh.jointColorCount[j.color], 1);
     }
@@ -714,18 +713,18 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       h->activeColors = active;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nc; i += kBlock) {
+    for (int i = threadIdx.x; i < nc; i += BLOCK) {
       const Contact& c = w.contacts[i];
       if (c.shapeA >= 0 && c.color >= 0 && c.color < kOverflowIndex) {
         int lane = sh.colorStart[c.color] + atomicAdd(&sh.colorFill[c.color], 1);
         cons.seti(CN_SLOT, lane, i);
       }
     }
-    for (int i = threadIdx.x; i < h->overflowCount; i += kBlock) {
+    for (int i = threadIdx.x; i < h->overflowCount; i += BLOCK) {
       cons.seti(CN_SLOT, sh.colorStart[kOverflowIndex] + i, w.overflow[i]);
     }
     int* jointLanes = w.scratch;  // joint ids by colour
-    for (int i = threadIdx.x; i < h->jointCount; i += kBlock) {
+    for (int i = threadIdx.x; i < h->jointCount; i += BLOCK) {
       const Joint& j = w.joints[i];
       if (j.color >= 0 && j.color < kOverflowIndex) {
         jointLanes[sh.jointColorStart[j.color] + atomicAdd(&sh.jointColorFill[j.color], 1)] = i;
@@ -733,7 +732,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     }
     // body states into LDS
     if (kLds) {
-      for (int i = threadIdx.x; i < nb; i += kBlock) ldsStates[i] = w.states[i];
+      for (int i = threadIdx.x; i < nb; i += BLOCK) ldsStates[i] = w.states[i];
     }
     __syncthreads();
     const int totalLanes = sh.colorStart[kGraphColorCount];
@@ -742,11 +741,11 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     const int coloredJoints = sh.jointColorStart[kOverflowIndex];
 
     // -- prepare joints (all), S/solver.c:967-987 -> S/joint.c:1348-1402
-    for (int i = threadIdx.x; i < h->jointCount; i += kBlock) {
+    for (int i = threadIdx.x; i < h->jointCount; i += BLOCK) {
       prepareRevolute(w.joints[i], w.sims, ctx.h, ctx.inv_h, ctx.warmStartScale);
     }
     // -- prepare contacts (all lanes), S/contact_solver.c:1438-1669
-    for (int lane = threadIdx.x; lane < totalLanes; lane += kBlock) {
+    for (int lane = threadIdx.x; lane < totalLanes; lane += BLOCK) {
       int slot = cons.geti(CN_SLOT, lane);
       const Contact& c = w.contacts[slot];
       const BodySim& sa = w.sims[c.bodyA];
@@ -760,10 +759,9 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     }
     __syncthreads();
 
-    BodyState dummy = identityState();
     for (int sub = 0; sub < ctx.subSteps; ++sub) {
       // integrate velocities
-      for (int i = threadIdx.x; i < nb; i += kBlock) {
+      for (int i = threadIdx.x; i < nb; i += BLOCK) {
         BodySim& sim = w.sims[i];
         if (sim.type == kStatic) continue;
         BodyState st = states[i];
@@ -775,38 +773,36 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       if (threadIdx.x == 0) {
         for (int j = 0; j < h->jointCount; ++j)
           if (w.joints[j].color == kOverflowIndex) warmStartRevolute(w.joints[j], states, w.sims);
-        dummy = identityState();
-        for (int k = 0; k < overflowCount; ++k) overflowWarmStart(cons, overflowStart + k, states, dummy);
+        for (int k = 0; k < overflowCount; ++k) overflowWarmStart(cons, overflowStart + k, states);
       }
       __syncthreads();
       for (int col = 0; col < kOverflowIndex; ++col) {
         int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
         if (cnt + jcnt == 0) continue;
-        for (int k = threadIdx.x; k < jcnt; k += kBlock)
+        for (int k = threadIdx.x; k < jcnt; k += BLOCK)
           warmStartRevolute(w.joints[jointLanes[sh.jointColorStart[col] + k]], states, w.sims);
-        for (int k = threadIdx.x; k < cnt; k += kBlock) warmStartLane(cons, sh.colorStart[col] + k, states);
+        for (int k = threadIdx.x; k < cnt; k += BLOCK) warmStartLane(cons, sh.colorStart[col] + k, states);
         __syncthreads();
       }
       // solve
       if (threadIdx.x == 0) {
         for (int j = 0; j < h->jointCount; ++j)
           if (w.joints[j].color == kOverflowIndex) solveRevolute(w.joints[j], states, w.sims, ctx.h, ctx.inv_h, true);
-        dummy = identityState();
         for (int k = 0; k < overflowCount; ++k)
-          overflowSolve(cons, overflowStart + k, states, dummy, ctx.inv_h, ctx.maxContactPushSpeed, true);
+          overflowSolve(cons, overflowStart + k, states, ctx.inv_h, ctx.maxContactPushSpeed, true);
       }
       __syncthreads();
       for (int col = 0; col < kOverflowIndex; ++col) {
         int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
         if (cnt + jcnt == 0) continue;
-        for (int k = threadIdx.x; k < jcnt; k += kBlock)
+        for (int k = threadIdx.x; k < jcnt; k += BLOCK)
           solveRevolute(w.joints[jointLanes[sh.jointColorStart[col] + k]], states, w.sims, ctx.h, ctx.inv_h, true);
-        for (int k = threadIdx.x; k < cnt; k += kBlock)
+        for (int k = threadIdx.x; k < cnt; k += BLOCK)
           solveLane(cons, sh.colorStart[col] + k, states, ctx.inv_h, -ctx.contactSpeed, true);
         __syncthreads();
       }
       // integrate positions
-      for (int i = threadIdx.x; i < nb; i += kBlock) {
+      for (int i = threadIdx.x; i < nb; i += BLOCK) {
         if (w.sims[i].type == kStatic) continue;
         BodyState st = states[i];
         integratePosition(st, ctx.h);
@@ -817,37 +813,35 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       if (threadIdx.x == 0) {
         for (int j = 0; j < h->jointCount; ++j)
           if (w.joints[j].color == kOverflowIndex) solveRevolute(w.joints[j], states, w.sims, ctx.h, ctx.inv_h, false);
-        dummy = identityState();
         for (int k = 0; k < overflowCount; ++k)
-          overflowSolve(cons, overflowStart + k, states, dummy, ctx.inv_h, ctx.maxContactPushSpeed, false);
+          overflowSolve(cons, overflowStart + k, states, ctx.inv_h, ctx.maxContactPushSpeed, false);
       }
       __syncthreads();
       for (int col = 0; col < kOverflowIndex; ++col) {
         int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
         if (cnt + jcnt == 0) continue;
-        for (int k = threadIdx.x; k < jcnt; k += kBlock)
+        for (int k = threadIdx.x; k < jcnt; k += BLOCK)
           solveRevolute(w.joints[jointLanes[sh.jointColorStart[col] + k]], states, w.sims, ctx.h, ctx.inv_h, false);
-        for (int k = threadIdx.x; k < cnt; k += kBlock)
+        for (int k = threadIdx.x; k < cnt; k += BLOCK)
           solveLane(cons, sh.colorStart[col] + k, states, ctx.inv_h, -ctx.contactSpeed, false);
         __syncthreads();
       }
     }
     // restitution: overflow then colours
     if (threadIdx.x == 0) {
-      dummy = identityState();
       for (int k = 0; k < overflowCount; ++k)
-        overflowRestitution(cons, overflowStart + k, states, dummy, ctx.restitutionThreshold);
+        overflowRestitution(cons, overflowStart + k, states, ctx.restitutionThreshold);
     }
     __syncthreads();
     for (int col = 0; col < kOverflowIndex; ++col) {
       int cnt = sh.colorCount[col];
       if (cnt == 0) continue;
-      for (int k = threadIdx.x; k < cnt; k += kBlock)
+      for (int k = threadIdx.x; k < cnt; k += BLOCK)
         restitutionLane(cons, sh.colorStart[col] + k, states, ctx.restitutionThreshold);
       __syncthreads();
     }
     // store impulses
-    for (int lane = threadIdx.x; lane < totalLanes; lane += kBlock) {
+    for (int lane = threadIdx.x; lane < totalLanes; lane += BLOCK) {
       int slot = cons.geti(CN_SLOT, lane);
       if (lane >= overflowStart) {
         overflowStore(cons, lane, w.contacts[slot]);
@@ -858,13 +852,13 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     (void)coloredJoints;
     __syncthreads();
   } else if (kLds) {
-    for (int i = threadIdx.x; i < nb; i += kBlock) ldsStates[i] = w.states[i];
+    for (int i = threadIdx.x; i < nb; i += BLOCK) ldsStates[i] = w.states[i];
     __syncthreads();
   }
 
   if (phases & PH_FINALIZE) {
     // b2FinalizeBodiesTask, S/solver.c:543-726
-    for (int i = threadIdx.x; i < nb; i += kBlock) {
+    for (int i = threadIdx.x; i < nb; i += BLOCK) {
       BodySim& sim = w.sims[i];
       if (sim.type == kStatic) continue;
       BodyState st = states[i];
@@ -934,22 +928,14 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     }
     __syncthreads();
   } else if (kLds && (phases & PH_SOLVE)) {
-    for (int i = threadIdx.x; i < nb; i += kBlock) w.states[i] = ldsStates[i];
+    for (int i = threadIdx.x; i < nb; i += BLOCK) w.states[i] = ldsStates[i];
     __syncthreads();
   }
 }
 
-// ---------------------------------------------------------------------------- kernel
+// ---------------------------------------------------------------------------- kernels
 
-__global__ __launch_bounds__(kBlock) void stepWorldsKernel(BatchView b, StepParams sp, const int* worldList) {
-  __shared__ SharedState sh;
-  __shared__ __align__(16) char lds[kLdsBytes];
-  const int wi = worldList ? worldList[blockIdx.x] : blockIdx.x;
-  WorldHeader* h = b.worlds + wi;
-  WorldPtrs w = worldPtrs(b, h);
-  if (threadIdx.x == 0) sh.error = 0;
-
-  SolveCtx ctx;
+__device__ void stepContext(const WorldHeader* h, const StepParams& sp, SolveCtx& ctx) {
   ctx.dt = sp.dt;
   ctx.inv_dt = sp.inv_dt;
   ctx.h = sp.h;
@@ -968,30 +954,82 @@ __global__ __launch_bounds__(kBlock) void stepWorldsKernel(BatchView b, StepPara
   ctx.gravity = h->gravity;
   ctx.enableContinuous = (h->flags & WF_CONTINUOUS) != 0;
   ctx.enableSleep = (h->flags & WF_SLEEP) != 0;
-  __syncthreads();
+}
+
+// Pairs, collide and contact state of one world per workgroup. ldsBytes of dynamic
+// LDS hold the sort-and-sweep arrays (28 B per shape) and the new-pair sort (12 B
+// per pair); larger worlds use their global scratch region instead.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void collideKernel(BatchView b, StepParams sp, const int* worldList, int ldsBytes) {
+  __shared__ SharedState sh;
+  extern __shared__ __align__(16) char lds[];
+  const int wi = worldList ? worldList[blockIdx.x] : blockIdx.x;
+  WorldHeader* h = b.worlds + wi;
+  WorldPtrs w = worldPtrs(b, h);
   if (threadIdx.x == 0) {
+    SolveCtx ctx;
+    stepContext(h, sp, ctx);
+    sh.error = 0;
     h->inv_h = sp.inv_h;
     h->contactSpeed = ctx.contactSpeed;
   }
-
-  if (sp.phases & PH_PAIRS) pairsPhase(w, sh, lds);
-  if (sp.phases & PH_COLLIDE) collidePhase(w, sh);
+  __syncthreads();
+  int ldsShapes = 1;
+  while (ldsShapes * 2 * 28 <= ldsBytes) ldsShapes *= 2;
+  if (ldsShapes * 28 > ldsBytes) ldsShapes = 0;
+  int ldsPairs = 1;
+  while (ldsPairs * 2 * 12 <= ldsBytes) ldsPairs *= 2;
+  if (ldsPairs * 12 > ldsBytes) ldsPairs = 0;
+  if (sp.phases & PH_PAIRS) pairsPhase<BLOCK>(w, sh, lds, ldsShapes, ldsPairs);
+  if (sp.phases & PH_COLLIDE) collidePhase<BLOCK>(w, sh);
   if (sp.phases & PH_STATE) statePhase(w, sh);
-  if (sp.phases & (PH_SOLVE | PH_FINALIZE)) {
-    if (h->bodyCount <= kLdsBodies) {
-      solveAndFinalize<true>(w, sh, reinterpret_cast<BodyState*>(lds), ctx, sp.phases);
-    } else {
-      solveAndFinalize<false>(w, sh, nullptr, ctx, sp.phases);
-    }
+  if (threadIdx.x == 0 && (sp.phases & (PH_SOLVE | PH_FINALIZE)) == 0) h->stepIndex += 1;
+}
+
+// Solver stages and finalize of one world per workgroup; body states live in
+// ldsBytes of dynamic LDS (32 B per body) when they fit.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void solveKernel(BatchView b, StepParams sp, const int* worldList, int ldsBytes) {
+  __shared__ SharedState sh;
+  extern __shared__ __align__(16) char lds[];
+  const int wi = worldList ? worldList[blockIdx.x] : blockIdx.x;
+  WorldHeader* h = b.worlds + wi;
+  WorldPtrs w = worldPtrs(b, h);
+  SolveCtx ctx;
+  stepContext(h, sp, ctx);
+  if (h->bodyCount * (int)sizeof(BodyState) <= ldsBytes) {
+    solveAndFinalize<BLOCK, true>(w, sh, reinterpret_cast<BodyState*>(lds), ctx, sp.phases);
+  } else {
+    solveAndFinalize<BLOCK, false>(w, sh, nullptr, ctx, sp.phases);
   }
   if (threadIdx.x == 0) h->stepIndex += 1;
+}
+
+template <int BLOCK>
+static int launchBlock(const BatchView* b, const StepParams* sp, const int* worldList, int nWorlds, hipStream_t stream,
+                       const LaunchConfig* cfg) {
+  if (sp->phases & (PH_PAIRS | PH_COLLIDE | PH_STATE)) {
+    hipLaunchKernelGGL(collideKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->collideLds, stream, *b, *sp, worldList,
+                       cfg->collideLds);
+  }
+  if (sp->phases & (PH_SOLVE | PH_FINALIZE)) {
+    hipLaunchKernelGGL(solveKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->solveLds, stream, *b, *sp, worldList,
+                       cfg->solveLds);
+  }
+  return (int)hipGetLastError();
 }
 
 }  // namespace f2d
 
 extern "C" int f2d_launch_step(const f2d::BatchView* b, const f2d::StepParams* sp, const int* worldList, int nWorlds,
-                               hipStream_t stream) {
+                               hipStream_t stream, const f2d::LaunchConfig* cfg) {
   if (nWorlds <= 0) return 0;
-  hipLaunchKernelGGL(f2d::stepWorldsKernel, dim3(nWorlds), dim3(f2d::kBlock), 0, stream, *b, *sp, worldList);
-  return (int)hipGetLastError();
+  switch (cfg->block) {
+    case 64:
+      return f2d::launchBlock<64>(b, sp, worldList, nWorlds, stream, cfg);
+    case 128:
+      return f2d::launchBlock<128>(b, sp, worldList, nWorlds, stream, cfg);
+    default:
+      return f2d::launchBlock<256>(b, sp, worldList, nWorlds, stream, cfg);
+  }
 }

@@ -167,6 +167,12 @@ long f2dg_export_world(void* batch, int world, void* out, long cap);
 /* overlay a snapshot onto a world built from the same scene spec */
 int f2dg_import_world(void* batch, int world, const void* snapshot, long bytes);
 const char* f2dg_last_error(void* batch);
+/* workgroup-size autotuner: times `steps` steps with 64, 128 and 256 lanes per world
+ * (ms_out[3] optional), keeps the fastest and returns it; the trial steps advance the batch */
+int f2dg_autotune(void* batch, float dt, int sub_steps, int steps, float* ms_out);
+/* force a workgroup size (64, 128 or 256) / read the current one */
+int f2dg_set_block(void* batch, int block);
+int f2dg_get_block(void* batch);
 /* aggregate counts after f2dg_sync: [awake bodies, shapes, touching contacts, touching points,
  * restitution contacts, restitution points, joints, overflow contacts] */
 int f2dg_batch_stats(void* batch, long* out);
