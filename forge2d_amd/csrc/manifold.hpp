@@ -12,6 +12,31 @@ namespace f2d {
 
 F2D_HD int makeId(int a, int b) { return ((a & 0xFF) << 8) | (b & 0xFF); }
 
+// Polygon loops run over the fixed maximum of 8 vertices with a guard on the
+// live count, and runtime-indexed reads are select chains, so the vertex arrays
+// stay in registers. Live iterations execute the same operations in the same
+// order as the reference's loops.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define F2D_UNROLL _Pragma("unroll")
+#else
+#define F2D_UNROLL
+#endif
+
+// Select a[i] for 0 <= i < 8 with a three-level tree of selects on the bits of i.
+F2D_HD float pickf(float a0, float a1, float a2, float a3, float a4, float a5, float a6, float a7, int i) {
+  float t0 = (i & 1) ? a1 : a0;
+  float t1 = (i & 1) ? a3 : a2;
+  float t2 = (i & 1) ? a5 : a4;
+  float t3 = (i & 1) ? a7 : a6;
+  float u0 = (i & 2) ? t1 : t0;
+  float u1 = (i & 2) ? t3 : t2;
+  return (i & 4) ? u1 : u0;
+}
+F2D_HD Vec2 pick(const Vec2* a, int i) {
+  return {pickf(a[0].x, a[1].x, a[2].x, a[3].x, a[4].x, a[5].x, a[6].x, a[7].x, i),
+          pickf(a[0].y, a[1].y, a[2].y, a[3].y, a[4].y, a[5].y, a[6].y, a[7].y, i)};
+}
+
 struct Poly {
   Vec2 v[kMaxPolygonVertices];
   Vec2 n[kMaxPolygonVertices];
@@ -124,18 +149,21 @@ F2D_HD void collidePolygonAndCircle(Manifold& m, const Poly& A, Xf xfA, Vec2 cen
   int normalIndex = 0;
   float separation = -kFltMax;
   int vertexCount = A.count;
-  for (int i = 0; i < vertexCount; ++i) {
-    float s = dot(A.n[i], sub(center, A.v[i]));
-    if (s > separation) {
-      separation = s;
-      normalIndex = i;
+  F2D_UNROLL
+  for (int i = 0; i < kMaxPolygonVertices; ++i) {
+    if (i < vertexCount) {
+      float s = dot(A.n[i], sub(center, A.v[i]));
+      if (s > separation) {
+        separation = s;
+        normalIndex = i;
+      }
     }
   }
   if (separation > radius + speculativeDistance) return;
   int vertIndex1 = normalIndex;
   int vertIndex2 = vertIndex1 + 1 < vertexCount ? vertIndex1 + 1 : 0;
-  Vec2 v1 = A.v[vertIndex1];
-  Vec2 v2 = A.v[vertIndex2];
+  Vec2 v1 = pick(A.v, vertIndex1);
+  Vec2 v2 = pick(A.v, vertIndex2);
   float u1 = dot(sub(center, v1), sub(v2, v1));
   float u2 = dot(sub(center, v2), sub(v1, v2));
   if (u1 < 0.0f && separation > kFltEpsilon) {
@@ -169,7 +197,7 @@ F2D_HD void collidePolygonAndCircle(Manifold& m, const Poly& A, Xf xfA, Vec2 cen
     mp.id = 0;
     m.pointCount = 1;
   } else {
-    Vec2 normal = A.n[normalIndex];
+    Vec2 normal = pick(A.n, normalIndex);
     m.normal = rotate(xfA.q, normal);
     Vec2 cA = mulAdd(center, radiusA - dot(sub(center, v1), normal), normal);
     Vec2 cB = mulSub(center, radiusB, normal);
@@ -338,11 +366,14 @@ F2D_HD void collideCapsules(Manifold& m, Vec2 a1, Vec2 a2, float radiusA, Xf xfA
     m.pointCount = 1;
   }
   m.normal = rotate(xfA.q, m.normal);
-  for (int i = 0; i < m.pointCount; ++i) {
-    ManifoldPoint& mp = m.points[i];
-    mp.anchorA = rotate(xfA.q, add(mp.anchorA, origin));
-    mp.anchorB = add(mp.anchorA, sub(xfA.p, xfB.p));
-    mp.point = add(xfA.p, mp.anchorA);
+  F2D_UNROLL
+  for (int i = 0; i < 2; ++i) {
+    if (i < m.pointCount) {
+      ManifoldPoint& mp = m.points[i];
+      mp.anchorA = rotate(xfA.q, add(mp.anchorA, origin));
+      mp.anchorB = add(mp.anchorA, sub(xfA.p, xfB.p));
+      mp.point = add(xfA.p, mp.anchorA);
+    }
   }
 }
 
@@ -400,30 +431,34 @@ F2D_HD SegmentDistanceResult segmentDistance(Vec2 p1, Vec2 q1, Vec2 p2, Vec2 q2)
 // b2ClipPolygons, S/manifold.c:545-678
 F2D_HD void clipPolygons(Manifold& m, const Poly& polyA, const Poly& polyB, int edgeA, int edgeB, bool flip) {
   zeroManifold(m);
-  const Poly* poly1;
-  int i11, i12;
-  const Poly* poly2;
-  int i21, i22;
+  int i11, i12, i21, i22;
+  Vec2 normal, v11, v12, v21, v22;
+  float r1, r2;
   if (flip) {
-    poly1 = &polyB;
-    poly2 = &polyA;
     i11 = edgeB;
     i12 = edgeB + 1 < polyB.count ? edgeB + 1 : 0;
     i21 = edgeA;
     i22 = edgeA + 1 < polyA.count ? edgeA + 1 : 0;
+    normal = pick(polyB.n, i11);
+    v11 = pick(polyB.v, i11);
+    v12 = pick(polyB.v, i12);
+    v21 = pick(polyA.v, i21);
+    v22 = pick(polyA.v, i22);
+    r1 = polyB.radius;
+    r2 = polyA.radius;
   } else {
-    poly1 = &polyA;
-    poly2 = &polyB;
     i11 = edgeA;
     i12 = edgeA + 1 < polyA.count ? edgeA + 1 : 0;
     i21 = edgeB;
     i22 = edgeB + 1 < polyB.count ? edgeB + 1 : 0;
+    normal = pick(polyA.n, i11);
+    v11 = pick(polyA.v, i11);
+    v12 = pick(polyA.v, i12);
+    v21 = pick(polyB.v, i21);
+    v22 = pick(polyB.v, i22);
+    r1 = polyA.radius;
+    r2 = polyB.radius;
   }
-  Vec2 normal = poly1->n[i11];
-  Vec2 v11 = poly1->v[i11];
-  Vec2 v12 = poly1->v[i12];
-  Vec2 v21 = poly2->v[i21];
-  Vec2 v22 = poly2->v[i22];
   Vec2 tangent = crossSV(1.0f, normal);
   float lower1 = 0.0f;
   float upper1 = dot(sub(v12, v11), tangent);
@@ -444,8 +479,6 @@ F2D_HD void clipPolygons(Manifold& m, const Poly& polyA, const Poly& polyB, int 
   }
   float separationLower = dot(sub(vLower, v11), normal);
   float separationUpper = dot(sub(vUpper, v11), normal);
-  float r1 = poly1->radius;
-  float r2 = poly2->radius;
   vLower = mulAdd(vLower, 0.5f * (r1 - r2 - separationLower), normal);
   vUpper = mulAdd(vUpper, 0.5f * (r1 - r2 - separationUpper), normal);
   float radius = r1 + r2;
@@ -476,17 +509,23 @@ F2D_HD float findMaxSeparation(int* edgeIndex, const Poly& poly1, const Poly& po
   int count2 = poly2.count;
   int bestIndex = 0;
   float maxSeparation = -kFltMax;
-  for (int i = 0; i < count1; ++i) {
-    Vec2 n = poly1.n[i];
-    Vec2 v1 = poly1.v[i];
-    float si = kFltMax;
-    for (int j = 0; j < count2; ++j) {
-      float sij = dot(n, sub(poly2.v[j], v1));
-      if (sij < si) si = sij;
-    }
-    if (si > maxSeparation) {
-      maxSeparation = si;
-      bestIndex = i;
+  F2D_UNROLL
+  for (int i = 0; i < kMaxPolygonVertices; ++i) {
+    if (i < count1) {
+      Vec2 n = poly1.n[i];
+      Vec2 v1 = poly1.v[i];
+      float si = kFltMax;
+      F2D_UNROLL
+      for (int j = 0; j < kMaxPolygonVertices; ++j) {
+        if (j < count2) {
+          float sij = dot(n, sub(poly2.v[j], v1));
+          if (sij < si) si = sij;
+        }
+      }
+      if (si > maxSeparation) {
+        maxSeparation = si;
+        bestIndex = i;
+      }
     }
   }
   *edgeIndex = bestIndex;
@@ -521,14 +560,16 @@ F2D_HD void collidePolygons(Manifold& m, const Poly& polygonA, Xf xfA, const Pol
   localPolyA.radius = polygonA.radius;
   localPolyA.v[0] = {0.0f, 0.0f};
   localPolyA.n[0] = polygonA.n[0];
-  for (int i = 1; i < localPolyA.count; ++i) {
+  F2D_UNROLL
+  for (int i = 1; i < kMaxPolygonVertices; ++i) {
     localPolyA.v[i] = sub(polygonA.v[i], origin);
     localPolyA.n[i] = polygonA.n[i];
   }
   Poly localPolyB;
   localPolyB.count = polygonB.count;
   localPolyB.radius = polygonB.radius;
-  for (int i = 0; i < localPolyB.count; ++i) {
+  F2D_UNROLL
+  for (int i = 0; i < kMaxPolygonVertices; ++i) {
     localPolyB.v[i] = transformPoint(xf, polygonB.v[i]);
     localPolyB.n[i] = rotate(xf.q, polygonB.n[i]);
   }
@@ -541,28 +582,34 @@ F2D_HD void collidePolygons(Manifold& m, const Poly& polygonA, Xf xfA, const Pol
   bool flip;
   if (separationA >= separationB) {
     flip = false;
-    Vec2 searchDirection = localPolyA.n[edgeA];
+    Vec2 searchDirection = pick(localPolyA.n, edgeA);
     int count = localPolyB.count;
     edgeB = 0;
     float minDot = kFltMax;
-    for (int i = 0; i < count; ++i) {
-      float d = dot(searchDirection, localPolyB.n[i]);
-      if (d < minDot) {
-        minDot = d;
-        edgeB = i;
+    F2D_UNROLL
+    for (int i = 0; i < kMaxPolygonVertices; ++i) {
+      if (i < count) {
+        float d = dot(searchDirection, localPolyB.n[i]);
+        if (d < minDot) {
+          minDot = d;
+          edgeB = i;
+        }
       }
     }
   } else {
     flip = true;
-    Vec2 searchDirection = localPolyB.n[edgeB];
+    Vec2 searchDirection = pick(localPolyB.n, edgeB);
     int count = localPolyA.count;
     edgeA = 0;
     float minDot = kFltMax;
-    for (int i = 0; i < count; ++i) {
-      float d = dot(searchDirection, localPolyA.n[i]);
-      if (d < minDot) {
-        minDot = d;
-        edgeA = i;
+    F2D_UNROLL
+    for (int i = 0; i < kMaxPolygonVertices; ++i) {
+      if (i < count) {
+        float d = dot(searchDirection, localPolyA.n[i]);
+        if (d < minDot) {
+          minDot = d;
+          edgeA = i;
+        }
       }
     }
   }
@@ -571,17 +618,18 @@ F2D_HD void collidePolygons(Manifold& m, const Poly& polygonA, Xf xfA, const Pol
     int i12 = edgeA + 1 < localPolyA.count ? edgeA + 1 : 0;
     int i21 = edgeB;
     int i22 = edgeB + 1 < localPolyB.count ? edgeB + 1 : 0;
-    Vec2 v11 = localPolyA.v[i11];
-    Vec2 v12 = localPolyA.v[i12];
-    Vec2 v21 = localPolyB.v[i21];
-    Vec2 v22 = localPolyB.v[i22];
+    Vec2 v11 = pick(localPolyA.v, i11);
+    Vec2 v12 = pick(localPolyA.v, i12);
+    Vec2 v21 = pick(localPolyB.v, i21);
+    Vec2 v22 = pick(localPolyB.v, i22);
     SegmentDistanceResult result = segmentDistance(v11, v12, v21, v22);
     float distance = sqrtf(result.distanceSquared);
     float separation = distance - radius;
     if (distance - radius > speculativeDistance) return;
     clipPolygons(m, localPolyA, localPolyB, edgeA, edgeB, flip);
     float minSeparation = kFltMax;
-    for (int i = 0; i < m.pointCount; ++i) minSeparation = minf(minSeparation, m.points[i].separation);
+    if (m.pointCount > 0) minSeparation = minf(minSeparation, m.points[0].separation);
+    if (m.pointCount > 1) minSeparation = minf(minSeparation, m.points[1].separation);
     if (separation + 0.1f * linearSlop < minSeparation) {
       if (result.fraction1 == 0.0f && result.fraction2 == 0.0f) {
         vertexVertex(m, v11, v21, localPolyA.radius, localPolyB.radius, distance, radius, makeId(i11, i21));
@@ -598,11 +646,14 @@ F2D_HD void collidePolygons(Manifold& m, const Poly& polygonA, Xf xfA, const Pol
   }
   if (m.pointCount > 0) {
     m.normal = rotate(xfA.q, m.normal);
-    for (int i = 0; i < m.pointCount; ++i) {
-      ManifoldPoint& mp = m.points[i];
-      mp.anchorA = rotate(xfA.q, add(mp.anchorA, origin));
-      mp.anchorB = add(mp.anchorA, sub(xfA.p, xfB.p));
-      mp.point = add(xfA.p, mp.anchorA);
+    F2D_UNROLL
+    for (int i = 0; i < 2; ++i) {
+      if (i < m.pointCount) {
+        ManifoldPoint& mp = m.points[i];
+        mp.anchorA = rotate(xfA.q, add(mp.anchorA, origin));
+        mp.anchorB = add(mp.anchorA, sub(xfA.p, xfB.p));
+        mp.point = add(xfA.p, mp.anchorA);
+      }
     }
   }
 }
@@ -611,6 +662,7 @@ F2D_HD void loadPoly(Poly& p, const ShapeGeom& g) {
   p.count = g.count;
   p.radius = g.radius;
   p.centroid = g.centroid;
+  F2D_UNROLL
   for (int i = 0; i < kMaxPolygonVertices; ++i) {
     p.v[i] = g.v[i];
     p.n[i] = g.n[i];
@@ -688,10 +740,13 @@ F2D_HD AABB computeShapeAABB(const ShapeGeom& g, Xf xf) {
   if (g.type == kPolygon) {
     Vec2 lower = transformPoint(xf, g.v[0]);
     Vec2 upper = lower;
-    for (int i = 1; i < g.count; ++i) {
-      Vec2 v = transformPoint(xf, g.v[i]);
-      lower = vmin(lower, v);
-      upper = vmax(upper, v);
+    F2D_UNROLL
+    for (int i = 1; i < kMaxPolygonVertices; ++i) {
+      if (i < g.count) {
+        Vec2 v = transformPoint(xf, g.v[i]);
+        lower = vmin(lower, v);
+        upper = vmax(upper, v);
+      }
     }
     Vec2 r = {g.radius, g.radius};
     return {sub(lower, r), add(upper, r)};

@@ -443,7 +443,9 @@ __device__ void updateContact(const WorldPtrs& w, Contact& c) {
     c.simFlags &= ~SIM_HIT_EVENT;
   }
   if (pointCount > 0) m.rollingImpulse = old.rollingImpulse;
-  for (int i = 0; i < pointCount; ++i) {
+  F2D_UNROLL
+  for (int i = 0; i < 2; ++i) {
+    if (i >= pointCount) break;
     ManifoldPoint& mp2 = m.points[i];
     mp2.anchorA = sub(mp2.anchorA, centerOffsetA);
     mp2.anchorB = sub(mp2.anchorB, centerOffsetB);
@@ -453,7 +455,9 @@ __device__ void updateContact(const WorldPtrs& w, Contact& c) {
     mp2.normalVelocity = 0.0f;
     int id2 = mp2.id & 0xFFFF;
     mp2.id = id2;  // persisted = false
-    for (int j = 0; j < old.pointCount; ++j) {
+    F2D_UNROLL
+    for (int j = 0; j < 2; ++j) {
+      if (j >= old.pointCount) break;
       ManifoldPoint& mp1 = old.points[j];
       if ((mp1.id & 0xFFFF) == id2) {
         mp2.normalImpulse = mp1.normalImpulse;
