@@ -47,17 +47,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(st: dict) -> float:
+def algorithmic_bytes(st: dict) -> dict:
     """Minimal FP32 field traffic of one step when every stage is a full pass over
-    HBM: per substep 88 B/body (integrate velocities + positions), 380 + 128 p per
-    contact (warm start, solve, relax), 576 per joint; per step 136 B/body and
-    228 B/shape (collide inputs, finalize, broadphase), 424 + 162 p per contact
-    (prepare, store), 148 + 52 p per contact with restitution, 120 per joint."""
+    HBM, split by kernel.
+
+    solveKernel, per substep: 88 B/body (integrate velocities 44 + positions 44),
+    380 + 128 p per contact constraint (warm start 84 + 24 p, solve and relax
+    148 + 52 p each), 576 per revolute joint; per step: 136 B/body and 112 B/shape
+    (finalize: body state + transform write, shape AABB + fat AABB), 148 + 100 p per
+    constraint (prepare 140 + 72 p, store 8 + 28 p), 148 + 52 p per constraint with
+    restitution, 120 per joint (prepare).
+    collideKernel, per step: 116 B/shape (fat-AABB sort-and-sweep and move flags),
+    276 + 62 p per contact (two transforms, shape geometry, old and new manifold).
+    The two add up to S.[88 Nb + 380 C + 128 P + 576 Nj] + [136 Nb + 228 Ns + 424 C
+    + 162 P + 148 Cr + 52 Pr + 120 Nj]."""
     Nb, Ns, C, P = st["bodies"], st["shapes"], st["contacts"], st["points"]
     Cr, Pr, Nj = st["restitution_contacts"], st["restitution_points"], st["joints"]
     per_sub = 88 * Nb + 380 * C + 128 * P + 576 * Nj
-    per_step = 136 * Nb + 228 * Ns + 424 * C + 162 * P + 148 * Cr + 52 * Pr + 120 * Nj
-    return float(SUBSTEPS * per_sub + per_step)
+    solve = SUBSTEPS * per_sub + 136 * Nb + 112 * Ns + 148 * C + 100 * P + 148 * Cr + 52 * Pr + 120 * Nj
+    collide = 116 * Ns + 276 * C + 62 * P
+    return {"solveKernel": float(solve), "collideKernel": float(collide), "step": float(solve + collide)}
 
 
 def load_pmc(workload: str):
@@ -109,12 +118,17 @@ def main():
     ap.add_argument("--per-world-sample", type=int, default=64)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain elementwise copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: every world of a batch lives on a single device and worlds never "
               f"exchange data; --gpus {args.gpus} is not supported (shard worlds across processes instead)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
 
     desc, make = WORKLOADS[args.workload]
     t0 = time.time()
@@ -149,14 +163,16 @@ def main():
     value = awake * SUBSTEPS * args.steps / wall
     log(f"[bench] batched: {ms_per_step:.3f} ms/step, {value:.4e} body-substeps/s")
 
-    # kernel duration with HIP events on the launch stream (one launch per step)
+    # kernel durations with HIP events on the launch stream, per kernel
     ev_steps = max(4, min(args.steps, 32))
-    ev_ms = batch.step_timed(DT, SUBSTEPS, steps=ev_steps)
-    launch_ms = ev_ms / ev_steps
-    bytes_step = algorithmic_bytes(st)
-    achieved = bytes_step / (launch_ms * 1e-3) / 1e9
+    launch_ms = {k: v / ev_steps for k, v in batch.kernel_times(DT, SUBSTEPS, steps=ev_steps).items()}
+    nbytes = algorithmic_bytes(st)
+    dominant = max(launch_ms, key=launch_ms.get)
+    achieved = nbytes[dominant] / (launch_ms[dominant] * 1e-3) / 1e9
+    step_gbs = nbytes["step"] / (sum(launch_ms.values()) * 1e-3) / 1e9
     pmc = load_pmc(args.workload)
-    traffic = pmc.get("hbm_bytes_per_launch") if pmc else None
+    traffic = pmc.get(dominant, {}).get("hbm_bytes_per_launch") if pmc else None
+    log(f"[bench] kernel ms/launch {launch_ms}, algorithmic bytes {nbytes}, {dominant} {achieved:.0f} GB/s")
 
     # per-step latency percentiles (each step synchronized on its own)
     lat = []
@@ -171,7 +187,7 @@ def main():
 
     # one-world-at-a-time GPU path on a sample of the same worlds
     per_world = None
-    if W > 1:
+    if W > 1 and args.per_world_sample > 0:
         n = min(W, args.per_world_sample)
         sub = F.Batch(spec_subset(spec, n))
         sub_awake = sub.awake_bodies()
@@ -219,12 +235,29 @@ def main():
         "batched_vs_per_world": (value / per_world["value"]) if per_world else None,
         "batched_vs_cpu": (value / cpu["value"]) if cpu else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "collideKernel + solveKernel (one step)", "launch_ms": launch_ms,
-                     "algorithmic_bytes_per_launch": bytes_step},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dominant,
+                     "launch_ms": launch_ms[dominant], "algorithmic_bytes_per_launch": nbytes[dominant],
+                     "other_kernel": {k: {"launch_ms": launch_ms[k], "algorithmic_bytes_per_launch": nbytes.get(k)}
+                                      for k in launch_ms if k != dominant},
+                     "step_achieved_gbs": step_gbs},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(mib: int):
+    """A plain copy kernel of known size (reads and writes mib MiB once per launch):
+    the ratio of its FETCH_SIZE / WRITE_SIZE counter readings to mib MiB scales the
+    same counters read for the physics kernels."""
+    import torch
+    n = mib * 2**20 // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: 3 launches of an elementwise copy of {mib} MiB")
+    del src, dst
 
 
 def spec_subset(spec: bytes, n: int) -> bytes:

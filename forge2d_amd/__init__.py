@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(HERE, "libforge2d_amd.so")
 
 PH_PAIRS, PH_COLLIDE, PH_STATE, PH_SOLVE, PH_FINALIZE = 1, 2, 4, 8, 16
 PH_ALL = 31
+PH_CONTINUOUS, PH_NO_CONTINUOUS = 64, 128
 
 _lib = None
 
@@ -42,6 +43,7 @@ SIGNATURES = [
     ("f2dg_step_worlds", _I, [_VP, _PI, _I, _F, _I, _I]),
     ("f2dg_step_each", _I, [_VP, _F, _I, _I]),
     ("f2dg_step_timed", _I, [_VP, _F, _I, _I, _PF]),
+    ("f2dg_kernel_times", _I, [_VP, _F, _I, _I, _PF]),
     ("f2dg_synchronize", _I, [_VP]),
     ("f2dg_sync", _I, [_VP]),
     ("f2dg_get_bodies", _I, [_VP, _I, _PF]),
@@ -162,6 +164,12 @@ class Batch:
         ms = ctypes.c_float(0.0)
         self._check(lib().f2dg_step_timed(self.h, dt, substeps, steps, ctypes.byref(ms)), "step_timed")
         return ms.value
+
+    def kernel_times(self, dt=1.0 / 60.0, substeps=4, steps=1) -> dict:
+        """Step `steps` times; return summed HIP-event ms of each kernel."""
+        ms = (ctypes.c_float * 3)()
+        self._check(lib().f2dg_kernel_times(self.h, dt, substeps, steps, ms), "kernel_times")
+        return {"collideKernel": ms[0], "solveKernel": ms[1], "continuousKernel": ms[2]}
 
     def synchronize(self):
         self._check(lib().f2dg_synchronize(self.h), "synchronize")

@@ -571,9 +571,12 @@ static int upload(Batch& b) {
   }
   BatchView& v = b.dev.view;
   std::vector<WorldHeader> headers(b.worlds.size());
+  b.launch.continuous = 0;
   for (size_t i = 0; i < b.worlds.size(); ++i) {
     HostWorld& hw = b.worlds[i];
     WorldHeader& h = hw.hdr;
+    if (h.flags & WF_CONTINUOUS) b.launch.continuous = 1;
+    h.fastCount = 0;
     h.contactCount = hw.contactCount;
     h.freeCount = (int)hw.freeIds.size();
     h.overflowCount = (int)hw.overflow.size();
@@ -1252,6 +1255,40 @@ int f2dg_step_timed(void* h, float dt, int subSteps, int steps, float* msOut) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *msOut = ms;
+  b->deviceNewer = true;
+  return 0;
+}
+
+// Time `steps` batch steps per kernel: HIP events on the batch's stream bracket the
+// collide launch, the solve launch and the continuous-collision launch of every
+// step; ms[0..2] = summed milliseconds of each (ms[2] = 0 when no world of the
+// batch has continuous collision enabled).
+int f2dg_kernel_times(void* h, float dt, int subSteps, int steps, float* ms) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || steps <= 0) return -1;
+  if (prepareForStep(*b) != 0) return -1;
+  hipEvent_t ev[4];
+  for (auto& e : ev) F2D_HIP(hipEventCreate(&e));
+  StepParams sp[3] = {makeParams(dt, subSteps, PH_PAIRS | PH_COLLIDE | PH_STATE | PH_NO_COUNT),
+                      makeParams(dt, subSteps, PH_SOLVE | PH_FINALIZE | PH_NO_CONTINUOUS),
+                      makeParams(dt, subSteps, PH_CONTINUOUS)};
+  int W = (int)b->worlds.size();
+  float sum[3] = {0.0f, 0.0f, 0.0f};
+  for (int s = 0; s < steps; ++s) {
+    F2D_HIP(hipEventRecord(ev[0], b->stream));
+    for (int k = 0; k < 3; ++k) {
+      if (f2d_launch_step(&b->dev.view, &sp[k], nullptr, W, b->stream, &b->launch) != 0) return -1;
+      F2D_HIP(hipEventRecord(ev[k + 1], b->stream));
+    }
+    F2D_HIP(hipEventSynchronize(ev[3]));
+    for (int k = 0; k < 3; ++k) {
+      float t = 0.0f;
+      F2D_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+      sum[k] += t;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < 3; ++k) ms[k] = sum[k];
   b->deviceNewer = true;
   return 0;
 }

@@ -185,7 +185,7 @@ struct alignas(16) WorldHeader {
   float contactSpeed;            // world->contactSpeed, set each step
   float inv_h;                   // world->inv_h
   float jointHertz, jointDampingRatio;
-  int pad[1];
+  int fastCount;                 // fast bodies listed by finalize for continuous collision
 };
 
 // Per-step scalars, S/world.c:743-771
@@ -201,6 +201,9 @@ enum : int {
   PH_SOLVE = 8,
   PH_FINALIZE = 16,
   PH_ALL = 31,
+  PH_NO_COUNT = 32,        // collide launch of a split step: the solve launch counts the step
+  PH_CONTINUOUS = 64,      // launch the continuous-collision kernel on its own
+  PH_NO_CONTINUOUS = 128,  // finalize without the continuous-collision launch (timing splits)
 };
 
 // Launch shape of a batch: workgroup size and dynamic LDS bytes per kernel.
@@ -208,6 +211,7 @@ struct LaunchConfig {
   int block;       // 64, 128 or 256 lanes per world
   int collideLds;  // sort-and-sweep + pair-sort LDS
   int solveLds;    // body-state LDS
+  int continuous;  // some world of the batch has continuous collision enabled
 };
 
 // Contact constraint lanes (one constraint per lane), S/contact_solver.c:1034-1064.

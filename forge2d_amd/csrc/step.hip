@@ -23,6 +23,7 @@
 #include "manifold.hpp"
 #include "solver.hpp"
 #include "joint.hpp"
+#include "toi.hpp"
 
 namespace f2d {
 
@@ -37,6 +38,7 @@ struct SharedState {
   int jointColorFill[kGraphColorCount];
   int waveSums[kMaxWaves];
   int counter;
+  int fastCount;
   int changedCount;
   int error;
   int pairCount;
@@ -862,6 +864,9 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
 
   if (phases & PH_FINALIZE) {
     // b2FinalizeBodiesTask, S/solver.c:543-726
+    if (threadIdx.x == 0) sh.fastCount = 0;
+    __syncthreads();
+    int* fastList = w.scratch;  // fast bodies for the continuous-collision launch
     for (int i = threadIdx.x; i < nb; i += BLOCK) {
       BodySim& sim = w.sims[i];
       if (sim.type == kStatic) continue;
@@ -881,17 +886,15 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       sim.torque = 0.0f;
       int flags = sim.flags & ~(BF_IS_FAST | BF_ENLARGED);
       // isSpeedCapped is reported for this step and cleared for the next
-      bool capped = (sim.flags & BF_SPEED_CAPPED) != 0;
       flags &= ~BF_SPEED_CAPPED;
-      (void)capped;
       bool isFast = false;
       if (!ctx.enableSleep || (sim.flags & BF_ENABLE_SLEEP) == 0 || sleepVelocity > sim.sleepThreshold) {
         sim.sleepTime = 0.0f;
         if (sim.type == kDynamic && ctx.enableContinuous && maxVelocity * ctx.dt > 0.5f * sim.minExtent) {
-          // continuous collision is not run on the device; the body advances like a slow one
-          isFast = false;
-          sim.center0 = sim.center;
-          sim.rotation0 = sim.xf.q;
+          // the continuous-collision launch sweeps this body (b2SolveContinuous)
+          isFast = true;
+          int slot = atomicAdd(&sh.fastCount, 1);
+          fastList[slot] = i;
         } else {
           sim.center0 = sim.center;
           sim.rotation0 = sim.xf.q;
@@ -904,8 +907,19 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       Xf xf = sim.xf;
       int rank = 0;
       bool enlargedBody = false;
+      const bool bullet = (sim.flags & BF_BULLET) != 0;
       for (int s = sim.headShape; s >= 0 && rank < h->shapeCount; s = w.shapes[s].next, ++rank) {
         Shape& shape = w.shapes[s];
+        if (isFast) {
+          // fast shapes get their AABBs from the continuous sweep; a fast bullet
+          // joins the move buffer with every shape (S/solver.c:1866-1879)
+          if (bullet) {
+            shape.flags |= SF_MOVED;
+            shape.moveRank = (i << 8) | (rank < 255 ? rank : 255);
+            enlargedBody = true;
+          }
+          continue;
+        }
         AABB aabb = computeShapeAABB(w.geoms[s], xf);
         aabb.lo.x -= kSpeculativeDistance;
         aabb.lo.y -= kSpeculativeDistance;
@@ -931,6 +945,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       w.states[i] = st;
     }
     __syncthreads();
+    if (threadIdx.x == 0) h->fastCount = sh.fastCount;
   } else if (kLds && (phases & PH_SOLVE)) {
     for (int i = threadIdx.x; i < nb; i += BLOCK) w.states[i] = ldsStates[i];
     __syncthreads();
@@ -987,7 +1002,7 @@ __global__ __launch_bounds__(BLOCK) void collideKernel(BatchView b, StepParams s
   if (sp.phases & PH_PAIRS) pairsPhase<BLOCK>(w, sh, lds, ldsShapes, ldsPairs);
   if (sp.phases & PH_COLLIDE) collidePhase<BLOCK>(w, sh);
   if (sp.phases & PH_STATE) statePhase(w, sh);
-  if (threadIdx.x == 0 && (sp.phases & (PH_SOLVE | PH_FINALIZE)) == 0) h->stepIndex += 1;
+  if (threadIdx.x == 0 && (sp.phases & (PH_SOLVE | PH_FINALIZE | PH_NO_COUNT)) == 0) h->stepIndex += 1;
 }
 
 // Solver stages and finalize of one world per workgroup; body states live in
@@ -1009,6 +1024,187 @@ __global__ __launch_bounds__(BLOCK) void solveKernel(BatchView b, StepParams sp,
   if (threadIdx.x == 0) h->stepIndex += 1;
 }
 
+// ---------------------------------------------------------------------------- phase: continuous
+
+// b2GetShapeCentroid, S/shape.c:601-618
+__device__ Vec2 shapeCentroid(const ShapeGeom& g) {
+  if (g.type == kCircle) return g.v[0];
+  if (g.type == kPolygon) return g.centroid;
+  return lerp(g.v[0], g.v[1], 0.5f);
+}
+
+// minExtent of b2ComputeShapeExtent, S/shape.c:722-790
+__device__ float shapeMinExtent(const ShapeGeom& g) {
+  if (g.type == kCircle || g.type == kCapsule) return g.radius;
+  if (g.type == kSegment) return 0.0f;
+  float mn = kHuge;
+  for (int i = 0; i < g.count && i < kMaxPolygonVertices; ++i) mn = minf(mn, dot(g.n[i], sub(g.v[i], g.centroid)));
+  return mn + g.radius;
+}
+
+// b2ContinuousQueryCallback, S/solver.c:212-388, for candidate shape t of a fast
+// shape s on body i. Returns the (possibly reduced) sweep fraction.
+__device__ float continuousCandidate(const WorldPtrs& w, int i, int s, const Proxy& proxyB, const Sweep& sweep,
+                                     int t, float fraction) {
+  if (t == s) return fraction;
+  const Shape& fs = w.shapes[s];
+  const Shape& ts = w.shapes[t];
+  if (ts.body == i) return fraction;
+  if (ts.flags & SF_SENSOR) return fraction;
+  if (!shouldShapesCollide(fs, ts)) return fraction;
+  const BodySim& other = w.sims[ts.body];
+  if (other.flags & BF_BULLET) return fraction;
+  if (!shouldBodiesCollide(w.sims, w.joints, i, ts.body)) return fraction;
+  Proxy proxyA = makeShapeProxy(w.geoms[t]);
+  Sweep sweepA = makeSweep(other);
+  float f = timeOfImpact(proxyA, sweepA, proxyB, sweep, fraction);
+  if (0.0f < f && f < fraction) return f;
+  if (f == 0.0f) {
+    // fallback: a small circle around the fast shape's centroid
+    const ShapeGeom& g = w.geoms[s];
+    Proxy core;
+    core.count = 1;
+    core.p[0] = shapeCentroid(g);
+    core.radius = 0.25f * shapeMinExtent(g);
+    f = timeOfImpact(proxyA, sweepA, core, sweep, fraction);
+    if (0.0f < f && f < fraction) return f;
+  }
+  return fraction;
+}
+
+__device__ __forceinline__ bool queryVisible(const Shape& t, AABB box) {
+  // b2DynamicTree_Query with B2_DEFAULT_MASK_BITS against the proxy's fat AABB
+  return (t.catLo | t.catHi) != 0 && t.proxyKey >= 0 && aabbOverlaps(t.fat, box);
+}
+
+// b2SolveContinuous, S/solver.c:390-541, for fast body i. Non-bullets sweep against
+// the static proxies (listed in shape order); bullets also against kinematic and
+// dynamic proxies, after every non-bullet body has its final transform.
+__device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList, int staticCount, bool bullet) {
+  BodySim& sim = w.sims[i];
+  const int ns = w.hdr->shapeCount;
+  Sweep sweep = makeSweep(sim);
+  Xf xf2 = {sub(sweep.c2, rotate(sweep.q2, sweep.localCenter)), sweep.q2};
+  float fraction = 1.0f;
+  int rank = 0;
+  for (int s = sim.headShape; s >= 0 && rank < ns; s = w.shapes[s].next, ++rank) {
+    Shape& fs = w.shapes[s];
+    AABB box1 = fs.aabb;
+    AABB box2 = computeShapeAABB(w.geoms[s], xf2);
+    AABB box;
+    box.lo = vmin(box1.lo, box2.lo);
+    box.hi = vmax(box1.hi, box2.hi);
+    fs.aabb = box2;
+    if (fs.flags & SF_SENSOR) continue;
+    Proxy proxyB = makeShapeProxy(w.geoms[s]);
+    for (int k = 0; k < staticCount; ++k) {
+      int t = staticList[k];
+      if (queryVisible(w.shapes[t], box)) fraction = continuousCandidate(w, i, s, proxyB, sweep, t, fraction);
+    }
+    if (bullet) {
+      for (int type = kKinematic; type <= kDynamic; ++type) {
+        for (int t = 0; t < ns; ++t) {
+          const Shape& ts = w.shapes[t];
+          if ((ts.proxyKey & 3) != type || !queryVisible(ts, box)) continue;
+          fraction = continuousCandidate(w, i, s, proxyB, sweep, t, fraction);
+        }
+      }
+    }
+  }
+  int flags = sim.flags;
+  rank = 0;
+  if (fraction < 1.0f) {
+    // time of impact: rewind the body to the hit fraction
+    Rot q = nlerpRot(sweep.q1, sweep.q2, fraction);
+    Vec2 c = lerp(sweep.c1, sweep.c2, fraction);
+    Vec2 origin = sub(c, rotate(q, sweep.localCenter));
+    Xf xf = {origin, q};
+    sim.xf = xf;
+    sim.center = c;
+    sim.rotation0 = q;
+    sim.center0 = c;
+    for (int s = sim.headShape; s >= 0 && rank < ns; s = w.shapes[s].next, ++rank) {
+      Shape& shape = w.shapes[s];
+      AABB aabb = computeShapeAABB(w.geoms[s], xf);
+      aabb.lo.x -= kSpeculativeDistance;
+      aabb.lo.y -= kSpeculativeDistance;
+      aabb.hi.x += kSpeculativeDistance;
+      aabb.hi.y += kSpeculativeDistance;
+      shape.aabb = aabb;
+      if (!aabbContains(shape.fat, aabb)) {
+        AABB fat;
+        fat.lo.x = aabb.lo.x - kAabbMargin;
+        fat.lo.y = aabb.lo.y - kAabbMargin;
+        fat.hi.x = aabb.hi.x + kAabbMargin;
+        fat.hi.y = aabb.hi.y + kAabbMargin;
+        shape.fat = fat;
+        shape.flags |= SF_MOVED;
+        shape.moveRank = (i << 8) | (rank < 255 ? rank : 255);
+        flags |= BF_ENLARGED;
+      }
+    }
+  } else {
+    sim.rotation0 = sim.xf.q;
+    sim.center0 = sim.center;
+    for (int s = sim.headShape; s >= 0 && rank < ns; s = w.shapes[s].next, ++rank) {
+      Shape& shape = w.shapes[s];
+      // the shape AABB is the end-of-sweep box computed above (no speculative margin)
+      if (!aabbContains(shape.fat, shape.aabb)) {
+        AABB fat;
+        fat.lo.x = shape.aabb.lo.x - kAabbMargin;
+        fat.lo.y = shape.aabb.lo.y - kAabbMargin;
+        fat.hi.x = shape.aabb.hi.x + kAabbMargin;
+        fat.hi.y = shape.aabb.hi.y + kAabbMargin;
+        shape.fat = fat;
+        shape.flags |= SF_MOVED;
+        shape.moveRank = (i << 8) | (rank < 255 ? rank : 255);
+        flags |= BF_ENLARGED;
+      }
+    }
+  }
+  sim.flags = flags;
+}
+
+// Continuous collision of the fast bodies finalize listed, one world per
+// workgroup, one fast body per lane. Worlds without fast bodies return at once.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void continuousKernel(BatchView b, const int* worldList) {
+  __shared__ int waveSums[kMaxWaves];
+  __shared__ int staticTotal;
+  const int wi = worldList ? worldList[blockIdx.x] : blockIdx.x;
+  WorldHeader* h = b.worlds + wi;
+  const int nFast = h->fastCount;
+  if (nFast == 0) return;
+  WorldPtrs w = worldPtrs(b, h);
+  const int ns = h->shapeCount;
+  const int* fastList = w.scratch;   // written by finalize, nFast <= shapeCount
+  int* staticList = w.scratch + ns;  // static proxies in shape order
+  if (threadIdx.x == 0) staticTotal = 0;
+  __syncthreads();
+  for (int base = 0; base < ns; base += BLOCK) {
+    int t = base + threadIdx.x;
+    bool isStatic = t < ns && w.shapes[t].proxyKey >= 0 && (w.shapes[t].proxyKey & 3) == kStatic;
+    int total = 0;
+    int slot = blockCompact<BLOCK>(isStatic, waveSums, &total);
+    if (isStatic) staticList[staticTotal + slot] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) staticTotal += total;
+    __syncthreads();
+  }
+  const int staticCount = staticTotal;
+  for (int k = threadIdx.x; k < nFast; k += BLOCK) {
+    int i = fastList[k];
+    if ((w.sims[i].flags & BF_BULLET) == 0) solveContinuous(w, i, staticList, staticCount, false);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nFast; k += BLOCK) {
+    int i = fastList[k];
+    if (w.sims[i].flags & BF_BULLET) solveContinuous(w, i, staticList, staticCount, true);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) h->fastCount = 0;
+}
+
 template <int BLOCK>
 static int launchBlock(const BatchView* b, const StepParams* sp, const int* worldList, int nWorlds, hipStream_t stream,
                        const LaunchConfig* cfg) {
@@ -1019,6 +1215,10 @@ static int launchBlock(const BatchView* b, const StepParams* sp, const int* worl
   if (sp->phases & (PH_SOLVE | PH_FINALIZE)) {
     hipLaunchKernelGGL(solveKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->solveLds, stream, *b, *sp, worldList,
                        cfg->solveLds);
+  }
+  bool ccd = (sp->phases & PH_CONTINUOUS) || ((sp->phases & PH_FINALIZE) && !(sp->phases & PH_NO_CONTINUOUS));
+  if (ccd && cfg->continuous) {
+    hipLaunchKernelGGL(continuousKernel<64>, dim3(nWorlds), dim3(64), 0, stream, *b, worldList);
   }
   return (int)hipGetLastError();
 }

@@ -64,7 +64,7 @@ struct Batch {
   int dWorldListCap = 0;
   std::string lastError;
   float contactCapFactor = 6.0f;
-  LaunchConfig launch = {0, 0, 0};
+  LaunchConfig launch = {0, 0, 0, 0};
 };
 
 }  // namespace f2d

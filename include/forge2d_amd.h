@@ -157,6 +157,9 @@ int f2dg_step_worlds(void* batch, const int32_t* world_ids, int n, float dt, int
 int f2dg_step_each(void* batch, float dt, int sub_steps, int steps);
 /* step and time with HIP events on the batch stream; *ms_out = total milliseconds */
 int f2dg_step_timed(void* batch, float dt, int sub_steps, int steps, float* ms_out);
+/* step and time each kernel: HIP events bracket the collide, solve and
+ * continuous-collision launches of every step; ms[0..2] = summed milliseconds */
+int f2dg_kernel_times(void* batch, float dt, int sub_steps, int steps, float* ms);
 int f2dg_synchronize(void* batch);
 /* copy device state to the host mirror; returns OR of world error bits, <0 on failure */
 int f2dg_sync(void* batch);

@@ -65,6 +65,49 @@ def bounce_world():
     return w
 
 
+def ccd_world():
+    """Fast bodies of every shape type fired at thin static segments and a static
+    box (b2SolveContinuous, S/solver.c:390-541): without the time-of-impact sweep
+    they would pass through in one step. Two bullets cross a dynamic stack, which
+    only bullets sweep against."""
+    rng = scenes.SplitMix64(4242)
+    w = scenes.World(scenes.world_header(sleep=False, continuous=True))
+    g = w.add_body(scenes.STATIC, 0.0, 0.0)
+    w.add_segment(g, (-20.0, 0.0), (20.0, 0.0))
+    w.add_segment(g, (-20.0, 0.0), (-20.0, 30.0))
+    w.add_segment(g, (20.0, 0.0), (20.0, 30.0))
+    w.add_box(g, 2.0, 0.1, cx=8.0, cy=6.0, angle=0.3)
+    for i in range(30):
+        x = -16.0 + 1.1 * i
+        y = 2.0 + rng.uniform(0.0, 6.0)
+        vx = rng.uniform(-20.0, 20.0) if i % 5 == 0 else 0.0
+        b = w.add_body(scenes.DYNAMIC, x, y, angle=rng.uniform(-3.0, 3.0), vx=vx, vy=-rng.uniform(40.0, 120.0),
+                       w=rng.uniform(-20.0, 20.0))
+        k = i % 4
+        if k == 0:
+            w.add_circle(b, rng.uniform(0.1, 0.3))
+        elif k == 1:
+            w.add_capsule(b, (-0.3, 0.0), (0.3, 0.0), rng.uniform(0.08, 0.15))
+        elif k == 2:
+            w.add_box(b, rng.uniform(0.1, 0.3), rng.uniform(0.1, 0.3))
+        else:
+            n = 3 + i % 6
+            pts = []
+            for j in range(n):
+                a = 2.0 * 3.141592653589793 * j / n + rng.uniform(-0.2, 0.2)
+                r = rng.uniform(0.15, 0.35)
+                pts.append((r * scenes.math.cos(a), r * scenes.math.sin(a)))
+            w.add_polygon(b, pts)
+    # a dynamic stack and two bullets crossing it
+    for r in range(6):
+        b = w.add_body(scenes.DYNAMIC, -4.0, 0.5 + 1.0 * r)
+        w.add_box(b, 0.5, 0.5)
+    for j, (y, vx) in enumerate(((1.5, 150.0), (3.5, -180.0))):
+        b = w.add_body(scenes.DYNAMIC, -12.0 if vx > 0 else 4.0, y, vx=vx, flags=scenes.BF_BULLET | scenes.BF_AWAKE)
+        w.add_circle(b, 0.1, density=4.0)
+    return w
+
+
 FIXTURES = {
     # name: (world factory, dumped steps)
     "pyramid12": (lambda: scenes.pyramid_world(12), [0, 1, 2, 3, 4, 5, 60, 61, 62]),
@@ -74,6 +117,10 @@ FIXTURES = {
               [0, 1, 2, 3, 4, 5, 6, 30, 31, 32, 60, 61, 62, 120, 121, 122]),
     "overflow": (overflow_world, [0, 1, 2, 3, 4, 20, 21, 22]),
     "bounce": (bounce_world, [0, 40, 41, 42, 45, 46, 47, 50, 51, 52, 60, 61, 62, 80, 81, 82]),
+    "ccd": (ccd_world, list(range(0, 13)) + [30, 31, 32]),
+    "mixed_ccd": (lambda: scenes.container_world(200, 15.0, 40.0, 20261202, continuous=True, chains=3,
+                                                 drop_band=(0.2, 1.0)),
+                  [0, 1, 2, 3, 40, 41, 42, 43, 44, 45, 46, 47, 48, 49, 50, 60, 61, 62]),
 }
 
 

@@ -12,6 +12,8 @@ For every pair of consecutive dumped steps (k-1, k) of every fixture:
     manifolds, impulses matched from k-1, bodies from k-1) stepped through the
     solver stages and finalize reproduces bodies, stored impulses, fat AABBs and
     the next move buffer of step k, within 1e-5 (in practice bit-exact).
+The ccd fixtures fire fast bodies and bullets at thin static geometry, so these
+cases also cover the time-of-impact sweeps of continuous collision.
 """
 import numpy as np
 import pytest
@@ -80,6 +82,7 @@ def test_solver_and_finalize_from_post_collide_state(gpu, name, i):
     assert max(errs.values()) <= TOL, errs
     rep = P.manifold_report(mine, ref, value_fields=(), with_impulses=True)
     assert rep["impulses"] == 0, rep
+    assert np.array_equal(mine.shapes["aabb"], ref.shapes["aabb"])
     assert np.array_equal(mine.shapes["fat"], ref.shapes["fat"])
     assert np.array_equal(mine.move, ref.move)
     for f in ("linear_impulse_x", "linear_impulse_y", "lower_impulse", "upper_impulse"):
