@@ -1,0 +1,80 @@
+/*
+ * Oracle: CPU restatement of the Box2D v3.1.1 world step (test infrastructure
+ * only; see b2oracle.c). Built twice by oracle/Makefile: liboracle_f32.so with
+ * real = float and liboracle_f64.so with real = double (ORACLE_F64).
+ */
+#ifndef B2ORACLE_H
+#define B2ORACLE_H
+
+#include <stdint.h>
+
+#ifdef ORACLE_F64
+typedef double real;
+#else
+typedef float real;
+#endif
+
+typedef struct { real x, y; } V2;
+typedef struct { real c, s; } R2;
+typedef struct { V2 p; R2 q; } XF;
+typedef struct { V2 lo, hi; } AB;
+
+enum { SHAPE_CIRCLE = 0, SHAPE_CAPSULE = 1, SHAPE_SEGMENT = 2, SHAPE_POLYGON = 3 };
+
+/* shape geometry in body-local coordinates (b2Circle / b2Capsule / b2Segment / b2Polygon) */
+typedef struct {
+  int type, count;
+  real radius;
+  V2 centroid;
+  V2 v[8], n[8];
+} GEO;
+
+/* b2ManifoldPoint / b2Manifold, I/collision.h:498-551 */
+typedef struct {
+  V2 point, anchorA, anchorB;
+  real separation, normalImpulse, tangentImpulse, totalNormalImpulse, normalVelocity;
+  int id, persisted;
+} MP;
+typedef struct {
+  V2 normal;
+  real rollingImpulse;
+  MP points[2];
+  int pointCount;
+} MF;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Manifold of two shapes in primary order (S/contact.c:151-184). */
+void b2o_manifold(const GEO* A, XF xfA, const GEO* B, XF xfB, MF* m);
+int b2o_has_manifold(int typeA, int typeB);
+int b2o_is_primary(int typeA, int typeB);
+
+/* World built from a scene spec (forge2d_amd/scenes.py layout, world `world`),
+ * the reference's shape geometry (148-byte records) and a world snapshot
+ * (forge2d_amd/snapshot.py layout). NULL on a malformed input. */
+void* b2o_create(const void* spec, long specBytes, int world, const void* geoms, long geomBytes, const void* snap,
+                 long snapBytes);
+void b2o_destroy(void* w);
+/* Contacts the broad phase creates at the start of the next step, in creation
+ * order: ids must be the id pool's LIFO allocations (checked; -1 on mismatch). */
+int b2o_create_contacts(void* w, const int32_t* ids, const int32_t* shapeA, const int32_t* shapeB, int n);
+/* b2Collide without the pair update: manifolds and the contact state loop. */
+int b2o_collide(void* w);
+/* b2Solve: solver stages, finalize, continuous collision, enlarged proxies. */
+int b2o_solve(void* w, float dt, int subSteps);
+/* Snapshot of the world (float32 records); returns the size, writes when cap suffices. */
+long b2o_export(void* w, void* out, long cap);
+/* per body: px, py, qc, qs, vx, vy, w in double */
+int b2o_bodies(void* w, double* out);
+/* Broad-phase overlap set by brute force: keys (shapeA << 32 | shapeB) in contact orientation,
+ * of every proxy pair whose fat AABBs overlap and that the reference's pair rules
+ * accept (S/broad_phase.c:160-302). Returns the count, writes up to cap. */
+long b2o_pairs(void* w, int64_t* keys, long cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -18,6 +18,7 @@
 //   ref_harness dump  <scene> <out> <world> <steps> <k1,k2,...>
 //   ref_harness time  <scene> <steps> <warmup> <threads> [first_world count]
 //   ref_harness pairs <scene> <out> <world> <steps>   (per-step created pair lists)
+//   ref_harness geom  <scene> <out> <world>           (shape geometry as built by the reference)
 
 #include "box2d/box2d.h"
 #include "body.h"
@@ -582,6 +583,71 @@ int main( int argc, char** argv )
 				write_snapshot( f, wid, s );
 				di++;
 			}
+		}
+		fclose( f );
+		b2DestroyWorld( wid );
+		return 0;
+	}
+	if ( strcmp( argv[1], "geom" ) == 0 && argc >= 5 )
+	{
+		// per shape, 148 bytes: int32 type, count; float radius, centroid x/y,
+		// 8 vertices, 8 normals (circle: v0 = center; capsule/segment: v0, v1)
+		Scene sc = load_scene( argv[2] );
+		int wi = atoi( argv[4] );
+		const SceneWorld* sw = sc.worlds + wi;
+		b2BodyId* ids = malloc( sizeof( b2BodyId ) * ( sw->hdr.body_count + 1 ) );
+		b2WorldId wid = build_world( sw, ids );
+		b2World* world = b2GetWorldFromId( wid );
+		FILE* f = fopen( argv[3], "wb" );
+		for ( int i = 0; i < world->shapes.count; ++i )
+		{
+			b2Shape* shape = world->shapes.data + i;
+			int32_t head[2] = { (int32_t)shape->type, 0 };
+			float rec[35];
+			memset( rec, 0, sizeof( rec ) );
+			switch ( shape->type )
+			{
+				case b2_circleShape:
+					head[1] = 1;
+					rec[0] = shape->circle.radius;
+					rec[1] = shape->circle.center.x;
+					rec[2] = shape->circle.center.y;
+					rec[3] = shape->circle.center.x;
+					rec[4] = shape->circle.center.y;
+					break;
+				case b2_capsuleShape:
+					head[1] = 2;
+					rec[0] = shape->capsule.radius;
+					rec[3] = shape->capsule.center1.x;
+					rec[4] = shape->capsule.center1.y;
+					rec[5] = shape->capsule.center2.x;
+					rec[6] = shape->capsule.center2.y;
+					break;
+				case b2_segmentShape:
+					head[1] = 2;
+					rec[3] = shape->segment.point1.x;
+					rec[4] = shape->segment.point1.y;
+					rec[5] = shape->segment.point2.x;
+					rec[6] = shape->segment.point2.y;
+					break;
+				case b2_polygonShape:
+					head[1] = shape->polygon.count;
+					rec[0] = shape->polygon.radius;
+					rec[1] = shape->polygon.centroid.x;
+					rec[2] = shape->polygon.centroid.y;
+					for ( int k = 0; k < shape->polygon.count; ++k )
+					{
+						rec[3 + 2 * k] = shape->polygon.vertices[k].x;
+						rec[4 + 2 * k] = shape->polygon.vertices[k].y;
+						rec[19 + 2 * k] = shape->polygon.normals[k].x;
+						rec[20 + 2 * k] = shape->polygon.normals[k].y;
+					}
+					break;
+				default:
+					break;
+			}
+			fwrite( head, sizeof( head ), 1, f );
+			fwrite( rec, sizeof( rec ), 1, f );
 		}
 		fclose( f );
 		b2DestroyWorld( wid );

@@ -8,6 +8,8 @@ after the listed steps, compressed, to ``tests/golden/<name>.npz``:
     spec   uint8   the scene spec (forge2d_amd/scenes.py layout)
     snaps  uint8   concatenated snapshots (forge2d_amd/snapshot.py layout)
     steps  int32   the step index of each snapshot
+    geoms  uint8   per shape, the reference's built geometry (148 B: int32 type,
+                   count; float radius, centroid, 8 vertices, 8 normals)
 
 Usage: python tests/golden/make_golden.py [name ...]
 """
@@ -136,9 +138,13 @@ def make(name: str):
         subprocess.run([HARNESS, "dump", sp, out, "0", str(max(steps)), ",".join(map(str, steps))], check=True)
         with open(out, "rb") as f:
             snaps = f.read()
+        gp = os.path.join(td, "geoms.bin")
+        subprocess.run([HARNESS, "geom", sp, gp, "0"], check=True)
+        with open(gp, "rb") as f:
+            geoms = f.read()
     path = os.path.join(HERE, name + ".npz")
     np.savez_compressed(path, spec=np.frombuffer(spec, np.uint8), snaps=np.frombuffer(snaps, np.uint8),
-                        steps=np.array(steps, np.int32))
+                        steps=np.array(steps, np.int32), geoms=np.frombuffer(geoms, np.uint8))
     print(f"{name}: {len(spec)} B spec, {len(snaps)} B snapshots -> {os.path.getsize(path)} B")
 
 

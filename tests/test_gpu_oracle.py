@@ -1,0 +1,82 @@
+"""GPU parity against the C oracle on seeded scenes larger than the fixtures.
+
+Each scene is stepped on the device; after every step the oracle restarts from
+the device's own previous state, creates the contacts the device's broad phase
+created (their ids must be the LIFO id pool's allocations, which the oracle
+checks) and computes the step itself. The device's manifolds, colours, impulses,
+bodies, fat AABBs and move array must equal the oracle's bit for bit, except for
+the fast bodies of a continuous sweep that several candidates hit (visited in a
+different order; compared within 1e-5) — see tests/test_oracle.py.
+
+The oracle needs the reference's own shape geometry (hulls, rounded boxes); it is
+written by the reference build's fixture harness (oracle/_ref/ref_harness geom).
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import forge2d_amd as F
+from forge2d_amd import scenes
+from forge2d_amd import snapshot as S
+from oracle import pyoracle as O
+from tests import parity_util as P
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+TOL = 1e-5
+
+SCENES = {
+    "pyramid25": lambda: scenes.pyramid_world(25),
+    "container400_ccd": lambda: scenes.container_world(400, 18.0, 40.0, 7001, continuous=True, chains=4,
+                                                       drop_band=(0.3, 1.0)),
+    "container300_nocd": lambda: scenes.container_world(300, 12.0, 30.0, 7002, continuous=False, chains=2,
+                                                        drop_band=(0.1, 0.8)),
+}
+
+
+def reference_geometry(spec: bytes) -> bytes:
+    with tempfile.TemporaryDirectory() as td:
+        sp, gp = os.path.join(td, "s.bin"), os.path.join(td, "g.bin")
+        with open(sp, "wb") as f:
+            f.write(spec)
+        subprocess.run([HARNESS, "geom", sp, gp, "0"], check=True, timeout=60)
+        with open(gp, "rb") as f:
+            return f.read()
+
+
+def compare(mine, o):
+    assert np.array_equal(mine.pair_keys(), o.pair_keys())
+    assert P.color_map(mine) == P.color_map(o)
+    fast = (o.bodies["flags"] & 1) != 0
+    for f in P.BODY_FIELDS + ["c0x", "c0y", "q0c", "q0s"]:
+        assert np.array_equal(mine.bodies[f][~fast], o.bodies[f][~fast]), f
+        assert P.rel_err(mine.bodies[f][fast], o.bodies[f][fast]) <= TOL, f
+    rep = P.manifold_report(mine, o, with_impulses=not fast.any())
+    assert rep["point_count"] == rep["ids"] == rep["normals"] == rep["values"] == rep["impulses"] == 0, rep
+    fast_shapes = fast[o.shapes["body"]]
+    assert np.array_equal(mine.shapes["fat"][~fast_shapes], o.shapes["fat"][~fast_shapes])
+    if not fast.any():
+        assert np.array_equal(mine.move, o.move)
+
+
+@pytest.mark.skipif(not (O.available("f32") and os.path.exists(HARNESS)), reason="oracle or reference build missing")
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_device_matches_oracle_every_step(gpu, name):
+    spec = scenes.to_bytes([SCENES[name]()])
+    geoms = reference_geometry(spec)
+    b = F.Batch(spec)
+    prev_raw = b.export(0)
+    prev = S.parse_one(prev_raw)
+    for k in range(1, 41):
+        print(f"[oracle-parity] {name} step {k}", flush=True)
+        b.step(steps=1)
+        assert b.sync() == 0
+        raw = b.export(0)
+        mine = S.parse_one(raw)
+        o = S.parse_one(O.step(spec, geoms, prev_raw, prev, mine).export())
+        compare(mine, o)
+        prev_raw, prev = raw, mine

@@ -1,0 +1,129 @@
+"""Pins of the CPU oracle (oracle/b2oracle.c) against the reference's own step.
+
+The oracle steps every fixture's snapshot k-1 to k; the contacts the reference's
+broad phase created in step k (their ids and shapes, read from snapshot k) are
+handed to it, because the dynamic tree's traversal order that assigns those ids
+is not restated. Everything else it computes: manifolds and feature ids,
+warm-start matching, the contact state loop and persistent colouring, the
+solver stages in the SSE2 lane order for coloured contacts and the scalar order
+for the overflow colour, revolute joints, finalize, continuous collision and the
+next move array. The float build must reproduce snapshot k bit for bit.
+
+The one order the oracle does not reproduce: when several candidates of one
+continuous sweep hit, the reference visits them in its trees' traversal order
+and the oracle in shape order; the earlier hit's fraction bounds the later root
+search, so the final fraction can differ in the last bits (within the
+time-of-impact tolerance, 0.25 * linear slop). Fast bodies are therefore
+compared within 1e-5 relative, every other quantity exactly.
+
+The double build states the deviation of the float reference from FP64 over one
+step from the same state.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from forge2d_amd import snapshot as S
+from oracle import pyoracle as O
+from tests import parity_util as P
+from tests.conftest import FIXTURE_NAMES, GOLDEN, load_fixture
+
+pytestmark = pytest.mark.skipif(not O.available("f32"), reason="oracle not built (make -C oracle port)")
+
+TOL = 1e-5
+
+
+def geoms(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"))["geoms"].tobytes()
+
+
+def step_pairs(name):
+    _, snaps, _ = load_fixture(name)
+    return [(name, i) for i in range(1, len(snaps)) if snaps[i].step == snaps[i - 1].step + 1]
+
+
+CASES = [c for n in FIXTURE_NAMES for c in step_pairs(n)]
+IDS = [f"{n}-{load_fixture(n)[1][i].step}" for n, i in CASES]
+
+
+@pytest.mark.parametrize("name,i", CASES, ids=IDS)
+def test_oracle_step_reproduces_reference(name, i):
+    spec, snaps, raws = load_fixture(name)
+    prev, ref = snaps[i - 1], snaps[i]
+    o = O.step(spec, geoms(name), raws[i - 1], prev, ref)
+    mine = S.parse_one(o.export())
+    assert np.array_equal(mine.pair_keys(), ref.pair_keys())
+    assert P.color_map(mine) == P.color_map(ref)
+    assert all(np.array_equal(a, b) for a, b in zip(mine.colors, ref.colors)), "colour array order"
+    assert np.array_equal(mine.free_ids, ref.free_ids)
+    fast = (ref.bodies["flags"] & 1) != 0
+    for f in P.BODY_FIELDS + ["c0x", "c0y", "q0c", "q0s"]:
+        assert np.array_equal(mine.bodies[f][~fast], ref.bodies[f][~fast]), f
+        assert P.rel_err(mine.bodies[f][fast], ref.bodies[f][fast]) <= TOL, f
+    fast_shapes = fast[ref.shapes["body"]]
+    for f in ("aabb", "fat"):
+        assert np.array_equal(mine.shapes[f][~fast_shapes], ref.shapes[f][~fast_shapes]), f
+        assert np.abs(mine.shapes[f][fast_shapes] - ref.shapes[f][fast_shapes]).max(initial=0.0) <= TOL, f
+    rep = P.manifold_report(mine, ref, with_impulses=True)
+    assert rep["point_count"] == rep["ids"] == rep["normals"] == rep["values"] == 0, rep
+    if not fast.any():
+        assert rep["impulses"] == 0, rep
+        assert np.array_equal(mine.move, ref.move)
+        for f in ("linear_impulse_x", "linear_impulse_y", "spring_impulse", "motor_impulse", "lower_impulse",
+                  "upper_impulse"):
+            assert np.array_equal(mine.joints[f], ref.joints[f]), f
+
+
+@pytest.mark.parametrize("name", FIXTURE_NAMES)
+def test_oracle_pair_set_is_next_contact_set(name):
+    """The brute-force overlap set of snapshot k-1's fat AABBs under the
+    reference's pair rules equals the contact set the reference holds after k."""
+    spec, snaps, raws = load_fixture(name)
+    for i in range(1, len(snaps)):
+        if snaps[i].step != snaps[i - 1].step + 1:
+            continue
+        o = O.Oracle(spec, geoms(name), raws[i - 1])
+        assert np.array_equal(o.pairs(), snaps[i].pair_keys()), snaps[i].step
+
+
+def test_oracle_id_pool_is_checked():
+    """Handing the oracle created ids that are not the LIFO pool's allocations
+    is refused: the id order is a property of the pool, not an input."""
+    spec, snaps, raws = load_fixture("pyramid12")
+    o = O.Oracle(spec, geoms("pyramid12"), raws[0])
+    ids, a, b = O.created_contacts(snaps[0], snaps[1])
+    assert len(ids) > 1
+    with pytest.raises(RuntimeError):
+        o.create_contacts((ids[::-1], a[::-1], b[::-1]))
+
+
+FP64_BOUNDS = {"px": 1e-4, "py": 1e-4, "qc": 1e-3, "qs": 1e-3, "vx": 1e-2, "vy": 1e-2, "w": 1e-2}
+
+
+def fp64_deviation(name):
+    """Per field, the largest deviation (relative, floor 1) of the float
+    reference's step k from the FP64 oracle's step k-1 -> k, over the fixture."""
+    spec, snaps, raws = load_fixture(name)
+    worst = dict.fromkeys(FP64_BOUNDS, 0.0)
+    for _, i in step_pairs(name):
+        o = O.step(spec, geoms(name), raws[i - 1], snaps[i - 1], snaps[i], precision="f64")
+        b64 = o.bodies()
+        for k, f in enumerate(FP64_BOUNDS):
+            ref = snaps[i].bodies[f].astype(np.float64)
+            err = np.abs(ref - b64[:, k]) / np.maximum(np.abs(b64[:, k]), 1.0)
+            worst[f] = max(worst[f], float(err.max(initial=0.0)))
+    return worst
+
+
+@pytest.mark.skipif(not O.available("f64"), reason="FP64 oracle not built")
+@pytest.mark.parametrize("name", FIXTURE_NAMES)
+def test_fp32_deviation_from_fp64_is_bounded(name):
+    """One step from the same reference state in FP64. Stacked contacts amplify
+    FP32 rounding in the impulses, so resting pyramids show velocity deviations
+    around 1e-3 while positions stay near 1e-5; the bounds are an order of
+    magnitude above the largest measured figure per field."""
+    worst = fp64_deviation(name)
+    print(name, {f: f"{v:.1e}" for f, v in worst.items()})
+    for f, bound in FP64_BOUNDS.items():
+        assert worst[f] <= bound, (f, worst[f])
