@@ -149,6 +149,13 @@ class Batch:
     def upload(self):
         self._check(lib().f2dg_upload(self.h), "upload")
 
+    def set_stream(self, stream_handle: int):
+        """Launch on this hipStream_t (an integer handle, e.g. torch.cuda.Stream().cuda_stream)."""
+        self._check(lib().f2dg_set_stream(self.h, ctypes.c_void_p(stream_handle)), "set_stream")
+
+    def body_count(self, world: int) -> int:
+        return lib().f2dg_body_count(self.h, world)
+
     def step(self, dt=1.0 / 60.0, substeps=4, steps=1, phases=PH_ALL):
         self._check(lib().f2dg_step(self.h, dt, substeps, steps, phases), "step")
 

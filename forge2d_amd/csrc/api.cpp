@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "manifold.hpp"
+#include "solver.hpp"
 #include "world_store.hpp"
 
 extern "C" int f2d_launch_step(const f2d::BatchView* b, const f2d::StepParams* sp, const int* worldList, int nWorlds,
@@ -271,6 +272,11 @@ int HostWorld::createRevolute(int bodyA, int bodyB, Vec2 la, Vec2 lb, float refA
       }
     }
     j.color = color;
+    // appended to the colour's joint array (b2AddJointToGraph, S/constraint_graph.c:278-296)
+    int n = 0;
+    for (const Joint& o : joints)
+      if (o.color == color) ++n;
+    j.localIndex = n;
   }
   joints.push_back(j);
   return id;
@@ -473,6 +479,8 @@ static void freeDevice(Batch& b) {
   if (b.dWorldList) (void)hipFree(b.dWorldList);
   b.dWorldList = nullptr;
   b.dWorldListCap = 0;
+  if (b.dOrder) (void)hipFree(b.dOrder);
+  b.dOrder = nullptr;
   b.committed = false;
 }
 
@@ -538,7 +546,7 @@ static int commit(Batch& b) {
       devAlloc(d, &v.contacts, nc) || devAlloc(d, &v.freeIds, nc) || devAlloc(d, &v.overflow, nc) ||
       devAlloc(d, &v.pairs, nc) || devAlloc(d, &v.pairShapes, nc) || devAlloc(d, &v.joints, nj) ||
       devAlloc(d, &v.colorBits, ncol) || devAlloc(d, &v.cons, (size_t)nc * CN_FIELD_COUNT) ||
-      devAlloc(d, &v.scratch, nscratch)) {
+      devAlloc(d, &v.scratch, nscratch) || devAlloc(d, &v.ccolor, nc)) {
     b.lastError = "device allocation failed";
     freeDevice(b);
     return -1;
@@ -558,9 +566,23 @@ static int commit(Batch& b) {
   int collide = 28 * np <= kLdsLimit ? 28 * np : 0;
   collide = std::max(collide, 12 * 256);
   b.launch.collideLds = collide;
-  int solve = maxBodies * (int)sizeof(BodyState);
+  int solve = maxBodies * (int)(sizeof(BodyState) + sizeof(BodyConst));
   b.launch.solveLds = solve <= kLdsLimit ? ((solve + 255) & ~255) : 0;
   if (b.launch.block == 0) b.launch.block = maxShapes <= 640 ? 64 : 256;
+  // Full-batch launches dispatch the largest worlds first (longest-processing-time
+  // order), so the last wave of workgroups holds the smallest worlds and the
+  // launch's tail is short. Worlds are independent: the order changes no result.
+  {
+    std::vector<int> order(W);
+    for (int i = 0; i < W; ++i) order[i] = i;
+    auto work = [&](int i) {
+      const HostWorld& hw = b.worlds[i];
+      return (long)hw.shapes.size() * 4 + (long)hw.sims.size() + (long)hw.joints.size() * 4;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    F2D_HIP(hipMalloc(&b.dOrder, sizeof(int) * std::max(W, 1)));
+    F2D_HIP(hipMemcpy(b.dOrder, order.data(), sizeof(int) * W, hipMemcpyHostToDevice));
+  }
   b.committed = true;
   return 0;
 }
@@ -571,6 +593,9 @@ static int upload(Batch& b) {
   }
   BatchView& v = b.dev.view;
   std::vector<WorldHeader> headers(b.worlds.size());
+  long totalContacts = 0;
+  for (const HostWorld& hw : b.worlds) totalContacts = std::max<long>(totalContacts, hw.hdr.contactOffset + hw.hdr.contactCap);
+  std::vector<int8_t> ccolor(totalContacts, -1);
   b.launch.continuous = 0;
   for (size_t i = 0; i < b.worlds.size(); ++i) {
     HostWorld& hw = b.worlds[i];
@@ -589,6 +614,10 @@ static int upload(Batch& b) {
     F2D_HIP(hipMemcpyAsync(v.shapes + h.shapeOffset, hw.shapes.data(), sizeof(Shape) * h.shapeCount, hipMemcpyHostToDevice, s));
     F2D_HIP(hipMemcpyAsync(v.geoms + h.shapeOffset, hw.geoms.data(), sizeof(ShapeGeom) * h.shapeCount, hipMemcpyHostToDevice, s));
     F2D_HIP(hipMemcpyAsync(v.contacts + h.contactOffset, hw.contacts.data(), sizeof(Contact) * h.contactCap, hipMemcpyHostToDevice, s));
+    for (int k = 0; k < h.contactCap; ++k) {
+      const Contact& c = hw.contacts[k];
+      ccolor[h.contactOffset + k] = (int8_t)(c.shapeA >= 0 ? c.color : -1);
+    }
     if (!hw.freeIds.empty())
       F2D_HIP(hipMemcpyAsync(v.freeIds + h.freeOffset, hw.freeIds.data(), sizeof(int) * hw.freeIds.size(), hipMemcpyHostToDevice, s));
     if (!hw.overflow.empty())
@@ -598,6 +627,7 @@ static int upload(Batch& b) {
     F2D_HIP(hipMemcpyAsync(v.colorBits + h.colorOffset, hw.colorBits.data(), sizeof(uint32_t) * hw.colorBits.size(), hipMemcpyHostToDevice, s));
   }
   F2D_HIP(hipMemcpyAsync(v.worlds, headers.data(), sizeof(WorldHeader) * headers.size(), hipMemcpyHostToDevice, b.stream));
+  F2D_HIP(hipMemcpyAsync(v.ccolor, ccolor.data(), ccolor.size(), hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipStreamSynchronize(b.stream));
   b.hostDirty = false;
   return 0;
@@ -1185,7 +1215,7 @@ int f2dg_step(void* h, float dt, int subSteps, int steps, int phases) {
   StepParams sp = makeParams(dt, subSteps, phases == 0 ? PH_ALL : phases);
   int W = (int)b->worlds.size();
   for (int s = 0; s < steps; ++s) {
-    int rc = f2d_launch_step(&b->dev.view, &sp, nullptr, W, b->stream, &b->launch);
+    int rc = f2d_launch_step(&b->dev.view, &sp, b->dOrder, W, b->stream, &b->launch);
     if (rc != 0) {
       std::fprintf(stderr, "forge2d_amd: step launch failed (%d)\n", rc);
       return -1;
@@ -1246,7 +1276,7 @@ int f2dg_step_timed(void* h, float dt, int subSteps, int steps, float* msOut) {
   int W = (int)b->worlds.size();
   F2D_HIP(hipEventRecord(e0, b->stream));
   for (int s = 0; s < steps; ++s) {
-    if (f2d_launch_step(&b->dev.view, &sp, nullptr, W, b->stream, &b->launch) != 0) return -1;
+    if (f2d_launch_step(&b->dev.view, &sp, b->dOrder, W, b->stream, &b->launch) != 0) return -1;
   }
   F2D_HIP(hipEventRecord(e1, b->stream));
   F2D_HIP(hipEventSynchronize(e1));
@@ -1277,7 +1307,7 @@ int f2dg_kernel_times(void* h, float dt, int subSteps, int steps, float* ms) {
   for (int s = 0; s < steps; ++s) {
     F2D_HIP(hipEventRecord(ev[0], b->stream));
     for (int k = 0; k < 3; ++k) {
-      if (f2d_launch_step(&b->dev.view, &sp[k], nullptr, W, b->stream, &b->launch) != 0) return -1;
+      if (f2d_launch_step(&b->dev.view, &sp[k], b->dOrder, W, b->stream, &b->launch) != 0) return -1;
       F2D_HIP(hipEventRecord(ev[k + 1], b->stream));
     }
     F2D_HIP(hipEventSynchronize(ev[3]));

@@ -27,6 +27,7 @@ enum : int {
   BF_IS_FAST = 16,
   BF_SPEED_CAPPED = 32,
   BF_ENLARGED = 64,
+  BF_CAPPED_STEP = 128,  // speed capped during the current step (reported at finalize)
 };
 // shape flags
 enum : int {
@@ -244,6 +245,7 @@ struct BatchView {
   long consStride;
   int* scratch;         // large-world sort scratch
   float4* sapBoxes;     // large-world sorted fat AABBs
+  int8_t* ccolor;       // per contact slot: constraint-graph colour, -1 when none or free
   int* events;          // reserved
   int worldCount;
 };

@@ -540,12 +540,59 @@ F2D_HD void integrateVelocity(BodySim& sim, BodyState& state, Vec2 gravity, floa
   if (dot(v, v) > maxLinearSpeedSquared) {
     float ratio = maxLinearSpeed / length(v);
     v = mulSV(ratio, v);
-    sim.flags |= BF_SPEED_CAPPED;
+    sim.flags |= BF_CAPPED_STEP;
   }
   if (w * w > maxAngularSpeedSquared && (sim.flags & BF_ALLOW_FAST_ROTATION) == 0) {
     float ratio = maxAngularSpeed / absf(w);
     w *= ratio;
-    sim.flags |= BF_SPEED_CAPPED;
+    sim.flags |= BF_CAPPED_STEP;
+  }
+  state.v = v;
+  state.w = w;
+}
+
+// The per-step body terms of b2IntegrateVelocitiesTask (S/solver.c:65-129): the
+// terms that do not change across substeps, evaluated once per step with the
+// reference's own expressions, so each substep's update is bit-identical to
+// recomputing them.
+enum : int { BC_STATIC = 1, BC_FAST_ROTATION = 2, BC_CAPPED = 4 };
+struct alignas(16) BodyConst {
+  Vec2 dv;        // h*invMass*force + h*gravityScale*gravity
+  float dw;       // h*invI*torque
+  float linDamp;  // 1 / (1 + h*linearDamping)
+  float angDamp;  // 1 / (1 + h*angularDamping)
+  float invMass, invI;
+  int flags;      // BC_*
+};
+
+F2D_HD BodyConst makeBodyConst(const BodySim& sim, Vec2 gravity, float h) {
+  BodyConst k;
+  k.linDamp = 1.0f / (1.0f + h * sim.linearDamping);
+  k.angDamp = 1.0f / (1.0f + h * sim.angularDamping);
+  float gravityScale = sim.invMass > 0.0f ? sim.gravityScale : 0.0f;
+  k.dv = add(mulSV(h * sim.invMass, sim.force), mulSV(h * gravityScale, gravity));
+  k.dw = h * sim.invI * sim.torque;
+  k.invMass = sim.invMass;
+  k.invI = sim.invI;
+  k.flags = (sim.type == kStatic ? BC_STATIC : 0) | ((sim.flags & BF_ALLOW_FAST_ROTATION) ? BC_FAST_ROTATION : 0);
+  return k;
+}
+
+// b2IntegrateVelocitiesTask body from the per-step terms
+F2D_HD void integrateVelocityConst(BodyConst& k, BodyState& state, float maxLinearSpeed, float maxAngularSpeed) {
+  float maxLinearSpeedSquared = maxLinearSpeed * maxLinearSpeed;
+  float maxAngularSpeedSquared = maxAngularSpeed * maxAngularSpeed;
+  Vec2 v = mulAdd(k.dv, k.linDamp, state.v);
+  float w = k.dw + k.angDamp * state.w;
+  if (dot(v, v) > maxLinearSpeedSquared) {
+    float ratio = maxLinearSpeed / length(v);
+    v = mulSV(ratio, v);
+    k.flags |= BC_CAPPED;
+  }
+  if (w * w > maxAngularSpeedSquared && (k.flags & BC_FAST_ROTATION) == 0) {
+    float ratio = maxAngularSpeed / absf(w);
+    w *= ratio;
+    k.flags |= BC_CAPPED;
   }
   state.v = v;
   state.w = w;

@@ -167,6 +167,7 @@ struct WorldPtrs {
   Cons cons;
   int* scratch;
   float4* sapBoxes;
+  int8_t* ccolor;
 };
 
 __device__ WorldPtrs worldPtrs(const BatchView& b, WorldHeader* h) {
@@ -186,6 +187,7 @@ __device__ WorldPtrs worldPtrs(const BatchView& b, WorldHeader* h) {
   w.cons.f = b.cons + h->constraintOffset;
   w.cons.stride = b.consStride;
   w.scratch = b.scratch + h->scratchOffset;
+  w.ccolor = b.ccolor + h->contactOffset;
   w.sapBoxes = b.sapBoxes + h->shapeOffset;
   return w;
 }
@@ -274,6 +276,7 @@ __device__ void createContact(const WorldPtrs& w, int shapeIdA, int shapeIdB) {
   c.simFlags = ((A.flags | B.flags) & SF_PRESOLVE) ? SIM_PRESOLVE : 0;
   c.color = -1;
   c.localIndex = -1;
+  w.ccolor[id] = -1;
   int bodyA = A.body, bodyB = B.body;
   {
     BodySim& ba = w.sims[bodyA];
@@ -563,6 +566,7 @@ __device__ void addContactToGraph(const WorldPtrs& w, int id) {
     }
   }
   c.color = color;
+  w.ccolor[id] = (int8_t)color;
   if (color == kOverflowIndex) {
     WorldHeader* h = w.hdr;
     if (h->overflowCount >= h->contactCap) {
@@ -598,6 +602,7 @@ __device__ void removeContactFromGraph(const WorldPtrs& w, int id) {
   }
   c.color = -1;
   c.localIndex = -1;
+  w.ccolor[id] = -1;
 }
 
 // b2DestroyContact, S/contact.c:337-454 (no island or event bookkeeping on the device)
@@ -683,6 +688,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
   const int nb = h->bodyCount;
   const int nc = h->contactCount;
   BodyState* states = kLds ? ldsStates : w.states;
+  BodyConst* ldsConsts = reinterpret_cast<BodyConst*>(ldsStates + nb);  // LDS only
   const Cons& cons = w.cons;
 
   if (phases & PH_SOLVE) {
@@ -695,8 +701,8 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nc; i += BLOCK) {
-      const Contact& c = w.contacts[i];
-      if (c.shapeA >= 0 && c.color >= 0 && c.color < kOverflowIndex) atomicAdd(&sh.colorCount[c.color], 1);
+      int color = w.ccolor[i];
+      if (color >= 0 && color < kOverflowIndex) atomicAdd(&sh.colorCount[color], 1);
     }
     for (int i = threadIdx.x; i < h->jointCount; i += BLOCK) {
       const Joint& j = w.joints[i];
@@ -720,9 +726,9 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nc; i += BLOCK) {
-      const Contact& c = w.contacts[i];
-      if (c.shapeA >= 0 && c.color >= 0 && c.color < kOverflowIndex) {
-        int lane = sh.colorStart[c.color] + atomicAdd(&sh.colorFill[c.color], 1);
+      int color = w.ccolor[i];
+      if (color >= 0 && color < kOverflowIndex) {
+        int lane = sh.colorStart[color] + atomicAdd(&sh.colorFill[color], 1);
         cons.seti(CN_SLOT, lane, i);
       }
     }
@@ -736,9 +742,12 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         jointLanes[sh.jointColorStart[j.color] + atomicAdd(&sh.jointColorFill[j.color], 1)] = i;
       }
     }
-    // body states into LDS
+    // body states and per-step body terms into LDS
     if (kLds) {
-      for (int i = threadIdx.x; i < nb; i += BLOCK) ldsStates[i] = w.states[i];
+      for (int i = threadIdx.x; i < nb; i += BLOCK) {
+        ldsStates[i] = w.states[i];
+        ldsConsts[i] = makeBodyConst(w.sims[i], ctx.gravity, ctx.h);
+      }
     }
     __syncthreads();
     const int totalLanes = sh.colorStart[kGraphColorCount];
@@ -754,12 +763,25 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     for (int lane = threadIdx.x; lane < totalLanes; lane += BLOCK) {
       int slot = cons.geti(CN_SLOT, lane);
       const Contact& c = w.contacts[slot];
-      const BodySim& sa = w.sims[c.bodyA];
-      const BodySim& sb = w.sims[c.bodyB];
-      int ia = sa.type == kStatic ? -1 : c.bodyA;
-      int ib = sb.type == kStatic ? -1 : c.bodyB;
-      float mA = ia < 0 ? 0.0f : sa.invMass, iA = ia < 0 ? 0.0f : sa.invI;
-      float mB = ib < 0 ? 0.0f : sb.invMass, iB = ib < 0 ? 0.0f : sb.invI;
+      bool staticA, staticB;
+      float mA, iA, mB, iB;
+      if (kLds) {
+        const BodyConst& ka = ldsConsts[c.bodyA];
+        const BodyConst& kb = ldsConsts[c.bodyB];
+        staticA = (ka.flags & BC_STATIC) != 0;
+        staticB = (kb.flags & BC_STATIC) != 0;
+        mA = ka.invMass, iA = ka.invI, mB = kb.invMass, iB = kb.invI;
+      } else {
+        const BodySim& sa = w.sims[c.bodyA];
+        const BodySim& sb = w.sims[c.bodyB];
+        staticA = sa.type == kStatic;
+        staticB = sb.type == kStatic;
+        mA = sa.invMass, iA = sa.invI, mB = sb.invMass, iB = sb.invI;
+      }
+      int ia = staticA ? -1 : c.bodyA;
+      int ib = staticB ? -1 : c.bodyB;
+      if (staticA) mA = 0.0f, iA = 0.0f;
+      if (staticB) mB = 0.0f, iB = 0.0f;
       prepareContactLane(cons, lane, c, slot, ia, ib, mA, iA, mB, iB, states, ctx.contactSoftness,
                          ctx.staticSoftness, ctx.warmStartScale);
     }
@@ -768,11 +790,20 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     for (int sub = 0; sub < ctx.subSteps; ++sub) {
       // integrate velocities
       for (int i = threadIdx.x; i < nb; i += BLOCK) {
-        BodySim& sim = w.sims[i];
-        if (sim.type == kStatic) continue;
-        BodyState st = states[i];
-        integrateVelocity(sim, st, ctx.gravity, ctx.h, ctx.maxLinearSpeed, ctx.maxAngularSpeed);
-        states[i] = st;
+        if (kLds) {
+          BodyConst k = ldsConsts[i];
+          if (k.flags & BC_STATIC) continue;
+          BodyState st = states[i];
+          integrateVelocityConst(k, st, ctx.maxLinearSpeed, ctx.maxAngularSpeed);
+          states[i] = st;
+          ldsConsts[i].flags = k.flags;
+        } else {
+          BodySim& sim = w.sims[i];
+          if (sim.type == kStatic) continue;
+          BodyState st = states[i];
+          integrateVelocity(sim, st, ctx.gravity, ctx.h, ctx.maxLinearSpeed, ctx.maxAngularSpeed);
+          states[i] = st;
+        }
       }
       __syncthreads();
       // warm start: overflow joints, overflow contacts, then colours
@@ -884,9 +915,10 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       sim.xf.p = sub(sim.center, rotate(sim.xf.q, sim.localCenter));
       sim.force = {0.0f, 0.0f};
       sim.torque = 0.0f;
-      int flags = sim.flags & ~(BF_IS_FAST | BF_ENLARGED);
-      // isSpeedCapped is reported for this step and cleared for the next
-      flags &= ~BF_SPEED_CAPPED;
+      // isSpeedCapped reports this step's capping (body->isSpeedCapped = sim->isSpeedCapped)
+      bool capped = kLds ? (ldsConsts[i].flags & BC_CAPPED) != 0 : (sim.flags & BF_CAPPED_STEP) != 0;
+      int flags = sim.flags & ~(BF_IS_FAST | BF_ENLARGED | BF_SPEED_CAPPED | BF_CAPPED_STEP);
+      if (capped) flags |= BF_SPEED_CAPPED;
       bool isFast = false;
       if (!ctx.enableSleep || (sim.flags & BF_ENABLE_SLEEP) == 0 || sleepVelocity > sim.sleepThreshold) {
         sim.sleepTime = 0.0f;
@@ -947,7 +979,10 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     __syncthreads();
     if (threadIdx.x == 0) h->fastCount = sh.fastCount;
   } else if (kLds && (phases & PH_SOLVE)) {
-    for (int i = threadIdx.x; i < nb; i += BLOCK) w.states[i] = ldsStates[i];
+    for (int i = threadIdx.x; i < nb; i += BLOCK) {
+      w.states[i] = ldsStates[i];
+      if (ldsConsts[i].flags & BC_CAPPED) w.sims[i].flags |= BF_CAPPED_STEP;
+    }
     __syncthreads();
   }
 }
@@ -1005,8 +1040,8 @@ __global__ __launch_bounds__(BLOCK) void collideKernel(BatchView b, StepParams s
   if (threadIdx.x == 0 && (sp.phases & (PH_SOLVE | PH_FINALIZE | PH_NO_COUNT)) == 0) h->stepIndex += 1;
 }
 
-// Solver stages and finalize of one world per workgroup; body states live in
-// ldsBytes of dynamic LDS (32 B per body) when they fit.
+// Solver stages and finalize of one world per workgroup; body states and the
+// per-step body terms live in ldsBytes of dynamic LDS (64 B per body) when they fit.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void solveKernel(BatchView b, StepParams sp, const int* worldList, int ldsBytes) {
   __shared__ SharedState sh;
@@ -1016,7 +1051,7 @@ __global__ __launch_bounds__(BLOCK) void solveKernel(BatchView b, StepParams sp,
   WorldPtrs w = worldPtrs(b, h);
   SolveCtx ctx;
   stepContext(h, sp, ctx);
-  if (h->bodyCount * (int)sizeof(BodyState) <= ldsBytes) {
+  if (h->bodyCount * (int)(sizeof(BodyState) + sizeof(BodyConst)) <= ldsBytes) {
     solveAndFinalize<BLOCK, true>(w, sh, reinterpret_cast<BodyState*>(lds), ctx, sp.phases);
   } else {
     solveAndFinalize<BLOCK, false>(w, sh, nullptr, ctx, sp.phases);

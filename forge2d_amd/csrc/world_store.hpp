@@ -62,6 +62,7 @@ struct Batch {
   bool ownsStream = false;
   int* dWorldList = nullptr;
   int dWorldListCap = 0;
+  int* dOrder = nullptr;     // every world, largest first: the launch order of full-batch steps
   std::string lastError;
   float contactCapFactor = 6.0f;
   LaunchConfig launch = {0, 0, 0, 0};

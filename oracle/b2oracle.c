@@ -1034,15 +1034,27 @@ void* b2o_create(const void* spec, long specBytes, int world, const void* geoms,
       }
     }
   }
-  /* joint colour arrays in local index order */
+  /* joint colour arrays in local index order (joint id order when the snapshot's
+     local indices are not a permutation of the colour's slots) */
   for (int c = 0; c < COLOR_COUNT; ++c) {
-    int n = 0;
+    int n = 0, valid = 1;
     for (int j = 0; j < w->jointCount; ++j)
       if (w->joints[j].color == c) n++;
     w->jointColorCount[c] = n;
+    for (int k = 0; k < n; ++k) w->jointColors[c][k] = -1;
     for (int j = 0; j < w->jointCount; ++j) {
       OJoint* J = w->joints + j;
-      if (J->color == c && J->localIndex >= 0 && J->localIndex < n) w->jointColors[c][J->localIndex] = j;
+      if (J->color != c) continue;
+      if (J->localIndex < 0 || J->localIndex >= n || w->jointColors[c][J->localIndex] >= 0) {
+        valid = 0;
+        break;
+      }
+      w->jointColors[c][J->localIndex] = j;
+    }
+    if (!valid) {
+      int k = 0;
+      for (int j = 0; j < w->jointCount; ++j)
+        if (w->joints[j].color == c) w->jointColors[c][k++] = j;
     }
   }
   /* awake set in local-index order */
