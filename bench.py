@@ -238,8 +238,11 @@ def main():
         "data": "synthetic: seeded scene generator (forge2d_amd/scenes.py), no sleep",
         "config": {"workload": f"{args.workload}: {desc}", "worlds": W, "awake_bodies": awake,
                    "substeps": SUBSTEPS, "dt": DT, "contacts": st["contacts"], "manifold_points": st["points"],
-                   "joints": st["joints"], "workgroup_lanes": block, "autotune_ms_per_4_steps": tune_ms,
-                   "parallelism": "one workgroup per world, largest worlds first; collide, solve and continuous-collision launches per step"},
+                   "joints": st["joints"], "size_classes": batch.size_classes(),
+                   "autotune_ms_per_4_steps": tune_ms,
+                   "parallelism": "one workgroup per world, worlds launched in size classes (own LDS size and workgroup "
+                                  "size, own stream), largest first; collide, solve and continuous-collision "
+                                  "launches per class per step"},
         "latency_ms": {"p50": p50, "p99": p99, "samples": len(lat)},
         "per_world_gpu": per_world,
         "batched_vs_per_world": (value / per_world["value"]) if per_world else None,

@@ -55,6 +55,7 @@ SIGNATURES = [
     ("f2dg_autotune", _I, [_VP, _F, _I, _I, _PF]),
     ("f2dg_set_block", _I, [_VP, _I]),
     ("f2dg_get_block", _I, [_VP]),
+    ("f2dg_size_classes", _I, [_VP, _PI, _I]),
     ("f2d_create_world", _U, [_F, _F, _F, _F, _F, _F, _F, _F, _I, _I]),
     ("f2d_destroy_world", None, [_U]),
     ("f2d_world_is_valid", _I, [_U]),
@@ -227,6 +228,15 @@ class Batch:
     @property
     def block(self) -> int:
         return lib().f2dg_get_block(self.h)
+
+    def size_classes(self) -> list:
+        """Launch size classes: limit, worlds, max bodies/shapes, workgroup lanes, LDS bytes."""
+        out = (ctypes.c_int32 * (7 * 8))()
+        n = lib().f2dg_size_classes(self.h, out, 8)
+        if n < 0:
+            raise RuntimeError("size_classes failed")
+        keys = ("limit", "worlds", "max_bodies", "max_shapes", "lanes", "collide_lds", "solve_lds")
+        return [dict(zip(keys, out[7 * k:7 * k + 7])) for k in range(n)]
 
     def stats(self) -> dict:
         out = (ctypes.c_long * 8)()

@@ -481,6 +481,14 @@ static void freeDevice(Batch& b) {
   b.dWorldListCap = 0;
   if (b.dOrder) (void)hipFree(b.dOrder);
   b.dOrder = nullptr;
+  for (SizeClass& c : b.classes) {
+    if (c.dList) (void)hipFree(c.dList);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    if (c.done) (void)hipEventDestroy(c.done);
+  }
+  b.classes.clear();
+  if (b.forkEvent) (void)hipEventDestroy(b.forkEvent);
+  b.forkEvent = nullptr;
   b.committed = false;
 }
 
@@ -546,7 +554,7 @@ static int commit(Batch& b) {
       devAlloc(d, &v.contacts, nc) || devAlloc(d, &v.freeIds, nc) || devAlloc(d, &v.overflow, nc) ||
       devAlloc(d, &v.pairs, nc) || devAlloc(d, &v.pairShapes, nc) || devAlloc(d, &v.joints, nj) ||
       devAlloc(d, &v.colorBits, ncol) || devAlloc(d, &v.cons, (size_t)nc * CN_FIELD_COUNT) ||
-      devAlloc(d, &v.scratch, nscratch) || devAlloc(d, &v.ccolor, nc)) {
+      devAlloc(d, &v.scratch, nscratch) || devAlloc(d, &v.ccolor, nc) || devAlloc(d, &v.staticShapes, ns)) {
     b.lastError = "device allocation failed";
     freeDevice(b);
     return -1;
@@ -582,8 +590,66 @@ static int commit(Batch& b) {
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
     F2D_HIP(hipMalloc(&b.dOrder, sizeof(int) * std::max(W, 1)));
     F2D_HIP(hipMemcpy(b.dOrder, order.data(), sizeof(int) * W, hipMemcpyHostToDevice));
+    // size classes by max(bodies, shapes): 64, 128, ..., 1024, and everything larger
+    const int limits[6] = {64, 128, 256, 512, 1024, 1 << 30};
+    std::vector<std::vector<int>> members(6);
+    for (int i : order) {
+      const HostWorld& hw = b.worlds[i];
+      int size = std::max((int)hw.sims.size(), (int)hw.shapes.size());
+      int k = 0;
+      while (size > limits[k]) ++k;
+      members[k].push_back(i);
+    }
+    for (int k = 0; k < 6; ++k) {
+      if (members[k].empty()) continue;
+      SizeClass c;
+      c.limit = limits[k];
+      c.count = (int)members[k].size();
+      for (int i : members[k]) {
+        c.maxBodies = std::max(c.maxBodies, (int)b.worlds[i].sims.size());
+        c.maxShapes = std::max(c.maxShapes, (int)b.worlds[i].shapes.size());
+      }
+      int cnp = nextPow2(std::max(c.maxShapes, 1));
+      int cc = 28 * cnp <= kLdsLimit ? 28 * cnp : 0;
+      c.launch.collideLds = std::max(cc, 12 * 256);
+      int cs = std::max(c.maxBodies, 1) * (int)(sizeof(BodyState) + sizeof(BodyConst));
+      c.launch.solveLds = cs <= kLdsLimit ? ((cs + 255) & ~255) : 0;
+      c.launch.block = c.limit <= 128 ? 64 : (c.limit <= 512 ? 128 : 256);
+      F2D_HIP(hipMalloc(&c.dList, sizeof(int) * c.count));
+      F2D_HIP(hipMemcpy(c.dList, members[k].data(), sizeof(int) * c.count, hipMemcpyHostToDevice));
+      F2D_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+      F2D_HIP(hipEventCreateWithFlags(&c.done, hipEventDisableTiming));
+      b.classes.push_back(c);
+    }
+    F2D_HIP(hipEventCreateWithFlags(&b.forkEvent, hipEventDisableTiming));
   }
   b.committed = true;
+  return 0;
+}
+
+// Launch one step of every world: each size class on its own stream, forked from
+// and joined back into the batch's stream, so classes overlap on the device.
+static int launchClasses(Batch& b, const StepParams& sp) {
+  if (b.classes.size() == 1) {
+    SizeClass& c = b.classes[0];
+    return f2d_launch_step(&b.dev.view, &sp, c.dList, c.count, b.stream, &c.launch);
+  }
+  F2D_HIP(hipEventRecord(b.forkEvent, b.stream));
+  for (SizeClass& c : b.classes) {
+    F2D_HIP(hipStreamWaitEvent(c.stream, b.forkEvent, 0));
+    if (f2d_launch_step(&b.dev.view, &sp, c.dList, c.count, c.stream, &c.launch) != 0) return -1;
+    F2D_HIP(hipEventRecord(c.done, c.stream));
+  }
+  for (SizeClass& c : b.classes) F2D_HIP(hipStreamWaitEvent(b.stream, c.done, 0));
+  return 0;
+}
+
+// Launch one step of every world, the classes one after another on the batch's
+// stream (for per-kernel timing with events on that stream).
+static int launchClassesSerial(Batch& b, const StepParams& sp) {
+  for (SizeClass& c : b.classes) {
+    if (f2d_launch_step(&b.dev.view, &sp, c.dList, c.count, b.stream, &c.launch) != 0) return -1;
+  }
   return 0;
 }
 
@@ -596,12 +662,20 @@ static int upload(Batch& b) {
   long totalContacts = 0;
   for (const HostWorld& hw : b.worlds) totalContacts = std::max<long>(totalContacts, hw.hdr.contactOffset + hw.hdr.contactCap);
   std::vector<int8_t> ccolor(totalContacts, -1);
+  long totalShapes = 0;
+  for (const HostWorld& hw : b.worlds) totalShapes = std::max<long>(totalShapes, hw.hdr.shapeOffset + hw.hdr.shapeCount);
+  std::vector<int> staticShapes(std::max<long>(totalShapes, 1), 0);
   b.launch.continuous = 0;
   for (size_t i = 0; i < b.worlds.size(); ++i) {
     HostWorld& hw = b.worlds[i];
     WorldHeader& h = hw.hdr;
     if (h.flags & WF_CONTINUOUS) b.launch.continuous = 1;
     h.fastCount = 0;
+    h.staticCount = 0;
+    for (int k = 0; k < h.shapeCount; ++k) {
+      const Shape& sh = hw.shapes[k];
+      if (sh.proxyKey >= 0 && (sh.proxyKey & 3) == kStatic) staticShapes[h.shapeOffset + h.staticCount++] = k;
+    }
     h.contactCount = hw.contactCount;
     h.freeCount = (int)hw.freeIds.size();
     h.overflowCount = (int)hw.overflow.size();
@@ -628,7 +702,9 @@ static int upload(Batch& b) {
   }
   F2D_HIP(hipMemcpyAsync(v.worlds, headers.data(), sizeof(WorldHeader) * headers.size(), hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipMemcpyAsync(v.ccolor, ccolor.data(), ccolor.size(), hipMemcpyHostToDevice, b.stream));
+  F2D_HIP(hipMemcpyAsync(v.staticShapes, staticShapes.data(), sizeof(int) * totalShapes, hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipStreamSynchronize(b.stream));
+  for (SizeClass& c : b.classes) c.launch.continuous = b.launch.continuous;
   b.hostDirty = false;
   return 0;
 }
@@ -1214,8 +1290,9 @@ int f2dg_step(void* h, float dt, int subSteps, int steps, int phases) {
   if (dt == 0.0f) return 0;
   StepParams sp = makeParams(dt, subSteps, phases == 0 ? PH_ALL : phases);
   int W = (int)b->worlds.size();
+  (void)W;
   for (int s = 0; s < steps; ++s) {
-    int rc = f2d_launch_step(&b->dev.view, &sp, b->dOrder, W, b->stream, &b->launch);
+    int rc = launchClasses(*b, sp);
     if (rc != 0) {
       std::fprintf(stderr, "forge2d_amd: step launch failed (%d)\n", rc);
       return -1;
@@ -1274,9 +1351,10 @@ int f2dg_step_timed(void* h, float dt, int subSteps, int steps, float* msOut) {
   F2D_HIP(hipEventCreate(&e1));
   StepParams sp = makeParams(dt, subSteps, PH_ALL);
   int W = (int)b->worlds.size();
+  (void)W;
   F2D_HIP(hipEventRecord(e0, b->stream));
   for (int s = 0; s < steps; ++s) {
-    if (f2d_launch_step(&b->dev.view, &sp, b->dOrder, W, b->stream, &b->launch) != 0) return -1;
+    if (launchClasses(*b, sp) != 0) return -1;
   }
   F2D_HIP(hipEventRecord(e1, b->stream));
   F2D_HIP(hipEventSynchronize(e1));
@@ -1302,12 +1380,11 @@ int f2dg_kernel_times(void* h, float dt, int subSteps, int steps, float* ms) {
   StepParams sp[3] = {makeParams(dt, subSteps, PH_PAIRS | PH_COLLIDE | PH_STATE | PH_NO_COUNT),
                       makeParams(dt, subSteps, PH_SOLVE | PH_FINALIZE | PH_NO_CONTINUOUS),
                       makeParams(dt, subSteps, PH_CONTINUOUS)};
-  int W = (int)b->worlds.size();
   float sum[3] = {0.0f, 0.0f, 0.0f};
   for (int s = 0; s < steps; ++s) {
     F2D_HIP(hipEventRecord(ev[0], b->stream));
     for (int k = 0; k < 3; ++k) {
-      if (f2d_launch_step(&b->dev.view, &sp[k], b->dOrder, W, b->stream, &b->launch) != 0) return -1;
+      if (launchClassesSerial(*b, sp[k]) != 0) return -1;
       F2D_HIP(hipEventRecord(ev[k + 1], b->stream));
     }
     F2D_HIP(hipEventSynchronize(ev[3]));
@@ -1418,34 +1495,75 @@ int f2dg_world_counters(void* h, int world, int* out) {
   return 0;
 }
 
-// Workgroup-size autotuner: time `steps` steps of the live batch with each
-// candidate size (64, 128, 256 lanes per world) and keep the fastest. The trial
-// steps advance the batch; results do not depend on the workgroup size.
+// Workgroup-size autotuner: for each size class, time `steps` steps of that
+// class's worlds with each candidate size (64, 128, 256 lanes per world) and keep
+// the fastest. msOut (optional, 3 floats) receives the whole-batch ms per
+// candidate summed over classes. The trial steps advance every world by the same
+// number of steps; results do not depend on the workgroup size.
 int f2dg_autotune(void* h, float dt, int subSteps, int steps, float* msOut) {
   Batch* b = static_cast<Batch*>(h);
   if (!b) return -1;
   if (prepareForStep(*b) != 0) return -1;
   const int cands[3] = {64, 128, 256};
-  float best = 1e30f;
-  int bestBlock = b->launch.block;
-  for (int k = 0; k < 3; ++k) {
-    b->launch.block = cands[k];
-    float ms = 0.0f;
-    if (f2dg_step_timed(h, dt, subSteps, steps, &ms) != 0) return -1;
-    if (msOut) msOut[k] = ms;
-    if (ms < best) {
-      best = ms;
-      bestBlock = cands[k];
+  StepParams sp = makeParams(dt, subSteps, PH_ALL);
+  hipEvent_t e0, e1;
+  F2D_HIP(hipEventCreate(&e0));
+  F2D_HIP(hipEventCreate(&e1));
+  if (msOut) msOut[0] = msOut[1] = msOut[2] = 0.0f;
+  for (SizeClass& c : b->classes) {
+    float best = 1e30f;
+    int bestBlock = c.launch.block;
+    for (int k = 0; k < 3; ++k) {
+      LaunchConfig cfg = c.launch;
+      cfg.block = cands[k];
+      F2D_HIP(hipEventRecord(e0, b->stream));
+      for (int s = 0; s < steps; ++s) {
+        if (f2d_launch_step(&b->dev.view, &sp, c.dList, c.count, b->stream, &cfg) != 0) return -1;
+      }
+      F2D_HIP(hipEventRecord(e1, b->stream));
+      F2D_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      F2D_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (msOut) msOut[k] += ms;
+      if (ms < best) {
+        best = ms;
+        bestBlock = cands[k];
+      }
     }
+    c.launch.block = bestBlock;
   }
-  b->launch.block = bestBlock;
-  return bestBlock;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->deviceNewer = true;
+  return b->classes.empty() ? b->launch.block : b->classes.back().launch.block;
+}
+
+// Size classes after commit: per class [limit, worlds, max bodies, max shapes,
+// workgroup lanes, collide LDS bytes, solve LDS bytes]; returns the class count.
+int f2dg_size_classes(void* h, int* out, int cap) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return -1;
+  if (!b->committed && upload(*b) != 0) return -1;
+  int n = (int)b->classes.size();
+  for (int k = 0; k < n && k < cap; ++k) {
+    const SizeClass& c = b->classes[k];
+    int* o = out + 7 * k;
+    o[0] = c.limit;
+    o[1] = c.count;
+    o[2] = c.maxBodies;
+    o[3] = c.maxShapes;
+    o[4] = c.launch.block;
+    o[5] = c.launch.collideLds;
+    o[6] = c.launch.solveLds;
+  }
+  return n;
 }
 
 int f2dg_set_block(void* h, int block) {
   Batch* b = static_cast<Batch*>(h);
   if (!b || (block != 64 && block != 128 && block != 256)) return -1;
   b->launch.block = block;
+  for (SizeClass& c : b->classes) c.launch.block = block;
   return 0;
 }
 

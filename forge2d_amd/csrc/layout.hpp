@@ -187,6 +187,7 @@ struct alignas(16) WorldHeader {
   float inv_h;                   // world->inv_h
   float jointHertz, jointDampingRatio;
   int fastCount;                 // fast bodies listed by finalize for continuous collision
+  int staticCount;               // static proxies listed at staticShapes[shapeOffset ...]
 };
 
 // Per-step scalars, S/world.c:743-771
@@ -246,6 +247,7 @@ struct BatchView {
   int* scratch;         // large-world sort scratch
   float4* sapBoxes;     // large-world sorted fat AABBs
   int8_t* ccolor;       // per contact slot: constraint-graph colour, -1 when none or free
+  int* staticShapes;    // per world, the shape ids of its static proxies in shape order
   int* events;          // reserved
   int worldCount;
 };

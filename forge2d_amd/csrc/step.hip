@@ -1204,29 +1204,16 @@ __device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList
 // workgroup, one fast body per lane. Worlds without fast bodies return at once.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void continuousKernel(BatchView b, const int* worldList) {
-  __shared__ int waveSums[kMaxWaves];
-  __shared__ int staticTotal;
   const int wi = worldList ? worldList[blockIdx.x] : blockIdx.x;
   WorldHeader* h = b.worlds + wi;
   const int nFast = h->fastCount;
   if (nFast == 0) return;
   WorldPtrs w = worldPtrs(b, h);
   const int ns = h->shapeCount;
-  const int* fastList = w.scratch;   // written by finalize, nFast <= shapeCount
-  int* staticList = w.scratch + ns;  // static proxies in shape order
-  if (threadIdx.x == 0) staticTotal = 0;
-  __syncthreads();
-  for (int base = 0; base < ns; base += BLOCK) {
-    int t = base + threadIdx.x;
-    bool isStatic = t < ns && w.shapes[t].proxyKey >= 0 && (w.shapes[t].proxyKey & 3) == kStatic;
-    int total = 0;
-    int slot = blockCompact<BLOCK>(isStatic, waveSums, &total);
-    if (isStatic) staticList[staticTotal + slot] = t;
-    __syncthreads();
-    if (threadIdx.x == 0) staticTotal += total;
-    __syncthreads();
-  }
-  const int staticCount = staticTotal;
+  const int* fastList = w.scratch;  // written by finalize, nFast <= shapeCount
+  const int* staticList = b.staticShapes + h->shapeOffset;
+  const int staticCount = h->staticCount;
+  (void)ns;
   for (int k = threadIdx.x; k < nFast; k += BLOCK) {
     int i = fastList[k];
     if ((w.sims[i].flags & BF_BULLET) == 0) solveContinuous(w, i, staticList, staticCount, false);

@@ -52,6 +52,19 @@ struct DeviceBatch {
   size_t bytes = 0;
 };
 
+// Worlds of similar size launched together: each class gets dynamic LDS sized
+// for its largest world and its own workgroup size, so small worlds are not
+// limited to the occupancy the batch's largest world allows.
+struct SizeClass {
+  int limit = 0;              // max(bodies, shapes) of the class's worlds is at most this
+  int count = 0;
+  int maxBodies = 0, maxShapes = 0;
+  int* dList = nullptr;       // world ids, largest first
+  LaunchConfig launch = {0, 0, 0, 0};
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+};
+
 struct Batch {
   std::vector<HostWorld> worlds;
   DeviceBatch dev;
@@ -63,6 +76,8 @@ struct Batch {
   int* dWorldList = nullptr;
   int dWorldListCap = 0;
   int* dOrder = nullptr;     // every world, largest first: the launch order of full-batch steps
+  std::vector<SizeClass> classes;
+  hipEvent_t forkEvent = nullptr;
   std::string lastError;
   float contactCapFactor = 6.0f;
   LaunchConfig launch = {0, 0, 0, 0};

@@ -170,9 +170,14 @@ long f2dg_export_world(void* batch, int world, void* out, long cap);
 /* overlay a snapshot onto a world built from the same scene spec */
 int f2dg_import_world(void* batch, int world, const void* snapshot, long bytes);
 const char* f2dg_last_error(void* batch);
-/* workgroup-size autotuner: times `steps` steps with 64, 128 and 256 lanes per world
- * (ms_out[3] optional), keeps the fastest and returns it; the trial steps advance the batch */
+/* workgroup-size autotuner: per size class, times `steps` steps with 64, 128 and 256 lanes
+ * per world and keeps the fastest (ms_out[3] optional: whole-batch ms per candidate); the
+ * trial steps advance every world equally; returns the largest class's choice */
 int f2dg_autotune(void* batch, float dt, int sub_steps, int steps, float* ms_out);
+/* worlds are launched in size classes by max(bodies, shapes) <= 64, 128, ..., 1024, larger;
+ * per class: [limit, worlds, max bodies, max shapes, workgroup lanes, collide LDS bytes,
+ * solve LDS bytes] into out (7 ints each, up to cap classes); returns the class count */
+int f2dg_size_classes(void* batch, int32_t* out, int cap);
 /* force a workgroup size (64, 128 or 256) / read the current one */
 int f2dg_set_block(void* batch, int block);
 int f2dg_get_block(void* batch);
