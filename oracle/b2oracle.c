@@ -987,6 +987,13 @@ void* b2o_create(const void* spec, long specBytes, int world, const void* geoms,
     for (int c = 0; c < COLOR_COUNT; ++c) {
       for (int k = 0; k < colorCounts[c]; ++k) {
         int id = ids[k];
+        if (id < 0 || id >= hh->contactCap || w->contacts[id].shapeA < 0) {
+          b2o_destroy(w);
+          return NULL;
+        }
+        /* a contact's local index is its position in its colour array */
+        w->contacts[id].color = c;
+        w->contacts[id].localIndex = w->colorCount[c];
         w->colors[c][w->colorCount[c]++] = id;
         if (c < OVERFLOW) {
           OContact* ct = w->contacts + id;
@@ -1274,6 +1281,8 @@ static void add_to_graph(OWorld* w, int id) {
 static void remove_from_graph(OWorld* w, int id) {
   OContact* c = w->contacts + id;
   int color = c->color, local = c->localIndex;
+  if (color < 0 || color >= COLOR_COUNT || local < 0 || local >= w->colorCount[color] || w->colors[color][local] != id)
+    abort();  /* the colour arrays and the contacts disagree: a broken input snapshot */
   int ba = w->shapes[c->shapeA].body, bb = w->shapes[c->shapeB].body;
   if (color != OVERFLOW) {
     w->bodySet[color][ba] = 0;
