@@ -28,6 +28,7 @@ _lib = None
 _VP, _I, _L, _F, _U = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_float, ctypes.c_uint32
 _PI = ctypes.POINTER(ctypes.c_int32)
 _PF = ctypes.POINTER(ctypes.c_float)
+_PD = ctypes.POINTER(ctypes.c_double)
 SIGNATURES = [
     ("f2dg_create_from_spec", _VP, [_VP, _L]),
     ("f2dg_destroy", None, [_VP]),
@@ -56,6 +57,11 @@ SIGNATURES = [
     ("f2dg_set_block", _I, [_VP, _I]),
     ("f2dg_get_block", _I, [_VP]),
     ("f2dg_size_classes", _I, [_VP, _PI, _I]),
+    ("f2dg_contact_event_counts", _I, [_VP, _I, _PI]),
+    ("f2dg_contact_begin_events", _I, [_VP, _I, _PD]),
+    ("f2dg_contact_end_events", _I, [_VP, _I, _PD]),
+    ("f2dg_contact_hit_events", _I, [_VP, _I, _PD]),
+    ("f2dg_body_events", _I, [_VP, _I, _PD]),
     ("f2d_create_world", _U, [_F, _F, _F, _F, _F, _F, _F, _F, _I, _I]),
     ("f2d_destroy_world", None, [_U]),
     ("f2d_world_is_valid", _I, [_U]),
@@ -88,6 +94,12 @@ SIGNATURES = [
     ("f2d_create_polygon_shape", _I, [ctypes.c_int32, _U, _VP, _PF, _I, _F, _PI]),
     ("f2d_create_revolute_joint", None, [_U, ctypes.c_int32, _U, ctypes.c_int32, _U, _F, _F, _F, _F, _F, _F, _I, _F,
                                          _F, _I, _F, _F, _I, _F, _F, _F, _I, _PI]),
+    ("f2d_world_get_contact_event_counts", None, [_U, _PI]),
+    ("f2d_world_get_contact_begin_events", _I, [_U, _PD]),
+    ("f2d_world_get_contact_end_events", _I, [_U, _PD]),
+    ("f2d_world_get_contact_hit_events", _I, [_U, _PD]),
+    ("f2d_world_get_body_event_count", _I, [_U]),
+    ("f2d_world_get_body_events", _I, [_U, _PD]),
     ("f2d_world_batch", _VP, [_U]),
 ]
 
@@ -228,6 +240,31 @@ class Batch:
     @property
     def block(self) -> int:
         return lib().f2dg_get_block(self.h)
+
+    def contact_events(self, world: int) -> dict:
+        """Begin (n x 13), end (n x 4) and hit (n x 9) contact events of the last step,
+        float64 records in the flat shim's layouts."""
+        L = lib()
+        c = (ctypes.c_int32 * 3)()
+        self._check(L.f2dg_contact_event_counts(self.h, world, c), "contact_event_counts")
+        out = {}
+        for key, fn, width, n in (("begin", L.f2dg_contact_begin_events, 13, c[0]),
+                                  ("end", L.f2dg_contact_end_events, 4, c[1]),
+                                  ("hit", L.f2dg_contact_hit_events, 9, c[2])):
+            a = np.zeros((n, width), np.float64)
+            if n:
+                fn(self.h, world, a.ctypes.data_as(_PD))
+            out[key] = a
+        return out
+
+    def body_events(self, world: int) -> np.ndarray:
+        """Body move events of the last step (n x 7)."""
+        L = lib()
+        n = L.f2dg_body_events(self.h, world, None)
+        a = np.zeros((max(n, 0), 7), np.float64)
+        if n > 0:
+            L.f2dg_body_events(self.h, world, a.ctypes.data_as(_PD))
+        return a
 
     def size_classes(self) -> list:
         """Launch size classes: limit, worlds, max bodies/shapes, workgroup lanes, LDS bytes."""

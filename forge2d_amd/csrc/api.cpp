@@ -324,6 +324,7 @@ static void initWorldHeader(HostWorld& hw, const SpecWorld& sw) {
   h.flags = sw.flags;
   h.gravity = {sw.gx, sw.gy};
   h.restitutionThreshold = sw.restitution_threshold;
+  h.hitEventThreshold = sw.hit_event_threshold;
   h.maxLinearSpeed = sw.maximum_linear_speed;
   h.contactHertz = sw.contact_hertz;
   h.contactDampingRatio = sw.contact_damping_ratio;
@@ -554,7 +555,8 @@ static int commit(Batch& b) {
       devAlloc(d, &v.contacts, nc) || devAlloc(d, &v.freeIds, nc) || devAlloc(d, &v.overflow, nc) ||
       devAlloc(d, &v.pairs, nc) || devAlloc(d, &v.pairShapes, nc) || devAlloc(d, &v.joints, nj) ||
       devAlloc(d, &v.colorBits, ncol) || devAlloc(d, &v.cons, (size_t)nc * CN_FIELD_COUNT) ||
-      devAlloc(d, &v.scratch, nscratch) || devAlloc(d, &v.ccolor, nc) || devAlloc(d, &v.staticShapes, ns)) {
+      devAlloc(d, &v.scratch, nscratch) || devAlloc(d, &v.ccolor, nc) || devAlloc(d, &v.staticShapes, ns) ||
+      devAlloc(d, &v.events, 3 * nc)) {
     b.lastError = "device allocation failed";
     freeDevice(b);
     return -1;
@@ -733,6 +735,12 @@ static int download(Batch& b) {
     if (!hw.joints.empty())
       F2D_HIP(hipMemcpyAsync(hw.joints.data(), v.joints + h.jointOffset, sizeof(Joint) * hw.joints.size(), hipMemcpyDeviceToHost, s));
     F2D_HIP(hipMemcpyAsync(hw.colorBits.data(), v.colorBits + h.colorOffset, sizeof(uint32_t) * hw.colorBits.size(), hipMemcpyDeviceToHost, s));
+    hw.beginIds.resize(h.beginCount);
+    hw.endPairs.resize(2 * h.endCount);
+    if (h.beginCount)
+      F2D_HIP(hipMemcpyAsync(hw.beginIds.data(), v.events + 3L * h.contactOffset, sizeof(int) * h.beginCount, hipMemcpyDeviceToHost, s));
+    if (h.endCount)
+      F2D_HIP(hipMemcpyAsync(hw.endPairs.data(), v.events + 3L * h.contactOffset + h.contactCap, sizeof(int) * 2 * h.endCount, hipMemcpyDeviceToHost, s));
     F2D_HIP(hipStreamSynchronize(s));
     WorldHeader keep = hw.hdr;
     hw.hdr = h;
@@ -1471,6 +1479,154 @@ const char* f2dg_last_error(void* h) {
   return b ? b->lastError.c_str() : "null batch";
 }
 
+// ---------------------------------------------------------------------------- events
+
+// Contact events of a world's last step, in the record layouts of the flat shim's
+// getters (F/native/wasm/f2d_shim.c:1186-1300). Begin and end events come from the
+// device's serial contact state loop in ascending contact id, as the reference
+// pushes them (S/world.c:587-690, S/contact.c:354-372); hit events are gathered
+// from the stored manifolds like S/solver.c:1758-1817, colour by colour (within a
+// colour in contact id order, the reference uses its colour array order).
+static int hitEvents(Batch& b, int world, double* out, uint32_t shapeWg);
+
+static void contactEventCounts(Batch& b, int world, int* out) {
+  download(b);
+  HostWorld& hw = b.worlds[world];
+  out[0] = (int)hw.beginIds.size();
+  out[1] = (int)hw.endPairs.size() / 2;
+  out[2] = 0;
+  out[2] = hitEvents(b, world, nullptr, 0);
+}
+
+static int beginEvents(Batch& b, int world, double* out, uint32_t shapeWg) {
+  download(b);
+  HostWorld& hw = b.worlds[world];
+  int n = (int)hw.beginIds.size();
+  for (int i = 0; out && i < n; ++i) {
+    const Contact& c = hw.contacts[hw.beginIds[i]];
+    double* r = out + 13 * i;
+    r[0] = c.shapeA + 1;
+    r[1] = shapeWg;
+    r[2] = c.shapeB + 1;
+    r[3] = shapeWg;
+    r[4] = c.m.normal.x;
+    r[5] = c.m.normal.y;
+    r[6] = c.m.pointCount;
+    for (int p = 0; p < 2; ++p) {
+      r[7 + 3 * p] = c.m.points[p].point.x;
+      r[8 + 3 * p] = c.m.points[p].point.y;
+      r[9 + 3 * p] = c.m.points[p].separation;
+    }
+  }
+  return n;
+}
+
+static int endEvents(Batch& b, int world, double* out, uint32_t shapeWg) {
+  download(b);
+  HostWorld& hw = b.worlds[world];
+  int n = (int)hw.endPairs.size() / 2;
+  for (int i = 0; out && i < n; ++i) {
+    double* r = out + 4 * i;
+    r[0] = hw.endPairs[2 * i] + 1;
+    r[1] = shapeWg;
+    r[2] = hw.endPairs[2 * i + 1] + 1;
+    r[3] = shapeWg;
+  }
+  return n;
+}
+
+static int hitEvents(Batch& b, int world, double* out, uint32_t shapeWg) {
+  download(b);
+  HostWorld& hw = b.worlds[world];
+  const float threshold = hw.hdr.hitEventThreshold;
+  int n = 0;
+  auto visit = [&](int id) {
+    const Contact& c = hw.contacts[id];
+    if ((c.simFlags & SIM_HIT_EVENT) == 0) return;
+    float approach = threshold;
+    Vec2 point = {0.0f, 0.0f};
+    bool hit = false;
+    for (int k = 0; k < c.m.pointCount; ++k) {
+      const ManifoldPoint& mp = c.m.points[k];
+      float a = -mp.normalVelocity;
+      if (a > approach && mp.totalNormalImpulse > 0.0f) {
+        approach = a;
+        point = mp.point;
+        hit = true;
+      }
+    }
+    if (!hit) return;
+    if (out) {
+      double* r = out + 9 * n;
+      r[0] = c.shapeA + 1;
+      r[1] = shapeWg;
+      r[2] = c.shapeB + 1;
+      r[3] = shapeWg;
+      r[4] = point.x;
+      r[5] = point.y;
+      r[6] = c.m.normal.x;
+      r[7] = c.m.normal.y;
+      r[8] = approach;
+    }
+    ++n;
+  };
+  for (int color = 0; color < kOverflowIndex; ++color)
+    for (int id = 0; id < hw.contactCount; ++id)
+      if (hw.contacts[id].shapeA >= 0 && hw.contacts[id].color == color) visit(id);
+  for (int id : hw.overflow) visit(id);
+  return n;
+}
+
+// Body move events: every awake body in solver order (S/solver.c:605-610).
+static int bodyEvents(Batch& b, int world, double* out, int worldIndex0) {
+  download(b);
+  HostWorld& hw = b.worlds[world];
+  int n = 0;
+  for (size_t i = 0; i < hw.sims.size(); ++i) {
+    const BodySim& s = hw.sims[i];
+    if (s.type == kStatic) continue;
+    if (out) {
+      double* r = out + 7 * n;
+      r[0] = (double)i + 1;
+      r[1] = worldIndex0 < 0 ? 0.0 : (double)((uint32_t)worldIndex0 | ((uint32_t)hw.bodyGen[i] << 16));
+      r[2] = s.xf.p.x;
+      r[3] = s.xf.p.y;
+      r[4] = s.xf.q.c;
+      r[5] = s.xf.q.s;
+      r[6] = 0.0;  // fellAsleep: worlds run with sleep off
+    }
+    ++n;
+  }
+  return n;
+}
+
+int f2dg_contact_event_counts(void* h, int world, int32_t* out) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || world < 0 || world >= (int)b->worlds.size()) return -1;
+  int c[3];
+  contactEventCounts(*b, world, c);
+  out[0] = c[0];
+  out[1] = c[1];
+  out[2] = c[2];
+  return 0;
+}
+int f2dg_contact_begin_events(void* h, int world, double* out) {
+  Batch* b = static_cast<Batch*>(h);
+  return (b && world >= 0 && world < (int)b->worlds.size()) ? beginEvents(*b, world, out, 0) : -1;
+}
+int f2dg_contact_end_events(void* h, int world, double* out) {
+  Batch* b = static_cast<Batch*>(h);
+  return (b && world >= 0 && world < (int)b->worlds.size()) ? endEvents(*b, world, out, 0) : -1;
+}
+int f2dg_contact_hit_events(void* h, int world, double* out) {
+  Batch* b = static_cast<Batch*>(h);
+  return (b && world >= 0 && world < (int)b->worlds.size()) ? hitEvents(*b, world, out, 0) : -1;
+}
+int f2dg_body_events(void* h, int world, double* out) {
+  Batch* b = static_cast<Batch*>(h);
+  return (b && world >= 0 && world < (int)b->worlds.size()) ? bodyEvents(*b, world, out, -1) : -1;
+}
+
 int f2dg_world_counters(void* h, int world, int* out) {
   // [bodies, shapes, live contacts, touching contacts, overflow, active colours, step index, errors]
   Batch* b = static_cast<Batch*>(h);
@@ -1949,5 +2105,43 @@ void f2d_create_revolute_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int3
 
 // Batch handle behind a single world, for the batch ABI (export, counters).
 void* f2d_world_batch(uint32_t w) { return batchOf(w); }
+
+}  // extern "C"
+
+// ---- events through the flat per-world ABI (F/native/wasm/f2d_shim.c:1186-1300)
+
+static uint32_t shapeWgOf(uint32_t w) { return ((w & 0xFFFF) - 1) | (1u << 16); }
+
+extern "C" {
+
+void f2d_world_get_contact_event_counts(uint32_t w, int32_t* out) {
+  Batch* b = batchOf(w);
+  if (!b) return;
+  int c[3];
+  contactEventCounts(*b, 0, c);
+  out[0] = c[0];
+  out[1] = c[1];
+  out[2] = c[2];
+}
+int f2d_world_get_contact_begin_events(uint32_t w, double* out) {
+  Batch* b = batchOf(w);
+  return b ? beginEvents(*b, 0, out, shapeWgOf(w)) : 0;
+}
+int f2d_world_get_contact_end_events(uint32_t w, double* out) {
+  Batch* b = batchOf(w);
+  return b ? endEvents(*b, 0, out, shapeWgOf(w)) : 0;
+}
+int f2d_world_get_contact_hit_events(uint32_t w, double* out) {
+  Batch* b = batchOf(w);
+  return b ? hitEvents(*b, 0, out, shapeWgOf(w)) : 0;
+}
+int f2d_world_get_body_event_count(uint32_t w) {
+  Batch* b = batchOf(w);
+  return b ? bodyEvents(*b, 0, nullptr, (int)(w & 0xFFFF) - 1) : 0;
+}
+int f2d_world_get_body_events(uint32_t w, double* out) {
+  Batch* b = batchOf(w);
+  return b ? bodyEvents(*b, 0, out, (int)(w & 0xFFFF) - 1) : 0;
+}
 
 }  // extern "C"

@@ -188,6 +188,8 @@ struct alignas(16) WorldHeader {
   float jointHertz, jointDampingRatio;
   int fastCount;                 // fast bodies listed by finalize for continuous collision
   int staticCount;               // static proxies listed at staticShapes[shapeOffset ...]
+  int beginCount, endCount;      // contact begin/end touch events of the last collide
+  float hitEventThreshold;       // b2WorldDef::hitEventThreshold
 };
 
 // Per-step scalars, S/world.c:743-771
@@ -248,7 +250,7 @@ struct BatchView {
   float4* sapBoxes;     // large-world sorted fat AABBs
   int8_t* ccolor;       // per contact slot: constraint-graph colour, -1 when none or free
   int* staticShapes;    // per world, the shape ids of its static proxies in shape order
-  int* events;          // reserved
+  int* events;          // per world at 3 * contactOffset: begin contact ids [contactCap], end shape pairs [2 * contactCap]
   int worldCount;
 };
 

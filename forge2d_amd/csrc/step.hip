@@ -168,6 +168,7 @@ struct WorldPtrs {
   int* scratch;
   float4* sapBoxes;
   int8_t* ccolor;
+  int* events;
 };
 
 __device__ WorldPtrs worldPtrs(const BatchView& b, WorldHeader* h) {
@@ -188,6 +189,7 @@ __device__ WorldPtrs worldPtrs(const BatchView& b, WorldHeader* h) {
   w.cons.stride = b.consStride;
   w.scratch = b.scratch + h->scratchOffset;
   w.ccolor = b.ccolor + h->contactOffset;
+  w.events = b.events + 3L * h->contactOffset;
   w.sapBoxes = b.sapBoxes + h->shapeOffset;
   return w;
 }
@@ -649,22 +651,40 @@ __device__ void statePhase(const WorldPtrs& w, SharedState& sh) {
   if (threadIdx.x == 0) {
     const int* list = w.scratch;
     const int n = sh.changedCount;
+    WorldHeader* h = w.hdr;
+    int* beginIds = w.events;
+    int* endPairs = w.events + h->contactCap;
+    int nBegin = 0, nEnd = 0;
     for (int k = 0; k < n; ++k) {
       int id = list[k];
       Contact& c = w.contacts[id];
       int simFlags = c.simFlags;
       if (simFlags & SIM_DISJOINT) {
+        // b2DestroyContact reports the end of a touching contact, S/contact.c:354-372
+        if ((c.flags & CF_TOUCHING) && (c.flags & CF_CONTACT_EVENTS) && nEnd < h->contactCap) {
+          endPairs[2 * nEnd] = c.shapeA;
+          endPairs[2 * nEnd + 1] = c.shapeB;
+          ++nEnd;
+        }
         destroyContact(w, id);
       } else if (simFlags & SIM_STARTED) {
+        if ((c.flags & CF_CONTACT_EVENTS) && nBegin < h->contactCap) beginIds[nBegin++] = id;
         c.flags |= CF_TOUCHING;
         c.simFlags &= ~SIM_STARTED;
         addContactToGraph(w, id);
       } else if (simFlags & SIM_STOPPED) {
         c.simFlags &= ~SIM_STOPPED;
         c.flags &= ~CF_TOUCHING;
+        if ((c.flags & CF_CONTACT_EVENTS) && nEnd < h->contactCap) {
+          endPairs[2 * nEnd] = c.shapeA;
+          endPairs[2 * nEnd + 1] = c.shapeB;
+          ++nEnd;
+        }
         removeContactFromGraph(w, id);
       }
     }
+    h->beginCount = nBegin;
+    h->endCount = nEnd;
   }
   __syncthreads();
 }

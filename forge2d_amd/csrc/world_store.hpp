@@ -33,6 +33,8 @@ struct HostWorld {
   float contactSpeed = 0.0f, inv_h = 0.0f;
   int colorWords = 0;
   bool hasCallbacks = false;
+  std::vector<int> beginIds;       // contact begin-touch events of the last step (contact ids)
+  std::vector<int> endPairs;       // contact end-touch events of the last step (shape pairs)
 
   void ensureColorWords();
   int createBody(int type, Vec2 p, Rot q, Vec2 v, float w, float linDamp, float angDamp, float gravityScale,

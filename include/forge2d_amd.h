@@ -128,6 +128,19 @@ void f2d_create_revolute_joint(uint32_t w, int32_t a_index1, uint32_t a_world_an
                                float upper_angle, int enable_motor, float max_motor_torque, float motor_speed,
                                float draw_size, int collide_connected, int32_t* out);
 
+/* Events of the last step, F/native/wasm/f2d_shim.c:1186-1300 (counts first, then the
+ * fill functions write float64 records and return the record count):
+ *   begin 13 [aI1, aWg, bI1, bWg, nx, ny, pointCount, p0x, p0y, sep0, p1x, p1y, sep1]
+ *   end    4 [aI1, aWg, bI1, bWg]
+ *   hit    9 [aI1, aWg, bI1, bWg, px, py, nx, ny, approachSpeed]
+ *   body   7 [index1, world_and_generation, x, y, qc, qs, fellAsleep] */
+void f2d_world_get_contact_event_counts(uint32_t w, int32_t* out);
+int f2d_world_get_contact_begin_events(uint32_t w, double* out);
+int f2d_world_get_contact_end_events(uint32_t w, double* out);
+int f2d_world_get_contact_hit_events(uint32_t w, double* out);
+int f2d_world_get_body_event_count(uint32_t w);
+int f2d_world_get_body_events(uint32_t w, double* out);
+
 /* The batch handle behind a single f2d world (no counterpart in the reference):
  * lets the f2dg_* export/counter calls read a world created through f2d_*. */
 void* f2d_world_batch(uint32_t w);
@@ -184,6 +197,13 @@ int f2dg_get_block(void* batch);
 /* aggregate counts after f2dg_sync: [awake bodies, shapes, touching contacts, touching points,
  * restitution contacts, restitution points, joints, overflow contacts] */
 int f2dg_batch_stats(void* batch, long* out);
+/* events of one world's last step, in the f2d_world_get_*_events layouts above (shape and
+ * body world_and_generation fields are 0); counts out = [begin, end, hit] */
+int f2dg_contact_event_counts(void* batch, int world, int32_t* out);
+int f2dg_contact_begin_events(void* batch, int world, double* out);
+int f2dg_contact_end_events(void* batch, int world, double* out);
+int f2dg_contact_hit_events(void* batch, int world, double* out);
+int f2dg_body_events(void* batch, int world, double* out);
 /* [bodies, shapes, live contacts, touching, overflow, active colours, step, error bits] */
 int f2dg_world_counters(void* batch, int world, int32_t* out);
 

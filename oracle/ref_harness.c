@@ -19,6 +19,7 @@
 //   ref_harness time  <scene> <steps> <warmup> <threads> [first_world count]
 //   ref_harness pairs <scene> <out> <world> <steps>   (per-step created pair lists)
 //   ref_harness geom  <scene> <out> <world>           (shape geometry as built by the reference)
+//   ref_harness events <scene> <out> <world> <steps> <k1,k2,...>  (contact and body events after steps)
 
 #include "box2d/box2d.h"
 #include "body.h"
@@ -648,6 +649,73 @@ int main( int argc, char** argv )
 			}
 			fwrite( head, sizeof( head ), 1, f );
 			fwrite( rec, sizeof( rec ), 1, f );
+		}
+		fclose( f );
+		b2DestroyWorld( wid );
+		return 0;
+	}
+	if ( strcmp( argv[1], "events" ) == 0 && argc >= 7 )
+	{
+		// per listed step: int32 [step, begin, end, hit, move] counts, then float64 records
+		// in the layout of the flat shim's event getters (F/native/wasm/f2d_shim.c:1186-1300):
+		// begin 13 [aI1, 0, bI1, 0, nx, ny, pointCount, p0x, p0y, sep0, p1x, p1y, sep1],
+		// end 4 [aI1, 0, bI1, 0], hit 9 [aI1, 0, bI1, 0, px, py, nx, ny, approachSpeed],
+		// move 7 [bodyI1, 0, x, y, qc, qs, fellAsleep]
+		Scene sc = load_scene( argv[2] );
+		int wi = atoi( argv[4] );
+		int steps = atoi( argv[5] );
+		int dumps[4096];
+		int nd = parse_list( argv[6], dumps, 4096 );
+		const SceneWorld* sw = sc.worlds + wi;
+		b2BodyId* ids = malloc( sizeof( b2BodyId ) * ( sw->hdr.body_count + 1 ) );
+		b2WorldId wid = build_world( sw, ids );
+		FILE* f = fopen( argv[3], "wb" );
+		int di = 0;
+		for ( int s = 1; s <= steps; ++s )
+		{
+			b2World_Step( wid, 1.0f / 60.0f, 4 );
+			if ( di >= nd || dumps[di] != s )
+			{
+				continue;
+			}
+			di++;
+			b2ContactEvents ce = b2World_GetContactEvents( wid );
+			b2BodyEvents be = b2World_GetBodyEvents( wid );
+			int32_t counts[5] = { s, ce.beginCount, ce.endCount, ce.hitCount, be.moveCount };
+			fwrite( counts, sizeof( counts ), 1, f );
+			for ( int i = 0; i < ce.beginCount; ++i )
+			{
+				const b2ContactBeginTouchEvent* e = ce.beginEvents + i;
+				double r[13] = { e->shapeIdA.index1, 0, e->shapeIdB.index1, 0, e->manifold.normal.x, e->manifold.normal.y,
+								 e->manifold.pointCount };
+				for ( int p = 0; p < 2; ++p )
+				{
+					r[7 + 3 * p] = e->manifold.points[p].point.x;
+					r[8 + 3 * p] = e->manifold.points[p].point.y;
+					r[9 + 3 * p] = e->manifold.points[p].separation;
+				}
+				fwrite( r, sizeof( r ), 1, f );
+			}
+			for ( int i = 0; i < ce.endCount; ++i )
+			{
+				const b2ContactEndTouchEvent* e = ce.endEvents + i;
+				double r[4] = { e->shapeIdA.index1, 0, e->shapeIdB.index1, 0 };
+				fwrite( r, sizeof( r ), 1, f );
+			}
+			for ( int i = 0; i < ce.hitCount; ++i )
+			{
+				const b2ContactHitEvent* e = ce.hitEvents + i;
+				double r[9] = { e->shapeIdA.index1, 0, e->shapeIdB.index1, 0, e->point.x, e->point.y, e->normal.x,
+								e->normal.y, e->approachSpeed };
+				fwrite( r, sizeof( r ), 1, f );
+			}
+			for ( int i = 0; i < be.moveCount; ++i )
+			{
+				const b2BodyMoveEvent* e = be.moveEvents + i;
+				double r[7] = { e->bodyId.index1, 0, e->transform.p.x, e->transform.p.y, e->transform.q.c, e->transform.q.s,
+								e->fellAsleep };
+				fwrite( r, sizeof( r ), 1, f );
+			}
 		}
 		fclose( f );
 		b2DestroyWorld( wid );

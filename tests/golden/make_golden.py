@@ -10,6 +10,8 @@ after the listed steps, compressed, to ``tests/golden/<name>.npz``:
     steps  int32   the step index of each snapshot
     geoms  uint8   per shape, the reference's built geometry (148 B: int32 type,
                    count; float radius, centroid, 8 vertices, 8 normals)
+    events uint8   ("events" only) per step: int32 [step, begin, end, hit, move]
+                   counts, then float64 records in the flat shim's event layouts
 
 Usage: python tests/golden/make_golden.py [name ...]
 """
@@ -110,6 +112,32 @@ def ccd_world():
     return w
 
 
+def events_world():
+    """Bouncing circles, capsules and boxes with contact and hit events enabled on
+    every shape (S/world.c:630-676 begin/end events, S/solver.c:1758-1817 hits)."""
+    rng = scenes.SplitMix64(99)
+    w = scenes.World(scenes.world_header(sleep=False, continuous=True))
+    ev = scenes.SF_CONTACT_EVENTS | scenes.SF_HIT_EVENTS | scenes.SF_INVOKE_CONTACT | scenes.SF_UPDATE_MASS
+    g = w.add_body(scenes.STATIC, 0.0, -1.0)
+    w.add_box(g, 20.0, 1.0, restitution=0.5, flags=ev)
+    for i in range(20):
+        x = -9.5 + 1.0 * i
+        b = w.add_body(scenes.DYNAMIC, x, 1.0 + rng.uniform(0.0, 3.0), angle=rng.uniform(-1.0, 1.0),
+                       vx=rng.uniform(-2.0, 2.0))
+        k = i % 3
+        e = rng.uniform(0.4, 0.9)
+        if k == 0:
+            w.add_circle(b, 0.3, restitution=e, flags=ev)
+        elif k == 1:
+            w.add_capsule(b, (-0.25, 0.0), (0.25, 0.0), 0.15, restitution=e, flags=ev)
+        else:
+            w.add_box(b, 0.3, 0.3, restitution=e, flags=ev)
+    return w
+
+
+EVENT_STEPS = list(range(1, 121))
+
+
 FIXTURES = {
     # name: (world factory, dumped steps)
     "pyramid12": (lambda: scenes.pyramid_world(12), [0, 1, 2, 3, 4, 5, 60, 61, 62]),
@@ -120,6 +148,7 @@ FIXTURES = {
     "overflow": (overflow_world, [0, 1, 2, 3, 4, 20, 21, 22]),
     "bounce": (bounce_world, [0, 40, 41, 42, 45, 46, 47, 50, 51, 52, 60, 61, 62, 80, 81, 82]),
     "ccd": (ccd_world, list(range(0, 13)) + [30, 31, 32]),
+    "events": (events_world, list(range(0, 121))),
     "mixed_ccd": (lambda: scenes.container_world(200, 15.0, 40.0, 20261202, continuous=True, chains=3,
                                                  drop_band=(0.2, 1.0)),
                   [0, 1, 2, 3, 40, 41, 42, 43, 44, 45, 46, 47, 48, 49, 50, 60, 61, 62]),
@@ -142,9 +171,16 @@ def make(name: str):
         subprocess.run([HARNESS, "geom", sp, gp, "0"], check=True)
         with open(gp, "rb") as f:
             geoms = f.read()
+        extra = {}
+        if name == "events":
+            ep = os.path.join(td, "events.bin")
+            subprocess.run([HARNESS, "events", sp, ep, "0", str(max(EVENT_STEPS)), ",".join(map(str, EVENT_STEPS))],
+                           check=True)
+            with open(ep, "rb") as f:
+                extra["events"] = np.frombuffer(f.read(), np.uint8)
     path = os.path.join(HERE, name + ".npz")
     np.savez_compressed(path, spec=np.frombuffer(spec, np.uint8), snaps=np.frombuffer(snaps, np.uint8),
-                        steps=np.array(steps, np.int32), geoms=np.frombuffer(geoms, np.uint8))
+                        steps=np.array(steps, np.int32), geoms=np.frombuffer(geoms, np.uint8), **extra)
     print(f"{name}: {len(spec)} B spec, {len(snaps)} B snapshots -> {os.path.getsize(path)} B")
 
 
