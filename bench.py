@@ -127,6 +127,11 @@ def main():
     ap.add_argument("--serve-budget-gib", type=float, default=200.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain elementwise copy of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--kernel-profile", action="store_true",
+                    help="run only the per-kernel timing phase (size classes launched one after another, "
+                         "W + K steps) so a kernel trace of this command holds exactly those dispatches")
+    ap.add_argument("--lanes", default="",
+                    help="comma list of workgroup lanes per size class instead of the autotuner's choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: every world of a batch lives on a single device and worlds never "
@@ -153,15 +158,25 @@ def main():
     batch.upload()
     log(f"[bench] uploaded {W} worlds, {awake} awake bodies, {batch.device_bytes() / 2**30:.2f} GiB device")
 
+    if args.lanes:
+        for i, lanes in enumerate(int(x) for x in args.lanes.split(",")):
+            batch.set_class_block(i, lanes)
+    if args.kernel_profile:
+        kernel_profile(args, batch, awake, desc)
+        return
+
     batch.step(DT, SUBSTEPS, steps=args.warmup)
     batch.synchronize()
-    if batch.sync() != 0:
+    if batch.error_bits() != 0:
         raise RuntimeError("device reported a capacity error after warmup")
     st = batch.stats()
     log(f"[bench] after warmup: {st}")
-    block, tune_ms = batch.autotune(DT, SUBSTEPS, steps=4)
-    batch.synchronize()
-    log(f"[bench] autotuned workgroup size: {block} lanes per world (ms per 4 steps: {tune_ms})")
+    tune_ms = None
+    if not args.lanes:
+        _, tune_ms = batch.autotune(DT, SUBSTEPS, steps=4)
+        batch.synchronize()
+        log(f"[bench] autotuned workgroup lanes per size class: "
+            f"{[c['lanes'] for c in batch.size_classes()]} (whole-batch ms per 4 steps at one size: {tune_ms})")
 
     # timed region: K steps, synchronized on both sides
     batch.synchronize()
@@ -192,7 +207,7 @@ def main():
         batch.synchronize()
         lat.append(1e3 * (time.perf_counter() - a))
     p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
-    if batch.sync() != 0:
+    if batch.error_bits() != 0:
         raise RuntimeError("device reported a capacity error")
 
     # one-world-at-a-time GPU path on a sample of the same worlds
@@ -239,7 +254,7 @@ def main():
         "config": {"workload": f"{args.workload}: {desc}", "worlds": W, "awake_bodies": awake,
                    "substeps": SUBSTEPS, "dt": DT, "contacts": st["contacts"], "manifold_points": st["points"],
                    "joints": st["joints"], "size_classes": batch.size_classes(),
-                   "autotune_ms_per_4_steps": tune_ms,
+                   "autotune_ms_per_4_steps": tune_ms, "lanes_override": args.lanes or None,
                    "parallelism": "one workgroup per world, worlds launched in size classes (own LDS size and workgroup "
                                   "size, own stream), largest first; collide, solve and continuous-collision "
                                   "launches per class per step"},
@@ -255,6 +270,41 @@ def main():
                      "step_achieved_gbs": step_gbs},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def kernel_profile(args, batch, awake, desc):
+    """The kernel-timing phase alone: W + K steps with the size classes launched
+    one after another on one stream, each kernel bracketed by HIP events. Under
+    rocprofv3 every f2d dispatch of the run belongs to these W + K steps, so the
+    per-step kernel time is the summed duration of that kernel's dispatches
+    divided by W + K."""
+    warm = batch.kernel_times(DT, SUBSTEPS, steps=args.warmup) if args.warmup else {}
+    if batch.error_bits() != 0:
+        raise RuntimeError("device reported a capacity error")
+    st = batch.stats()
+    launch_ms = {k: v / args.steps for k, v in batch.kernel_times(DT, SUBSTEPS, steps=args.steps).items()}
+    if batch.error_bits() != 0:
+        raise RuntimeError("device reported a capacity error")
+    nbytes = algorithmic_bytes(st)
+    dominant = max(launch_ms, key=launch_ms.get)
+    achieved = nbytes[dominant] / (launch_ms[dominant] * 1e-3) / 1e9
+    classes = batch.size_classes()
+    pmc = load_pmc(args.workload)
+    out = {
+        "metric": METRIC, "mode": "kernel-profile", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "dtype": "f32", "config": {"workload": f"{args.workload}: {desc}", "awake_bodies": awake,
+                                   "size_classes": classes},
+        "kernel_ms_per_step": launch_ms,
+        "warmup_kernel_ms": warm,
+        "dispatches_per_kernel": {k: len(classes) * (args.steps + args.warmup) for k in launch_ms},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": pmc.get(dominant, {}).get("hbm_bytes_per_launch") if pmc else None,
+                     "kernel": dominant, "launch_ms": launch_ms[dominant],
+                     "algorithmic_bytes_per_launch": nbytes[dominant]},
+    }
+    log(f"[bench] kernel profile: {launch_ms} ms per step, {dominant} {achieved:.0f} GB/s")
     print(json.dumps(out), flush=True)
 
 

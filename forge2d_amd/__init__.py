@@ -56,6 +56,8 @@ SIGNATURES = [
     ("f2dg_autotune", _I, [_VP, _F, _I, _I, _PF]),
     ("f2dg_set_block", _I, [_VP, _I]),
     ("f2dg_get_block", _I, [_VP]),
+    ("f2dg_set_class_block", _I, [_VP, _I, _I]),
+    ("f2dg_error_bits", _I, [_VP]),
     ("f2dg_size_classes", _I, [_VP, _PI, _I]),
     ("f2dg_contact_event_counts", _I, [_VP, _I, _PI]),
     ("f2dg_contact_begin_events", _I, [_VP, _I, _PD]),
@@ -200,6 +202,13 @@ class Batch:
             raise RuntimeError("sync failed")
         return rc
 
+    def error_bits(self) -> int:
+        """OR of the worlds' device error bits, from the world headers alone."""
+        rc = lib().f2dg_error_bits(self.h)
+        if rc < 0:
+            raise RuntimeError("error_bits failed")
+        return rc
+
     def bodies(self, world: int) -> np.ndarray:
         n = lib().f2dg_body_count(self.h, world)
         out = np.zeros((n, 7), np.float32)
@@ -236,6 +245,9 @@ class Batch:
 
     def set_block(self, block: int):
         self._check(lib().f2dg_set_block(self.h, block), "set_block")
+
+    def set_class_block(self, cls: int, block: int):
+        self._check(lib().f2dg_set_class_block(self.h, cls, block), "set_class_block")
 
     @property
     def block(self) -> int:

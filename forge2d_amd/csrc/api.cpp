@@ -1425,6 +1425,20 @@ int f2dg_sync(void* h) {
   return err;
 }
 
+// OR of the worlds' device error bits, reading only the world headers (no mirror update).
+int f2dg_error_bits(void* h) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return -1;
+  if (!b->committed) return 0;
+  std::vector<WorldHeader> headers(b->worlds.size());
+  F2D_HIP(hipMemcpyAsync(headers.data(), b->dev.view.worlds, sizeof(WorldHeader) * headers.size(),
+                         hipMemcpyDeviceToHost, b->stream));
+  F2D_HIP(hipStreamSynchronize(b->stream));
+  int err = 0;
+  for (const WorldHeader& w : headers) err |= w.error;
+  return err;
+}
+
 // Per-body [px, py, qc, qs, vx, vy, w] of one world (after f2dg_sync).
 int f2dg_get_bodies(void* h, int world, float* out) {
   Batch* b = static_cast<Batch*>(h);
@@ -1720,6 +1734,15 @@ int f2dg_set_block(void* h, int block) {
   if (!b || (block != 64 && block != 128 && block != 256)) return -1;
   b->launch.block = block;
   for (SizeClass& c : b->classes) c.launch.block = block;
+  return 0;
+}
+
+// Force the workgroup size of one size class (classes as f2dg_size_classes lists them).
+int f2dg_set_class_block(void* h, int cls, int block) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || (block != 64 && block != 128 && block != 256)) return -1;
+  if (cls < 0 || cls >= (int)b->classes.size()) return -1;
+  b->classes[cls].launch.block = block;
   return 0;
 }
 

@@ -41,7 +41,7 @@ enum : int {
 // contact flags (b2ContactFlags, S/contact.h:17-40) and sim flags (:66-86)
 enum : int {
   CF_TOUCHING = 0x1,
-  CF_CONTACT_EVENTS = 0x2,
+  CF_CONTACT_EVENTS = 0x4,  // b2_contactEnableContactEvents (0x2 is the unused hit-event flag)
   SIM_TOUCHING = 0x00010000,
   SIM_DISJOINT = 0x00020000,
   SIM_STARTED = 0x00040000,

@@ -176,6 +176,8 @@ int f2dg_kernel_times(void* batch, float dt, int sub_steps, int steps, float* ms
 int f2dg_synchronize(void* batch);
 /* copy device state to the host mirror; returns OR of world error bits, <0 on failure */
 int f2dg_sync(void* batch);
+/* OR of the world error bits read from the device world headers alone (no mirror copy) */
+int f2dg_error_bits(void* batch);
 /* per body: px, py, qc, qs, vx, vy, w */
 int f2dg_get_bodies(void* batch, int world, float* out);
 /* world snapshot (forge2d_amd/snapshot.py layout); returns the size, writes when cap suffices */
@@ -194,6 +196,8 @@ int f2dg_size_classes(void* batch, int32_t* out, int cap);
 /* force a workgroup size (64, 128 or 256) / read the current one */
 int f2dg_set_block(void* batch, int block);
 int f2dg_get_block(void* batch);
+/* force the workgroup size of size class `cls` (index into f2dg_size_classes) */
+int f2dg_set_class_block(void* batch, int cls, int block);
 /* aggregate counts after f2dg_sync: [awake bodies, shapes, touching contacts, touching points,
  * restitution contacts, restitution points, joints, overflow contacts] */
 int f2dg_batch_stats(void* batch, long* out);
