@@ -12,6 +12,15 @@ Records (little-endian, packed):
   HEADER 64 B, BODY 112 B, SHAPE 48 B, CONTACT 168 B, JOINT 64 B, then
   int32 colour counts[12], int32 colour ids[sum], int32 move keys[n],
   int32 free contact ids[n].
+Version 2 appends the island section (island graph and solver sets):
+  int32 header[16] = [island array count, free island ids, awake islands,
+  split island id, solver-set array count, free set ids, awake bodies, island
+  id-pool next index, set id-pool next index, 0...]; ISLAND 64 B records; int32
+  free island ids; int32 awake island ids (awake set order); int32 free set ids;
+  int32 awake body ids (awake set order); per body int32[4] (island id, prev,
+  next, head contact key); per contact slot int32[8] (island id, prev, next, set
+  index, body-list edge keys prevA, nextA, prevB, nextB); per joint int32[4]
+  (island id, prev, next, set index).
 """
 from __future__ import annotations
 
@@ -57,6 +66,17 @@ JOINT_DT = np.dtype([
     ("motor_impulse", "<f4"), ("lower_impulse", "<f4"), ("upper_impulse", "<f4"), ("ref_angle", "<f4"),
     ("reserved", "<f4"),
 ])
+ISLAND_DT = np.dtype([
+    ("id", "<i4"), ("set_index", "<i4"), ("local_index", "<i4"), ("head_body", "<i4"), ("tail_body", "<i4"),
+    ("body_count", "<i4"), ("head_contact", "<i4"), ("tail_contact", "<i4"), ("contact_count", "<i4"),
+    ("head_joint", "<i4"), ("tail_joint", "<i4"), ("joint_count", "<i4"), ("parent", "<i4"),
+    ("remove_count", "<i4"), ("reserved", "<i4", (2,)),
+])
+LINK_DT = np.dtype([("island", "<i4"), ("prev", "<i4"), ("next", "<i4"), ("set_index", "<i4")])
+BODY_LINK_DT = np.dtype([("island", "<i4"), ("prev", "<i4"), ("next", "<i4"), ("head_contact", "<i4")])
+CONTACT_LINK_DT = np.dtype([("island", "<i4"), ("prev", "<i4"), ("next", "<i4"), ("set_index", "<i4"),
+                            ("prev_a", "<i4"), ("next_a", "<i4"), ("prev_b", "<i4"), ("next_b", "<i4")])
+assert ISLAND_DT.itemsize == 64 and LINK_DT.itemsize == 16 and CONTACT_LINK_DT.itemsize == 32
 assert HEADER_DT.itemsize == 64 and BODY_DT.itemsize == 112 and SHAPE_DT.itemsize == 48
 assert POINT_DT.itemsize == 52 and CONTACT_DT.itemsize == 168 and JOINT_DT.itemsize == 64
 
@@ -65,8 +85,16 @@ CONTACT_TOUCHING = 0x1
 SIM_TOUCHING = 0x00010000
 
 
+class Islands:
+    """Island graph and solver-set bookkeeping of a version-2 snapshot."""
+    __slots__ = ("island_count", "awake_island_count", "split_island", "set_count", "awake_body_count",
+                 "island_next", "set_next", "islands", "free_islands", "awake_islands", "free_sets", "awake_bodies",
+                 "body_links", "contact_links", "joint_links")
+
+
 class Snapshot:
-    __slots__ = ("header", "bodies", "shapes", "contacts", "joints", "colors", "move", "free_ids", "nbytes")
+    __slots__ = ("header", "bodies", "shapes", "contacts", "joints", "colors", "move", "free_ids", "islands",
+                 "nbytes")
 
     @property
     def step(self) -> int:
@@ -113,8 +141,32 @@ def parse_one(buf, off: int = 0) -> Snapshot:
     o += s.move.nbytes
     s.free_ids = np.frombuffer(buf, "<i4", int(h["free_id_count"]), o)
     o += s.free_ids.nbytes
+    s.islands = None
+    if int(h["version"]) >= 2:
+        s.islands, o = _parse_islands(buf, o, len(s.bodies), len(s.contacts), len(s.joints))
     s.nbytes = o - off
     return s
+
+
+def _parse_islands(buf, o, nb, nc, nj):
+    hd = np.frombuffer(buf, "<i4", 16, o)
+    o += 64
+    isl = Islands()
+    isl.island_count, nfree, isl.awake_island_count, isl.split_island = (int(x) for x in hd[:4])
+    isl.set_count, nsetfree, isl.awake_body_count, isl.island_next, isl.set_next = (int(x) for x in hd[4:9])
+    isl.islands = np.frombuffer(buf, ISLAND_DT, isl.island_count, o)
+    o += isl.islands.nbytes
+    for name, n in (("free_islands", nfree), ("awake_islands", isl.awake_island_count), ("free_sets", nsetfree),
+                    ("awake_bodies", isl.awake_body_count)):
+        a = np.frombuffer(buf, "<i4", n, o)
+        o += a.nbytes
+        setattr(isl, name, a)
+    for name, n, dt in (("body_links", nb, BODY_LINK_DT), ("contact_links", nc, CONTACT_LINK_DT),
+                        ("joint_links", nj, LINK_DT)):
+        a = np.frombuffer(buf, dt, n, o)
+        o += a.nbytes
+        setattr(isl, name, a)
+    return isl, o
 
 
 def parse_all(buf) -> list[Snapshot]:

@@ -688,6 +688,9 @@ typedef struct {
   R2 dq;
   real invMass, invI, minExtent, maxExtent, linDamp, angDamp, gravityScale, sleepTime, sleepThreshold;
   int headShape;
+  int enableSleep;
+  /* island graph (S/body.h:40-71) */
+  int islandId, islandPrev, islandNext, headContactKey, headJointKey, isMarked;
 } OBody;
 
 typedef struct {
@@ -704,6 +707,9 @@ typedef struct {
   int flags, simFlags, color, localIndex, inState;
   real friction, restitution, rolling, tangentSpeed;
   MF m;
+  /* body-list edges and island links (S/contact.h:42-70) */
+  int bodyA, bodyB, prevA, nextA, prevB, nextB;
+  int islandId, islandPrev, islandNext, setIndex, isMarked;
 } OContact;
 
 typedef struct {
@@ -719,7 +725,16 @@ typedef struct {
   real mA, mB, iA, iB, deltaAngle, axialMass;
   Soft constraintSoftness, springSoftness;
   V2 anchorA, anchorB, deltaCenter;
+  /* body-list edges and island links (S/joint.h:27-60) */
+  int prevA, nextA, prevB, nextB;
+  int islandId, islandPrev, islandNext, setIndex, isMarked;
 } OJoint;
+
+/* b2Island, S/island.h:27-58 (record layout of the snapshot's island section) */
+typedef struct {
+  int id, setIndex, localIndex, headBody, tailBody, bodyCount, headContact, tailContact, contactCount, headJoint,
+      tailJoint, jointCount, parent, removeCount, r0, r1;
+} OIsland;
 
 typedef struct {
   int step, bodyCount, shapeCount, contactCap, jointCount;
@@ -741,6 +756,20 @@ typedef struct {
   real restitutionThreshold, contactHertz, contactDampingRatio, maxContactPushSpeed, maxLinearSpeed;
   int enableSleep, enableContinuous, enableWarmStarting;
   real contactSpeed, inv_h;
+  /* islands and solver sets (S/island.c, S/solver_set.c): present for version-2 snapshots */
+  int hasIslands;
+  OIsland* islands;
+  int islandCount;                 /* island array count = island id-pool next index */
+  int* islandFree;
+  int islandFreeCount;
+  int* awakeIslands;               /* awake set islandSims order */
+  int awakeIslandCount;
+  int* setFree;
+  int setFreeCount, setCount;      /* solver-set id pool (free list, next index) */
+  int* awakeBodies;                /* awake set bodySims order */
+  int awakeBodyCount;
+  int splitIslandId;
+  int islandCap;
 } OWorld;
 
 /* ---- scene spec and snapshot records (forge2d_amd/scenes.py, forge2d_amd/snapshot.py) */
@@ -788,6 +817,9 @@ typedef struct {
   float lix, liy, spring, motor, lower, upper, ref, r;
 } SnapJ;
 typedef struct { int32_t type, count; float radius, cx, cy, v[16], n[16]; } GeoRec;
+typedef struct { int32_t islandId, prev, next, headContact; } SnapBL;
+typedef struct { int32_t islandId, prev, next, setIndex, prevA, nextA, prevB, nextB; } SnapCL;
+typedef struct { int32_t islandId, prev, next, setIndex; } SnapJL;
 #pragma pack(pop)
 
 #define SNAP_MAGIC 0x50414E53
@@ -796,6 +828,11 @@ typedef struct { int32_t type, count; float radius, cx, cy, v[16], n[16]; } GeoR
 #define BF_BULLET 1
 #define BF_FIXED_ROT 2
 #define BF_FAST_ROT 4
+#define BF_ENABLE_SLEEP 8
+#define STATIC_SET 0      /* b2_staticSet, b2_disabledSet, b2_awakeSet, b2_firstSleepingSet (S/solver_set.h) */
+#define DISABLED_SET 1
+#define FIRST_SLEEPING_SET 3
+#define TIME_TO_SLEEP ((real)0.5f)  /* B2_TIME_TO_SLEEP, S/constants.h:47 */
 
 static void* xcalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
 
@@ -814,6 +851,11 @@ void b2o_destroy(void* h) {
   free(w->freeIds);
   free(w->move);
   free(w->simOf);
+  free(w->islands);
+  free(w->islandFree);
+  free(w->awakeIslands);
+  free(w->setFree);
+  free(w->awakeBodies);
   free(w);
 }
 
@@ -853,6 +895,26 @@ void* b2o_create(const void* spec, long specBytes, int world, const void* geoms,
   const int32_t* moves = colorIds + hh->colorTotal;
   const int32_t* frees = moves + hh->moveCount;
   if ((const uint8_t*)(frees + hh->freeCount) > snp + snapBytes) return NULL;
+  /* version 2: island section (forge2d_amd/snapshot.py) */
+  const int32_t* ih = frees + hh->freeCount;
+  const OIsland* isRecs = NULL;
+  const int32_t *isFree = NULL, *isAwake = NULL, *setFree = NULL, *awakeBodies = NULL;
+  const SnapBL* bl = NULL;
+  const SnapCL* cl = NULL;
+  const SnapJL* jl = NULL;
+  if (hh->version >= 2) {
+    if ((const uint8_t*)(ih + 16) > snp + snapBytes) return NULL;
+    isRecs = (const OIsland*)(ih + 16);
+    isFree = (const int32_t*)(isRecs + ih[0]);
+    isAwake = isFree + ih[1];
+    setFree = isAwake + ih[2];
+    awakeBodies = setFree + ih[5];
+    bl = (const SnapBL*)(awakeBodies + ih[6]);
+    cl = (const SnapCL*)(bl + hh->bodyCount);
+    jl = (const SnapJL*)(cl + hh->contactCap);
+    if ((const uint8_t*)(jl + hh->jointCount) > snp + snapBytes) return NULL;
+    if (ih[0] < 0 || ih[1] < 0 || ih[2] < 0 || ih[5] < 0 || ih[6] < 0 || ih[6] > hh->bodyCount) return NULL;
+  }
 
   OWorld* w = (OWorld*)xcalloc(1, sizeof(OWorld));
   w->step = hh->step;
@@ -905,8 +967,16 @@ void* b2o_create(const void* spec, long specBytes, int world, const void* geoms,
     b->angDamp = sb[i].ad;
     b->gravityScale = sb[i].gs;
     b->sleepThreshold = sb[i].st;
+    b->enableSleep = (sb[i].flags & BF_ENABLE_SLEEP) != 0;
     b->dq.c = 1;
     b->headShape = -1;
+    b->islandId = b->islandPrev = b->islandNext = b->headContactKey = b->headJointKey = -1;
+    if (bl) {
+      b->islandId = bl[i].islandId;
+      b->islandPrev = bl[i].prev;
+      b->islandNext = bl[i].next;
+      b->headContactKey = bl[i].headContact;
+    }
   }
   const GeoRec* gr = (const GeoRec*)geoms;
   for (int i = 0; i < w->shapeCount; ++i) {
@@ -943,6 +1013,8 @@ void* b2o_create(const void* spec, long specBytes, int world, const void* geoms,
     w->contacts[i].shapeA = -1;
     w->contacts[i].shapeB = -1;
     w->contacts[i].color = -1;
+    w->contacts[i].islandId = w->contacts[i].islandPrev = w->contacts[i].islandNext = -1;
+    w->contacts[i].setIndex = -1;
   }
   for (int i = 0; i < hh->contactCap; ++i) {
     const SnapC* r = nc + i;
@@ -958,6 +1030,20 @@ void* b2o_create(const void* spec, long specBytes, int world, const void* geoms,
     c->restitution = r->restitution;
     c->rolling = r->rolling;
     c->tangentSpeed = r->tangentSpeed;
+    c->bodyA = ss[c->shapeA].body;
+    c->bodyB = ss[c->shapeB].body;
+    c->setIndex = AWAKE_SET;
+    c->prevA = c->nextA = c->prevB = c->nextB = -1;
+    if (cl) {
+      c->islandId = cl[i].islandId;
+      c->islandPrev = cl[i].prev;
+      c->islandNext = cl[i].next;
+      c->setIndex = cl[i].setIndex;
+      c->prevA = cl[i].prevA;
+      c->nextA = cl[i].nextA;
+      c->prevB = cl[i].prevB;
+      c->nextB = cl[i].nextB;
+    }
     c->m.normal = v2f(r->nx, r->ny);
     c->m.rollingImpulse = r->rollingImpulse;
     c->m.pointCount = r->pointCount;
@@ -1029,6 +1115,34 @@ void* b2o_create(const void* spec, long specBytes, int world, const void* geoms,
     /* b2CreateJointInternal, S/joint.c:289-290 */
     J->constraintHertz = (real)60.0f;
     J->constraintDampingRatio = (real)2.0f;
+    /* body joint lists: b2CreateJoint prepends joint j to both bodies' lists (S/joint.c:175-201) */
+    {
+      int keyA = j << 1, keyB = (j << 1) | 1;
+      OBody* A = w->bodies + J->bodyA;
+      OBody* B = w->bodies + J->bodyB;
+      J->prevA = -1;
+      J->nextA = A->headJointKey;
+      if (A->headJointKey >= 0) {
+        OJoint* hj = w->joints + (A->headJointKey >> 1);
+        if (A->headJointKey & 1) hj->prevB = keyA; else hj->prevA = keyA;
+      }
+      A->headJointKey = keyA;
+      J->prevB = -1;
+      J->nextB = B->headJointKey;
+      if (B->headJointKey >= 0) {
+        OJoint* hj = w->joints + (B->headJointKey >> 1);
+        if (B->headJointKey & 1) hj->prevB = keyB; else hj->prevA = keyB;
+      }
+      B->headJointKey = keyB;
+    }
+    J->islandId = J->islandPrev = J->islandNext = -1;
+    J->setIndex = AWAKE_SET;
+    if (jl) {
+      J->islandId = jl[j].islandId;
+      J->islandPrev = jl[j].prev;
+      J->islandNext = jl[j].next;
+      J->setIndex = jl[j].setIndex;
+    }
     J->linearImpulse = v2f(r->lix, r->liy);
     J->springImpulse = r->spring;
     J->motorImpulse = r->motor;
@@ -1091,6 +1205,32 @@ void* b2o_create(const void* spec, long specBytes, int world, const void* geoms,
   w->moveCount = 0;
   w->contactSpeed = hh->contactSpeed;
   w->inv_h = hh->inv_h;
+  w->splitIslandId = -1;
+  w->awakeBodies = (int*)xcalloc(w->bodyCount, sizeof(int));
+  if (isRecs) {
+    w->hasIslands = 1;
+    w->islandCount = ih[0];
+    w->islandCap = ih[0] > w->bodyCount ? ih[0] : w->bodyCount;
+    w->islands = (OIsland*)xcalloc(w->islandCap, sizeof(OIsland));
+    memcpy(w->islands, isRecs, sizeof(OIsland) * (size_t)ih[0]);
+    w->islandFree = (int*)xcalloc(w->islandCap, sizeof(int));
+    memcpy(w->islandFree, isFree, sizeof(int) * (size_t)ih[1]);
+    w->islandFreeCount = ih[1];
+    w->awakeIslands = (int*)xcalloc(w->islandCap, sizeof(int));
+    memcpy(w->awakeIslands, isAwake, sizeof(int) * (size_t)ih[2]);
+    w->awakeIslandCount = ih[2];
+    w->splitIslandId = ih[3];
+    w->setCount = ih[4];
+    w->setFree = (int*)xcalloc(w->islandCap + 8, sizeof(int));
+    memcpy(w->setFree, setFree, sizeof(int) * (size_t)ih[5]);
+    w->setFreeCount = ih[5];
+    memcpy(w->awakeBodies, awakeBodies, sizeof(int) * (size_t)ih[6]);
+    w->awakeBodyCount = ih[6];
+  } else {
+    for (int i = 0; i < w->bodyCount; ++i)
+      if (w->simOf[i] >= 0) w->awakeBodies[w->simOf[i]] = i;
+    for (int i = 0; i < w->bodyCount; ++i) w->awakeBodyCount += w->simOf[i] >= 0;
+  }
   return w;
 }
 
@@ -1153,6 +1293,8 @@ static real shape_radius(const GEO* g) {
   return (g->type == SHAPE_CAPSULE || g->type == SHAPE_CIRCLE || g->type == SHAPE_POLYGON) ? g->radius : 0;
 }
 
+static void body_list_add(OWorld* w, int cid);
+
 int b2o_create_contacts(void* h, const int32_t* ids, const int32_t* shapeA, const int32_t* shapeB, int n) {
   OWorld* w = (OWorld*)h;
   for (int k = 0; k < n; ++k) {
@@ -1186,6 +1328,14 @@ int b2o_create_contacts(void* h, const int32_t* ids, const int32_t* shapeA, cons
     const OShape* b = w->shapes + c->shapeB;
     c->friction = SQRT(a->friction * b->friction);
     c->restitution = maxr(a->restitution, b->restitution);
+    c->bodyA = a->body;
+    c->bodyB = b->body;
+    c->islandId = c->islandPrev = c->islandNext = -1;
+    /* awake set if either body is awake, else the disabled set (S/contact.c:213-224) */
+    c->setIndex = (w->bodies[c->bodyA].setIndex == AWAKE_SET || w->bodies[c->bodyB].setIndex == AWAKE_SET)
+                      ? AWAKE_SET
+                      : DISABLED_SET;
+    if (w->hasIslands) body_list_add(w, id);
   }
   return 0;
 }
@@ -1298,6 +1448,422 @@ static void remove_from_graph(OWorld* w, int id) {
   c->localIndex = -1;
 }
 
+/* ------------------------------------------------------------------ islands and sleep */
+
+/* b2AllocId / b2FreeId, S/id_pool.c:19-46: LIFO free list, else the next fresh id */
+static int pool_alloc(int* freeArr, int* freeCount, int* next) {
+  if (*freeCount > 0) return freeArr[--*freeCount];
+  return (*next)++;
+}
+
+/* b2AssignJointColor + b2CreateJointInGraph, S/constraint_graph.c:214-290 */
+static void add_joint_to_graph(OWorld* w, int j) {
+  OJoint* J = w->joints + j;
+  int ba = J->bodyA, bb = J->bodyB;
+  int staticA = w->bodies[ba].setIndex == STATIC_SET, staticB = w->bodies[bb].setIndex == STATIC_SET;
+  int color = OVERFLOW;
+  for (int i = 0; i < OVERFLOW; ++i) {
+    if (!staticA && !staticB) {
+      if (w->bodySet[i][ba] || w->bodySet[i][bb]) continue;
+      w->bodySet[i][ba] = w->bodySet[i][bb] = 1;
+    } else if (!staticA) {
+      if (w->bodySet[i][ba]) continue;
+      w->bodySet[i][ba] = 1;
+    } else {
+      if (w->bodySet[i][bb]) continue;
+      w->bodySet[i][bb] = 1;
+    }
+    color = i;
+    break;
+  }
+  J->color = color;
+  J->localIndex = w->jointColorCount[color];
+  w->jointColors[color][w->jointColorCount[color]++] = j;
+}
+
+/* b2RemoveJointFromGraph, S/constraint_graph.c:292-322 (swap-remove) */
+static void remove_joint_from_graph(OWorld* w, int j) {
+  OJoint* J = w->joints + j;
+  int color = J->color, local = J->localIndex;
+  if (color < 0 || local < 0 || local >= w->jointColorCount[color] || w->jointColors[color][local] != j) abort();
+  if (color != OVERFLOW) w->bodySet[color][J->bodyA] = w->bodySet[color][J->bodyB] = 0;
+  int last = --w->jointColorCount[color];
+  if (local != last) {
+    int moved = w->jointColors[color][last];
+    w->jointColors[color][local] = moved;
+    w->joints[moved].localIndex = local;
+  }
+  J->color = -1;
+  J->localIndex = -1;
+}
+
+/* b2CreateIsland, S/island.c:18-55 (awake set) */
+static int island_create(OWorld* w) {
+  int id = pool_alloc(w->islandFree, &w->islandFreeCount, &w->islandCount);
+  if (id >= w->islandCap) abort();
+  OIsland* is = w->islands + id;
+  is->id = id;
+  is->setIndex = AWAKE_SET;
+  is->localIndex = w->awakeIslandCount;
+  is->headBody = is->tailBody = -1;
+  is->bodyCount = 0;
+  is->headContact = is->tailContact = -1;
+  is->contactCount = 0;
+  is->headJoint = is->tailJoint = -1;
+  is->jointCount = 0;
+  is->parent = -1;
+  is->removeCount = 0;
+  w->awakeIslands[w->awakeIslandCount++] = id;
+  return id;
+}
+
+/* b2DestroyIsland, S/island.c:57-83: remove-swap from the awake set's island list;
+   the record keeps its other fields */
+static void island_destroy(OWorld* w, int id) {
+  if (w->splitIslandId == id) w->splitIslandId = -1;
+  OIsland* is = w->islands + id;
+  if (is->setIndex != AWAKE_SET) abort();
+  int li = is->localIndex, last = --w->awakeIslandCount;
+  if (li != last) {
+    int moved = w->awakeIslands[last];
+    w->awakeIslands[li] = moved;
+    w->islands[moved].localIndex = li;
+  }
+  is->id = -1;
+  is->setIndex = -1;
+  is->localIndex = -1;
+  w->islandFree[w->islandFreeCount++] = id;
+}
+
+/* b2AddContactToIsland, S/island.c:85-112: prepend */
+static void island_add_contact(OWorld* w, int islandId, int cid) {
+  OIsland* is = w->islands + islandId;
+  OContact* c = w->contacts + cid;
+  if (is->headContact != -1) {
+    c->islandNext = is->headContact;
+    w->contacts[is->headContact].islandPrev = cid;
+  }
+  is->headContact = cid;
+  if (is->tailContact == -1) is->tailContact = is->headContact;
+  is->contactCount += 1;
+  c->islandId = islandId;
+}
+
+/* union-find root with path halving, as b2LinkContact walks it (S/island.c:150-196) */
+static int island_root(OWorld* w, int id) {
+  OIsland* is = w->islands + id;
+  while (is->parent != -1) {
+    OIsland* parent = w->islands + is->parent;
+    int pid = is->parent;
+    if (parent->parent != -1) is->parent = parent->parent;
+    is = parent;
+    id = pid;
+  }
+  return id;
+}
+
+static void wake_island_set(OWorld* w, int setIndex);
+
+/* b2LinkContact, S/island.c:117-218 */
+static void link_contact(OWorld* w, int cid) {
+  OContact* c = w->contacts + cid;
+  OBody* A = w->bodies + c->bodyA;
+  OBody* B = w->bodies + c->bodyB;
+  if (A->setIndex == AWAKE_SET && B->setIndex >= FIRST_SLEEPING_SET) wake_island_set(w, B->setIndex);
+  if (B->setIndex == AWAKE_SET && A->setIndex >= FIRST_SLEEPING_SET) wake_island_set(w, A->setIndex);
+  int iA = A->islandId, iB = B->islandId;
+  if (iA == iB) {
+    island_add_contact(w, iA, cid);
+    return;
+  }
+  int rA = iA != -1 ? island_root(w, iA) : -1;
+  int rB = iB != -1 ? island_root(w, iB) : -1;
+  if (rA != -1 && rB != -1 && rA != rB) w->islands[rB].parent = rA;
+  island_add_contact(w, rA != -1 ? rA : rB, cid);
+}
+
+/* b2UnlinkContact, S/island.c:221-262 */
+static void unlink_contact(OWorld* w, int cid) {
+  OContact* c = w->contacts + cid;
+  OIsland* is = w->islands + c->islandId;
+  if (c->islandPrev != -1) w->contacts[c->islandPrev].islandNext = c->islandNext;
+  if (c->islandNext != -1) w->contacts[c->islandNext].islandPrev = c->islandPrev;
+  if (is->headContact == cid) is->headContact = c->islandNext;
+  if (is->tailContact == cid) is->tailContact = c->islandPrev;
+  is->contactCount -= 1;
+  is->removeCount += 1;
+  c->islandId = c->islandPrev = c->islandNext = -1;
+}
+
+/* b2MergeIsland, S/island.c:446-531 */
+static void merge_island(OWorld* w, int id) {
+  OIsland* is = w->islands + id;
+  int rootId = is->parent;
+  OIsland* root = w->islands + rootId;
+  for (int b = is->headBody; b != -1; b = w->bodies[b].islandNext) w->bodies[b].islandId = rootId;
+  for (int c = is->headContact; c != -1; c = w->contacts[c].islandNext) w->contacts[c].islandId = rootId;
+  for (int j = is->headJoint; j != -1; j = w->joints[j].islandNext) w->joints[j].islandId = rootId;
+  w->bodies[root->tailBody].islandNext = is->headBody;
+  w->bodies[is->headBody].islandPrev = root->tailBody;
+  root->tailBody = is->tailBody;
+  root->bodyCount += is->bodyCount;
+  if (root->headContact == -1) {
+    root->headContact = is->headContact;
+    root->tailContact = is->tailContact;
+    root->contactCount = is->contactCount;
+  } else if (is->headContact != -1) {
+    w->contacts[root->tailContact].islandNext = is->headContact;
+    w->contacts[is->headContact].islandPrev = root->tailContact;
+    root->tailContact = is->tailContact;
+    root->contactCount += is->contactCount;
+  }
+  if (root->headJoint == -1) {
+    root->headJoint = is->headJoint;
+    root->tailJoint = is->tailJoint;
+    root->jointCount = is->jointCount;
+  } else if (is->headJoint != -1) {
+    w->joints[root->tailJoint].islandNext = is->headJoint;
+    w->joints[is->headJoint].islandPrev = root->tailJoint;
+    root->tailJoint = is->tailJoint;
+    root->jointCount += is->jointCount;
+  }
+  root->removeCount += is->removeCount;
+}
+
+/* b2MergeAwakeIslands, S/island.c:537-592 */
+static void merge_awake_islands(OWorld* w) {
+  for (int i = 0; i < w->awakeIslandCount; ++i) {
+    int id = w->awakeIslands[i];
+    int root = island_root(w, id);
+    if (root != id) w->islands[id].parent = root;
+  }
+  for (int i = w->awakeIslandCount - 1; i >= 0; --i) {
+    int id = w->awakeIslands[i];
+    if (w->islands[id].parent == -1) continue;
+    merge_island(w, id);
+    island_destroy(w, id);
+  }
+}
+
+/* b2SplitIsland, S/island.c:596-834: depth-first search from every body of the base island */
+static void split_island(OWorld* w, int baseId) {
+  OIsland* base = w->islands + baseId;
+  if (base->setIndex != AWAKE_SET || base->removeCount == 0) return;
+  int n = base->bodyCount;
+  int* stack = (int*)xcalloc(n, sizeof(int));
+  int* seeds = (int*)xcalloc(n, sizeof(int));
+  int k = 0;
+  for (int b = base->headBody; b != -1; b = w->bodies[b].islandNext) {
+    seeds[k++] = b;
+    w->bodies[b].isMarked = 0;
+  }
+  for (int c = base->headContact; c != -1; c = w->contacts[c].islandNext) w->contacts[c].isMarked = 0;
+  for (int j = base->headJoint; j != -1; j = w->joints[j].islandNext) w->joints[j].isMarked = 0;
+  island_destroy(w, baseId);
+  for (int i = 0; i < n; ++i) {
+    int seed = seeds[i];
+    if (w->bodies[seed].isMarked) continue;
+    int top = 0;
+    stack[top++] = seed;
+    w->bodies[seed].isMarked = 1;
+    int id = island_create(w);
+    OIsland* is = w->islands + id;
+    while (top > 0) {
+      int bid = stack[--top];
+      OBody* body = w->bodies + bid;
+      body->islandId = id;
+      if (is->tailBody != -1) w->bodies[is->tailBody].islandNext = bid;
+      body->islandPrev = is->tailBody;
+      body->islandNext = -1;
+      is->tailBody = bid;
+      if (is->headBody == -1) is->headBody = bid;
+      is->bodyCount += 1;
+      for (int key = body->headContactKey; key != -1;) {
+        int cid = key >> 1, edge = key & 1;
+        OContact* c = w->contacts + cid;
+        key = edge ? c->nextB : c->nextA;
+        if (c->isMarked) continue;
+        if ((c->flags & C_TOUCHING) == 0) continue;
+        c->isMarked = 1;
+        int other = edge ? c->bodyA : c->bodyB;
+        OBody* ob = w->bodies + other;
+        if (!ob->isMarked && ob->setIndex != STATIC_SET) {
+          if (top >= n) abort();
+          stack[top++] = other;
+          ob->isMarked = 1;
+        }
+        c->islandId = id;
+        if (is->tailContact != -1) w->contacts[is->tailContact].islandNext = cid;
+        c->islandPrev = is->tailContact;
+        c->islandNext = -1;
+        is->tailContact = cid;
+        if (is->headContact == -1) is->headContact = cid;
+        is->contactCount += 1;
+      }
+      for (int key = body->headJointKey; key != -1;) {
+        int jid = key >> 1, edge = key & 1;
+        OJoint* J = w->joints + jid;
+        key = edge ? J->nextB : J->nextA;
+        if (J->isMarked) continue;
+        J->isMarked = 1;
+        int other = edge ? J->bodyA : J->bodyB;
+        OBody* ob = w->bodies + other;
+        if (ob->setIndex == DISABLED_SET) continue;
+        if (!ob->isMarked && ob->setIndex == AWAKE_SET) {
+          if (top >= n) abort();
+          stack[top++] = other;
+          ob->isMarked = 1;
+        }
+        J->islandId = id;
+        if (is->tailJoint != -1) w->joints[is->tailJoint].islandNext = jid;
+        J->islandPrev = is->tailJoint;
+        J->islandNext = -1;
+        is->tailJoint = jid;
+        if (is->headJoint == -1) is->headJoint = jid;
+        is->jointCount += 1;
+      }
+    }
+  }
+  free(stack);
+  free(seeds);
+}
+
+/* the island of a sleeping solver set: b2TrySleepIsland gives every sleeping set
+   exactly one island (sets merge only through joints created between sleeping
+   bodies, which the step never does) */
+static int island_of_set(OWorld* w, int setIndex) {
+  for (int i = 0; i < w->islandCount; ++i)
+    if (w->islands[i].id == i && w->islands[i].setIndex == setIndex) return i;
+  abort();
+}
+
+/* b2WakeSolverSet, S/solver_set.c:31-154: the set's bodies, contacts and joints are
+   its island's lists in order */
+static void wake_island_set(OWorld* w, int setIndex) {
+  int id = island_of_set(w, setIndex);
+  OIsland* is = w->islands + id;
+  for (int b = is->headBody; b != -1; b = w->bodies[b].islandNext) {
+    OBody* body = w->bodies + b;
+    body->setIndex = AWAKE_SET;
+    body->localIndex = w->awakeBodyCount;
+    w->awakeBodies[w->awakeBodyCount++] = b;
+    body->sleepTime = 0;
+    body->v = v2(0, 0);  /* b2_identityBodyState */
+    body->w = 0;
+    for (int key = body->headContactKey; key != -1;) {
+      OContact* c = w->contacts + (key >> 1);
+      key = (key & 1) ? c->nextB : c->nextA;
+      if (c->setIndex != DISABLED_SET) continue;
+      c->setIndex = AWAKE_SET;
+      c->localIndex = -1;
+    }
+  }
+  for (int c = is->headContact; c != -1; c = w->contacts[c].islandNext) {
+    add_to_graph(w, c);
+    w->contacts[c].setIndex = AWAKE_SET;
+  }
+  for (int j = is->headJoint; j != -1; j = w->joints[j].islandNext) {
+    add_joint_to_graph(w, j);
+    w->joints[j].setIndex = AWAKE_SET;
+  }
+  is->setIndex = AWAKE_SET;
+  is->localIndex = w->awakeIslandCount;
+  w->awakeIslands[w->awakeIslandCount++] = id;
+  w->setFree[w->setFreeCount++] = setIndex;  /* b2DestroySolverSet */
+}
+
+/* b2TrySleepIsland, S/solver_set.c:156-420 */
+static void try_sleep_island(OWorld* w, int id) {
+  OIsland* is = w->islands + id;
+  if (is->removeCount > 0) return;
+  int setId = pool_alloc(w->setFree, &w->setFreeCount, &w->setCount);
+  int sleepIndex = 0;
+  for (int b = is->headBody; b != -1; b = w->bodies[b].islandNext) {
+    OBody* body = w->bodies + b;
+    int li = body->localIndex, last = --w->awakeBodyCount;
+    if (li != last) {
+      int moved = w->awakeBodies[last];
+      w->awakeBodies[li] = moved;
+      w->bodies[moved].localIndex = li;
+    }
+    body->setIndex = setId;
+    body->localIndex = sleepIndex++;
+    body->v = v2(0, 0);  /* the awake body state is dropped */
+    body->w = 0;
+    for (int key = body->headContactKey; key != -1;) {
+      int edge = key & 1;
+      OContact* c = w->contacts + (key >> 1);
+      key = edge ? c->nextB : c->nextA;
+      if (c->setIndex == DISABLED_SET) continue;
+      if (c->color != -1) continue;
+      int other = edge ? c->bodyA : c->bodyB;
+      if (w->bodies[other].setIndex == AWAKE_SET) continue;
+      c->setIndex = DISABLED_SET;
+      c->localIndex = -1;
+    }
+  }
+  int k = 0;
+  for (int c = is->headContact; c != -1; c = w->contacts[c].islandNext) {
+    remove_from_graph(w, c);
+    w->contacts[c].setIndex = setId;
+    w->contacts[c].localIndex = k++;
+  }
+  k = 0;
+  for (int j = is->headJoint; j != -1; j = w->joints[j].islandNext) {
+    remove_joint_from_graph(w, j);
+    w->joints[j].setIndex = setId;
+    w->joints[j].localIndex = k++;
+  }
+  int li = is->localIndex, last = --w->awakeIslandCount;
+  if (li != last) {
+    int moved = w->awakeIslands[last];
+    w->awakeIslands[li] = moved;
+    w->islands[moved].localIndex = li;
+  }
+  is->setIndex = setId;
+  is->localIndex = 0;
+}
+
+/* b2CreateContact's body-list prepend, S/contact.c:245-279 */
+static void body_list_add(OWorld* w, int cid) {
+  OContact* c = w->contacts + cid;
+  int keyA = cid << 1, keyB = (cid << 1) | 1;
+  OBody* A = w->bodies + c->bodyA;
+  OBody* B = w->bodies + c->bodyB;
+  c->prevA = -1;
+  c->nextA = A->headContactKey;
+  if (A->headContactKey != -1) {
+    OContact* h = w->contacts + (A->headContactKey >> 1);
+    if (A->headContactKey & 1) h->prevB = keyA; else h->prevA = keyA;
+  }
+  A->headContactKey = keyA;
+  c->prevB = -1;
+  c->nextB = B->headContactKey;
+  if (B->headContactKey != -1) {
+    OContact* h = w->contacts + (B->headContactKey >> 1);
+    if (B->headContactKey & 1) h->prevB = keyB; else h->prevA = keyB;
+  }
+  B->headContactKey = keyB;
+}
+
+/* b2DestroyContact's body-list removal, S/contact.c:374-414 */
+static void body_list_remove(OWorld* w, int cid) {
+  OContact* c = w->contacts + cid;
+  for (int e = 0; e < 2; ++e) {
+    int prev = e ? c->prevB : c->prevA, next = e ? c->nextB : c->nextA;
+    OBody* body = w->bodies + (e ? c->bodyB : c->bodyA);
+    if (prev != -1) {
+      OContact* p = w->contacts + (prev >> 1);
+      if (prev & 1) p->nextB = next; else p->nextA = next;
+    }
+    if (next != -1) {
+      OContact* q = w->contacts + (next >> 1);
+      if (next & 1) q->prevB = prev; else q->prevA = prev;
+    }
+    if (body->headContactKey == ((cid << 1) | e)) body->headContactKey = next;
+  }
+}
+
 /* b2Collide + the state loop of b2Collide's caller, S/world.c:357-445, 587-690 */
 int b2o_collide(void* h) {
   OWorld* w = (OWorld*)h;
@@ -1306,6 +1872,7 @@ int b2o_collide(void* h) {
     OContact* c = w->contacts + id;
     c->inState = 0;
     if (c->shapeA < 0) continue;
+    if (c->setIndex != AWAKE_SET) continue;  /* only the awake set's contacts collide (S/world.c:496-521) */
     const OShape* a = w->shapes + c->shapeA;
     const OShape* b = w->shapes + c->shapeB;
     if (!aabb_overlaps(a->fat, b->fat)) {
@@ -1334,19 +1901,26 @@ int b2o_collide(void* h) {
     c->inState = 0;
     if (c->simFlags & SIM_DISJOINT) {
       /* b2DestroyContact, S/contact.c:337-454 */
+      if (w->hasIslands) {
+        body_list_remove(w, id);
+        if (c->islandId != -1) unlink_contact(w, id);
+      }
       if (c->color >= 0) remove_from_graph(w, id);
       c->shapeA = -1;
       c->shapeB = -1;
       c->flags = 0;
       c->simFlags = 0;
+      c->setIndex = -1;
       w->freeIds[w->freeCount++] = id;
     } else if (c->simFlags & SIM_STARTED) {
       c->flags |= C_TOUCHING;
+      if (w->hasIslands) link_contact(w, id);
       c->simFlags &= ~SIM_STARTED;
       add_to_graph(w, id);
     } else if (c->simFlags & SIM_STOPPED) {
       c->simFlags &= ~SIM_STOPPED;
       c->flags &= ~C_TOUCHING;
+      if (w->hasIslands) unlink_contact(w, id);
       remove_from_graph(w, id);
     }
   }
@@ -2412,6 +2986,20 @@ int b2o_solve(void* h, float dt_, int subSteps) {
   w->contactSpeed = w->maxContactPushSpeed / x.staticSoftness.massScale;
   x.warmStartScale = w->enableWarmStarting ? 1 : 0;
 
+  /* b2Solve, S/solver.c:1190-1220: merge islands; nothing more when no body is awake */
+  if (w->hasIslands) {
+    merge_awake_islands(w);
+    for (int i = 0; i < w->bodyCount; ++i) w->simOf[i] = -1;
+    for (int k = 0; k < w->awakeBodyCount; ++k) w->simOf[w->awakeBodies[k]] = k;
+    if (w->awakeBodyCount == 0) {
+      w->step += 1;
+      return 0;
+    }
+    /* the split candidate of the last step, S/solver.c:1479-1706 */
+    if (w->splitIslandId != -1) split_island(w, w->splitIslandId);
+    w->splitIslandId = -1;
+  }
+
   /* awake bodies in sim order */
   x.awake = (int*)xcalloc(w->bodyCount, sizeof(int));
   x.st = (St*)xcalloc(w->bodyCount, sizeof(St));
@@ -2444,7 +3032,8 @@ int b2o_solve(void* h, float dt_, int subSteps) {
   }
 
   /* prepare joints (S/solver.c:967-987), then contacts (:990-1011), overflow last */
-  for (int j = 0; j < w->jointCount; ++j) prepare_joint(&x, w->joints + j);
+  for (int j = 0; j < w->jointCount; ++j)
+    if (w->joints[j].color >= 0) prepare_joint(&x, w->joints + j);
   for (int c = 0; c < COLOR_COUNT; ++c)
     for (int k = 0; k < w->colorCount[c]; ++k)
       prepare_contact(&x, w->colors[c][k], cons + colorStart[c] + k, c != OVERFLOW);
@@ -2548,6 +3137,9 @@ int b2o_solve(void* h, float dt_, int subSteps) {
   uint8_t* enlargedSim = (uint8_t*)xcalloc(x.nAwake, 1);
   int* bullets = (int*)xcalloc(x.nAwake, sizeof(int));
   int nBullets = 0;
+  uint8_t* islandAwake = (uint8_t*)xcalloc(w->awakeIslandCount + 1, 1);
+  int splitCandidate = -1;
+  real splitSleepTime = 0;
   for (int i = 0; i < x.nAwake; ++i) {
     OBody* sim = w->bodies + x.awake[i];
     V2 v = sim->v;
@@ -2565,7 +3157,7 @@ int b2o_solve(void* h, float dt_, int subSteps) {
     sim->torque = 0;
     sim->isSpeedCapped = sim->isSpeedCapped == 2;  /* body->isSpeedCapped = sim->isSpeedCapped */
     sim->isFast = 0;
-    if (!w->enableSleep || sleepVelocity > sim->sleepThreshold) {
+    if (!w->enableSleep || !sim->enableSleep || sleepVelocity > sim->sleepThreshold) {
       sim->sleepTime = 0;
       if (sim->type == DYNAMIC_BODY && w->enableContinuous && maxVelocity * x.dt > (real)0.5f * sim->minExtent) {
         sim->isFast = 1;
@@ -2582,6 +3174,17 @@ int b2o_solve(void* h, float dt_, int subSteps) {
       sim->center0 = sim->center;
       sim->rot0 = sim->xf.q;
       sim->sleepTime += x.dt;
+    }
+    /* any body of an island can keep it awake; the sleepiest body of an island
+       with removed constraints names the island to split (S/solver.c:658-676) */
+    if (w->hasIslands) {
+      OIsland* is = w->islands + sim->islandId;
+      if (sim->sleepTime < TIME_TO_SLEEP) {
+        islandAwake[is->localIndex] = 1;
+      } else if (is->removeCount > 0 && sim->sleepTime > splitSleepTime) {
+        splitCandidate = sim->islandId;
+        splitSleepTime = sim->sleepTime;
+      }
     }
     XF xf = sim->xf;
     for (int s = sim->headShape; s >= 0; s = w->shapes[s].next) {
@@ -2628,6 +3231,15 @@ int b2o_solve(void* h, float dt_, int subSteps) {
     for (int s = b->headShape; s >= 0; s = w->shapes[s].next) w->shapes[s].enlarged = 0;
   }
   for (int i = 0; i < x.nAwake; ++i) w->bodies[x.awake[i]].enlarge = 0;
+  /* island sleep, S/solver.c:1995-2049: in reverse awake-island order */
+  if (w->hasIslands && w->enableSleep) {
+    w->splitIslandId = splitCandidate;
+    for (int i = w->awakeIslandCount - 1; i >= 0; --i) {
+      if (islandAwake[i]) continue;
+      try_sleep_island(w, w->awakeIslands[i]);
+    }
+  }
+  free(islandAwake);
   w->step += 1;
   free(enlargedSim);
   free(bullets);
@@ -2646,12 +3258,17 @@ long b2o_export(void* h, void* out, long cap) {
   long size = (long)sizeof(SnapH) + (long)w->bodyCount * sizeof(SnapB) + (long)w->shapeCount * sizeof(SnapS) +
               (long)w->contactCount * sizeof(SnapC) + (long)w->jointCount * sizeof(SnapJ) + 4L * COLOR_COUNT +
               4L * total + 4L * w->moveCount + 4L * w->freeCount;
+  if (w->hasIslands)
+    size += 64L + (long)sizeof(OIsland) * w->islandCount +
+            4L * (w->islandFreeCount + w->awakeIslandCount + w->setFreeCount + w->awakeBodyCount) +
+            (long)sizeof(SnapBL) * w->bodyCount + (long)sizeof(SnapCL) * w->contactCount +
+            (long)sizeof(SnapJL) * w->jointCount;
   if (!out || cap < size) return size;
   uint8_t* o = (uint8_t*)out;
   memset(o, 0, size);
   SnapH* hh = (SnapH*)o;
   hh->magic = SNAP_MAGIC;
-  hh->version = 1;
+  hh->version = w->hasIslands ? 2 : 1;
   hh->step = w->step;
   hh->bodyCount = w->bodyCount;
   hh->shapeCount = w->shapeCount;
@@ -2780,6 +3397,53 @@ long b2o_export(void* h, void* out, long cap) {
     for (int k = 0; k < w->colorCount[c]; ++k) *t++ = w->colors[c][k];
   for (int k = 0; k < w->moveCount; ++k) *t++ = w->move[k];
   for (int k = 0; k < w->freeCount; ++k) *t++ = w->freeIds[k];
+  if (w->hasIslands) {
+    t[0] = w->islandCount;
+    t[1] = w->islandFreeCount;
+    t[2] = w->awakeIslandCount;
+    t[3] = w->splitIslandId;
+    t[4] = w->setCount;
+    t[5] = w->setFreeCount;
+    t[6] = w->awakeBodyCount;
+    t[7] = w->islandCount;
+    t[8] = w->setCount;
+    t += 16;
+    memcpy(t, w->islands, sizeof(OIsland) * (size_t)w->islandCount);
+    t += 16 * w->islandCount;
+    for (int k = 0; k < w->islandFreeCount; ++k) *t++ = w->islandFree[k];
+    for (int k = 0; k < w->awakeIslandCount; ++k) *t++ = w->awakeIslands[k];
+    for (int k = 0; k < w->setFreeCount; ++k) *t++ = w->setFree[k];
+    for (int k = 0; k < w->awakeBodyCount; ++k) *t++ = w->awakeBodies[k];
+    for (int i = 0; i < w->bodyCount; ++i) {
+      const OBody* b = w->bodies + i;
+      *t++ = b->islandId;
+      *t++ = b->islandPrev;
+      *t++ = b->islandNext;
+      *t++ = b->headContactKey;
+    }
+    for (int i = 0; i < w->contactCount; ++i) {
+      const OContact* c = w->contacts + i;
+      if (c->shapeA < 0) {
+        for (int k = 0; k < 8; ++k) *t++ = -1;
+        continue;
+      }
+      *t++ = c->islandId;
+      *t++ = c->islandPrev;
+      *t++ = c->islandNext;
+      *t++ = c->setIndex;
+      *t++ = c->prevA;
+      *t++ = c->nextA;
+      *t++ = c->prevB;
+      *t++ = c->nextB;
+    }
+    for (int j = 0; j < w->jointCount; ++j) {
+      const OJoint* J = w->joints + j;
+      *t++ = J->islandId;
+      *t++ = J->islandPrev;
+      *t++ = J->islandNext;
+      *t++ = J->setIndex;
+    }
+  }
   return size;
 }
 

@@ -25,6 +25,7 @@
 #include "body.h"
 #include "constraint_graph.h"
 #include "contact.h"
+#include "island.h"
 #include "joint.h"
 #include "shape.h"
 #include "solver_set.h"
@@ -305,12 +306,84 @@ static b2WorldId build_world( const SceneWorld* sw, b2BodyId* bodyIds )
 	return wid;
 }
 
+// Island and solver-set section of a version-2 snapshot (S/island.h:27-58,
+// S/solver_set.h, S/id_pool.h): 16 header ints, then the island records, the
+// island and solver-set free lists, the awake set's island and body orders, and
+// per body the island links and contact-list head, per contact the island links,
+// set index and both body-list edges, per joint the island links and set index.
+static void write_islands( FILE* f, b2World* world )
+{
+	b2SolverSet* awake = world->solverSets.data + b2_awakeSet;
+	int32_t hdr[16] = { 0 };
+	hdr[0] = world->islands.count;
+	hdr[1] = world->islandIdPool.freeArray.count;
+	hdr[2] = awake->islandSims.count;
+	hdr[3] = world->splitIslandId;
+	hdr[4] = world->solverSets.count;
+	hdr[5] = world->solverSetIdPool.freeArray.count;
+	hdr[6] = awake->bodySims.count;
+	hdr[7] = world->islandIdPool.nextIndex;
+	hdr[8] = world->solverSetIdPool.nextIndex;
+	fwrite( hdr, 4, 16, f );
+	for ( int i = 0; i < world->islands.count; ++i )
+	{
+		const b2Island* is = world->islands.data + i;
+		int32_t r[16] = { is->islandId,	   is->setIndex,	is->localIndex,	  is->headBody,
+						  is->tailBody,	   is->bodyCount,	is->headContact,  is->tailContact,
+						  is->contactCount, is->headJoint,	is->tailJoint,	  is->jointCount,
+						  is->parentIsland, is->constraintRemoveCount, 0, 0 };
+		fwrite( r, 4, 16, f );
+	}
+	fwrite( world->islandIdPool.freeArray.data, 4, world->islandIdPool.freeArray.count, f );
+	for ( int i = 0; i < awake->islandSims.count; ++i )
+		fwrite( &awake->islandSims.data[i].islandId, 4, 1, f );
+	fwrite( world->solverSetIdPool.freeArray.data, 4, world->solverSetIdPool.freeArray.count, f );
+	for ( int i = 0; i < awake->bodySims.count; ++i )
+		fwrite( &awake->bodySims.data[i].bodyId, 4, 1, f );
+	for ( int i = 0; i < world->bodies.count; ++i )
+	{
+		const b2Body* b = world->bodies.data + i;
+		int32_t r[4] = { b->islandId, b->islandPrev, b->islandNext, b->headContactKey };
+		fwrite( r, 4, 4, f );
+	}
+	for ( int i = 0; i < world->contacts.count; ++i )
+	{
+		const b2Contact* c = world->contacts.data + i;
+		int32_t r[8] = { -1, -1, -1, -1, -1, -1, -1, -1 };
+		if ( c->contactId != B2_NULL_INDEX )
+		{
+			r[0] = c->islandId;
+			r[1] = c->islandPrev;
+			r[2] = c->islandNext;
+			r[3] = c->setIndex;
+			r[4] = c->edges[0].prevKey;
+			r[5] = c->edges[0].nextKey;
+			r[6] = c->edges[1].prevKey;
+			r[7] = c->edges[1].nextKey;
+		}
+		fwrite( r, 4, 8, f );
+	}
+	for ( int i = 0; i < world->joints.count; ++i )
+	{
+		const b2Joint* j = world->joints.data + i;
+		int32_t r[4] = { -1, -1, -1, -1 };
+		if ( j->jointId != B2_NULL_INDEX )
+		{
+			r[0] = j->islandId;
+			r[1] = j->islandPrev;
+			r[2] = j->islandNext;
+			r[3] = j->setIndex;
+		}
+		fwrite( r, 4, 4, f );
+	}
+}
+
 static void write_snapshot( FILE* f, b2WorldId wid, int step )
 {
 	b2World* world = b2GetWorldFromId( wid );
 	SnapHeader h = { 0 };
 	h.magic = 0x50414E53;
-	h.version = 1;
+	h.version = 2;
 	h.step = step;
 	h.body_count = world->bodies.count;
 	h.shape_count = world->shapes.count;
@@ -476,6 +549,7 @@ static void write_snapshot( FILE* f, b2WorldId wid, int step )
 	}
 	fwrite( world->broadPhase.moveArray.data, 4, world->broadPhase.moveArray.count, f );
 	fwrite( world->contactIdPool.freeArray.data, 4, world->contactIdPool.freeArray.count, f );
+	write_islands( f, world );
 }
 
 static int parse_list( const char* s, int* out, int cap )

@@ -135,6 +135,40 @@ def events_world():
     return w
 
 
+def sleep_world():
+    """Islands and sleep (S/island.c, S/solver_set.c, S/solver.c:1995-2049) with
+    sleep enabled: two box pyramids settle and fall asleep as islands; a ball
+    dropped from 30 m lands on one and wakes it (b2LinkContact wakes the sleeping
+    solver set); a box slides into another and bounces off, so their merged island
+    loses a contact and is split before it sleeps (S/island.c:602-830); a hanging
+    revolute chain sleeps with its joints and is woken by a second ball."""
+    w = scenes.World(scenes.world_header(sleep=True, continuous=True))
+    g = w.add_body(scenes.STATIC, 0.0, -1.0)
+    w.add_box(g, 40.0, 1.0, friction=0.6)
+    for cx, rows in ((-10.0, 5), (10.0, 4)):
+        for r in range(rows):
+            n = rows - r
+            for i in range(n):
+                b = w.add_body(scenes.DYNAMIC, cx + (i - (n - 1) / 2.0) * 1.0, 0.5 + r * 1.0)
+                w.add_box(b, 0.5, 0.5, friction=0.6)
+    a = w.add_body(scenes.DYNAMIC, 0.0, 0.5)
+    w.add_box(a, 0.5, 0.5, friction=0.6, restitution=0.6)
+    s = w.add_body(scenes.DYNAMIC, -2.5, 0.5, vx=5.0)
+    w.add_box(s, 0.5, 0.5, friction=0.6, restitution=0.6)
+    anchor = w.add_body(scenes.STATIC, 20.0, 10.0)
+    prev, prev_anchor = anchor, (0.0, 0.0)
+    for k in range(4):
+        b = w.add_body(scenes.DYNAMIC, 20.0, 10.0 - 0.2 - 0.4 * k)
+        w.add_box(b, 0.05, 0.2, friction=0.6)
+        w.add_revolute(prev, b, prev_anchor, (0.0, 0.2), enable_limit=True, lower=-0.5, upper=0.5)
+        prev, prev_anchor = b, (0.0, -0.2)
+    ball = w.add_body(scenes.DYNAMIC, -10.2, 30.0)
+    w.add_circle(ball, 0.5, density=2.0, friction=0.6)
+    ball2 = w.add_body(scenes.DYNAMIC, 20.3, 40.0)
+    w.add_circle(ball2, 0.3, density=1.0, friction=0.6)
+    return w
+
+
 EVENT_STEPS = list(range(1, 121))
 
 
@@ -149,6 +183,7 @@ FIXTURES = {
     "bounce": (bounce_world, [0, 40, 41, 42, 45, 46, 47, 50, 51, 52, 60, 61, 62, 80, 81, 82]),
     "ccd": (ccd_world, list(range(0, 13)) + [30, 31, 32]),
     "events": (events_world, list(range(0, 121))),
+    "sleep": (sleep_world, list(range(0, 261))),
     "mixed_ccd": (lambda: scenes.container_world(200, 15.0, 40.0, 20261202, continuous=True, chains=3,
                                                  drop_band=(0.2, 1.0)),
                   [0, 1, 2, 3, 40, 41, 42, 43, 44, 45, 46, 47, 48, 49, 50, 60, 61, 62]),

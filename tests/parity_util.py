@@ -86,3 +86,33 @@ def color_map(snap: S.Snapshot) -> dict:
             rec = snap.contacts[int(cid)]
             out[(int(rec["shape_a"]), int(rec["shape_b"]))] = c
     return out
+
+
+def island_differences(mine: S.Snapshot, ref: S.Snapshot) -> list:
+    """Names of the island-graph and solver-set quantities that differ between two
+    version-2 snapshots: each body's set and index in it and its sleep timer; the
+    island records (union-find parents, body / contact / joint lists, counts and
+    removed-constraint counters), the island and solver-set id pools, the awake
+    set's island and body orders, the split candidate, and the island links of
+    bodies, live contacts and joints, with the joints' colours."""
+    out = []
+    for f in ("set_index", "local_index", "sleep_time"):
+        if not np.array_equal(mine.bodies[f], ref.bodies[f]):
+            out.append("body." + f)
+    a, b = mine.islands, ref.islands
+    if a is None or b is None:
+        return out + (["island section"] if (a is None) != (b is None) else [])
+    for f in ("island_count", "awake_island_count", "split_island", "set_count", "awake_body_count"):
+        if getattr(a, f) != getattr(b, f):
+            out.append(f)
+    for f in ("islands", "free_islands", "awake_islands", "free_sets", "awake_bodies", "body_links", "joint_links"):
+        if not np.array_equal(getattr(a, f), getattr(b, f)):
+            out.append(f)
+    live = ref.contacts["id"] >= 0
+    if len(a.contact_links) != len(b.contact_links) or not np.array_equal(a.contact_links[live],
+                                                                           b.contact_links[live]):
+        out.append("contact_links")
+    for f in ("color", "local_index"):
+        if not np.array_equal(mine.joints[f], ref.joints[f]):
+            out.append("joint." + f)
+    return out

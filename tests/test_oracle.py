@@ -16,6 +16,15 @@ search, so the final fraction can differ in the last bits (within the
 time-of-impact tolerance, 0.25 * linear slop). Fast bodies are therefore
 compared within 1e-5 relative, every other quantity exactly.
 
+Islands and sleep (S/island.c, S/solver_set.c, S/solver.c:1995-2049) are part of
+the restated step: union-find links when contacts begin touching, unlinks and
+removed-constraint counts when they stop, the merge of awake islands, the split of
+the last step's candidate by depth-first search, sleep timers, islands put to
+sleep (bodies out of the awake set, contacts and joints out of the graph) and
+sleeping sets woken by a new touching contact. The island section of snapshot k
+(island records, id pools, awake orders, links) must match too; the "sleep"
+fixture exercises all of it over 260 steps.
+
 The double build states the deviation of the float reference from FP64 over one
 step from the same state.
 """
@@ -67,6 +76,7 @@ def test_oracle_step_reproduces_reference(name, i):
         assert np.abs(mine.shapes[f][fast_shapes] - ref.shapes[f][fast_shapes]).max(initial=0.0) <= TOL, f
     rep = P.manifold_report(mine, ref, with_impulses=True)
     assert rep["point_count"] == rep["ids"] == rep["normals"] == rep["values"] == 0, rep
+    assert P.island_differences(mine, ref) == []
     if not fast.any():
         assert rep["impulses"] == 0, rep
         assert np.array_equal(mine.move, ref.move)
