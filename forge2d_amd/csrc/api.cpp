@@ -81,7 +81,34 @@ int HostWorld::createBody(int type, Vec2 p, Rot q, Vec2 v, float w, float linDam
     st.v = {0.0f, 0.0f};
     st.w = 0.0f;
   }
+  // solver set and island (b2CreateBody, S/body.c:198-324): static bodies join the
+  // static set; every other body joins the awake set with an island of its own
+  s.islandId = s.islandPrev = s.islandNext = -1;
+  int id = (int)sims.size();
+  if (type == kStatic) {
+    s.setIndex = kStaticSet;
+    s.localIndex = staticBodies++;
+  } else {
+    s.setIndex = kAwakeSet;
+    s.localIndex = hdr.awakeBodyCount;
+  }
   sims.push_back(s);
+  if (type != kStatic) {
+    awakeBodies.push_back(id);
+    hdr.awakeBodyCount = (int)awakeBodies.size();
+    // b2CreateIslandForBody, S/body.c:84-97
+    islands.resize(islands.size() + 1);
+    islandFree.resize(islands.size());
+    awakeIslands.resize(islands.size());
+    setFree.resize(islands.size() + 4);
+    hdr.islandCap = (int)islands.size();
+    IslandCtx c = islandCtx();
+    int isl = createIsland(c);
+    Island& is = islands[isl];
+    is.headBody = is.tailBody = id;
+    is.bodyCount = 1;
+    sims[id].islandId = isl;
+  }
   states.push_back(st);
   bodyMass.push_back(0.0f);
   bodyInertia.push_back(0.0f);
@@ -278,8 +305,41 @@ int HostWorld::createRevolute(int bodyA, int bodyB, Vec2 la, Vec2 lb, float refA
       if (o.color == color) ++n;
     j.localIndex = n;
   }
+  // joints between two static bodies stay in the static set; the others join the
+  // awake set and the island graph, and islands merge at once (b2CreateJoint,
+  // S/joint.c:204-309; b2LinkJoint with mergeIslands, S/island.c:289-385)
+  j.islandId = j.islandPrev = j.islandNext = -1;
+  j.setIndex = (staticA && staticB) ? kStaticSet : kAwakeSet;
   joints.push_back(j);
+  if (j.setIndex == kAwakeSet) {
+    IslandCtx c = islandCtx();
+    linkJointIslands(c, id);
+    mergeAwakeIslands(c);
+  }
   return id;
+}
+
+IslandCtx HostWorld::islandCtx() {
+  // the list-walk bounds read the world's current sizes
+  hdr.bodyCount = (int)sims.size();
+  hdr.jointCount = (int)joints.size();
+  hdr.contactCap = std::max(hdr.contactCap, (int)contacts.size());
+  IslandCtx c;
+  c.h = &hdr;
+  c.islands = islands.data();
+  c.islandFree = islandFree.data();
+  c.awakeIslands = awakeIslands.data();
+  c.setFree = setFree.data();
+  c.awakeBodies = awakeBodies.data();
+  c.moveOrder = moveOrder.data();
+  c.awakeBits = nullptr;
+  c.stack = nullptr;
+  c.seeds = nullptr;
+  c.sims = sims.data();
+  c.contacts = contacts.data();
+  c.links = links.data();
+  c.joints = joints.data();
+  return c;
 }
 
 // ---------------------------------------------------------------------------- scene spec
@@ -331,6 +391,9 @@ static void initWorldHeader(HostWorld& hw, const SpecWorld& sw) {
   h.maxContactPushSpeed = sw.max_contact_push_speed;
   h.jointHertz = 60.0f;
   h.jointDampingRatio = 2.0f;
+  // b2CreateWorld, S/world.c:138-187: the static, disabled and awake sets exist
+  h.setCount = kFirstSleepingSet;
+  h.splitIslandId = -1;
 }
 
 // Shape kinds of the spec: 0 circle, 1 capsule, 2 segment, 3 polygon (hull of points), 4 box
@@ -496,10 +559,20 @@ static void freeDevice(Batch& b) {
 // Lay out all worlds in flat arrays and allocate the device batch.
 static int commit(Batch& b) {
   freeDevice(b);
-  long nb = 0, ns = 0, nc = 0, nj = 0, ncol = 0, nscratch = 0;
+  long nb = 0, ns = 0, nc = 0, nj = 0, ncol = 0, nscratch = 0, nisl = 0;
   for (HostWorld& hw : b.worlds) {
     hw.ensureColorWords();
     WorldHeader& h = hw.hdr;
+    // island region: one island per non-static body at most
+    h.islandCap = std::max(1, (int)hw.islands.size());
+    hw.islands.resize(h.islandCap);
+    hw.islandFree.resize(h.islandCap);
+    hw.awakeIslands.resize(h.islandCap);
+    hw.setFree.resize(h.islandCap + 4);
+    hw.awakeBodies.resize(hw.sims.size());
+    hw.moveOrder.resize(hw.sims.size());
+    h.islandOffset = (int)nisl;
+    nisl += islandRegion(h.islandCap, (int)hw.sims.size()).total;
     h.bodyOffset = (int)nb;
     h.bodyCount = (int)hw.sims.size();
     h.shapeOffset = (int)ns;
@@ -542,6 +615,7 @@ static int commit(Batch& b) {
       hw.contacts[k].shapeB = -1;
       hw.contacts[k].color = -1;
     }
+    hw.links.resize(cap, make_int4(-1, -1, -1, -1));
   }
   if (nc > (1L << 31) / CN_FIELD_COUNT) {
     b.lastError = "batch too large for 32-bit constraint indexing";
@@ -556,13 +630,20 @@ static int commit(Batch& b) {
       devAlloc(d, &v.pairs, nc) || devAlloc(d, &v.pairShapes, nc) || devAlloc(d, &v.joints, nj) ||
       devAlloc(d, &v.colorBits, ncol) || devAlloc(d, &v.cons, (size_t)nc * CN_FIELD_COUNT) ||
       devAlloc(d, &v.scratch, nscratch) || devAlloc(d, &v.ccolor, nc) || devAlloc(d, &v.staticShapes, ns) ||
-      devAlloc(d, &v.events, 3 * nc)) {
+      devAlloc(d, &v.events, 3 * nc) || devAlloc(d, &v.islandData, std::max(nisl, 4L)) ||
+      devAlloc(d, &v.contactLinks, nc)) {
     b.lastError = "device allocation failed";
     freeDevice(b);
     return -1;
   }
   v.consStride = nc;
   v.worldCount = W;
+  b.nBodies = nb;
+  b.nShapes = ns;
+  b.nContacts = nc;
+  b.nJoints = nj;
+  b.nColorWords = ncol;
+  b.nIsland = std::max(nisl, 4L);
   // dynamic LDS per kernel: sort-and-sweep (28 B per shape, power-of-two padded)
   // and new-pair sort (12 B per pair), body states (32 B per body); worlds that do
   // not fit fall back to their global scratch inside the kernel
@@ -667,11 +748,16 @@ static int upload(Batch& b) {
   long totalShapes = 0;
   for (const HostWorld& hw : b.worlds) totalShapes = std::max<long>(totalShapes, hw.hdr.shapeOffset + hw.hdr.shapeCount);
   std::vector<int> staticShapes(std::max<long>(totalShapes, 1), 0);
+  long totalIsland = 4;
+  for (const HostWorld& hw : b.worlds)
+    totalIsland = std::max<long>(totalIsland, hw.hdr.islandOffset + islandRegion(hw.hdr.islandCap, hw.hdr.bodyCount).total);
+  std::vector<int> islandData(totalIsland, 0);
+  std::vector<int4> links(std::max<long>(totalContacts, 1), make_int4(-1, -1, -1, -1));
   b.launch.continuous = 0;
   for (size_t i = 0; i < b.worlds.size(); ++i) {
     HostWorld& hw = b.worlds[i];
     WorldHeader& h = hw.hdr;
-    if (h.flags & WF_CONTINUOUS) b.launch.continuous = 1;
+    if (h.flags & (WF_CONTINUOUS | WF_SLEEP)) b.launch.continuous = 1;  // the last kernel also runs island sleep
     h.fastCount = 0;
     h.staticCount = 0;
     for (int k = 0; k < h.shapeCount; ++k) {
@@ -693,6 +779,17 @@ static int upload(Batch& b) {
     for (int k = 0; k < h.contactCap; ++k) {
       const Contact& c = hw.contacts[k];
       ccolor[h.contactOffset + k] = (int8_t)(c.shapeA >= 0 ? c.color : -1);
+      if (k < (int)hw.links.size()) links[h.contactOffset + k] = hw.links[k];
+    }
+    {
+      IslandRegion r = islandRegion(h.islandCap, h.bodyCount);
+      int* base = islandData.data() + h.islandOffset;
+      std::memcpy(base + r.islands, hw.islands.data(), sizeof(Island) * h.islandCap);
+      std::memcpy(base + r.islandFree, hw.islandFree.data(), sizeof(int) * h.islandFreeCount);
+      std::memcpy(base + r.awakeIslands, hw.awakeIslands.data(), sizeof(int) * h.awakeIslandCount);
+      std::memcpy(base + r.setFree, hw.setFree.data(), sizeof(int) * h.setFreeCount);
+      std::memcpy(base + r.awakeBodies, hw.awakeBodies.data(), sizeof(int) * h.awakeBodyCount);
+      std::memcpy(base + r.moveOrder, hw.moveOrder.data(), sizeof(int) * h.moveBodyCount);
     }
     if (!hw.freeIds.empty())
       F2D_HIP(hipMemcpyAsync(v.freeIds + h.freeOffset, hw.freeIds.data(), sizeof(int) * hw.freeIds.size(), hipMemcpyHostToDevice, s));
@@ -705,46 +802,69 @@ static int upload(Batch& b) {
   F2D_HIP(hipMemcpyAsync(v.worlds, headers.data(), sizeof(WorldHeader) * headers.size(), hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipMemcpyAsync(v.ccolor, ccolor.data(), ccolor.size(), hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipMemcpyAsync(v.staticShapes, staticShapes.data(), sizeof(int) * totalShapes, hipMemcpyHostToDevice, b.stream));
+  F2D_HIP(hipMemcpyAsync(v.islandData, islandData.data(), sizeof(int) * islandData.size(), hipMemcpyHostToDevice, b.stream));
+  F2D_HIP(hipMemcpyAsync(v.contactLinks, links.data(), sizeof(int4) * totalContacts, hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipStreamSynchronize(b.stream));
   for (SizeClass& c : b.classes) c.launch.continuous = b.launch.continuous;
   b.hostDirty = false;
   return 0;
 }
 
+// Copy the device state of every world back to the host mirror: one copy per flat
+// array, then a scatter into the worlds' vectors.
 static int download(Batch& b) {
   if (!b.committed || !b.deviceNewer) return 0;
   BatchView& v = b.dev.view;
+  hipStream_t s = b.stream;
   std::vector<WorldHeader> headers(b.worlds.size());
-  F2D_HIP(hipMemcpyAsync(headers.data(), v.worlds, sizeof(WorldHeader) * headers.size(), hipMemcpyDeviceToHost, b.stream));
-  F2D_HIP(hipStreamSynchronize(b.stream));
+  std::vector<BodySim> sims(b.nBodies);
+  std::vector<BodyState> states(b.nBodies);
+  std::vector<Shape> shapes(b.nShapes);
+  std::vector<Contact> contacts(b.nContacts);
+  std::vector<int> freeIds(b.nContacts), overflow(b.nContacts), events(3 * b.nContacts), islandData(b.nIsland);
+  std::vector<Joint> joints(b.nJoints);
+  std::vector<uint32_t> colorBits(b.nColorWords);
+  std::vector<int4> links(b.nContacts);
+  F2D_HIP(hipMemcpyAsync(headers.data(), v.worlds, sizeof(WorldHeader) * headers.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(sims.data(), v.sims, sizeof(BodySim) * sims.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(states.data(), v.states, sizeof(BodyState) * states.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(shapes.data(), v.shapes, sizeof(Shape) * shapes.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(contacts.data(), v.contacts, sizeof(Contact) * contacts.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(freeIds.data(), v.freeIds, sizeof(int) * freeIds.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(overflow.data(), v.overflow, sizeof(int) * overflow.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(events.data(), v.events, sizeof(int) * events.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(islandData.data(), v.islandData, sizeof(int) * islandData.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(joints.data(), v.joints, sizeof(Joint) * joints.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(colorBits.data(), v.colorBits, sizeof(uint32_t) * colorBits.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(links.data(), v.contactLinks, sizeof(int4) * links.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipStreamSynchronize(s));
   for (size_t i = 0; i < b.worlds.size(); ++i) {
     HostWorld& hw = b.worlds[i];
     const WorldHeader& h = headers[i];
-    hipStream_t s = b.stream;
-    F2D_HIP(hipMemcpyAsync(hw.sims.data(), v.sims + h.bodyOffset, sizeof(BodySim) * h.bodyCount, hipMemcpyDeviceToHost, s));
-    F2D_HIP(hipMemcpyAsync(hw.states.data(), v.states + h.bodyOffset, sizeof(BodyState) * h.bodyCount, hipMemcpyDeviceToHost, s));
-    F2D_HIP(hipMemcpyAsync(hw.shapes.data(), v.shapes + h.shapeOffset, sizeof(Shape) * h.shapeCount, hipMemcpyDeviceToHost, s));
-    hw.contacts.resize(h.contactCap);
-    F2D_HIP(hipMemcpyAsync(hw.contacts.data(), v.contacts + h.contactOffset, sizeof(Contact) * h.contactCap, hipMemcpyDeviceToHost, s));
-    hw.freeIds.resize(h.freeCount);
-    if (h.freeCount)
-      F2D_HIP(hipMemcpyAsync(hw.freeIds.data(), v.freeIds + h.freeOffset, sizeof(int) * h.freeCount, hipMemcpyDeviceToHost, s));
-    hw.overflow.resize(h.overflowCount);
-    if (h.overflowCount)
-      F2D_HIP(hipMemcpyAsync(hw.overflow.data(), v.overflow + h.overflowOffset, sizeof(int) * h.overflowCount, hipMemcpyDeviceToHost, s));
-    if (!hw.joints.empty())
-      F2D_HIP(hipMemcpyAsync(hw.joints.data(), v.joints + h.jointOffset, sizeof(Joint) * hw.joints.size(), hipMemcpyDeviceToHost, s));
-    F2D_HIP(hipMemcpyAsync(hw.colorBits.data(), v.colorBits + h.colorOffset, sizeof(uint32_t) * hw.colorBits.size(), hipMemcpyDeviceToHost, s));
-    hw.beginIds.resize(h.beginCount);
-    hw.endPairs.resize(2 * h.endCount);
-    if (h.beginCount)
-      F2D_HIP(hipMemcpyAsync(hw.beginIds.data(), v.events + 3L * h.contactOffset, sizeof(int) * h.beginCount, hipMemcpyDeviceToHost, s));
-    if (h.endCount)
-      F2D_HIP(hipMemcpyAsync(hw.endPairs.data(), v.events + 3L * h.contactOffset + h.contactCap, sizeof(int) * 2 * h.endCount, hipMemcpyDeviceToHost, s));
-    F2D_HIP(hipStreamSynchronize(s));
-    WorldHeader keep = hw.hdr;
+    std::copy(sims.begin() + h.bodyOffset, sims.begin() + h.bodyOffset + h.bodyCount, hw.sims.begin());
+    std::copy(states.begin() + h.bodyOffset, states.begin() + h.bodyOffset + h.bodyCount, hw.states.begin());
+    std::copy(shapes.begin() + h.shapeOffset, shapes.begin() + h.shapeOffset + h.shapeCount, hw.shapes.begin());
+    hw.contacts.assign(contacts.begin() + h.contactOffset, contacts.begin() + h.contactOffset + h.contactCap);
+    hw.links.assign(links.begin() + h.contactOffset, links.begin() + h.contactOffset + h.contactCap);
+    hw.freeIds.assign(freeIds.begin() + h.freeOffset, freeIds.begin() + h.freeOffset + h.freeCount);
+    hw.overflow.assign(overflow.begin() + h.overflowOffset, overflow.begin() + h.overflowOffset + h.overflowCount);
+    std::copy(joints.begin() + h.jointOffset, joints.begin() + h.jointOffset + hw.joints.size(), hw.joints.begin());
+    std::copy(colorBits.begin() + h.colorOffset, colorBits.begin() + h.colorOffset + hw.colorBits.size(),
+              hw.colorBits.begin());
+    const int* ev = events.data() + 3L * h.contactOffset;
+    hw.beginIds.assign(ev, ev + h.beginCount);
+    hw.endPairs.assign(ev + h.contactCap, ev + h.contactCap + 2 * h.endCount);
+    {
+      IslandRegion r = islandRegion(h.islandCap, h.bodyCount);
+      const int* base = islandData.data() + h.islandOffset;
+      std::memcpy(hw.islands.data(), base + r.islands, sizeof(Island) * h.islandCap);
+      std::memcpy(hw.islandFree.data(), base + r.islandFree, sizeof(int) * h.islandCap);
+      std::memcpy(hw.awakeIslands.data(), base + r.awakeIslands, sizeof(int) * h.islandCap);
+      std::memcpy(hw.setFree.data(), base + r.setFree, sizeof(int) * (h.islandCap + 4));
+      std::memcpy(hw.awakeBodies.data(), base + r.awakeBodies, sizeof(int) * h.bodyCount);
+      std::memcpy(hw.moveOrder.data(), base + r.moveOrder, sizeof(int) * h.bodyCount);
+    }
     hw.hdr = h;
-    (void)keep;
     hw.contactCount = h.contactCount;
     hw.stepIndex = h.stepIndex;
     hw.contactSpeed = h.contactSpeed;
@@ -836,7 +956,7 @@ static void put(std::vector<unsigned char>& out, const T& v) {
 static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
   SnapHeader h = {};
   h.magic = 0x50414E53;
-  h.version = 1;
+  h.version = 2;
   h.step = hw.stepIndex;
   h.body_count = (int)hw.sims.size();
   h.shape_count = (int)hw.shapes.size();
@@ -866,8 +986,8 @@ static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
     const BodyState& st = hw.states[i];
     SnapBody sb = {};
     sb.type = s.type;
-    sb.set_index = s.type == kStatic ? 0 : 2;
-    sb.local_index = (int)i;
+    sb.set_index = s.setIndex;
+    sb.local_index = s.localIndex;
     sb.flags = ((s.flags & BF_IS_FAST) ? 1 : 0) | ((s.flags & BF_BULLET) ? 2 : 0) | ((s.flags & BF_SPEED_CAPPED) ? 4 : 0);
     sb.px = s.xf.p.x;
     sb.py = s.xf.p.y;
@@ -881,7 +1001,7 @@ static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
     sb.q0s = s.rotation0.s;
     sb.lcx = s.localCenter.x;
     sb.lcy = s.localCenter.y;
-    if (s.type != kStatic) {
+    if (s.setIndex == kAwakeSet) {  // only awake bodies keep a velocity state
       sb.vx = st.v.x;
       sb.vy = st.v.y;
       sb.w = st.w;
@@ -918,7 +1038,7 @@ static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
     if (c.shapeA >= 0) {
       sc.shape_a = c.shapeA;
       sc.shape_b = c.shapeB;
-      sc.flags = c.flags;
+      sc.flags = c.flags & CF_REF_MASK;
       sc.color = c.color;
       sc.local_index = c.localIndex;
       sc.sim_flags = c.simFlags;
@@ -975,6 +1095,127 @@ static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
     for (int id : colorLists[c]) put(out, (int32_t)id);
   for (auto& m : moves) put(out, (int32_t)m.second);
   for (int id : hw.freeIds) put(out, (int32_t)id);
+  // island section (version 2)
+  const WorldHeader& wh = hw.hdr;
+  int32_t ih[16] = {wh.islandCount, wh.islandFreeCount, wh.awakeIslandCount, wh.splitIslandId, wh.setCount,
+                    wh.setFreeCount, wh.awakeBodyCount, wh.islandCount, wh.setCount, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < 16; ++k) put(out, ih[k]);
+  for (int k = 0; k < wh.islandCount; ++k) put(out, hw.islands[k]);
+  for (int k = 0; k < wh.islandFreeCount; ++k) put(out, (int32_t)hw.islandFree[k]);
+  for (int k = 0; k < wh.awakeIslandCount; ++k) put(out, (int32_t)hw.awakeIslands[k]);
+  for (int k = 0; k < wh.setFreeCount; ++k) put(out, (int32_t)hw.setFree[k]);
+  for (int k = 0; k < wh.awakeBodyCount; ++k) put(out, (int32_t)hw.awakeBodies[k]);
+  for (const BodySim& b : hw.sims) {
+    int32_t r[4] = {b.islandId, b.islandPrev, b.islandNext, b.headContactKey};
+    for (int32_t x : r) put(out, x);
+  }
+  for (int i = 0; i < hw.contactCount; ++i) {
+    const Contact& c = hw.contacts[i];
+    int32_t r[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    if (c.shapeA >= 0) {
+      const int4 l = i < (int)hw.links.size() ? hw.links[i] : make_int4(-1, -1, -1, kAwakeSet);
+      r[0] = l.x;
+      r[1] = l.y;
+      r[2] = l.z;
+      r[3] = l.w;
+      r[4] = c.prevA;
+      r[5] = c.nextA;
+      r[6] = c.prevB;
+      r[7] = c.nextB;
+    }
+    for (int32_t x : r) put(out, x);
+  }
+  for (const Joint& j : hw.joints) {
+    int32_t r[4] = {j.islandId, j.islandPrev, j.islandNext, j.setIndex};
+    for (int32_t x : r) put(out, x);
+  }
+}
+
+// The island section of a version-2 snapshot: island records, id pools, the awake
+// set's orders, body sets and island links, contact sets, island links and body-list
+// edges, joint sets and island links.
+static int importIslands(HostWorld& hw, const SnapHeader& h, const unsigned char* buf, long off, long bytes,
+                         std::string& err) {
+  auto readInts = [&](int32_t* dst, long n) -> bool {
+    if (n < 0 || off + 4 * n > bytes) return false;
+    std::memcpy(dst, buf + off, 4 * n);
+    off += 4 * n;
+    return true;
+  };
+  int32_t ih[16];
+  if (!readInts(ih, 16)) {
+    err = "snapshot island section truncated";
+    return -1;
+  }
+  WorldHeader& wh = hw.hdr;
+  const int cap = std::max((int)hw.islands.size(), 1);
+  if (ih[0] < 0 || ih[0] > cap || ih[1] < 0 || ih[1] > cap || ih[2] < 0 || ih[2] > cap || ih[5] < 0 ||
+      ih[5] > cap + 4 || ih[6] < 0 || ih[6] > (int)hw.sims.size()) {
+    err = "snapshot island section exceeds this world's island capacity";
+    return -1;
+  }
+  hw.islands.resize(cap);
+  hw.islandFree.resize(cap);
+  hw.awakeIslands.resize(cap);
+  hw.setFree.resize(cap + 4);
+  hw.awakeBodies.resize(hw.sims.size());
+  hw.moveOrder.resize(hw.sims.size());
+  bool ok = readInts(reinterpret_cast<int32_t*>(hw.islands.data()), 16L * ih[0]) &&
+            readInts(hw.islandFree.data(), ih[1]) && readInts(hw.awakeIslands.data(), ih[2]) &&
+            readInts(hw.setFree.data(), ih[5]) && readInts(hw.awakeBodies.data(), ih[6]);
+  std::vector<int32_t> bl(4L * hw.sims.size()), cl(8L * h.contact_cap), jl(4L * hw.joints.size());
+  ok = ok && readInts(bl.data(), (long)bl.size()) && readInts(cl.data(), (long)cl.size()) &&
+       readInts(jl.data(), (long)jl.size());
+  if (!ok) {
+    err = "snapshot island section truncated";
+    return -1;
+  }
+  wh.islandCount = ih[0];
+  wh.islandFreeCount = ih[1];
+  wh.awakeIslandCount = ih[2];
+  wh.splitIslandId = ih[3];
+  wh.setCount = ih[4];
+  wh.setFreeCount = ih[5];
+  wh.awakeBodyCount = ih[6];
+  wh.moveBodyCount = 0;
+  // body sets and indices come from the body records
+  const unsigned char* bodies = buf + sizeof(SnapHeader);
+  for (size_t i = 0; i < hw.sims.size(); ++i) {
+    SnapBody sb;
+    std::memcpy(&sb, bodies + i * sizeof(SnapBody), sizeof(sb));
+    BodySim& s = hw.sims[i];
+    s.setIndex = sb.set_index;
+    s.localIndex = sb.local_index;
+    s.islandId = bl[4 * i];
+    s.islandPrev = bl[4 * i + 1];
+    s.islandNext = bl[4 * i + 2];
+    s.headContactKey = bl[4 * i + 3];
+  }
+  for (int i = 0; i < h.contact_cap; ++i) {
+    Contact& c = hw.contacts[i];
+    if (c.shapeA < 0) continue;
+    const int32_t* r = cl.data() + 8L * i;
+    hw.links[i] = make_int4(r[0], r[1], r[2], r[3]);
+    c.prevA = r[4];
+    c.nextA = r[5];
+    c.prevB = r[6];
+    c.nextB = r[7];
+    if (r[3] != kAwakeSet) c.flags |= CF_OFF_AWAKE;
+    if (r[3] >= kFirstSleepingSet) {  // index in its sleeping set
+      SnapContact sc;
+      std::memcpy(&sc, buf + sizeof(SnapHeader) + hw.sims.size() * sizeof(SnapBody) +
+                           hw.shapes.size() * sizeof(SnapShape) + (size_t)i * sizeof(SnapContact), sizeof(sc));
+      c.localIndex = sc.local_index;
+    }
+  }
+  for (size_t i = 0; i < hw.joints.size(); ++i) {
+    Joint& j = hw.joints[i];
+    j.islandId = jl[4 * i];
+    j.islandPrev = jl[4 * i + 1];
+    j.islandNext = jl[4 * i + 2];
+    j.setIndex = jl[4 * i + 3];
+  }
+  return 0;
 }
 
 // Overlay a snapshot onto a world built from the same scene spec.
@@ -1175,6 +1416,9 @@ static int importWorld(HostWorld& hw, const unsigned char* buf, long bytes, std:
   hw.stepIndex = h.step;
   hw.contactSpeed = h.contact_speed;
   hw.inv_h = h.inv_h;
+  hw.links.assign(h.contact_cap, make_int4(-1, -1, -1, -1));
+  for (int id : live) hw.links[id] = make_int4(-1, -1, -1, kAwakeSet);
+  if (h.version >= 2) return importIslands(hw, h, buf, off, bytes, err);
   return 0;
 }
 

@@ -28,6 +28,7 @@ enum : int {
   BF_SPEED_CAPPED = 32,
   BF_ENLARGED = 64,
   BF_CAPPED_STEP = 128,  // speed capped during the current step (reported at finalize)
+  BF_FELL_ASLEEP = 256,  // put to sleep by the last step (its move event's fellAsleep)
 };
 // shape flags
 enum : int {
@@ -42,6 +43,8 @@ enum : int {
 enum : int {
   CF_TOUCHING = 0x1,
   CF_CONTACT_EVENTS = 0x4,  // b2_contactEnableContactEvents (0x2 is the unused hit-event flag)
+  CF_OFF_AWAKE = 0x100,     // not in the awake set (disabled or sleeping set): not collided
+  CF_REF_MASK = 0x7,        // the bits that are the reference's b2ContactFlags
   SIM_TOUCHING = 0x00010000,
   SIM_DISJOINT = 0x00020000,
   SIM_STARTED = 0x00040000,
@@ -89,7 +92,9 @@ struct alignas(16) BodySim {
   int jointCount;
   float sleepThreshold;
   float sleepTime;
-  int pad[2];
+  int islandId, islandPrev, islandNext;  // island graph (b2Body, S/body.h:40-47)
+  int setIndex;    // solver set: kStaticSet, kAwakeSet or a sleeping set
+  int localIndex;  // index within the set (the awake set's order is the sim order)
 };
 
 struct alignas(16) ShapeGeom {
@@ -160,7 +165,8 @@ struct alignas(16) Joint {
   Softness constraintSoftness, springSoftness;
   Vec2 anchorA, anchorB, deltaCenter;
   float deltaAngle, axialMass;
-  int pad;
+  int islandId, islandPrev, islandNext;  // island graph (b2Joint, S/joint.h:44-48)
+  int setIndex;
 };
 
 // Per-world header. Offsets index the batch's flat arrays.
@@ -190,7 +196,55 @@ struct alignas(16) WorldHeader {
   int staticCount;               // static proxies listed at staticShapes[shapeOffset ...]
   int beginCount, endCount;      // contact begin/end touch events of the last collide
   float hitEventThreshold;       // b2WorldDef::hitEventThreshold
+  // islands and solver sets (S/island.c, S/solver_set.c); region at islandOffset of
+  // the batch's island array, laid out by islandRegion()
+  int islandOffset, islandCap;   // cap: islands a world can hold (its non-static bodies)
+  int islandCount;               // island array count = island id-pool next index
+  int islandFreeCount;
+  int awakeIslandCount;
+  int splitIslandId;             // candidate chosen by the last step's finalize
+  int setCount, setFreeCount;    // solver-set id pool (next index, free list depth)
+  int awakeBodyCount;
+  int moveBodyCount;             // awake bodies finalized by the last step (move events)
+  int sleepPending;              // finalize ran with sleep enabled: island sleep follows
+  unsigned long long splitKey;   // finalize: sleepiest body of an island with removed constraints
 };
+
+// Solver sets, S/solver_set.h
+enum : int { kStaticSet = 0, kDisabledSet = 1, kAwakeSet = 2, kFirstSleepingSet = 3 };
+
+// b2Island, S/island.h:27-58 (also the snapshot's island record)
+struct Island {
+  int id, setIndex, localIndex;
+  int headBody, tailBody, bodyCount;
+  int headContact, tailContact, contactCount;
+  int headJoint, tailJoint, jointCount;
+  int parent;          // union-find parent, -1 for a root
+  int removeCount;     // constraints removed since the island was built (split candidate)
+  int reserved[2];
+};
+
+// Per-world island region (ints): island records, the island id pool's free list,
+// the awake set's island order, the solver-set id pool's free list, the awake set's
+// body order, the body order of the last finalize (move events), the finalize's
+// awake-island bits, and the split's depth-first stack and seed list.
+struct IslandRegion {
+  long islands, islandFree, awakeIslands, setFree, awakeBodies, moveOrder, awakeBits, stack, seeds, total;
+};
+F2D_HD IslandRegion islandRegion(int cap, int bodies) {
+  IslandRegion r;
+  r.islands = 0;
+  r.islandFree = r.islands + 16L * cap;
+  r.awakeIslands = r.islandFree + cap;
+  r.setFree = r.awakeIslands + cap;
+  r.awakeBodies = r.setFree + cap + 4;
+  r.moveOrder = r.awakeBodies + bodies;
+  r.awakeBits = r.moveOrder + bodies;
+  r.stack = r.awakeBits + (cap + 31) / 32;
+  r.seeds = r.stack + bodies;
+  r.total = (r.seeds + bodies + 3) & ~3L;
+  return r;
+}
 
 // Per-step scalars, S/world.c:743-771
 struct StepParams {
@@ -251,6 +305,8 @@ struct BatchView {
   int8_t* ccolor;       // per contact slot: constraint-graph colour, -1 when none or free
   int* staticShapes;    // per world, the shape ids of its static proxies in shape order
   int* events;          // per world at 3 * contactOffset: begin contact ids [contactCap], end shape pairs [2 * contactCap]
+  int* islandData;      // per world at islandOffset: the island region
+  int4* contactLinks;   // per contact slot: island id, island prev, island next, solver set
   int worldCount;
 };
 

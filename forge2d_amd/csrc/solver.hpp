@@ -574,7 +574,8 @@ F2D_HD BodyConst makeBodyConst(const BodySim& sim, Vec2 gravity, float h) {
   k.dw = h * sim.invI * sim.torque;
   k.invMass = sim.invMass;
   k.invI = sim.invI;
-  k.flags = (sim.type == kStatic ? BC_STATIC : 0) | ((sim.flags & BF_ALLOW_FAST_ROTATION) ? BC_FAST_ROTATION : 0);
+  // bodies outside the awake set (static or sleeping) take no solver updates
+  k.flags = (sim.setIndex != kAwakeSet ? BC_STATIC : 0) | ((sim.flags & BF_ALLOW_FAST_ROTATION) ? BC_FAST_ROTATION : 0);
   return k;
 }
 

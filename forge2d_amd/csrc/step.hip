@@ -24,6 +24,7 @@
 #include "solver.hpp"
 #include "joint.hpp"
 #include "toi.hpp"
+#include "island.hpp"
 
 namespace f2d {
 
@@ -42,6 +43,7 @@ struct SharedState {
   int changedCount;
   int error;
   int pairCount;
+  int noAwake;  // no awake body: b2Solve returns after merging islands
 };
 
 // ---------------------------------------------------------------------------- block helpers
@@ -169,6 +171,8 @@ struct WorldPtrs {
   float4* sapBoxes;
   int8_t* ccolor;
   int* events;
+  int4* links;      // contact island links and solver set
+  IslandCtx isl;
 };
 
 __device__ WorldPtrs worldPtrs(const BatchView& b, WorldHeader* h) {
@@ -191,6 +195,8 @@ __device__ WorldPtrs worldPtrs(const BatchView& b, WorldHeader* h) {
   w.ccolor = b.ccolor + h->contactOffset;
   w.events = b.events + 3L * h->contactOffset;
   w.sapBoxes = b.sapBoxes + h->shapeOffset;
+  w.links = b.contactLinks + h->contactOffset;
+  w.isl = islandCtx(h, b.islandData + h->islandOffset, w.sims, w.contacts, w.links, w.joints);
   return w;
 }
 
@@ -313,6 +319,10 @@ __device__ void createContact(const WorldPtrs& w, int shapeIdA, int shapeIdB) {
   c.restitution = maxf(A.restitution, B.restitution);
   c.rollingResistance = 0.0f;
   c.tangentSpeed = 0.0f;
+  // the awake set when either body is awake, else the disabled set (S/contact.c:213-224)
+  int set = (w.sims[bodyA].setIndex == kAwakeSet || w.sims[bodyB].setIndex == kAwakeSet) ? kAwakeSet : kDisabledSet;
+  if (set != kAwakeSet) c.flags |= CF_OFF_AWAKE;
+  w.links[id] = make_int4(-1, -1, -1, set);
 }
 
 template <int BLOCK>
@@ -495,7 +505,7 @@ __device__ void collidePhase(const WorldPtrs& w, SharedState& sh) {
   const int n = h->contactCount;
   for (int i = threadIdx.x; i < n; i += BLOCK) {
     Contact& c = w.contacts[i];
-    if (c.shapeA < 0) continue;
+    if (c.shapeA < 0 || (c.flags & CF_OFF_AWAKE)) continue;  // only the awake set collides (S/world.c:496-521)
     AABB fa = w.shapes[c.shapeA].fat, fb = w.shapes[c.shapeB].fat;
     if (!aabbOverlaps(fa, fb)) {
       c.simFlags |= SIM_DISJOINT;
@@ -514,7 +524,8 @@ __device__ void collidePhase(const WorldPtrs& w, SharedState& sh) {
     bool f = false;
     if (i < n) {
       const Contact& c = w.contacts[i];
-      f = c.shapeA >= 0 && (c.simFlags & (SIM_DISJOINT | SIM_STARTED | SIM_STOPPED)) != 0;
+      f = c.shapeA >= 0 && (c.flags & CF_OFF_AWAKE) == 0 &&
+          (c.simFlags & (SIM_DISJOINT | SIM_STARTED | SIM_STOPPED)) != 0;
     }
     int total;
     int slot = blockCompact<BLOCK>(f, sh.waveSums, &total);
@@ -642,8 +653,196 @@ __device__ void destroyContact(const WorldPtrs& w, int id) {
   c.shapeB = -1;
   c.flags = 0;
   c.simFlags = 0;
+  w.links[id] = make_int4(-1, -1, -1, -1);
   WorldHeader* h = w.hdr;
   w.freeIds[h->freeCount++] = id;
+}
+
+// b2AssignJointColor + b2CreateJointInGraph, S/constraint_graph.c:214-290. The
+// joint's index in its colour is the count of joints already there.
+__device__ void addJointToGraph(const WorldPtrs& w, int jid) {
+  Joint& j = w.joints[jid];
+  const bool staticA = w.sims[j.bodyA].setIndex == kStaticSet;
+  const bool staticB = w.sims[j.bodyB].setIndex == kStaticSet;
+  int color = kOverflowIndex;
+  for (int i = 0; i < kOverflowIndex; ++i) {
+    uint32_t* set = colorSet(w, i);
+    if (!staticA && !staticB) {
+      if (getBit(set, j.bodyA) || getBit(set, j.bodyB)) continue;
+      setBit(set, j.bodyA);
+      setBit(set, j.bodyB);
+    } else if (!staticA) {
+      if (getBit(set, j.bodyA)) continue;
+      setBit(set, j.bodyA);
+    } else {
+      if (getBit(set, j.bodyB)) continue;
+      setBit(set, j.bodyB);
+    }
+    color = i;
+    break;
+  }
+  int local = 0;
+  for (int k = 0; k < w.hdr->jointCount; ++k) local += (k != jid && w.joints[k].color == color) ? 1 : 0;
+  j.color = color;
+  j.localIndex = local;
+}
+
+// b2RemoveJointFromGraph, S/constraint_graph.c:292-322: the colour's last joint
+// takes the removed joint's index
+__device__ void removeJointFromGraph(const WorldPtrs& w, int jid) {
+  Joint& j = w.joints[jid];
+  const int color = j.color;
+  if (color < 0) return;
+  if (color != kOverflowIndex) {
+    uint32_t* set = colorSet(w, color);
+    clearBit(set, j.bodyA);
+    clearBit(set, j.bodyB);
+  }
+  int count = 0;
+  for (int k = 0; k < w.hdr->jointCount; ++k) count += w.joints[k].color == color ? 1 : 0;
+  const int last = count - 1;
+  if (j.localIndex != last) {
+    for (int k = 0; k < w.hdr->jointCount; ++k) {
+      if (k != jid && w.joints[k].color == color && w.joints[k].localIndex == last) {
+        w.joints[k].localIndex = j.localIndex;
+        break;
+      }
+    }
+  }
+  j.color = -1;
+  j.localIndex = -1;
+}
+
+// b2WakeSolverSet, S/solver_set.c:31-154. A sleeping set holds exactly one island
+// (sets merge only through joints created between sleeping bodies); its bodies,
+// touching contacts and joints are that island's lists in order.
+__device__ void wakeIslandSet(const WorldPtrs& w, int islandId) {
+  IslandCtx c = w.isl;
+  WorldHeader* h = w.hdr;
+  Island& is = c.islands[islandId];
+  const int setIndex = is.setIndex;
+  for (int b = is.headBody, g = h->bodyCount; b != -1 && g > 0; b = w.sims[b].islandNext, --g) {
+    BodySim& body = w.sims[b];
+    body.setIndex = kAwakeSet;
+    body.localIndex = h->awakeBodyCount;
+    c.awakeBodies[h->awakeBodyCount++] = b;
+    body.sleepTime = 0.0f;
+    w.states[b] = identityState();
+    // its non-touching contacts leave the disabled set
+    for (int key = body.headContactKey, gk = 2 * h->contactCap; key != -1 && gk > 0; --gk) {
+      int cid = key >> 1;
+      Contact& ct = w.contacts[cid];
+      key = (key & 1) ? ct.nextB : ct.nextA;
+      if (w.links[cid].w != kDisabledSet) continue;
+      w.links[cid].w = kAwakeSet;
+      ct.flags &= ~CF_OFF_AWAKE;
+      ct.localIndex = -1;
+    }
+  }
+  for (int q = is.headContact, g = h->contactCap; q != -1 && g > 0; q = w.links[q].z, --g) {
+    w.contacts[q].localIndex = -1;
+    addContactToGraph(w, q);
+    w.links[q].w = kAwakeSet;
+    w.contacts[q].flags &= ~CF_OFF_AWAKE;
+  }
+  for (int j = is.headJoint, g = h->jointCount; j != -1 && g > 0; j = w.joints[j].islandNext, --g) {
+    addJointToGraph(w, j);
+    w.joints[j].setIndex = kAwakeSet;
+  }
+  is.setIndex = kAwakeSet;
+  is.localIndex = h->awakeIslandCount;
+  c.awakeIslands[h->awakeIslandCount++] = islandId;
+  c.setFree[h->setFreeCount++] = setIndex;  // b2DestroySolverSet
+}
+
+// b2LinkContact, S/island.c:117-218: a touching contact between an awake and a
+// sleeping body wakes the sleeping set first
+__device__ void linkContact(const WorldPtrs& w, int id) {
+  const Contact& ct = w.contacts[id];
+  const BodySim& A = w.sims[ct.bodyA];
+  const BodySim& B = w.sims[ct.bodyB];
+  if (A.setIndex == kAwakeSet && B.setIndex >= kFirstSleepingSet) wakeIslandSet(w, B.islandId);
+  if (B.setIndex == kAwakeSet && A.setIndex >= kFirstSleepingSet) wakeIslandSet(w, A.islandId);
+  IslandCtx c = w.isl;
+  linkContactIslands(c, id);
+}
+
+// b2TrySleepIsland, S/solver_set.c:156-420: the island's bodies leave the awake
+// set (remove-swap), its non-touching contacts to sleeping or static bodies go to
+// the disabled set, its touching contacts and joints leave the constraint graph.
+__device__ void trySleepIsland(const WorldPtrs& w, int islandId) {
+  IslandCtx c = w.isl;
+  WorldHeader* h = w.hdr;
+  Island& is = c.islands[islandId];
+  if (is.removeCount > 0) return;
+  const int setId = poolAlloc(c.setFree, h->setFreeCount, h->setCount);
+  int sleepIndex = 0;
+  for (int b = is.headBody, g = h->bodyCount; b != -1 && g > 0; b = w.sims[b].islandNext, --g) {
+    BodySim& body = w.sims[b];
+    body.flags |= BF_FELL_ASLEEP;
+    const int li = body.localIndex, last = --h->awakeBodyCount;
+    if (li != last) {
+      const int moved = c.awakeBodies[last];
+      c.awakeBodies[li] = moved;
+      w.sims[moved].localIndex = li;
+    }
+    body.setIndex = setId;
+    body.localIndex = sleepIndex++;
+    w.states[b] = identityState();
+    for (int key = body.headContactKey, gk = 2 * h->contactCap; key != -1 && gk > 0; --gk) {
+      const int cid = key >> 1, edge = key & 1;
+      Contact& ct = w.contacts[cid];
+      key = edge ? ct.nextB : ct.nextA;
+      if (w.links[cid].w == kDisabledSet) continue;
+      if (ct.color != -1) continue;
+      const int other = edge ? ct.bodyA : ct.bodyB;
+      if (w.sims[other].setIndex == kAwakeSet) continue;
+      w.links[cid].w = kDisabledSet;
+      ct.flags |= CF_OFF_AWAKE;
+      ct.localIndex = -1;
+    }
+  }
+  int k = 0;
+  for (int q = is.headContact, g = h->contactCap; q != -1 && g > 0; q = w.links[q].z, --g) {
+    removeContactFromGraph(w, q);
+    w.links[q].w = setId;
+    w.contacts[q].flags |= CF_OFF_AWAKE;
+    w.contacts[q].localIndex = k++;
+  }
+  k = 0;
+  for (int j = is.headJoint, g = h->jointCount; j != -1 && g > 0; j = w.joints[j].islandNext, --g) {
+    removeJointFromGraph(w, j);
+    w.joints[j].setIndex = setId;
+    w.joints[j].localIndex = k++;
+  }
+  const int li = is.localIndex, last = --h->awakeIslandCount;
+  if (li != last) {
+    const int moved = c.awakeIslands[last];
+    c.awakeIslands[li] = moved;
+    c.islands[moved].localIndex = li;
+  }
+  is.setIndex = setId;
+  is.localIndex = 0;
+}
+
+// Island sleep after finalize and continuous collision, S/solver.c:1995-2049: the
+// split candidate for the next step, then every awake island no body kept awake,
+// in reverse awake order (thread 0)
+__device__ void sleepPhase(const WorldPtrs& w) {
+  IslandCtx c = w.isl;
+  WorldHeader* h = w.hdr;
+  h->sleepPending = 0;
+  int candidate = -1;
+  const unsigned long long key = h->splitKey;
+  if (key != 0ull) {
+    const int simIndex = (int)(0xFFFFFFFFu - (uint32_t)key);
+    if (simIndex >= 0 && simIndex < h->moveBodyCount) candidate = w.sims[c.moveOrder[simIndex]].islandId;
+  }
+  h->splitIslandId = candidate;
+  for (int i = h->awakeIslandCount - 1; i >= 0; --i) {
+    if ((c.awakeBits[i >> 5] >> (i & 31)) & 1u) continue;
+    trySleepIsland(w, c.awakeIslands[i]);
+  }
 }
 
 // Serial contact state loop, S/world.c:587-690
@@ -666,10 +865,15 @@ __device__ void statePhase(const WorldPtrs& w, SharedState& sh) {
           endPairs[2 * nEnd + 1] = c.shapeB;
           ++nEnd;
         }
+        if (w.links[id].x != -1) {
+          IslandCtx ic = w.isl;
+          unlinkContact(ic, id);
+        }
         destroyContact(w, id);
       } else if (simFlags & SIM_STARTED) {
         if ((c.flags & CF_CONTACT_EVENTS) && nBegin < h->contactCap) beginIds[nBegin++] = id;
         c.flags |= CF_TOUCHING;
+        linkContact(w, id);
         c.simFlags &= ~SIM_STARTED;
         addContactToGraph(w, id);
       } else if (simFlags & SIM_STOPPED) {
@@ -679,6 +883,10 @@ __device__ void statePhase(const WorldPtrs& w, SharedState& sh) {
           endPairs[2 * nEnd] = c.shapeA;
           endPairs[2 * nEnd + 1] = c.shapeB;
           ++nEnd;
+        }
+        if (w.links[id].x != -1) {
+          IslandCtx ic = w.isl;
+          unlinkContact(ic, id);
         }
         removeContactFromGraph(w, id);
       }
@@ -711,6 +919,24 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
   BodyConst* ldsConsts = reinterpret_cast<BodyConst*>(ldsStates + nb);  // LDS only
   const Cons& cons = w.cons;
 
+  // b2Solve, S/solver.c:1190-1220 and :1479-1706: merge islands linked since the last
+  // step, stop when no body is awake, split the last step's candidate island
+  if (phases & PH_SOLVE) {
+    if (threadIdx.x == 0) {
+      IslandCtx ic = w.isl;
+      mergeAwakeIslands(ic);
+      sh.noAwake = h->awakeBodyCount == 0 ? 1 : 0;
+      if (!sh.noAwake && h->splitIslandId != -1) splitIsland(ic, h->splitIslandId);
+      if (!sh.noAwake) h->splitIslandId = -1;
+    }
+    __syncthreads();
+    if (sh.noAwake) return;
+  } else if (phases & PH_FINALIZE) {
+    if (threadIdx.x == 0) sh.noAwake = h->awakeBodyCount == 0 ? 1 : 0;
+    __syncthreads();
+    if (sh.noAwake) return;
+  }
+
   if (phases & PH_SOLVE) {
     // -- bucket touching constraints by colour (overflow keeps its list order)
     if (threadIdx.x < kGraphColorCount) {
@@ -726,7 +952,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     }
     for (int i = threadIdx.x; i < h->jointCount; i += BLOCK) {
       const Joint& j = w.joints[i];
-      if (j.color >= 0 && j.color < kOverflowIndex) atomicAdd(&sh.jointColorCount[j.color], 1);
+      if (j.color >= 0) atomicAdd(&sh.jointColorCount[j.color], 1);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -762,6 +988,18 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         jointLanes[sh.jointColorStart[j.color] + atomicAdd(&sh.jointColorFill[j.color], 1)] = i;
       }
     }
+    if (threadIdx.x == 0) {
+      // overflow joints are solved one after another in their colour's array order
+      int* list = jointLanes + sh.jointColorStart[kOverflowIndex];
+      const int n = sh.jointColorCount[kOverflowIndex];
+      for (int k = 0; k < n; ++k) list[k] = -1;
+      for (int i = 0; i < h->jointCount; ++i) {
+        const Joint& j = w.joints[i];
+        if (j.color == kOverflowIndex && j.localIndex >= 0 && j.localIndex < n) list[j.localIndex] = i;
+      }
+      for (int k = 0; k < n; ++k)
+        if (list[k] < 0) h->error |= ERR_OVERFLOW_CAP;
+    }
     // body states and per-step body terms into LDS
     if (kLds) {
       for (int i = threadIdx.x; i < nb; i += BLOCK) {
@@ -774,10 +1012,12 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     const int overflowStart = sh.colorStart[kOverflowIndex];
     const int overflowCount = sh.colorCount[kOverflowIndex];
     const int coloredJoints = sh.jointColorStart[kOverflowIndex];
+    const int overflowJoints = sh.jointColorCount[kOverflowIndex];
+    const int* overflowJointList = jointLanes + coloredJoints;
 
     // -- prepare joints (all), S/solver.c:967-987 -> S/joint.c:1348-1402
     for (int i = threadIdx.x; i < h->jointCount; i += BLOCK) {
-      prepareRevolute(w.joints[i], w.sims, ctx.h, ctx.inv_h, ctx.warmStartScale);
+      if (w.joints[i].color >= 0) prepareRevolute(w.joints[i], w.sims, ctx.h, ctx.inv_h, ctx.warmStartScale);
     }
     // -- prepare contacts (all lanes), S/contact_solver.c:1438-1669
     for (int lane = threadIdx.x; lane < totalLanes; lane += BLOCK) {
@@ -819,7 +1059,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
           ldsConsts[i].flags = k.flags;
         } else {
           BodySim& sim = w.sims[i];
-          if (sim.type == kStatic) continue;
+          if (sim.setIndex != kAwakeSet) continue;
           BodyState st = states[i];
           integrateVelocity(sim, st, ctx.gravity, ctx.h, ctx.maxLinearSpeed, ctx.maxAngularSpeed);
           states[i] = st;
@@ -828,8 +1068,8 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       __syncthreads();
       // warm start: overflow joints, overflow contacts, then colours
       if (threadIdx.x == 0) {
-        for (int j = 0; j < h->jointCount; ++j)
-          if (w.joints[j].color == kOverflowIndex) warmStartRevolute(w.joints[j], states, w.sims);
+        for (int k = 0; k < overflowJoints; ++k)
+          if (overflowJointList[k] >= 0) warmStartRevolute(w.joints[overflowJointList[k]], states, w.sims);
         for (int k = 0; k < overflowCount; ++k) overflowWarmStart(cons, overflowStart + k, states);
       }
       __syncthreads();
@@ -843,8 +1083,9 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       }
       // solve
       if (threadIdx.x == 0) {
-        for (int j = 0; j < h->jointCount; ++j)
-          if (w.joints[j].color == kOverflowIndex) solveRevolute(w.joints[j], states, w.sims, ctx.h, ctx.inv_h, true);
+        for (int k = 0; k < overflowJoints; ++k)
+          if (overflowJointList[k] >= 0)
+            solveRevolute(w.joints[overflowJointList[k]], states, w.sims, ctx.h, ctx.inv_h, true);
         for (int k = 0; k < overflowCount; ++k)
           overflowSolve(cons, overflowStart + k, states, ctx.inv_h, ctx.maxContactPushSpeed, true);
       }
@@ -860,7 +1101,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       }
       // integrate positions
       for (int i = threadIdx.x; i < nb; i += BLOCK) {
-        if (w.sims[i].type == kStatic) continue;
+        if (kLds ? (ldsConsts[i].flags & BC_STATIC) != 0 : w.sims[i].setIndex != kAwakeSet) continue;
         BodyState st = states[i];
         integratePosition(st, ctx.h);
         states[i] = st;
@@ -868,8 +1109,9 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       __syncthreads();
       // relax
       if (threadIdx.x == 0) {
-        for (int j = 0; j < h->jointCount; ++j)
-          if (w.joints[j].color == kOverflowIndex) solveRevolute(w.joints[j], states, w.sims, ctx.h, ctx.inv_h, false);
+        for (int k = 0; k < overflowJoints; ++k)
+          if (overflowJointList[k] >= 0)
+            solveRevolute(w.joints[overflowJointList[k]], states, w.sims, ctx.h, ctx.inv_h, false);
         for (int k = 0; k < overflowCount; ++k)
           overflowSolve(cons, overflowStart + k, states, ctx.inv_h, ctx.maxContactPushSpeed, false);
       }
@@ -915,12 +1157,20 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
 
   if (phases & PH_FINALIZE) {
     // b2FinalizeBodiesTask, S/solver.c:543-726
-    if (threadIdx.x == 0) sh.fastCount = 0;
+    const IslandCtx& ic = w.isl;
+    if (threadIdx.x == 0) {
+      sh.fastCount = 0;
+      h->splitKey = 0ull;
+      h->moveBodyCount = h->awakeBodyCount;
+    }
+    for (int k = threadIdx.x; k < (h->awakeIslandCount + 31) / 32; k += BLOCK) ic.awakeBits[k] = 0u;
+    // the awake order of this finalize (body move events follow it)
+    for (int k = threadIdx.x; k < h->awakeBodyCount; k += BLOCK) ic.moveOrder[k] = ic.awakeBodies[k];
     __syncthreads();
     int* fastList = w.scratch;  // fast bodies for the continuous-collision launch
     for (int i = threadIdx.x; i < nb; i += BLOCK) {
       BodySim& sim = w.sims[i];
-      if (sim.type == kStatic) continue;
+      if (sim.setIndex != kAwakeSet) continue;
       BodyState st = states[i];
       Vec2 v = st.v;
       float wv = st.w;
@@ -937,7 +1187,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       sim.torque = 0.0f;
       // isSpeedCapped reports this step's capping (body->isSpeedCapped = sim->isSpeedCapped)
       bool capped = kLds ? (ldsConsts[i].flags & BC_CAPPED) != 0 : (sim.flags & BF_CAPPED_STEP) != 0;
-      int flags = sim.flags & ~(BF_IS_FAST | BF_ENLARGED | BF_SPEED_CAPPED | BF_CAPPED_STEP);
+      int flags = sim.flags & ~(BF_IS_FAST | BF_ENLARGED | BF_SPEED_CAPPED | BF_CAPPED_STEP | BF_FELL_ASLEEP);
       if (capped) flags |= BF_SPEED_CAPPED;
       bool isFast = false;
       if (!ctx.enableSleep || (sim.flags & BF_ENABLE_SLEEP) == 0 || sleepVelocity > sim.sleepThreshold) {
@@ -956,6 +1206,19 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         sim.rotation0 = sim.xf.q;
         sim.sleepTime += ctx.dt;
       }
+      // any body of an island keeps it awake; the sleepiest body (first in the awake
+      // order on ties) of an island with removed constraints names the island to
+      // split next step (S/solver.c:658-676)
+      if (ctx.enableSleep && sim.islandId >= 0 && sim.islandId < h->islandCap) {
+        const Island& is = ic.islands[sim.islandId];
+        if (sim.sleepTime < kTimeToSleep) {
+          atomicOr(&ic.awakeBits[is.localIndex >> 5], 1u << (is.localIndex & 31));
+        } else if (is.removeCount > 0) {
+          unsigned long long key = ((unsigned long long)__float_as_uint(sim.sleepTime) << 32) |
+                                   (unsigned long long)(0xFFFFFFFFu - (uint32_t)sim.localIndex);
+          atomicMax(&h->splitKey, key);
+        }
+      }
       Xf xf = sim.xf;
       int rank = 0;
       bool enlargedBody = false;
@@ -967,7 +1230,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
           // joins the move buffer with every shape (S/solver.c:1866-1879)
           if (bullet) {
             shape.flags |= SF_MOVED;
-            shape.moveRank = (i << 8) | (rank < 255 ? rank : 255);
+            shape.moveRank = (sim.localIndex << 8) | (rank < 255 ? rank : 255);
             enlargedBody = true;
           }
           continue;
@@ -987,7 +1250,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
           shape.fat = fat;
           // enlarged proxy joins the move buffer in sim order (S/solver.c:1848-1903)
           shape.flags |= SF_MOVED;
-          shape.moveRank = (i << 8) | (rank < 255 ? rank : 255);
+          shape.moveRank = (sim.localIndex << 8) | (rank < 255 ? rank : 255);
           enlargedBody = true;
         }
       }
@@ -997,7 +1260,10 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       w.states[i] = st;
     }
     __syncthreads();
-    if (threadIdx.x == 0) h->fastCount = sh.fastCount;
+    if (threadIdx.x == 0) {
+      h->fastCount = sh.fastCount;
+      h->sleepPending = ctx.enableSleep ? 1 : 0;
+    }
   } else if (kLds && (phases & PH_SOLVE)) {
     for (int i = threadIdx.x; i < nb; i += BLOCK) {
       w.states[i] = ldsStates[i];
@@ -1194,7 +1460,7 @@ __device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList
         fat.hi.y = aabb.hi.y + kAabbMargin;
         shape.fat = fat;
         shape.flags |= SF_MOVED;
-        shape.moveRank = (i << 8) | (rank < 255 ? rank : 255);
+        shape.moveRank = (sim.localIndex << 8) | (rank < 255 ? rank : 255);
         flags |= BF_ENLARGED;
       }
     }
@@ -1212,7 +1478,7 @@ __device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList
         fat.hi.y = shape.aabb.hi.y + kAabbMargin;
         shape.fat = fat;
         shape.flags |= SF_MOVED;
-        shape.moveRank = (i << 8) | (rank < 255 ? rank : 255);
+        shape.moveRank = (sim.localIndex << 8) | (rank < 255 ? rank : 255);
         flags |= BF_ENLARGED;
       }
     }
@@ -1221,13 +1487,15 @@ __device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList
 }
 
 // Continuous collision of the fast bodies finalize listed, one world per
-// workgroup, one fast body per lane. Worlds without fast bodies return at once.
+// workgroup, one fast body per lane, then island sleep for worlds with sleep
+// enabled. Worlds with neither return at once.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void continuousKernel(BatchView b, const int* worldList) {
   const int wi = worldList ? worldList[blockIdx.x] : blockIdx.x;
   WorldHeader* h = b.worlds + wi;
   const int nFast = h->fastCount;
-  if (nFast == 0) return;
+  const bool sleep = h->sleepPending != 0;
+  if (nFast == 0 && !sleep) return;
   WorldPtrs w = worldPtrs(b, h);
   const int ns = h->shapeCount;
   const int* fastList = w.scratch;  // written by finalize, nFast <= shapeCount
@@ -1244,7 +1512,10 @@ __global__ __launch_bounds__(BLOCK) void continuousKernel(BatchView b, const int
     if (w.sims[i].flags & BF_BULLET) solveContinuous(w, i, staticList, staticCount, true);
   }
   __syncthreads();
-  if (threadIdx.x == 0) h->fastCount = 0;
+  if (threadIdx.x == 0) {
+    h->fastCount = 0;
+    if (sleep) sleepPhase(w);
+  }
 }
 
 template <int BLOCK>

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "geometry.hpp"
+#include "island.hpp"
 
 namespace f2d {
 
@@ -35,6 +36,14 @@ struct HostWorld {
   bool hasCallbacks = false;
   std::vector<int> beginIds;       // contact begin-touch events of the last step (contact ids)
   std::vector<int> endPairs;       // contact end-touch events of the last step (shape pairs)
+  // island graph and solver sets (counters in hdr): records, id-pool free lists,
+  // the awake set's island and body orders, the last finalize's body order
+  std::vector<Island> islands;
+  std::vector<int> islandFree, awakeIslands, setFree, awakeBodies, moveOrder;
+  std::vector<int4> links;         // per contact slot: island id, prev, next, solver set
+  int staticBodies = 0;            // static set size (static bodies' index in it)
+
+  IslandCtx islandCtx();           // host view of the island data (valid until the vectors grow)
 
   void ensureColorWords();
   int createBody(int type, Vec2 p, Rot q, Vec2 v, float w, float linDamp, float angDamp, float gravityScale,
@@ -82,6 +91,8 @@ struct Batch {
   hipEvent_t forkEvent = nullptr;
   std::string lastError;
   float contactCapFactor = 6.0f;
+  // flat array lengths of the committed batch (bulk host mirror copies)
+  long nBodies = 0, nShapes = 0, nContacts = 0, nJoints = 0, nColorWords = 0, nIsland = 0;
   LaunchConfig launch = {0, 0, 0, 0};
 };
 

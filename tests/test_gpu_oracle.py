@@ -23,6 +23,7 @@ from forge2d_amd import scenes
 from forge2d_amd import snapshot as S
 from oracle import pyoracle as O
 from tests import parity_util as P
+from tests.golden import make_golden
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -41,7 +42,12 @@ SCENES = {
                                                        drop_band=(0.3, 1.0)),
     "container300_nocd": lambda: scenes.container_world(300, 12.0, 30.0, 7002, continuous=False, chains=2,
                                                         drop_band=(0.1, 0.8)),
+    # sleep enabled: islands settle and sleep, get woken, split
+    "sleep_world": lambda: make_golden.sleep_world(),
+    "container150_sleep": lambda: scenes.container_world(150, 10.0, 20.0, 7004, sleep=True, continuous=True,
+                                                         chains=2, drop_band=(0.05, 0.5)),
 }
+STEPS = {"pyramid48": 12, "container2600": 12, "sleep_world": 260, "container150_sleep": 240}
 
 
 def reference_geometry(spec: bytes) -> bytes:
@@ -67,6 +73,7 @@ def compare(mine, o):
     assert np.array_equal(mine.shapes["fat"][~fast_shapes], o.shapes["fat"][~fast_shapes])
     if not fast.any():
         assert np.array_equal(mine.move, o.move)
+    assert P.island_differences(mine, o) == []
 
 
 @pytest.mark.skipif(not (O.available("f32") and os.path.exists(HARNESS)), reason="oracle or reference build missing")
@@ -77,7 +84,7 @@ def test_device_matches_oracle_every_step(gpu, name):
     b = F.Batch(spec)
     prev_raw = b.export(0)
     prev = S.parse_one(prev_raw)
-    steps = 12 if name in ("pyramid48", "container2600") else 40
+    steps = STEPS.get(name, 40)
     for k in range(1, steps + 1):
         print(f"[oracle-parity] {name} step {k}", flush=True)
         b.step(steps=1)

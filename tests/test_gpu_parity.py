@@ -13,7 +13,10 @@ For every pair of consecutive dumped steps (k-1, k) of every fixture:
     solver stages and finalize reproduces bodies, stored impulses, fat AABBs and
     the next move buffer of step k, within 1e-5 (in practice bit-exact).
 The ccd fixtures fire fast bodies and bullets at thin static geometry, so these
-cases also cover the time-of-impact sweeps of continuous collision.
+cases also cover the time-of-impact sweeps of continuous collision. The sleep
+fixture (every step 0-260) covers islands: links, merges, splits, islands put to
+sleep and woken, compared exactly (island records, id pools, awake orders, links)
+whenever the step's new contacts got the reference's ids.
 """
 import numpy as np
 import pytest
@@ -62,6 +65,12 @@ def test_full_step_from_reference_state(gpu, name, i):
         assert max(errs.values()) <= TOL, errs
         rep = P.manifold_report(mine, ref, with_impulses=True)
         assert rep["impulses"] == 0, rep
+    # islands and solver sets follow the contact ids (links happen in id order), so
+    # they are compared when the device's new contacts got the reference's ids
+    same_ids = {(int(c["shape_a"]), int(c["shape_b"])): int(c["id"]) for c in mine.live_contacts()} == \
+               {(int(c["shape_a"]), int(c["shape_b"])): int(c["id"]) for c in ref.live_contacts()}
+    if same_ids and cm == cr:
+        assert P.island_differences(mine, ref) == []
 
 
 SOLVE_CASES = [c for c in CASES if load_fixture(c[0])[1][c[1]].live_contacts().size > 0]

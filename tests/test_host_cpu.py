@@ -50,6 +50,9 @@ def test_host_construction_matches_reference_step0(name):
     assert np.array_equal(mine.shapes["body"], ref.shapes["body"])
     assert np.array_equal(mine.move, ref.move)
     assert np.array_equal(mine.joints["color"], ref.joints["color"])
+    # solver sets and islands at creation: one island per non-static body, joints
+    # linked and their islands merged at once (S/body.c:84-97, S/joint.c:289-309)
+    assert P.island_differences(mine, ref) == []
 
 
 @pytest.mark.parametrize("name", FIXTURE_NAMES)
@@ -71,6 +74,7 @@ def test_snapshot_import_export_roundtrip(name):
         assert np.array_equal(back.colors[S.OVERFLOW], ref.colors[S.OVERFLOW])
         assert np.array_equal(back.move, ref.move)
         assert np.array_equal(back.free_ids, ref.free_ids)
+        assert P.island_differences(back, ref) == []
 
 
 def test_scene_generators_are_deterministic():
