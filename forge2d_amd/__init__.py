@@ -86,6 +86,12 @@ SIGNATURES = [
     ("f2d_body_apply_linear_impulse_to_center", None, [ctypes.c_int32, _U, _F, _F, _I]),
     ("f2d_body_get_world_center", None, [ctypes.c_int32, _U, _PF]),
     ("f2d_body_get_local_center", None, [ctypes.c_int32, _U, _PF]),
+    ("f2d_body_is_awake", _I, [ctypes.c_int32, _U]),
+    ("f2d_body_set_awake", None, [ctypes.c_int32, _U, _I]),
+    ("f2d_body_is_sleep_enabled", _I, [ctypes.c_int32, _U]),
+    ("f2d_body_enable_sleep", None, [ctypes.c_int32, _U, _I]),
+    ("f2d_body_get_sleep_threshold", _F, [ctypes.c_int32, _U]),
+    ("f2d_body_set_sleep_threshold", None, [ctypes.c_int32, _U, _F]),
     ("f2d_body_get_mass", _F, [ctypes.c_int32, _U]),
     ("f2d_body_get_rotational_inertia", _F, [ctypes.c_int32, _U]),
     ("f2d_body_get_type", _I, [ctypes.c_int32, _U]),

@@ -25,6 +25,7 @@
 
 extern "C" int f2d_launch_step(const f2d::BatchView* b, const f2d::StepParams* sp, const int* worldList, int nWorlds,
                                hipStream_t stream, const f2d::LaunchConfig* cfg);
+extern "C" int f2d_launch_island_command(const f2d::BatchView* b, int world, int op, int arg, hipStream_t stream);
 
 namespace f2d {
 
@@ -94,6 +95,7 @@ int HostWorld::createBody(int type, Vec2 p, Rot q, Vec2 v, float w, float linDam
   }
   sims.push_back(s);
   if (type != kStatic) {
+    awakeBodies.resize(hdr.awakeBodyCount);  // the mirror may hold a committed batch's full-length vector
     awakeBodies.push_back(id);
     hdr.awakeBodyCount = (int)awakeBodies.size();
     // b2CreateIslandForBody, S/body.c:84-97
@@ -1505,7 +1507,7 @@ long f2dg_awake_body_count(void* h) {
   if (!b) return -1;
   long n = 0;
   for (auto& w : b->worlds)
-    for (auto& s : w.sims) n += s.type != kStatic ? 1 : 0;
+    for (auto& s : w.sims) n += s.setIndex == kAwakeSet ? 1 : 0;
   return n;
 }
 
@@ -1835,25 +1837,26 @@ static int hitEvents(Batch& b, int world, double* out, uint32_t shapeWg) {
   return n;
 }
 
-// Body move events: every awake body in solver order (S/solver.c:605-610).
+// Body move events: every body the last step finalized, in the awake set's order
+// of that finalize (S/solver.c:605-610); fellAsleep marks the bodies whose island
+// went to sleep at the end of the step (S/solver_set.c:196-205).
 static int bodyEvents(Batch& b, int world, double* out, int worldIndex0) {
   download(b);
   HostWorld& hw = b.worlds[world];
-  int n = 0;
-  for (size_t i = 0; i < hw.sims.size(); ++i) {
-    const BodySim& s = hw.sims[i];
-    if (s.type == kStatic) continue;
-    if (out) {
-      double* r = out + 7 * n;
+  const int n = std::min(hw.hdr.moveBodyCount, (int)hw.moveOrder.size());
+  if (out) {
+    for (int k = 0; k < n; ++k) {
+      const int i = hw.moveOrder[k];
+      const BodySim& s = hw.sims[i];
+      double* r = out + 7 * k;
       r[0] = (double)i + 1;
       r[1] = worldIndex0 < 0 ? 0.0 : (double)((uint32_t)worldIndex0 | ((uint32_t)hw.bodyGen[i] << 16));
       r[2] = s.xf.p.x;
       r[3] = s.xf.p.y;
       r[4] = s.xf.q.c;
       r[5] = s.xf.q.s;
-      r[6] = 0.0;  // fellAsleep: worlds run with sleep off
+      r[6] = (s.flags & BF_FELL_ASLEEP) ? 1.0 : 0.0;
     }
-    ++n;
   }
   return n;
 }
@@ -2004,7 +2007,7 @@ int f2dg_batch_stats(void* h, long* out) {
   if (download(*b) != 0) return -1;
   long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (const HostWorld& hw : b->worlds) {
-    for (const BodySim& s : hw.sims) v[0] += s.type != kStatic ? 1 : 0;
+    for (const BodySim& s : hw.sims) v[0] += s.setIndex == kAwakeSet ? 1 : 0;
     v[1] += (long)hw.shapes.size();
     for (int i = 0; i < hw.contactCount && i < (int)hw.contacts.size(); ++i) {
       const Contact& c = hw.contacts[i];
@@ -2093,13 +2096,29 @@ void f2d_world_get_gravity(uint32_t w, float* out) {
   out[1] = b->worlds[0].hdr.gravity.y;
 }
 
+// Island operations of the body and world API run on the device copy of the world
+// (the wake and sleep code of the step): the host's edits are uploaded first and the
+// mirror is refreshed on the next read.
+static void islandCommand(Batch& b, int world, int op, int arg) {
+  if (b.hostDirty || !b.committed) {
+    if (upload(b) != 0) return;
+  }
+  if (f2d_launch_island_command(&b.dev.view, world, op, arg, b.stream) != 0) return;
+  (void)hipStreamSynchronize(b.stream);
+  b.deviceNewer = true;
+}
+
+// b2World_EnableSleeping, S/world.c:1710-1737: disabling sleep wakes every sleeping set
 void f2d_world_enable_sleeping(uint32_t w, int enabled) {
   Batch* b = batchOf(w);
   if (!b) return;
   download(*b);
   WorldHeader& h = b->worlds[0].hdr;
+  const bool was = (h.flags & WF_SLEEP) != 0;
+  if (was == (enabled != 0)) return;
   h.flags = enabled ? (h.flags | WF_SLEEP) : (h.flags & ~WF_SLEEP);
   b->hostDirty = true;
+  if (!enabled) islandCommand(*b, 0, 2, 0);
 }
 
 int f2d_world_is_sleeping_enabled(uint32_t w) {
@@ -2182,30 +2201,106 @@ float f2d_body_get_angular_velocity(int32_t index1, uint32_t wg) {
   return hw ? hw->states[id].w : 0.0f;
 }
 
+// b2WakeBody, S/body.c:332-344
+static void wakeBody(uint32_t wg, int id) {
+  Batch* b = bodyBatch(wg);
+  if (b && b->worlds[0].sims[id].setIndex >= kFirstSleepingSet) islandCommand(*b, 0, 1, id);
+}
+
+// b2Body_SetLinearVelocity, S/body.c:767-788: a non-zero velocity wakes the body;
+// a body that stays asleep keeps no velocity
 void f2d_body_set_linear_velocity(int32_t index1, uint32_t wg, float x, float y) {
   int id;
   HostWorld* hw = bodyWorld(index1, wg, &id);
   if (!hw || hw->sims[id].type == kStatic) return;
+  if (x * x + y * y > 0.0f) {
+    wakeBody(wg, id);
+    hw = bodyWorld(index1, wg, &id);
+  }
+  if (hw->sims[id].setIndex != kAwakeSet) return;
   hw->states[id].v = {x, y};
   bodyBatch(wg)->hostDirty = true;
 }
 
+// b2Body_ApplyForceToCenter, S/body.c:918-933: wakes on request; forces reach awake bodies only
 void f2d_body_apply_force_to_center(int32_t index1, uint32_t wg, float fx, float fy, int wake) {
-  (void)wake;
   int id;
   HostWorld* hw = bodyWorld(index1, wg, &id);
   if (!hw || hw->sims[id].type != kDynamic) return;
+  if (wake) {
+    wakeBody(wg, id);
+    hw = bodyWorld(index1, wg, &id);
+  }
+  if (hw->sims[id].setIndex != kAwakeSet) return;
   hw->sims[id].force = add(hw->sims[id].force, {fx, fy});
   bodyBatch(wg)->hostDirty = true;
 }
 
-// b2Body_ApplyLinearImpulseToCenter, S/body.c: v += invMass * impulse
+// b2Body_ApplyLinearImpulseToCenter, S/body.c:975-995: v += invMass * impulse
 void f2d_body_apply_linear_impulse_to_center(int32_t index1, uint32_t wg, float ix, float iy, int wake) {
-  (void)wake;
   int id;
   HostWorld* hw = bodyWorld(index1, wg, &id);
   if (!hw || hw->sims[id].type != kDynamic) return;
+  if (wake) {
+    wakeBody(wg, id);
+    hw = bodyWorld(index1, wg, &id);
+  }
+  if (hw->sims[id].setIndex != kAwakeSet) return;
   hw->states[id].v = mulAdd(hw->states[id].v, hw->sims[id].invMass, {ix, iy});
+  bodyBatch(wg)->hostDirty = true;
+}
+
+// b2Body_IsAwake, S/body.c:1476-1481
+int f2d_body_is_awake(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  return hw ? (hw->sims[id].setIndex == kAwakeSet ? 1 : 0) : 0;
+}
+
+// b2Body_SetAwake, S/body.c:1483-1515
+void f2d_body_set_awake(int32_t index1, uint32_t wg, int awake) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  if (awake && hw->sims[id].setIndex >= kFirstSleepingSet) {
+    wakeBody(wg, id);
+  } else if (!awake && hw->sims[id].setIndex == kAwakeSet) {
+    islandCommand(*bodyBatch(wg), 0, 3, id);
+  }
+}
+
+// b2Body_IsSleepEnabled / b2Body_EnableSleep, S/body.c:1517-1522, 1538-1552: disabling sleep wakes the body
+int f2d_body_is_sleep_enabled(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  return hw ? ((hw->sims[id].flags & BF_ENABLE_SLEEP) ? 1 : 0) : 0;
+}
+
+void f2d_body_enable_sleep(int32_t index1, uint32_t wg, int enable) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  if (enable) {
+    hw->sims[id].flags |= BF_ENABLE_SLEEP;
+  } else {
+    hw->sims[id].flags &= ~BF_ENABLE_SLEEP;
+  }
+  bodyBatch(wg)->hostDirty = true;
+  if (!enable) wakeBody(wg, id);
+}
+
+// b2Body_GetSleepThreshold / b2Body_SetSleepThreshold, S/body.c:1524-1536
+float f2d_body_get_sleep_threshold(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  return hw ? hw->sims[id].sleepThreshold : 0.0f;
+}
+
+void f2d_body_set_sleep_threshold(int32_t index1, uint32_t wg, float threshold) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  hw->sims[id].sleepThreshold = threshold;
   bodyBatch(wg)->hostDirty = true;
 }
 

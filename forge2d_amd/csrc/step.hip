@@ -1518,6 +1518,35 @@ __global__ __launch_bounds__(BLOCK) void continuousKernel(BatchView b, const int
   }
 }
 
+// Island commands of the body and world API, one world, thread 0:
+//   1  b2WakeBody: wake the sleeping set of body `arg` (S/body.c:332-344)
+//   2  b2World_EnableSleeping(false): wake every sleeping set in set order (S/world.c:1710-1737)
+//   3  b2Body_SetAwake(false): split the body's island if it lost constraints, then
+//      try to put it to sleep (S/body.c:1483-1515)
+__global__ void islandCommandKernel(BatchView b, int world, int op, int arg) {
+  if (threadIdx.x != 0) return;
+  WorldHeader* h = b.worlds + world;
+  WorldPtrs w = worldPtrs(b, h);
+  IslandCtx c = w.isl;
+  if (op == 1) {
+    if (arg >= 0 && arg < h->bodyCount && w.sims[arg].setIndex >= kFirstSleepingSet) wakeIslandSet(w, w.sims[arg].islandId);
+  } else if (op == 2) {
+    for (int set = kFirstSleepingSet; set < h->setCount; ++set) {
+      for (int i = 0; i < h->islandCount; ++i) {
+        if (c.islands[i].id == i && c.islands[i].setIndex == set) {
+          wakeIslandSet(w, i);
+          break;
+        }
+      }
+    }
+  } else if (op == 3) {
+    if (arg >= 0 && arg < h->bodyCount && w.sims[arg].setIndex == kAwakeSet && w.sims[arg].islandId >= 0) {
+      if (c.islands[w.sims[arg].islandId].removeCount > 0) splitIsland(c, w.sims[arg].islandId);
+      trySleepIsland(w, w.sims[arg].islandId);
+    }
+  }
+}
+
 template <int BLOCK>
 static int launchBlock(const BatchView* b, const StepParams* sp, const int* worldList, int nWorlds, hipStream_t stream,
                        const LaunchConfig* cfg) {
@@ -1537,6 +1566,11 @@ static int launchBlock(const BatchView* b, const StepParams* sp, const int* worl
 }
 
 }  // namespace f2d
+
+extern "C" int f2d_launch_island_command(const f2d::BatchView* b, int world, int op, int arg, hipStream_t stream) {
+  hipLaunchKernelGGL(f2d::islandCommandKernel, dim3(1), dim3(64), 0, stream, *b, world, op, arg);
+  return (int)hipGetLastError();
+}
 
 extern "C" int f2d_launch_step(const f2d::BatchView* b, const f2d::StepParams* sp, const int* worldList, int nWorlds,
                                hipStream_t stream, const f2d::LaunchConfig* cfg) {

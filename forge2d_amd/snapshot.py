@@ -227,6 +227,17 @@ def to_bytes(s: Snapshot) -> bytes:
              np.array([len(c) for c in s.colors], "<i4").tobytes()]
     parts += [np.asarray(c, "<i4").tobytes() for c in s.colors]
     parts += [np.asarray(s.move, "<i4").tobytes(), np.asarray(s.free_ids, "<i4").tobytes()]
+    isl = getattr(s, "islands", None)
+    h["version"] = 2 if isl is not None else 1
+    parts[0] = h.tobytes()
+    if isl is not None:
+        hd = np.zeros(16, "<i4")
+        hd[:9] = [isl.island_count, len(isl.free_islands), isl.awake_island_count, isl.split_island, isl.set_count,
+                  len(isl.free_sets), isl.awake_body_count, isl.island_next, isl.set_next]
+        parts.append(hd.tobytes())
+        for a in (isl.islands, isl.free_islands, isl.awake_islands, isl.free_sets, isl.awake_bodies,
+                  isl.body_links, isl.contact_links, isl.joint_links):
+            parts.append(np.ascontiguousarray(a).tobytes())
     return b"".join(parts)
 
 
@@ -240,6 +251,7 @@ def copy(s: Snapshot) -> Snapshot:
     o.colors = [np.array(c, "<i4") for c in s.colors]
     o.move = np.array(s.move, "<i4")
     o.free_ids = np.array(s.free_ids, "<i4")
+    o.islands = getattr(s, "islands", None)
     o.nbytes = s.nbytes
     return o
 
@@ -295,4 +307,7 @@ def post_collide(prev: Snapshot, cur: Snapshot) -> Snapshot:
     s.joints["color"] = cur.joints["color"]
     s.move = np.zeros(0, "<i4")
     s.header["step"] = prev.header["step"]
+    # the island graph after the collide is not part of either snapshot: the
+    # composed state carries none (version 1), for worlds with sleep off
+    s.islands = None
     return s

@@ -101,6 +101,14 @@ void f2d_body_apply_linear_impulse_to_center(int32_t index1, uint32_t world_and_
 void f2d_body_get_world_center(int32_t index1, uint32_t world_and_generation, float* out);
 void f2d_body_get_local_center(int32_t index1, uint32_t world_and_generation, float* out);
 /* F/native/wasm/f2d_shim.c (F2D_BODY_GET_FLOAT b2Body_GetMass / GetRotationalInertia / GetType) */
+/* F/native/wasm/f2d_shim.c:389-394 (b2Body_IsAwake / SetAwake / IsSleepEnabled / EnableSleep /
+ * Get/SetSleepThreshold): waking and putting to sleep run the step's island code on the device */
+int f2d_body_is_awake(int32_t index1, uint32_t world_and_generation);
+void f2d_body_set_awake(int32_t index1, uint32_t world_and_generation, int awake);
+int f2d_body_is_sleep_enabled(int32_t index1, uint32_t world_and_generation);
+void f2d_body_enable_sleep(int32_t index1, uint32_t world_and_generation, int enable);
+float f2d_body_get_sleep_threshold(int32_t index1, uint32_t world_and_generation);
+void f2d_body_set_sleep_threshold(int32_t index1, uint32_t world_and_generation, float threshold);
 float f2d_body_get_mass(int32_t index1, uint32_t world_and_generation);
 float f2d_body_get_rotational_inertia(int32_t index1, uint32_t world_and_generation);
 int f2d_body_get_type(int32_t index1, uint32_t world_and_generation);

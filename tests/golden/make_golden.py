@@ -10,8 +10,8 @@ after the listed steps, compressed, to ``tests/golden/<name>.npz``:
     steps  int32   the step index of each snapshot
     geoms  uint8   per shape, the reference's built geometry (148 B: int32 type,
                    count; float radius, centroid, 8 vertices, 8 normals)
-    events uint8   ("events" only) per step: int32 [step, begin, end, hit, move]
-                   counts, then float64 records in the flat shim's event layouts
+    events uint8   ("events" and "sleep") per step: int32 [step, begin, end, hit,
+                   move] counts, then float64 records in the flat shim's event layouts
 
 Usage: python tests/golden/make_golden.py [name ...]
 """
@@ -170,6 +170,8 @@ def sleep_world():
 
 
 EVENT_STEPS = list(range(1, 121))
+# fixtures that also record the reference's contact and body events per step
+EVENT_FIXTURES = {"events": EVENT_STEPS, "sleep": list(range(1, 261))}
 
 
 FIXTURES = {
@@ -207,9 +209,10 @@ def make(name: str):
         with open(gp, "rb") as f:
             geoms = f.read()
         extra = {}
-        if name == "events":
+        if name in EVENT_FIXTURES:
+            ev_steps = EVENT_FIXTURES[name]
             ep = os.path.join(td, "events.bin")
-            subprocess.run([HARNESS, "events", sp, ep, "0", str(max(EVENT_STEPS)), ",".join(map(str, EVENT_STEPS))],
+            subprocess.run([HARNESS, "events", sp, ep, "0", str(max(ev_steps)), ",".join(map(str, ev_steps))],
                            check=True)
             with open(ep, "rb") as f:
                 extra["events"] = np.frombuffer(f.read(), np.uint8)

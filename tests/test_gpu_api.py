@@ -3,7 +3,7 @@ reference's own behavioural tests with their tolerances:
 
 * F/test/ffi/smoke_test.dart:13-46 and F/test/api/world_test.dart:56-74 — a 1x1 box
   dropped from y=10 rests on a 50x1 ground box at (0, 0.5) +- 0.01 after 200 steps
-  (the sleep assertion is not replayed: sleeping is not run on the device);
+  (here with sleep off; tests/test_gpu_sleep.py replays it with the sleep assertion);
 * F/test/api/body_test.dart:90-96 — a 1x1 box of density 2 has mass 2 +- 1e-6;
 * F/test/api/body_test.dart:115-132 — an impulse of m*1 gives v = 1 +- 1e-5, and a
   force of m*60 for one 1/60 s step gives v = 1 +- 0.01 (gravity off);

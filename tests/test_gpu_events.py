@@ -2,7 +2,9 @@
 S/solver.c:1758-1817 hits; S/solver.c:605-610 body moves) against the
 reference's own events, recorded by the fixture harness for every step of the
 "events" fixture (bouncing circles, capsules and boxes with contact and hit events
-enabled on every shape).
+enabled on every shape), and of the "sleep" fixture, whose body move events carry
+the order of the awake set and the fellAsleep flag of islands put to sleep
+(S/solver_set.c:196-205).
 
 For every step k: import the reference state after k-1, step once, read the
 events through the C-ABI getters (the flat shim's record layouts) and compare
@@ -26,8 +28,8 @@ from tests.conftest import GOLDEN, load_fixture
 pytestmark = pytest.mark.gpu
 
 
-def reference_events():
-    raw = np.load(os.path.join(GOLDEN, "events.npz"))["events"].tobytes()
+def reference_events(name):
+    raw = np.load(os.path.join(GOLDEN, name + ".npz"))["events"].tobytes()
     out, o = {}, 0
     while o < len(raw):
         step, nb, ne, nh, nm = struct.unpack_from("<5i", raw, o)
@@ -44,9 +46,10 @@ def sorted_rows(a):
     return a[np.lexsort(a.T[::-1])] if len(a) else a
 
 
-def test_contact_and_body_events_match_reference(gpu):
-    spec, snaps, raws = load_fixture("events")
-    ref_events = reference_events()
+@pytest.mark.parametrize("name", ["events", "sleep"])
+def test_contact_and_body_events_match_reference(gpu, name):
+    spec, snaps, raws = load_fixture(name)
+    ref_events = reference_events(name)
     checked = ordered = 0
     for i in range(1, len(snaps)):
         prev, ref = snaps[i - 1], snaps[i]

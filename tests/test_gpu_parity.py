@@ -18,13 +18,17 @@ fixture (every step 0-260) covers islands: links, merges, splits, islands put to
 sleep and woken, compared exactly (island records, id pools, awake orders, links)
 whenever the step's new contacts got the reference's ids.
 """
+import os
+
 import numpy as np
 import pytest
 
 import forge2d_amd as F
 from forge2d_amd import snapshot as S
+from forge2d_amd import scenes
+from oracle import pyoracle as O
 from tests import parity_util as P
-from tests.conftest import FIXTURE_NAMES, load_fixture
+from tests.conftest import FIXTURE_NAMES, GOLDEN, load_fixture
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-5
@@ -73,6 +77,14 @@ def test_full_step_from_reference_state(gpu, name, i):
         assert P.island_differences(mine, ref) == []
 
 
+def sleep_enabled(spec: bytes) -> bool:
+    return bool(int(scenes.parse(spec)[0][0]["flags"]) & scenes.WF_SLEEP)
+
+
+def fixture_geoms(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"))["geoms"].tobytes()
+
+
 SOLVE_CASES = [c for c in CASES if load_fixture(c[0])[1][c[1]].live_contacts().size > 0]
 
 
@@ -81,9 +93,18 @@ def test_solver_and_finalize_from_post_collide_state(gpu, name, i):
     spec, snaps, raws = load_fixture(name)
     prev, ref = snaps[i - 1], snaps[i]
     print(f"[parity] solve {name} {prev.step}->{ref.step}", flush=True)
-    composed = S.post_collide(prev, ref)
+    if sleep_enabled(spec):
+        # with sleep on, the solver also merges, splits and sleeps islands: start it
+        # from the post-collide state the oracle's collide (pinned bit-exact to the
+        # reference) computes, island graph included
+        o = O.Oracle(spec, fixture_geoms(name), raws[i - 1])
+        o.create_contacts(O.created_contacts(prev, ref))
+        o.collide()
+        post = o.export()
+    else:
+        post = S.to_bytes(S.post_collide(prev, ref))
     b = F.Batch(spec)
-    b.import_state(0, S.to_bytes(composed))
+    b.import_state(0, post)
     b.step(steps=1, phases=F.PH_SOLVE | F.PH_FINALIZE)
     assert b.sync() == 0
     mine = b.snapshot(0)
@@ -101,6 +122,8 @@ def test_solver_and_finalize_from_post_collide_state(gpu, name, i):
         assert np.array_equal(mine.move, ref.move)
     for f in ("linear_impulse_x", "linear_impulse_y", "lower_impulse", "upper_impulse"):
         assert np.array_equal(mine.joints[f], ref.joints[f]), f
+    if sleep_enabled(spec):
+        assert P.island_differences(mine, ref) == []
 
 
 @pytest.mark.parametrize("name", FIXTURE_NAMES)
