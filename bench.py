@@ -132,6 +132,8 @@ def main():
                          "W + K steps) so a kernel trace of this command holds exactly those dispatches")
     ap.add_argument("--lanes", default="",
                     help="comma list of workgroup lanes per size class instead of the autotuner's choice")
+    ap.add_argument("--solve-lds", default="",
+                    help="comma list of solve-launch LDS bytes per size class (with --lanes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: every world of a batch lives on a single device and worlds never "
@@ -161,6 +163,9 @@ def main():
     if args.lanes:
         for i, lanes in enumerate(int(x) for x in args.lanes.split(",")):
             batch.set_class_block(i, lanes)
+    if args.solve_lds:
+        for i, nbytes in enumerate(int(x) for x in args.solve_lds.split(",")):
+            batch.set_class_solve_lds(i, nbytes)
     if args.kernel_profile:
         kernel_profile(args, batch, awake, desc)
         return

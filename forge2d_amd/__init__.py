@@ -57,6 +57,7 @@ SIGNATURES = [
     ("f2dg_set_block", _I, [_VP, _I]),
     ("f2dg_get_block", _I, [_VP]),
     ("f2dg_set_class_block", _I, [_VP, _I, _I]),
+    ("f2dg_set_class_solve_lds", _I, [_VP, _I, _I]),
     ("f2dg_error_bits", _I, [_VP]),
     ("f2dg_size_classes", _I, [_VP, _PI, _I]),
     ("f2dg_contact_event_counts", _I, [_VP, _I, _PI]),
@@ -254,6 +255,9 @@ class Batch:
 
     def set_class_block(self, cls: int, block: int):
         self._check(lib().f2dg_set_class_block(self.h, cls, block), "set_class_block")
+
+    def set_class_solve_lds(self, cls: int, nbytes: int):
+        self._check(lib().f2dg_set_class_solve_lds(self.h, cls, nbytes), "set_class_solve_lds")
 
     @property
     def block(self) -> int:

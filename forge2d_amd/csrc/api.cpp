@@ -559,6 +559,17 @@ static void freeDevice(Batch& b) {
 }
 
 // Lay out all worlds in flat arrays and allocate the device batch.
+// Dynamic LDS of a solve launch: the body states and per-step body terms (64 B per
+// body) of the class's largest world, plus room for `lanesPerBody` constraint lanes
+// per body (164 B each); a world whose constraints do not fit keeps them in HBM.
+static int solveLdsBytes(int maxBodies, float lanesPerBody, int limit) {
+  const int bodies = std::max(maxBodies, 1) * (int)(sizeof(BodyState) + sizeof(BodyConst));
+  if (bodies > limit) return 0;
+  long total = bodies + (long)(lanesPerBody * (float)std::max(maxBodies, 1)) * 4L * CN_FIELD_COUNT;
+  total = std::min<long>(total, limit);
+  return (int)((total + 255) & ~255L) <= limit ? (int)((total + 255) & ~255L) : limit;
+}
+
 static int commit(Batch& b) {
   freeDevice(b);
   long nb = 0, ns = 0, nc = 0, nj = 0, ncol = 0, nscratch = 0, nisl = 0;
@@ -697,8 +708,7 @@ static int commit(Batch& b) {
       int cnp = nextPow2(std::max(c.maxShapes, 1));
       int cc = 28 * cnp <= kLdsLimit ? 28 * cnp : 0;
       c.launch.collideLds = std::max(cc, 12 * 256);
-      int cs = std::max(c.maxBodies, 1) * (int)(sizeof(BodyState) + sizeof(BodyConst));
-      c.launch.solveLds = cs <= kLdsLimit ? ((cs + 255) & ~255) : 0;
+      c.launch.solveLds = solveLdsBytes(c.maxBodies, 1.0f, kLdsLimit);
       c.launch.block = c.limit <= 128 ? 64 : (c.limit <= 512 ? 128 : 256);
       F2D_HIP(hipMalloc(&c.dList, sizeof(int) * c.count));
       F2D_HIP(hipMemcpy(c.dList, members[k].data(), sizeof(int) * c.count, hipMemcpyHostToDevice));
@@ -1948,11 +1958,46 @@ int f2dg_autotune(void* h, float dt, int subSteps, int steps, float* msOut) {
       }
     }
     c.launch.block = bestBlock;
+    // LDS room for constraint lanes: more keeps more worlds' constraints on chip,
+    // less lets more workgroups share a compute unit; every class runs the same
+    // number of trial steps, so all worlds advance equally
+    const float lanesPerBody[4] = {0.0f, 0.5f, 1.0f, 2.0f};
+    best = 1e30f;
+    int bestLds = c.launch.solveLds;
+    for (int k = 0; k < 4; ++k) {
+      LaunchConfig cfg = c.launch;
+      cfg.solveLds = solveLdsBytes(c.maxBodies, lanesPerBody[k], 65536);
+      F2D_HIP(hipEventRecord(e0, b->stream));
+      for (int s = 0; s < steps; ++s) {
+        if (f2d_launch_step(&b->dev.view, &sp, c.dList, c.count, b->stream, &cfg) != 0) return -1;
+      }
+      F2D_HIP(hipEventRecord(e1, b->stream));
+      F2D_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      F2D_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        bestLds = cfg.solveLds;
+      }
+    }
+    c.launch.solveLds = bestLds;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->deviceNewer = true;
   return b->classes.empty() ? b->launch.block : b->classes.back().launch.block;
+}
+
+// Force the solve launch's dynamic LDS bytes of one size class (at least its body
+// terms; at most 64 KiB).
+int f2dg_set_class_solve_lds(void* h, int cls, int bytes) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || cls < 0 || cls >= (int)b->classes.size() || bytes < 0 || bytes > 65536) return -1;
+  SizeClass& c = b->classes[cls];
+  const int bodies = std::max(c.maxBodies, 1) * (int)(sizeof(BodyState) + sizeof(BodyConst));
+  if (bytes < bodies && bytes != 0) return -1;
+  c.launch.solveLds = bytes;
+  return 0;
 }
 
 // Size classes after commit: per class [limit, worlds, max bodies, max shapes,

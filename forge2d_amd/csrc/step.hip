@@ -911,13 +911,15 @@ struct SolveCtx {
 
 template <int BLOCK, bool kLds>
 __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState* ldsStates, const SolveCtx& ctx,
-                                 int phases) {
+                                 int phases, int ldsBytes) {
   WorldHeader* h = w.hdr;
   const int nb = h->bodyCount;
   const int nc = h->contactCount;
   BodyState* states = kLds ? ldsStates : w.states;
   BodyConst* ldsConsts = reinterpret_cast<BodyConst*>(ldsStates + nb);  // LDS only
-  const Cons& cons = w.cons;
+  // the constraint SoA: the world's lanes in LDS behind the body terms when they fit
+  // the workgroup's LDS, else the world's region of the batch array in HBM
+  Cons cons = w.cons;
 
   // b2Solve, S/solver.c:1190-1220 and :1479-1706: merge islands linked since the last
   // step, stop when no body is awake, split the last step's candidate island
@@ -971,6 +973,14 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       h->activeColors = active;
     }
     __syncthreads();
+    if (kLds) {
+      const long lanes = sh.colorStart[kGraphColorCount];
+      const long need = (long)nb * (long)(sizeof(BodyState) + sizeof(BodyConst)) + 4L * CN_FIELD_COUNT * lanes;
+      if (lanes > 0 && need <= ldsBytes) {
+        cons.f = reinterpret_cast<float*>(ldsConsts + nb);
+        cons.stride = lanes;
+      }
+    }
     for (int i = threadIdx.x; i < nc; i += BLOCK) {
       int color = w.ccolor[i];
       if (color >= 0 && color < kOverflowIndex) {
@@ -1312,6 +1322,7 @@ __global__ __launch_bounds__(BLOCK) void collideKernel(BatchView b, StepParams s
     sh.error = 0;
     h->inv_h = sp.inv_h;
     h->contactSpeed = ctx.contactSpeed;
+    h->moveBodyCount = 0;  // body move events start empty each step (S/world.c:709)
   }
   __syncthreads();
   int ldsShapes = 1;
@@ -1338,9 +1349,9 @@ __global__ __launch_bounds__(BLOCK) void solveKernel(BatchView b, StepParams sp,
   SolveCtx ctx;
   stepContext(h, sp, ctx);
   if (h->bodyCount * (int)(sizeof(BodyState) + sizeof(BodyConst)) <= ldsBytes) {
-    solveAndFinalize<BLOCK, true>(w, sh, reinterpret_cast<BodyState*>(lds), ctx, sp.phases);
+    solveAndFinalize<BLOCK, true>(w, sh, reinterpret_cast<BodyState*>(lds), ctx, sp.phases, ldsBytes);
   } else {
-    solveAndFinalize<BLOCK, false>(w, sh, nullptr, ctx, sp.phases);
+    solveAndFinalize<BLOCK, false>(w, sh, nullptr, ctx, sp.phases, ldsBytes);
   }
   if (threadIdx.x == 0) h->stepIndex += 1;
 }

@@ -193,9 +193,10 @@ long f2dg_export_world(void* batch, int world, void* out, long cap);
 /* overlay a snapshot onto a world built from the same scene spec */
 int f2dg_import_world(void* batch, int world, const void* snapshot, long bytes);
 const char* f2dg_last_error(void* batch);
-/* workgroup-size autotuner: per size class, times `steps` steps with 64, 128 and 256 lanes
- * per world and keeps the fastest (ms_out[3] optional: whole-batch ms per candidate); the
- * trial steps advance every world equally; returns the largest class's choice */
+/* autotuner: per size class, times `steps` steps with 64, 128 and 256 lanes per world and
+ * keeps the fastest (ms_out[3] optional: whole-batch ms per candidate), then times 4 solve
+ * LDS sizes (room for 0, 0.5, 1 and 2 constraint lanes per body) and keeps the fastest;
+ * the trial steps advance every world equally; returns the largest class's lanes */
 int f2dg_autotune(void* batch, float dt, int sub_steps, int steps, float* ms_out);
 /* worlds are launched in size classes by max(bodies, shapes) <= 64, 128, ..., 1024, larger;
  * per class: [limit, worlds, max bodies, max shapes, workgroup lanes, collide LDS bytes,
@@ -206,6 +207,9 @@ int f2dg_set_block(void* batch, int block);
 int f2dg_get_block(void* batch);
 /* force the workgroup size of size class `cls` (index into f2dg_size_classes) */
 int f2dg_set_class_block(void* batch, int cls, int block);
+/* force the solve launch's dynamic LDS bytes of size class `cls` (body terms of its
+ * largest world + constraint lanes; worlds whose lanes do not fit keep them in HBM) */
+int f2dg_set_class_solve_lds(void* batch, int cls, int bytes);
 /* aggregate counts after f2dg_sync: [awake bodies, shapes, touching contacts, touching points,
  * restitution contacts, restitution points, joints, overflow contacts] */
 int f2dg_batch_stats(void* batch, long* out);
