@@ -132,6 +132,8 @@ def main():
                          "W + K steps) so a kernel trace of this command holds exactly those dispatches")
     ap.add_argument("--lanes", default="",
                     help="comma list of workgroup lanes per size class instead of the autotuner's choice")
+    ap.add_argument("--phase-times", type=int, default=0, metavar="STEPS",
+                    help="after the timed steps, time the step's phases as separate launches over STEPS steps")
     ap.add_argument("--solve-lds", default="",
                     help="comma list of solve-launch LDS bytes per size class (with --lanes)")
     args = ap.parse_args()
@@ -204,6 +206,11 @@ def main():
     traffic = pmc.get(dominant, {}).get("hbm_bytes_per_launch") if pmc else None
     log(f"[bench] kernel ms/launch {launch_ms}, algorithmic bytes {nbytes}, {dominant} {achieved:.0f} GB/s")
 
+    phase_ms = None
+    if args.phase_times > 0:
+        phase_ms = {k: v / args.phase_times for k, v in batch.phase_times(DT, SUBSTEPS, args.phase_times).items()}
+        log(f"[bench] phase ms per step: {phase_ms}")
+
     # per-step latency percentiles (each step synchronized on its own)
     lat = []
     for _ in range(min(args.steps, 32)):
@@ -264,6 +271,7 @@ def main():
                                   "size, own stream), largest first; collide, solve and continuous-collision "
                                   "launches per class per step"},
         "latency_ms": {"p50": p50, "p99": p99, "samples": len(lat)},
+        "phase_ms_per_step": phase_ms,
         "per_world_gpu": per_world,
         "batched_vs_per_world": (value / per_world["value"]) if per_world else None,
         "batched_vs_cpu": (value / cpu["value"]) if cpu else None,

@@ -45,6 +45,7 @@ SIGNATURES = [
     ("f2dg_step_each", _I, [_VP, _F, _I, _I]),
     ("f2dg_step_timed", _I, [_VP, _F, _I, _I, _PF]),
     ("f2dg_kernel_times", _I, [_VP, _F, _I, _I, _PF]),
+    ("f2dg_phase_times", _I, [_VP, _F, _I, _I, _PF]),
     ("f2dg_synchronize", _I, [_VP]),
     ("f2dg_sync", _I, [_VP]),
     ("f2dg_get_bodies", _I, [_VP, _I, _PF]),
@@ -199,6 +200,12 @@ class Batch:
         ms = (ctypes.c_float * 3)()
         self._check(lib().f2dg_kernel_times(self.h, dt, substeps, steps, ms), "kernel_times")
         return {"collideKernel": ms[0], "solveKernel": ms[1], "continuousKernel": ms[2]}
+
+    def phase_times(self, dt=1.0 / 60.0, substeps=4, steps=1) -> dict:
+        """Summed milliseconds per phase over `steps` steps run as separate launches."""
+        ms = (ctypes.c_float * 4)()
+        self._check(lib().f2dg_phase_times(self.h, dt, substeps, steps, ms), "phase_times")
+        return {"pairs": ms[0], "collide_state": ms[1], "solve_finalize": ms[2], "continuous_sleep": ms[3]}
 
     def synchronize(self):
         self._check(lib().f2dg_synchronize(self.h), "synchronize")

@@ -1664,6 +1664,40 @@ int f2dg_kernel_times(void* h, float dt, int subSteps, int steps, float* ms) {
   return 0;
 }
 
+// Step and time the step's phases as separate launches (size classes one after
+// another): ms[0] broad-phase pairs, ms[1] narrow phase + contact state loop,
+// ms[2] solver + finalize, ms[3] continuous collision + island sleep. The result
+// equals a fused step's.
+int f2dg_phase_times(void* h, float dt, int subSteps, int steps, float* ms) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || steps <= 0) return -1;
+  if (prepareForStep(*b) != 0) return -1;
+  hipEvent_t ev[5];
+  for (auto& e : ev) F2D_HIP(hipEventCreate(&e));
+  StepParams sp[4] = {makeParams(dt, subSteps, PH_PAIRS | PH_NO_COUNT),
+                      makeParams(dt, subSteps, PH_COLLIDE | PH_STATE | PH_NO_COUNT),
+                      makeParams(dt, subSteps, PH_SOLVE | PH_FINALIZE | PH_NO_CONTINUOUS),
+                      makeParams(dt, subSteps, PH_CONTINUOUS)};
+  float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int s = 0; s < steps; ++s) {
+    F2D_HIP(hipEventRecord(ev[0], b->stream));
+    for (int k = 0; k < 4; ++k) {
+      if (launchClassesSerial(*b, sp[k]) != 0) return -1;
+      F2D_HIP(hipEventRecord(ev[k + 1], b->stream));
+    }
+    F2D_HIP(hipEventSynchronize(ev[4]));
+    for (int k = 0; k < 4; ++k) {
+      float t = 0.0f;
+      F2D_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+      sum[k] += t;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < 4; ++k) ms[k] = sum[k];
+  b->deviceNewer = true;
+  return 0;
+}
+
 int f2dg_synchronize(void* h) {
   Batch* b = static_cast<Batch*>(h);
   if (!b) return -1;

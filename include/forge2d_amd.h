@@ -181,6 +181,9 @@ int f2dg_step_timed(void* batch, float dt, int sub_steps, int steps, float* ms_o
 /* step and time each kernel: HIP events bracket the collide, solve and
  * continuous-collision launches of every step; ms[0..2] = summed milliseconds */
 int f2dg_kernel_times(void* batch, float dt, int sub_steps, int steps, float* ms);
+/* step with the phases as separate launches and time each: ms[0] pairs, ms[1] narrow
+ * phase + contact state, ms[2] solver + finalize, ms[3] continuous + island sleep */
+int f2dg_phase_times(void* batch, float dt, int sub_steps, int steps, float* ms);
 int f2dg_synchronize(void* batch);
 /* copy device state to the host mirror; returns OR of world error bits, <0 on failure */
 int f2dg_sync(void* batch);
