@@ -781,6 +781,7 @@ static int upload(Batch& b) {
     h.overflowCount = (int)hw.overflow.size();
     h.stepIndex = hw.stepIndex;
     h.jointCount = (int)hw.joints.size();
+    h.mergePending = 1;  // imported or edited islands may carry parent links
     headers[i] = h;
     hipStream_t s = b.stream;
     F2D_HIP(hipMemcpyAsync(v.sims + h.bodyOffset, hw.sims.data(), sizeof(BodySim) * h.bodyCount, hipMemcpyHostToDevice, s));

@@ -13,8 +13,12 @@
 
 #if defined(__HIPCC__)
 #define F2D_HD __host__ __device__ __forceinline__
+// serial, rarely taken bookkeeping (island graph): kept out of line so its registers
+// and stack do not weigh on the lanes of the kernels that call it
+#define F2D_HD_COLD __host__ __device__ __noinline__ inline
 #else
 #define F2D_HD inline
+#define F2D_HD_COLD inline
 #endif
 
 namespace f2d {

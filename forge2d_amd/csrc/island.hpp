@@ -142,7 +142,7 @@ F2D_HD int islandRoot(IslandCtx& c, int id) {
 
 // The island part of b2LinkContact, S/island.c:148-218 (the caller wakes sleeping
 // sets first)
-F2D_HD void linkContactIslands(IslandCtx& c, int cid) {
+F2D_HD_COLD void linkContactIslands(IslandCtx& c, int cid) {
   const Contact& ct = c.contacts[cid];
   int iA = c.sims[ct.bodyA].islandId, iB = c.sims[ct.bodyB].islandId;
   if (iA == iB) {
@@ -151,12 +151,15 @@ F2D_HD void linkContactIslands(IslandCtx& c, int cid) {
   }
   int rA = iA != -1 ? islandRoot(c, iA) : -1;
   int rB = iB != -1 ? islandRoot(c, iB) : -1;
-  if (rA != -1 && rB != -1 && rA != rB) c.islands[rB].parent = rA;
+  if (rA != -1 && rB != -1 && rA != rB) {
+    c.islands[rB].parent = rA;
+    c.h->mergePending = 1;
+  }
   addContactToIsland(c, rA != -1 ? rA : rB, cid);
 }
 
 // b2UnlinkContact, S/island.c:221-262
-F2D_HD void unlinkContact(IslandCtx& c, int cid) {
+F2D_HD_COLD void unlinkContact(IslandCtx& c, int cid) {
   int4& l = c.links[cid];
   Island& is = c.islands[l.x];
   if (l.y != -1) c.links[l.y].z = l.z;
@@ -170,7 +173,7 @@ F2D_HD void unlinkContact(IslandCtx& c, int cid) {
 
 // The island part of b2LinkJoint, S/island.c:289-385, for joints created between
 // awake or static bodies (creation time; the caller merges islands afterwards)
-F2D_HD void linkJointIslands(IslandCtx& c, int jid) {
+F2D_HD_COLD void linkJointIslands(IslandCtx& c, int jid) {
   const Joint& j = c.joints[jid];
   int iA = c.sims[j.bodyA].islandId, iB = c.sims[j.bodyB].islandId;
   if (iA == iB) {
@@ -179,7 +182,10 @@ F2D_HD void linkJointIslands(IslandCtx& c, int jid) {
   }
   int rA = iA != -1 ? islandRoot(c, iA) : -1;
   int rB = iB != -1 ? islandRoot(c, iB) : -1;
-  if (rA != -1 && rB != -1 && rA != rB) c.islands[rB].parent = rA;
+  if (rA != -1 && rB != -1 && rA != rB) {
+    c.islands[rB].parent = rA;
+    c.h->mergePending = 1;
+  }
   addJointToIsland(c, rA != -1 ? rA : rB, jid);
 }
 
@@ -223,8 +229,12 @@ F2D_HD void mergeIsland(IslandCtx& c, int id) {
 }
 
 // b2MergeAwakeIslands, S/island.c:537-592
-F2D_HD void mergeAwakeIslands(IslandCtx& c) {
+F2D_HD_COLD void mergeAwakeIslands(IslandCtx& c) {
   WorldHeader* h = c.h;
+  // without a parent link since the last merge every awake island is a root and the
+  // merge changes nothing
+  if (!h->mergePending) return;
+  h->mergePending = 0;
   for (int i = 0; i < h->awakeIslandCount; ++i) {
     int id = c.awakeIslands[i];
     int root = islandRoot(c, id);
@@ -244,7 +254,7 @@ F2D_HD void mergeAwakeIslands(IslandCtx& c) {
 // and in a joint's setIndex high bit; they are cleared for the base island first.
 constexpr int kMarkBit = 1 << 30;
 
-F2D_HD void splitIsland(IslandCtx& c, int baseId) {
+F2D_HD_COLD void splitIsland(IslandCtx& c, int baseId) {
   Island& base = c.islands[baseId];
   if (base.setIndex != kAwakeSet || base.removeCount == 0) return;
   const int n = base.bodyCount;

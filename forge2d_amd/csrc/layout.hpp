@@ -207,6 +207,7 @@ struct alignas(16) WorldHeader {
   int awakeBodyCount;
   int moveBodyCount;             // awake bodies finalized by the last step (move events)
   int sleepPending;              // finalize ran with sleep enabled: island sleep follows
+  int mergePending;              // an island got a union-find parent since the last merge
   unsigned long long splitKey;   // finalize: sleepiest body of an island with removed constraints
 };
 

@@ -660,7 +660,7 @@ __device__ void destroyContact(const WorldPtrs& w, int id) {
 
 // b2AssignJointColor + b2CreateJointInGraph, S/constraint_graph.c:214-290. The
 // joint's index in its colour is the count of joints already there.
-__device__ void addJointToGraph(const WorldPtrs& w, int jid) {
+__device__ __noinline__ void addJointToGraph(const WorldPtrs& w, int jid) {
   Joint& j = w.joints[jid];
   const bool staticA = w.sims[j.bodyA].setIndex == kStaticSet;
   const bool staticB = w.sims[j.bodyB].setIndex == kStaticSet;
@@ -689,7 +689,7 @@ __device__ void addJointToGraph(const WorldPtrs& w, int jid) {
 
 // b2RemoveJointFromGraph, S/constraint_graph.c:292-322: the colour's last joint
 // takes the removed joint's index
-__device__ void removeJointFromGraph(const WorldPtrs& w, int jid) {
+__device__ __noinline__ void removeJointFromGraph(const WorldPtrs& w, int jid) {
   Joint& j = w.joints[jid];
   const int color = j.color;
   if (color < 0) return;
@@ -716,7 +716,7 @@ __device__ void removeJointFromGraph(const WorldPtrs& w, int jid) {
 // b2WakeSolverSet, S/solver_set.c:31-154. A sleeping set holds exactly one island
 // (sets merge only through joints created between sleeping bodies); its bodies,
 // touching contacts and joints are that island's lists in order.
-__device__ void wakeIslandSet(const WorldPtrs& w, int islandId) {
+__device__ __noinline__ void wakeIslandSet(const WorldPtrs& w, int islandId) {
   IslandCtx c = w.isl;
   WorldHeader* h = w.hdr;
   Island& is = c.islands[islandId];
@@ -757,7 +757,7 @@ __device__ void wakeIslandSet(const WorldPtrs& w, int islandId) {
 
 // b2LinkContact, S/island.c:117-218: a touching contact between an awake and a
 // sleeping body wakes the sleeping set first
-__device__ void linkContact(const WorldPtrs& w, int id) {
+__device__ __noinline__ void linkContact(const WorldPtrs& w, int id) {
   const Contact& ct = w.contacts[id];
   const BodySim& A = w.sims[ct.bodyA];
   const BodySim& B = w.sims[ct.bodyB];
@@ -770,7 +770,7 @@ __device__ void linkContact(const WorldPtrs& w, int id) {
 // b2TrySleepIsland, S/solver_set.c:156-420: the island's bodies leave the awake
 // set (remove-swap), its non-touching contacts to sleeping or static bodies go to
 // the disabled set, its touching contacts and joints leave the constraint graph.
-__device__ void trySleepIsland(const WorldPtrs& w, int islandId) {
+__device__ __noinline__ void trySleepIsland(const WorldPtrs& w, int islandId) {
   IslandCtx c = w.isl;
   WorldHeader* h = w.hdr;
   Island& is = c.islands[islandId];
@@ -828,7 +828,7 @@ __device__ void trySleepIsland(const WorldPtrs& w, int islandId) {
 // Island sleep after finalize and continuous collision, S/solver.c:1995-2049: the
 // split candidate for the next step, then every awake island no body kept awake,
 // in reverse awake order (thread 0)
-__device__ void sleepPhase(const WorldPtrs& w) {
+__device__ __noinline__ void sleepPhase(const WorldPtrs& w) {
   IslandCtx c = w.isl;
   WorldHeader* h = w.hdr;
   h->sleepPending = 0;
