@@ -667,7 +667,13 @@ static int commit(Batch& b) {
   }
   const int kLdsLimit = 65536;
   int np = nextPow2(maxShapes);
-  int collide = 28 * np <= kLdsLimit ? 28 * np : 0;
+  // sort-and-sweep arrays (28 B per shape), the pair-test cache (32 B per shape) and
+  // the existing-pair hash set (16 B per shape), as far as they fit
+  auto collideBytes = [&](int p) {
+    p = std::max(p, 4);
+    return 76 * p <= kLdsLimit ? 76 * p : (44 * p <= kLdsLimit ? 44 * p : (28 * p <= kLdsLimit ? 28 * p : 0));
+  };
+  int collide = collideBytes(np);
   collide = std::max(collide, 12 * 256);
   b.launch.collideLds = collide;
   int solve = maxBodies * (int)(sizeof(BodyState) + sizeof(BodyConst));
@@ -706,7 +712,7 @@ static int commit(Batch& b) {
         c.maxShapes = std::max(c.maxShapes, (int)b.worlds[i].shapes.size());
       }
       int cnp = nextPow2(std::max(c.maxShapes, 1));
-      int cc = 28 * cnp <= kLdsLimit ? 28 * cnp : 0;
+      int cc = collideBytes(cnp);
       c.launch.collideLds = std::max(cc, 12 * 256);
       c.launch.solveLds = solveLdsBytes(c.maxBodies, 1.0f, kLdsLimit);
       c.launch.block = c.limit <= 128 ? 64 : (c.limit <= 512 ? 128 : 256);

@@ -200,18 +200,68 @@ __device__ WorldPtrs worldPtrs(const BatchView& b, WorldHeader* h) {
   return w;
 }
 
+// Existing contacts of a world as an LDS hash set of shape-pair keys (open
+// addressing, linear probing): the pair test asks it instead of walking the two
+// bodies' contact lists in HBM. Keys are (min << 16 | max) of the shape ids.
+__device__ __forceinline__ uint32_t pairSetKey(int s, int t) {
+  return s < t ? ((uint32_t)s << 16) | (uint32_t)t : ((uint32_t)t << 16) | (uint32_t)s;
+}
+__device__ __forceinline__ uint32_t pairSetSlot(uint32_t key, uint32_t mask) {
+  return (key * 2654435761u) & mask;
+}
+__device__ __forceinline__ bool pairSetHas(const uint32_t* table, uint32_t mask, uint32_t key) {
+  for (uint32_t i = pairSetSlot(key, mask), n = 0; n <= mask; i = (i + 1) & mask, ++n) {
+    uint32_t v = table[i];
+    if (v == key) return true;
+    if (v == 0xFFFFFFFFu) return false;
+  }
+  return false;
+}
+
+// What the pair test reads of a shape: body, proxy key, flags (plus whether the
+// body has joints), filter. Cached in LDS, sorted like the sweep, for worlds that fit.
+constexpr int kPairBodyHasJoints = 1 << 29;
+struct alignas(16) PairInfo {
+  int body, proxyKey, flags, group;
+  uint32_t catLo, catHi, maskLo, maskHi;
+};
+
+__device__ __forceinline__ PairInfo pairInfo(const WorldPtrs& w, int s) {
+  const Shape& S = w.shapes[s];
+  PairInfo p;
+  p.body = S.body;
+  p.proxyKey = S.proxyKey;
+  p.flags = S.flags | (w.sims[S.body].jointCount > 0 ? kPairBodyHasJoints : 0);
+  p.group = S.group;
+  p.catLo = S.catLo;
+  p.catHi = S.catHi;
+  p.maskLo = S.maskLo;
+  p.maskHi = S.maskHi;
+  return p;
+}
+
+// b2ShouldShapesCollide, S/shape.h:122-130
+__device__ __forceinline__ bool shouldPairCollide(const PairInfo& a, const PairInfo& b) {
+  if (a.group == b.group && a.group != 0) return a.group > 0;
+  uint64_t maskA = (uint64_t)a.maskLo | ((uint64_t)a.maskHi << 32);
+  uint64_t catA = (uint64_t)a.catLo | ((uint64_t)a.catHi << 32);
+  uint64_t maskB = (uint64_t)b.maskLo | ((uint64_t)b.maskHi << 32);
+  uint64_t catB = (uint64_t)b.catLo | ((uint64_t)b.catHi << 32);
+  return (maskA & catB) != 0 && (catA & maskB) != 0;
+}
+
 // Evaluate one overlapping candidate (s, t) of the sweep. Emits a new pair when the
 // reference's pair query would: at least one proxy moved, tree rules, filters, no
 // existing contact. The emitter is the proxy whose query reports the pair.
-__device__ void considerPair(const WorldPtrs& w, SharedState& sh, int s, int t) {
-  const Shape& S = w.shapes[s];
-  const Shape& T = w.shapes[t];
+__device__ void considerPair(const WorldPtrs& w, SharedState& sh, int s, int t, const PairInfo& S,
+                             const PairInfo& T, const uint32_t* pairSet, uint32_t pairMask) {
   int bs = S.body, bt = T.body;
   if (bs == bt) return;
-  int typeS = w.sims[bs].type, typeT = w.sims[bt].type;
+  // the proxy type is the body type (S/broad_phase.h:18-21)
+  int typeS = S.proxyKey & 3, typeT = T.proxyKey & 3;
   if (typeS != kDynamic && typeT != kDynamic) return;
   bool movedS = (S.flags & SF_MOVED) != 0, movedT = (T.flags & SF_MOVED) != 0;
-  int emitter, partner;
+  int emitter;
   if (typeS == kDynamic && typeT == kDynamic) {
     if (movedS && movedT) {
       emitter = S.proxyKey < T.proxyKey ? s : t;
@@ -235,19 +285,22 @@ __device__ void considerPair(const WorldPtrs& w, SharedState& sh, int s, int t) 
       return;
     }
   }
-  partner = emitter == s ? t : s;
+  const int partner = emitter == s ? t : s;
   if ((S.flags | T.flags) & SF_SENSOR) return;
-  if (!shouldShapesCollide(S, T)) return;
-  if (!shouldBodiesCollide(w.sims, w.joints, bs, bt)) return;
-  if (contactExists(w.sims, w.contacts, bs, bt, s, t)) return;
-  const Shape& E = w.shapes[emitter];
-  const Shape& P = w.shapes[partner];
+  if (!shouldPairCollide(S, T)) return;
+  // b2ShouldBodiesCollide only vetoes pairs whose bodies share a joint
+  if ((S.flags & kPairBodyHasJoints) && (T.flags & kPairBodyHasJoints) &&
+      !shouldBodiesCollide(w.sims, w.joints, bs, bt))
+    return;
+  if (pairSet ? pairSetHas(pairSet, pairMask, pairSetKey(s, t)) : contactExists(w.sims, w.contacts, bs, bt, s, t))
+    return;
+  const PairInfo& P = partner == s ? S : T;
   // Order within one move result: dynamic-tree partners, then static, then kinematic
   // (the query order reversed by the list prepend, S/broad_phase.c:345-371, 285-287).
   int ptype = P.proxyKey & 3;
   int treeRank = ptype == kDynamic ? 0 : (ptype == kStatic ? 1 : 2);
   uint32_t partnerOrder = 0x3FFFFFFFu - (uint32_t)(P.proxyKey >> 2);
-  uint64_t key = ((uint64_t)(uint32_t)E.moveRank << 32) | ((uint64_t)treeRank << 30) | partnerOrder;
+  uint64_t key = ((uint64_t)(uint32_t)w.shapes[emitter].moveRank << 32) | ((uint64_t)treeRank << 30) | partnerOrder;
   int idx = atomicAdd(&sh.pairCount, 1);
   if (idx < w.hdr->pairCap) {
     w.pairKeys[idx] = key;
@@ -326,7 +379,7 @@ __device__ void createContact(const WorldPtrs& w, int shapeIdA, int shapeIdB) {
 }
 
 template <int BLOCK>
-__device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int ldsShapes, int ldsPairs) {
+__device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int ldsShapes, int ldsPairs, int ldsBytes) {
   WorldHeader* h = w.hdr;
   const int n = h->shapeCount;
   if (threadIdx.x == 0) sh.pairCount = 0;
@@ -336,8 +389,12 @@ __device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int l
   anyMoved = __syncthreads_or(anyMoved);
   if (!anyMoved) return;
 
-  const int np = nextPow2(n);
-  const bool inLds = np <= ldsShapes;
+  // LDS layout by the world's padded shape count: sort keys (8 B), sorted fat boxes
+  // (16 B) and shape ids (4 B) per shape, then the pair-test cache (32 B per shape)
+  // and the existing-pair hash set when they fit
+  const int np = nextPow2(max(n, 4));
+  const bool inLds = 28L * np <= (long)ldsBytes;
+  (void)ldsShapes;
   uint64_t* keys = inLds ? reinterpret_cast<uint64_t*>(lds) : reinterpret_cast<uint64_t*>(w.scratch);
   int* dummyVals = nullptr;
   (void)dummyVals;
@@ -365,24 +422,55 @@ __device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int l
     }
   }
   // sorted boxes and indices
-  float4* boxes = inLds ? reinterpret_cast<float4*>(lds + ldsShapes * 8) : w.sapBoxes;
-  int* order = inLds ? reinterpret_cast<int*>(lds + ldsShapes * 8 + ldsShapes * 16) : (w.scratch + 2 * np);
+  float4* boxes = inLds ? reinterpret_cast<float4*>(lds + np * 8) : w.sapBoxes;
+  int* order = inLds ? reinterpret_cast<int*>(lds + np * 24) : (w.scratch + 2 * np);
+  PairInfo* info = (inLds && 60L * np <= (long)ldsBytes) ? reinterpret_cast<PairInfo*>(lds + np * 28) : nullptr;
   for (int i = threadIdx.x; i < n; i += BLOCK) {
     int idx = (int)(uint32_t)keys[i];
     AABB f = w.shapes[idx].fat;
     boxes[i] = make_float4(f.lo.x, f.lo.y, f.hi.x, f.hi.y);
     order[i] = idx;
+    if (info) info[i] = pairInfo(w, idx);
   }
   __syncthreads();
+  // the existing pairs as an LDS hash set when it fits behind the sort arrays
+  uint32_t* pairSet = nullptr;
+  uint32_t pairMask = 0;
+  if (inLds && n < 65536) {
+    const int live = h->contactCount - h->freeCount;
+    const int tsize = nextPow2(max(2 * live, 64));
+    const long base = info ? 60L * np : 28L * np;
+    if (base + 4L * tsize <= (long)ldsBytes) {
+      pairSet = reinterpret_cast<uint32_t*>(lds + base);
+      pairMask = (uint32_t)tsize - 1u;
+      for (int i = threadIdx.x; i < tsize; i += BLOCK) pairSet[i] = 0xFFFFFFFFu;
+      __syncthreads();
+      for (int i = threadIdx.x; i < h->contactCount; i += BLOCK) {
+        const Contact& c = w.contacts[i];
+        if (c.shapeA < 0) continue;
+        const uint32_t key = pairSetKey(c.shapeA, c.shapeB);
+        uint32_t k = pairSetSlot(key, pairMask);
+        for (uint32_t probe = 0; probe <= pairMask; ++probe, k = (k + 1) & pairMask) {
+          if (atomicCAS(&pairSet[k], 0xFFFFFFFFu, key) == 0xFFFFFFFFu) break;
+        }
+      }
+      __syncthreads();
+    }
+  }
   for (int i = threadIdx.x; i < n; i += BLOCK) {
     float4 a = boxes[i];
     int s = order[i];
+    const PairInfo S = info ? info[i] : pairInfo(w, s);
     for (int j = i + 1; j < n; ++j) {
       float4 bx = boxes[j];
       if (bx.x > a.z) break;
       // b2AABB_Overlaps on the two fat boxes (x already known to overlap on one side)
       bool overlap = !(bx.x > a.z || bx.y > a.w || a.x > bx.z || a.y > bx.w);
-      if (overlap) considerPair(w, sh, s, order[j]);
+      if (overlap) {
+        const int t = order[j];
+        const PairInfo T = info ? info[j] : pairInfo(w, t);
+        considerPair(w, sh, s, t, S, T, pairSet, pairMask);
+      }
     }
   }
   __syncthreads();
@@ -1307,8 +1395,9 @@ __device__ void stepContext(const WorldHeader* h, const StepParams& sp, SolveCtx
 }
 
 // Pairs, collide and contact state of one world per workgroup. ldsBytes of dynamic
-// LDS hold the sort-and-sweep arrays (28 B per shape) and the new-pair sort (12 B
-// per pair); larger worlds use their global scratch region instead.
+// LDS hold the sort-and-sweep arrays (28 B per shape) with the existing-pair hash
+// set behind them, then the new-pair sort (12 B per pair); larger worlds use their
+// global scratch region and walk the contact lists instead.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void collideKernel(BatchView b, StepParams sp, const int* worldList, int ldsBytes) {
   __shared__ SharedState sh;
@@ -1331,7 +1420,7 @@ __global__ __launch_bounds__(BLOCK) void collideKernel(BatchView b, StepParams s
   int ldsPairs = 1;
   while (ldsPairs * 2 * 12 <= ldsBytes) ldsPairs *= 2;
   if (ldsPairs * 12 > ldsBytes) ldsPairs = 0;
-  if (sp.phases & PH_PAIRS) pairsPhase<BLOCK>(w, sh, lds, ldsShapes, ldsPairs);
+  if (sp.phases & PH_PAIRS) pairsPhase<BLOCK>(w, sh, lds, ldsShapes, ldsPairs, ldsBytes);
   if (sp.phases & PH_COLLIDE) collidePhase<BLOCK>(w, sh);
   if (sp.phases & PH_STATE) statePhase(w, sh);
   if (threadIdx.x == 0 && (sp.phases & (PH_SOLVE | PH_FINALIZE | PH_NO_COUNT)) == 0) h->stepIndex += 1;
