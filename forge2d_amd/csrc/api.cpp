@@ -560,12 +560,14 @@ static void freeDevice(Batch& b) {
 
 // Lay out all worlds in flat arrays and allocate the device batch.
 // Dynamic LDS of a solve launch: the body states and per-step body terms (64 B per
-// body) of the class's largest world, plus room for `lanesPerBody` constraint lanes
-// per body (164 B each); a world whose constraints do not fit keeps them in HBM.
-static int solveLdsBytes(int maxBodies, float lanesPerBody, int limit) {
+// body) of the class's largest world, its joints, plus room for `lanesPerBody`
+// constraint lanes per body (164 B each); a world whose joints or constraints do
+// not fit keeps them in HBM.
+static int solveLdsBytes(int maxBodies, int maxJoints, float lanesPerBody, int limit) {
   const int bodies = std::max(maxBodies, 1) * (int)(sizeof(BodyState) + sizeof(BodyConst));
   if (bodies > limit) return 0;
-  long total = bodies + (long)(lanesPerBody * (float)std::max(maxBodies, 1)) * 4L * CN_FIELD_COUNT;
+  long total = bodies + (long)maxJoints * (long)sizeof(Joint) +
+               (long)(lanesPerBody * (float)std::max(maxBodies, 1)) * 4L * CN_FIELD_COUNT;
   total = std::min<long>(total, limit);
   return (int)((total + 255) & ~255L) <= limit ? (int)((total + 255) & ~255L) : limit;
 }
@@ -710,11 +712,12 @@ static int commit(Batch& b) {
       for (int i : members[k]) {
         c.maxBodies = std::max(c.maxBodies, (int)b.worlds[i].sims.size());
         c.maxShapes = std::max(c.maxShapes, (int)b.worlds[i].shapes.size());
+        c.maxJoints = std::max(c.maxJoints, (int)b.worlds[i].joints.size());
       }
       int cnp = nextPow2(std::max(c.maxShapes, 1));
       int cc = collideBytes(cnp);
       c.launch.collideLds = std::max(cc, 12 * 256);
-      c.launch.solveLds = solveLdsBytes(c.maxBodies, 1.0f, kLdsLimit);
+      c.launch.solveLds = solveLdsBytes(c.maxBodies, c.maxJoints, 1.0f, kLdsLimit);
       c.launch.block = c.limit <= 128 ? 64 : (c.limit <= 512 ? 128 : 256);
       F2D_HIP(hipMalloc(&c.dList, sizeof(int) * c.count));
       F2D_HIP(hipMemcpy(c.dList, members[k].data(), sizeof(int) * c.count, hipMemcpyHostToDevice));
@@ -2007,7 +2010,7 @@ int f2dg_autotune(void* h, float dt, int subSteps, int steps, float* msOut) {
     int bestLds = c.launch.solveLds;
     for (int k = 0; k < 4; ++k) {
       LaunchConfig cfg = c.launch;
-      cfg.solveLds = solveLdsBytes(c.maxBodies, lanesPerBody[k], 65536);
+      cfg.solveLds = solveLdsBytes(c.maxBodies, c.maxJoints, lanesPerBody[k], 65536);
       F2D_HIP(hipEventRecord(e0, b->stream));
       for (int s = 0; s < steps; ++s) {
         if (f2d_launch_step(&b->dev.view, &sp, c.dList, c.count, b->stream, &cfg) != 0) return -1;

@@ -57,6 +57,8 @@ F2D_HD void prepareRevolute(Joint& j, const BodySim* sims, float h, float inv_h,
   j.invIB = b.invI;
   j.anchorA = rotate(a.xf.q, sub(j.localOriginAnchorA, a.localCenter));
   j.anchorB = rotate(b.xf.q, sub(j.localOriginAnchorB, b.localCenter));
+  j.simA = a.setIndex == kAwakeSet ? j.bodyA : -1;  // joint->indexA, S/joint.c:1380-1388
+  j.simB = b.setIndex == kAwakeSet ? j.bodyB : -1;
   j.deltaCenter = sub(b.center, a.center);
   j.deltaAngle = relativeAngle(b.xf.q, a.xf.q);
   float k = j.invIA + j.invIB;
@@ -71,12 +73,9 @@ F2D_HD void prepareRevolute(Joint& j, const BodySim* sims, float h, float inv_h,
   }
 }
 
-// The body index of a joint side: -1 for static bodies (they have no solver state).
-F2D_HD int jointIndex(const BodySim* sims, int body) { return sims[body].type == kStatic ? -1 : body; }
-
-F2D_HD void warmStartRevolute(Joint& j, BodyState* states, const BodySim* sims) {
+F2D_HD void warmStartRevolute(Joint& j, BodyState* states) {
   float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
-  int ia = jointIndex(sims, j.bodyA), ib = jointIndex(sims, j.bodyB);
+  int ia = j.simA, ib = j.simB;
   BodyState sA = loadState(states, ia);
   BodyState sB = loadState(states, ib);
   Vec2 rA = rotate(sA.dq, j.anchorA);
@@ -90,9 +89,9 @@ F2D_HD void warmStartRevolute(Joint& j, BodyState* states, const BodySim* sims) 
   storeVel(states, ib, sB.v, sB.w);
 }
 
-F2D_HD void solveRevolute(Joint& j, BodyState* states, const BodySim* sims, float h, float inv_h, bool useBias) {
+F2D_HD void solveRevolute(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
   float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
-  int ia = jointIndex(sims, j.bodyA), ib = jointIndex(sims, j.bodyB);
+  int ia = j.simA, ib = j.simB;
   const BodyState sA = loadState(states, ia);
   const BodyState sB = loadState(states, ib);
   Vec2 vA = sA.v;

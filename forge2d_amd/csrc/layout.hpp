@@ -167,6 +167,8 @@ struct alignas(16) Joint {
   float deltaAngle, axialMass;
   int islandId, islandPrev, islandNext;  // island graph (b2Joint, S/joint.h:44-48)
   int setIndex;
+  int simA, simB;  // solver state index of each body, -1 outside the awake set (set by prepare)
+  int pad[2];
 };
 
 // Per-world header. Offsets index the batch's flat arrays.

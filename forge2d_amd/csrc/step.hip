@@ -1006,8 +1006,10 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
   BodyState* states = kLds ? ldsStates : w.states;
   BodyConst* ldsConsts = reinterpret_cast<BodyConst*>(ldsStates + nb);  // LDS only
   // the constraint SoA: the world's lanes in LDS behind the body terms when they fit
-  // the workgroup's LDS, else the world's region of the batch array in HBM
+  // the workgroup's LDS, else the world's region of the batch array in HBM; the
+  // same for the joints
   Cons cons = w.cons;
+  Joint* jts = w.joints;
 
   // b2Solve, S/solver.c:1190-1220 and :1479-1706: merge islands linked since the last
   // step, stop when no body is awake, split the last step's candidate island
@@ -1061,11 +1063,18 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       h->activeColors = active;
     }
     __syncthreads();
+    // LDS behind the body terms: the joints (copied in here, back after the
+    // substeps), then the constraint lanes, as far as each fits
     if (kLds) {
       const long lanes = sh.colorStart[kGraphColorCount];
-      const long need = (long)nb * (long)(sizeof(BodyState) + sizeof(BodyConst)) + 4L * CN_FIELD_COUNT * lanes;
-      if (lanes > 0 && need <= ldsBytes) {
-        cons.f = reinterpret_cast<float*>(ldsConsts + nb);
+      long off = (long)nb * (long)(sizeof(BodyState) + sizeof(BodyConst));
+      const long jointBytes = (long)h->jointCount * (long)sizeof(Joint);
+      if (h->jointCount > 0 && off + jointBytes <= ldsBytes) {
+        jts = reinterpret_cast<Joint*>(reinterpret_cast<char*>(ldsStates) + off);
+        off += jointBytes;
+      }
+      if (lanes > 0 && off + 4L * CN_FIELD_COUNT * lanes <= ldsBytes) {
+        cons.f = reinterpret_cast<float*>(reinterpret_cast<char*>(ldsStates) + off);
         cons.stride = lanes;
       }
     }
@@ -1104,6 +1113,12 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         ldsStates[i] = w.states[i];
         ldsConsts[i] = makeBodyConst(w.sims[i], ctx.gravity, ctx.h);
       }
+      if (jts != w.joints) {
+        const int words = h->jointCount * (int)(sizeof(Joint) / sizeof(float4));
+        const float4* src = reinterpret_cast<const float4*>(w.joints);
+        float4* dst = reinterpret_cast<float4*>(jts);
+        for (int i = threadIdx.x; i < words; i += BLOCK) dst[i] = src[i];
+      }
     }
     __syncthreads();
     const int totalLanes = sh.colorStart[kGraphColorCount];
@@ -1115,7 +1130,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
 
     // -- prepare joints (all), S/solver.c:967-987 -> S/joint.c:1348-1402
     for (int i = threadIdx.x; i < h->jointCount; i += BLOCK) {
-      if (w.joints[i].color >= 0) prepareRevolute(w.joints[i], w.sims, ctx.h, ctx.inv_h, ctx.warmStartScale);
+      if (jts[i].color >= 0) prepareRevolute(jts[i], w.sims, ctx.h, ctx.inv_h, ctx.warmStartScale);
     }
     // -- prepare contacts (all lanes), S/contact_solver.c:1438-1669
     for (int lane = threadIdx.x; lane < totalLanes; lane += BLOCK) {
@@ -1167,7 +1182,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       // warm start: overflow joints, overflow contacts, then colours
       if (threadIdx.x == 0) {
         for (int k = 0; k < overflowJoints; ++k)
-          if (overflowJointList[k] >= 0) warmStartRevolute(w.joints[overflowJointList[k]], states, w.sims);
+          if (overflowJointList[k] >= 0) warmStartRevolute(jts[overflowJointList[k]], states);
         for (int k = 0; k < overflowCount; ++k) overflowWarmStart(cons, overflowStart + k, states);
       }
       __syncthreads();
@@ -1175,7 +1190,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
         if (cnt + jcnt == 0) continue;
         for (int k = threadIdx.x; k < jcnt; k += BLOCK)
-          warmStartRevolute(w.joints[jointLanes[sh.jointColorStart[col] + k]], states, w.sims);
+          warmStartRevolute(jts[jointLanes[sh.jointColorStart[col] + k]], states);
         for (int k = threadIdx.x; k < cnt; k += BLOCK) warmStartLane(cons, sh.colorStart[col] + k, states);
         __syncthreads();
       }
@@ -1183,7 +1198,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       if (threadIdx.x == 0) {
         for (int k = 0; k < overflowJoints; ++k)
           if (overflowJointList[k] >= 0)
-            solveRevolute(w.joints[overflowJointList[k]], states, w.sims, ctx.h, ctx.inv_h, true);
+            solveRevolute(jts[overflowJointList[k]], states, ctx.h, ctx.inv_h, true);
         for (int k = 0; k < overflowCount; ++k)
           overflowSolve(cons, overflowStart + k, states, ctx.inv_h, ctx.maxContactPushSpeed, true);
       }
@@ -1192,7 +1207,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
         if (cnt + jcnt == 0) continue;
         for (int k = threadIdx.x; k < jcnt; k += BLOCK)
-          solveRevolute(w.joints[jointLanes[sh.jointColorStart[col] + k]], states, w.sims, ctx.h, ctx.inv_h, true);
+          solveRevolute(jts[jointLanes[sh.jointColorStart[col] + k]], states, ctx.h, ctx.inv_h, true);
         for (int k = threadIdx.x; k < cnt; k += BLOCK)
           solveLane(cons, sh.colorStart[col] + k, states, ctx.inv_h, -ctx.contactSpeed, true);
         __syncthreads();
@@ -1209,7 +1224,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       if (threadIdx.x == 0) {
         for (int k = 0; k < overflowJoints; ++k)
           if (overflowJointList[k] >= 0)
-            solveRevolute(w.joints[overflowJointList[k]], states, w.sims, ctx.h, ctx.inv_h, false);
+            solveRevolute(jts[overflowJointList[k]], states, ctx.h, ctx.inv_h, false);
         for (int k = 0; k < overflowCount; ++k)
           overflowSolve(cons, overflowStart + k, states, ctx.inv_h, ctx.maxContactPushSpeed, false);
       }
@@ -1218,11 +1233,18 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
         if (cnt + jcnt == 0) continue;
         for (int k = threadIdx.x; k < jcnt; k += BLOCK)
-          solveRevolute(w.joints[jointLanes[sh.jointColorStart[col] + k]], states, w.sims, ctx.h, ctx.inv_h, false);
+          solveRevolute(jts[jointLanes[sh.jointColorStart[col] + k]], states, ctx.h, ctx.inv_h, false);
         for (int k = threadIdx.x; k < cnt; k += BLOCK)
           solveLane(cons, sh.colorStart[col] + k, states, ctx.inv_h, -ctx.contactSpeed, false);
         __syncthreads();
       }
+    }
+    // the joints' impulses and prepared data back to HBM
+    if (jts != w.joints) {
+      const int words = h->jointCount * (int)(sizeof(Joint) / sizeof(float4));
+      const float4* src = reinterpret_cast<const float4*>(jts);
+      float4* dst = reinterpret_cast<float4*>(w.joints);
+      for (int i = threadIdx.x; i < words; i += BLOCK) dst[i] = src[i];
     }
     // restitution: overflow then colours
     if (threadIdx.x == 0) {

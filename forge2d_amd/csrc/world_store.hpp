@@ -69,7 +69,7 @@ struct DeviceBatch {
 struct SizeClass {
   int limit = 0;              // max(bodies, shapes) of the class's worlds is at most this
   int count = 0;
-  int maxBodies = 0, maxShapes = 0;
+  int maxBodies = 0, maxShapes = 0, maxJoints = 0;
   int* dList = nullptr;       // world ids, largest first
   LaunchConfig launch = {0, 0, 0, 0};
   hipStream_t stream = nullptr;
