@@ -103,9 +103,10 @@ def cpu_baseline(spec: bytes, worlds: int, budget_s: float):
             out = subprocess.run([harness, "time", path, str(steps), "0", str(threads), "0", str(n)],
                                  capture_output=True, text=True, timeout=600, check=True)
             r = json.loads(out.stdout.strip().splitlines()[-1])
-        return {"value": r["body_substeps_per_s"], "unit": "body-substeps/s", "cores": threads, "kind": "reference",
+        used = int(r["threads"])
+        return {"value": r["body_substeps_per_s"], "unit": "body-substeps/s", "cores": used, "kind": "reference",
                 "sample": f"first {r['worlds']} worlds of the workload ({int(r['bodies'])} bodies), {steps} steps "
-                          f"from the initial state, {threads} threads stepping disjoint worlds, reference Box2D "
+                          f"from the initial state, {used} threads stepping disjoint worlds, reference Box2D "
                           f"v3.1.1 built with the forge2d hook flags (-std=c17 -O2, SSE2)",
                 "step_ms_p50": r["step_ms_p50"], "step_ms_p99": r["step_ms_p99"], "wall_s": r["wall_s"]}
 

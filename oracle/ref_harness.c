@@ -831,12 +831,14 @@ int main( int argc, char** argv )
 		double p50 = job.step_ms[n / 2];
 		double p99 = job.step_ms[(long)( 0.99 * ( n - 1 ) )];
 		double body_substeps = bodies * job.steps * job.substeps;
-		// throughput over the timed steps only: per-thread time is summed step time
-		double busy = total_ms / 1e3 / threads;
+		// throughput over the timed steps only: per-thread time is summed step time,
+		// over the threads that had a world to step
+		int used = threads < job.count ? threads : job.count;
+		double busy = total_ms / 1e3 / ( used > 0 ? used : 1 );
 		printf( "{\"worlds\": %d, \"bodies\": %.0f, \"steps\": %d, \"warmup\": %d, \"threads\": %d, "
 				"\"wall_s\": %.4f, \"step_ms_mean\": %.6f, \"step_ms_p50\": %.6f, \"step_ms_p99\": %.6f, "
 				"\"body_substeps_per_s\": %.6e}\n",
-				job.count, bodies, job.steps, job.warmup, threads, wall, total_ms / n, p50, p99, body_substeps / busy );
+				job.count, bodies, job.steps, job.warmup, used, wall, total_ms / n, p50, p99, body_substeps / busy );
 		return 0;
 	}
 	fprintf( stderr, "bad arguments\n" );
