@@ -644,7 +644,7 @@ static int commit(Batch& b) {
       devAlloc(d, &v.contacts, nc) || devAlloc(d, &v.freeIds, nc) || devAlloc(d, &v.overflow, nc) ||
       devAlloc(d, &v.pairs, nc) || devAlloc(d, &v.pairShapes, nc) || devAlloc(d, &v.joints, nj) ||
       devAlloc(d, &v.colorBits, ncol) || devAlloc(d, &v.cons, (size_t)nc * CN_FIELD_COUNT) ||
-      devAlloc(d, &v.scratch, nscratch) || devAlloc(d, &v.ccolor, nc) || devAlloc(d, &v.staticShapes, ns) ||
+      devAlloc(d, &v.scratch, nscratch) || devAlloc(d, &v.ccolor, nc) || devAlloc(d, &v.staticShapes, ns) || devAlloc(d, &v.sweepOrder, ns) ||
       devAlloc(d, &v.events, 3 * nc) || devAlloc(d, &v.islandData, std::max(nisl, 4L)) ||
       devAlloc(d, &v.contactLinks, nc)) {
     b.lastError = "device allocation failed";
@@ -769,6 +769,7 @@ static int upload(Batch& b) {
   long totalShapes = 0;
   for (const HostWorld& hw : b.worlds) totalShapes = std::max<long>(totalShapes, hw.hdr.shapeOffset + hw.hdr.shapeCount);
   std::vector<int> staticShapes(std::max<long>(totalShapes, 1), 0);
+  std::vector<int> sweepOrder(std::max<long>(totalShapes, 1), 0);
   long totalIsland = 4;
   for (const HostWorld& hw : b.worlds)
     totalIsland = std::max<long>(totalIsland, hw.hdr.islandOffset + islandRegion(hw.hdr.islandCap, hw.hdr.bodyCount).total);
@@ -784,6 +785,7 @@ static int upload(Batch& b) {
     for (int k = 0; k < h.shapeCount; ++k) {
       const Shape& sh = hw.shapes[k];
       if (sh.proxyKey >= 0 && (sh.proxyKey & 3) == kStatic) staticShapes[h.shapeOffset + h.staticCount++] = k;
+      sweepOrder[h.shapeOffset + k] = k;  // any permutation; the first pair update sorts it
     }
     h.contactCount = hw.contactCount;
     h.freeCount = (int)hw.freeIds.size();
@@ -824,6 +826,7 @@ static int upload(Batch& b) {
   F2D_HIP(hipMemcpyAsync(v.worlds, headers.data(), sizeof(WorldHeader) * headers.size(), hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipMemcpyAsync(v.ccolor, ccolor.data(), ccolor.size(), hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipMemcpyAsync(v.staticShapes, staticShapes.data(), sizeof(int) * totalShapes, hipMemcpyHostToDevice, b.stream));
+  F2D_HIP(hipMemcpyAsync(v.sweepOrder, sweepOrder.data(), sizeof(int) * totalShapes, hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipMemcpyAsync(v.islandData, islandData.data(), sizeof(int) * islandData.size(), hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipMemcpyAsync(v.contactLinks, links.data(), sizeof(int4) * totalContacts, hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipStreamSynchronize(b.stream));

@@ -172,6 +172,7 @@ struct WorldPtrs {
   int8_t* ccolor;
   int* events;
   int4* links;      // contact island links and solver set
+  int* sweepOrder;
   IslandCtx isl;
 };
 
@@ -195,6 +196,7 @@ __device__ WorldPtrs worldPtrs(const BatchView& b, WorldHeader* h) {
   w.ccolor = b.ccolor + h->contactOffset;
   w.events = b.events + 3L * h->contactOffset;
   w.sapBoxes = b.sapBoxes + h->shapeOffset;
+  w.sweepOrder = b.sweepOrder + h->shapeOffset;
   w.links = b.contactLinks + h->contactOffset;
   w.isl = islandCtx(h, b.islandData + h->islandOffset, w.sims, w.contacts, w.links, w.joints);
   return w;
@@ -396,29 +398,53 @@ __device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int l
   const bool inLds = 28L * np <= (long)ldsBytes;
   (void)ldsShapes;
   uint64_t* keys = inLds ? reinterpret_cast<uint64_t*>(lds) : reinterpret_cast<uint64_t*>(w.scratch);
-  int* dummyVals = nullptr;
-  (void)dummyVals;
+  // keys (lower fat x, shape id) laid out in the last step's sweep order, which is
+  // nearly sorted when bodies move little per step
+  int* sweepOrder = w.sweepOrder;
   for (int i = threadIdx.x; i < np; i += BLOCK) {
     uint64_t k = ~0ull;
-    if (i < n) k = ((uint64_t)orderedKey(w.shapes[i].fat.lo.x) << 32) | (uint32_t)i;
+    if (i < n) {
+      const int s = sweepOrder[i];
+      k = ((uint64_t)orderedKey(w.shapes[s].fat.lo.x) << 32) | (uint32_t)s;
+    }
     keys[i] = k;
   }
   __syncthreads();
-  // sort keys only (index rides in the low word)
-  for (int k = 2; k <= np; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < np; i += BLOCK) {
-        int ixj = i ^ j;
-        if (ixj > i) {
-          bool up = (i & k) == 0;
-          uint64_t a = keys[i], b = keys[ixj];
-          if ((a > b) == up) {
-            keys[i] = b;
-            keys[ixj] = a;
-          }
+  // odd-even transposition passes while they still swap (a few for coherent
+  // motion); a bounded number of them, then a full bitonic sort. The keys are
+  // distinct, so the sorted order is the same either way.
+  bool sorted = false;
+  for (int pass = 0; pass < 16 && !sorted; ++pass) {
+    int swapped = 0;
+    for (int parity = 0; parity < 2; ++parity) {
+      for (int i = 2 * threadIdx.x + parity; i + 1 < n; i += 2 * BLOCK) {
+        uint64_t a = keys[i], b = keys[i + 1];
+        if (a > b) {
+          keys[i] = b;
+          keys[i + 1] = a;
+          swapped = 1;
         }
       }
       __syncthreads();
+    }
+    sorted = __syncthreads_or(swapped) == 0;
+  }
+  if (!sorted) {
+    for (int k = 2; k <= np; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < np; i += BLOCK) {
+          int ixj = i ^ j;
+          if (ixj > i) {
+            bool up = (i & k) == 0;
+            uint64_t a = keys[i], b = keys[ixj];
+            if ((a > b) == up) {
+              keys[i] = b;
+              keys[ixj] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
     }
   }
   // sorted boxes and indices
@@ -430,6 +456,7 @@ __device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int l
     AABB f = w.shapes[idx].fat;
     boxes[i] = make_float4(f.lo.x, f.lo.y, f.hi.x, f.hi.y);
     order[i] = idx;
+    sweepOrder[i] = idx;
     if (info) info[i] = pairInfo(w, idx);
   }
   __syncthreads();
