@@ -645,7 +645,7 @@ static int commit(Batch& b) {
       devAlloc(d, &v.pairs, nc) || devAlloc(d, &v.pairShapes, nc) || devAlloc(d, &v.joints, nj) ||
       devAlloc(d, &v.colorBits, ncol) || devAlloc(d, &v.cons, (size_t)nc * CN_FIELD_COUNT) ||
       devAlloc(d, &v.scratch, nscratch) || devAlloc(d, &v.ccolor, nc) || devAlloc(d, &v.staticShapes, ns) || devAlloc(d, &v.sweepOrder, ns) ||
-      devAlloc(d, &v.events, 3 * nc) || devAlloc(d, &v.islandQueue, nc) || devAlloc(d, &v.islandData, std::max(nisl, 4L)) ||
+      devAlloc(d, &v.events, 3 * nc) || devAlloc(d, &v.islandData, std::max(nisl, 4L)) ||
       devAlloc(d, &v.contactLinks, nc)) {
     b.lastError = "device allocation failed";
     freeDevice(b);
@@ -793,7 +793,6 @@ static int upload(Batch& b) {
     h.stepIndex = hw.stepIndex;
     h.jointCount = (int)hw.joints.size();
     h.mergePending = 1;  // imported or edited islands may carry parent links
-    h.islandQueueCount = 0;
     headers[i] = h;
     hipStream_t s = b.stream;
     F2D_HIP(hipMemcpyAsync(v.sims + h.bodyOffset, hw.sims.data(), sizeof(BodySim) * h.bodyCount, hipMemcpyHostToDevice, s));

@@ -210,7 +210,6 @@ struct alignas(16) WorldHeader {
   int moveBodyCount;             // awake bodies finalized by the last step (move events)
   int sleepPending;              // finalize ran with sleep enabled: island sleep follows
   int mergePending;              // an island got a union-find parent since the last merge
-  int islandQueueCount;          // deferred island operations (worlds with sleep off)
   unsigned long long splitKey;   // finalize: sleepiest body of an island with removed constraints
 };
 
@@ -309,7 +308,6 @@ struct BatchView {
   int8_t* ccolor;       // per contact slot: constraint-graph colour, -1 when none or free
   int* staticShapes;    // per world, the shape ids of its static proxies in shape order
   int* sweepOrder;      // per world, its shape ids in the last pair update's sweep order
-  int* islandQueue;     // per world at contactOffset: island link/unlink operations of the last state loop
   int* events;          // per world at 3 * contactOffset: begin contact ids [contactCap], end shape pairs [2 * contactCap]
   int* islandData;      // per world at islandOffset: the island region
   int4* contactLinks;   // per contact slot: island id, island prev, island next, solver set

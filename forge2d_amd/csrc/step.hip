@@ -173,7 +173,6 @@ struct WorldPtrs {
   int* events;
   int4* links;      // contact island links and solver set
   int* sweepOrder;
-  int* islandQueue;
   IslandCtx isl;
 };
 
@@ -198,7 +197,6 @@ __device__ WorldPtrs worldPtrs(const BatchView& b, WorldHeader* h) {
   w.events = b.events + 3L * h->contactOffset;
   w.sapBoxes = b.sapBoxes + h->shapeOffset;
   w.sweepOrder = b.sweepOrder + h->shapeOffset;
-  w.islandQueue = b.islandQueue + h->contactOffset;
   w.links = b.contactLinks + h->contactOffset;
   w.isl = islandCtx(h, b.islandData + h->islandOffset, w.sims, w.contacts, w.links, w.joints);
   return w;
@@ -962,14 +960,7 @@ __device__ __noinline__ void sleepPhase(const WorldPtrs& w) {
   }
 }
 
-// Island operations a state loop defers (worlds with sleep off): link a contact
-// that began touching, unlink one that stopped, unlink and clear one destroyed.
-enum : int { IQ_LINK = 0, IQ_UNLINK = 1, IQ_UNLINK_CLEAR = 2 };
-
-// Serial contact state loop, S/world.c:587-690. With sleep off no island can be
-// asleep, so linking never wakes a set and the island graph does not feed back
-// into this step: its operations are queued in order and replayed by the island
-// kernel, one world per lane (the same operations in the same order).
+// Serial contact state loop, S/world.c:587-690
 __device__ void statePhase(const WorldPtrs& w, SharedState& sh) {
   if (threadIdx.x == 0) {
     const int* list = w.scratch;
@@ -978,8 +969,6 @@ __device__ void statePhase(const WorldPtrs& w, SharedState& sh) {
     int* beginIds = w.events;
     int* endPairs = w.events + h->contactCap;
     int nBegin = 0, nEnd = 0;
-    const bool deferIslands = (h->flags & WF_SLEEP) == 0;
-    int nQueue = h->islandQueueCount;
     for (int k = 0; k < n; ++k) {
       int id = list[k];
       Contact& c = w.contacts[id];
@@ -991,27 +980,15 @@ __device__ void statePhase(const WorldPtrs& w, SharedState& sh) {
           endPairs[2 * nEnd + 1] = c.shapeB;
           ++nEnd;
         }
-        if (deferIslands && w.links[id].x != -1) {
-          // the queued unlink still needs the island links: keep them, clear later
-          const int4 keep = w.links[id];
-          destroyContact(w, id);
-          w.links[id] = keep;
-          w.islandQueue[nQueue++] = (id << 2) | IQ_UNLINK_CLEAR;
-        } else {
-          if (w.links[id].x != -1) {
-            IslandCtx ic = w.isl;
-            unlinkContact(ic, id);
-          }
-          destroyContact(w, id);
+        if (w.links[id].x != -1) {
+          IslandCtx ic = w.isl;
+          unlinkContact(ic, id);
         }
+        destroyContact(w, id);
       } else if (simFlags & SIM_STARTED) {
         if ((c.flags & CF_CONTACT_EVENTS) && nBegin < h->contactCap) beginIds[nBegin++] = id;
         c.flags |= CF_TOUCHING;
-        if (deferIslands) {
-          w.islandQueue[nQueue++] = (id << 2) | IQ_LINK;
-        } else {
-          linkContact(w, id);
-        }
+        linkContact(w, id);
         c.simFlags &= ~SIM_STARTED;
         addContactToGraph(w, id);
       } else if (simFlags & SIM_STOPPED) {
@@ -1022,9 +999,7 @@ __device__ void statePhase(const WorldPtrs& w, SharedState& sh) {
           endPairs[2 * nEnd + 1] = c.shapeB;
           ++nEnd;
         }
-        if (deferIslands) {
-          w.islandQueue[nQueue++] = (id << 2) | IQ_UNLINK;
-        } else if (w.links[id].x != -1) {
+        if (w.links[id].x != -1) {
           IslandCtx ic = w.isl;
           unlinkContact(ic, id);
         }
@@ -1033,35 +1008,8 @@ __device__ void statePhase(const WorldPtrs& w, SharedState& sh) {
     }
     h->beginCount = nBegin;
     h->endCount = nEnd;
-    h->islandQueueCount = nQueue;
   }
   __syncthreads();
-}
-
-// Replay of the queued island operations and the merge of awake islands, one
-// world per lane (worlds with sleep off; their state loops queued the operations).
-__device__ void replayIslandQueue(const WorldPtrs& w) {
-  WorldHeader* h = w.hdr;
-  IslandCtx ic = w.isl;
-  const int n = h->islandQueueCount;
-  for (int k = 0; k < n; ++k) {
-    const int op = w.islandQueue[k];
-    const int id = op >> 2;
-    switch (op & 3) {
-      case IQ_LINK:
-        linkContactIslands(ic, id);
-        break;
-      case IQ_UNLINK:
-        if (w.links[id].x != -1) unlinkContact(ic, id);
-        break;
-      default:
-        if (w.links[id].x != -1) unlinkContact(ic, id);
-        w.links[id] = make_int4(-1, -1, -1, -1);
-        break;
-    }
-  }
-  h->islandQueueCount = 0;
-  mergeAwakeIslands(ic);
 }
 
 // ---------------------------------------------------------------------------- phase: solve + finalize
@@ -1721,18 +1669,6 @@ __global__ __launch_bounds__(BLOCK) void continuousKernel(BatchView b, const int
   }
 }
 
-// The deferred island operations of a launch's worlds, one world per lane.
-__global__ __launch_bounds__(64) void islandKernel(BatchView b, const int* worldList, int nWorlds) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  if (k >= nWorlds) return;
-  const int wi = worldList ? worldList[k] : k;
-  WorldHeader* h = b.worlds + wi;
-  if (h->islandQueueCount == 0 && !h->mergePending) return;
-  if (h->flags & WF_SLEEP) return;  // sleep worlds keep their island work inline
-  WorldPtrs w = worldPtrs(b, h);
-  replayIslandQueue(w);
-}
-
 // Island commands of the body and world API, one world, thread 0:
 //   1  b2WakeBody: wake the sleeping set of body `arg` (S/body.c:332-344)
 //   2  b2World_EnableSleeping(false): wake every sleeping set in set order (S/world.c:1710-1737)
@@ -1768,9 +1704,6 @@ static int launchBlock(const BatchView* b, const StepParams* sp, const int* worl
   if (sp->phases & (PH_PAIRS | PH_COLLIDE | PH_STATE)) {
     hipLaunchKernelGGL(collideKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->collideLds, stream, *b, *sp, worldList,
                        cfg->collideLds);
-  }
-  if (sp->phases & PH_STATE) {
-    hipLaunchKernelGGL(islandKernel, dim3((nWorlds + 63) / 64), dim3(64), 0, stream, *b, worldList, nWorlds);
   }
   if (sp->phases & (PH_SOLVE | PH_FINALIZE)) {
     hipLaunchKernelGGL(solveKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->solveLds, stream, *b, *sp, worldList,
