@@ -4,13 +4,11 @@ Requests arrive over time; each names a world of a population and a number of
 steps k. A world that is not resident is created on the device (admission is
 bounded by an HBM budget; when the budget is full, idle worlds are evicted in
 least-recently-used order, a whole device batch at a time); a resident world is
-continued from its device state. New worlds are admitted in batches of similar
-size (a batch's step lasts as long as its largest world's), each batch on its own
-HIP stream. The scheduler keeps every batch busy on its own: whenever a batch's
-last step has finished (an event on its stream), its worlds that still owe steps
-get the next step at once, without waiting for the other batches; requests that
-arrive meanwhile join at the batch's next launch. A request completes when its
-k-th step has finished; its latency is completion time minus arrival time.
+continued from its device state. Every scheduler iteration launches one step for
+every world that still owes steps — across all resident batches, each on its
+own HIP stream so their kernels overlap — and requests that arrive meanwhile
+join at the next iteration. A request completes when its k-th step has
+finished; its latency is completion time minus arrival time.
 
 Worlds never interact, so a world's trajectory is the same whether it is stepped
 alone or in the mix (`tests/test_gpu_serving.py` checks this bit for bit).
@@ -60,26 +58,6 @@ class Generation:
         self.bytes = self.batch.device_bytes()
         self.bodies = [self.batch.body_count(i) for i in range(len(specs))]
         self.alive = len(specs)
-        self.inflight = None   # (event, [(world, resident), ...]) of the step on the device
-        self.event = None
-
-    def launch(self, items, dt, substeps):
-        """One step of the given resident worlds; returns at once."""
-        import torch
-
-        self.batch.step_worlds([res.index for _, res in items], dt, substeps, 1)
-        if self.stream is not None:
-            if self.event is None:
-                self.event = torch.cuda.Event()
-            self.event.record(self.stream)
-        else:
-            self.batch.synchronize()  # the library's own stream: wait here
-        self.inflight = list(items)
-
-    def done(self) -> bool:
-        if self.inflight is None:
-            return False
-        return self.stream is None or self.event.query()
 
     def close(self):
         self.batch.close()
@@ -150,206 +128,6 @@ class ServingEngine:
             g.close()
             self.evicted += 1
         return self.used + need <= self.budget
-
-    # new worlds are batched with worlds of similar size: bodies below each bound
-    SIZE_BOUNDS = (256, 1024, 4096, 1 << 30)
-
-    def _admit_grouped(self, worlds: list[int], now: float):
-        groups: dict[int, list[int]] = collections.defaultdict(list)
-        for w in worlds:
-            k = next(i for i, b in enumerate(self.SIZE_BOUNDS) if self.pop.sizes[w] < b)
-            groups[k].append(w)
-        for k in sorted(groups):
-            ws = groups[k]
-            if not self._admit(ws, now):
-                # budget full of busy worlds: admit what fits one by one, the rest waits
-                for w in ws:
-                    if not self._admit([w], now):
-                        break
-
-    def _admit(self, worlds: list[int], now: float):
-        need = sum(self._estimate(w) for w in worlds)
-        if self.used + need > self.budget and not self._evict_until(need, now):
-            return False
-        stream = None
-        if self.use_streams:
-            import torch
-
-            stream = torch.cuda.Stream()
-        g = Generation(worlds, [self.pop.spec(w) for w in worlds], stream)
-        self.generations.append(g)
-        self.used += g.bytes
-        for i, w in enumerate(worlds):
-            self.resident[w] = _Resident(g, i, 0, now)
-        return True
-
-    # -- scheduling ------------------------------------------------------------------------------------------
-    def run(self, requests: list[Request], horizon: float, log=None) -> dict:
-        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
-        waiting: collections.deque = collections.deque()
-        finished: list[Request] = []
-        t0 = time.perf_counter()
-        launches = 0
-        while True:
-            now = time.perf_counter() - t0
-            while pending and pending[0].arrival <= now:
-                waiting.append(pending.popleft())
-            # admit new worlds (grouped by size), queue steps on resident ones
-            new_worlds = []
-            for r in waiting:
-                if r.world not in self.resident and r.world not in new_worlds:
-                    new_worlds.append(r.world)
-            if new_worlds:
-                self._admit_grouped(new_worlds, now)
-            still = collections.deque()
-            for r in waiting:
-                res = self.resident.get(r.world)
-                if res is None:
-                    still.append(r)
-                    continue
-                res.queue.append(r)
-                res.owed += r.steps
-            waiting = still
-            # finished steps: account them, completing requests
-            progressed = False
-            for g in self.generations:
-                if not g.done():
-                    continue
-                now = time.perf_counter() - t0
-                for w, res in g.inflight:
-                    res.owed -= 1
-                    res.steps_done += 1
-                    res.last_used = now
-                    self.body_substeps += g.bodies[res.index] * self.substeps
-                    if res.queue:
-                        head = res.queue[0]
-                        if head.steps_left < 0:
-                            head.steps_left = head.steps
-                        head.steps_left -= 1
-                        if head.steps_left == 0:
-                            head.done = now
-                            finished.append(res.queue.popleft())
-                g.inflight = None
-                progressed = True
-            # idle batches with owed steps: launch their next step at once
-            owing: dict[int, list] = collections.defaultdict(list)
-            for w, res in self.resident.items():
-                if res.owed > 0 and res.batch.inflight is None:
-                    owing[id(res.batch)].append((w, res))
-            for items in owing.values():
-                items[0][1].batch.launch(items, self.dt, self.substeps)
-                launches += 1
-                progressed = True
-            self.iterations = launches
-            busy = any(g.inflight is not None for g in self.generations)
-            if not busy and not owing:
-                if not pending and not waiting:
-                    break
-                if pending and not waiting:
-                    time.sleep(max(0.0, min(pending[0].arrival - now, 0.001)))
-                continue
-            if not progressed:
-                time.sleep(0.0001)
-            if log and launches and launches % 1000 == 0 and progressed:
-                log(f"[serve] t={now:.1f}s launches={launches} resident={len(self.resident)} "
-                    f"done={len(finished)} used={self.used / 2**30:.2f} GiB")
-        wall = time.perf_counter() - t0
-        lat = np.array([r.done - r.arrival for r in finished]) * 1e3
-        return {
-            "requests": len(requests), "completed": len(finished), "wall_s": wall,
-            "latency_ms": {"p50": float(np.percentile(lat, 50)) if lat.size else None,
-                           "p99": float(np.percentile(lat, 99)) if lat.size else None,
-                           "mean": float(lat.mean()) if lat.size else None},
-            "body_substeps_per_s": self.body_substeps / wall, "iterations": self.iterations,
-            "resident_worlds": len(self.resident), "batches": len(self.generations),
-            "hbm_used_gib": self.used / 2**30, "evicted_batches": self.evicted,
-        }
-
-    def close(self):
-        self.batch.close()
-
-
-def world_spec(w: scenes.World) -> bytes:
-    """The per-world part of a scene spec (header, bodies, shapes, joints)."""
-    full = scenes.to_bytes([w])
-    return full[len(scenes.MAGIC) + 8:]
-
-
-class Population:
-    """World sizes log-uniform in [lo, hi] bodies; worlds are the container mix
-    (circles, capsules, polygons) built from a few hundred seeded templates."""
-
-    def __init__(self, count: int, lo: int = 50, hi: int = 20000, templates: int = 96, seed: int = 5):
-        rng = np.random.default_rng(seed)
-        self.sizes = np.exp(rng.uniform(math.log(lo), math.log(hi), count)).round().astype(int)
-        edges = np.exp(np.linspace(math.log(lo), math.log(hi), templates))
-        self.template_of = np.clip(np.searchsorted(edges, self.sizes), 0, templates - 1)
-        self._edges = edges
-        self._cache: dict[int, bytes] = {}
-
-    def spec(self, world: int) -> bytes:
-        t = int(self.template_of[world])
-        if t not in self._cache:
-            n = int(round(self._edges[t]))
-            side = max(8.0, math.sqrt(n) * 1.2)
-            self._cache[t] = world_spec(scenes.container_world(n, side, 2.0 * side, scenes.stream_seed(5, t),
-                                                               continuous=True, drop_band=(0.05, 0.95)))
-        return self._cache[t]
-
-
-class ServingEngine:
-    def __init__(self, population: Population, hbm_budget: int, dt: float = 1.0 / 60.0, substeps: int = 4,
-                 use_streams: bool = True):
-        self.pop = population
-        self.budget = hbm_budget
-        self.dt, self.substeps = dt, substeps
-        self.resident: dict[int, _Resident] = {}
-        self.generations: list[Generation] = []
-        self.used = 0
-        self.use_streams = use_streams
-        self.body_substeps = 0
-        self.iterations = 0
-        self.evicted = 0
-        self.refused = 0
-
-    # -- admission -------------------------------------------------------------------------------------------
-    def _estimate(self, world: int) -> int:
-        # conservative device bytes of a world: ~3.3 KB per body measured (contact
-        # slots, constraint lanes and pair scratch at 6 per shape dominate)
-        return 65536 + 4096 * int(self.pop.sizes[world])
-
-    def _evict_until(self, need: int, now: float) -> bool:
-        idle = [g for g in self.generations if all(self.resident[m].owed == 0 for m in g.members
-                                                    if m in self.resident)]
-        idle.sort(key=lambda g: max(self.resident[m].last_used for m in g.members if m in self.resident))
-        for g in idle:
-            if self.used + need <= self.budget:
-                break
-            if any(m in self.resident and self.resident[m].queue for m in g.members):
-                continue
-            for m in g.members:
-                self.resident.pop(m, None)
-            self.used -= g.bytes
-            self.generations.remove(g)
-            g.close()
-            self.evicted += 1
-        return self.used + need <= self.budget
-
-    # new worlds are batched with worlds of similar size: bodies below each bound
-    SIZE_BOUNDS = (256, 1024, 4096, 1 << 30)
-
-    def _admit_grouped(self, worlds: list[int], now: float):
-        groups: dict[int, list[int]] = collections.defaultdict(list)
-        for w in worlds:
-            k = next(i for i, b in enumerate(self.SIZE_BOUNDS) if self.pop.sizes[w] < b)
-            groups[k].append(w)
-        for k in sorted(groups):
-            ws = groups[k]
-            if not self._admit(ws, now):
-                # budget full of busy worlds: admit what fits one by one, the rest waits
-                for w in ws:
-                    if not self._admit([w], now):
-                        break
 
     def _admit(self, worlds: list[int], now: float):
         need = sum(self._estimate(w) for w in worlds)
