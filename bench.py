@@ -111,6 +111,81 @@ def cpu_baseline(spec: bytes, worlds: int, budget_s: float):
                 "step_ms_p50": r["step_ms_p50"], "step_ms_p99": r["step_ms_p99"], "wall_s": r["wall_s"]}
 
 
+def ray_queries(spec: bytes, rays_per_world: int, seed: int = 17) -> np.ndarray:
+    """Closest-ray queries (f2dQuery records), rays_per_world per world in world order:
+    origins uniform over the box of the world's initial body positions (1 m margin),
+    directions uniform, lengths uniform in [0.1, 1] x the box diagonal."""
+    parsed = scenes.parse(spec)
+    lo = np.array([[b["px"].min(), b["py"].min()] for _, b, _, _ in parsed], np.float64) - 1.0
+    hi = np.array([[b["px"].max(), b["py"].max()] for _, b, _, _ in parsed], np.float64) + 1.0
+    rng = np.random.default_rng(seed)
+    world = np.repeat(np.arange(len(parsed)), rays_per_world)
+    n = len(world)
+    origin = lo[world] + rng.random((n, 2)) * (hi - lo)[world]
+    angle = rng.uniform(-np.pi, np.pi, n)
+    length = rng.uniform(0.1, 1.0, n) * np.linalg.norm(hi - lo, axis=1)[world]
+    q = np.zeros(n, F.QUERY_DTYPE)
+    q["world"], q["kind"], q["cat_lo"], q["mask_lo"], q["mask_hi"] = world, F.Q_RAY_CLOSEST, 1, F.ALL_BITS, F.ALL_BITS
+    q["p"][:, 0], q["p"][:, 1] = origin[:, 0], origin[:, 1]
+    q["p"][:, 2], q["p"][:, 3] = length * np.cos(angle), length * np.sin(angle)
+    return q
+
+
+def measure_queries(args, batch, spec: bytes, W: int):
+    """Closest-ray throughput of the query kernel on the stepped batch (queries, hits and
+    counts resident in HBM, one launch per pass), beside the reference's
+    b2World_CastRayClosest on a sample of the same worlds and rays."""
+    import torch
+
+    q = ray_queries(spec, args.query_rays)
+    n = len(q)
+    dq = torch.from_numpy(q.view(np.uint8).copy()).cuda()
+    dh = torch.zeros(n * F.HIT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    dc = torch.zeros(n, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    L = F.lib()
+
+    def launch():
+        if L.f2dg_query_device(batch.h, dq.data_ptr(), n, dh.data_ptr(), dc.data_ptr(), 1) != 0:
+            raise RuntimeError("query launch failed")
+
+    launch()
+    batch.synchronize()
+    reps = 10
+    a = time.perf_counter()
+    for _ in range(reps):
+        launch()
+    batch.synchronize()
+    ms = 1e3 * (time.perf_counter() - a) / reps
+    counts = dc.cpu().numpy()
+    shapes = np.array([batch.shape_count(i) for i in range(W)], np.int64)
+    scan = 32.0 * args.query_rays * float(shapes.sum())
+    out = {"kind": "closest ray (b2World_CastRayClosest)", "rays": n, "rays_per_world": args.query_rays,
+           "ms_per_launch": ms, "rays_per_s": n / (ms * 1e-3), "hit_rate": float((counts > 0).mean()),
+           "kernel": "queryKernel", "scan_bytes_per_launch": scan, "scan_gbs": scan / (ms * 1e-3) / 1e9,
+           "sample": f"{args.query_rays} seeded rays per world over the world's box, after the bench's steps; "
+                     f"32 B of shape record (fat AABB, filter) scanned per shape per ray, largely L2-served"}
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not args.no_cpu and os.path.exists(harness):
+        m = min(W, 1024)
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        with tempfile.TemporaryDirectory() as td:
+            sp, qp = os.path.join(td, "scene.bin"), os.path.join(td, "q.bin")
+            with open(sp, "wb") as f:
+                f.write(spec)
+            with open(qp, "wb") as f:
+                f.write(q[q["world"] < m].tobytes())
+            r = subprocess.run([harness, "qtime", sp, qp, "60", str(threads), "0", str(m)], capture_output=True,
+                               text=True, timeout=600, check=True)
+            c = json.loads(r.stdout.strip().splitlines()[-1])
+        out["cpu_baseline"] = {"value": c["rays_per_s"], "unit": "rays/s", "cores": c["threads"], "kind": "reference",
+                               "sample": f"first {m} worlds, the same {c['rays']} rays after 60 steps, "
+                                         f"{c['threads']} threads querying disjoint worlds, reference Box2D v3.1.1 "
+                                         f"dynamic-tree ray casts (-std=c17 -O2)", "hit_rate": c["hits"] / max(1, c["rays"])}
+        out["vs_cpu"] = out["rays_per_s"] / c["rays_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +210,8 @@ def main():
                     help="comma list of workgroup lanes per size class instead of the autotuner's choice")
     ap.add_argument("--phase-times", type=int, default=0, metavar="STEPS",
                     help="after the timed steps, time the step's phases as separate launches over STEPS steps")
+    ap.add_argument("--query-rays", type=int, default=64, metavar="N",
+                    help="closest-ray queries per world timed after the step measurement (0: none)")
     ap.add_argument("--solve-lds", default="",
                     help="comma list of solve-launch LDS bytes per size class (with --lanes)")
     args = ap.parse_args()
@@ -150,6 +227,12 @@ def main():
         serve(args)
         return
 
+    if args.query_rays > 0 and not args.kernel_profile:
+        # the query measurement keeps its rays and hits in torch tensors: torch's HIP
+        # context comes up before the library's first device call
+        import torch
+
+        torch.zeros(1, device="cuda")
     desc, make = WORKLOADS[args.workload]
     t0 = time.time()
     worlds = make(args.worlds)
@@ -251,6 +334,11 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e}")
 
+    queries = None
+    if args.query_rays > 0:
+        queries = measure_queries(args, batch, spec, W)
+        log(f"[bench] queries: {queries}")
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -283,6 +371,7 @@ def main():
                                       for k in launch_ms if k != dominant},
                      "step_achieved_gbs": step_gbs},
         "cpu_baseline": cpu,
+        "queries": queries,
     }
     print(json.dumps(out), flush=True)
 

@@ -111,7 +111,37 @@ SIGNATURES = [
     ("f2d_world_get_body_event_count", _I, [_U]),
     ("f2d_world_get_body_events", _I, [_U, _PD]),
     ("f2d_world_batch", _VP, [_U]),
+    ("f2d_set_query_callbacks", None, [_VP, _VP]),
+    ("f2d_world_cast_ray_closest", _I, [_U, _F, _F, _F, _F, _U, _U, _U, _U, _PD]),
+    ("f2d_world_cast_ray", None, [_U, _F, _F, _F, _F, _U, _U, _U, _U]),
+    ("f2d_world_overlap_aabb", None, [_U, _F, _F, _F, _F, _U, _U, _U, _U]),
+    ("f2d_world_explode", None, [_U, _U, _U, _F, _F, _F, _F, _F]),
+    ("f2dg_query", _I, [_VP, _VP, _I, _VP, _PI, _I]),
+    ("f2dg_query_device", _I, [_VP, _VP, _I, _VP, _VP, _I]),
 ]
+
+# query kinds and records of f2dg_query (include/forge2d_amd.h)
+Q_RAY_CLOSEST, Q_RAY_ALL, Q_OVERLAP_AABB, Q_EXPLODE = 1, 2, 3, 4
+QUERY_DTYPE = np.dtype([("world", "<i4"), ("kind", "<i4"), ("cat_lo", "<u4"), ("cat_hi", "<u4"),
+                        ("mask_lo", "<u4"), ("mask_hi", "<u4"), ("p", "<f4", (6,))])
+HIT_DTYPE = np.dtype([("shape", "<i4"), ("px", "<f4"), ("py", "<f4"), ("nx", "<f4"), ("ny", "<f4"),
+                      ("fraction", "<f4")])
+CAST_RAY_FN = ctypes.CFUNCTYPE(ctypes.c_float, ctypes.c_int32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_float, ctypes.c_float)
+OVERLAP_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int32, ctypes.c_uint32)
+ALL_BITS = 0xFFFFFFFF
+
+
+def make_queries(rows) -> np.ndarray:
+    """rows of (world, kind, params[, (cat_lo, cat_hi, mask_lo, mask_hi)]) -> f2dQuery records."""
+    q = np.zeros(len(rows), QUERY_DTYPE)
+    for i, r in enumerate(rows):
+        world, kind, params = r[0], r[1], r[2]
+        bits = r[3] if len(r) > 3 else (1, 0, ALL_BITS, ALL_BITS)
+        q[i]["world"], q[i]["kind"] = world, kind
+        q[i]["cat_lo"], q[i]["cat_hi"], q[i]["mask_lo"], q[i]["mask_hi"] = bits
+        q[i]["p"][:len(params)] = params
+    return q
 
 
 class LibraryMissing(RuntimeError):
@@ -175,6 +205,9 @@ class Batch:
     def set_stream(self, stream_handle: int):
         """Launch on this hipStream_t (an integer handle, e.g. torch.cuda.Stream().cuda_stream)."""
         self._check(lib().f2dg_set_stream(self.h, ctypes.c_void_p(stream_handle)), "set_stream")
+
+    def shape_count(self, world: int) -> int:
+        return lib().f2dg_shape_count(self.h, world)
 
     def body_count(self, world: int) -> int:
         return lib().f2dg_body_count(self.h, world)
@@ -313,6 +346,16 @@ class Batch:
 
     def device_bytes(self) -> int:
         return lib().f2dg_device_bytes(self.h)
+
+    def query(self, queries: np.ndarray, cap: int = 64) -> tuple[np.ndarray, np.ndarray]:
+        """Run f2dQuery records (make_queries); returns (counts, hits[n, cap])."""
+        queries = np.ascontiguousarray(queries, QUERY_DTYPE)
+        n = len(queries)
+        hits = np.zeros((max(n, 1), cap), HIT_DTYPE)
+        counts = np.zeros(max(n, 1), np.int32)
+        self._check(lib().f2dg_query(self.h, queries.ctypes.data, n, hits.ctypes.data,
+                                     counts.ctypes.data_as(_PI), cap), "query")
+        return counts[:n], hits[:n]
 
     def awake_bodies(self) -> int:
         return lib().f2dg_awake_body_count(self.h)

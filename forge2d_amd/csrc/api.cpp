@@ -26,6 +26,9 @@
 extern "C" int f2d_launch_step(const f2d::BatchView* b, const f2d::StepParams* sp, const int* worldList, int nWorlds,
                                hipStream_t stream, const f2d::LaunchConfig* cfg);
 extern "C" int f2d_launch_island_command(const f2d::BatchView* b, int world, int op, int arg, hipStream_t stream);
+extern "C" int f2d_launch_queries(const f2d::BatchView* b, const f2d::Query* q, int n, f2d::QueryHit* hits,
+                                  int* counts, int cap, hipStream_t stream);
+extern "C" int f2d_launch_explosions(const f2d::BatchView* b, const f2d::Query* q, int n, hipStream_t stream);
 
 namespace f2d {
 
@@ -555,6 +558,13 @@ static void freeDevice(Batch& b) {
   b.classes.clear();
   if (b.forkEvent) (void)hipEventDestroy(b.forkEvent);
   b.forkEvent = nullptr;
+  if (b.dQueries) (void)hipFree(b.dQueries);
+  if (b.dHits) (void)hipFree(b.dHits);
+  if (b.dCounts) (void)hipFree(b.dCounts);
+  b.dQueries = nullptr;
+  b.dHits = nullptr;
+  b.dCounts = nullptr;
+  b.dQueryCap = b.dHitCap = 0;
   b.committed = false;
 }
 
@@ -2596,6 +2606,225 @@ int f2d_world_get_body_event_count(uint32_t w) {
 int f2d_world_get_body_events(uint32_t w, double* out) {
   Batch* b = batchOf(w);
   return b ? bodyEvents(*b, 0, out, (int)(w & 0xFFFF) - 1) : 0;
+}
+
+}  // extern "C"
+
+// ---- world queries (S/world.c:2050-2308, 2640-2747) ------------------------------
+
+// the public header's query records and host callbacks (include/forge2d_amd.h)
+typedef float (*f2dHostCastRayFn)(int32_t shape_index1, uint32_t shape_wg, float px, float py, float nx, float ny,
+                                  float fraction);
+typedef int (*f2dHostOverlapFn)(int32_t shape_index1, uint32_t shape_wg);
+typedef struct f2dQuery {
+  int32_t world;
+  int32_t kind;
+  uint32_t category_lo, category_hi, mask_lo, mask_hi;
+  float p[6];
+} f2dQuery;
+typedef struct f2dQueryHit {
+  int32_t shape;
+  float px, py, nx, ny, fraction;
+} f2dQueryHit;
+
+namespace f2d {
+
+static int ensureQueryBuffers(Batch& b, long n, long hitCount) {
+  if (n > b.dQueryCap) {
+    if (b.dQueries) (void)hipFree(b.dQueries);
+    if (b.dCounts) (void)hipFree(b.dCounts);
+    b.dQueries = nullptr;
+    b.dCounts = nullptr;
+    b.dQueryCap = 0;
+    const long c = std::max<long>(n, 64);
+    F2D_HIP(hipMalloc(&b.dQueries, sizeof(Query) * c));
+    F2D_HIP(hipMalloc(&b.dCounts, sizeof(int) * c));
+    b.dQueryCap = c;
+  }
+  if (hitCount > b.dHitCap) {
+    if (b.dHits) (void)hipFree(b.dHits);
+    b.dHits = nullptr;
+    b.dHitCap = 0;
+    const long c = std::max<long>(hitCount, 256);
+    F2D_HIP(hipMalloc(&b.dHits, sizeof(QueryHit) * c));
+    b.dHitCap = c;
+  }
+  return 0;
+}
+
+// Rays and overlaps in one launch; explosions afterwards in rounds: the k-th explosion
+// of a world runs in round k, so explosions of one world apply in query order and
+// those of different worlds side by side.
+static int runQueries(Batch& b, const Query* q, int n, QueryHit* hits, int* counts, int cap) {
+  if (n < 0 || cap < 1) return -1;
+  if (n == 0) return 0;
+  if (b.hostDirty || !b.committed) {
+    if (upload(b) != 0) return -1;
+  }
+  std::vector<int> roundOf(b.worlds.size(), 0);
+  std::vector<std::vector<Query>> rounds;
+  for (int i = 0; i < n; ++i) {
+    if (q[i].kind != QK_EXPLODE || q[i].world < 0 || q[i].world >= (int)b.worlds.size()) continue;
+    const int r = roundOf[q[i].world]++;
+    if ((int)rounds.size() <= r) rounds.resize(r + 1);
+    rounds[r].push_back(q[i]);
+  }
+  long explosions = 0;
+  for (const auto& r : rounds) explosions += (long)r.size();
+  if (ensureQueryBuffers(b, n + explosions, (long)n * cap) != 0) return -1;
+  F2D_HIP(hipMemcpyAsync(b.dQueries, q, sizeof(Query) * n, hipMemcpyHostToDevice, b.stream));
+  if (f2d_launch_queries(&b.dev.view, b.dQueries, n, b.dHits, b.dCounts, cap, b.stream) != 0) return -1;
+  long off = n;
+  for (const auto& r : rounds) {
+    F2D_HIP(hipMemcpyAsync(b.dQueries + off, r.data(), sizeof(Query) * r.size(), hipMemcpyHostToDevice, b.stream));
+    if (f2d_launch_explosions(&b.dev.view, b.dQueries + off, (int)r.size(), b.stream) != 0) return -1;
+    off += (long)r.size();
+  }
+  F2D_HIP(hipMemcpyAsync(counts, b.dCounts, sizeof(int) * n, hipMemcpyDeviceToHost, b.stream));
+  F2D_HIP(hipStreamSynchronize(b.stream));
+  // copy back only the hit slots that hold results
+  long used = 0;
+  for (int i = 0; i < n; ++i) used = std::max<long>(used, (long)i * cap + std::min(std::max(counts[i], 0), cap));
+  if (used > 0) F2D_HIP(hipMemcpy(hits, b.dHits, sizeof(QueryHit) * used, hipMemcpyDeviceToHost));
+  if (explosions > 0) b.deviceNewer = true;
+  return 0;
+}
+
+static f2dHostCastRayFn g_castRayFn = nullptr;
+static f2dHostOverlapFn g_overlapFn = nullptr;
+
+static int flatQuery(Batch& b, const Query& q, std::vector<QueryHit>& out) {
+  const int cap = std::max((int)b.worlds[0].shapes.size(), 1);
+  out.resize(cap);
+  int count = 0;
+  if (runQueries(b, &q, 1, out.data(), &count, cap) != 0) return -1;
+  out.resize(std::max(count, 0));
+  return 0;
+}
+
+// the tree (static 0, kinematic 1, dynamic 2) a shape's proxy lives in
+static int treeOf(const HostWorld& hw, int shape) { return hw.shapes[shape].proxyKey & 3; }
+
+}  // namespace f2d
+
+extern "C" {
+
+int f2dg_query(void* h, const f2dQuery* queries, int n, f2dQueryHit* hits, int32_t* counts, int cap) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || (n > 0 && (!queries || !hits || !counts))) return -1;
+  static_assert(sizeof(f2dQuery) == sizeof(Query) && sizeof(f2dQueryHit) == sizeof(QueryHit), "query records");
+  return runQueries(*b, reinterpret_cast<const Query*>(queries), n, reinterpret_cast<QueryHit*>(hits), counts, cap);
+}
+
+int f2dg_query_device(void* h, const void* queries, int n, void* hits, void* counts, int cap) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || n < 0 || cap < 1) return -1;
+  if (n == 0) return 0;
+  if (b->hostDirty || !b->committed) {
+    if (upload(*b) != 0) return -1;
+  }
+  return f2d_launch_queries(&b->dev.view, static_cast<const Query*>(queries), n, static_cast<QueryHit*>(hits),
+                            static_cast<int*>(counts), cap, b->stream);
+}
+
+void f2d_set_query_callbacks(f2dHostCastRayFn cast_ray, f2dHostOverlapFn overlap) {
+  g_castRayFn = cast_ray;
+  g_overlapFn = overlap;
+}
+
+static Query flatQueryRecord(int kind, float a, float b_, float c, float d, uint32_t catLo, uint32_t catHi,
+                             uint32_t maskLo, uint32_t maskHi) {
+  Query q;
+  std::memset(&q, 0, sizeof(q));
+  q.world = 0;
+  q.kind = kind;
+  q.catLo = catLo;
+  q.catHi = catHi;
+  q.maskLo = maskLo;
+  q.maskHi = maskHi;
+  q.p[0] = a;
+  q.p[1] = b_;
+  q.p[2] = c;
+  q.p[3] = d;
+  return q;
+}
+
+// f2d_shim.c:1313-1333 -> b2World_CastRayClosest
+int f2d_world_cast_ray_closest(uint32_t w, float ox, float oy, float tx, float ty, uint32_t category_lo,
+                               uint32_t category_hi, uint32_t mask_lo, uint32_t mask_hi, double* out) {
+  Batch* b = batchOf(w);
+  if (!b) return 0;
+  Query q = flatQueryRecord(QK_RAY_CLOSEST, ox, oy, tx, ty, category_lo, category_hi, mask_lo, mask_hi);
+  QueryHit hit;
+  int count = 0;
+  if (runQueries(*b, &q, 1, &hit, &count, 1) != 0 || count < 1) return 0;
+  out[0] = hit.shape + 1;
+  out[1] = shapeWgOf(w);
+  out[2] = hit.px;
+  out[3] = hit.py;
+  out[4] = hit.nx;
+  out[5] = hit.ny;
+  out[6] = hit.fraction;
+  return 1;
+}
+
+// f2d_shim.c:1343-1350 -> b2World_CastRay (S/world.c:2222-2254): per tree, the hits
+// reach the host callback nearest first; a return of 0 ends the cast, a value in
+// [0, 1] becomes the world's fraction (the next tree's ray length), a value in
+// (0, current] clips the current tree's ray so hits beyond it are skipped
+void f2d_world_cast_ray(uint32_t w, float ox, float oy, float tx, float ty, uint32_t category_lo,
+                        uint32_t category_hi, uint32_t mask_lo, uint32_t mask_hi) {
+  Batch* b = batchOf(w);
+  if (!b || !g_castRayFn) return;
+  Query q = flatQueryRecord(QK_RAY_ALL, ox, oy, tx, ty, category_lo, category_hi, mask_lo, mask_hi);
+  std::vector<QueryHit> hits;
+  if (flatQuery(*b, q, hits) != 0) return;
+  const HostWorld& hw = b->worlds[0];
+  std::stable_sort(hits.begin(), hits.end(), [](const QueryHit& x, const QueryHit& y) {
+    return x.fraction < y.fraction || (x.fraction == y.fraction && x.shape < y.shape);
+  });
+  float worldFraction = 1.0f;
+  for (int tree = kStatic; tree <= kDynamic; ++tree) {
+    float maxFraction = worldFraction;
+    for (const QueryHit& hit : hits) {
+      if (treeOf(hw, hit.shape) != tree || hit.fraction > maxFraction) continue;
+      const float value = g_castRayFn(hit.shape + 1, shapeWgOf(w), hit.px, hit.py, hit.nx, hit.ny, hit.fraction);
+      if (0.0f <= value && value <= 1.0f) worldFraction = value;
+      if (value == 0.0f) return;
+      if (0.0f < value && value <= maxFraction) maxFraction = value;
+    }
+    if (worldFraction == 0.0f) return;
+  }
+}
+
+// f2d_shim.c:1357-1367 -> b2World_OverlapAABB (S/world.c:2071-2105): per tree in shape-id
+// order; a false return ends that tree's walk
+void f2d_world_overlap_aabb(uint32_t w, float lx, float ly, float ux, float uy, uint32_t category_lo,
+                            uint32_t category_hi, uint32_t mask_lo, uint32_t mask_hi) {
+  Batch* b = batchOf(w);
+  if (!b || !g_overlapFn) return;
+  Query q = flatQueryRecord(QK_OVERLAP_AABB, lx, ly, ux, uy, category_lo, category_hi, mask_lo, mask_hi);
+  std::vector<QueryHit> hits;
+  if (flatQuery(*b, q, hits) != 0) return;
+  const HostWorld& hw = b->worlds[0];
+  for (int tree = kStatic; tree <= kDynamic; ++tree) {
+    for (const QueryHit& hit : hits) {
+      if (treeOf(hw, hit.shape) != tree) continue;
+      if (!g_overlapFn(hit.shape + 1, shapeWgOf(w))) break;
+    }
+  }
+}
+
+// f2d_shim.c:177-189 -> b2World_Explode
+void f2d_world_explode(uint32_t w, uint32_t mask_lo, uint32_t mask_hi, float x, float y, float radius,
+                       float falloff, float impulse_per_length) {
+  Batch* b = batchOf(w);
+  if (!b) return;
+  Query q = flatQueryRecord(QK_EXPLODE, x, y, radius, falloff, 0u, 0u, mask_lo, mask_hi);
+  q.p[4] = impulse_per_length;
+  QueryHit unused;
+  int count = 0;
+  (void)runQueries(*b, &q, 1, &unused, &count, 1);
 }
 
 }  // extern "C"

@@ -267,6 +267,24 @@ enum : int {
   PH_NO_CONTINUOUS = 128,  // finalize without the continuous-collision launch (timing splits)
 };
 
+// A world query of the batch query ABI (f2dQuery in the public header). Rays:
+// p = origin x/y, translation x/y; AABB overlaps: p = lower x/y, upper x/y;
+// explosions: p = position x/y, radius, falloff, impulse per length, with the mask
+// in maskLo/maskHi.
+enum QueryKind : int { QK_RAY_CLOSEST = 1, QK_RAY_ALL = 2, QK_OVERLAP_AABB = 3, QK_EXPLODE = 4 };
+struct Query {
+  int world;
+  int kind;
+  uint32_t catLo, catHi, maskLo, maskHi;
+  float p[6];
+};
+// A ray hit (shape id, point, normal, fraction) or an overlapping shape (shape id)
+struct QueryHit {
+  int shape;
+  float px, py, nx, ny, fraction;
+};
+static_assert(sizeof(Query) == 48 && sizeof(QueryHit) == 24, "query records");
+
 // Launch shape of a batch: workgroup size and dynamic LDS bytes per kernel.
 struct LaunchConfig {
   int block;       // 64, 128 or 256 lanes per world

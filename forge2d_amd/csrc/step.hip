@@ -25,6 +25,7 @@
 #include "joint.hpp"
 #include "toi.hpp"
 #include "island.hpp"
+#include "query.hpp"
 
 namespace f2d {
 
@@ -1698,6 +1699,211 @@ __global__ void islandCommandKernel(BatchView b, int world, int op, int arg) {
   }
 }
 
+// ---------------------------------------------------------------------------- queries
+
+constexpr int kQueryLanes = 64;  // one wavefront per query
+
+// The box of the ray segment at maxFraction 1 (segmentAABB of b2DynamicTree_RayCast,
+// S/dynamic_tree.c:1172-1200). A shape's exact hit lies on the shape, inside its fat
+// AABB by at least the speculative margin, and on the segment, so the shapes this box
+// culls are exactly those whose ray casts cannot hit.
+__device__ __forceinline__ AABB rayBox(Vec2 p1, Vec2 d) {
+  Vec2 p2 = mulAdd(p1, 1.0f, d);
+  return {vmin(p1, p2), vmax(p1, p2)};
+}
+
+// Ray casts and AABB overlaps over the worlds of a batch, one query per wavefront,
+// the lanes over the world's shapes (the shapes' fat AABBs are the reference's tree
+// leaves, so the candidate sets are the trees' without a tree):
+//   QK_RAY_CLOSEST  b2World_CastRayClosest (S/world.c:2277-2308): the hit of
+//                   smallest fraction > 0 (initial overlaps are ignored, as
+//                   b2RayCastClosestFcn does); ties go to the lower shape id
+//   QK_RAY_ALL      b2World_CastRay with a callback that returns 1 (S/world.c:2185-2254):
+//                   every hit, initial overlaps included, in shape-id order
+//   QK_OVERLAP_AABB b2World_OverlapAABB (S/world.c:2050-2105): every shape whose fat
+//                   AABB overlaps the box, in shape-id order
+// counts[q] receives the number of results (which may exceed cap: only the first cap
+// are written to hits[q * cap ...]), or -1 for a world index outside the batch.
+__global__ __launch_bounds__(kQueryLanes) void queryKernel(BatchView b, const Query* qs, QueryHit* hits, int* counts,
+                                                           int cap) {
+  __shared__ unsigned long long best;
+  const int qi = blockIdx.x;
+  const int lane = threadIdx.x;
+  const Query q = qs[qi];
+  if (q.world < 0 || q.world >= b.worldCount) {
+    if (lane == 0) counts[qi] = -1;
+    return;
+  }
+  const WorldHeader* h = b.worlds + q.world;
+  const Shape* shapes = b.shapes + h->shapeOffset;
+  const ShapeGeom* geoms = b.geoms + h->shapeOffset;
+  const BodySim* sims = b.sims + h->bodyOffset;
+  const int ns = h->shapeCount;
+  QueryHit* out = hits + (long)qi * cap;
+  const RayInput in = {{q.p[0], q.p[1]}, {q.p[2], q.p[3]}, 1.0f};
+  const AABB segment = rayBox(in.origin, in.translation);
+  if (q.kind == QK_RAY_CLOSEST) {
+    if (lane == 0) best = ~0ull;
+    __syncthreads();
+    unsigned long long key = ~0ull;
+    CastOutput mine = noHit();
+    for (int s = lane; s < ns; s += kQueryLanes) {
+      const Shape& sh = shapes[s];
+      if (!shouldQuery(sh, q.catLo, q.catHi, q.maskLo, q.maskHi) || !aabbOverlaps(sh.fat, segment)) continue;
+      const CastOutput o = rayCastShape(in, geoms[s], sims[sh.body].xf);
+      if (!o.hit || !(o.fraction > 0.0f)) continue;
+      // positive fractions order as their bit patterns
+      const unsigned long long k = ((unsigned long long)__float_as_uint(o.fraction) << 32) | (unsigned)s;
+      if (k < key) {
+        key = k;
+        mine = o;
+      }
+    }
+    atomicMin(&best, key);
+    __syncthreads();
+    if (best == ~0ull) {
+      if (lane == 0) counts[qi] = 0;
+    } else if (key == best) {
+      QueryHit r;
+      r.shape = (int)(key & 0xFFFFFFFFull);
+      r.px = mine.point.x;
+      r.py = mine.point.y;
+      r.nx = mine.normal.x;
+      r.ny = mine.normal.y;
+      r.fraction = mine.fraction;
+      if (cap > 0) out[0] = r;
+      counts[qi] = 1;
+    }
+    return;
+  }
+  if (q.kind != QK_RAY_ALL && q.kind != QK_OVERLAP_AABB) {
+    if (lane == 0) counts[qi] = 0;
+    return;
+  }
+  const AABB box = {{q.p[0], q.p[1]}, {q.p[2], q.p[3]}};
+  int total = 0;
+  for (int base = 0; base < ns; base += kQueryLanes) {
+    const int s = base + lane;
+    bool hit = false;
+    CastOutput o = noHit();
+    if (s < ns) {
+      const Shape& sh = shapes[s];
+      if (shouldQuery(sh, q.catLo, q.catHi, q.maskLo, q.maskHi)) {
+        if (q.kind == QK_RAY_ALL) {
+          if (aabbOverlaps(sh.fat, segment)) {
+            o = rayCastShape(in, geoms[s], sims[sh.body].xf);
+            hit = o.hit;
+          }
+        } else {
+          hit = aabbOverlaps(sh.fat, box);
+        }
+      }
+    }
+    const unsigned long long m = __ballot(hit);
+    if (hit) {
+      const int pos = total + __popcll(m & ((1ull << lane) - 1ull));
+      if (pos < cap) {
+        QueryHit r;
+        r.shape = s;
+        r.px = o.point.x;
+        r.py = o.point.y;
+        r.nx = o.normal.x;
+        r.ny = o.normal.y;
+        r.fraction = o.fraction;
+        out[pos] = r;
+      }
+    }
+    total += __popcll(m);
+  }
+  if (lane == 0) counts[qi] = total;
+}
+
+// b2World_Explode / ExplosionCallback, S/world.c:2640-2747: one explosion per
+// workgroup, at most one per world per launch. Lanes test 64 shapes at a time
+// (dynamic proxies whose category meets the mask and whose fat AABB meets the blast
+// box), compute the distance to the blast centre and the impulse; then thread 0
+// wakes and pushes their bodies in shape-id order, so a body with several shapes
+// sums its impulses in a fixed order.
+__global__ __launch_bounds__(kQueryLanes) void explodeKernel(BatchView b, const Query* qs) {
+  __shared__ int listBody[kQueryLanes];
+  __shared__ Vec2 listImpulse[kQueryLanes];
+  __shared__ Vec2 listPoint[kQueryLanes];
+  const Query q = qs[blockIdx.x];
+  if (q.kind != QK_EXPLODE || q.world < 0 || q.world >= b.worldCount) return;
+  WorldHeader* h = b.worlds + q.world;
+  WorldPtrs w = worldPtrs(b, h);
+  const Vec2 position = {q.p[0], q.p[1]};
+  const float radius = q.p[2], falloff = q.p[3], impulsePerLength = q.p[4];
+  AABB box;
+  box.lo.x = position.x - (radius + falloff);
+  box.lo.y = position.y - (radius + falloff);
+  box.hi.x = position.x + (radius + falloff);
+  box.hi.y = position.y + (radius + falloff);
+  const Xf identity = {{0.0f, 0.0f}, {1.0f, 0.0f}};
+  const int ns = h->shapeCount;
+  for (int base = 0; base < ns; base += kQueryLanes) {
+    const int s = base + threadIdx.x;
+    bool hit = false;
+    int body = -1;
+    Vec2 impulse = {0.0f, 0.0f}, closest = {0.0f, 0.0f};
+    if (s < ns) {
+      const Shape& sh = w.shapes[s];
+      if ((sh.proxyKey & 3) == kDynamic && ((sh.catLo & q.maskLo) | (sh.catHi & q.maskHi)) != 0u &&
+          aabbOverlaps(sh.fat, box)) {
+        body = sh.body;
+        const Xf xf = w.sims[body].xf;
+        const ShapeGeom& g = w.geoms[s];
+        Proxy pa = makeShapeProxy(g);
+        Proxy pb;
+        pb.count = 1;
+        pb.p[0] = position;
+        pb.radius = 0.0f;
+        SimplexCache cache;
+        cache.count = 0;
+        const DistanceOut d = shapeDistanceOut(cache, pa, pb, xf, identity, true);
+        if (!(d.distance > radius + falloff)) {
+          hit = true;
+          closest = d.pointA;
+          if (d.distance == 0.0f) closest = transformPoint(xf, shapeCentroid(g));
+          Vec2 direction = sub(closest, position);
+          if (lengthSq(direction) > 100.0f * kFltEpsilon * kFltEpsilon) {
+            direction = normalize(direction);
+          } else {
+            direction = {1.0f, 0.0f};
+          }
+          const Vec2 localLine = invRotate(xf.q, leftPerp(direction));
+          const float perimeter = projectedPerimeter(g, localLine);
+          float scale = 1.0f;
+          if (d.distance > radius && falloff > 0.0f) scale = clampf((radius + falloff - d.distance) / falloff, 0.0f, 1.0f);
+          const float magnitude = impulsePerLength * perimeter * scale;
+          impulse = mulSV(magnitude, direction);
+        }
+      }
+    }
+    const unsigned long long m = __ballot(hit);
+    if (hit) {
+      const int k = __popcll(m & ((1ull << threadIdx.x) - 1ull));
+      listBody[k] = body;
+      listImpulse[k] = impulse;
+      listPoint[k] = closest;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int count = __popcll(m);
+      for (int k = 0; k < count; ++k) {
+        const int bi = listBody[k];
+        if (w.sims[bi].setIndex >= kFirstSleepingSet && w.sims[bi].islandId >= 0) wakeIslandSet(w, w.sims[bi].islandId);
+        if (w.sims[bi].setIndex != kAwakeSet) continue;
+        const BodySim& sim = w.sims[bi];
+        BodyState& st = w.states[bi];
+        st.v = mulAdd(st.v, sim.invMass, listImpulse[k]);
+        st.w += sim.invI * cross(sub(listPoint[k], sim.center), listImpulse[k]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 template <int BLOCK>
 static int launchBlock(const BatchView* b, const StepParams* sp, const int* worldList, int nWorlds, hipStream_t stream,
                        const LaunchConfig* cfg) {
@@ -1717,6 +1923,19 @@ static int launchBlock(const BatchView* b, const StepParams* sp, const int* worl
 }
 
 }  // namespace f2d
+
+extern "C" int f2d_launch_queries(const f2d::BatchView* b, const f2d::Query* q, int n, f2d::QueryHit* hits,
+                                  int* counts, int cap, hipStream_t stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(f2d::queryKernel, dim3(n), dim3(f2d::kQueryLanes), 0, stream, *b, q, hits, counts, cap);
+  return (int)hipGetLastError();
+}
+
+extern "C" int f2d_launch_explosions(const f2d::BatchView* b, const f2d::Query* q, int n, hipStream_t stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(f2d::explodeKernel, dim3(n), dim3(f2d::kQueryLanes), 0, stream, *b, q);
+  return (int)hipGetLastError();
+}
 
 extern "C" int f2d_launch_island_command(const f2d::BatchView* b, int world, int op, int arg, hipStream_t stream) {
   hipLaunchKernelGGL(f2d::islandCommandKernel, dim3(1), dim3(64), 0, stream, *b, world, op, arg);

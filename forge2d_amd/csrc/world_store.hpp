@@ -94,6 +94,12 @@ struct Batch {
   // flat array lengths of the committed batch (bulk host mirror copies)
   long nBodies = 0, nShapes = 0, nContacts = 0, nJoints = 0, nColorWords = 0, nIsland = 0;
   LaunchConfig launch = {0, 0, 0, 0};
+  // query staging on the device (grown on demand, freed with the batch)
+  Query* dQueries = nullptr;
+  long dQueryCap = 0;
+  QueryHit* dHits = nullptr;
+  long dHitCap = 0;
+  int* dCounts = nullptr;
 };
 
 }  // namespace f2d

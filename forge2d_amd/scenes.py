@@ -110,10 +110,11 @@ class World:
         return len(self.bodies) - 1
 
     def add_shape(self, body, kind, data, npts=0, radius=0.0, friction=0.6, restitution=0.0, density=1.0,
-                  flags=SF_INVOKE_CONTACT | SF_UPDATE_MASS, group=0, rolling=0.0, tangent_speed=0.0):
+                  flags=SF_INVOKE_CONTACT | SF_UPDATE_MASS, group=0, rolling=0.0, tangent_speed=0.0,
+                  category=(1, 0), mask=(0xFFFFFFFF, 0xFFFFFFFF)):
         d = list(data) + [0.0] * (16 - len(data))
-        self.shapes.append((body, kind, friction, restitution, rolling, tangent_speed, density, 1, 0, 0xFFFFFFFF,
-                            0xFFFFFFFF, group, flags, npts, radius, d))
+        self.shapes.append((body, kind, friction, restitution, rolling, tangent_speed, density, category[0],
+                            category[1], mask[0], mask[1], group, flags, npts, radius, d))
         return len(self.shapes) - 1
 
     def add_box(self, body, hx, hy, cx=0.0, cy=0.0, angle=0.0, radius=0.0, **kw):

@@ -149,6 +149,32 @@ int f2d_world_get_contact_hit_events(uint32_t w, double* out);
 int f2d_world_get_body_event_count(uint32_t w);
 int f2d_world_get_body_events(uint32_t w, double* out);
 
+/* Queries, F/native/wasm/f2d_shim.c:1308-1367 (rays, AABB overlap) and :177-189
+ * (explosion). A device kernel tests the world's shapes against the ray or box (their
+ * fat AABBs are the reference's tree leaves); hit shapes, points, normals and
+ * fractions are the reference's bit for bit. The reference calls the host imports
+ * f2d_host_cast_ray / f2d_host_overlap (f2d_shim.c:27-31) during its tree walks; here
+ * the host registers them with f2d_set_query_callbacks and they are called after the
+ * device pass, tree by tree (static, kinematic, dynamic bodies' shapes) as the
+ * reference walks them: ray hits nearest first within a tree with the reference's
+ * clipping rules (a return of 0 ends the cast, -1 skips the hit, a fraction clips the
+ * ray for the hits after it), overlaps in shape-id order (false ends that tree).
+ * f2d_world_cast_ray_closest writes [index1, world_and_generation, px, py, nx, ny,
+ * fraction] and returns 1 on a hit. f2d_world_explode wakes and pushes the dynamic
+ * bodies in reach in shape-id order. */
+typedef float (*f2dHostCastRayFn)(int32_t shape_index1, uint32_t shape_wg, float px, float py, float nx, float ny,
+                                  float fraction);
+typedef int (*f2dHostOverlapFn)(int32_t shape_index1, uint32_t shape_wg);
+void f2d_set_query_callbacks(f2dHostCastRayFn cast_ray, f2dHostOverlapFn overlap);
+int f2d_world_cast_ray_closest(uint32_t w, float ox, float oy, float tx, float ty, uint32_t category_lo,
+                               uint32_t category_hi, uint32_t mask_lo, uint32_t mask_hi, double* out);
+void f2d_world_cast_ray(uint32_t w, float ox, float oy, float tx, float ty, uint32_t category_lo,
+                        uint32_t category_hi, uint32_t mask_lo, uint32_t mask_hi);
+void f2d_world_overlap_aabb(uint32_t w, float lx, float ly, float ux, float uy, uint32_t category_lo,
+                            uint32_t category_hi, uint32_t mask_lo, uint32_t mask_hi);
+void f2d_world_explode(uint32_t w, uint32_t mask_lo, uint32_t mask_hi, float x, float y, float radius,
+                       float falloff, float impulse_per_length);
+
 /* The batch handle behind a single f2d world (no counterpart in the reference):
  * lets the f2dg_* export/counter calls read a world created through f2d_*. */
 void* f2d_world_batch(uint32_t w);
@@ -225,6 +251,32 @@ int f2dg_contact_hit_events(void* batch, int world, double* out);
 int f2dg_body_events(void* batch, int world, double* out);
 /* [bodies, shapes, live contacts, touching, overflow, active colours, step, error bits] */
 int f2dg_world_counters(void* batch, int world, int32_t* out);
+
+/* World queries over the worlds of a batch, one wavefront per query (the reference
+ * queries one world per call). kind 1: closest ray hit (b2World_CastRayClosest,
+ * S/world.c:2277-2308; p = origin x/y, translation x/y); 2: every ray hit
+ * (b2World_CastRay with a callback returning 1, :2185-2254), shape-id order; 3: every
+ * shape whose fat AABB overlaps the box p = lower x/y, upper x/y (b2World_OverlapAABB,
+ * :2050-2105), shape-id order; 4: explosion (b2World_Explode, :2640-2747; p = position
+ * x/y, radius, falloff, impulse per length; mask_lo/hi = the explosion's mask). */
+typedef struct f2dQuery {
+  int32_t world;
+  int32_t kind;
+  uint32_t category_lo, category_hi, mask_lo, mask_hi;
+  float p[6];
+} f2dQuery;
+typedef struct f2dQueryHit {
+  int32_t shape; /* world-local shape id (index1 - 1) */
+  float px, py, nx, ny, fraction;
+} f2dQueryHit;
+/* Host arrays. Runs rays and overlaps in one launch, then the explosions in query
+ * order (one launch per round, each world at most once per round). counts[i] = the
+ * number of results of query i (0 for explosions, -1 for a world outside the batch);
+ * the first min(count, cap) are written to hits[i * cap ...]. cap >= 1. */
+int f2dg_query(void* batch, const f2dQuery* queries, int n, f2dQueryHit* hits, int32_t* counts, int cap);
+/* Device arrays (queries, hits, counts), kinds 1-3 only: one launch on the batch's stream,
+ * no synchronization. */
+int f2dg_query_device(void* batch, const void* queries, int n, void* hits, void* counts, int cap);
 
 #ifdef __cplusplus
 }

@@ -3497,3 +3497,449 @@ long b2o_pairs(void* h, int64_t* keys, long cap) {
   }
   return n;
 }
+
+/* ------------------------------------------------------------------ world queries (S/world.c:2050-2308, 2640-2747) */
+
+/* The tree walks of the reference visit every proxy whose fat AABB meets the query
+   (the leaves hold the fat AABBs); here each query tests every shape in shape-id
+   order. The hit of a ray against a shape does not depend on the walk, and the
+   closest hit is the one of smallest fraction (ties: lower shape id). */
+
+typedef struct { V2 normal, point; real fraction; int hit; } CastOut;  /* b2CastOutput, I/collision.h:75-90 */
+
+static CastOut cast_none(void) {
+  CastOut o;
+  memset(&o, 0, sizeof(o));
+  return o;
+}
+
+static V2 xfinvpt(XF t, V2 p) {  /* b2InvTransformPoint, I/math_functions.h:509-514 */
+  real vx = p.x - t.p.x, vy = p.y - t.p.y;
+  return v2(t.q.c * vx + t.q.s * vy, -t.q.s * vx + t.q.c * vy);
+}
+
+/* b2RayCastCircle, S/geometry.c:506-580 */
+static CastOut ray_circle(V2 origin, V2 translation, real maxFraction, V2 p, real r) {
+  CastOut o = cast_none();
+  V2 s = sub2(origin, p);
+  real rr = r * r;
+  real length;
+  V2 d = lennorm(&length, translation);
+  if (length == 0) {
+    if (lensq(s) < rr) {
+      o.point = origin;
+      o.hit = 1;
+    }
+    return o;
+  }
+  real t = -dot2(s, d);
+  V2 c = madd(s, t, d);
+  real cc = dot2(c, c);
+  if (cc > rr) return o;
+  real h = SQRT(rr - cc);
+  real fraction = t - h;
+  if (fraction < 0 || maxFraction * length < fraction) {
+    if (lensq(s) < rr) {
+      o.point = origin;
+      o.hit = 1;
+    }
+    return o;
+  }
+  V2 hp = madd(s, fraction, d);
+  o.fraction = fraction / length;
+  o.normal = norm2(hp);
+  o.point = madd(p, r, o.normal);
+  o.hit = 1;
+  return o;
+}
+
+/* b2RayCastCapsule, S/geometry.c:582-716 */
+static CastOut ray_capsule(V2 origin, V2 translation, real maxFraction, V2 v1, V2 v2_, real radius) {
+  CastOut o = cast_none();
+  real capLen;
+  V2 a = lennorm(&capLen, sub2(v2_, v1));
+  if (capLen < EPS) return ray_circle(origin, translation, maxFraction, v1, radius);
+  V2 q = sub2(origin, v1);
+  real qa = dot2(q, a);
+  V2 qp = madd(q, -qa, a);
+  if (dot2(qp, qp) < radius * radius) {
+    if (qa < 0) return ray_circle(origin, translation, maxFraction, v1, radius);
+    if (qa > capLen) return ray_circle(origin, translation, maxFraction, v2_, radius);
+    o.point = origin;
+    o.hit = 1;
+    return o;
+  }
+  V2 n = v2(a.y, -a.x);
+  real rayLen;
+  V2 u = lennorm(&rayLen, translation);
+  real den = -a.x * u.y + u.x * a.y;
+  if (-EPS < den && den < EPS) return o;
+  V2 b1 = msub(q, radius, n), b2 = madd(q, radius, n);
+  real invDen = 1 / den;
+  real s21 = (a.x * b1.y - b1.x * a.y) * invDen;
+  real s22 = (a.x * b2.y - b2.x * a.y) * invDen;
+  real s2;
+  V2 b;
+  if (s21 < s22) {
+    s2 = s21;
+    b = b1;
+  } else {
+    s2 = s22;
+    b = b2;
+    n = neg2(n);
+  }
+  if (s2 < 0 || maxFraction * rayLen < s2) return o;
+  real s1 = (-b.x * u.y + u.x * b.y) * invDen;
+  if (s1 < 0) return ray_circle(origin, translation, maxFraction, v1, radius);
+  if (capLen < s1) return ray_circle(origin, translation, maxFraction, v2_, radius);
+  o.fraction = s2 / rayLen;
+  o.point = add2(lerp2(v1, v2_, s1 / capLen), muls(radius, n));
+  o.normal = n;
+  o.hit = 1;
+  return o;
+}
+
+/* b2RayCastSegment (two-sided), S/geometry.c:718-797 */
+static CastOut ray_segment(V2 origin, V2 d, real maxFraction, V2 v1, V2 v2_) {
+  CastOut o = cast_none();
+  real length;
+  V2 eUnit = lennorm(&length, sub2(v2_, v1));
+  if (length == 0) return o;
+  V2 normal = rperp(eUnit);
+  real numerator = dot2(normal, sub2(v1, origin));
+  real denominator = dot2(normal, d);
+  if (denominator == 0) return o;
+  real t = numerator / denominator;
+  if (t < 0 || maxFraction < t) return o;
+  V2 p = madd(origin, t, d);
+  real s = dot2(sub2(p, v1), eUnit);
+  if (s < 0 || length < s) return o;
+  if (numerator > 0) normal = neg2(normal);
+  o.fraction = t;
+  o.point = p;
+  o.normal = normal;
+  o.hit = 1;
+  return o;
+}
+
+/* b2ShapeDistance with its witness points and normal, S/distance.c:421-606 */
+typedef struct { V2 pointA, pointB, normal; real distance; } DistOut;
+static DistOut distance_out(Cache* cache, const Proxy* pa, const Proxy* pb, XF xfA, XF xfB, int useRadii) {
+  DistOut out;
+  memset(&out, 0, sizeof(out));
+  Proxy lb;
+  XF t = xfinvmul(xfA, xfB);
+  lb.count = pb->count;
+  lb.radius = pb->radius;
+  for (int i = 0; i < lb.count; ++i) lb.p[i] = xfpt(t, pb->p[i]);
+  SV s[3];
+  memset(s, 0, sizeof(s));
+  int count = cache->count;
+  for (int i = 0; i < count; ++i) {
+    s[i].iA = cache->iA[i];
+    s[i].iB = cache->iB[i];
+    s[i].wA = pa->p[s[i].iA];
+    s[i].wB = lb.p[s[i].iB];
+    s[i].w = sub2(s[i].wA, s[i].wB);
+    s[i].a = -1;
+  }
+  if (count == 0) {
+    s[0].iA = s[0].iB = 0;
+    s[0].wA = pa->p[0];
+    s[0].wB = lb.p[0];
+    s[0].w = sub2(s[0].wA, s[0].wB);
+    s[0].a = 1;
+    count = 1;
+  }
+  V2 nonUnit = v2(0, 0);
+  int saveA[3], saveB[3], iteration = 0, overlap = 0;
+  while (iteration < 20) {
+    int saveCount = count;
+    for (int i = 0; i < saveCount; ++i) {
+      saveA[i] = s[i].iA;
+      saveB[i] = s[i].iB;
+    }
+    V2 d = count == 1 ? neg2(s[0].w) : (count == 2 ? simplex2(s, &count) : simplex3(s, &count));
+    if (count == 3 || dot2(d, d) < EPS * EPS) {
+      overlap = 1;
+      break;
+    }
+    nonUnit = d;
+    SV* v = s + count;
+    v->iA = support(pa, d);
+    v->wA = pa->p[v->iA];
+    v->iB = support(&lb, neg2(d));
+    v->wB = lb.p[v->iB];
+    v->w = sub2(v->wA, v->wB);
+    ++iteration;
+    int dup = 0;
+    for (int i = 0; i < saveCount; ++i) dup |= (v->iA == saveA[i] && v->iB == saveB[i]);
+    if (dup) break;
+    count += 1;
+  }
+  V2 a, b;
+  if (count == 1) {
+    a = s[0].wA;
+    b = s[0].wB;
+  } else if (count == 2) {
+    a = v2(s[0].a * s[0].wA.x + s[1].a * s[1].wA.x, s[0].a * s[0].wA.y + s[1].a * s[1].wA.y);
+    b = v2(s[0].a * s[0].wB.x + s[1].a * s[1].wB.x, s[0].a * s[0].wB.y + s[1].a * s[1].wB.y);
+  } else {
+    a = v2(s[0].a * s[0].wA.x + s[1].a * s[1].wA.x + s[2].a * s[2].wA.x,
+           s[0].a * s[0].wA.y + s[1].a * s[1].wA.y + s[2].a * s[2].wA.y);
+    b = a;
+  }
+  out.pointA = xfpt(xfA, a);
+  out.pointB = xfpt(xfA, b);
+  if (overlap) return out; /* distance 0, no normal, the cache kept */
+  out.normal = rot2(xfA.q, norm2(nonUnit));
+  real dx = b.x - a.x, dy = b.y - a.y;
+  out.distance = SQRT(dx * dx + dy * dy);
+  cache->count = count;
+  for (int i = 0; i < count; ++i) {
+    cache->iA[i] = s[i].iA;
+    cache->iB[i] = s[i].iB;
+  }
+  if (useRadii && out.distance > (real)0.1f * LINEAR_SLOP) {
+    out.distance = maxr(0, out.distance - pa->radius - pb->radius);
+    out.pointA = madd(out.pointA, pa->radius, out.normal);
+    out.pointB = msub(out.pointB, pb->radius, out.normal);
+  }
+  return out;
+}
+
+/* b2ShapeCast without encroachment, S/distance.c:608-700 */
+static CastOut shape_cast(const Proxy* pa, XF xfA, const Proxy* pb, XF xfB, V2 delta, real maxFraction) {
+  real target = maxr(LINEAR_SLOP, pa->radius + pb->radius - LINEAR_SLOP);
+  real tolerance = (real)0.25f * LINEAR_SLOP;
+  Cache cache;
+  memset(&cache, 0, sizeof(cache));
+  real fraction = 0;
+  XF xb = xfB;
+  CastOut o = cast_none();
+  for (int it = 0; it < 20; ++it) {
+    DistOut d = distance_out(&cache, pa, pb, xfA, xb, 0);
+    if (d.distance < target + tolerance) {
+      o.hit = 1;
+      if (it == 0) {
+        V2 c1 = madd(d.pointA, pa->radius, d.normal);
+        V2 c2 = madd(d.pointB, -pb->radius, d.normal);
+        o.point = lerp2(c1, c2, (real)0.5f);
+        return o;
+      }
+      o.fraction = fraction;
+      o.point = madd(d.pointA, pa->radius, d.normal);
+      o.normal = d.normal;
+      return o;
+    }
+    real den = dot2(delta, d.normal);
+    if (den >= 0) return o;
+    fraction += (target - d.distance) / den;
+    if (fraction >= maxFraction) return o;
+    xb.p = madd(xfB.p, fraction, delta);
+  }
+  return o;
+}
+
+/* b2RayCastPolygon, S/geometry.c:799-891 */
+static CastOut ray_polygon(V2 origin, V2 d, real maxFraction, const GEO* g) {
+  int count = g->count < 8 ? g->count : 8;
+  if (g->radius == 0) {
+    V2 base = g->v[0];
+    V2 p1 = sub2(origin, base);
+    real lower = 0, upper = maxFraction;
+    int index = -1;
+    CastOut o = cast_none();
+    for (int i = 0; i < count; ++i) {
+      V2 vertex = sub2(g->v[i], base);
+      real num = dot2(g->n[i], sub2(vertex, p1));
+      real den = dot2(g->n[i], d);
+      if (den == 0) {
+        if (num < 0) return o;
+      } else if (den < 0 && num < lower * den) {
+        lower = num / den;
+        index = i;
+      } else if (den > 0 && num < upper * den) {
+        upper = num / den;
+      }
+      if (upper < lower) return o;
+    }
+    if (index >= 0) {
+      o.fraction = lower;
+      o.normal = g->n[index];
+      o.point = madd(origin, lower, d);
+    } else {
+      o.point = origin;
+    }
+    o.hit = 1;
+    return o;
+  }
+  Proxy pa, pb;
+  memset(&pa, 0, sizeof(pa));
+  memset(&pb, 0, sizeof(pb));
+  pa.count = count;
+  for (int i = 0; i < count; ++i) pa.p[i] = g->v[i];
+  pa.radius = g->radius;
+  pb.count = 1;
+  pb.p[0] = origin;
+  XF id;
+  id.p = v2(0, 0);
+  id.q.c = 1;
+  id.q.s = 0;
+  return shape_cast(&pa, id, &pb, id, d, maxFraction);
+}
+
+/* b2RayCastShape, S/shape.c:792-820 */
+static CastOut ray_shape(V2 origin, V2 translation, const GEO* g, XF xf) {
+  V2 lo = xfinvpt(xf, origin);
+  V2 ld = invrot2(xf.q, translation);
+  CastOut o;
+  if (g->type == SHAPE_CAPSULE) {
+    o = ray_capsule(lo, ld, 1, g->v[0], g->v[1], g->radius);
+  } else if (g->type == SHAPE_CIRCLE) {
+    o = ray_circle(lo, ld, 1, g->v[0], g->radius);
+  } else if (g->type == SHAPE_POLYGON) {
+    o = ray_polygon(lo, ld, 1, g);
+  } else {
+    o = ray_segment(lo, ld, 1, g->v[0], g->v[1]);
+  }
+  o.point = xfpt(xf, o.point);
+  o.normal = rot2(xf.q, o.normal);
+  return o;
+}
+
+/* b2GetShapeProjectedPerimeter, S/shape.c:656-700 */
+static real projected_perimeter(const GEO* g, V2 line) {
+  if (g->type == SHAPE_CAPSULE) return absr(dot2(sub2(g->v[1], g->v[0]), line)) + 2 * g->radius;
+  if (g->type == SHAPE_CIRCLE) return 2 * g->radius;
+  if (g->type == SHAPE_POLYGON) {
+    int count = g->count < 8 ? g->count : 8;
+    real value = dot2(g->v[0], line), lower = value, upper = value;
+    for (int i = 1; i < count; ++i) {
+      value = dot2(g->v[i], line);
+      lower = minr(lower, value);
+      upper = maxr(upper, value);
+    }
+    return (upper - lower) + 2 * g->radius;
+  }
+  return absr(dot2(g->v[1], line) - dot2(g->v[0], line));
+}
+
+/* ExplosionCallback / b2World_Explode, S/world.c:2640-2747, shapes in shape-id order */
+static void explode(OWorld* w, uint64_t mask, V2 position, real radius, real falloff, real impulsePerLength) {
+  AB box;
+  box.lo = v2(position.x - (radius + falloff), position.y - (radius + falloff));
+  box.hi = v2(position.x + (radius + falloff), position.y + (radius + falloff));
+  XF id;
+  id.p = v2(0, 0);
+  id.q.c = 1;
+  id.q.s = 0;
+  for (int s = 0; s < w->shapeCount; ++s) {
+    OShape* sh = w->shapes + s;
+    if (sh->proxyKey < 0 || (sh->proxyKey & 3) != DYNAMIC_BODY || (sh->cat & mask) == 0) continue;
+    if (!aabb_overlaps(sh->fat, box)) continue;
+    OBody* body = w->bodies + sh->body;
+    XF xf = body->xf;
+    Proxy pa = make_proxy(&sh->g), pb;
+    memset(&pb, 0, sizeof(pb));
+    pb.count = 1;
+    pb.p[0] = position;
+    Cache cache;
+    memset(&cache, 0, sizeof(cache));
+    DistOut d = distance_out(&cache, &pa, &pb, xf, id, 1);
+    if (d.distance > radius + falloff) continue;
+    if (body->setIndex >= FIRST_SLEEPING_SET) wake_island_set(w, body->setIndex);
+    if (body->setIndex != AWAKE_SET) continue;
+    V2 closest = d.pointA;
+    if (d.distance == 0) closest = xfpt(xf, shape_centroid(&sh->g));
+    V2 dir = sub2(closest, position);
+    if (lensq(dir) > (real)(100.0f * FLT_EPSILON * FLT_EPSILON)) {
+      dir = norm2(dir);
+    } else {
+      dir = v2(1, 0);
+    }
+    V2 localLine = invrot2(xf.q, lperp(dir));
+    real perimeter = projected_perimeter(&sh->g, localLine);
+    real scale = 1;
+    if (d.distance > radius && falloff > 0) scale = clampr((radius + falloff - d.distance) / falloff, 0, 1);
+    real magnitude = impulsePerLength * perimeter * scale;
+    V2 impulse = muls(magnitude, dir);
+    body->v = madd(body->v, body->invMass, impulse);
+    body->w += body->invI * cross2(sub2(closest, body->center), impulse);
+  }
+}
+
+int b2o_query(void* h, const B2OQuery* qs, int n, B2OHit* hits, int32_t* counts, int cap) {
+  OWorld* w = (OWorld*)h;
+  for (int qi = 0; qi < n; ++qi) {
+    const B2OQuery* q = qs + qi;
+    uint64_t qcat = (uint64_t)q->catLo | ((uint64_t)q->catHi << 32);
+    uint64_t qmask = (uint64_t)q->maskLo | ((uint64_t)q->maskHi << 32);
+    B2OHit* out = hits + (long)qi * cap;
+    int total = 0;
+    if (q->kind == 4) {
+      explode(w, qmask, v2(q->p[0], q->p[1]), q->p[2], q->p[3], q->p[4]);
+      counts[qi] = 0;
+      continue;
+    }
+    V2 origin = v2(q->p[0], q->p[1]), d = v2(q->p[2], q->p[3]);
+    V2 p2 = madd(origin, 1, d);
+    AB seg;
+    seg.lo = v2(minr(origin.x, p2.x), minr(origin.y, p2.y));
+    seg.hi = v2(maxr(origin.x, p2.x), maxr(origin.y, p2.y));
+    AB box;
+    box.lo = v2(q->p[0], q->p[1]);
+    box.hi = v2(q->p[2], q->p[3]);
+    int best = -1;
+    CastOut bestOut = cast_none();
+    for (int s = 0; s < w->shapeCount; ++s) {
+      const OShape* sh = w->shapes + s;
+      if (sh->proxyKey < 0) continue;
+      /* b2ShouldQueryCollide, S/shape.h:132-135 */
+      if ((sh->cat & qmask) == 0 || (sh->mask & qcat) == 0) continue;
+      if (q->kind == 3) {
+        if (!aabb_overlaps(sh->fat, box)) continue;
+        if (total < cap) {
+          memset(out + total, 0, sizeof(B2OHit));
+          out[total].shape = s;
+        }
+        total++;
+        continue;
+      }
+      if (!aabb_overlaps(sh->fat, seg)) continue;
+      CastOut o = ray_shape(origin, d, &sh->g, w->bodies[sh->body].xf);
+      if (!o.hit) continue;
+      if (q->kind == 1) {
+        if (!(o.fraction > 0)) continue; /* b2RayCastClosestFcn ignores initial overlaps */
+        if (best < 0 || o.fraction < bestOut.fraction) {
+          best = s;
+          bestOut = o;
+        }
+        continue;
+      }
+      if (total < cap) {
+        B2OHit* r = out + total;
+        r->shape = s;
+        r->px = (float)o.point.x;
+        r->py = (float)o.point.y;
+        r->nx = (float)o.normal.x;
+        r->ny = (float)o.normal.y;
+        r->fraction = (float)o.fraction;
+      }
+      total++;
+    }
+    if (q->kind == 1 && best >= 0) {
+      if (cap > 0) {
+        out[0].shape = best;
+        out[0].px = (float)bestOut.point.x;
+        out[0].py = (float)bestOut.point.y;
+        out[0].nx = (float)bestOut.normal.x;
+        out[0].ny = (float)bestOut.normal.y;
+        out[0].fraction = (float)bestOut.fraction;
+      }
+      total = 1;
+    }
+    counts[qi] = total;
+  }
+  return 0;
+}

@@ -73,6 +73,21 @@ int b2o_bodies(void* w, double* out);
  * accept (S/broad_phase.c:160-302). Returns the count, writes up to cap. */
 long b2o_pairs(void* w, int64_t* keys, long cap);
 
+/* World queries (the f2dQuery / f2dQueryHit records of include/forge2d_amd.h):
+ * kind 1 closest ray hit, 2 every ray hit, 3 fat-AABB overlaps (shape-id order),
+ * 4 explosion (applied to the world). counts[i] = results of query i, the first cap
+ * written to hits[i * cap ...]. */
+typedef struct {
+  int32_t world, kind;
+  uint32_t catLo, catHi, maskLo, maskHi;
+  float p[6];
+} B2OQuery;
+typedef struct {
+  int32_t shape;
+  float px, py, nx, ny, fraction;
+} B2OHit;
+int b2o_query(void* w, const B2OQuery* queries, int n, B2OHit* hits, int32_t* counts, int cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,7 @@ module, as the checker. It loads oracle/_build/liboracle_{f32,f64}.so, built by
     o.collide()                     # manifolds + contact state loop + colouring
     o.solve(dt, substeps)           # solver stages, finalize, continuous, moves
     o.export() -> snapshot bytes;  o.bodies() -> (n, 7) float64
+    o.query(queries) -> (counts, hits)   # ray casts, AABB overlaps, explosions
 """
 from __future__ import annotations
 
@@ -48,6 +49,8 @@ def _lib(precision: str):
         L.b2o_bodies.argtypes = [vp, vp]
         L.b2o_pairs.restype = L_
         L.b2o_pairs.argtypes = [vp, vp, L_]
+        L.b2o_query.restype = ctypes.c_int
+        L.b2o_query.argtypes = [vp, vp, ctypes.c_int, vp, vp, ctypes.c_int]
         _libs[precision] = L
     return _libs[precision]
 
@@ -113,6 +116,18 @@ class Oracle:
         from forge2d_amd import snapshot as S
 
         return S.parse_one(self.export()).bodies
+
+    def query(self, queries: np.ndarray, cap: int = 64):
+        """f2dQuery records (forge2d_amd.make_queries) -> (counts, hits[n, cap]); explosions
+        are applied to the oracle's world."""
+        from forge2d_amd import HIT_DTYPE, QUERY_DTYPE
+
+        queries = np.ascontiguousarray(queries, QUERY_DTYPE)
+        n = len(queries)
+        hits = np.zeros((max(n, 1), cap), HIT_DTYPE)
+        counts = np.zeros(max(n, 1), np.int32)
+        self.L.b2o_query(self.h, queries.ctypes.data, n, hits.ctypes.data, counts.ctypes.data, cap)
+        return counts[:n], hits[:n]
 
     def pairs(self) -> np.ndarray:
         n = self.L.b2o_pairs(self.h, None, 0)

@@ -17,6 +17,7 @@ Usage: python tests/golden/make_golden.py [name ...]
 """
 from __future__ import annotations
 
+import math
 import os
 import subprocess
 import sys
@@ -169,6 +170,140 @@ def sleep_world():
     return w
 
 
+def query_world():
+    """World queries (S/world.c:2050-2308, 2640-2747): every shape type on static,
+    kinematic and dynamic bodies, rounded boxes (ray cast by shape cast), a body with
+    two shapes, and shape categories 1, 2, 4 and one in the high word (some masks
+    leave category 2 out), sleep off so that explosions push awake bodies only."""
+    rng = scenes.SplitMix64(4242)
+    w = scenes.World(scenes.world_header(sleep=False, continuous=True))
+    g = w.add_body(scenes.STATIC, 0.0, -1.0)
+    w.add_box(g, 30.0, 1.0)
+    walls = w.add_body(scenes.STATIC, 0.0, 0.0)
+    w.add_segment(walls, (-30.0, 0.0), (-30.0, 20.0))
+    w.add_segment(walls, (30.0, 0.0), (30.0, 20.0), category=(2, 0))
+    rb = w.add_body(scenes.STATIC, 12.0, 4.0, angle=0.3)
+    w.add_box(rb, 1.0, 0.5, radius=0.2, category=(4, 0))
+    sc = w.add_body(scenes.STATIC, -12.0, 6.0)
+    w.add_circle(sc, 1.0, category=(2, 0))
+    k1 = w.add_body(scenes.KINEMATIC, 0.0, 12.0, vx=1.0, w=0.5)
+    w.add_capsule(k1, (-1.0, 0.0), (1.0, 0.0), 0.3, category=(2, 0))
+    k2 = w.add_body(scenes.KINEMATIC, -5.0, 14.0, w=1.0)
+    w.add_box(k2, 0.8, 0.4)
+    cats = [(1, 0), (2, 0), (4, 0), (0, 1)]
+    for i in range(36):
+        b = w.add_body(scenes.DYNAMIC, rng.uniform(-9.0, 9.0), rng.uniform(1.0, 10.0),
+                       angle=rng.uniform(-3.0, 3.0))
+        _ = scenes._random_shape(w, rng, b)
+        shape = list(w.shapes[-1])
+        shape[7], shape[8] = cats[i % 4]
+        if i % 5 == 0:
+            shape[9] = 0xFFFFFFFD  # mask without category 2
+        w.shapes[-1] = tuple(shape)
+    for i in range(3):
+        b = w.add_body(scenes.DYNAMIC, -6.0 + 6.0 * i, 16.0, angle=0.2 * i)
+        w.add_box(b, 0.5, 0.3, radius=0.1)
+    two = w.add_body(scenes.DYNAMIC, 24.0, 3.0)
+    w.add_box(two, 0.5, 0.5)
+    w.add_circle(two, 0.4, cx=0.8, cy=0.0)
+    for r in range(4):
+        n = 4 - r
+        for i in range(n):
+            b = w.add_body(scenes.DYNAMIC, -20.0 + (i - (n - 1) / 2.0), 0.5 + r)
+            w.add_box(b, 0.5, 0.5)
+    return w
+
+
+def random_queries(snap, rng, explosions, n_rays=160, n_boxes=40):
+    """Seeded rays (some from inside bodies, some of zero length), AABB boxes and the
+    given explosions, with query filters over categories 1, 2, 4 and the high word."""
+    from forge2d_amd import make_queries, Q_RAY_CLOSEST, Q_RAY_ALL, Q_OVERLAP_AABB, Q_EXPLODE
+
+    filters = [(1, 0, 0xFFFFFFFF, 0xFFFFFFFF)] * 7 + [(1, 0, 2, 0), (1, 0, 4, 1), (2, 0, 0xFFFFFFFF, 0xFFFFFFFF)]
+    pos = np.stack([snap.bodies["px"], snap.bodies["py"]], 1)
+    rows = []
+    for i in range(n_rays):
+        if i % 8 == 0:
+            o = pos[rng.randint(0, len(pos) - 1)]
+            ox, oy = float(o[0]), float(o[1])
+        else:
+            ox, oy = rng.uniform(-32.0, 32.0), rng.uniform(-3.0, 22.0)
+        a = rng.uniform(-math.pi, math.pi)
+        ln = 0.0 if i % 23 == 0 else rng.uniform(0.5, 40.0)
+        tx, ty = ln * math.cos(a), ln * math.sin(a)
+        kind = Q_RAY_CLOSEST if i % 2 == 0 else Q_RAY_ALL
+        rows.append((0, kind, [ox, oy, tx, ty], filters[rng.randint(0, len(filters) - 1)]))
+    for i in range(n_boxes):
+        x, y = rng.uniform(-32.0, 32.0), rng.uniform(-3.0, 22.0)
+        rows.append((0, Q_OVERLAP_AABB, [x, y, x + rng.uniform(0.1, 12.0), y + rng.uniform(0.1, 8.0)],
+                     filters[rng.randint(0, len(filters) - 1)]))
+    for e in explosions:
+        rows.append((0, Q_EXPLODE, e[:5], (0, 0, e[5], e[6])))
+    return make_queries(rows)
+
+
+QUERY_FIXTURES = {
+    # name: (world factory, steps before the queries, query seed, explosions
+    #        (x, y, radius, falloff, impulse per length, mask lo, mask hi))
+    "queries": (query_world, 90, 7,
+                [(0.0, 1.0, 3.0, 0.0, 2.0, 0xFFFFFFFF, 0xFFFFFFFF), (-4.0, 0.5, 2.0, 2.0, 1.5, 0xFFFFFFFF, 0xFFFFFFFF),
+                 (5.0, 1.0, 0.0, 3.0, 1.0, 0xFFFFFFFF, 0xFFFFFFFF), (0.0, 2.0, 6.0, 1.0, 0.7, 0xFFFFFFFD, 0xFFFFFFFF),
+                 (-20.0, 2.0, 1.5, 1.0, 3.0, 0xFFFFFFFF, 0xFFFFFFFF)]),
+    # after 200 steps the right pyramid sleeps as one island: the explosion wakes it
+    "queries_sleep": (sleep_world, 200, 8, [(10.0, 1.0, 1.5, 1.0, 2.0, 0xFFFFFFFF, 0xFFFFFFFF)]),
+    # two boxes sleep apart (two islands): one explosion wakes both, in the reference's
+    # tree order (the restatements wake them in shape-id order)
+    "queries_wake2": (sleep_world, 200, 9, [(-0.3, 0.5, 0.6, 0.5, 1.0, 0xFFFFFFFF, 0xFFFFFFFF)]),
+}
+
+
+def make_queries_fixture(name: str):
+    """Query fixture: the reference's results and its snapshots before and after."""
+    from forge2d_amd import snapshot as S
+
+    factory, steps, seed, explosions = QUERY_FIXTURES[name]
+    spec = scenes.to_bytes([factory()])
+    with tempfile.TemporaryDirectory() as td:
+        sp, dp = os.path.join(td, "scene.bin"), os.path.join(td, "d.bin")
+        with open(sp, "wb") as f:
+            f.write(spec)
+        subprocess.run([HARNESS, "dump", sp, dp, "0", str(steps), str(steps)], check=True)
+        with open(dp, "rb") as f:
+            snap = S.parse_one(f.read())
+        queries = random_queries(snap, scenes.SplitMix64(seed), explosions)
+        qp, rp = os.path.join(td, "q.bin"), os.path.join(td, "r.bin")
+        with open(qp, "wb") as f:
+            f.write(queries.tobytes())
+        subprocess.run([HARNESS, "query", sp, qp, rp, "0", str(steps)], check=True)
+        with open(rp, "rb") as f:
+            results = f.read()
+        with open(rp + ".snap", "rb") as f:
+            snaps = f.read()
+        gp = os.path.join(td, "geoms.bin")
+        subprocess.run([HARNESS, "geom", sp, gp, "0"], check=True)
+        with open(gp, "rb") as f:
+            geoms = f.read()
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, spec=np.frombuffer(spec, np.uint8), snaps=np.frombuffer(snaps, np.uint8),
+                        steps=np.array([steps], np.int32), geoms=np.frombuffer(geoms, np.uint8),
+                        queries=np.frombuffer(queries.tobytes(), np.uint8), results=np.frombuffer(results, np.uint8))
+    print(f"{name}: {len(queries)} queries, {len(results)} B results -> {os.path.getsize(path)} B")
+
+
+def parse_results(buf: bytes, n: int):
+    """The harness's query results: per query an int32 count and that many 24-byte hits."""
+    from forge2d_amd import HIT_DTYPE
+
+    out, off = [], 0
+    for _ in range(n):
+        c = int(np.frombuffer(buf, np.int32, 1, off)[0])
+        off += 4
+        out.append(np.frombuffer(buf, HIT_DTYPE, c, off).copy())
+        off += 24 * c
+    assert off == len(buf)
+    return out
+
+
 EVENT_STEPS = list(range(1, 121))
 # fixtures that also record the reference's contact and body events per step
 EVENT_FIXTURES = {"events": EVENT_STEPS, "sleep": list(range(1, 261))}
@@ -225,5 +360,8 @@ def make(name: str):
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
-    for n in sys.argv[1:] or FIXTURES:
-        make(n)
+    for n in sys.argv[1:] or list(FIXTURES) + list(QUERY_FIXTURES):
+        if n in QUERY_FIXTURES:
+            make_queries_fixture(n)
+        else:
+            make(n)
