@@ -26,8 +26,8 @@
 extern "C" int f2d_launch_step(const f2d::BatchView* b, const f2d::StepParams* sp, const int* worldList, int nWorlds,
                                hipStream_t stream, const f2d::LaunchConfig* cfg);
 extern "C" int f2d_launch_island_command(const f2d::BatchView* b, int world, int op, int arg, hipStream_t stream);
-extern "C" int f2d_launch_queries(const f2d::BatchView* b, const f2d::Query* q, int n, f2d::QueryHit* hits,
-                                  int* counts, int cap, hipStream_t stream);
+extern "C" int f2d_launch_queries(const f2d::BatchView* b, float4* boxes, long nShapes, const f2d::Query* q, int n,
+                                  f2d::QueryHit* hits, int* counts, int cap, hipStream_t stream);
 extern "C" int f2d_launch_explosions(const f2d::BatchView* b, const f2d::Query* q, int n, hipStream_t stream);
 
 namespace f2d {
@@ -561,6 +561,8 @@ static void freeDevice(Batch& b) {
   if (b.dQueries) (void)hipFree(b.dQueries);
   if (b.dHits) (void)hipFree(b.dHits);
   if (b.dCounts) (void)hipFree(b.dCounts);
+  if (b.dBoxes) (void)hipFree(b.dBoxes);
+  b.dBoxes = nullptr;
   b.dQueries = nullptr;
   b.dHits = nullptr;
   b.dCounts = nullptr;
@@ -728,7 +730,8 @@ static int commit(Batch& b) {
       int cc = collideBytes(cnp);
       c.launch.collideLds = std::max(cc, 12 * 256);
       c.launch.solveLds = solveLdsBytes(c.maxBodies, c.maxJoints, 1.0f, kLdsLimit);
-      c.launch.block = c.limit <= 128 ? 64 : (c.limit <= 512 ? 128 : 256);
+      // one workgroup holds a whole world: the largest worlds get the most lanes
+      c.launch.block = c.limit <= 128 ? 64 : (c.limit <= 512 ? 128 : (c.limit <= 1024 ? 256 : 1024));
       F2D_HIP(hipMalloc(&c.dList, sizeof(int) * c.count));
       F2D_HIP(hipMemcpy(c.dList, members[k].data(), sizeof(int) * c.count, hipMemcpyHostToDevice));
       F2D_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -1988,7 +1991,7 @@ int f2dg_autotune(void* h, float dt, int subSteps, int steps, float* msOut) {
   Batch* b = static_cast<Batch*>(h);
   if (!b) return -1;
   if (prepareForStep(*b) != 0) return -1;
-  const int cands[3] = {64, 128, 256};
+  const int cands[5] = {64, 128, 256, 512, 1024};
   StepParams sp = makeParams(dt, subSteps, PH_ALL);
   hipEvent_t e0, e1;
   F2D_HIP(hipEventCreate(&e0));
@@ -1997,7 +2000,9 @@ int f2dg_autotune(void* h, float dt, int subSteps, int steps, float* msOut) {
   for (SizeClass& c : b->classes) {
     float best = 1e30f;
     int bestBlock = c.launch.block;
-    for (int k = 0; k < 3; ++k) {
+    // 512 and 1024 lanes only for worlds of more than 256 shapes or bodies
+    const int nCands = c.limit > 256 ? 5 : 3;
+    for (int k = 0; k < nCands; ++k) {
       LaunchConfig cfg = c.launch;
       cfg.block = cands[k];
       F2D_HIP(hipEventRecord(e0, b->stream));
@@ -2008,7 +2013,7 @@ int f2dg_autotune(void* h, float dt, int subSteps, int steps, float* msOut) {
       F2D_HIP(hipEventSynchronize(e1));
       float ms = 0.0f;
       F2D_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (msOut) msOut[k] += ms;
+      if (msOut && k < 3) msOut[k] += ms;
       if (ms < best) {
         best = ms;
         bestBlock = cands[k];
@@ -2080,7 +2085,7 @@ int f2dg_size_classes(void* h, int* out, int cap) {
 
 int f2dg_set_block(void* h, int block) {
   Batch* b = static_cast<Batch*>(h);
-  if (!b || (block != 64 && block != 128 && block != 256)) return -1;
+  if (!b || (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)) return -1;
   b->launch.block = block;
   for (SizeClass& c : b->classes) c.launch.block = block;
   return 0;
@@ -2089,7 +2094,7 @@ int f2dg_set_block(void* h, int block) {
 // Force the workgroup size of one size class (classes as f2dg_size_classes lists them).
 int f2dg_set_class_block(void* h, int cls, int block) {
   Batch* b = static_cast<Batch*>(h);
-  if (!b || (block != 64 && block != 128 && block != 256)) return -1;
+  if (!b || (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)) return -1;
   if (cls < 0 || cls >= (int)b->classes.size()) return -1;
   b->classes[cls].launch.block = block;
   return 0;
@@ -2641,6 +2646,7 @@ static int ensureQueryBuffers(Batch& b, long n, long hitCount) {
     F2D_HIP(hipMalloc(&b.dCounts, sizeof(int) * c));
     b.dQueryCap = c;
   }
+  if (!b.dBoxes && b.nShapes > 0) F2D_HIP(hipMalloc(&b.dBoxes, sizeof(float4) * b.nShapes));
   if (hitCount > b.dHitCap) {
     if (b.dHits) (void)hipFree(b.dHits);
     b.dHits = nullptr;
@@ -2673,7 +2679,8 @@ static int runQueries(Batch& b, const Query* q, int n, QueryHit* hits, int* coun
   for (const auto& r : rounds) explosions += (long)r.size();
   if (ensureQueryBuffers(b, n + explosions, (long)n * cap) != 0) return -1;
   F2D_HIP(hipMemcpyAsync(b.dQueries, q, sizeof(Query) * n, hipMemcpyHostToDevice, b.stream));
-  if (f2d_launch_queries(&b.dev.view, b.dQueries, n, b.dHits, b.dCounts, cap, b.stream) != 0) return -1;
+  if (f2d_launch_queries(&b.dev.view, b.dBoxes, b.nShapes, b.dQueries, n, b.dHits, b.dCounts, cap, b.stream) != 0)
+    return -1;
   long off = n;
   for (const auto& r : rounds) {
     F2D_HIP(hipMemcpyAsync(b.dQueries + off, r.data(), sizeof(Query) * r.size(), hipMemcpyHostToDevice, b.stream));
@@ -2723,8 +2730,9 @@ int f2dg_query_device(void* h, const void* queries, int n, void* hits, void* cou
   if (b->hostDirty || !b->committed) {
     if (upload(*b) != 0) return -1;
   }
-  return f2d_launch_queries(&b->dev.view, static_cast<const Query*>(queries), n, static_cast<QueryHit*>(hits),
-                            static_cast<int*>(counts), cap, b->stream);
+  if (ensureQueryBuffers(*b, 0, 0) != 0) return -1;
+  return f2d_launch_queries(&b->dev.view, b->dBoxes, b->nShapes, static_cast<const Query*>(queries), n,
+                            static_cast<QueryHit*>(hits), static_cast<int*>(counts), cap, b->stream);
 }
 
 void f2d_set_query_callbacks(f2dHostCastRayFn cast_ray, f2dHostOverlapFn overlap) {

@@ -29,7 +29,7 @@
 
 namespace f2d {
 
-constexpr int kMaxWaves = 4;  // largest workgroup: 256 lanes
+constexpr int kMaxWaves = 16;  // largest workgroup: 1024 lanes
 
 struct SharedState {
   int colorCount[kGraphColorCount];
@@ -1516,25 +1516,29 @@ __device__ float shapeMinExtent(const ShapeGeom& g) {
 }
 
 // b2ContinuousQueryCallback, S/solver.c:212-388, for candidate shape t of a fast
-// shape s on body i. Returns the (possibly reduced) sweep fraction.
-__device__ float continuousCandidate(const WorldPtrs& w, int i, int s, const Proxy& proxyB, const Sweep& sweep,
-                                     int t, float fraction) {
+// shape s on body i. Returns the (possibly reduced) sweep fraction. Not inlined:
+// the time-of-impact proxies (dynamically indexed arrays) live in this call's frame,
+// so only lanes that reach a candidate touch them.
+__device__ __noinline__ float continuousCandidate(const Shape* shapes, const ShapeGeom* geoms, const BodySim* sims,
+                                                  const Joint* joints, int i, int s, Sweep sweep, int t,
+                                                  float fraction) {
   if (t == s) return fraction;
-  const Shape& fs = w.shapes[s];
-  const Shape& ts = w.shapes[t];
+  const Shape& fs = shapes[s];
+  const Shape& ts = shapes[t];
   if (ts.body == i) return fraction;
   if (ts.flags & SF_SENSOR) return fraction;
   if (!shouldShapesCollide(fs, ts)) return fraction;
-  const BodySim& other = w.sims[ts.body];
+  const BodySim& other = sims[ts.body];
   if (other.flags & BF_BULLET) return fraction;
-  if (!shouldBodiesCollide(w.sims, w.joints, i, ts.body)) return fraction;
-  Proxy proxyA = makeShapeProxy(w.geoms[t]);
+  if (!shouldBodiesCollide(sims, joints, i, ts.body)) return fraction;
+  Proxy proxyB = makeShapeProxy(geoms[s]);
+  Proxy proxyA = makeShapeProxy(geoms[t]);
   Sweep sweepA = makeSweep(other);
   float f = timeOfImpact(proxyA, sweepA, proxyB, sweep, fraction);
   if (0.0f < f && f < fraction) return f;
   if (f == 0.0f) {
     // fallback: a small circle around the fast shape's centroid
-    const ShapeGeom& g = w.geoms[s];
+    const ShapeGeom& g = geoms[s];
     Proxy core;
     core.count = 1;
     core.p[0] = shapeCentroid(g);
@@ -1569,17 +1573,17 @@ __device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList
     box.hi = vmax(box1.hi, box2.hi);
     fs.aabb = box2;
     if (fs.flags & SF_SENSOR) continue;
-    Proxy proxyB = makeShapeProxy(w.geoms[s]);
     for (int k = 0; k < staticCount; ++k) {
       int t = staticList[k];
-      if (queryVisible(w.shapes[t], box)) fraction = continuousCandidate(w, i, s, proxyB, sweep, t, fraction);
+      if (queryVisible(w.shapes[t], box))
+        fraction = continuousCandidate(w.shapes, w.geoms, w.sims, w.joints, i, s, sweep, t, fraction);
     }
     if (bullet) {
       for (int type = kKinematic; type <= kDynamic; ++type) {
         for (int t = 0; t < ns; ++t) {
           const Shape& ts = w.shapes[t];
           if ((ts.proxyKey & 3) != type || !queryVisible(ts, box)) continue;
-          fraction = continuousCandidate(w, i, s, proxyB, sweep, t, fraction);
+          fraction = continuousCandidate(w.shapes, w.geoms, w.sims, w.joints, i, s, sweep, t, fraction);
         }
       }
     }
@@ -1705,8 +1709,8 @@ constexpr int kQueryLanes = 64;  // one wavefront per query
 
 // The box of the ray segment at maxFraction 1 (segmentAABB of b2DynamicTree_RayCast,
 // S/dynamic_tree.c:1172-1200). A shape's exact hit lies on the shape, inside its fat
-// AABB by at least the speculative margin, and on the segment, so the shapes this box
-// culls are exactly those whose ray casts cannot hit.
+// AABB by at least the speculative margin, and on the segment, so the shapes the
+// tree's leaf test culls are exactly those whose ray casts cannot hit.
 __device__ __forceinline__ AABB rayBox(Vec2 p1, Vec2 d) {
   Vec2 p2 = mulAdd(p1, 1.0f, d);
   return {vmin(p1, p2), vmax(p1, p2)};
@@ -1724,8 +1728,48 @@ __device__ __forceinline__ AABB rayBox(Vec2 p1, Vec2 d) {
 //                   AABB overlaps the box, in shape-id order
 // counts[q] receives the number of results (which may exceed cap: only the first cap
 // are written to hits[q * cap ...]), or -1 for a world index outside the batch.
-__global__ __launch_bounds__(kQueryLanes) void queryKernel(BatchView b, const Query* qs, QueryHit* hits, int* counts,
-                                                           int cap) {
+// The shapes' fat AABBs packed 16 B per shape in batch shape order (one pass before
+// the queries): the scan reads one float4 per shape instead of a line of its
+// 96-byte record, and the record only for shapes whose box passes.
+__global__ void packBoxesKernel(const Shape* shapes, float4* boxes, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const AABB f = shapes[i].fat;
+  boxes[i] = make_float4(f.lo.x, f.lo.y, f.hi.x, f.hi.y);
+}
+
+__device__ __forceinline__ bool boxOverlaps(float4 f, AABB b) {
+  AABB a = {{f.x, f.y}, {f.z, f.w}};
+  return aabbOverlaps(a, b);
+}
+
+// The leaf test of b2DynamicTree_RayCast (S/dynamic_tree.c:1172-1245): the segment's
+// box overlaps the fat AABB and the box is not separated from the ray's line,
+// |dot(v, p1 - c)| <= dot(|v|, h) with v perpendicular to the ray
+struct RaySat {
+  Vec2 p1, v, absV;
+  AABB segment;
+};
+__device__ __forceinline__ RaySat raySat(Vec2 p1, Vec2 d) {
+  RaySat r;
+  r.p1 = p1;
+  const Vec2 n = normalize(d);
+  r.v = crossSV(1.0f, n);
+  r.absV = {absf(r.v.x), absf(r.v.y)};
+  r.segment = rayBox(p1, d);
+  return r;
+}
+__device__ __forceinline__ bool rayMeetsBox(const RaySat& r, float4 f) {
+  if (!boxOverlaps(f, r.segment)) return false;
+  const Vec2 c = {0.5f * (f.x + f.z), 0.5f * (f.y + f.w)};
+  const Vec2 h = {0.5f * (f.z - f.x), 0.5f * (f.w - f.y)};
+  const float term1 = absf(dot(r.v, sub(r.p1, c)));
+  const float term2 = dot(r.absV, h);
+  return !(term2 < term1);
+}
+
+__global__ __launch_bounds__(kQueryLanes) void queryKernel(BatchView b, const float4* packed, const Query* qs,
+                                                           QueryHit* hits, int* counts, int cap) {
   __shared__ unsigned long long best;
   const int qi = blockIdx.x;
   const int lane = threadIdx.x;
@@ -1738,18 +1782,20 @@ __global__ __launch_bounds__(kQueryLanes) void queryKernel(BatchView b, const Qu
   const Shape* shapes = b.shapes + h->shapeOffset;
   const ShapeGeom* geoms = b.geoms + h->shapeOffset;
   const BodySim* sims = b.sims + h->bodyOffset;
+  const float4* boxes = packed + h->shapeOffset;
   const int ns = h->shapeCount;
   QueryHit* out = hits + (long)qi * cap;
   const RayInput in = {{q.p[0], q.p[1]}, {q.p[2], q.p[3]}, 1.0f};
-  const AABB segment = rayBox(in.origin, in.translation);
+  const RaySat sat = raySat(in.origin, in.translation);
   if (q.kind == QK_RAY_CLOSEST) {
     if (lane == 0) best = ~0ull;
     __syncthreads();
     unsigned long long key = ~0ull;
     CastOutput mine = noHit();
     for (int s = lane; s < ns; s += kQueryLanes) {
+      if (!rayMeetsBox(sat, boxes[s])) continue;
       const Shape& sh = shapes[s];
-      if (!shouldQuery(sh, q.catLo, q.catHi, q.maskLo, q.maskHi) || !aabbOverlaps(sh.fat, segment)) continue;
+      if (!shouldQuery(sh, q.catLo, q.catHi, q.maskLo, q.maskHi)) continue;
       const CastOutput o = rayCastShape(in, geoms[s], sims[sh.body].xf);
       if (!o.hit || !(o.fraction > 0.0f)) continue;
       // positive fractions order as their bit patterns
@@ -1786,16 +1832,14 @@ __global__ __launch_bounds__(kQueryLanes) void queryKernel(BatchView b, const Qu
     const int s = base + lane;
     bool hit = false;
     CastOutput o = noHit();
-    if (s < ns) {
+    if (s < ns && (q.kind == QK_RAY_ALL ? rayMeetsBox(sat, boxes[s]) : boxOverlaps(boxes[s], box))) {
       const Shape& sh = shapes[s];
       if (shouldQuery(sh, q.catLo, q.catHi, q.maskLo, q.maskHi)) {
         if (q.kind == QK_RAY_ALL) {
-          if (aabbOverlaps(sh.fat, segment)) {
-            o = rayCastShape(in, geoms[s], sims[sh.body].xf);
-            hit = o.hit;
-          }
+          o = rayCastShape(in, geoms[s], sims[sh.body].xf);
+          hit = o.hit;
         } else {
-          hit = aabbOverlaps(sh.fat, box);
+          hit = true;
         }
       }
     }
@@ -1924,10 +1968,15 @@ static int launchBlock(const BatchView* b, const StepParams* sp, const int* worl
 
 }  // namespace f2d
 
-extern "C" int f2d_launch_queries(const f2d::BatchView* b, const f2d::Query* q, int n, f2d::QueryHit* hits,
-                                  int* counts, int cap, hipStream_t stream) {
+// boxes: scratch of 16 B per shape of the batch (nShapes), packed here before the queries
+extern "C" int f2d_launch_queries(const f2d::BatchView* b, float4* boxes, long nShapes, const f2d::Query* q, int n,
+                                  f2d::QueryHit* hits, int* counts, int cap, hipStream_t stream) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(f2d::queryKernel, dim3(n), dim3(f2d::kQueryLanes), 0, stream, *b, q, hits, counts, cap);
+  if (nShapes > 0) {
+    hipLaunchKernelGGL(f2d::packBoxesKernel, dim3((unsigned)((nShapes + 255) / 256)), dim3(256), 0, stream, b->shapes,
+                       boxes, nShapes);
+  }
+  hipLaunchKernelGGL(f2d::queryKernel, dim3(n), dim3(f2d::kQueryLanes), 0, stream, *b, boxes, q, hits, counts, cap);
   return (int)hipGetLastError();
 }
 
@@ -1950,6 +1999,10 @@ extern "C" int f2d_launch_step(const f2d::BatchView* b, const f2d::StepParams* s
       return f2d::launchBlock<64>(b, sp, worldList, nWorlds, stream, cfg);
     case 128:
       return f2d::launchBlock<128>(b, sp, worldList, nWorlds, stream, cfg);
+    case 512:
+      return f2d::launchBlock<512>(b, sp, worldList, nWorlds, stream, cfg);
+    case 1024:
+      return f2d::launchBlock<1024>(b, sp, worldList, nWorlds, stream, cfg);
     default:
       return f2d::launchBlock<256>(b, sp, worldList, nWorlds, stream, cfg);
   }

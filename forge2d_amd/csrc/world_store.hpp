@@ -100,6 +100,7 @@ struct Batch {
   QueryHit* dHits = nullptr;
   long dHitCap = 0;
   int* dCounts = nullptr;
+  float4* dBoxes = nullptr;  // packed fat AABBs of every shape (nShapes)
 };
 
 }  // namespace f2d

@@ -222,8 +222,9 @@ long f2dg_export_world(void* batch, int world, void* out, long cap);
 /* overlay a snapshot onto a world built from the same scene spec */
 int f2dg_import_world(void* batch, int world, const void* snapshot, long bytes);
 const char* f2dg_last_error(void* batch);
-/* autotuner: per size class, times `steps` steps with 64, 128 and 256 lanes per world and
- * keeps the fastest (ms_out[3] optional: whole-batch ms per candidate), then times 4 solve
+/* autotuner: per size class, times `steps` steps with 64, 128 and 256 lanes per world (and
+ * 512 and 1024 for classes of more than 256 shapes) and keeps the fastest (ms_out[3]
+ * optional: whole-batch ms per 64/128/256 candidate), then times 4 solve
  * LDS sizes (room for 0, 0.5, 1 and 2 constraint lanes per body) and keeps the fastest;
  * the trial steps advance every world equally; returns the largest class's lanes */
 int f2dg_autotune(void* batch, float dt, int sub_steps, int steps, float* ms_out);
@@ -231,7 +232,7 @@ int f2dg_autotune(void* batch, float dt, int sub_steps, int steps, float* ms_out
  * per class: [limit, worlds, max bodies, max shapes, workgroup lanes, collide LDS bytes,
  * solve LDS bytes] into out (7 ints each, up to cap classes); returns the class count */
 int f2dg_size_classes(void* batch, int32_t* out, int cap);
-/* force a workgroup size (64, 128 or 256) / read the current one */
+/* force a workgroup size (64, 128, 256, 512 or 1024) / read the current one */
 int f2dg_set_block(void* batch, int block);
 int f2dg_get_block(void* batch);
 /* force the workgroup size of size class `cls` (index into f2dg_size_classes) */
@@ -274,8 +275,8 @@ typedef struct f2dQueryHit {
  * number of results of query i (0 for explosions, -1 for a world outside the batch);
  * the first min(count, cap) are written to hits[i * cap ...]. cap >= 1. */
 int f2dg_query(void* batch, const f2dQuery* queries, int n, f2dQueryHit* hits, int32_t* counts, int cap);
-/* Device arrays (queries, hits, counts), kinds 1-3 only: one launch on the batch's stream,
- * no synchronization. */
+/* Device arrays (queries, hits, counts), kinds 1-3 only: a packing pass over the shapes'
+ * fat AABBs and the query launch on the batch's stream, no synchronization. */
 int f2dg_query_device(void* batch, const void* queries, int n, void* hits, void* counts, int cap);
 
 #ifdef __cplusplus

@@ -3,6 +3,8 @@
 * a world stepped inside a batch of many worlds gives the same bits as the same
   world stepped alone (one launch per world), so batching never changes results;
 * stepping a subset of worlds (serving) leaves the others untouched;
+* every workgroup size the autotuner may pick (64 to 1024 lanes per world) gives
+  the same bits;
 * trajectory invariants over 256 steps at full configuration shape: a settled
   pyramid keeps its height, nothing tunnels through the ground, kinetic energy
   decays (config 1 and a config-3 batch).
@@ -37,6 +39,24 @@ def test_mixed_batch_equals_alone(gpu):
         alone.step(steps=8)
         assert alone.sync() == 0
         assert np.array_equal(batch.bodies(w), alone.bodies(0)), w
+
+
+@pytest.mark.parametrize("block", [64, 128, 512, 1024])
+def test_workgroup_size_does_not_change_results(gpu, block):
+    worlds = scenes.config4(worlds=6, lo=40, hi=400) + [scenes.container_world(1500, 30.0, 50.0, 7005, chains=3)]
+    ref = F.Batch(worlds)
+    ref.upload()
+    ref.set_block(256)
+    other = F.Batch(worlds)
+    other.upload()
+    other.set_block(block)
+    assert {c["lanes"] for c in other.size_classes()} == {block}
+    print(f"[batch] workgroup {block} lanes vs 256", flush=True)
+    ref.step(steps=16)
+    other.step(steps=16)
+    assert ref.sync() == 0 and other.sync() == 0
+    for w in range(len(worlds)):
+        assert ref.export(w) == other.export(w), w
 
 
 def test_step_subset_leaves_other_worlds(gpu):
