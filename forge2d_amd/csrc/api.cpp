@@ -690,6 +690,8 @@ static int commit(Batch& b) {
   int collide = collideBytes(np);
   collide = std::max(collide, 12 * 256);
   b.launch.collideLds = collide;
+  // worlds whose sort arrays (28 B per shape) exceed the collide LDS sweep over many workgroups
+  b.launch.sweepChunks = 28L * std::max(np, 4) > (long)collide ? (maxShapes + 255) / 256 : 0;
   int solve = maxBodies * (int)(sizeof(BodyState) + sizeof(BodyConst));
   b.launch.solveLds = solve <= kLdsLimit ? ((solve + 255) & ~255) : 0;
   if (b.launch.block == 0) b.launch.block = maxShapes <= 640 ? 64 : 256;
@@ -729,6 +731,7 @@ static int commit(Batch& b) {
       int cnp = nextPow2(std::max(c.maxShapes, 1));
       int cc = collideBytes(cnp);
       c.launch.collideLds = std::max(cc, 12 * 256);
+      c.launch.sweepChunks = 28L * std::max(cnp, 4) > (long)c.launch.collideLds ? (c.maxShapes + 255) / 256 : 0;
       c.launch.solveLds = solveLdsBytes(c.maxBodies, c.maxJoints, 1.0f, kLdsLimit);
       // one workgroup holds a whole world: the largest worlds get the most lanes
       c.launch.block = c.limit <= 128 ? 64 : (c.limit <= 512 ? 128 : (c.limit <= 1024 ? 256 : 1024));

@@ -210,6 +210,8 @@ struct alignas(16) WorldHeader {
   int moveBodyCount;             // awake bodies finalized by the last step (move events)
   int sleepPending;              // finalize ran with sleep enabled: island sleep follows
   int mergePending;              // an island got a union-find parent since the last merge
+  int sweepCount;                // shapes the multi-workgroup sweep launch sweeps (0: none)
+  int sweepPairs;                // new pairs that launch found
   unsigned long long splitKey;   // finalize: sleepiest body of an island with removed constraints
 };
 
@@ -265,6 +267,8 @@ enum : int {
   PH_NO_COUNT = 32,        // collide launch of a split step: the solve launch counts the step
   PH_CONTINUOUS = 64,      // launch the continuous-collision kernel on its own
   PH_NO_CONTINUOUS = 128,  // finalize without the continuous-collision launch (timing splits)
+  PH_PAIRS_PREP = 256,     // pair update of a large world: sort only, the sweep runs in its own launch
+  PH_PAIRS_FINISH = 512,   // pair update of a large world: create the pairs the sweep launch found
 };
 
 // A world query of the batch query ABI (f2dQuery in the public header). Rays:
@@ -291,6 +295,8 @@ struct LaunchConfig {
   int collideLds;  // sort-and-sweep + pair-sort LDS
   int solveLds;    // body-state LDS
   int continuous;  // some world of the batch has continuous collision enabled
+  int sweepChunks; // > 0: worlds whose sort arrays exceed LDS sweep in their own launch, this
+                   // many 256-shape chunks per world
 };
 
 // Contact constraint lanes (one constraint per lane), S/contact_solver.c:1034-1064.

@@ -256,7 +256,7 @@ __device__ __forceinline__ bool shouldPairCollide(const PairInfo& a, const PairI
 // Evaluate one overlapping candidate (s, t) of the sweep. Emits a new pair when the
 // reference's pair query would: at least one proxy moved, tree rules, filters, no
 // existing contact. The emitter is the proxy whose query reports the pair.
-__device__ void considerPair(const WorldPtrs& w, SharedState& sh, int s, int t, const PairInfo& S,
+__device__ void considerPair(const WorldPtrs& w, int* pairCount, int s, int t, const PairInfo& S,
                              const PairInfo& T, const uint32_t* pairSet, uint32_t pairMask) {
   int bs = S.body, bt = T.body;
   if (bs == bt) return;
@@ -304,7 +304,7 @@ __device__ void considerPair(const WorldPtrs& w, SharedState& sh, int s, int t, 
   int treeRank = ptype == kDynamic ? 0 : (ptype == kStatic ? 1 : 2);
   uint32_t partnerOrder = 0x3FFFFFFFu - (uint32_t)(P.proxyKey >> 2);
   uint64_t key = ((uint64_t)(uint32_t)w.shapes[emitter].moveRank << 32) | ((uint64_t)treeRank << 30) | partnerOrder;
-  int idx = atomicAdd(&sh.pairCount, 1);
+  int idx = atomicAdd(pairCount, 1);
   if (idx < w.hdr->pairCap) {
     w.pairKeys[idx] = key;
     w.pairShapes[idx] = make_int2(emitter, partner);
@@ -381,11 +381,46 @@ __device__ void createContact(const WorldPtrs& w, int shapeIdA, int shapeIdB) {
   w.links[id] = make_int4(-1, -1, -1, set);
 }
 
+// The sweep of sorted shape i: every later shape whose lower x lies within its box,
+// tested on the fat boxes, then the reference's pair rules (considerPair).
+__device__ __forceinline__ void sweepShape(const WorldPtrs& w, const float4* boxes, const int* order,
+                                           const PairInfo* info, int i, int n, int* pairCount,
+                                           const uint32_t* pairSet, uint32_t pairMask) {
+  float4 a = boxes[i];
+  int s = order[i];
+  const PairInfo S = info ? info[i] : pairInfo(w, s);
+  for (int j = i + 1; j < n; ++j) {
+    float4 bx = boxes[j];
+    if (bx.x > a.z) break;
+    // b2AABB_Overlaps on the two fat boxes (x already known to overlap on one side)
+    bool overlap = !(bx.x > a.z || bx.y > a.w || a.x > bx.z || a.y > bx.w);
+    if (overlap) {
+      const int t = order[j];
+      const PairInfo T = info ? info[j] : pairInfo(w, t);
+      considerPair(w, pairCount, s, t, S, T, pairSet, pairMask);
+    }
+  }
+}
+
+// Pair update of one world (b2UpdateBroadPhasePairs). mode 0: the whole update in
+// this workgroup. A world whose sort arrays do not fit LDS can be split over three
+// launches: mode 1 sorts (worlds that fit LDS run the whole update here), the sweep
+// launch (sweepKernel) spreads the sweep over many workgroups, and mode 2 creates the
+// pairs it found, in the same order.
 template <int BLOCK>
-__device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int ldsShapes, int ldsPairs, int ldsBytes) {
+__device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int ldsShapes, int ldsPairs, int ldsBytes,
+                           int mode) {
   WorldHeader* h = w.hdr;
   const int n = h->shapeCount;
   if (threadIdx.x == 0) sh.pairCount = 0;
+  int count = 0;
+  if (mode == 2) {
+    if (h->sweepCount == 0) return;
+    count = h->sweepPairs;
+    __syncthreads();
+    if (threadIdx.x == 0) h->sweepCount = 0;
+  } else {
+  if (mode == 1 && threadIdx.x == 0) h->sweepCount = 0;
   // any moved proxy at all?
   int anyMoved = 0;
   for (int i = threadIdx.x; i < n; i += BLOCK) anyMoved |= (w.shapes[i].flags & SF_MOVED) ? 1 : 0;
@@ -461,6 +496,14 @@ __device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int l
     if (info) info[i] = pairInfo(w, idx);
   }
   __syncthreads();
+  if (mode == 1 && !inLds) {
+    // the sweep runs in its own launch over many workgroups
+    if (threadIdx.x == 0) {
+      h->sweepPairs = 0;
+      h->sweepCount = n;
+    }
+    return;
+  }
   // the existing pairs as an LDS hash set when it fits behind the sort arrays
   uint32_t* pairSet = nullptr;
   uint32_t pairMask = 0;
@@ -485,24 +528,10 @@ __device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int l
       __syncthreads();
     }
   }
-  for (int i = threadIdx.x; i < n; i += BLOCK) {
-    float4 a = boxes[i];
-    int s = order[i];
-    const PairInfo S = info ? info[i] : pairInfo(w, s);
-    for (int j = i + 1; j < n; ++j) {
-      float4 bx = boxes[j];
-      if (bx.x > a.z) break;
-      // b2AABB_Overlaps on the two fat boxes (x already known to overlap on one side)
-      bool overlap = !(bx.x > a.z || bx.y > a.w || a.x > bx.z || a.y > bx.w);
-      if (overlap) {
-        const int t = order[j];
-        const PairInfo T = info ? info[j] : pairInfo(w, t);
-        considerPair(w, sh, s, t, S, T, pairSet, pairMask);
-      }
-    }
-  }
+  for (int i = threadIdx.x; i < n; i += BLOCK) sweepShape(w, boxes, order, info, i, n, &sh.pairCount, pairSet, pairMask);
   __syncthreads();
-  int count = sh.pairCount;
+  count = sh.pairCount;
+  }
   if (count > h->pairCap) {
     if (threadIdx.x == 0) h->error |= ERR_PAIR_CAP;
     count = h->pairCap;
@@ -1472,10 +1501,28 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   int ldsPairs = 1;
   while (ldsPairs * 2 * 12 <= ldsBytes) ldsPairs *= 2;
   if (ldsPairs * 12 > ldsBytes) ldsPairs = 0;
-  if (sp.phases & PH_PAIRS) pairsPhase<BLOCK>(w, sh, lds, ldsShapes, ldsPairs, ldsBytes);
+  if (sp.phases & PH_PAIRS) {
+    pairsPhase<BLOCK>(w, sh, lds, ldsShapes, ldsPairs, ldsBytes, (sp.phases & PH_PAIRS_PREP) ? 1 : 0);
+  } else if (sp.phases & PH_PAIRS_FINISH) {
+    pairsPhase<BLOCK>(w, sh, lds, ldsShapes, ldsPairs, ldsBytes, 2);
+  }
   if (sp.phases & PH_COLLIDE) collidePhase<BLOCK>(w, sh);
   if (sp.phases & PH_STATE) statePhase(w, sh);
   if (threadIdx.x == 0 && (sp.phases & (PH_SOLVE | PH_FINALIZE | PH_NO_COUNT)) == 0) h->stepIndex += 1;
+}
+
+// The sweep of the worlds a mode-1 pair update left in HBM (sweepCount > 0), one
+// sorted shape per lane, blockIdx.y the world of the class, blockIdx.x its chunk of
+// 256 shapes; new pairs go to the world's pair buffer through a global counter.
+__global__ __launch_bounds__(256) void sweepKernel(BatchView b, const int* worldList) {
+  const int wi = worldList ? worldList[blockIdx.y] : blockIdx.y;
+  WorldHeader* h = b.worlds + wi;
+  const int n = h->sweepCount;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  WorldPtrs w = worldPtrs(b, h);
+  const int np = nextPow2(max(n, 4));
+  sweepShape(w, w.sapBoxes, w.scratch + 2 * np, nullptr, i, n, &h->sweepPairs, nullptr, 0u);
 }
 
 // Solver stages and finalize of one world per workgroup; body states and the
@@ -1716,18 +1763,6 @@ __device__ __forceinline__ AABB rayBox(Vec2 p1, Vec2 d) {
   return {vmin(p1, p2), vmax(p1, p2)};
 }
 
-// Ray casts and AABB overlaps over the worlds of a batch, one query per wavefront,
-// the lanes over the world's shapes (the shapes' fat AABBs are the reference's tree
-// leaves, so the candidate sets are the trees' without a tree):
-//   QK_RAY_CLOSEST  b2World_CastRayClosest (S/world.c:2277-2308): the hit of
-//                   smallest fraction > 0 (initial overlaps are ignored, as
-//                   b2RayCastClosestFcn does); ties go to the lower shape id
-//   QK_RAY_ALL      b2World_CastRay with a callback that returns 1 (S/world.c:2185-2254):
-//                   every hit, initial overlaps included, in shape-id order
-//   QK_OVERLAP_AABB b2World_OverlapAABB (S/world.c:2050-2105): every shape whose fat
-//                   AABB overlaps the box, in shape-id order
-// counts[q] receives the number of results (which may exceed cap: only the first cap
-// are written to hits[q * cap ...]), or -1 for a world index outside the batch.
 // The shapes' fat AABBs packed 16 B per shape in batch shape order (one pass before
 // the queries): the scan reads one float4 per shape instead of a line of its
 // 96-byte record, and the record only for shapes whose box passes.
@@ -1768,14 +1803,23 @@ __device__ __forceinline__ bool rayMeetsBox(const RaySat& r, float4 f) {
   return !(term2 < term1);
 }
 
-__global__ __launch_bounds__(kQueryLanes) void queryKernel(BatchView b, const float4* packed, const Query* qs,
-                                                           QueryHit* hits, int* counts, int cap) {
-  __shared__ unsigned long long best;
-  const int qi = blockIdx.x;
+// Ray casts and AABB overlaps over the worlds of a batch, one wavefront per query,
+// the lanes over the world's shapes (the shapes' fat AABBs are the reference's tree
+// leaves, so the candidate sets are the trees' without a tree):
+//   QK_RAY_CLOSEST  b2World_CastRayClosest (S/world.c:2277-2308): the hit of
+//                   smallest fraction > 0 (initial overlaps are ignored, as
+//                   b2RayCastClosestFcn does); ties go to the lower shape id
+//   QK_RAY_ALL      b2World_CastRay with a callback that returns 1 (S/world.c:2185-2254):
+//                   every hit, initial overlaps included, in shape-id order
+//   QK_OVERLAP_AABB b2World_OverlapAABB (S/world.c:2050-2105): every shape whose fat
+//                   AABB overlaps the box, in shape-id order
+// counts[q] receives the number of results (which may exceed cap: only the first cap
+// are written to hits[q * cap ...]), or -1 for a world index outside the batch.
+__device__ void runQuery(const BatchView& b, const float4* packed, const Query& q, QueryHit* out, int* count,
+                         int cap, unsigned long long* best) {
   const int lane = threadIdx.x;
-  const Query q = qs[qi];
   if (q.world < 0 || q.world >= b.worldCount) {
-    if (lane == 0) counts[qi] = -1;
+    if (lane == 0) *count = -1;
     return;
   }
   const WorldHeader* h = b.worlds + q.world;
@@ -1784,12 +1828,9 @@ __global__ __launch_bounds__(kQueryLanes) void queryKernel(BatchView b, const fl
   const BodySim* sims = b.sims + h->bodyOffset;
   const float4* boxes = packed + h->shapeOffset;
   const int ns = h->shapeCount;
-  QueryHit* out = hits + (long)qi * cap;
   const RayInput in = {{q.p[0], q.p[1]}, {q.p[2], q.p[3]}, 1.0f};
   const RaySat sat = raySat(in.origin, in.translation);
   if (q.kind == QK_RAY_CLOSEST) {
-    if (lane == 0) best = ~0ull;
-    __syncthreads();
     unsigned long long key = ~0ull;
     CastOutput mine = noHit();
     for (int s = lane; s < ns; s += kQueryLanes) {
@@ -1805,11 +1846,12 @@ __global__ __launch_bounds__(kQueryLanes) void queryKernel(BatchView b, const fl
         mine = o;
       }
     }
-    atomicMin(&best, key);
+    atomicMin(best, key);
     __syncthreads();
-    if (best == ~0ull) {
-      if (lane == 0) counts[qi] = 0;
-    } else if (key == best) {
+    const unsigned long long winner = *best;
+    if (winner == ~0ull) {
+      if (lane == 0) *count = 0;
+    } else if (key == winner) {
       QueryHit r;
       r.shape = (int)(key & 0xFFFFFFFFull);
       r.px = mine.point.x;
@@ -1818,12 +1860,12 @@ __global__ __launch_bounds__(kQueryLanes) void queryKernel(BatchView b, const fl
       r.ny = mine.normal.y;
       r.fraction = mine.fraction;
       if (cap > 0) out[0] = r;
-      counts[qi] = 1;
+      *count = 1;
     }
     return;
   }
   if (q.kind != QK_RAY_ALL && q.kind != QK_OVERLAP_AABB) {
-    if (lane == 0) counts[qi] = 0;
+    if (lane == 0) *count = 0;
     return;
   }
   const AABB box = {{q.p[0], q.p[1]}, {q.p[2], q.p[3]}};
@@ -1859,7 +1901,24 @@ __global__ __launch_bounds__(kQueryLanes) void queryKernel(BatchView b, const fl
     }
     total += __popcll(m);
   }
-  if (lane == 0) counts[qi] = total;
+  if (lane == 0) *count = total;
+}
+
+// kQueriesPerGroup consecutive queries per wavefront, one after another (a million
+// one-query workgroups would be bound by workgroup dispatch)
+constexpr int kQueriesPerGroup = 8;
+
+__global__ __launch_bounds__(kQueryLanes) void queryKernel(BatchView b, const float4* packed, const Query* qs, int n,
+                                                           QueryHit* hits, int* counts, int cap) {
+  __shared__ unsigned long long best;
+  const int first = blockIdx.x * kQueriesPerGroup;
+  const int last = min(first + kQueriesPerGroup, n);
+  for (int qi = first; qi < last; ++qi) {
+    if (threadIdx.x == 0) best = ~0ull;
+    __syncthreads();
+    runQuery(b, packed, qs[qi], hits + (long)qi * cap, counts + qi, cap, &best);
+    __syncthreads();
+  }
 }
 
 // b2World_Explode / ExplosionCallback, S/world.c:2640-2747: one explosion per
@@ -1951,7 +2010,18 @@ __global__ __launch_bounds__(kQueryLanes) void explodeKernel(BatchView b, const 
 template <int BLOCK>
 static int launchBlock(const BatchView* b, const StepParams* sp, const int* worldList, int nWorlds, hipStream_t stream,
                        const LaunchConfig* cfg) {
-  if (sp->phases & (PH_PAIRS | PH_COLLIDE | PH_STATE)) {
+  if ((sp->phases & PH_PAIRS) && cfg->sweepChunks > 0 && nWorlds <= 65535) {
+    // large worlds: sort, sweep over many workgroups, then create, collide, state
+    StepParams a = *sp;
+    a.phases = PH_PAIRS | PH_PAIRS_PREP | PH_NO_COUNT;
+    hipLaunchKernelGGL(collideKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->collideLds, stream, *b, a, worldList,
+                       cfg->collideLds);
+    hipLaunchKernelGGL(sweepKernel, dim3(cfg->sweepChunks, nWorlds), dim3(256), 0, stream, *b, worldList);
+    StepParams c = *sp;
+    c.phases = (sp->phases & ~PH_PAIRS) | PH_PAIRS_FINISH;
+    hipLaunchKernelGGL(collideKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->collideLds, stream, *b, c, worldList,
+                       cfg->collideLds);
+  } else if (sp->phases & (PH_PAIRS | PH_COLLIDE | PH_STATE)) {
     hipLaunchKernelGGL(collideKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->collideLds, stream, *b, *sp, worldList,
                        cfg->collideLds);
   }
@@ -1976,7 +2046,9 @@ extern "C" int f2d_launch_queries(const f2d::BatchView* b, float4* boxes, long n
     hipLaunchKernelGGL(f2d::packBoxesKernel, dim3((unsigned)((nShapes + 255) / 256)), dim3(256), 0, stream, b->shapes,
                        boxes, nShapes);
   }
-  hipLaunchKernelGGL(f2d::queryKernel, dim3(n), dim3(f2d::kQueryLanes), 0, stream, *b, boxes, q, hits, counts, cap);
+  const unsigned groups = (unsigned)((n + f2d::kQueriesPerGroup - 1) / f2d::kQueriesPerGroup);
+  hipLaunchKernelGGL(f2d::queryKernel, dim3(groups), dim3(f2d::kQueryLanes), 0, stream, *b, boxes, q, n, hits, counts,
+                     cap);
   return (int)hipGetLastError();
 }
 

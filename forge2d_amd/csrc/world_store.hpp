@@ -71,7 +71,7 @@ struct SizeClass {
   int count = 0;
   int maxBodies = 0, maxShapes = 0, maxJoints = 0;
   int* dList = nullptr;       // world ids, largest first
-  LaunchConfig launch = {0, 0, 0, 0};
+  LaunchConfig launch = {0, 0, 0, 0, 0};
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
 };
@@ -93,7 +93,7 @@ struct Batch {
   float contactCapFactor = 6.0f;
   // flat array lengths of the committed batch (bulk host mirror copies)
   long nBodies = 0, nShapes = 0, nContacts = 0, nJoints = 0, nColorWords = 0, nIsland = 0;
-  LaunchConfig launch = {0, 0, 0, 0};
+  LaunchConfig launch = {0, 0, 0, 0, 0};
   // query staging on the device (grown on demand, freed with the batch)
   Query* dQueries = nullptr;
   long dQueryCap = 0;
