@@ -311,74 +311,113 @@ __device__ void considerPair(const WorldPtrs& w, int* pairCount, int s, int t, c
   }
 }
 
-// b2CreateContact, S/contact.c:186-328 (serial, thread 0)
-__device__ void createContact(const WorldPtrs& w, int shapeIdA, int shapeIdB) {
-  int ta = w.geoms[shapeIdA].type, tb = w.geoms[shapeIdB].type;
-  if (!hasManifoldFcn(ta, tb)) return;
-  if (!isPrimary(ta, tb)) {
-    int tmp = shapeIdA;
-    shapeIdA = shapeIdB;
-    shapeIdB = tmp;
-  }
+// b2CreateContact (S/contact.c:186-328) for the sorted new pairs (pv: pair indices in
+// creation order), with the result of a serial loop over them: ids are the LIFO
+// pool's pops in creation order (free list top first, then fresh ids), so each pair's
+// id follows from its rank among the pairs that get a contact; the records are filled
+// in parallel and only the body contact-list prepends run serially, in pair order.
+// pk is reused for the oriented shapes and pv for the ids.
+template <int BLOCK>
+__device__ void createContacts(const WorldPtrs& w, SharedState& sh, uint64_t* pk, int* pv, int count) {
   WorldHeader* h = w.hdr;
-  int id;
-  if (h->freeCount > 0) {
-    id = w.freeIds[--h->freeCount];
-  } else {
-    if (h->contactCount >= h->contactCap) {
-      h->error |= ERR_CONTACT_CAP;
-      return;
+  const int freeCount = h->freeCount, fresh = h->contactCount, cap = h->contactCap;
+  int created = 0;
+  for (int base = 0; base < count; base += BLOCK) {
+    const int i = base + threadIdx.x;
+    int a = -1, bsh = -1;
+    bool valid = false;
+    if (i < count) {
+      int2 pr = w.pairShapes[pv[i]];
+      int e = pr.x, p = pr.y;
+      // orientation by proxy key, S/broad_phase.c:228-237, then the primary type order
+      if (w.shapes[p].proxyKey < w.shapes[e].proxyKey) {
+        a = p;
+        bsh = e;
+      } else {
+        a = e;
+        bsh = p;
+      }
+      int ta = w.geoms[a].type, tb = w.geoms[bsh].type;
+      valid = hasManifoldFcn(ta, tb);
+      if (valid && !isPrimary(ta, tb)) {
+        int t = a;
+        a = bsh;
+        bsh = t;
+      }
     }
-    id = h->contactCount++;
-  }
-  const Shape& A = w.shapes[shapeIdA];
-  const Shape& B = w.shapes[shapeIdB];
-  Contact& c = w.contacts[id];
-  c.shapeA = shapeIdA;
-  c.shapeB = shapeIdB;
-  c.flags = ((A.flags | B.flags) & SF_CONTACT_EVENTS) ? CF_CONTACT_EVENTS : 0;
-  c.simFlags = ((A.flags | B.flags) & SF_PRESOLVE) ? SIM_PRESOLVE : 0;
-  c.color = -1;
-  c.localIndex = -1;
-  w.ccolor[id] = -1;
-  int bodyA = A.body, bodyB = B.body;
-  {
-    BodySim& ba = w.sims[bodyA];
-    c.bodyA = bodyA;
-    c.prevA = -1;
-    c.nextA = ba.headContactKey;
-    int keyA = (id << 1) | 0;
-    int head = ba.headContactKey;
-    if (head >= 0) {
-      Contact& hc = w.contacts[head >> 1];
-      if ((head & 1) == 0) hc.prevA = keyA; else hc.prevB = keyA;
+    int total;
+    const int rank = created + blockCompact<BLOCK>(valid, sh.waveSums, &total);
+    if (i < count) {
+      int id = -1;
+      if (valid) {
+        if (rank < freeCount) {
+          id = w.freeIds[freeCount - 1 - rank];
+        } else if (fresh + (rank - freeCount) < cap) {
+          id = fresh + (rank - freeCount);
+        } else {
+          atomicOr(&h->error, ERR_CONTACT_CAP);
+        }
+      }
+      pk[i] = ((uint64_t)(uint32_t)a << 32) | (uint32_t)bsh;
+      pv[i] = id;
+      if (id >= 0) {
+        const Shape& A = w.shapes[a];
+        const Shape& B = w.shapes[bsh];
+        Contact& c = w.contacts[id];
+        c.shapeA = a;
+        c.shapeB = bsh;
+        c.flags = ((A.flags | B.flags) & SF_CONTACT_EVENTS) ? CF_CONTACT_EVENTS : 0;
+        c.simFlags = ((A.flags | B.flags) & SF_PRESOLVE) ? SIM_PRESOLVE : 0;
+        c.color = -1;
+        c.localIndex = -1;
+        w.ccolor[id] = -1;
+        c.bodyA = A.body;
+        c.bodyB = B.body;
+        zeroManifold(c.m);
+        c.friction = sqrtf(A.friction * B.friction);
+        c.restitution = maxf(A.restitution, B.restitution);
+        c.rollingResistance = 0.0f;
+        c.tangentSpeed = 0.0f;
+        // the awake set when either body is awake, else the disabled set (S/contact.c:213-224)
+        int set = (w.sims[A.body].setIndex == kAwakeSet || w.sims[B.body].setIndex == kAwakeSet) ? kAwakeSet
+                                                                                                   : kDisabledSet;
+        if (set != kAwakeSet) c.flags |= CF_OFF_AWAKE;
+        w.links[id] = make_int4(-1, -1, -1, set);
+      }
     }
-    ba.headContactKey = keyA;
-    ba.contactCount += 1;
+    created += total;
   }
-  {
-    BodySim& bb = w.sims[bodyB];
-    c.bodyB = bodyB;
-    c.prevB = -1;
-    c.nextB = bb.headContactKey;
-    int keyB = (id << 1) | 1;
-    int head = bb.headContactKey;
-    if (head >= 0) {
-      Contact& hc = w.contacts[head >> 1];
-      if ((head & 1) == 0) hc.prevA = keyB; else hc.prevB = keyB;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int used = min(created, freeCount + (cap - fresh));
+    h->freeCount = freeCount - min(used, freeCount);
+    h->contactCount = fresh + max(used - freeCount, 0);
+    // body contact lists: prepend each new contact to both bodies' lists in creation order
+    for (int i = 0; i < count; ++i) {
+      const int id = pv[i];
+      if (id < 0) continue;
+      Contact& c = w.contacts[id];
+      for (int edge = 0; edge < 2; ++edge) {
+        BodySim& body = w.sims[edge == 0 ? c.bodyA : c.bodyB];
+        const int key = (id << 1) | edge;
+        const int head = body.headContactKey;
+        if (edge == 0) {
+          c.prevA = -1;
+          c.nextA = head;
+        } else {
+          c.prevB = -1;
+          c.nextB = head;
+        }
+        if (head >= 0) {
+          Contact& hc = w.contacts[head >> 1];
+          if ((head & 1) == 0) hc.prevA = key; else hc.prevB = key;
+        }
+        body.headContactKey = key;
+        body.contactCount += 1;
+      }
     }
-    bb.headContactKey = keyB;
-    bb.contactCount += 1;
   }
-  zeroManifold(c.m);
-  c.friction = sqrtf(A.friction * B.friction);
-  c.restitution = maxf(A.restitution, B.restitution);
-  c.rollingResistance = 0.0f;
-  c.tangentSpeed = 0.0f;
-  // the awake set when either body is awake, else the disabled set (S/contact.c:213-224)
-  int set = (w.sims[bodyA].setIndex == kAwakeSet || w.sims[bodyB].setIndex == kAwakeSet) ? kAwakeSet : kDisabledSet;
-  if (set != kAwakeSet) c.flags |= CF_OFF_AWAKE;
-  w.links[id] = make_int4(-1, -1, -1, set);
+  __syncthreads();
 }
 
 // The sweep of sorted shape i: every later shape whose lower x lies within its box,
@@ -450,7 +489,8 @@ __device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int l
   // motion); a bounded number of them, then a full bitonic sort. The keys are
   // distinct, so the sorted order is the same either way.
   bool sorted = false;
-  for (int pass = 0; pass < 16 && !sorted; ++pass) {
+  const int maxPasses = inLds ? 16 : 64;  // a bitonic sort in HBM costs about 200 passes
+  for (int pass = 0; pass < maxPasses && !sorted; ++pass) {
     int swapped = 0;
     for (int parity = 0; parity < 2; ++parity) {
       for (int i = 2 * threadIdx.x + parity; i + 1 < n; i += 2 * BLOCK) {
@@ -549,22 +589,7 @@ __device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int l
     }
     __syncthreads();
     bitonicSort<BLOCK>(pk, pv, pp);
-    if (threadIdx.x == 0) {
-      for (int i = 0; i < count; ++i) {
-        int2 pr = w.pairShapes[pv[i]];
-        int e = pr.x, p = pr.y;
-        // orientation by proxy key, S/broad_phase.c:228-237
-        int a, bsh;
-        if (w.shapes[p].proxyKey < w.shapes[e].proxyKey) {
-          a = p;
-          bsh = e;
-        } else {
-          a = e;
-          bsh = p;
-        }
-        createContact(w, a, bsh);
-      }
-    }
+    createContacts<BLOCK>(w, sh, pk, pv, count);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += BLOCK) w.shapes[i].flags &= ~SF_MOVED;
