@@ -248,6 +248,20 @@ F2D_HD_COLD void mergeAwakeIslands(IslandCtx& c) {
   }
 }
 
+// The merges of mergeAwakeIslands once every awake island points at its root (the
+// solve kernel finds the roots on all lanes): children merge into their roots in
+// reverse awake order.
+F2D_HD_COLD void mergeAwakeChildren(IslandCtx& c) {
+  WorldHeader* h = c.h;
+  h->mergePending = 0;
+  for (int i = h->awakeIslandCount - 1; i >= 0; --i) {
+    int id = c.awakeIslands[i];
+    if (c.islands[id].parent == -1) continue;
+    mergeIsland(c, id);
+    destroyIsland(c, id);
+  }
+}
+
 // b2SplitIsland, S/island.c:596-834: one depth-first search per unvisited body of
 // the base island, following touching contacts (not into static bodies) and
 // joints (into awake bodies). Marks live in bit 30 of a body's / contact's flags

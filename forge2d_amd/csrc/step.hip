@@ -1096,9 +1096,26 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
   // b2Solve, S/solver.c:1190-1220 and :1479-1706: merge islands linked since the last
   // step, stop when no body is awake, split the last step's candidate island
   if (phases & PH_SOLVE) {
+    // b2MergeAwakeIslands (S/island.c:547-590). Its first loop leaves every awake
+    // island pointing at its root; here every lane walks its islands' parent chains
+    // read-only (a chain read while other lanes shorten it still ends at the same
+    // root) and writes its own island's root, then one lane merges in reverse
+    // awake order.
+    const bool merge = h->mergePending != 0;
+    if (merge) {
+      IslandCtx ic = w.isl;
+      for (int i = threadIdx.x; i < h->awakeIslandCount; i += BLOCK) {
+        const int id = ic.awakeIslands[i];
+        int root = id;
+        for (int guard = h->islandCount; ic.islands[root].parent != -1 && guard > 0; --guard)
+          root = ic.islands[root].parent;
+        if (root != id) ic.islands[id].parent = root;
+      }
+      __syncthreads();
+    }
     if (threadIdx.x == 0) {
       IslandCtx ic = w.isl;
-      mergeAwakeIslands(ic);
+      if (merge) mergeAwakeChildren(ic);
       sh.noAwake = h->awakeBodyCount == 0 ? 1 : 0;
       if (!sh.noAwake && h->splitIslandId != -1) splitIsland(ic, h->splitIslandId);
       if (!sh.noAwake) h->splitIslandId = -1;
@@ -1177,7 +1194,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         jointLanes[sh.jointColorStart[j.color] + atomicAdd(&sh.jointColorFill[j.color], 1)] = i;
       }
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && sh.jointColorCount[kOverflowIndex] > 0) {
       // overflow joints are solved one after another in their colour's array order
       int* list = jointLanes + sh.jointColorStart[kOverflowIndex];
       const int n = sh.jointColorCount[kOverflowIndex];
@@ -1209,6 +1226,8 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     const int coloredJoints = sh.jointColorStart[kOverflowIndex];
     const int overflowJoints = sh.jointColorCount[kOverflowIndex];
     const int* overflowJointList = jointLanes + coloredJoints;
+    // the overflow stages (one lane, then a barrier) only when there is overflow
+    const bool anyOverflow = overflowJoints + overflowCount > 0;
 
     // -- prepare joints (all), S/solver.c:967-987 -> S/joint.c:1348-1402
     for (int i = threadIdx.x; i < h->jointCount; i += BLOCK) {
@@ -1262,12 +1281,14 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       }
       __syncthreads();
       // warm start: overflow joints, overflow contacts, then colours
-      if (threadIdx.x == 0) {
-        for (int k = 0; k < overflowJoints; ++k)
-          if (overflowJointList[k] >= 0) warmStartRevolute(jts[overflowJointList[k]], states);
-        for (int k = 0; k < overflowCount; ++k) overflowWarmStart(cons, overflowStart + k, states);
+      if (anyOverflow) {
+        if (threadIdx.x == 0) {
+          for (int k = 0; k < overflowJoints; ++k)
+            if (overflowJointList[k] >= 0) warmStartRevolute(jts[overflowJointList[k]], states);
+          for (int k = 0; k < overflowCount; ++k) overflowWarmStart(cons, overflowStart + k, states);
+        }
+        __syncthreads();
       }
-      __syncthreads();
       for (int col = 0; col < kOverflowIndex; ++col) {
         int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
         if (cnt + jcnt == 0) continue;
@@ -1277,14 +1298,16 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         __syncthreads();
       }
       // solve
-      if (threadIdx.x == 0) {
-        for (int k = 0; k < overflowJoints; ++k)
-          if (overflowJointList[k] >= 0)
-            solveRevolute(jts[overflowJointList[k]], states, ctx.h, ctx.inv_h, true);
-        for (int k = 0; k < overflowCount; ++k)
-          overflowSolve(cons, overflowStart + k, states, ctx.inv_h, ctx.maxContactPushSpeed, true);
+      if (anyOverflow) {
+        if (threadIdx.x == 0) {
+          for (int k = 0; k < overflowJoints; ++k)
+            if (overflowJointList[k] >= 0)
+              solveRevolute(jts[overflowJointList[k]], states, ctx.h, ctx.inv_h, true);
+          for (int k = 0; k < overflowCount; ++k)
+            overflowSolve(cons, overflowStart + k, states, ctx.inv_h, ctx.maxContactPushSpeed, true);
+        }
+        __syncthreads();
       }
-      __syncthreads();
       for (int col = 0; col < kOverflowIndex; ++col) {
         int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
         if (cnt + jcnt == 0) continue;
@@ -1303,14 +1326,16 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       }
       __syncthreads();
       // relax
-      if (threadIdx.x == 0) {
-        for (int k = 0; k < overflowJoints; ++k)
-          if (overflowJointList[k] >= 0)
-            solveRevolute(jts[overflowJointList[k]], states, ctx.h, ctx.inv_h, false);
-        for (int k = 0; k < overflowCount; ++k)
-          overflowSolve(cons, overflowStart + k, states, ctx.inv_h, ctx.maxContactPushSpeed, false);
+      if (anyOverflow) {
+        if (threadIdx.x == 0) {
+          for (int k = 0; k < overflowJoints; ++k)
+            if (overflowJointList[k] >= 0)
+              solveRevolute(jts[overflowJointList[k]], states, ctx.h, ctx.inv_h, false);
+          for (int k = 0; k < overflowCount; ++k)
+            overflowSolve(cons, overflowStart + k, states, ctx.inv_h, ctx.maxContactPushSpeed, false);
+        }
+        __syncthreads();
       }
-      __syncthreads();
       for (int col = 0; col < kOverflowIndex; ++col) {
         int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
         if (cnt + jcnt == 0) continue;
@@ -1329,11 +1354,13 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       for (int i = threadIdx.x; i < words; i += BLOCK) dst[i] = src[i];
     }
     // restitution: overflow then colours
-    if (threadIdx.x == 0) {
-      for (int k = 0; k < overflowCount; ++k)
-        overflowRestitution(cons, overflowStart + k, states, ctx.restitutionThreshold);
+    if (overflowCount > 0) {
+      if (threadIdx.x == 0) {
+        for (int k = 0; k < overflowCount; ++k)
+          overflowRestitution(cons, overflowStart + k, states, ctx.restitutionThreshold);
+      }
+      __syncthreads();
     }
-    __syncthreads();
     for (int col = 0; col < kOverflowIndex; ++col) {
       int cnt = sh.colorCount[col];
       if (cnt == 0) continue;
