@@ -30,6 +30,8 @@
 namespace f2d {
 
 constexpr int kMaxWaves = 16;  // largest workgroup: 1024 lanes
+constexpr int kLongSweeps = 64;   // long sweeps a workgroup continues together
+constexpr int kLongRun = 32;      // candidates one lane tests before it hands a sweep over
 
 struct SharedState {
   int colorCount[kGraphColorCount];
@@ -45,6 +47,15 @@ struct SharedState {
   int error;
   int pairCount;
   int noAwake;  // no awake body: b2Solve returns after merging islands
+};
+
+// Sweeps of shapes whose x-range covers many others (ground and ceiling segments),
+// continued by the whole workgroup: shape, next candidate (collide and sweep
+// kernels only, so the solve kernel's LDS stays small)
+struct LongSweeps {
+  int count;
+  int shape[kLongSweeps];
+  int next[kLongSweeps];
 };
 
 // ---------------------------------------------------------------------------- block helpers
@@ -420,15 +431,20 @@ __device__ void createContacts(const WorldPtrs& w, SharedState& sh, uint64_t* pk
   __syncthreads();
 }
 
-// The sweep of sorted shape i: every later shape whose lower x lies within its box,
-// tested on the fat boxes, then the reference's pair rules (considerPair).
-__device__ __forceinline__ void sweepShape(const WorldPtrs& w, const float4* boxes, const int* order,
-                                           const PairInfo* info, int i, int n, int* pairCount,
-                                           const uint32_t* pairSet, uint32_t pairMask) {
+// The sweep of sorted shape i: every later shape j (from jBegin, step jStep) whose
+// lower x lies within its box, tested on the fat boxes, then the reference's pair
+// rules (considerPair). With jStep 1 it stops after maxRun candidates and returns
+// the next j (the rest of a long sweep, continued by the whole workgroup), else -1.
+__device__ __forceinline__ int sweepShape(const WorldPtrs& w, const float4* boxes, const int* order,
+                                          const PairInfo* info, int i, int jBegin, int jStep, int n, int maxRun,
+                                          int* pairCount, const uint32_t* pairSet, uint32_t pairMask) {
   float4 a = boxes[i];
   int s = order[i];
   const PairInfo S = info ? info[i] : pairInfo(w, s);
-  for (int j = i + 1; j < n; ++j) {
+  int run = 0;
+  for (int j = jBegin; j < n; j += jStep) {
+    if (run == maxRun) return j;
+    ++run;
     float4 bx = boxes[j];
     if (bx.x > a.z) break;
     // b2AABB_Overlaps on the two fat boxes (x already known to overlap on one side)
@@ -439,6 +455,40 @@ __device__ __forceinline__ void sweepShape(const WorldPtrs& w, const float4* box
       considerPair(w, pairCount, s, t, S, T, pairSet, pairMask);
     }
   }
+  return -1;
+}
+
+// Every sweep of a workgroup's shapes [first, first + count): each lane sweeps its
+// shapes up to kLongRun candidates; longer sweeps are continued by all lanes, one
+// after another, each lane testing every BLOCK-th candidate (the sorted boxes end
+// a lane's part at the first box past the shape's upper x). Every (i, j) candidate
+// is tested exactly once; the pairs' order is fixed later by their sort.
+template <int BLOCK>
+__device__ void sweepShapes(const WorldPtrs& w, LongSweeps& sh, const float4* boxes, const int* order,
+                            const PairInfo* info, int first, int count, int n, int* pairCount,
+                            const uint32_t* pairSet, uint32_t pairMask) {
+  if (threadIdx.x == 0) sh.count = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < count; k += BLOCK) {
+    const int i = first + k;
+    const int next = sweepShape(w, boxes, order, info, i, i + 1, 1, n, kLongRun, pairCount, pairSet, pairMask);
+    if (next >= 0) {
+      const int slot = atomicAdd(&sh.count, 1);
+      if (slot < kLongSweeps) {
+        sh.shape[slot] = i;
+        sh.next[slot] = next;
+      } else {
+        sweepShape(w, boxes, order, info, i, next, 1, n, -1, pairCount, pairSet, pairMask);
+      }
+    }
+  }
+  __syncthreads();
+  const int nLong = min(sh.count, kLongSweeps);
+  for (int k = 0; k < nLong; ++k) {
+    const int i = sh.shape[k];
+    sweepShape(w, boxes, order, info, i, sh.next[k] + threadIdx.x, BLOCK, n, -1, pairCount, pairSet, pairMask);
+  }
+  __syncthreads();
 }
 
 // Pair update of one world (b2UpdateBroadPhasePairs). mode 0: the whole update in
@@ -568,8 +618,8 @@ __device__ void pairsPhase(const WorldPtrs& w, SharedState& sh, char* lds, int l
       __syncthreads();
     }
   }
-  for (int i = threadIdx.x; i < n; i += BLOCK) sweepShape(w, boxes, order, info, i, n, &sh.pairCount, pairSet, pairMask);
-  __syncthreads();
+  __shared__ LongSweeps longSweeps;
+  sweepShapes<BLOCK>(w, longSweeps, boxes, order, info, 0, n, n, &sh.pairCount, pairSet, pairMask);
   count = sh.pairCount;
   }
   if (count > h->pairCap) {
@@ -1563,18 +1613,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   if (threadIdx.x == 0 && (sp.phases & (PH_SOLVE | PH_FINALIZE | PH_NO_COUNT)) == 0) h->stepIndex += 1;
 }
 
-// The sweep of the worlds a mode-1 pair update left in HBM (sweepCount > 0), one
-// sorted shape per lane, blockIdx.y the world of the class, blockIdx.x its chunk of
-// 256 shapes; new pairs go to the world's pair buffer through a global counter.
+// The sweep of the worlds a mode-1 pair update left in HBM (sweepCount > 0): a
+// workgroup per 256 sorted shapes (blockIdx.x), blockIdx.y the world of the class;
+// new pairs go to the world's pair buffer through a global counter.
 __global__ __launch_bounds__(256) void sweepKernel(BatchView b, const int* worldList) {
+  __shared__ LongSweeps sh;
   const int wi = worldList ? worldList[blockIdx.y] : blockIdx.y;
   WorldHeader* h = b.worlds + wi;
   const int n = h->sweepCount;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  const int first = blockIdx.x * 256;
+  if (first >= n) return;  // the whole workgroup: n is uniform
   WorldPtrs w = worldPtrs(b, h);
   const int np = nextPow2(max(n, 4));
-  sweepShape(w, w.sapBoxes, w.scratch + 2 * np, nullptr, i, n, &h->sweepPairs, nullptr, 0u);
+  sweepShapes<256>(w, sh, w.sapBoxes, w.scratch + 2 * np, nullptr, first, min(256, n - first), n, &h->sweepPairs,
+                   nullptr, 0u);
 }
 
 // Solver stages and finalize of one world per workgroup; body states and the
