@@ -1708,7 +1708,17 @@ __device__ __forceinline__ bool queryVisible(const Shape& t, AABB box) {
 // b2SolveContinuous, S/solver.c:390-541, for fast body i. Non-bullets sweep against
 // the static proxies (listed in shape order); bullets also against kinematic and
 // dynamic proxies, after every non-bullet body has its final transform.
-__device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList, int staticCount, bool bullet) {
+// The static proxies' fat boxes and visibility staged in LDS by the continuous
+// kernel (the first kMaxStagedStatics; the rest are read from their records)
+constexpr int kMaxStagedStatics = 64;
+constexpr int kContinuousLanes = 64;  // fast bodies of one world per workgroup pass
+struct StaticProxies {
+  AABB fat[kMaxStagedStatics];
+  int visible[kMaxStagedStatics];
+};
+
+__device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList, int staticCount, bool bullet,
+                                const StaticProxies& statics) {
   BodySim& sim = w.sims[i];
   const int ns = w.hdr->shapeCount;
   Sweep sweep = makeSweep(sim);
@@ -1725,9 +1735,11 @@ __device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList
     fs.aabb = box2;
     if (fs.flags & SF_SENSOR) continue;
     for (int k = 0; k < staticCount; ++k) {
-      int t = staticList[k];
-      if (queryVisible(w.shapes[t], box))
-        fraction = continuousCandidate(w.shapes, w.geoms, w.sims, w.joints, i, s, sweep, t, fraction);
+      const bool staged = k < kMaxStagedStatics;
+      const int t = staticList[k];
+      if (staged ? !(statics.visible[k] && aabbOverlaps(statics.fat[k], box)) : !queryVisible(w.shapes[t], box))
+        continue;
+      fraction = continuousCandidate(w.shapes, w.geoms, w.sims, w.joints, i, s, sweep, t, fraction);
     }
     if (bullet) {
       for (int type = kKinematic; type <= kDynamic; ++type) {
@@ -1809,14 +1821,22 @@ __global__ __launch_bounds__(BLOCK) void continuousKernel(BatchView b, const int
   const int* staticList = b.staticShapes + h->shapeOffset;
   const int staticCount = h->staticCount;
   (void)ns;
-  for (int k = threadIdx.x; k < nFast; k += BLOCK) {
-    int i = fastList[k];
-    if ((w.sims[i].flags & BF_BULLET) == 0) solveContinuous(w, i, staticList, staticCount, false);
+  // every fast body tests every static proxy: their boxes once per world in LDS
+  __shared__ StaticProxies statics;
+  for (int k = threadIdx.x; k < min(staticCount, kMaxStagedStatics); k += BLOCK) {
+    const Shape& t = w.shapes[staticList[k]];
+    statics.fat[k] = t.fat;
+    statics.visible[k] = ((t.catLo | t.catHi) != 0 && t.proxyKey >= 0) ? 1 : 0;
   }
   __syncthreads();
   for (int k = threadIdx.x; k < nFast; k += BLOCK) {
     int i = fastList[k];
-    if (w.sims[i].flags & BF_BULLET) solveContinuous(w, i, staticList, staticCount, true);
+    if ((w.sims[i].flags & BF_BULLET) == 0) solveContinuous(w, i, staticList, staticCount, false, statics);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nFast; k += BLOCK) {
+    int i = fastList[k];
+    if (w.sims[i].flags & BF_BULLET) solveContinuous(w, i, staticList, staticCount, true, statics);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -2135,7 +2155,8 @@ static int launchBlock(const BatchView* b, const StepParams* sp, const int* worl
   }
   bool ccd = (sp->phases & PH_CONTINUOUS) || ((sp->phases & PH_FINALIZE) && !(sp->phases & PH_NO_CONTINUOUS));
   if (ccd && cfg->continuous) {
-    hipLaunchKernelGGL(continuousKernel<64>, dim3(nWorlds), dim3(64), 0, stream, *b, worldList);
+    hipLaunchKernelGGL(continuousKernel<kContinuousLanes>, dim3(nWorlds), dim3(kContinuousLanes), 0, stream, *b,
+                       worldList);
   }
   return (int)hipGetLastError();
 }
