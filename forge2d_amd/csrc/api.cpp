@@ -735,6 +735,7 @@ static int commit(Batch& b) {
       c.launch.solveLds = solveLdsBytes(c.maxBodies, c.maxJoints, 1.0f, kLdsLimit);
       // one workgroup holds a whole world: the largest worlds get the most lanes
       c.launch.block = c.limit <= 128 ? 64 : (c.limit <= 512 ? 128 : (c.limit <= 1024 ? 256 : 1024));
+      if (b.forcedBlock) c.launch.block = b.forcedBlock;
       F2D_HIP(hipMalloc(&c.dList, sizeof(int) * c.count));
       F2D_HIP(hipMemcpy(c.dList, members[k].data(), sizeof(int) * c.count, hipMemcpyHostToDevice));
       F2D_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -2090,6 +2091,7 @@ int f2dg_set_block(void* h, int block) {
   Batch* b = static_cast<Batch*>(h);
   if (!b || (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)) return -1;
   b->launch.block = block;
+  b->forcedBlock = block;  // also for classes laid out later
   for (SizeClass& c : b->classes) c.launch.block = block;
   return 0;
 }

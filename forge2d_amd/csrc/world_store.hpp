@@ -101,6 +101,7 @@ struct Batch {
   long dHitCap = 0;
   int* dCounts = nullptr;
   float4* dBoxes = nullptr;  // packed fat AABBs of every shape (nShapes)
+  int forcedBlock = 0;       // f2dg_set_block before the batch is laid out: every class's size
 };
 
 }  // namespace f2d

@@ -232,7 +232,8 @@ int f2dg_autotune(void* batch, float dt, int sub_steps, int steps, float* ms_out
  * per class: [limit, worlds, max bodies, max shapes, workgroup lanes, collide LDS bytes,
  * solve LDS bytes] into out (7 ints each, up to cap classes); returns the class count */
 int f2dg_size_classes(void* batch, int32_t* out, int cap);
-/* force a workgroup size (64, 128, 256, 512 or 1024) / read the current one */
+/* force a workgroup size (64, 128, 256, 512 or 1024) for every size class, before or after the
+ * batch is laid out / read the current one */
 int f2dg_set_block(void* batch, int block);
 int f2dg_get_block(void* batch);
 /* force the workgroup size of size class `cls` (index into f2dg_size_classes) */
