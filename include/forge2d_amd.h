@@ -9,8 +9,9 @@
  *          (packages/forge2d/native/wasm/f2d_shim.c, "F/native/wasm/f2d_shim.c"
  *          below): same names, argument order and meaning, same id packing
  *          (world ids = index1 | generation << 16; body/shape/joint ids =
- *          (int32 index1, uint32 world0 | generation << 16)). The step-path
- *          subset is implemented; World.step maps to f2d_world_step, which
+ *          (int32 index1, uint32 world0 | generation << 16)). The step path,
+ *          its events, sleep and the world queries are implemented;
+ *          World.step maps to f2d_world_step, which
  *          replaces b2World_Step (packages/forge2d/third_party/box2d/src/world.c:695-812)
  *          as bound by RawBox2DFfi.worldStep (lib/src/backend/raw_box2d_ffi.dart:103-104).
  *
