@@ -21,6 +21,12 @@ Version 2 appends the island section (island graph and solver sets):
   next, head contact key); per contact slot int32[8] (island id, prev, next, set
   index, body-list edge keys prevA, nextA, prevB, nextB); per joint int32[4]
   (island id, prev, next, set index).
+Version 3 appends the broad-phase trees (static, kinematic, dynamic; the
+reference's b2DynamicTree, S/dynamic_tree.c): per tree int32 header[8] = [root,
+node count, proxy count, fresh index, freed count, 0, 0, 0], TREE_NODE 40 B records
+for node ids [0, fresh index) (free nodes zeroed with -1 links), then the freed node
+ids as a stack, bottom first (the next allocation pops the last; once empty, fresh
+ids are taken in ascending order).
 """
 from __future__ import annotations
 
@@ -76,6 +82,14 @@ LINK_DT = np.dtype([("island", "<i4"), ("prev", "<i4"), ("next", "<i4"), ("set_i
 BODY_LINK_DT = np.dtype([("island", "<i4"), ("prev", "<i4"), ("next", "<i4"), ("head_contact", "<i4")])
 CONTACT_LINK_DT = np.dtype([("island", "<i4"), ("prev", "<i4"), ("next", "<i4"), ("set_index", "<i4"),
                             ("prev_a", "<i4"), ("next_a", "<i4"), ("prev_b", "<i4"), ("next_b", "<i4")])
+# b2TreeNode, S/dynamic_tree.c:19-50: box, category bits, children (leaf: user data =
+# shape id in child1), parent, height, flags (allocated 1, enlarged 2, leaf 4)
+TREE_NODE_DT = np.dtype([
+    ("aabb", "<f4", (4,)), ("cat_lo", "<u4"), ("cat_hi", "<u4"), ("child1", "<i4"), ("child2", "<i4"),
+    ("parent", "<i4"), ("height", "<u2"), ("flags", "<u2"),
+])
+TREE_ALLOCATED, TREE_ENLARGED, TREE_LEAF = 1, 2, 4
+assert TREE_NODE_DT.itemsize == 40
 assert ISLAND_DT.itemsize == 64 and LINK_DT.itemsize == 16 and CONTACT_LINK_DT.itemsize == 32
 assert HEADER_DT.itemsize == 64 and BODY_DT.itemsize == 112 and SHAPE_DT.itemsize == 48
 assert POINT_DT.itemsize == 52 and CONTACT_DT.itemsize == 168 and JOINT_DT.itemsize == 64
@@ -92,9 +106,31 @@ class Islands:
                  "body_links", "contact_links", "joint_links")
 
 
+class Tree:
+    """One broad-phase tree of a version-3 snapshot."""
+    __slots__ = ("root", "node_count", "proxy_count", "fresh", "nodes", "freed")
+
+    def leaf_order(self) -> np.ndarray:
+        """Shape ids of the leaves in the order a query visits them (child2 before
+        child1, S/dynamic_tree.c:1114-1170)."""
+        out = []
+        if self.root < 0:
+            return np.zeros(0, np.int32)
+        stack = [self.root]
+        while stack:
+            n = stack.pop()
+            rec = self.nodes[n]
+            if rec["flags"] & TREE_LEAF:
+                out.append(int(rec["child1"]))
+            else:
+                stack.append(int(rec["child1"]))
+                stack.append(int(rec["child2"]))
+        return np.array(out, np.int32)
+
+
 class Snapshot:
     __slots__ = ("header", "bodies", "shapes", "contacts", "joints", "colors", "move", "free_ids", "islands",
-                 "nbytes")
+                 "trees", "nbytes")
 
     @property
     def step(self) -> int:
@@ -142,10 +178,34 @@ def parse_one(buf, off: int = 0) -> Snapshot:
     s.free_ids = np.frombuffer(buf, "<i4", int(h["free_id_count"]), o)
     o += s.free_ids.nbytes
     s.islands = None
+    s.trees = None
     if int(h["version"]) >= 2:
         s.islands, o = _parse_islands(buf, o, len(s.bodies), len(s.contacts), len(s.joints))
+    if int(h["version"]) >= 3:
+        s.trees = []
+        for _ in range(3):
+            t, o = _parse_tree(buf, o)
+            s.trees.append(t)
     s.nbytes = o - off
     return s
+
+
+def _parse_tree(buf, o):
+    hd = np.frombuffer(buf, "<i4", 8, o)
+    o += 32
+    t = Tree()
+    t.root, t.node_count, t.proxy_count, t.fresh = (int(x) for x in hd[:4])
+    t.nodes = np.frombuffer(buf, TREE_NODE_DT, t.fresh, o)
+    o += t.nodes.nbytes
+    t.freed = np.frombuffer(buf, "<i4", int(hd[4]), o)
+    o += t.freed.nbytes
+    return t, o
+
+
+def _tree_bytes(t: Tree) -> bytes:
+    hd = np.zeros(8, "<i4")
+    hd[:5] = [t.root, t.node_count, t.proxy_count, t.fresh, len(t.freed)]
+    return hd.tobytes() + np.ascontiguousarray(t.nodes).tobytes() + np.asarray(t.freed, "<i4").tobytes()
 
 
 def _parse_islands(buf, o, nb, nc, nj):
@@ -228,7 +288,8 @@ def to_bytes(s: Snapshot) -> bytes:
     parts += [np.asarray(c, "<i4").tobytes() for c in s.colors]
     parts += [np.asarray(s.move, "<i4").tobytes(), np.asarray(s.free_ids, "<i4").tobytes()]
     isl = getattr(s, "islands", None)
-    h["version"] = 2 if isl is not None else 1
+    trees = getattr(s, "trees", None)
+    h["version"] = (3 if trees is not None else 2) if isl is not None else 1
     parts[0] = h.tobytes()
     if isl is not None:
         hd = np.zeros(16, "<i4")
@@ -238,6 +299,8 @@ def to_bytes(s: Snapshot) -> bytes:
         for a in (isl.islands, isl.free_islands, isl.awake_islands, isl.free_sets, isl.awake_bodies,
                   isl.body_links, isl.contact_links, isl.joint_links):
             parts.append(np.ascontiguousarray(a).tobytes())
+        if trees is not None:
+            parts += [_tree_bytes(t) for t in trees]
     return b"".join(parts)
 
 
@@ -252,6 +315,7 @@ def copy(s: Snapshot) -> Snapshot:
     o.move = np.array(s.move, "<i4")
     o.free_ids = np.array(s.free_ids, "<i4")
     o.islands = getattr(s, "islands", None)
+    o.trees = getattr(s, "trees", None)
     o.nbytes = s.nbytes
     return o
 
@@ -310,4 +374,5 @@ def post_collide(prev: Snapshot, cur: Snapshot) -> Snapshot:
     # the island graph after the collide is not part of either snapshot: the
     # composed state carries none (version 1), for worlds with sleep off
     s.islands = None
+    s.trees = None
     return s

@@ -60,7 +60,12 @@ void b2o_destroy(void* w);
 /* Contacts the broad phase creates at the start of the next step, in creation
  * order: ids must be the id pool's LIFO allocations (checked; -1 on mismatch). */
 int b2o_create_contacts(void* w, const int32_t* ids, const int32_t* shapeA, const int32_t* shapeB, int n);
-/* b2Collide without the pair update: manifolds and the contact state loop. */
+/* b2UpdateBroadPhasePairs from a version-3 snapshot's trees and move array: the
+ * contacts the step's pair update creates, ids from the pool. Returns their count
+ * (-1 without trees). */
+int b2o_update_pairs(void* w);
+/* b2Collide without the pair update: the tree rebuild (version 3), manifolds and
+ * the contact state loop. */
 int b2o_collide(void* w);
 /* b2Solve: solver stages, finalize, continuous collision, enlarged proxies. */
 int b2o_solve(void* w, float dt, int subSteps);

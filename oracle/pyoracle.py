@@ -5,7 +5,8 @@ module, as the checker. It loads oracle/_build/liboracle_{f32,f64}.so, built by
 ``make -C oracle port``.
 
     o = Oracle(spec, geoms, snapshot_bytes, precision="f32")
-    o.create_contacts(created)      # contacts the broad phase creates this step
+    o.update_pairs()                # the pair update over the snapshot's trees (version 3)
+    o.create_contacts(created)      # or: contacts the broad phase creates this step, given
     o.collide()                     # manifolds + contact state loop + colouring
     o.solve(dt, substeps)           # solver stages, finalize, continuous, moves
     o.export() -> snapshot bytes;  o.bodies() -> (n, 7) float64
@@ -37,6 +38,8 @@ def _lib(precision: str):
         L.b2o_create.restype = vp
         L.b2o_create.argtypes = [vp, L_, ctypes.c_int, vp, L_, vp, L_]
         L.b2o_destroy.argtypes = [vp]
+        L.b2o_update_pairs.restype = ctypes.c_int
+        L.b2o_update_pairs.argtypes = [vp]
         L.b2o_create_contacts.restype = ctypes.c_int
         L.b2o_create_contacts.argtypes = [vp, vp, vp, vp, ctypes.c_int]
         L.b2o_collide.restype = ctypes.c_int
@@ -94,6 +97,13 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle: created contact ids are not the id pool's allocations")
 
+    def update_pairs(self) -> int:
+        """The step's pair update over the trees: contacts created, ids from the pool."""
+        n = self.L.b2o_update_pairs(self.h)
+        if n < 0:
+            raise RuntimeError("oracle: the snapshot holds no broad-phase trees (version < 3)")
+        return n
+
     def collide(self):
         self.L.b2o_collide(self.h)
 
@@ -138,10 +148,13 @@ class Oracle:
 
 def step(spec: bytes, geoms: bytes, before_raw: bytes, before, after=None, precision: str = "f32",
          dt: float = 1.0 / 60.0, substeps: int = 4) -> "Oracle":
-    """One full step from snapshot ``before``: the contacts the reference created in
-    that step (read from ``after``) are created first, then collide and solve."""
+    """One full step from snapshot ``before``: the pair update over its trees when it
+    has them (version 3), else the contacts created in that step as ``after`` records
+    them; then collide and solve."""
     o = Oracle(spec, geoms, before_raw, precision)
-    if after is not None:
+    if getattr(before, "trees", None) is not None:
+        o.update_pairs()
+    elif after is not None:
         o.create_contacts(created_contacts(before, after))
     o.collide()
     o.solve(dt, substeps)

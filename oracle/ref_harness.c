@@ -13,6 +13,8 @@
 //   contacts            S/contact.h:60-131
 //   constraint graph    S/constraint_graph.h:22-45
 //   broad-phase moves   S/broad_phase.h:74-82, contact id pool S/id_pool.h:8-12
+//   broad-phase trees   I/collision.h:621-655 (b2DynamicTree), node record of
+//                       S/dynamic_tree.c:19-50 (read through TreeNodeRec below)
 //
 // Modes:
 //   ref_harness dump  <scene> <out> <world> <steps> <k1,k2,...>
@@ -380,12 +382,75 @@ static void write_islands( FILE* f, b2World* world )
 	}
 }
 
+// The 40-byte node record of S/dynamic_tree.c:19-50 (its type is private to that file):
+// box, category bits, children or user data, parent or free-list next, height, flags.
+typedef struct
+{
+	float aabb[4];
+	uint64_t categoryBits;
+	int32_t child1, child2;
+	int32_t parent;
+	uint16_t height, flags;
+} TreeNodeRec;
+_Static_assert( sizeof( TreeNodeRec ) == 40, "tree node record" );
+
+// Tree section of a version-3 snapshot: for the static, kinematic and dynamic trees
+// (S/broad_phase.h:56-60) an int32 header [root, node count, proxy count, fresh index,
+// freed count, 0, 0, 0], the node records [0, fresh index), then the freed node ids
+// as a stack (bottom first, the next allocation last). The node free list of
+// S/dynamic_tree.c:122-170 is those freed ids above the never-used ids [fresh index,
+// capacity) in ascending order, so this pair describes it exactly.
+static void write_trees( FILE* f, b2World* world )
+{
+	for ( int t = 0; t < b2_bodyTypeCount; ++t )
+	{
+		b2DynamicTree* tree = world->broadPhase.trees + t;
+		const TreeNodeRec* nodes = (const TreeNodeRec*)tree->nodes;
+		int cap = tree->nodeCapacity;
+		int* list = malloc( sizeof( int ) * ( cap + 1 ) );
+		int n = 0;
+		for ( int k = tree->freeList; k != B2_NULL_INDEX && n < cap; k = nodes[k].parent )
+		{
+			list[n++] = k;
+		}
+		int fresh = cap, freed = n;
+		if ( n > 0 && list[n - 1] == cap - 1 )
+		{
+			int j = n - 1;
+			while ( j > 0 && list[j - 1] == list[j] - 1 )
+			{
+				j -= 1;
+			}
+			fresh = list[j];
+			freed = j;
+		}
+		int32_t hdr[8] = { tree->root, tree->nodeCount, tree->proxyCount, fresh, freed, 0, 0, 0 };
+		fwrite( hdr, 4, 8, f );
+		for ( int i = 0; i < fresh; ++i )
+		{
+			TreeNodeRec r = nodes[i];
+			if ( ( r.flags & b2_allocatedNode ) == 0 )
+			{
+				// a free node: only its place in the free list is state
+				memset( &r, 0, sizeof( r ) );
+				r.child1 = r.child2 = r.parent = B2_NULL_INDEX;
+			}
+			fwrite( &r, sizeof( r ), 1, f );
+		}
+		for ( int i = freed - 1; i >= 0; --i )
+		{
+			fwrite( list + i, 4, 1, f );
+		}
+		free( list );
+	}
+}
+
 static void write_snapshot( FILE* f, b2WorldId wid, int step )
 {
 	b2World* world = b2GetWorldFromId( wid );
 	SnapHeader h = { 0 };
 	h.magic = 0x50414E53;
-	h.version = 2;
+	h.version = 3;
 	h.step = step;
 	h.body_count = world->bodies.count;
 	h.shape_count = world->shapes.count;
@@ -552,6 +617,7 @@ static void write_snapshot( FILE* f, b2WorldId wid, int step )
 	fwrite( world->broadPhase.moveArray.data, 4, world->broadPhase.moveArray.count, f );
 	fwrite( world->contactIdPool.freeArray.data, 4, world->contactIdPool.freeArray.count, f );
 	write_islands( f, world );
+	write_trees( f, world );
 }
 
 static int parse_list( const char* s, int* out, int cap )

@@ -118,6 +118,33 @@ def island_differences(mine: S.Snapshot, ref: S.Snapshot) -> list:
     return out
 
 
+def tree_differences(mine: S.Snapshot, ref: S.Snapshot, box_tol: float = 0.0) -> list:
+    """Names of the broad-phase tree quantities that differ between two version-3
+    snapshots, per tree (static, kinematic, dynamic): root, node and proxy counts,
+    the node free list (freed stack and fresh index), every node's links, height,
+    flags and category bits, and its box (exactly, or within ``box_tol`` absolute)."""
+    out = []
+    if mine.trees is None or ref.trees is None:
+        return ["tree section"] if (mine.trees is None) != (ref.trees is None) else []
+    for k, (a, b) in enumerate(zip(mine.trees, ref.trees)):
+        name = ("static", "kinematic", "dynamic")[k]
+        for f in ("root", "node_count", "proxy_count", "fresh"):
+            if getattr(a, f) != getattr(b, f):
+                out.append(f"{name}.{f}")
+        if not np.array_equal(a.freed, b.freed):
+            out.append(f"{name}.freed")
+        if len(a.nodes) != len(b.nodes):
+            out.append(f"{name}.nodes")
+            continue
+        for f in ("child1", "child2", "parent", "height", "flags", "cat_lo", "cat_hi"):
+            if not np.array_equal(a.nodes[f], b.nodes[f]):
+                out.append(f"{name}.{f}")
+        d = np.abs(a.nodes["aabb"].astype(np.float64) - b.nodes["aabb"].astype(np.float64))
+        if d.max(initial=0.0) > box_tol:
+            out.append(f"{name}.aabb")
+    return out
+
+
 QUERY_FIXTURES = ["queries", "queries_sleep", "queries_wake2"]
 # fixtures whose explosion wakes several sleeping sets: the reference wakes them in its
 # dynamic tree's order, the restatements in shape-id order

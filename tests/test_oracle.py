@@ -77,6 +77,7 @@ def test_oracle_step_reproduces_reference(name, i):
     rep = P.manifold_report(mine, ref, with_impulses=True)
     assert rep["point_count"] == rep["ids"] == rep["normals"] == rep["values"] == 0, rep
     assert P.island_differences(mine, ref) == []
+    assert P.tree_differences(mine, ref, 0.0 if not fast.any() else TOL) == []
     if not fast.any():
         assert rep["impulses"] == 0, rep
         assert np.array_equal(mine.move, ref.move)
