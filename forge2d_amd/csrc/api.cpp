@@ -209,10 +209,9 @@ int HostWorld::createShape(int body, const ShapeGeom& g, float density, float fr
   sh.fat.lo.y = aabb.lo.y - margin;
   sh.fat.hi.x = aabb.hi.x + margin;
   sh.fat.hi.y = aabb.hi.y + margin;
-  // proxy id of a fresh leaf: the tree allocates the leaf, then a new parent
-  // (S/dynamic_tree.c:122-158, 625-660, 770-790)
-  int proxyId = treeNodes[type];
-  treeNodes[type] += treeNodes[type] == 0 ? 1 : 2;
+  // b2BroadPhase_CreateProxy (S/broad_phase.c:89-100): a leaf with the fat box in the
+  // body type's tree, inserted with rotations (S/dynamic_tree.c:772-795)
+  const int proxyId = treeCreateProxy(treeRef(type, 2), sh.fat, (int)shapes.size());
   sh.proxyKey = (proxyId << 2) | type;
   if (type != kStatic || invokeContact || (flags & SF_SENSOR)) {
     sh.flags |= SF_MOVED;
@@ -226,6 +225,39 @@ int HostWorld::createShape(int body, const ShapeGeom& g, float density, float fr
   sims[body].headShape = id;
   if (updateMass) updateBodyMass(body);
   return id;
+}
+
+TreeRef HostWorld::treeRef(int type, int extra) {
+  TreeHeader& h = tree[type];
+  const size_t need = (size_t)std::max(h.fresh + extra, 1);
+  if (tnodes[type].size() < need) {
+    const size_t cap = std::max(need, tnodes[type].size() + tnodes[type].size() / 2 + 16);
+    tnodes[type].resize(cap);
+    tfreed[type].resize(cap);
+  }
+  h.cap = (int)tnodes[type].size();
+  return TreeRef{&h, tnodes[type].data(), tfreed[type].data()};
+}
+
+// The step enlarged these leaves (b2BroadPhase_EnlargeProxy, S/broad_phase.c:125-135)
+// and left them to the next tree pass; the resulting tree does not depend on the
+// order they are applied in.
+void HostWorld::applyPendingEnlarges() {
+  for (Shape& sh : shapes) {
+    if ((sh.flags & SF_TREE_PENDING) == 0) continue;
+    sh.flags &= ~SF_TREE_PENDING;
+    if (sh.proxyKey < 0) continue;
+    treeEnlargeProxy(treeRef(sh.proxyKey & 3), sh.proxyKey >> 2, sh.fat);
+  }
+}
+
+void HostWorld::computeTreeRanks() {
+  std::vector<int> rank(shapes.size(), 0), walk;
+  for (int t = 0; t < 3; ++t) {
+    walk.resize((size_t)tree[t].fresh + 2);
+    if (tree[t].root >= 0 && !tnodes[t].empty()) treeLeafRanks(tnodes[t].data(), tree[t].root, walk.data(), rank.data());
+  }
+  for (size_t s = 0; s < shapes.size(); ++s) shapes[s].treeRank = rank[s];
 }
 
 // b2CreateJoint + b2CreateRevoluteJoint + b2AssignJointColor, S/joint.c:146-280, 507-560,
@@ -586,9 +618,16 @@ static int solveLdsBytes(int maxBodies, int maxJoints, float lanesPerBody, int l
 
 static int commit(Batch& b) {
   freeDevice(b);
-  long nb = 0, ns = 0, nc = 0, nj = 0, ncol = 0, nscratch = 0, nisl = 0;
+  long nb = 0, ns = 0, nc = 0, nj = 0, ncol = 0, nscratch = 0, nisl = 0, ntree = 0, nclist = 0;
   for (HostWorld& hw : b.worlds) {
     hw.ensureColorWords();
+    // tree regions: a step's rebuild allocates exactly the nodes it frees, so the
+    // node count a tree reaches on the host bounds it on the device
+    for (int t = 0; t < 3; ++t) {
+      hw.treeRef(t, 16);
+      hw.tree[t].offset = (int)ntree;
+      ntree += hw.tree[t].cap;
+    }
     WorldHeader& h = hw.hdr;
     // island region: one island per non-static body at most
     h.islandCap = std::max(1, (int)hw.islands.size());
@@ -600,6 +639,8 @@ static int commit(Batch& b) {
     hw.moveOrder.resize(hw.sims.size());
     h.islandOffset = (int)nisl;
     nisl += islandRegion(h.islandCap, (int)hw.sims.size()).total;
+    h.colorListOffset = (int)nclist;
+    nclist += colorListInts((int)hw.sims.size());
     h.bodyOffset = (int)nb;
     h.bodyCount = (int)hw.sims.size();
     h.shapeOffset = (int)ns;
@@ -658,7 +699,10 @@ static int commit(Batch& b) {
       devAlloc(d, &v.colorBits, ncol) || devAlloc(d, &v.cons, (size_t)nc * CN_FIELD_COUNT) ||
       devAlloc(d, &v.scratch, nscratch) || devAlloc(d, &v.ccolor, nc) || devAlloc(d, &v.staticShapes, ns) || devAlloc(d, &v.sweepOrder, ns) ||
       devAlloc(d, &v.events, 3 * nc) || devAlloc(d, &v.islandData, std::max(nisl, 4L)) ||
-      devAlloc(d, &v.contactLinks, nc)) {
+      devAlloc(d, &v.contactLinks, nc) || devAlloc(d, &v.treeHdrs, 3L * W) ||
+      devAlloc(d, &v.colorLists, std::max(nclist, 1L)) ||
+      devAlloc(d, &v.treeNodes, std::max(ntree, 1L)) || devAlloc(d, &v.treeFreed, std::max(ntree, 1L)) ||
+      devAlloc(d, &v.treeWork, 10 * std::max(ntree, 1L))) {
     b.lastError = "device allocation failed";
     freeDevice(b);
     return -1;
@@ -671,6 +715,8 @@ static int commit(Batch& b) {
   b.nJoints = nj;
   b.nColorWords = ncol;
   b.nIsland = std::max(nisl, 4L);
+  b.nTreeNodes = std::max(ntree, 1L);
+  b.nColorListInts = std::max(nclist, 1L);
   // dynamic LDS per kernel: sort-and-sweep (28 B per shape, power-of-two padded)
   // and new-pair sort (12 B per pair), body states (32 B per body); worlds that do
   // not fit fall back to their global scratch inside the kernel
@@ -792,10 +838,15 @@ static int upload(Batch& b) {
     totalIsland = std::max<long>(totalIsland, hw.hdr.islandOffset + islandRegion(hw.hdr.islandCap, hw.hdr.bodyCount).total);
   std::vector<int> islandData(totalIsland, 0);
   std::vector<int4> links(std::max<long>(totalContacts, 1), make_int4(-1, -1, -1, -1));
+  std::vector<TreeHeader> treeHdrs(3 * b.worlds.size());
+  std::vector<TreeNode> treeNodes(b.nTreeNodes);
+  std::vector<int> treeFreed(b.nTreeNodes, 0);
+  std::vector<int> colorLists(b.nColorListInts, 0);
   b.launch.continuous = 0;
   for (size_t i = 0; i < b.worlds.size(); ++i) {
     HostWorld& hw = b.worlds[i];
     WorldHeader& h = hw.hdr;
+    hw.computeTreeRanks();
     if (h.flags & (WF_CONTINUOUS | WF_SLEEP)) b.launch.continuous = 1;  // the last kernel also runs island sleep
     h.fastCount = 0;
     h.staticCount = 0;
@@ -839,7 +890,24 @@ static int upload(Batch& b) {
     if (!hw.joints.empty())
       F2D_HIP(hipMemcpyAsync(v.joints + h.jointOffset, hw.joints.data(), sizeof(Joint) * hw.joints.size(), hipMemcpyHostToDevice, s));
     F2D_HIP(hipMemcpyAsync(v.colorBits + h.colorOffset, hw.colorBits.data(), sizeof(uint32_t) * hw.colorBits.size(), hipMemcpyHostToDevice, s));
+    {
+      int* cl = colorLists.data() + h.colorListOffset;
+      for (int c = 0; c < kOverflowIndex; ++c) {
+        cl[c] = (int)hw.colorList[c].size();
+        std::copy(hw.colorList[c].begin(), hw.colorList[c].end(), cl + kColorListHeader + (long)c * h.bodyCount);
+      }
+    }
+    for (int t = 0; t < 3; ++t) {
+      const TreeHeader& th = hw.tree[t];
+      treeHdrs[3 * i + t] = th;
+      std::copy(hw.tnodes[t].begin(), hw.tnodes[t].begin() + th.cap, treeNodes.begin() + th.offset);
+      std::copy(hw.tfreed[t].begin(), hw.tfreed[t].begin() + th.cap, treeFreed.begin() + th.offset);
+    }
   }
+  F2D_HIP(hipMemcpyAsync(v.colorLists, colorLists.data(), sizeof(int) * colorLists.size(), hipMemcpyHostToDevice, b.stream));
+  F2D_HIP(hipMemcpyAsync(v.treeHdrs, treeHdrs.data(), sizeof(TreeHeader) * treeHdrs.size(), hipMemcpyHostToDevice, b.stream));
+  F2D_HIP(hipMemcpyAsync(v.treeNodes, treeNodes.data(), sizeof(TreeNode) * treeNodes.size(), hipMemcpyHostToDevice, b.stream));
+  F2D_HIP(hipMemcpyAsync(v.treeFreed, treeFreed.data(), sizeof(int) * treeFreed.size(), hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipMemcpyAsync(v.worlds, headers.data(), sizeof(WorldHeader) * headers.size(), hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipMemcpyAsync(v.ccolor, ccolor.data(), ccolor.size(), hipMemcpyHostToDevice, b.stream));
   F2D_HIP(hipMemcpyAsync(v.staticShapes, staticShapes.data(), sizeof(int) * totalShapes, hipMemcpyHostToDevice, b.stream));
@@ -867,6 +935,14 @@ static int download(Batch& b) {
   std::vector<Joint> joints(b.nJoints);
   std::vector<uint32_t> colorBits(b.nColorWords);
   std::vector<int4> links(b.nContacts);
+  std::vector<TreeHeader> treeHdrs(3 * b.worlds.size());
+  std::vector<TreeNode> treeNodes(b.nTreeNodes);
+  std::vector<int> treeFreed(b.nTreeNodes);
+  std::vector<int> colorLists(b.nColorListInts);
+  F2D_HIP(hipMemcpyAsync(colorLists.data(), v.colorLists, sizeof(int) * colorLists.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(treeHdrs.data(), v.treeHdrs, sizeof(TreeHeader) * treeHdrs.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(treeNodes.data(), v.treeNodes, sizeof(TreeNode) * treeNodes.size(), hipMemcpyDeviceToHost, s));
+  F2D_HIP(hipMemcpyAsync(treeFreed.data(), v.treeFreed, sizeof(int) * treeFreed.size(), hipMemcpyDeviceToHost, s));
   F2D_HIP(hipMemcpyAsync(headers.data(), v.worlds, sizeof(WorldHeader) * headers.size(), hipMemcpyDeviceToHost, s));
   F2D_HIP(hipMemcpyAsync(sims.data(), v.sims, sizeof(BodySim) * sims.size(), hipMemcpyDeviceToHost, s));
   F2D_HIP(hipMemcpyAsync(states.data(), v.states, sizeof(BodyState) * states.size(), hipMemcpyDeviceToHost, s));
@@ -894,6 +970,7 @@ static int download(Batch& b) {
     std::copy(colorBits.begin() + h.colorOffset, colorBits.begin() + h.colorOffset + hw.colorBits.size(),
               hw.colorBits.begin());
     const int* ev = events.data() + 3L * h.contactOffset;
+    hw.eventsRetired = false;
     hw.beginIds.assign(ev, ev + h.beginCount);
     hw.endPairs.assign(ev + h.contactCap, ev + h.contactCap + 2 * h.endCount);
     {
@@ -905,6 +982,21 @@ static int download(Batch& b) {
       std::memcpy(hw.setFree.data(), base + r.setFree, sizeof(int) * (h.islandCap + 4));
       std::memcpy(hw.awakeBodies.data(), base + r.awakeBodies, sizeof(int) * h.bodyCount);
       std::memcpy(hw.moveOrder.data(), base + r.moveOrder, sizeof(int) * h.bodyCount);
+    }
+    for (int t = 0; t < 3; ++t) {
+      const TreeHeader& th = treeHdrs[3 * i + t];
+      hw.tree[t] = th;
+      hw.tnodes[t].assign(treeNodes.begin() + th.offset, treeNodes.begin() + th.offset + th.cap);
+      hw.tfreed[t].assign(treeFreed.begin() + th.offset, treeFreed.begin() + th.offset + th.cap);
+    }
+    // the host mirror holds the trees as the reference has them after the step
+    hw.applyPendingEnlarges();
+    {
+      const int* cl = colorLists.data() + h.colorListOffset;
+      for (int c = 0; c < kOverflowIndex; ++c) {
+        const int* first = cl + kColorListHeader + (long)c * h.bodyCount;
+        hw.colorList[c].assign(first, first + cl[c]);
+      }
     }
     hw.hdr = h;
     hw.contactCount = h.contactCount;
@@ -985,6 +1077,16 @@ struct SnapJoint {
   float linear_impulse_x, linear_impulse_y, spring_impulse, motor_impulse, lower_impulse, upper_impulse, ref_angle,
       reserved;
 };
+
+// b2TreeNode as a snapshot record (S/dynamic_tree.c:19-50): box, category bits,
+// children (a leaf's shape id in child1), parent, height, flags
+struct SnapTreeNode {
+  float box[4];
+  uint64_t cat;
+  int32_t child1, child2, parent;
+  uint16_t height, flags;
+};
+static_assert(sizeof(SnapTreeNode) == 40, "snapshot tree node");
 #pragma pack(pop)
 static_assert(sizeof(SnapHeader) == 64 && sizeof(SnapBody) == 112 && sizeof(SnapShape) == 48, "snap layout");
 static_assert(sizeof(SnapContact) == 168 && sizeof(SnapJoint) == 64, "snap layout");
@@ -995,10 +1097,38 @@ static void put(std::vector<unsigned char>& out, const T& v) {
   out.insert(out.end(), p, p + sizeof(T));
 }
 
+// Category bits of every node of a tree: a leaf's are its shape's, an internal
+// node's the union of its children's (what the reference's node records hold).
+static void treeCategories(const HostWorld& hw, int t, std::vector<uint64_t>& cat) {
+  const TreeHeader& th = hw.tree[t];
+  cat.assign((size_t)std::max(th.fresh, 1), 0);
+  if (th.root < 0) return;
+  std::vector<int> order, walk{th.root};
+  while (!walk.empty()) {
+    const int i = walk.back();
+    walk.pop_back();
+    order.push_back(i);
+    const TreeNode& n = hw.tnodes[t][i];
+    if (!(n.flags & TN_LEAF)) {
+      walk.push_back(n.child1);
+      walk.push_back(n.child2);
+    }
+  }
+  for (auto it = order.rbegin(); it != order.rend(); ++it) {
+    const TreeNode& n = hw.tnodes[t][*it];
+    if (n.flags & TN_LEAF) {
+      const Shape& sh = hw.shapes[n.child1];
+      cat[*it] = (uint64_t)sh.catLo | ((uint64_t)sh.catHi << 32);
+    } else {
+      cat[*it] = cat[n.child1] | cat[n.child2];
+    }
+  }
+}
+
 static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
   SnapHeader h = {};
   h.magic = 0x50414E53;
-  h.version = 2;
+  h.version = 3;
   h.step = hw.stepIndex;
   h.body_count = (int)hw.sims.size();
   h.shape_count = (int)hw.shapes.size();
@@ -1006,10 +1136,7 @@ static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
   h.joint_count = (int)hw.joints.size();
   h.overflow_count = (int)hw.overflow.size();
   std::vector<int> colorLists[kGraphColorCount];
-  for (int i = 0; i < hw.contactCount; ++i) {
-    const Contact& c = hw.contacts[i];
-    if (c.shapeA >= 0 && c.color >= 0 && c.color < kOverflowIndex) colorLists[c.color].push_back(i);
-  }
+  for (int c = 0; c < kOverflowIndex; ++c) colorLists[c] = hw.colorList[c];
   colorLists[kOverflowIndex] = hw.overflow;
   int total = 0;
   for (auto& l : colorLists) total += (int)l.size();
@@ -1171,12 +1298,108 @@ static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
     int32_t r[4] = {j.islandId, j.islandPrev, j.islandNext, j.setIndex};
     for (int32_t x : r) put(out, x);
   }
+  // tree section (version 3): header, node records of ids [0, fresh), freed-id stack
+  std::vector<uint64_t> cat;
+  for (int t = 0; t < 3; ++t) {
+    const TreeHeader& th = hw.tree[t];
+    treeCategories(hw, t, cat);
+    int32_t hd[8] = {th.root, th.nodeCount, th.proxyCount, th.fresh, th.freedCount, 0, 0, 0};
+    for (int32_t x : hd) put(out, x);
+    for (int i = 0; i < th.fresh; ++i) {
+      const TreeNode& n = hw.tnodes[t][i];
+      SnapTreeNode r = {};
+      r.child1 = r.child2 = r.parent = -1;
+      if (n.flags & TN_ALLOCATED) {
+        r.box[0] = n.box.lo.x;
+        r.box[1] = n.box.lo.y;
+        r.box[2] = n.box.hi.x;
+        r.box[3] = n.box.hi.y;
+        r.cat = cat[i];
+        r.child1 = n.child1;
+        r.child2 = n.child2;
+        r.parent = n.parent;
+        r.height = n.height;
+        r.flags = n.flags;
+      }
+      put(out, r);
+    }
+    for (int k = 0; k < th.freedCount; ++k) put(out, (int32_t)hw.tfreed[t][k]);
+  }
+}
+
+// The tree section of a version-3 snapshot; older snapshots keep the trees this
+// world was built with, their leaves set to the snapshot's fat boxes.
+static int importTrees(HostWorld& hw, const SnapHeader& h, const unsigned char* buf, long& off, long bytes,
+                       std::string& err) {
+  for (Shape& sh : hw.shapes) sh.flags &= ~SF_TREE_PENDING;
+  if (h.version < 3) {
+    for (int t = 0; t < 3; ++t) {
+      TreeRef r = hw.treeRef(t);
+      for (int i = 0; i < r.h->fresh; ++i) {
+        TreeNode& n = r.n[i];
+        n.flags &= ~TN_ENLARGED;
+        if ((n.flags & TN_ALLOCATED) && (n.flags & TN_LEAF)) n.box = hw.shapes[n.child1].fat;
+      }
+      // internal boxes from their children, deepest first
+      std::vector<int> order, walk;
+      if (r.h->root >= 0) walk.push_back(r.h->root);
+      while (!walk.empty()) {
+        const int i = walk.back();
+        walk.pop_back();
+        order.push_back(i);
+        if (!(r.n[i].flags & TN_LEAF)) {
+          walk.push_back(r.n[i].child1);
+          walk.push_back(r.n[i].child2);
+        }
+      }
+      for (auto it = order.rbegin(); it != order.rend(); ++it)
+        if (!(r.n[*it].flags & TN_LEAF)) refitNode(r.n, *it);
+    }
+    return 0;
+  }
+  for (int t = 0; t < 3; ++t) {
+    int32_t hd[8];
+    if (off + 32 > bytes) {
+      err = "snapshot tree section truncated";
+      return -1;
+    }
+    std::memcpy(hd, buf + off, 32);
+    off += 32;
+    if (hd[3] < 0 || hd[4] < 0 || hd[4] > hd[3] || hd[2] != hw.tree[t].proxyCount ||
+        off + (long)sizeof(SnapTreeNode) * hd[3] + 4L * hd[4] > bytes) {
+      err = "snapshot tree section does not match this world's proxies";
+      return -1;
+    }
+    TreeHeader& th = hw.tree[t];
+    th.root = hd[0];
+    th.nodeCount = hd[1];
+    th.proxyCount = hd[2];
+    th.fresh = hd[3];
+    th.freedCount = hd[4];
+    th.error = 0;
+    TreeRef r = hw.treeRef(t, 16);
+    for (int i = 0; i < th.fresh; ++i) {
+      SnapTreeNode rec;
+      std::memcpy(&rec, buf + off, sizeof(rec));
+      off += sizeof(rec);
+      TreeNode& n = r.n[i];
+      n.box = {{rec.box[0], rec.box[1]}, {rec.box[2], rec.box[3]}};
+      n.child1 = rec.child1;
+      n.child2 = rec.child2;
+      n.parent = rec.parent;
+      n.height = rec.height;
+      n.flags = rec.flags;
+    }
+    std::memcpy(r.freed, buf + off, 4L * th.freedCount);
+    off += 4L * th.freedCount;
+  }
+  return 0;
 }
 
 // The island section of a version-2 snapshot: island records, id pools, the awake
 // set's orders, body sets and island links, contact sets, island links and body-list
 // edges, joint sets and island links.
-static int importIslands(HostWorld& hw, const SnapHeader& h, const unsigned char* buf, long off, long bytes,
+static int importIslands(HostWorld& hw, const SnapHeader& h, const unsigned char* buf, long& off, long bytes,
                          std::string& err) {
   auto readInts = [&](int32_t* dst, long n) -> bool {
     if (n < 0 || off + 4 * n > bytes) return false;
@@ -1416,6 +1639,7 @@ static int importWorld(HostWorld& hw, const unsigned char* buf, long bytes, std:
   std::fill(hw.colorBits.begin(), hw.colorBits.end(), 0u);
   auto setBit = [&](int c, int b) { hw.colorBits[(size_t)c * hw.colorWords + (b >> 5)] |= 1u << (b & 31); };
   hw.overflow.clear();
+  for (auto& l : hw.colorList) l.clear();
   for (int c = 0; c < kGraphColorCount; ++c) {
     for (int k = 0; k < counts[c]; ++k) {
       int32_t id;
@@ -1426,7 +1650,8 @@ static int importWorld(HostWorld& hw, const unsigned char* buf, long bytes, std:
         ct.localIndex = (int)hw.overflow.size();
         hw.overflow.push_back(id);
       } else {
-        ct.localIndex = -1;
+        ct.localIndex = (int)hw.colorList[c].size();
+        hw.colorList[c].push_back(id);
         if (hw.sims[ct.bodyA].type != kStatic) setBit(c, ct.bodyA);
         if (hw.sims[ct.bodyB].type != kStatic) setBit(c, ct.bodyB);
       }
@@ -1460,8 +1685,8 @@ static int importWorld(HostWorld& hw, const unsigned char* buf, long bytes, std:
   hw.inv_h = h.inv_h;
   hw.links.assign(h.contact_cap, make_int4(-1, -1, -1, -1));
   for (int id : live) hw.links[id] = make_int4(-1, -1, -1, kAwakeSet);
-  if (h.version >= 2) return importIslands(hw, h, buf, off, bytes, err);
-  return 0;
+  if (h.version >= 2 && importIslands(hw, h, buf, off, bytes, err) != 0) return -1;
+  return importTrees(hw, h, buf, off, bytes, err);
 }
 
 // ---------------------------------------------------------------------------- handle tables
@@ -1872,6 +2097,7 @@ static int endEvents(Batch& b, int world, double* out, uint32_t shapeWg) {
 static int hitEvents(Batch& b, int world, double* out, uint32_t shapeWg) {
   download(b);
   HostWorld& hw = b.worlds[world];
+  if (hw.eventsRetired) return 0;
   const float threshold = hw.hdr.hitEventThreshold;
   int n = 0;
   auto visit = [&](int id) {
@@ -1904,9 +2130,9 @@ static int hitEvents(Batch& b, int world, double* out, uint32_t shapeWg) {
     }
     ++n;
   };
+  // colour by colour in array order, the overflow colour last (S/solver.c:1768-1816)
   for (int color = 0; color < kOverflowIndex; ++color)
-    for (int id = 0; id < hw.contactCount; ++id)
-      if (hw.contacts[id].shapeA >= 0 && hw.contacts[id].color == color) visit(id);
+    for (int id : hw.colorList[color]) visit(id);
   for (int id : hw.overflow) visit(id);
   return n;
 }
@@ -2185,12 +2411,37 @@ int f2d_world_is_valid(uint32_t w) { return batchOf(w) != nullptr; }
 void f2d_world_step(uint32_t w, float time_step, int sub_steps) {
   Batch* b = batchOf(w);
   if (!b) return;
-  if (time_step == 0.0f) return;
+  if (time_step == 0.0f) {
+    // a paused step only retires the last step's events (S/world.c:709-725)
+    download(*b);
+    for (HostWorld& hw : b->worlds) {
+      hw.beginIds.clear();
+      hw.endPairs.clear();
+      hw.hdr.beginCount = hw.hdr.endCount = 0;
+      hw.hdr.moveBodyCount = 0;
+      hw.eventsRetired = true;
+    }
+    return;
+  }
+  // a failed launch or a device fault ends the process, as the reference's asserts do
   if (f2dg_step(b, time_step, sub_steps, 1, PH_ALL) != 0) {
-    std::fprintf(stderr, "forge2d_amd: world step failed\n");
+    hipError_t e = hipGetLastError();
+    std::fprintf(stderr, "forge2d_amd: world step launch failed: %s (%s)\n", hipGetErrorString(e),
+                 b->lastError.c_str());
     std::abort();
   }
-  if (f2dg_synchronize(b) != 0) std::abort();
+  hipError_t e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "forge2d_amd: world step failed on the device: %s\n", hipGetErrorString(e));
+    std::abort();
+  }
+  const int bits = f2dg_error_bits(b);
+  if (bits != 0) {
+    std::fprintf(stderr, "forge2d_amd: world step exceeded a device capacity (error bits 0x%x:%s%s%s%s)\n", bits,
+                 (bits & ERR_CONTACT_CAP) ? " contacts" : "", (bits & ERR_PAIR_CAP) ? " new pairs" : "",
+                 (bits & ERR_OVERFLOW_CAP) ? " overflow constraints" : "", (bits & ERR_SCRATCH) ? " scratch" : "");
+    std::abort();
+  }
 }
 
 void f2d_world_set_gravity(uint32_t w, float x, float y) {

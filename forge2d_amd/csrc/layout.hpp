@@ -12,6 +12,7 @@
 #pragma once
 
 #include "b2math.hpp"
+#include "tree.hpp"
 
 namespace f2d {
 
@@ -38,6 +39,7 @@ enum : int {
   SF_MOVED = 8,      // proxy is in the move buffer for the next pair update
   SF_PRESOLVE = 16,
   SF_SENSOR = 32,
+  SF_TREE_PENDING = 64,  // the step enlarged this proxy's leaf; the tree kernel applies it
 };
 // contact flags (b2ContactFlags, S/contact.h:17-40) and sim flags (:66-86)
 enum : int {
@@ -118,7 +120,8 @@ struct alignas(16) Shape {
   uint32_t catLo, catHi, maskLo, maskHi;
   int group;
   int moveRank;   // order of this proxy in the move buffer when it moves
-  int pad[2];
+  int treeRank;   // position of this proxy's leaf in its tree's query order (child2 first)
+  int pad;
 };
 
 // b2ManifoldPoint, I/collision.h:498-536
@@ -212,8 +215,17 @@ struct alignas(16) WorldHeader {
   int mergePending;              // an island got a union-find parent since the last merge
   int sweepCount;                // shapes the multi-workgroup sweep launch sweeps (0: none)
   int sweepPairs;                // new pairs that launch found
+  int colorListOffset;           // colour arrays of colours 0..10 (kColorListHeader counts, then
+                                 // one array of bodyCount slots per colour) in the batch's colorLists
   unsigned long long splitKey;   // finalize: sleepiest body of an island with removed constraints
 };
+
+// Per-world colour arrays (the b2GraphColor contact arrays, S/constraint_graph.h:22-45):
+// a header of per-colour counts, then colour c's contact ids in array order at
+// kColorListHeader + c * bodyCount. A colour other than the overflow holds each
+// non-static body once, so bodyCount slots suffice.
+constexpr int kColorListHeader = 16;
+F2D_HD long colorListInts(int bodyCount) { return kColorListHeader + (long)kOverflowIndex * bodyCount; }
 
 // Solver sets, S/solver_set.h
 enum : int { kStaticSet = 0, kDisabledSet = 1, kAwakeSet = 2, kFirstSleepingSet = 3 };
@@ -325,6 +337,7 @@ struct BatchView {
   int2* pairShapes;     // new-pair shapes (query order, unoriented)
   uint32_t* colorBits;
   int* overflow;        // overflow-colour contact ids in array order
+  int* colorLists;      // per world at colorListOffset: colour arrays of colours 0..10
   float* cons;          // constraint SoA: field f of lane i at cons[f * consStride + i]
   long consStride;
   int* scratch;         // large-world sort scratch
@@ -335,6 +348,13 @@ struct BatchView {
   int* events;          // per world at 3 * contactOffset: begin contact ids [contactCap], end shape pairs [2 * contactCap]
   int* islandData;      // per world at islandOffset: the island region
   int4* contactLinks;   // per contact slot: island id, island prev, island next, solver set
+  // broad-phase trees: per world three headers (static, kinematic, dynamic); each
+  // tree's nodes and freed-id stack at its header's offset, rebuild scratch at
+  // 10 ints per node slot from 10 * offset
+  TreeHeader* treeHdrs;
+  TreeNode* treeNodes;
+  int* treeFreed;
+  int* treeWork;
   int worldCount;
 };
 

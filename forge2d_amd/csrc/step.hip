@@ -178,6 +178,7 @@ struct WorldPtrs {
   int2* pairShapes;
   uint32_t* colorBits;
   int* overflow;
+  int* colorLists;  // counts, then colour arrays of colours 0..10 (kColorListHeader layout)
   Cons cons;
   int* scratch;
   float4* sapBoxes;
@@ -202,6 +203,7 @@ __device__ WorldPtrs worldPtrs(const BatchView& b, WorldHeader* h) {
   w.pairShapes = b.pairShapes + h->pairOffset;
   w.colorBits = b.colorBits + h->colorOffset;
   w.overflow = b.overflow + h->overflowOffset;
+  w.colorLists = b.colorLists + h->colorListOffset;
   w.cons.f = b.cons + h->constraintOffset;
   w.cons.stride = b.consStride;
   w.scratch = b.scratch + h->scratchOffset;
@@ -311,10 +313,12 @@ __device__ void considerPair(const WorldPtrs& w, int* pairCount, int s, int t, c
   const PairInfo& P = partner == s ? S : T;
   // Order within one move result: dynamic-tree partners, then static, then kinematic
   // (the query order reversed by the list prepend, S/broad_phase.c:345-371, 285-287).
+  // Within one tree the query visits partners in its leaf order (child2 first) and the
+  // prepend reverses it (S/dynamic_tree.c:1114-1170, S/broad_phase.c:285-287).
   int ptype = P.proxyKey & 3;
-  int treeRank = ptype == kDynamic ? 0 : (ptype == kStatic ? 1 : 2);
-  uint32_t partnerOrder = 0x3FFFFFFFu - (uint32_t)(P.proxyKey >> 2);
-  uint64_t key = ((uint64_t)(uint32_t)w.shapes[emitter].moveRank << 32) | ((uint64_t)treeRank << 30) | partnerOrder;
+  int treeOrder = ptype == kDynamic ? 0 : (ptype == kStatic ? 1 : 2);
+  uint32_t partnerOrder = 0x3FFFFFFFu - (uint32_t)w.shapes[partner].treeRank;
+  uint64_t key = ((uint64_t)(uint32_t)w.shapes[emitter].moveRank << 32) | ((uint64_t)treeOrder << 30) | partnerOrder;
   int idx = atomicAdd(pairCount, 1);
   if (idx < w.hdr->pairCap) {
     w.pairKeys[idx] = key;
@@ -810,7 +814,10 @@ __device__ void addContactToGraph(const WorldPtrs& w, int id) {
     c.localIndex = h->overflowCount;
     w.overflow[h->overflowCount++] = id;
   } else {
-    c.localIndex = -1;
+    // appended to the colour's array (b2ContactSimArray_Add, S/constraint_graph.c:160-175)
+    const int n = w.colorLists[color]++;
+    w.colorLists[kColorListHeader + (long)color * w.hdr->bodyCount + n] = id;
+    c.localIndex = n;
   }
 }
 
@@ -823,6 +830,14 @@ __device__ void removeContactFromGraph(const WorldPtrs& w, int id) {
     uint32_t* set = colorSet(w, color);
     clearBit(set, c.bodyA);
     clearBit(set, c.bodyB);
+    // remove-swap from the colour's array (S/constraint_graph.c:198-209)
+    int* list = w.colorLists + kColorListHeader + (long)color * w.hdr->bodyCount;
+    const int idx = c.localIndex, last = --w.colorLists[color];
+    if (idx != last && idx >= 0) {
+      const int moved = list[last];
+      list[idx] = moved;
+      w.contacts[moved].localIndex = idx;
+    }
   } else {
     WorldHeader* h = w.hdr;
     int idx = c.localIndex;
@@ -1527,6 +1542,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
           fat.hi.x = aabb.hi.x + kAabbMargin;
           fat.hi.y = aabb.hi.y + kAabbMargin;
           shape.fat = fat;
+          shape.flags |= SF_TREE_PENDING;
           // enlarged proxy joins the move buffer in sim order (S/solver.c:1848-1903)
           shape.flags |= SF_MOVED;
           shape.moveRank = (sim.localIndex << 8) | (rank < 255 ? rank : 255);
@@ -1778,6 +1794,7 @@ __device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList
         fat.hi.x = aabb.hi.x + kAabbMargin;
         fat.hi.y = aabb.hi.y + kAabbMargin;
         shape.fat = fat;
+          shape.flags |= SF_TREE_PENDING;
         shape.flags |= SF_MOVED;
         shape.moveRank = (sim.localIndex << 8) | (rank < 255 ? rank : 255);
         flags |= BF_ENLARGED;
@@ -1796,6 +1813,7 @@ __device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList
         fat.hi.x = shape.aabb.hi.x + kAabbMargin;
         fat.hi.y = shape.aabb.hi.y + kAabbMargin;
         shape.fat = fat;
+          shape.flags |= SF_TREE_PENDING;
         shape.flags |= SF_MOVED;
         shape.moveRank = (sim.localIndex << 8) | (rank < 255 ? rank : 255);
         flags |= BF_ENLARGED;
@@ -2131,6 +2149,65 @@ __global__ __launch_bounds__(kQueryLanes) void explodeKernel(BatchView b, const 
   }
 }
 
+// ---------------------------------------------------------------------------- broad-phase trees
+
+// The tree work of b2Collide for one world per workgroup (S/world.c:447-459, 494-501):
+// the leaves the last step enlarged (b2BroadPhase_EnlargeProxy, S/solver.c:1848-1903,
+// 1959-1985) take their shapes' fat boxes and mark their ancestors grown -- all lanes,
+// each leaf walking up until it meets a node another walk marked; then the partial
+// rebuild of the dynamic and kinematic trees (S/broad_phase.c:488-492) and every
+// proxy's rank in its tree's query order for the next pair update. Grown nodes are
+// freed by the rebuild, so their boxes are not refitted here.
+__global__ void __launch_bounds__(64) treeKernel(BatchView b, const int* worldList) {
+  const int world = worldList[blockIdx.x];
+  const WorldHeader* h = b.worlds + world;
+  Shape* shapes = b.shapes + h->shapeOffset;
+  TreeHeader* th = b.treeHdrs + 3L * world;
+  bool any = false;
+  for (int s = threadIdx.x; s < h->shapeCount; s += 64) {
+    const int flags = shapes[s].flags;
+    if ((flags & SF_TREE_PENDING) == 0) continue;
+    shapes[s].flags = flags & ~SF_TREE_PENDING;
+    const int key = shapes[s].proxyKey;
+    if (key < 0) continue;
+    TreeNode* n = b.treeNodes + th[key & 3].offset;
+    const int leaf = key >> 2;
+    n[leaf].box = shapes[s].fat;
+    any = true;
+    for (int p = n[leaf].parent; p >= 0; p = n[p].parent) {
+      unsigned* word = reinterpret_cast<unsigned*>(&n[p].height);
+      const unsigned old = atomicOr(word, (unsigned)TN_ENLARGED << 16);
+      if (old & ((unsigned)TN_ENLARGED << 16)) break;
+    }
+  }
+  (void)any;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int t = kDynamic; t >= kKinematic; --t) {
+    TreeHeader* hd = th + t;
+    if (hd->proxyCount == 0) continue;
+    TreeRef r{hd, b.treeNodes + hd->offset, b.treeFreed + hd->offset};
+    int* work = b.treeWork + 10L * hd->offset;
+    const int cap = hd->cap;
+    treeRebuild(r, work, reinterpret_cast<Vec2*>(work + cap), work + 3 * cap,
+                reinterpret_cast<BuildItem*>(work + 4 * cap));
+    // ranks in query order (child2 first): a depth-first walk over the new tree
+    int* walk = work;
+    int sp = 0, rank = 0;
+    if (hd->root >= 0) walk[sp++] = hd->root;
+    while (sp > 0) {
+      const int i = walk[--sp];
+      const TreeNode& n = r.n[i];
+      if (n.flags & TN_LEAF) {
+        shapes[n.child1].treeRank = rank++;
+      } else {
+        walk[sp++] = n.child1;
+        walk[sp++] = n.child2;
+      }
+    }
+  }
+}
+
 template <int BLOCK>
 static int launchBlock(const BatchView* b, const StepParams* sp, const int* worldList, int nWorlds, hipStream_t stream,
                        const LaunchConfig* cfg) {
@@ -2148,6 +2225,9 @@ static int launchBlock(const BatchView* b, const StepParams* sp, const int* worl
   } else if (sp->phases & (PH_PAIRS | PH_COLLIDE | PH_STATE)) {
     hipLaunchKernelGGL(collideKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->collideLds, stream, *b, *sp, worldList,
                        cfg->collideLds);
+  }
+  if (sp->phases & PH_COLLIDE) {
+    hipLaunchKernelGGL(treeKernel, dim3(nWorlds), dim3(64), 0, stream, *b, worldList);
   }
   if (sp->phases & (PH_SOLVE | PH_FINALIZE)) {
     hipLaunchKernelGGL(solveKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->solveLds, stream, *b, *sp, worldList,

@@ -23,11 +23,15 @@ struct HostWorld {
   std::vector<Contact> contacts;   // size = contact capacity after commit
   std::vector<int> freeIds;        // LIFO stack
   std::vector<int> overflow;
+  std::vector<int> colorList[kOverflowIndex];  // colours 0..10: contact ids in array order
   std::vector<uint32_t> colorBits; // kOverflowIndex x colorWords
   std::vector<float> bodyMass, bodyInertia, shapeDensity;
   std::vector<uint16_t> bodyGen;
   std::vector<int> bodyFixedRotation;
-  int treeNodes[3] = {0, 0, 0};    // dynamic-tree node allocation mirror per body type
+  // broad-phase trees (static, kinematic, dynamic): headers, nodes, freed-id stacks
+  TreeHeader tree[3] = {{-1, 0, 0, 0, 0, 0, 0, 0}, {-1, 0, 0, 0, 0, 0, 0, 0}, {-1, 0, 0, 0, 0, 0, 0, 0}};
+  std::vector<TreeNode> tnodes[3];
+  std::vector<int> tfreed[3];
   int createdSeq = 0;              // proxies created since the last step
   int contactCount = 0;
   int stepIndex = 0;
@@ -36,6 +40,7 @@ struct HostWorld {
   bool hasCallbacks = false;
   std::vector<int> beginIds;       // contact begin-touch events of the last step (contact ids)
   std::vector<int> endPairs;       // contact end-touch events of the last step (shape pairs)
+  bool eventsRetired = false;      // a paused step (dt = 0) since the last step: no events
   // island graph and solver sets (counters in hdr): records, id-pool free lists,
   // the awake set's island and body orders, the last finalize's body order
   std::vector<Island> islands;
@@ -44,6 +49,9 @@ struct HostWorld {
   int staticBodies = 0;            // static set size (static bodies' index in it)
 
   IslandCtx islandCtx();           // host view of the island data (valid until the vectors grow)
+  TreeRef treeRef(int type, int extra = 0);  // host view of a tree with room for `extra` more nodes
+  void applyPendingEnlarges();     // enlargements the device recorded but has not applied yet
+  void computeTreeRanks();         // every proxy's rank in its tree's query order
 
   void ensureColorWords();
   int createBody(int type, Vec2 p, Rot q, Vec2 v, float w, float linDamp, float angDamp, float gravityScale,
@@ -92,7 +100,7 @@ struct Batch {
   std::string lastError;
   float contactCapFactor = 6.0f;
   // flat array lengths of the committed batch (bulk host mirror copies)
-  long nBodies = 0, nShapes = 0, nContacts = 0, nJoints = 0, nColorWords = 0, nIsland = 0;
+  long nBodies = 0, nShapes = 0, nContacts = 0, nJoints = 0, nColorWords = 0, nIsland = 0, nTreeNodes = 1, nColorListInts = 1;
   LaunchConfig launch = {0, 0, 0, 0, 0};
   // query staging on the device (grown on demand, freed with the batch)
   Query* dQueries = nullptr;

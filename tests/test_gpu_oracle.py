@@ -1,12 +1,14 @@
 """GPU parity against the C oracle on seeded scenes larger than the fixtures.
 
 Each scene is stepped on the device; after every step the oracle restarts from
-the device's own previous state, creates the contacts the device's broad phase
-created (their ids must be the LIFO id pool's allocations, which the oracle
-checks) and computes the step itself. The device's manifolds, colours, impulses,
-bodies, fat AABBs and move array must equal the oracle's bit for bit, except for
-the fast bodies of a continuous sweep that several candidates hit (visited in a
-different order; compared within 1e-5) — see tests/test_oracle.py.
+the device's own previous state (its broad-phase trees included), runs the pair
+update over its own trees -- which fixes the new contacts and their ids
+independently of the device's sort and sweep -- and computes the step itself. The
+device's contact ids, manifolds, colours, impulses, bodies, fat AABBs, move array,
+island graph and trees must equal the oracle's bit for bit, except for the fast
+bodies of a continuous sweep that several candidates hit (visited in a different
+order; compared within 1e-5) -- see tests/test_oracle.py. The contact set after a
+step must also equal the brute-force overlap set of the state before it.
 
 The oracle needs the reference's own shape geometry (hulls, rounded boxes); it is
 written by the reference build's fixture harness (oracle/_ref/ref_harness geom).
@@ -60,9 +62,16 @@ def reference_geometry(spec: bytes) -> bytes:
             return f.read()
 
 
+def contacts_by_id(s):
+    return {int(c["id"]): (int(c["shape_a"]), int(c["shape_b"])) for c in s.live_contacts()}
+
+
 def compare(mine, o):
     assert np.array_equal(mine.pair_keys(), o.pair_keys())
+    assert contacts_by_id(mine) == contacts_by_id(o)
+    assert np.array_equal(mine.free_ids, o.free_ids)
     assert P.color_map(mine) == P.color_map(o)
+    assert all(np.array_equal(a, c) for a, c in zip(mine.colors, o.colors)), "colour array order"
     fast = (o.bodies["flags"] & 1) != 0
     for f in P.BODY_FIELDS + ["c0x", "c0y", "q0c", "q0s"]:
         assert np.array_equal(mine.bodies[f][~fast], o.bodies[f][~fast]), f
@@ -74,6 +83,7 @@ def compare(mine, o):
     if not fast.any():
         assert np.array_equal(mine.move, o.move)
     assert P.island_differences(mine, o) == []
+    assert P.tree_differences(mine, o, TOL if fast.any() else 0.0) == []
 
 
 @pytest.mark.skipif(not (O.available("f32") and os.path.exists(HARNESS)), reason="oracle or reference build missing")
@@ -91,6 +101,8 @@ def test_device_matches_oracle_every_step(gpu, name):
         assert b.sync() == 0
         raw = b.export(0)
         mine = S.parse_one(raw)
-        o = S.parse_one(O.step(spec, geoms, prev_raw, prev, mine).export())
+        # the pair set by brute force over the state before the step (S/broad_phase.c:160-302)
+        assert np.array_equal(O.Oracle(spec, geoms, prev_raw).pairs(), mine.pair_keys()), k
+        o = S.parse_one(O.step(spec, geoms, prev_raw, prev).export())
         compare(mine, o)
         prev_raw, prev = raw, mine

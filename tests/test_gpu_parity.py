@@ -5,8 +5,9 @@ For every pair of consecutive dumped steps (k-1, k) of every fixture:
   * full step: import the reference state after step k-1, run one device step and
     compare with the reference after step k: the contact pair set and every
     manifold (point count, feature ids, anchors, separation, normal) bit-exact;
-    colouring valid; persisting contacts keep their colour; when the colouring
-    equals the reference's, body position, rotation and velocity within 1e-5
+    new contacts get the reference's ids (the pair update follows the broad-phase
+    trees, which are kept and compared node for node); colours, the overflow order
+    and the free-id list equal; body position, rotation and velocity within 1e-5
     relative (floor 1: absolute below unit magnitude) — in practice bit-exact.
   * solver + finalize: the reference's post-collide state of step k (colours,
     manifolds, impulses matched from k-1, bodies from k-1) stepped through the
@@ -16,7 +17,7 @@ The ccd fixtures fire fast bodies and bullets at thin static geometry, so these
 cases also cover the time-of-impact sweeps of continuous collision. The sleep
 fixture (every step 0-260) covers islands: links, merges, splits, islands put to
 sleep and woken, compared exactly (island records, id pools, awake orders, links)
-whenever the step's new contacts got the reference's ids.
+on every step.
 """
 import os
 
@@ -46,6 +47,10 @@ def consecutive_pairs(name):
 CASES = [c for n in FIXTURE_NAMES for c in consecutive_pairs(n)]
 
 
+def contacts_by_id(s):
+    return {int(c["id"]): (int(c["shape_a"]), int(c["shape_b"])) for c in s.live_contacts()}
+
+
 @pytest.mark.parametrize("name,i", CASES, ids=[f"{n}-{load_fixture(n)[1][i].step}" for n, i in CASES])
 def test_full_step_from_reference_state(gpu, name, i):
     spec, snaps, raws = load_fixture(name)
@@ -57,24 +62,26 @@ def test_full_step_from_reference_state(gpu, name, i):
     assert b.sync() == 0
     mine = b.snapshot(0)
     assert np.array_equal(np.sort(mine.pair_keys()), np.sort(ref.pair_keys())), "contact pair set differs"
+    # new contacts take the reference's ids: the pair update follows its trees
+    assert contacts_by_id(mine) == contacts_by_id(ref), "contact ids differ"
+    assert np.array_equal(mine.free_ids, ref.free_ids)
     rep = P.manifold_report(mine, ref)
     assert rep["point_count"] == 0 and rep["ids"] == 0, rep
     assert rep["values"] == 0 and rep["normals"] == 0, rep
     assert P.coloring_violations(mine) == 0
-    cm, cr = P.color_map(mine), P.color_map(ref)
-    persisting = [k for k in P.color_map(prev) if k in cr]
-    assert all(cm[k] == cr[k] for k in persisting)
-    if cm == cr:
-        errs = P.body_errors(mine, ref)
-        assert max(errs.values()) <= TOL, errs
-        rep = P.manifold_report(mine, ref, with_impulses=True)
-        assert rep["impulses"] == 0, rep
-    # islands and solver sets follow the contact ids (links happen in id order), so
-    # they are compared when the device's new contacts got the reference's ids
-    same_ids = {(int(c["shape_a"]), int(c["shape_b"])): int(c["id"]) for c in mine.live_contacts()} == \
-               {(int(c["shape_a"]), int(c["shape_b"])): int(c["id"]) for c in ref.live_contacts()}
-    if same_ids and cm == cr:
-        assert P.island_differences(mine, ref) == []
+    assert P.color_map(mine) == P.color_map(ref)
+    assert all(np.array_equal(a, c) for a, c in zip(mine.colors, ref.colors)), "colour array order"
+    errs = P.body_errors(mine, ref)
+    assert max(errs.values()) <= TOL, errs
+    rep = P.manifold_report(mine, ref, with_impulses=True)
+    assert rep["impulses"] == 0, rep
+    assert P.island_differences(mine, ref) == []
+    # leaves of fast bodies carry boxes within the sweep tolerance (several hits of
+    # one sweep are visited in another order than the reference's)
+    fast = (ref.bodies["flags"] & 1) != 0
+    assert P.tree_differences(mine, ref, TOL if fast.any() else 0.0) == []
+    if not fast.any():
+        assert np.array_equal(mine.move, ref.move)
 
 
 def sleep_enabled(spec: bytes) -> bool:
@@ -95,10 +102,10 @@ def test_solver_and_finalize_from_post_collide_state(gpu, name, i):
     print(f"[parity] solve {name} {prev.step}->{ref.step}", flush=True)
     if sleep_enabled(spec):
         # with sleep on, the solver also merges, splits and sleeps islands: start it
-        # from the post-collide state the oracle's collide (pinned bit-exact to the
-        # reference) computes, island graph included
+        # from the post-collide state the oracle's pair update and collide (pinned
+        # bit-exact to the reference) compute, island graph included
         o = O.Oracle(spec, fixture_geoms(name), raws[i - 1])
-        o.create_contacts(O.created_contacts(prev, ref))
+        o.update_pairs()
         o.collide()
         post = o.export()
     else:
@@ -139,6 +146,9 @@ def test_first_step_pairs_and_manifolds(gpu, name):
     mine = b.snapshot(0)
     ref = snaps[1]
     assert np.array_equal(np.sort(mine.pair_keys()), np.sort(ref.pair_keys()))
+    assert contacts_by_id(mine) == contacts_by_id(ref)
     rep = P.manifold_report(mine, ref)
     assert rep["point_count"] == rep["ids"] == rep["values"] == rep["normals"] == 0, rep
     assert P.coloring_violations(mine) == 0
+    assert P.color_map(mine) == P.color_map(ref)
+    assert P.tree_differences(mine, ref, TOL) == []

@@ -53,6 +53,8 @@ def test_host_construction_matches_reference_step0(name):
     # solver sets and islands at creation: one island per non-static body, joints
     # linked and their islands merged at once (S/body.c:84-97, S/joint.c:289-309)
     assert P.island_differences(mine, ref) == []
+    # broad-phase trees built by inserting each proxy with rotations (S/dynamic_tree.c:625-795)
+    assert P.tree_differences(mine, ref) == []
 
 
 @pytest.mark.parametrize("name", FIXTURE_NAMES)
@@ -71,10 +73,11 @@ def test_snapshot_import_export_roundtrip(name):
         assert not rep["only_mine"] and not rep["only_ref"]
         assert rep["point_count"] == rep["ids"] == rep["values"] == rep["impulses"] == 0
         assert P.color_map(back) == P.color_map(ref)
-        assert np.array_equal(back.colors[S.OVERFLOW], ref.colors[S.OVERFLOW])
+        assert all(np.array_equal(a, c) for a, c in zip(back.colors, ref.colors)), "colour array order"
         assert np.array_equal(back.move, ref.move)
         assert np.array_equal(back.free_ids, ref.free_ids)
         assert P.island_differences(back, ref) == []
+        assert P.tree_differences(back, ref) == []
 
 
 def test_scene_generators_are_deterministic():
