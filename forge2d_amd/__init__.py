@@ -230,9 +230,9 @@ class Batch:
 
     def kernel_times(self, dt=1.0 / 60.0, substeps=4, steps=1) -> dict:
         """Step `steps` times; return summed HIP-event ms of each kernel."""
-        ms = (ctypes.c_float * 3)()
+        ms = (ctypes.c_float * 4)()
         self._check(lib().f2dg_kernel_times(self.h, dt, substeps, steps, ms), "kernel_times")
-        return {"collideKernel": ms[0], "solveKernel": ms[1], "continuousKernel": ms[2]}
+        return {"collideKernel": ms[0], "solveKernel": ms[1], "continuousKernel": ms[2], "treeKernel": ms[3]}
 
     def phase_times(self, dt=1.0 / 60.0, substeps=4, steps=1) -> dict:
         """Summed milliseconds per phase over `steps` steps run as separate launches."""

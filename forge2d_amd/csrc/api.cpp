@@ -257,7 +257,7 @@ void HostWorld::computeTreeRanks() {
     walk.resize((size_t)tree[t].fresh + 2);
     if (tree[t].root >= 0 && !tnodes[t].empty()) treeLeafRanks(tnodes[t].data(), tree[t].root, walk.data(), rank.data());
   }
-  for (size_t s = 0; s < shapes.size(); ++s) shapes[s].treeRank = rank[s];
+  treeRanks = rank;
 }
 
 // b2CreateJoint + b2CreateRevoluteJoint + b2AssignJointColor, S/joint.c:146-280, 507-560,
@@ -616,6 +616,18 @@ static int solveLdsBytes(int maxBodies, int maxJoints, float lanesPerBody, int l
   return (int)((total + 255) & ~255L) <= limit ? (int)((total + 255) & ~255L) : limit;
 }
 
+// LDS for the tree kernel's work arrays of a world's larger moving tree (worlds that
+// need more than `limit` work from HBM)
+static int treeLdsBytes(const HostWorld& hw, int limit) {
+  long need = 0;
+  for (int t = kKinematic; t <= kDynamic; ++t)
+    if (hw.tree[t].proxyCount > 0) need = std::max(need, 4 * treeWorkInts(hw.tree[t].fresh, hw.tree[t].proxyCount));
+  return need <= limit ? (int)((need + 255) & ~255L) : 0;
+}
+
+// lanes of the tree launch: about two proxies per lane per pass
+static int treeBlockFor(int proxies) { return proxies <= 96 ? 64 : (proxies <= 192 ? 128 : 256); }
+
 static int commit(Batch& b) {
   freeDevice(b);
   long nb = 0, ns = 0, nc = 0, nj = 0, ncol = 0, nscratch = 0, nisl = 0, ntree = 0, nclist = 0;
@@ -702,7 +714,7 @@ static int commit(Batch& b) {
       devAlloc(d, &v.contactLinks, nc) || devAlloc(d, &v.treeHdrs, 3L * W) ||
       devAlloc(d, &v.colorLists, std::max(nclist, 1L)) ||
       devAlloc(d, &v.treeNodes, std::max(ntree, 1L)) || devAlloc(d, &v.treeFreed, std::max(ntree, 1L)) ||
-      devAlloc(d, &v.treeWork, 10 * std::max(ntree, 1L))) {
+      devAlloc(d, &v.treeWork, treeWorkOffset((int)ntree, W, 0) + 1) || devAlloc(d, &v.treeRanks, ns)) {
     b.lastError = "device allocation failed";
     freeDevice(b);
     return -1;
@@ -738,6 +750,13 @@ static int commit(Batch& b) {
   b.launch.collideLds = collide;
   // worlds whose sort arrays (28 B per shape) exceed the collide LDS sweep over many workgroups
   b.launch.sweepChunks = 28L * std::max(np, 4) > (long)collide ? (maxShapes + 255) / 256 : 0;
+  b.launch.treeLds = 0;
+  int maxProxies = 0;
+  for (const HostWorld& hw : b.worlds) {
+    b.launch.treeLds = std::max(b.launch.treeLds, treeLdsBytes(hw, kLdsLimit));
+    maxProxies = std::max(maxProxies, std::max(hw.tree[kDynamic].proxyCount, hw.tree[kKinematic].proxyCount));
+  }
+  b.launch.treeBlock = treeBlockFor(maxProxies);
   int solve = maxBodies * (int)(sizeof(BodyState) + sizeof(BodyConst));
   b.launch.solveLds = solve <= kLdsLimit ? ((solve + 255) & ~255) : 0;
   if (b.launch.block == 0) b.launch.block = maxShapes <= 640 ? 64 : 256;
@@ -779,6 +798,14 @@ static int commit(Batch& b) {
       c.launch.collideLds = std::max(cc, 12 * 256);
       c.launch.sweepChunks = 28L * std::max(cnp, 4) > (long)c.launch.collideLds ? (c.maxShapes + 255) / 256 : 0;
       c.launch.solveLds = solveLdsBytes(c.maxBodies, c.maxJoints, 1.0f, kLdsLimit);
+      c.launch.treeLds = 0;
+      int maxProxies = 0;
+      for (int i : members[k]) {
+        c.launch.treeLds = std::max(c.launch.treeLds, treeLdsBytes(b.worlds[i], kLdsLimit));
+        maxProxies = std::max(maxProxies, std::max(b.worlds[i].tree[kDynamic].proxyCount,
+                                                   b.worlds[i].tree[kKinematic].proxyCount));
+      }
+      c.launch.treeBlock = treeBlockFor(maxProxies);
       // one workgroup holds a whole world: the largest worlds get the most lanes
       c.launch.block = c.limit <= 128 ? 64 : (c.limit <= 512 ? 128 : (c.limit <= 1024 ? 256 : 1024));
       if (b.forcedBlock) c.launch.block = b.forcedBlock;
@@ -866,6 +893,7 @@ static int upload(Batch& b) {
     F2D_HIP(hipMemcpyAsync(v.sims + h.bodyOffset, hw.sims.data(), sizeof(BodySim) * h.bodyCount, hipMemcpyHostToDevice, s));
     F2D_HIP(hipMemcpyAsync(v.states + h.bodyOffset, hw.states.data(), sizeof(BodyState) * h.bodyCount, hipMemcpyHostToDevice, s));
     F2D_HIP(hipMemcpyAsync(v.shapes + h.shapeOffset, hw.shapes.data(), sizeof(Shape) * h.shapeCount, hipMemcpyHostToDevice, s));
+    F2D_HIP(hipMemcpyAsync(v.treeRanks + h.shapeOffset, hw.treeRanks.data(), sizeof(int) * h.shapeCount, hipMemcpyHostToDevice, s));
     F2D_HIP(hipMemcpyAsync(v.geoms + h.shapeOffset, hw.geoms.data(), sizeof(ShapeGeom) * h.shapeCount, hipMemcpyHostToDevice, s));
     F2D_HIP(hipMemcpyAsync(v.contacts + h.contactOffset, hw.contacts.data(), sizeof(Contact) * h.contactCap, hipMemcpyHostToDevice, s));
     for (int k = 0; k < h.contactCap; ++k) {
@@ -985,6 +1013,7 @@ static int download(Batch& b) {
     }
     for (int t = 0; t < 3; ++t) {
       const TreeHeader& th = treeHdrs[3 * i + t];
+      if (th.error) headers[i].error |= ERR_TREE;
       hw.tree[t] = th;
       hw.tnodes[t].assign(treeNodes.begin() + th.offset, treeNodes.begin() + th.offset + th.cap);
       hw.tfreed[t].assign(treeFreed.begin() + th.offset, treeFreed.begin() + th.offset + th.cap);
@@ -1894,27 +1923,31 @@ int f2dg_kernel_times(void* h, float dt, int subSteps, int steps, float* ms) {
   Batch* b = static_cast<Batch*>(h);
   if (!b || steps <= 0) return -1;
   if (prepareForStep(*b) != 0) return -1;
-  hipEvent_t ev[4];
+  hipEvent_t ev[5];
   for (auto& e : ev) F2D_HIP(hipEventCreate(&e));
-  StepParams sp[3] = {makeParams(dt, subSteps, PH_PAIRS | PH_COLLIDE | PH_STATE | PH_NO_COUNT),
+  // launch order of a step: collide, tree, solve, continuous; reported as
+  // ms[0] collide, ms[1] solve, ms[2] continuous, ms[3] tree
+  StepParams sp[4] = {makeParams(dt, subSteps, PH_PAIRS | PH_COLLIDE | PH_STATE | PH_NO_COUNT | PH_NO_TREE),
+                      makeParams(dt, subSteps, PH_TREE | PH_NO_COUNT),
                       makeParams(dt, subSteps, PH_SOLVE | PH_FINALIZE | PH_NO_CONTINUOUS),
                       makeParams(dt, subSteps, PH_CONTINUOUS)};
-  float sum[3] = {0.0f, 0.0f, 0.0f};
+  const int slot[4] = {0, 3, 1, 2};
+  float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   for (int s = 0; s < steps; ++s) {
     F2D_HIP(hipEventRecord(ev[0], b->stream));
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
       if (launchClassesSerial(*b, sp[k]) != 0) return -1;
       F2D_HIP(hipEventRecord(ev[k + 1], b->stream));
     }
-    F2D_HIP(hipEventSynchronize(ev[3]));
-    for (int k = 0; k < 3; ++k) {
+    F2D_HIP(hipEventSynchronize(ev[4]));
+    for (int k = 0; k < 4; ++k) {
       float t = 0.0f;
       F2D_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-      sum[k] += t;
+      sum[slot[k]] += t;
     }
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
-  for (int k = 0; k < 3; ++k) ms[k] = sum[k];
+  for (int k = 0; k < 4; ++k) ms[k] = sum[k];
   b->deviceNewer = true;
   return 0;
 }
@@ -1976,11 +2009,15 @@ int f2dg_error_bits(void* h) {
   if (!b) return -1;
   if (!b->committed) return 0;
   std::vector<WorldHeader> headers(b->worlds.size());
+  std::vector<TreeHeader> trees(3 * b->worlds.size());
   F2D_HIP(hipMemcpyAsync(headers.data(), b->dev.view.worlds, sizeof(WorldHeader) * headers.size(),
+                         hipMemcpyDeviceToHost, b->stream));
+  F2D_HIP(hipMemcpyAsync(trees.data(), b->dev.view.treeHdrs, sizeof(TreeHeader) * trees.size(),
                          hipMemcpyDeviceToHost, b->stream));
   F2D_HIP(hipStreamSynchronize(b->stream));
   int err = 0;
   for (const WorldHeader& w : headers) err |= w.error;
+  for (const TreeHeader& t : trees) err |= t.error ? ERR_TREE : 0;
   return err;
 }
 
@@ -2437,9 +2474,10 @@ void f2d_world_step(uint32_t w, float time_step, int sub_steps) {
   }
   const int bits = f2dg_error_bits(b);
   if (bits != 0) {
-    std::fprintf(stderr, "forge2d_amd: world step exceeded a device capacity (error bits 0x%x:%s%s%s%s)\n", bits,
+    std::fprintf(stderr, "forge2d_amd: world step exceeded a device capacity (error bits 0x%x:%s%s%s%s%s)\n", bits,
                  (bits & ERR_CONTACT_CAP) ? " contacts" : "", (bits & ERR_PAIR_CAP) ? " new pairs" : "",
-                 (bits & ERR_OVERFLOW_CAP) ? " overflow constraints" : "", (bits & ERR_SCRATCH) ? " scratch" : "");
+                 (bits & ERR_OVERFLOW_CAP) ? " overflow constraints" : "", (bits & ERR_SCRATCH) ? " scratch" : "",
+                 (bits & ERR_TREE) ? " tree" : "");
     std::abort();
   }
 }

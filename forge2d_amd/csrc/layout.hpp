@@ -62,6 +62,7 @@ enum : int {
   ERR_PAIR_CAP = 2,
   ERR_OVERFLOW_CAP = 4,
   ERR_SCRATCH = 8,
+  ERR_TREE = 16,  // a broad-phase tree and its ranks disagreed; the tree was not rebuilt
 };
 
 // b2BodyState, S/body.h:100-113 (32 bytes, identical field order)
@@ -120,8 +121,7 @@ struct alignas(16) Shape {
   uint32_t catLo, catHi, maskLo, maskHi;
   int group;
   int moveRank;   // order of this proxy in the move buffer when it moves
-  int treeRank;   // position of this proxy's leaf in its tree's query order (child2 first)
-  int pad;
+  int pad[2];
 };
 
 // b2ManifoldPoint, I/collision.h:498-536
@@ -281,6 +281,8 @@ enum : int {
   PH_NO_CONTINUOUS = 128,  // finalize without the continuous-collision launch (timing splits)
   PH_PAIRS_PREP = 256,     // pair update of a large world: sort only, the sweep runs in its own launch
   PH_PAIRS_FINISH = 512,   // pair update of a large world: create the pairs the sweep launch found
+  PH_NO_TREE = 1024,       // collide without the tree launch (timing splits)
+  PH_TREE = 2048,          // the tree launch on its own (timing splits)
 };
 
 // A world query of the batch query ABI (f2dQuery in the public header). Rays:
@@ -302,6 +304,9 @@ struct QueryHit {
 static_assert(sizeof(Query) == 48 && sizeof(QueryHit) == 24, "query records");
 
 // Launch shape of a batch: workgroup size and dynamic LDS bytes per kernel.
+// HBM work region of tree t of a world whose node region starts at `offset`
+F2D_HD long treeWorkOffset(int offset, int world, int t) { return 25L * offset + 160L * (3L * world + t); }
+
 struct LaunchConfig {
   int block;       // 64, 128 or 256 lanes per world
   int collideLds;  // sort-and-sweep + pair-sort LDS
@@ -309,6 +314,8 @@ struct LaunchConfig {
   int continuous;  // some world of the batch has continuous collision enabled
   int sweepChunks; // > 0: worlds whose sort arrays exceed LDS sweep in their own launch, this
                    // many 256-shape chunks per world
+  int treeLds;     // tree rebuild work arrays (worlds whose trees need more use HBM)
+  int treeBlock;   // lanes of the tree launch (64, 128 or 256)
 };
 
 // Contact constraint lanes (one constraint per lane), S/contact_solver.c:1034-1064.
@@ -349,12 +356,12 @@ struct BatchView {
   int* islandData;      // per world at islandOffset: the island region
   int4* contactLinks;   // per contact slot: island id, island prev, island next, solver set
   // broad-phase trees: per world three headers (static, kinematic, dynamic); each
-  // tree's nodes and freed-id stack at its header's offset, rebuild scratch at
-  // 10 ints per node slot from 10 * offset
+  // tree's nodes and freed-id stack at its header's offset
   TreeHeader* treeHdrs;
   TreeNode* treeNodes;
   int* treeFreed;
-  int* treeWork;
+  int* treeWork;        // per tree at treeWorkOffset: rebuild work arrays of trees beyond LDS
+  int* treeRanks;       // per shape (shape offsets): its leaf's place in its tree's query order
   int worldCount;
 };
 

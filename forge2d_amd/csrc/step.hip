@@ -26,6 +26,7 @@
 #include "toi.hpp"
 #include "island.hpp"
 #include "query.hpp"
+#include "tree_device.hpp"
 
 namespace f2d {
 
@@ -186,6 +187,7 @@ struct WorldPtrs {
   int* events;
   int4* links;      // contact island links and solver set
   int* sweepOrder;
+  const int* treeRanks;  // per shape: its leaf's place in its tree's query order
   IslandCtx isl;
 };
 
@@ -211,6 +213,7 @@ __device__ WorldPtrs worldPtrs(const BatchView& b, WorldHeader* h) {
   w.events = b.events + 3L * h->contactOffset;
   w.sapBoxes = b.sapBoxes + h->shapeOffset;
   w.sweepOrder = b.sweepOrder + h->shapeOffset;
+  w.treeRanks = b.treeRanks + h->shapeOffset;
   w.links = b.contactLinks + h->contactOffset;
   w.isl = islandCtx(h, b.islandData + h->islandOffset, w.sims, w.contacts, w.links, w.joints);
   return w;
@@ -317,7 +320,7 @@ __device__ void considerPair(const WorldPtrs& w, int* pairCount, int s, int t, c
   // prepend reverses it (S/dynamic_tree.c:1114-1170, S/broad_phase.c:285-287).
   int ptype = P.proxyKey & 3;
   int treeOrder = ptype == kDynamic ? 0 : (ptype == kStatic ? 1 : 2);
-  uint32_t partnerOrder = 0x3FFFFFFFu - (uint32_t)w.shapes[partner].treeRank;
+  uint32_t partnerOrder = 0x3FFFFFFFu - (uint32_t)w.treeRanks[partner];
   uint64_t key = ((uint64_t)(uint32_t)w.shapes[emitter].moveRank << 32) | ((uint64_t)treeOrder << 30) | partnerOrder;
   int idx = atomicAdd(pairCount, 1);
   if (idx < w.hdr->pairCap) {
@@ -2152,59 +2155,42 @@ __global__ __launch_bounds__(kQueryLanes) void explodeKernel(BatchView b, const 
 // ---------------------------------------------------------------------------- broad-phase trees
 
 // The tree work of b2Collide for one world per workgroup (S/world.c:447-459, 494-501):
-// the leaves the last step enlarged (b2BroadPhase_EnlargeProxy, S/solver.c:1848-1903,
-// 1959-1985) take their shapes' fat boxes and mark their ancestors grown -- all lanes,
-// each leaf walking up until it meets a node another walk marked; then the partial
-// rebuild of the dynamic and kinematic trees (S/broad_phase.c:488-492) and every
-// proxy's rank in its tree's query order for the next pair update. Grown nodes are
-// freed by the rebuild, so their boxes are not refitted here.
-__global__ void __launch_bounds__(64) treeKernel(BatchView b, const int* worldList) {
+// for the dynamic and the kinematic tree, the leaves the last step enlarged
+// (S/solver.c:1848-1903, 1959-1985), the partial rebuild (S/broad_phase.c:488-492) and
+// every proxy's rank in the tree's query order for the next pair update
+// (rebuildTree). The work arrays live in LDS when the tree fits, else in the
+// tree's HBM work region.
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) treeKernel(BatchView b, const int* worldList, int ldsBytes) {
+  extern __shared__ __align__(16) int treeLds[];
   const int world = worldList[blockIdx.x];
   const WorldHeader* h = b.worlds + world;
   Shape* shapes = b.shapes + h->shapeOffset;
+  int* ranks = b.treeRanks + h->shapeOffset;
   TreeHeader* th = b.treeHdrs + 3L * world;
-  bool any = false;
-  for (int s = threadIdx.x; s < h->shapeCount; s += 64) {
-    const int flags = shapes[s].flags;
-    if ((flags & SF_TREE_PENDING) == 0) continue;
-    shapes[s].flags = flags & ~SF_TREE_PENDING;
-    const int key = shapes[s].proxyKey;
-    if (key < 0) continue;
-    TreeNode* n = b.treeNodes + th[key & 3].offset;
-    const int leaf = key >> 2;
-    n[leaf].box = shapes[s].fat;
-    any = true;
-    for (int p = n[leaf].parent; p >= 0; p = n[p].parent) {
-      unsigned* word = reinterpret_cast<unsigned*>(&n[p].height);
-      const unsigned old = atomicOr(word, (unsigned)TN_ENLARGED << 16);
-      if (old & ((unsigned)TN_ENLARGED << 16)) break;
-    }
-  }
-  (void)any;
-  __syncthreads();
-  if (threadIdx.x != 0) return;
   for (int t = kDynamic; t >= kKinematic; --t) {
     TreeHeader* hd = th + t;
-    if (hd->proxyCount == 0) continue;
-    TreeRef r{hd, b.treeNodes + hd->offset, b.treeFreed + hd->offset};
-    int* work = b.treeWork + 10L * hd->offset;
-    const int cap = hd->cap;
-    treeRebuild(r, work, reinterpret_cast<Vec2*>(work + cap), work + 3 * cap,
-                reinterpret_cast<BuildItem*>(work + 4 * cap));
-    // ranks in query order (child2 first): a depth-first walk over the new tree
-    int* walk = work;
-    int sp = 0, rank = 0;
-    if (hd->root >= 0) walk[sp++] = hd->root;
-    while (sp > 0) {
-      const int i = walk[--sp];
-      const TreeNode& n = r.n[i];
-      if (n.flags & TN_LEAF) {
-        shapes[n.child1].treeRank = rank++;
-      } else {
-        walk[sp++] = n.child1;
-        walk[sp++] = n.child2;
-      }
+    const int F = hd->fresh, P = hd->proxyCount;
+    if (P == 0) continue;
+    // two inlined copies: the work arrays' address space stays known (LDS or HBM)
+    if (treeWorkInts(F, P) * 4 <= ldsBytes) {
+      rebuildTree<BLOCK>(hd, b.treeNodes + hd->offset, shapes, h->shapeCount, t, ranks, treeWorkAt(treeLds, F, P));
+    } else {
+      rebuildTree<BLOCK>(hd, b.treeNodes + hd->offset, shapes, h->shapeCount, t, ranks,
+                         treeWorkAt(b.treeWork + treeWorkOffset(hd->offset, world, t), F, P));
     }
+    __syncthreads();
+  }
+}
+
+static void launchTree(const BatchView* b, const int* worldList, int nWorlds, hipStream_t stream,
+                       const LaunchConfig* cfg) {
+  if (cfg->treeBlock >= 256) {
+    hipLaunchKernelGGL(treeKernel<256>, dim3(nWorlds), dim3(256), cfg->treeLds, stream, *b, worldList, cfg->treeLds);
+  } else if (cfg->treeBlock >= 128) {
+    hipLaunchKernelGGL(treeKernel<128>, dim3(nWorlds), dim3(128), cfg->treeLds, stream, *b, worldList, cfg->treeLds);
+  } else {
+    hipLaunchKernelGGL(treeKernel<64>, dim3(nWorlds), dim3(64), cfg->treeLds, stream, *b, worldList, cfg->treeLds);
   }
 }
 
@@ -2226,8 +2212,8 @@ static int launchBlock(const BatchView* b, const StepParams* sp, const int* worl
     hipLaunchKernelGGL(collideKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->collideLds, stream, *b, *sp, worldList,
                        cfg->collideLds);
   }
-  if (sp->phases & PH_COLLIDE) {
-    hipLaunchKernelGGL(treeKernel, dim3(nWorlds), dim3(64), 0, stream, *b, worldList);
+  if (((sp->phases & PH_COLLIDE) && !(sp->phases & PH_NO_TREE)) || (sp->phases & PH_TREE)) {
+    launchTree(b, worldList, nWorlds, stream, cfg);
   }
   if (sp->phases & (PH_SOLVE | PH_FINALIZE)) {
     hipLaunchKernelGGL(solveKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->solveLds, stream, *b, *sp, worldList,

@@ -13,8 +13,9 @@
 // ids [fresh, capacity), which the reference's list holds in ascending order:
 // popping the stack, then taking fresh ids, hands out the ids the reference does.
 //
-// The functions work on any address space (global or LDS copies of a tree) and run
-// on one thread; the host runs the same code.
+// These functions run on the host (proxy creation, snapshot import, the
+// enlargements a device step left pending); the device's per-step rebuild is the
+// block-parallel restatement in tree_device.hpp.
 #pragma once
 
 #include "b2math.hpp"
@@ -43,6 +44,9 @@ struct alignas(16) TreeHeader {
   int offset;      // region start in the batch's node and freed-id arrays
   int error;       // set when the region's capacity was exceeded
 };
+
+// ints of the device rebuild's work arrays for F node slots and P proxies
+F2D_HD long treeWorkInts(int F, int P) { return 4L * F + 20L * P + 160; }
 
 struct TreeRef {
   TreeHeader* h;
@@ -332,122 +336,6 @@ F2D_HD void treeEnlargeProxy(TreeRef t, int id, AABB box) {
     n[p].flags |= TN_ENLARGED;
     p = n[p].parent;
   }
-}
-
-// b2PartitionMid :1426-1533 (the reference's B2_TREE_HEURISTIC 0): split at the
-// middle of the centres' bounds along the longer axis with a Hoare partition of
-// items and centres; ranges of one or two items, and splits that leave a side
-// empty, split at count / 2
-F2D_HD int partitionMid(int* items, Vec2* c, int count) {
-  if (count <= 2) return count / 2;
-  Vec2 lo = c[0], hi = c[0];
-  for (int i = 1; i < count; ++i) {
-    lo.x = minf(lo.x, c[i].x);
-    lo.y = minf(lo.y, c[i].y);
-    hi.x = maxf(hi.x, c[i].x);
-    hi.y = maxf(hi.y, c[i].y);
-  }
-  const bool useX = hi.x - lo.x > hi.y - lo.y;
-  const float pivot = useX ? 0.5f * (lo.x + hi.x) : 0.5f * (lo.y + hi.y);
-  int i1 = 0, i2 = count;
-  while (i1 < i2) {
-    while (i1 < i2 && (useX ? c[i1].x : c[i1].y) < pivot) i1 += 1;
-    while (i1 < i2 && (useX ? c[i2 - 1].x : c[i2 - 1].y) >= pivot) i2 -= 1;
-    if (i1 < i2) {
-      const int ti = items[i1];
-      items[i1] = items[i2 - 1];
-      items[i2 - 1] = ti;
-      const Vec2 tc = c[i1];
-      c[i1] = c[i2 - 1];
-      c[i2 - 1] = tc;
-      i1 += 1;
-      i2 -= 1;
-    }
-  }
-  return (i1 > 0 && i1 < count) ? i1 : count / 2;
-}
-
-// A range of build items on the build's stack (b2RebuildItem :1704-1715)
-struct BuildItem {
-  int node, childCount, start, split, end;
-};
-
-// b2BuildTree :1717-1870: depth first over item ranges; a node is allocated when its
-// range is pushed, children attach as their ranges finish. `stack` holds `count`
-// BuildItems. Returns the root, -1 without capacity.
-F2D_HD int buildTree(TreeRef t, int* items, Vec2* c, int count, BuildItem* stack) {
-  TreeNode* n = t.n;
-  if (count == 1) {
-    n[items[0]].parent = -1;
-    return items[0];
-  }
-  int top = 0;
-  stack[0].node = treeAlloc(t);
-  if (stack[0].node < 0) return -1;
-  stack[0].childCount = -1;
-  stack[0].start = 0;
-  stack[0].end = count;
-  stack[0].split = partitionMid(items, c, count);
-  for (;;) {
-    BuildItem& it = stack[top];
-    it.childCount += 1;
-    if (it.childCount == 2) {
-      if (top == 0) break;
-      BuildItem& par = stack[top - 1];
-      if (par.childCount == 0) n[par.node].child1 = it.node; else n[par.node].child2 = it.node;
-      n[it.node].parent = par.node;
-      refitNode(n, it.node);
-      top -= 1;
-    } else {
-      const int s0 = it.childCount == 0 ? it.start : it.split;
-      const int s1 = it.childCount == 0 ? it.split : it.end;
-      if (s1 - s0 == 1) {
-        const int ch = items[s0];
-        if (it.childCount == 0) n[it.node].child1 = ch; else n[it.node].child2 = ch;
-        n[ch].parent = it.node;
-      } else {
-        const int node = treeAlloc(t);
-        if (node < 0) return -1;
-        top += 1;
-        BuildItem& ni = stack[top];
-        ni.node = node;
-        ni.childCount = -1;
-        ni.start = s0;
-        ni.end = s1;
-        ni.split = s0 + partitionMid(items + s0, c + s0, s1 - s0);
-      }
-    }
-  }
-  refitNode(n, stack[0].node);
-  return stack[0].node;
-}
-
-// b2DynamicTree_Rebuild :1872-1989 with fullBuild false: leaves and internal nodes
-// that did not grow become the build's items, in child1-first depth-first order;
-// grown internal nodes are freed. Scratch: items and centres for proxyCount
-// entries, `walk` for the tree's height, `stack` for proxyCount BuildItems.
-F2D_HD void treeRebuild(TreeRef t, int* items, Vec2* c, int* walk, BuildItem* stack) {
-  if (t.h->proxyCount == 0 || t.h->root < 0) return;
-  TreeNode* n = t.n;
-  int count = 0, sp = 0;
-  int i = t.h->root;
-  for (;;) {
-    if (n[i].height == 0 || (n[i].flags & TN_ENLARGED) == 0) {
-      items[count] = i;
-      c[count] = aabbCenter(n[i].box);
-      count += 1;
-      n[i].parent = -1;
-    } else {
-      const int doomed = i;
-      walk[sp++] = n[i].child2;
-      i = n[i].child1;
-      treeFree(t, doomed);
-      continue;
-    }
-    if (sp == 0) break;
-    i = walk[--sp];
-  }
-  t.h->root = buildTree(t, items, c, count, stack);
 }
 
 // Each leaf's position in the order a query visits leaves (depth first, child2

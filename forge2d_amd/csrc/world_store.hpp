@@ -32,6 +32,7 @@ struct HostWorld {
   TreeHeader tree[3] = {{-1, 0, 0, 0, 0, 0, 0, 0}, {-1, 0, 0, 0, 0, 0, 0, 0}, {-1, 0, 0, 0, 0, 0, 0, 0}};
   std::vector<TreeNode> tnodes[3];
   std::vector<int> tfreed[3];
+  std::vector<int> treeRanks;      // per shape: its leaf's place in its tree's query order
   int createdSeq = 0;              // proxies created since the last step
   int contactCount = 0;
   int stepIndex = 0;

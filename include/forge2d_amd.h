@@ -205,8 +205,9 @@ int f2dg_step_worlds(void* batch, const int32_t* world_ids, int n, float dt, int
 int f2dg_step_each(void* batch, float dt, int sub_steps, int steps);
 /* step and time with HIP events on the batch stream; *ms_out = total milliseconds */
 int f2dg_step_timed(void* batch, float dt, int sub_steps, int steps, float* ms_out);
-/* step and time each kernel: HIP events bracket the collide, solve and
- * continuous-collision launches of every step; ms[0..2] = summed milliseconds */
+/* step and time each kernel: HIP events bracket the collide, tree, solve and
+ * continuous-collision launches of every step; summed milliseconds in ms[0] collide,
+ * ms[1] solve, ms[2] continuous collision, ms[3] tree */
 int f2dg_kernel_times(void* batch, float dt, int sub_steps, int steps, float* ms);
 /* step with the phases as separate launches and time each: ms[0] pairs, ms[1] narrow
  * phase + contact state, ms[2] solver + finalize, ms[3] continuous + island sleep */
