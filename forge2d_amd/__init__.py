@@ -118,6 +118,7 @@ SIGNATURES = [
     ("f2d_world_explode", None, [_U, _U, _U, _F, _F, _F, _F, _F]),
     ("f2dg_query", _I, [_VP, _VP, _I, _VP, _PI, _I]),
     ("f2dg_query_device", _I, [_VP, _VP, _I, _VP, _VP, _I]),
+    ("f2dg_query_device_static", _I, [_VP, _VP, _I, _VP, _VP, _I]),
 ]
 
 # query kinds and records of f2dg_query (include/forge2d_amd.h)

@@ -2270,3 +2270,66 @@ extern "C" int f2d_launch_step(const f2d::BatchView* b, const f2d::StepParams* s
       return f2d::launchBlock<256>(b, sp, worldList, nWorlds, stream, cfg);
   }
 }
+
+// ---------------------------------------------------------------------------- allocation-free queries
+
+#include <mutex>
+
+#include "world_store.hpp"
+
+namespace f2d {
+
+// Packed fat boxes of the allocation-free query entry point: file-scope device
+// memory, one copy per GPU, held until the process ends. A batch with more shapes
+// than this is refused.
+constexpr long kStaticBoxCapacity = 1L << 18;  // 262,144 shapes (4 MiB)
+static __device__ __align__(256) float4 scratch[kStaticBoxCapacity];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratchLock;
+static float4* g_scratchAt[kMaxDevices];  // the array's device address, per device
+
+// The array's address on the current device, looked up once per device. In host
+// code `scratch` names a host-side stand-in, never the device memory.
+static float4* staticBoxes(std::string& err) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    err = "no current HIP device";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratchLock);
+  if (!g_scratchAt[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratchAt[dev] = static_cast<float4*>(p);
+  }
+  return g_scratchAt[dev];
+}
+
+}  // namespace f2d
+
+// f2dg_query_device with the packed boxes in the static array instead of a buffer
+// the library allocates: no allocation of any kind on this path. The batch must be
+// uploaded already (f2dg_upload or a step) with no host edits pending.
+extern "C" int f2dg_query_device_static(void* h, const void* queries, int n, void* hits, void* counts, int cap) {
+  f2d::Batch* b = static_cast<f2d::Batch*>(h);
+  if (!b || n < 0 || cap < 1) return -1;
+  if (!b->committed || b->hostDirty) {
+    b->lastError = "f2dg_query_device_static: upload the batch first (it allocates nothing itself)";
+    return -1;
+  }
+  if (b->nShapes > f2d::kStaticBoxCapacity) {
+    b->lastError = "f2dg_query_device_static: the batch has " + std::to_string(b->nShapes) +
+                   " shapes, more than the static box array holds (" + std::to_string(f2d::kStaticBoxCapacity) + ")";
+    return -1;
+  }
+  float4* boxes = f2d::staticBoxes(b->lastError);
+  if (!boxes) return -1;
+  if (n == 0) return 0;
+  return f2d_launch_queries(&b->dev.view, boxes, b->nShapes, static_cast<const f2d::Query*>(queries), n,
+                            static_cast<f2d::QueryHit*>(hits), static_cast<int*>(counts), cap, b->stream);
+}

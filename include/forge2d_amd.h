@@ -281,6 +281,14 @@ int f2dg_query(void* batch, const f2dQuery* queries, int n, f2dQueryHit* hits, i
 /* Device arrays (queries, hits, counts), kinds 1-3 only: a packing pass over the shapes'
  * fat AABBs and the query launch on the batch's stream, no synchronization. */
 int f2dg_query_device(void* batch, const void* queries, int n, void* hits, void* counts, int cap);
+/* No counterpart in the reference: f2dg_query_device for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The packed boxes go to a static device array of 262,144 boxes (4 MiB per GPU, held
+ * until the process ends) instead of a library buffer; nothing is allocated. The
+ * batch must be uploaded with no host edits pending, and hold at most 262,144
+ * shapes; otherwise -1 and f2dg_last_error says why. Calls on one GPU share the
+ * array: calls whose launches overlap on one GPU (other streams or threads) race. */
+int f2dg_query_device_static(void* batch, const void* queries, int n, void* hits, void* counts, int cap);
 
 #ifdef __cplusplus
 }
