@@ -339,10 +339,10 @@ class Batch:
         return [dict(zip(keys, out[7 * k:7 * k + 7])) for k in range(n)]
 
     def stats(self) -> dict:
-        out = (ctypes.c_long * 8)()
-        lib().f2dg_batch_stats(self.h, out)
+        out = (ctypes.c_long * 10)()
+        self._check(lib().f2dg_batch_stats(self.h, out), "batch_stats")
         keys = ["bodies", "shapes", "contacts", "points", "restitution_contacts", "restitution_points", "joints",
-                "overflow"]
+                "overflow", "tree_proxies", "live_contacts"]
         return dict(zip(keys, list(out)))
 
     def device_bytes(self) -> int:

@@ -1,3 +1,4 @@
+// Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
 // C-ABI of the MI355X world step.
 //
 // Two entry families, both plain C (scalars, pointers, sizes):
@@ -2287,6 +2288,11 @@ int f2dg_autotune(void* h, float dt, int subSteps, int steps, float* msOut) {
       }
     }
     c.launch.block = bestBlock;
+    // classes with fewer candidates run untimed steps so that every class, and so
+    // every world, advances by the same 9 x steps trial steps
+    for (int s = 0; s < (5 - nCands) * steps; ++s) {
+      if (f2d_launch_step(&b->dev.view, &sp, c.dList, c.count, b->stream, &c.launch) != 0) return -1;
+    }
     // LDS room for constraint lanes: more keeps more worlds' constraints on chip,
     // less lets more workgroups share a compute unit; every class runs the same
     // number of trial steps, so all worlds advance equally
@@ -2380,13 +2386,16 @@ int f2dg_batch_stats(void* h, long* out) {
   Batch* b = static_cast<Batch*>(h);
   if (!b) return -1;
   if (download(*b) != 0) return -1;
-  long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  long v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (const HostWorld& hw : b->worlds) {
     for (const BodySim& s : hw.sims) v[0] += s.setIndex == kAwakeSet ? 1 : 0;
     v[1] += (long)hw.shapes.size();
+    v[8] += hw.tree[1].proxyCount + hw.tree[2].proxyCount;
     for (int i = 0; i < hw.contactCount && i < (int)hw.contacts.size(); ++i) {
       const Contact& c = hw.contacts[i];
-      if (c.shapeA < 0 || (c.flags & CF_TOUCHING) == 0) continue;
+      if (c.shapeA < 0) continue;
+      v[9] += 1;
+      if ((c.flags & CF_TOUCHING) == 0) continue;
       v[2] += 1;
       v[3] += c.m.pointCount;
       if (c.restitution > 0.0f) {
@@ -2397,7 +2406,7 @@ int f2dg_batch_stats(void* h, long* out) {
     v[6] += (long)hw.joints.size();
     v[7] += (long)hw.overflow.size();
   }
-  for (int k = 0; k < 8; ++k) out[k] = v[k];
+  for (int k = 0; k < 10; ++k) out[k] = v[k];
   return 0;
 }
 

@@ -1,3 +1,4 @@
+// Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
 // Scalar 2D math with the reference's exact operation order.
 //
 // Every function here evaluates the same IEEE-754 single-precision operations in

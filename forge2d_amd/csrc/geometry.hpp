@@ -1,3 +1,4 @@
+// Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
 // Host-side shape construction and mass properties with the reference's exact
 // arithmetic, so a scene built here is bit-identical to one built by the
 // reference: b2MakeBox/b2MakeOffsetRoundedBox S/geometry.c:143-218,

@@ -1,3 +1,4 @@
+// Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
 // Island graph bookkeeping: union-find linking of touching contacts and joints,
 // merging of awake islands, and the depth-first split of an island that lost
 // constraints (S/island.c). The same code runs on the host (joint creation) and

@@ -1,3 +1,4 @@
+// Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
 // Revolute joint stages, one joint per lane, in the reference's operation order:
 // prepare S/joint.c:1348-1355 + S/revolute_joint.c:202-259, warm start :261-287,
 // solve :289-465. Static bodies read the identity body state as in the reference.

@@ -1,3 +1,4 @@
+// Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
 // Per-pair contact manifolds with Box2D v3.1 feature ids.
 //
 // Each function computes the manifold of one shape pair with the operation

@@ -1,3 +1,4 @@
+// Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
 // World queries on a device-resident world: ray casts against each shape type,
 // GJK distance with witness points, the conservative-advancement shape cast that
 // rounded polygons are ray cast with, and the explosion's projected perimeter —

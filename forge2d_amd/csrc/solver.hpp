@@ -1,3 +1,4 @@
+// Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
 // Contact and body solver stages, one constraint (or body) per lane.
 //
 // Coloured constraints follow the 4-wide SIMD solver's per-lane arithmetic

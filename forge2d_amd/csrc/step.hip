@@ -1,3 +1,4 @@
+// Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
 // Fused world step for gfx950: one workgroup advances one world through a whole
 // b2World_Step (S/world.c:695-812) with workgroup barriers between the stages.
 //

@@ -1,3 +1,4 @@
+// Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
 // Time of impact for continuous collision: GJK distance (S/distance.c:421-606)
 // and conservative advancement over separating axes (S/distance.c:931-1414), in
 // the reference's operation order so the fractions agree bit for bit.

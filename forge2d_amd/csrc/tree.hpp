@@ -1,3 +1,4 @@
+// Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
 // Broad-phase bounding-volume trees of one world (b2DynamicTree, S/dynamic_tree.c).
 //
 // Each world keeps the reference's three trees (static, kinematic and dynamic

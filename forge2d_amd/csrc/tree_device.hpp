@@ -1,3 +1,4 @@
+// Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
 // The partial tree rebuild of one world on one wavefront (b2DynamicTree_Rebuild with
 // fullBuild false, S/dynamic_tree.c:1717-1989), with the result of the reference's
 // serial code: the same nodes, links, boxes, heights, node ids and free list.

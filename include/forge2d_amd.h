@@ -228,7 +228,8 @@ const char* f2dg_last_error(void* batch);
  * 512 and 1024 for classes of more than 256 shapes) and keeps the fastest (ms_out[3]
  * optional: whole-batch ms per 64/128/256 candidate), then times 4 solve
  * LDS sizes (room for 0, 0.5, 1 and 2 constraint lanes per body) and keeps the fastest;
- * the trial steps advance every world equally; returns the largest class's lanes */
+ * classes with 3 lane candidates run 2 x steps untimed steps more, so every world advances by
+ * 9 x steps trial steps; returns the largest class's lanes */
 int f2dg_autotune(void* batch, float dt, int sub_steps, int steps, float* ms_out);
 /* worlds are launched in size classes by max(bodies, shapes) <= 64, 128, ..., 1024, larger;
  * per class: [limit, worlds, max bodies, max shapes, workgroup lanes, collide LDS bytes,
@@ -243,8 +244,9 @@ int f2dg_set_class_block(void* batch, int cls, int block);
 /* force the solve launch's dynamic LDS bytes of size class `cls` (body terms of its
  * largest world + constraint lanes; worlds whose lanes do not fit keep them in HBM) */
 int f2dg_set_class_solve_lds(void* batch, int cls, int bytes);
-/* aggregate counts after f2dg_sync: [awake bodies, shapes, touching contacts, touching points,
- * restitution contacts, restitution points, joints, overflow contacts] */
+/* aggregate counts after f2dg_sync (10 longs): [awake bodies, shapes, touching contacts,
+ * touching points, restitution contacts, restitution points, joints, overflow contacts,
+ * proxies of the kinematic and dynamic trees, live contacts (touching or not)] */
 int f2dg_batch_stats(void* batch, long* out);
 /* events of one world's last step, in the f2d_world_get_*_events layouts above (shape and
  * body world_and_generation fields are 0); counts out = [begin, end, hit] */

@@ -1,3 +1,4 @@
+/* Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES). */
 /*
  * Oracle: a CPU restatement of the Box2D v3.1.1 world step that forge2d's
  * World.step drives. TEST INFRASTRUCTURE ONLY: the parity tests, the smoke

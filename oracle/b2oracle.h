@@ -1,3 +1,4 @@
+/* Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES). */
 /*
  * Oracle: CPU restatement of the Box2D v3.1.1 world step (test infrastructure
  * only; see b2oracle.c). Built twice by oracle/Makefile: liboracle_f32.so with

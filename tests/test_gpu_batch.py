@@ -59,6 +59,19 @@ def test_workgroup_size_does_not_change_results(gpu, block):
         assert ref.export(w) == other.export(w), w
 
 
+def test_autotune_advances_every_world_equally(gpu):
+    """The autotuner's trial steps advance worlds of small classes (3 lane
+    candidates) and of large classes (5 candidates) by the same step count."""
+    worlds = scenes.config4(worlds=4, lo=10, hi=60) + [scenes.container_world(600, 20.0, 40.0, 7006, chains=2)]
+    b = F.Batch(worlds)
+    b.upload()
+    assert min(c["limit"] for c in b.size_classes()) <= 256 < max(c["limit"] for c in b.size_classes())
+    b.autotune(steps=2)
+    assert b.sync() == 0
+    steps = {b.counters(w)["step"] for w in range(len(worlds))}
+    assert steps == {18}, steps
+
+
 def test_step_subset_leaves_other_worlds(gpu):
     worlds = scenes.config3(worlds=6, rows_lo=5, rows_hi=8)
     b = F.Batch(worlds)
