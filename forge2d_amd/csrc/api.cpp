@@ -883,6 +883,9 @@ static int upload(Batch& b) {
       if (sh.proxyKey >= 0 && (sh.proxyKey & 3) == kStatic) staticShapes[h.shapeOffset + h.staticCount++] = k;
       sweepOrder[h.shapeOffset + k] = k;  // any permutation; the first pair update sorts it
     }
+    // continuous collision visits static candidates in the static tree's query order
+    std::sort(staticShapes.begin() + h.shapeOffset, staticShapes.begin() + h.shapeOffset + h.staticCount,
+              [&](int x, int y) { return hw.treeRanks[x] < hw.treeRanks[y]; });
     h.contactCount = hw.contactCount;
     h.freeCount = (int)hw.freeIds.size();
     h.overflowCount = (int)hw.overflow.size();

@@ -350,7 +350,7 @@ struct BatchView {
   int* scratch;         // large-world sort scratch
   float4* sapBoxes;     // large-world sorted fat AABBs
   int8_t* ccolor;       // per contact slot: constraint-graph colour, -1 when none or free
-  int* staticShapes;    // per world, the shape ids of its static proxies in shape order
+  int* staticShapes;    // per world, the shape ids of its static proxies in the static tree's query order
   int* sweepOrder;      // per world, its shape ids in the last pair update's sweep order
   int* events;          // per world at 3 * contactOffset: begin contact ids [contactCap], end shape pairs [2 * contactCap]
   int* islandData;      // per world at islandOffset: the island region

@@ -1726,8 +1726,11 @@ __device__ __forceinline__ bool queryVisible(const Shape& t, AABB box) {
 }
 
 // b2SolveContinuous, S/solver.c:390-541, for fast body i. Non-bullets sweep against
-// the static proxies (listed in shape order); bullets also against kinematic and
-// dynamic proxies, after every non-bullet body has its final transform.
+// the static proxies; bullets also against kinematic and dynamic proxies, after
+// every non-bullet body has its final transform. Candidates are visited in each
+// tree's query order (b2DynamicTree_Query, S/dynamic_tree.c:1114-1170: the leaves
+// whose fat boxes overlap, in depth-first order): each hit bounds the next
+// candidate's root search, so the order fixes the fraction's bits when several hit.
 // The static proxies' fat boxes and visibility staged in LDS by the continuous
 // kernel (the first kMaxStagedStatics; the rest are read from their records)
 constexpr int kMaxStagedStatics = 64;
@@ -1762,11 +1765,24 @@ __device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList
       fraction = continuousCandidate(w.shapes, w.geoms, w.sims, w.joints, i, s, sweep, t, fraction);
     }
     if (bullet) {
+      // the visible proxies of each tree by ascending query rank; a scan finds the
+      // next one (bullets are few)
       for (int type = kKinematic; type <= kDynamic; ++type) {
-        for (int t = 0; t < ns; ++t) {
-          const Shape& ts = w.shapes[t];
-          if ((ts.proxyKey & 3) != type || !queryVisible(ts, box)) continue;
-          fraction = continuousCandidate(w.shapes, w.geoms, w.sims, w.joints, i, s, sweep, t, fraction);
+        int last = -1;
+        for (int n = 0; n < ns; ++n) {
+          int next = -1, nextRank = 0x7FFFFFFF;
+          for (int t = 0; t < ns; ++t) {
+            const Shape& ts = w.shapes[t];
+            if ((ts.proxyKey & 3) != type || !queryVisible(ts, box)) continue;
+            const int r = w.treeRanks[t];
+            if (r > last && r < nextRank) {
+              nextRank = r;
+              next = t;
+            }
+          }
+          if (next < 0) break;
+          fraction = continuousCandidate(w.shapes, w.geoms, w.sims, w.joints, i, s, sweep, next, fraction);
+          last = nextRank;
         }
       }
     }

@@ -3560,8 +3560,35 @@ static int query_visible(const OShape* t, AB box) {
   return t->cat != 0 && t->proxyKey >= 0 && aabb_overlaps(t->fat, box);
 }
 
-/* b2SolveContinuous :390-541. Candidates in shape id order (the reference visits its
-   static tree; the minimum fraction is order independent unless two hits tie). */
+/* b2DynamicTree_Query (S/dynamic_tree.c:1114-1170) with b2ContinuousQueryCallback: the
+   candidates of one fast shape in the tree's depth-first order (child2 popped first).
+   Each hit bounds the next candidate's root search, so the order fixes the bits of
+   the final fraction when several candidates hit. */
+static real tree_query_continuous(OWorld* w, const OTree* t, AB box, int b, int s, const Proxy* proxyB,
+                                  const Sweep* sweep, real fraction) {
+  if (t->nodeCount == 0) return fraction;
+  int* stack = (int*)xcalloc(t->fresh + 1, sizeof(int));
+  int sp = 0;
+  stack[sp++] = t->root;
+  while (sp > 0) {
+    int id = stack[--sp];
+    const TNode* n = t->n + id;
+    if (aabb_overlaps(n->box, box) && n->cat != 0) {
+      if (n->flags & TN_LEAF) {
+        fraction = continuous_candidate(w, b, s, proxyB, sweep, n->c1, fraction);
+      } else {
+        stack[sp++] = n->c1;
+        stack[sp++] = n->c2;
+      }
+    }
+  }
+  free(stack);
+  return fraction;
+}
+
+/* b2SolveContinuous :390-541: the static tree, and for bullets the kinematic and the
+   dynamic tree, queried with the union of the shape's boxes at the start and the end
+   of the step. Snapshots without trees fall back to shape id order. */
 static void solve_continuous(OWorld* w, int b) {
   OBody* sim = w->bodies + b;
   Sweep sweep = make_sweep(sim);
@@ -3579,6 +3606,14 @@ static void solve_continuous(OWorld* w, int b) {
     fs->aabb = box2;
     if (fs->sensor) continue;
     Proxy proxyB = make_proxy(&fs->g);
+    if (w->hasTrees) {
+      fraction = tree_query_continuous(w, w->trees + STATIC_BODY, box, b, s, &proxyB, &sweep, fraction);
+      if (sim->bullet) {
+        fraction = tree_query_continuous(w, w->trees + KINEMATIC_BODY, box, b, s, &proxyB, &sweep, fraction);
+        fraction = tree_query_continuous(w, w->trees + DYNAMIC_BODY, box, b, s, &proxyB, &sweep, fraction);
+      }
+      continue;
+    }
     for (int type = STATIC_BODY; type <= DYNAMIC_BODY; ++type) {
       if (type != STATIC_BODY && !sim->bullet) break;
       for (int t = 0; t < w->shapeCount; ++t) {

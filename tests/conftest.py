@@ -9,7 +9,8 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-FIXTURE_NAMES = ["pyramid12", "pyramid40", "mixed", "overflow", "bounce", "ccd", "mixed_ccd", "events", "sleep"]
+FIXTURE_NAMES = ["pyramid12", "pyramid40", "mixed", "overflow", "bounce", "ccd", "mixed_ccd", "events", "sleep",
+                 "features"]
 
 
 def pytest_configure(config):

@@ -170,6 +170,91 @@ def sleep_world():
     return w
 
 
+def features_world():
+    """Body, shape and joint options that the other fixtures leave at their
+    defaults: kinematic bodies (a moving platform and a spinner), linear and
+    angular damping, gravity scales of 0.5, 2 and -0.3, fixed rotation, fast
+    rotation, a conveyor (tangent speed, S/contact_solver.c), collision filter
+    groups and category masks (S/shape.c b2ShouldShapesCollide), rounded boxes and
+    polygons, and revolute joints with a motor, a spring, a motor within limits and
+    collideConnected (S/revolute_joint.c:202-465)."""
+    rng = scenes.SplitMix64(5151)
+    w = scenes.World(scenes.world_header(sleep=False, continuous=True))
+    g = w.add_body(scenes.STATIC, 0.0, -1.0)
+    w.add_box(g, 40.0, 1.0, friction=0.7)
+    ceiling = w.add_body(scenes.STATIC, 0.0, 0.0)
+    w.add_segment(ceiling, (20.0, 9.0), (30.0, 9.0))
+    w.add_segment(ceiling, (-30.0, 3.0), (-22.0, 1.0))  # ramp
+    # conveyor belt: a static box whose surface moves at 2 m/s
+    conv = w.add_body(scenes.STATIC, -14.0, 2.0)
+    w.add_box(conv, 4.0, 0.25, friction=0.8, tangent_speed=2.0)
+    for i in range(4):
+        b = w.add_body(scenes.DYNAMIC, -16.5 + 1.2 * i, 2.9 + 0.3 * i)
+        w.add_box(b, 0.4, 0.4, friction=0.8, radius=0.05 * (i % 2))
+    # kinematic platform carrying boxes, and a kinematic spinner
+    plat = w.add_body(scenes.KINEMATIC, 0.0, 3.0, vx=0.5)
+    w.add_box(plat, 3.0, 0.2)
+    for i in range(3):
+        b = w.add_body(scenes.DYNAMIC, -1.5 + 1.5 * i, 3.75)
+        w.add_box(b, 0.5, 0.5, friction=0.6)
+    spin = w.add_body(scenes.KINEMATIC, 8.0, 1.5, w=1.5)
+    w.add_capsule(spin, (-1.5, 0.0), (1.5, 0.0), 0.2)
+    for i in range(4):
+        b = w.add_body(scenes.DYNAMIC, 6.8 + 0.8 * i, 3.0 + 0.6 * i)
+        w.add_circle(b, 0.25)
+    # damping and gravity scale
+    for i, (ld, ad) in enumerate(((0.8, 1.5), (0.2, 0.0), (0.0, 4.0))):
+        b = w.add_body(scenes.DYNAMIC, 12.0 + 1.5 * i, 2.0, vx=4.0 - 3.0 * i, w=5.0, lin_damp=ld, ang_damp=ad)
+        w.add_circle(b, 0.3)
+    for i, gs in enumerate((0.5, 2.0, -0.3)):
+        b = w.add_body(scenes.DYNAMIC, 22.0 + 2.0 * i, 5.0, gravity_scale=gs, angle=0.3 * i)
+        w.add_box(b, 0.4, 0.3)
+    # fixed rotation on the ramp; fast rotation
+    b = w.add_body(scenes.DYNAMIC, -27.0, 4.0, flags=scenes.BF_FIXED_ROT | scenes.BF_AWAKE | scenes.BF_ENABLE_SLEEP)
+    w.add_box(b, 0.5, 0.25)
+    b = w.add_body(scenes.DYNAMIC, -24.5, 4.0, angle=0.5, flags=scenes.BF_FIXED_ROT | scenes.BF_AWAKE)
+    w.add_capsule(b, (-0.3, 0.0), (0.3, 0.0), 0.2)
+    b = w.add_body(scenes.DYNAMIC, 17.0, 1.0, w=60.0, flags=scenes.BF_FAST_ROT | scenes.BF_AWAKE)
+    w.add_box(b, 0.6, 0.1)
+    # filter groups: a negative group never collides, a positive group always does
+    for i in range(2):
+        b = w.add_body(scenes.DYNAMIC, -6.0 + 0.3 * i, 1.0 + 0.9 * i)
+        w.add_box(b, 0.5, 0.5, group=-3)
+    for i in range(2):
+        b = w.add_body(scenes.DYNAMIC, -8.5 + 0.2 * i, 1.0 + 1.0 * i)
+        w.add_box(b, 0.5, 0.5, group=4, category=(2, 0), mask=(1, 0))
+    # category masks: category 4 ignores category 2; a circle that masks out the ground
+    b = w.add_body(scenes.DYNAMIC, -8.4, 4.5)
+    w.add_circle(b, 0.3, category=(4, 0), mask=(0xFFFFFFFD, 0xFFFFFFFF))
+    b = w.add_body(scenes.DYNAMIC, 30.0, 3.0)
+    w.add_circle(b, 0.3, mask=(0xFFFFFFFE, 0xFFFFFFFF))
+    # rounded polygons
+    for i in range(4):
+        b = w.add_body(scenes.DYNAMIC, 3.0 + 0.9 * i, 6.0 + rng.uniform(0.0, 1.0), angle=rng.uniform(-1.0, 1.0))
+        pts = [(-0.3, -0.2), (0.35, -0.25), (0.1, 0.3)] if i % 2 == 0 else [(-0.3, -0.3), (0.3, -0.3), (0.3, 0.3),
+                                                                            (-0.3, 0.3), (-0.4, 0.0)]
+        w.add_polygon(b, pts, radius=0.04 + 0.03 * i)
+    # revolute joints: motor wheel, spring pendulum, motor within limits, collideConnected
+    anchor = w.add_body(scenes.STATIC, -2.0, 8.0)
+    wheel = w.add_body(scenes.DYNAMIC, -2.0, 8.0)
+    w.add_box(wheel, 1.0, 0.1)
+    w.add_revolute(anchor, wheel, (0.0, 0.0), (0.0, 0.0), enable_motor=True, motor_speed=3.0, max_motor_torque=20.0)
+    bob = w.add_body(scenes.DYNAMIC, 1.5, 8.0)
+    w.add_circle(bob, 0.3)
+    w.add_revolute(anchor, bob, (3.5, 0.0), (0.0, 0.0), enable_spring=True, hertz=2.0, damping=0.3,
+                   enable_limit=True, lower=-0.8, upper=0.8)
+    arm = w.add_body(scenes.DYNAMIC, -5.0, 8.0)
+    w.add_box(arm, 1.0, 0.15)
+    w.add_revolute(anchor, arm, (-2.0, 0.0), (1.0, 0.0), enable_limit=True, lower=-0.4, upper=0.6,
+                   enable_motor=True, motor_speed=-2.0, max_motor_torque=5.0)
+    a = w.add_body(scenes.DYNAMIC, 10.0, 6.0)
+    w.add_box(a, 0.6, 0.3)
+    c = w.add_body(scenes.DYNAMIC, 10.8, 6.0)
+    w.add_box(c, 0.6, 0.3)
+    w.add_revolute(a, c, (0.5, 0.0), (-0.3, 0.0), collide_connected=True)
+    return w
+
+
 def query_world():
     """World queries (S/world.c:2050-2308, 2640-2747): every shape type on static,
     kinematic and dynamic bodies, rounded boxes (ray cast by shape cast), a body with
@@ -312,7 +397,7 @@ EVENT_FIXTURES = {"events": EVENT_STEPS, "sleep": list(range(1, 261))}
 FIXTURES = {
     # name: (world factory, dumped steps)
     "pyramid12": (lambda: scenes.pyramid_world(12), [0, 1, 2, 3, 4, 5, 60, 61, 62]),
-    "pyramid40": (lambda: scenes.pyramid_world(40), [0, 1, 2, 3]),
+    "pyramid40": (lambda: scenes.pyramid_world(40), [0, 1, 2, 3, 254, 255, 256]),
     "mixed": (lambda: scenes.container_world(150, 12.0, 24.0, 20251202, continuous=False, chains=2,
                                              drop_band=(0.1, 0.6)),
               [0, 1, 2, 3, 4, 5, 6, 30, 31, 32, 60, 61, 62, 120, 121, 122]),
@@ -321,10 +406,55 @@ FIXTURES = {
     "ccd": (ccd_world, list(range(0, 13)) + [30, 31, 32]),
     "events": (events_world, list(range(0, 121))),
     "sleep": (sleep_world, list(range(0, 261))),
+    "features": (features_world, list(range(0, 7)) + [30, 31, 32, 60, 61, 62, 120, 121, 122]),
     "mixed_ccd": (lambda: scenes.container_world(200, 15.0, 40.0, 20261202, continuous=True, chains=3,
                                                  drop_band=(0.2, 1.0)),
                   [0, 1, 2, 3, 40, 41, 42, 43, 44, 45, 46, 47, 48, 49, 50, 60, 61, 62]),
 }
+
+
+# large worlds whose snapshots are too big to keep: the reference's state digests
+# (tests.parity_util.state_digest) after the listed steps, from the initial state
+DIGEST_FIXTURES = {
+    # config 2: one 20,000-body world; its first steps and a window after 300 steps
+    "config2": (lambda: scenes.config2()[0], [1, 2, 3, 300, 301, 600]),
+}
+
+
+def config2_outside(px, py) -> list:
+    """Dynamic bodies whose centre left config 2's container [-100, 100] x [0, 200]."""
+    px, py = np.asarray(px)[1:], np.asarray(py)[1:]
+    return (np.nonzero((np.abs(px) >= 100.0) | (py <= 0.0) | (py >= 200.0))[0] + 1).tolist()
+
+
+def make_digest(name: str):
+    import hashlib
+    import json
+
+    from forge2d_amd import snapshot as S
+    from tests import parity_util as P
+
+    factory, steps = DIGEST_FIXTURES[name]
+    spec = scenes.to_bytes([factory()])
+    out = {"spec_sha256": hashlib.sha256(spec).hexdigest(), "steps": steps, "digests": []}
+    with tempfile.TemporaryDirectory() as td:
+        sp, dp = os.path.join(td, "scene.bin"), os.path.join(td, "snaps.bin")
+        with open(sp, "wb") as f:
+            f.write(spec)
+        subprocess.run([HARNESS, "dump", sp, dp, "0", str(max(steps)), ",".join(map(str, steps))], check=True)
+        with open(dp, "rb") as f:
+            snaps = S.parse_all(f.read())
+    for sn in snaps:
+        d = P.state_digest(sn)
+        if name == "config2":
+            d["outside_container"] = config2_outside(sn.bodies["px"], sn.bodies["py"])
+        out["digests"].append(d)
+    path = os.path.join(HERE, name + "_digest.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=0)
+    print(f"{name}: digests of steps {steps} -> {os.path.getsize(path)} B "
+          f"(live contacts {[d['live_contacts'] for d in out['digests']]}, "
+          f"fast bodies {[d['fast_bodies'] for d in out['digests']]})")
 
 
 def make(name: str):
@@ -360,8 +490,10 @@ def make(name: str):
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
-    for n in sys.argv[1:] or list(FIXTURES) + list(QUERY_FIXTURES):
-        if n in QUERY_FIXTURES:
+    for n in sys.argv[1:] or list(FIXTURES) + list(QUERY_FIXTURES) + list(DIGEST_FIXTURES):
+        if n in DIGEST_FIXTURES:
+            make_digest(n)
+        elif n in QUERY_FIXTURES:
             make_queries_fixture(n)
         else:
             make(n)

@@ -194,3 +194,47 @@ def query_mismatches(queries, expected, counts, hits) -> list:
                 bad.append(i)
                 break
     return bad
+
+
+def _h(a) -> str:
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()[:32]
+
+
+DIGEST_BODY_FIELDS = ["type", "set_index", "flags", "px", "py", "qc", "qs", "cx", "cy", "c0x", "c0y", "q0c", "q0s",
+                      "vx", "vy", "w", "inv_mass", "inv_i", "min_ext", "max_ext", "sleep_time"]
+DIGEST_CONTACT_FIELDS = ["id", "shape_a", "shape_b", "flags", "color", "sim_flags", "point_count", "friction",
+                         "restitution", "rolling", "tangent_speed", "nx", "ny", "rolling_impulse", "pts"]
+
+
+def state_digest(s: S.Snapshot) -> dict:
+    """Hashes of one snapshot's state, field by field, for comparing a large world
+    with a reference run without storing the reference's snapshots: bodies, shape
+    boxes and proxy keys, live contacts by id with their manifolds and impulses,
+    colour arrays, the free-id list, the move array and the trees."""
+    d = {"step": int(s.step), "bodies": int(len(s.bodies)), "live_contacts": int(len(s.live_contacts())),
+         "fast_bodies": int(((s.bodies["flags"] & 1) != 0).sum())}
+    for f in DIGEST_BODY_FIELDS:
+        d["body." + f] = _h(s.bodies[f])
+    for f in ("aabb", "fat", "proxy_key"):
+        d["shape." + f] = _h(s.shapes[f])
+    live = np.sort(s.live_contacts(), order="id")
+    for f in DIGEST_CONTACT_FIELDS:
+        d["contact." + f] = _h(live[f])
+    d["colors"] = _h(np.concatenate([np.asarray(c, np.int32) for c in s.colors] +
+                                    [np.array([len(c) for c in s.colors], np.int32)]))
+    d["free_ids"] = _h(np.asarray(s.free_ids, np.int32))
+    d["move"] = _h(np.asarray(s.move, np.int32))
+    if s.trees is not None:
+        for k, t in enumerate(s.trees):
+            name = ("static", "kinematic", "dynamic")[k]
+            d[f"tree.{name}"] = _h(np.array([t.root, t.node_count, t.proxy_count, t.fresh], np.int32))
+            d[f"tree.{name}.freed"] = _h(np.asarray(t.freed, np.int32))
+            for f in ("child1", "child2", "parent", "height", "flags", "aabb"):
+                d[f"tree.{name}.{f}"] = _h(t.nodes[f])
+    return d
+
+
+def digest_differences(mine: dict, ref: dict) -> list:
+    """Names of the digest entries that differ."""
+    return [k for k in ref if mine.get(k) != ref[k]]

@@ -4,11 +4,10 @@ Each scene is stepped on the device; after every step the oracle restarts from
 the device's own previous state (its broad-phase trees included), runs the pair
 update over its own trees -- which fixes the new contacts and their ids
 independently of the device's sort and sweep -- and computes the step itself. The
-device's contact ids, manifolds, colours, impulses, bodies, fat AABBs, move array,
-island graph and trees must equal the oracle's bit for bit, except for the fast
-bodies of a continuous sweep that several candidates hit (visited in a different
-order; compared within 1e-5) -- see tests/test_oracle.py. The contact set after a
-step must also equal the brute-force overlap set of the state before it.
+device's contact ids, manifolds, colours, impulses, bodies (fast bodies of
+continuous collision included), boxes, move array, island graph and trees must
+equal the oracle's bit for bit. The contact set after a step must also equal the
+brute-force overlap set of the state before it.
 
 The oracle needs the reference's own shape geometry (hulls, rounded boxes); it is
 written by the reference build's fixture harness (oracle/_ref/ref_harness geom).
@@ -30,7 +29,6 @@ from tests.golden import make_golden
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
-TOL = 1e-5
 
 # (scene, steps): the last two exceed the 64 KB of LDS a world may use, so their
 # body states (pyramid48) and their sort-and-sweep too (container2600) run from
@@ -72,18 +70,15 @@ def compare(mine, o):
     assert np.array_equal(mine.free_ids, o.free_ids)
     assert P.color_map(mine) == P.color_map(o)
     assert all(np.array_equal(a, c) for a, c in zip(mine.colors, o.colors)), "colour array order"
-    fast = (o.bodies["flags"] & 1) != 0
     for f in P.BODY_FIELDS + ["c0x", "c0y", "q0c", "q0s"]:
-        assert np.array_equal(mine.bodies[f][~fast], o.bodies[f][~fast]), f
-        assert P.rel_err(mine.bodies[f][fast], o.bodies[f][fast]) <= TOL, f
-    rep = P.manifold_report(mine, o, with_impulses=not fast.any())
+        assert np.array_equal(mine.bodies[f], o.bodies[f]), f
+    rep = P.manifold_report(mine, o, with_impulses=True)
     assert rep["point_count"] == rep["ids"] == rep["normals"] == rep["values"] == rep["impulses"] == 0, rep
-    fast_shapes = fast[o.shapes["body"]]
-    assert np.array_equal(mine.shapes["fat"][~fast_shapes], o.shapes["fat"][~fast_shapes])
-    if not fast.any():
-        assert np.array_equal(mine.move, o.move)
+    for f in ("aabb", "fat"):
+        assert np.array_equal(mine.shapes[f], o.shapes[f]), f
+    assert np.array_equal(mine.move, o.move)
     assert P.island_differences(mine, o) == []
-    assert P.tree_differences(mine, o, TOL if fast.any() else 0.0) == []
+    assert P.tree_differences(mine, o, 0.0) == []
 
 
 @pytest.mark.skipif(not (O.available("f32") and os.path.exists(HARNESS)), reason="oracle or reference build missing")

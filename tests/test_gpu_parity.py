@@ -8,11 +8,13 @@ For every pair of consecutive dumped steps (k-1, k) of every fixture:
     new contacts get the reference's ids (the pair update follows the broad-phase
     trees, which are kept and compared node for node); colours, the overflow order
     and the free-id list equal; body position, rotation and velocity within 1e-5
-    relative (floor 1: absolute below unit magnitude) — in practice bit-exact.
-  * solver + finalize: the reference's post-collide state of step k (colours,
-    manifolds, impulses matched from k-1, bodies from k-1) stepped through the
-    solver stages and finalize reproduces bodies, stored impulses, fat AABBs and
-    the next move buffer of step k, within 1e-5 (in practice bit-exact).
+    relative (floor 1: absolute below unit magnitude), and in fact bit for bit:
+    continuous collision visits its candidates in the trees' query order.
+  * solver + finalize: the post-collide state of step k (colours, manifolds,
+    impulses matched from k-1, bodies from k-1, trees rebuilt), as the oracle's
+    pair update and collide compute it from the reference's step k-1, stepped
+    through the solver stages and finalize reproduces the reference's bodies,
+    stored impulses, boxes and next move buffer of step k, bit for bit.
 The ccd fixtures fire fast bodies and bullets at thin static geometry, so these
 cases also cover the time-of-impact sweeps of continuous collision. The sleep
 fixture (every step 0-260) covers islands: links, merges, splits, islands put to
@@ -73,15 +75,13 @@ def test_full_step_from_reference_state(gpu, name, i):
     assert all(np.array_equal(a, c) for a, c in zip(mine.colors, ref.colors)), "colour array order"
     errs = P.body_errors(mine, ref)
     assert max(errs.values()) <= TOL, errs
+    # in practice bit for bit, fast bodies of continuous collision included
+    assert P.body_mismatch_count(mine, ref, P.BODY_FIELDS + ["c0x", "c0y", "q0c", "q0s"]) == 0
     rep = P.manifold_report(mine, ref, with_impulses=True)
     assert rep["impulses"] == 0, rep
     assert P.island_differences(mine, ref) == []
-    # leaves of fast bodies carry boxes within the sweep tolerance (several hits of
-    # one sweep are visited in another order than the reference's)
-    fast = (ref.bodies["flags"] & 1) != 0
-    assert P.tree_differences(mine, ref, TOL if fast.any() else 0.0) == []
-    if not fast.any():
-        assert np.array_equal(mine.move, ref.move)
+    assert P.tree_differences(mine, ref, 0.0) == []
+    assert np.array_equal(mine.move, ref.move)
 
 
 def sleep_enabled(spec: bytes) -> bool:
@@ -100,16 +100,16 @@ def test_solver_and_finalize_from_post_collide_state(gpu, name, i):
     spec, snaps, raws = load_fixture(name)
     prev, ref = snaps[i - 1], snaps[i]
     print(f"[parity] solve {name} {prev.step}->{ref.step}", flush=True)
-    if sleep_enabled(spec):
-        # with sleep on, the solver also merges, splits and sleeps islands: start it
-        # from the post-collide state the oracle's pair update and collide (pinned
-        # bit-exact to the reference) compute, island graph included
-        o = O.Oracle(spec, fixture_geoms(name), raws[i - 1])
-        o.update_pairs()
-        o.collide()
-        post = o.export()
-    else:
-        post = S.to_bytes(S.post_collide(prev, ref))
+    # the post-collide state as the oracle's pair update and collide compute it
+    # (pinned bit-exact to the reference): contacts, manifolds and matched impulses,
+    # the island graph, and the trees rebuilt at the start of the step, whose query
+    # order continuous collision follows
+    o = O.Oracle(spec, fixture_geoms(name), raws[i - 1])
+    o.update_pairs()
+    o.collide()
+    post = o.export()
+    ref_post = S.post_collide(prev, ref)
+    assert np.array_equal(S.parse_one(post).pair_keys(), ref_post.pair_keys())
     b = F.Batch(spec)
     b.import_state(0, post)
     b.step(steps=1, phases=F.PH_SOLVE | F.PH_FINALIZE)
@@ -117,16 +117,12 @@ def test_solver_and_finalize_from_post_collide_state(gpu, name, i):
     mine = b.snapshot(0)
     errs = P.body_errors(mine, ref, P.BODY_FIELDS + ["c0x", "c0y", "q0c", "q0s"])
     assert max(errs.values()) <= TOL, errs
+    assert P.body_mismatch_count(mine, ref, P.BODY_FIELDS + ["c0x", "c0y", "q0c", "q0s"]) == 0
     rep = P.manifold_report(mine, ref, value_fields=(), with_impulses=True)
     assert rep["impulses"] == 0, rep
-    # shapes of fast bodies: a sweep with several hits visits candidates in another
-    # order than the reference's trees (bounded by the time-of-impact tolerance)
-    fast = ((ref.bodies["flags"] & 1) != 0)[ref.shapes["body"]]
     for f in ("aabb", "fat"):
-        assert np.array_equal(mine.shapes[f][~fast], ref.shapes[f][~fast]), f
-        assert np.abs(mine.shapes[f][fast] - ref.shapes[f][fast]).max(initial=0.0) <= TOL, f
-    if not fast.any():
-        assert np.array_equal(mine.move, ref.move)
+        assert np.array_equal(mine.shapes[f], ref.shapes[f]), f
+    assert np.array_equal(mine.move, ref.move)
     for f in ("linear_impulse_x", "linear_impulse_y", "lower_impulse", "upper_impulse"):
         assert np.array_equal(mine.joints[f], ref.joints[f]), f
     if sleep_enabled(spec):

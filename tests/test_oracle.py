@@ -1,20 +1,22 @@
 """Pins of the CPU oracle (oracle/b2oracle.c) against the reference's own step.
 
-The oracle steps every fixture's snapshot k-1 to k; the contacts the reference's
-broad phase created in step k (their ids and shapes, read from snapshot k) are
-handed to it, because the dynamic tree's traversal order that assigns those ids
-is not restated. Everything else it computes: manifolds and feature ids,
-warm-start matching, the contact state loop and persistent colouring, the
+The oracle steps every fixture's snapshot k-1 to k and computes all of it: the
+pair update over the snapshot's broad-phase trees (the query order that assigns
+new contact ids from the pool), the partial tree rebuild, manifolds and feature
+ids, warm-start matching, the contact state loop and persistent colouring, the
 solver stages in the SSE2 lane order for coloured contacts and the scalar order
-for the overflow colour, revolute joints, finalize, continuous collision and the
-next move array. The float build must reproduce snapshot k bit for bit.
+for the overflow colour, revolute joints (limits, motors, springs), finalize,
+continuous collision and the next move array. The float build must reproduce
+snapshot k bit for bit. The "features" fixture covers the body, shape and joint
+options the other scenes leave at their defaults: kinematic bodies, damping,
+gravity scale, fixed and fast rotation, tangent speed, filter groups and masks,
+rounded polygons, revolute motor, spring and collideConnected.
 
-The one order the oracle does not reproduce: when several candidates of one
-continuous sweep hit, the reference visits them in its trees' traversal order
-and the oracle in shape order; the earlier hit's fraction bounds the later root
-search, so the final fraction can differ in the last bits (within the
-time-of-impact tolerance, 0.25 * linear slop). Fast bodies are therefore
-compared within 1e-5 relative, every other quantity exactly.
+Continuous collision visits the candidates of a fast shape in the trees' query
+order (static tree, then for bullets the kinematic and dynamic trees): when
+several candidates hit, the earlier hit's fraction bounds the later root search,
+so the order fixes the last bits of the fraction (the ccd fixture's step 3 differs
+in shape id order). Every quantity is compared exactly.
 
 Islands and sleep (S/island.c, S/solver_set.c, S/solver.c:1995-2049) are part of
 the restated step: union-find links when contacts begin touching, unlinks and
@@ -40,7 +42,6 @@ from tests.conftest import FIXTURE_NAMES, GOLDEN, load_fixture
 
 pytestmark = pytest.mark.skipif(not O.available("f32"), reason="oracle not built (make -C oracle port)")
 
-TOL = 1e-5
 
 
 def geoms(name):
@@ -66,24 +67,18 @@ def test_oracle_step_reproduces_reference(name, i):
     assert P.color_map(mine) == P.color_map(ref)
     assert all(np.array_equal(a, b) for a, b in zip(mine.colors, ref.colors)), "colour array order"
     assert np.array_equal(mine.free_ids, ref.free_ids)
-    fast = (ref.bodies["flags"] & 1) != 0
     for f in P.BODY_FIELDS + ["c0x", "c0y", "q0c", "q0s"]:
-        assert np.array_equal(mine.bodies[f][~fast], ref.bodies[f][~fast]), f
-        assert P.rel_err(mine.bodies[f][fast], ref.bodies[f][fast]) <= TOL, f
-    fast_shapes = fast[ref.shapes["body"]]
+        assert np.array_equal(mine.bodies[f], ref.bodies[f]), f
     for f in ("aabb", "fat"):
-        assert np.array_equal(mine.shapes[f][~fast_shapes], ref.shapes[f][~fast_shapes]), f
-        assert np.abs(mine.shapes[f][fast_shapes] - ref.shapes[f][fast_shapes]).max(initial=0.0) <= TOL, f
+        assert np.array_equal(mine.shapes[f], ref.shapes[f]), f
+    assert np.array_equal(mine.move, ref.move)
     rep = P.manifold_report(mine, ref, with_impulses=True)
-    assert rep["point_count"] == rep["ids"] == rep["normals"] == rep["values"] == 0, rep
+    assert rep["point_count"] == rep["ids"] == rep["normals"] == rep["values"] == rep["impulses"] == 0, rep
     assert P.island_differences(mine, ref) == []
-    assert P.tree_differences(mine, ref, 0.0 if not fast.any() else TOL) == []
-    if not fast.any():
-        assert rep["impulses"] == 0, rep
-        assert np.array_equal(mine.move, ref.move)
-        for f in ("linear_impulse_x", "linear_impulse_y", "spring_impulse", "motor_impulse", "lower_impulse",
-                  "upper_impulse"):
-            assert np.array_equal(mine.joints[f], ref.joints[f]), f
+    assert P.tree_differences(mine, ref, 0.0) == []
+    for f in ("linear_impulse_x", "linear_impulse_y", "spring_impulse", "motor_impulse", "lower_impulse",
+              "upper_impulse"):
+        assert np.array_equal(mine.joints[f], ref.joints[f]), f
 
 
 @pytest.mark.parametrize("name", FIXTURE_NAMES)
