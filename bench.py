@@ -259,6 +259,8 @@ def main():
                     help="closest-ray queries per world timed after the step measurement (0: none)")
     ap.add_argument("--solve-lds", default="",
                     help="comma list of solve-launch LDS bytes per size class (with --lanes)")
+    ap.add_argument("--tune-period", type=int, default=64, metavar="STEPS",
+                    help="online lane tuning: a three-step trial round every STEPS steps (0: off)")
     ap.add_argument("--full-steps", type=int, default=512, metavar="N",
                     help="also step a fresh batch of the workload N steps from its initial state (0: skip)")
     ap.add_argument("--trace-summary", default="", metavar="CSV",
@@ -337,6 +339,8 @@ def main():
         batch.synchronize()
         log(f"[bench] autotuned workgroup lanes per size class: "
             f"{[c['lanes'] for c in batch.size_classes()]} (whole-batch ms per 4 steps at one size: {tune_ms})")
+    if args.tune_period > 0 and not args.lanes:
+        batch.set_online_tuning(args.tune_period)
     start = args.warmup + tune_steps
 
     # timed region: K steps, synchronized on both sides
@@ -387,7 +391,7 @@ def main():
         log(f"[bench] per-world GPU path: {per_world}")
 
     # the workload's whole run: a fresh batch stepped from the initial state
-    full = full_run(spec, args.full_steps) if args.full_steps > 0 else None
+    full = full_run(spec, args.full_steps, args.tune_period if not args.lanes else 0) if args.full_steps > 0 else None
 
     # kernel durations with HIP events on the launch stream (size classes one after
     # another): the last step-kernel dispatches of this process, so a kernel trace of
@@ -437,6 +441,7 @@ def main():
                    "contacts": st["contacts"], "contacts_per_body": st["contacts"] / max(awake, 1),
                    "manifold_points": st["points"], "joints": st["joints"], "size_classes": batch.size_classes(),
                    "autotune_ms_per_4_steps": tune_ms, "autotune_steps": tune_steps,
+                   "online_tuning_period": args.tune_period if not args.lanes else 0,
                    "lanes_override": args.lanes or None,
                    "parallelism": "one workgroup per world, worlds launched in size classes (own LDS size and workgroup "
                                   "size, own stream), largest first; collide, tree, solve and continuous-collision "
@@ -502,13 +507,16 @@ def kernel_profile(args, batch, awake, desc):
     print(json.dumps(out), flush=True)
 
 
-def full_run(spec: bytes, steps: int) -> dict:
+def full_run(spec: bytes, steps: int, tune_period: int) -> dict:
     """The workload's whole run on a fresh batch from its initial state: `steps`
     steps in chunks of 64, each chunk synchronized and timed, with the touching
-    contacts per body after the first step and at the end."""
+    contacts per body after the first step and at the end (online lane tuning as in
+    the timed region, no up-front autotuning)."""
     b = F.Batch(spec)
     awake = b.awake_bodies()
     b.upload()
+    if tune_period > 0:
+        b.set_online_tuning(tune_period)
     b.step(DT, SUBSTEPS, steps=1)
     b.synchronize()
     c0 = b.stats()["contacts"]

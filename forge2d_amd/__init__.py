@@ -55,6 +55,7 @@ SIGNATURES = [
     ("f2dg_world_counters", _I, [_VP, _I, _PI]),
     ("f2dg_batch_stats", _I, [_VP, ctypes.POINTER(ctypes.c_long)]),
     ("f2dg_autotune", _I, [_VP, _F, _I, _I, _PF]),
+    ("f2dg_set_online_tuning", _I, [_VP, _I]),
     ("f2dg_set_block", _I, [_VP, _I]),
     ("f2dg_get_block", _I, [_VP]),
     ("f2dg_set_class_block", _I, [_VP, _I, _I]),
@@ -290,6 +291,10 @@ class Batch:
         if blk < 0:
             raise RuntimeError("autotune failed")
         return blk, {64: ms[0], 128: ms[1], 256: ms[2]}
+
+    def set_online_tuning(self, period: int):
+        """Re-pick every size class's lanes per world every `period` steps (0: off)."""
+        self._check(lib().f2dg_set_online_tuning(self.h, period), "set_online_tuning")
 
     def set_block(self, block: int):
         self._check(lib().f2dg_set_block(self.h, block), "set_block")

@@ -587,6 +587,10 @@ static void freeDevice(Batch& b) {
     if (c.dList) (void)hipFree(c.dList);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     if (c.done) (void)hipEventDestroy(c.done);
+    for (hipEvent_t& e : c.tuneEv) {
+      if (e) (void)hipEventDestroy(e);
+      e = nullptr;
+    }
   }
   b.classes.clear();
   if (b.forkEvent) (void)hipEventDestroy(b.forkEvent);
@@ -1835,22 +1839,84 @@ long f2dg_device_bytes(void* h) {
 }
 
 // Step every world of the batch `steps` times: one launch per step, one workgroup per world.
+// Lane candidates of a size class for online tuning: 64, 128 and 256 lanes per
+// world, 256, 512 and 1024 for worlds of more than 512 shapes or bodies.
+static int laneCandidate(const SizeClass& c, int k) {
+  static const int small[3] = {64, 128, 256}, large[3] = {256, 512, 1024};
+  return c.limit > 512 ? large[k] : small[k];
+}
+
+// One online tuning round: three steps, step k with every class at its k-th lane
+// candidate, the classes one after another on the batch's stream with HIP events
+// around each; then every class keeps its fastest candidate. The steps are
+// ordinary steps (the lanes never change results).
+static int tuneRound(Batch& b, const StepParams& sp) {
+  for (SizeClass& c : b.classes) {
+    for (hipEvent_t& e : c.tuneEv) {
+      if (!e) F2D_HIP(hipEventCreate(&e));
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    for (SizeClass& c : b.classes) {
+      LaunchConfig cfg = c.launch;
+      cfg.block = laneCandidate(c, k);
+      F2D_HIP(hipEventRecord(c.tuneEv[2 * k], b.stream));
+      if (f2d_launch_step(&b.dev.view, &sp, c.dList, c.count, b.stream, &cfg) != 0) return -1;
+      F2D_HIP(hipEventRecord(c.tuneEv[2 * k + 1], b.stream));
+    }
+  }
+  F2D_HIP(hipEventSynchronize(b.classes.back().tuneEv[5]));
+  for (SizeClass& c : b.classes) {
+    float best = 1e30f;
+    for (int k = 0; k < 3; ++k) {
+      float ms = 0.0f;
+      F2D_HIP(hipEventElapsedTime(&ms, c.tuneEv[2 * k], c.tuneEv[2 * k + 1]));
+      if (ms < best) {
+        best = ms;
+        c.launch.block = laneCandidate(c, k);
+      }
+    }
+  }
+  return 0;
+}
+
 int f2dg_step(void* h, float dt, int subSteps, int steps, int phases) {
   Batch* b = static_cast<Batch*>(h);
   if (!b) return -1;
   if (prepareForStep(*b) != 0) return -1;
   if (dt == 0.0f) return 0;
   StepParams sp = makeParams(dt, subSteps, phases == 0 ? PH_ALL : phases);
-  int W = (int)b->worlds.size();
-  (void)W;
-  for (int s = 0; s < steps; ++s) {
+  for (int s = 0; s < steps;) {
+    if (b->tunePeriod > 0 && sp.phases == PH_ALL && b->tuneClock >= b->tuneNext && steps - s >= 3) {
+      if (tuneRound(*b, sp) != 0) {
+        std::fprintf(stderr, "forge2d_amd: step launch failed in a tuning round\n");
+        return -1;
+      }
+      b->tuneNext = b->tuneClock + b->tunePeriod;
+      s += 3;
+      b->tuneClock += 3;
+      continue;
+    }
     int rc = launchClasses(*b, sp);
     if (rc != 0) {
       std::fprintf(stderr, "forge2d_amd: step launch failed (%d)\n", rc);
       return -1;
     }
+    ++s;
+    if (b->tunePeriod > 0) ++b->tuneClock;
   }
   b->deviceNewer = true;
+  return 0;
+}
+
+// Online lane tuning for f2dg_step: every `period` steps a three-step trial round
+// picks each size class's lanes per world again (0 switches it off).
+int f2dg_set_online_tuning(void* h, int period) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b || period < 0) return -1;
+  b->tunePeriod = period;
+  b->tuneClock = 0;
+  b->tuneNext = 0;
   return 0;
 }
 

@@ -83,6 +83,7 @@ struct SizeClass {
   LaunchConfig launch = {0, 0, 0, 0, 0};
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  hipEvent_t tuneEv[6] = {};  // online lane tuning: start and end of each candidate's trial step
 };
 
 struct Batch {
@@ -111,6 +112,9 @@ struct Batch {
   int* dCounts = nullptr;
   float4* dBoxes = nullptr;  // packed fat AABBs of every shape (nShapes)
   int forcedBlock = 0;       // f2dg_set_block before the batch is laid out: every class's size
+  int tunePeriod = 0;        // online lane tuning: a trial round every tunePeriod steps (0: off)
+  long tuneClock = 0;        // steps taken through f2dg_step since tuning was switched on
+  long tuneNext = 0;         // the clock at which the next trial round is due
 };
 
 }  // namespace f2d

@@ -231,6 +231,12 @@ const char* f2dg_last_error(void* batch);
  * classes with 3 lane candidates run 2 x steps untimed steps more, so every world advances by
  * 9 x steps trial steps; returns the largest class's lanes */
 int f2dg_autotune(void* batch, float dt, int sub_steps, int steps, float* ms_out);
+/* online lane tuning of f2dg_step: every `period` steps (0: off), a round of three ordinary
+ * steps runs each size class at 64, 128 and 256 lanes per world (256, 512 and 1024 for
+ * classes of more than 512 shapes), the classes one after another, timed with HIP events,
+ * and each class keeps its fastest; as worlds settle and gather contacts the best size
+ * changes, and the lanes never change results */
+int f2dg_set_online_tuning(void* batch, int period);
 /* worlds are launched in size classes by max(bodies, shapes) <= 64, 128, ..., 1024, larger;
  * per class: [limit, worlds, max bodies, max shapes, workgroup lanes, collide LDS bytes,
  * solve LDS bytes] into out (7 ints each, up to cap classes); returns the class count */

@@ -72,6 +72,25 @@ def test_autotune_advances_every_world_equally(gpu):
     assert steps == {18}, steps
 
 
+def test_online_tuning_does_not_change_results(gpu):
+    """Online lane tuning (trial rounds every 5 steps, classes stepped one after
+    another at each candidate size) gives the same bits as plain stepping."""
+    worlds = scenes.config4(worlds=8, lo=20, hi=400) + [scenes.container_world(1500, 30.0, 50.0, 7007, chains=3)]
+    ref = F.Batch(worlds)
+    tuned = F.Batch(worlds)
+    tuned.set_online_tuning(5)
+    ref.step(steps=1)
+    tuned.step(steps=1)
+    for k in (7, 3, 9):
+        ref.step(steps=k)
+        tuned.step(steps=k)
+    assert ref.sync() == 0 and tuned.sync() == 0
+    for c in tuned.size_classes():
+        assert c["lanes"] in ((256, 512, 1024) if c["limit"] > 512 else (64, 128, 256)), c
+    for w in range(len(worlds)):
+        assert ref.export(w) == tuned.export(w), w
+
+
 def test_step_subset_leaves_other_worlds(gpu):
     worlds = scenes.config3(worlds=6, rows_lo=5, rows_hi=8)
     b = F.Batch(worlds)
