@@ -203,13 +203,12 @@ def measure_queries(args, batch, spec: bytes, W: int):
     batch.synchronize()
     ms = 1e3 * (time.perf_counter() - a) / reps
     counts = dc.cpu().numpy()
-    shapes = np.array([batch.shape_count(i) for i in range(W)], np.int64)
-    scan = 32.0 * args.query_rays * float(shapes.sum())
     out = {"kind": "closest ray (b2World_CastRayClosest)", "rays": n, "rays_per_world": args.query_rays,
            "ms_per_launch": ms, "rays_per_s": n / (ms * 1e-3), "hit_rate": float((counts > 0).mean()),
-           "kernel": "queryKernel", "scan_bytes_per_launch": scan, "scan_gbs": scan / (ms * 1e-3) / 1e9,
-           "sample": f"{args.query_rays} seeded rays per world over the world's box, after the bench's steps; "
-                     f"32 B of shape record (fat AABB, filter) scanned per shape per ray, largely L2-served"}
+           "kernel": "queryKernel",
+           "sample": f"{args.query_rays} seeded rays per world over the world's box, after the bench's steps; one "
+                     f"lane per ray walking the world's three trees (pending enlargements applied and rays grouped "
+                     f"by world in the same launch sequence)"}
     harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     if not args.no_cpu and os.path.exists(harness):
         m = min(W, 1024)

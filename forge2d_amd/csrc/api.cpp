@@ -27,8 +27,9 @@
 extern "C" int f2d_launch_step(const f2d::BatchView* b, const f2d::StepParams* sp, const int* worldList, int nWorlds,
                                hipStream_t stream, const f2d::LaunchConfig* cfg);
 extern "C" int f2d_launch_island_command(const f2d::BatchView* b, int world, int op, int arg, hipStream_t stream);
-extern "C" int f2d_launch_queries(const f2d::BatchView* b, float4* boxes, long nShapes, const f2d::Query* q, int n,
+extern "C" int f2d_launch_queries(const f2d::BatchView* b, int* group, const f2d::Query* q, int n,
                                   f2d::QueryHit* hits, int* counts, int cap, hipStream_t stream);
+extern "C" int f2d_launch_enlarge_pending(const f2d::BatchView* b, hipStream_t stream);
 extern "C" int f2d_launch_explosions(const f2d::BatchView* b, const f2d::Query* q, int n, hipStream_t stream);
 
 namespace f2d {
@@ -598,8 +599,9 @@ static void freeDevice(Batch& b) {
   if (b.dQueries) (void)hipFree(b.dQueries);
   if (b.dHits) (void)hipFree(b.dHits);
   if (b.dCounts) (void)hipFree(b.dCounts);
-  if (b.dBoxes) (void)hipFree(b.dBoxes);
-  b.dBoxes = nullptr;
+  if (b.dGroup) (void)hipFree(b.dGroup);
+  b.dGroup = nullptr;
+  b.dGroupCap = 0;
   b.dQueries = nullptr;
   b.dHits = nullptr;
   b.dCounts = nullptr;
@@ -3006,7 +3008,23 @@ typedef struct f2dQueryHit {
 
 namespace f2d {
 
+// The query grouping scratch for n queries: the order, then a count and first slot
+// per world (and one for queries naming no world of the batch).
+static int ensureGroupBuffer(Batch& b, long n) {
+  const long groupInts = n + (long)b.worlds.size() + 1;
+  if (n > 0 && groupInts > b.dGroupCap) {
+    if (b.dGroup) (void)hipFree(b.dGroup);
+    b.dGroup = nullptr;
+    b.dGroupCap = 0;
+    const long c = std::max<long>(groupInts, 1024);
+    F2D_HIP(hipMalloc(&b.dGroup, sizeof(int) * c));
+    b.dGroupCap = c;
+  }
+  return 0;
+}
+
 static int ensureQueryBuffers(Batch& b, long n, long hitCount) {
+  if (ensureGroupBuffer(b, n) != 0) return -1;
   if (n > b.dQueryCap) {
     if (b.dQueries) (void)hipFree(b.dQueries);
     if (b.dCounts) (void)hipFree(b.dCounts);
@@ -3018,7 +3036,6 @@ static int ensureQueryBuffers(Batch& b, long n, long hitCount) {
     F2D_HIP(hipMalloc(&b.dCounts, sizeof(int) * c));
     b.dQueryCap = c;
   }
-  if (!b.dBoxes && b.nShapes > 0) F2D_HIP(hipMalloc(&b.dBoxes, sizeof(float4) * b.nShapes));
   if (hitCount > b.dHitCap) {
     if (b.dHits) (void)hipFree(b.dHits);
     b.dHits = nullptr;
@@ -3051,8 +3068,7 @@ static int runQueries(Batch& b, const Query* q, int n, QueryHit* hits, int* coun
   for (const auto& r : rounds) explosions += (long)r.size();
   if (ensureQueryBuffers(b, n + explosions, (long)n * cap) != 0) return -1;
   F2D_HIP(hipMemcpyAsync(b.dQueries, q, sizeof(Query) * n, hipMemcpyHostToDevice, b.stream));
-  if (f2d_launch_queries(&b.dev.view, b.dBoxes, b.nShapes, b.dQueries, n, b.dHits, b.dCounts, cap, b.stream) != 0)
-    return -1;
+  if (f2d_launch_queries(&b.dev.view, b.dGroup, b.dQueries, n, b.dHits, b.dCounts, cap, b.stream) != 0) return -1;
   long off = n;
   for (const auto& r : rounds) {
     F2D_HIP(hipMemcpyAsync(b.dQueries + off, r.data(), sizeof(Query) * r.size(), hipMemcpyHostToDevice, b.stream));
@@ -3065,7 +3081,7 @@ static int runQueries(Batch& b, const Query* q, int n, QueryHit* hits, int* coun
   long used = 0;
   for (int i = 0; i < n; ++i) used = std::max<long>(used, (long)i * cap + std::min(std::max(counts[i], 0), cap));
   if (used > 0) F2D_HIP(hipMemcpy(hits, b.dHits, sizeof(QueryHit) * used, hipMemcpyDeviceToHost));
-  if (explosions > 0) b.deviceNewer = true;
+  b.deviceNewer = true;  // pending tree enlargements applied, explosions
   return 0;
 }
 
@@ -3102,8 +3118,9 @@ int f2dg_query_device(void* h, const void* queries, int n, void* hits, void* cou
   if (b->hostDirty || !b->committed) {
     if (upload(*b) != 0) return -1;
   }
-  if (ensureQueryBuffers(*b, 0, 0) != 0) return -1;
-  return f2d_launch_queries(&b->dev.view, b->dBoxes, b->nShapes, static_cast<const Query*>(queries), n,
+  if (ensureGroupBuffer(*b, n) != 0) return -1;
+  b->deviceNewer = true;  // pending tree enlargements applied
+  return f2d_launch_queries(&b->dev.view, b->dGroup, static_cast<const Query*>(queries), n,
                             static_cast<QueryHit*>(hits), static_cast<int*>(counts), cap, b->stream);
 }
 
@@ -3148,8 +3165,8 @@ int f2d_world_cast_ray_closest(uint32_t w, float ox, float oy, float tx, float t
   return 1;
 }
 
-// f2d_shim.c:1343-1350 -> b2World_CastRay (S/world.c:2222-2254): per tree, the hits
-// reach the host callback nearest first; a return of 0 ends the cast, a value in
+// f2d_shim.c:1343-1350 -> b2World_CastRay (S/world.c:2222-2254): the hits reach the
+// host callback in the trees' walk order; a return of 0 ends the cast, a value in
 // [0, 1] becomes the world's fraction (the next tree's ray length), a value in
 // (0, current] clips the current tree's ray so hits beyond it are skipped
 void f2d_world_cast_ray(uint32_t w, float ox, float oy, float tx, float ty, uint32_t category_lo,
@@ -3160,9 +3177,9 @@ void f2d_world_cast_ray(uint32_t w, float ox, float oy, float tx, float ty, uint
   std::vector<QueryHit> hits;
   if (flatQuery(*b, q, hits) != 0) return;
   const HostWorld& hw = b->worlds[0];
-  std::stable_sort(hits.begin(), hits.end(), [](const QueryHit& x, const QueryHit& y) {
-    return x.fraction < y.fraction || (x.fraction == y.fraction && x.shape < y.shape);
-  });
+  // the hits come in the trees' walk order with the whole ray (maxFraction 1): the
+  // reference's walk with the callback's fractions visits the same hits in the same
+  // order, minus those beyond the fraction the callback last clipped the ray to
   float worldFraction = 1.0f;
   for (int tree = kStatic; tree <= kDynamic; ++tree) {
     float maxFraction = worldFraction;
@@ -3177,8 +3194,8 @@ void f2d_world_cast_ray(uint32_t w, float ox, float oy, float tx, float ty, uint
   }
 }
 
-// f2d_shim.c:1357-1367 -> b2World_OverlapAABB (S/world.c:2071-2105): per tree in shape-id
-// order; a false return ends that tree's walk
+// f2d_shim.c:1357-1367 -> b2World_OverlapAABB (S/world.c:2071-2105): per tree in its
+// depth-first order; a false return ends that tree's walk
 void f2d_world_overlap_aabb(uint32_t w, float lx, float ly, float ux, float uy, uint32_t category_lo,
                             uint32_t category_hi, uint32_t mask_lo, uint32_t mask_hi) {
   Batch* b = batchOf(w);

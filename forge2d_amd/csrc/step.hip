@@ -1914,189 +1914,284 @@ __global__ void islandCommandKernel(BatchView b, int world, int op, int arg) {
 
 // ---------------------------------------------------------------------------- queries
 
-constexpr int kQueryLanes = 64;  // one wavefront per query
+constexpr int kQueryLanes = 64;  // one query per lane
+constexpr int kQueryStack = 64;  // per-lane traversal stack (entries): a tree of height h needs h + 1
 
-// The box of the ray segment at maxFraction 1 (segmentAABB of b2DynamicTree_RayCast,
-// S/dynamic_tree.c:1172-1200). A shape's exact hit lies on the shape, inside its fat
-// AABB by at least the speculative margin, and on the segment, so the shapes the
-// tree's leaf test culls are exactly those whose ray casts cannot hit.
-__device__ __forceinline__ AABB rayBox(Vec2 p1, Vec2 d) {
-  Vec2 p2 = mulAdd(p1, 1.0f, d);
-  return {vmin(p1, p2), vmax(p1, p2)};
+__device__ __forceinline__ Vec2 boxCenter(const AABB& a) {  // b2AABB_Center
+  return {0.5f * (a.lo.x + a.hi.x), 0.5f * (a.lo.y + a.hi.y)};
+}
+__device__ __forceinline__ Vec2 boxExtents(const AABB& a) {  // b2AABB_Extents
+  return {0.5f * (a.hi.x - a.lo.x), 0.5f * (a.hi.y - a.lo.y)};
 }
 
-// The shapes' fat AABBs packed 16 B per shape in batch shape order (one pass before
-// the queries): the scan reads one float4 per shape instead of a line of its
-// 96-byte record, and the record only for shapes whose box passes.
-__global__ void packBoxesKernel(const Shape* shapes, float4* boxes, long n) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const AABB f = shapes[i].fat;
-  boxes[i] = make_float4(f.lo.x, f.lo.y, f.hi.x, f.hi.y);
+// The leaves the last step enlarged (SF_TREE_PENDING), applied to the world's
+// kinematic and dynamic trees as b2DynamicTree_EnlargeProxy does
+// (S/dynamic_tree.c:827-866): the leaf takes the fat box, its ancestors grow to hold
+// it and are marked enlarged. The step's tree launch applies them at the start of
+// the next step; queries between steps walk the trees and need them now. The
+// boxes and marks do not depend on the order of the leaves, and the next tree
+// launch rebuilds the marked nodes as it would have. Thread 0 of one workgroup per
+// world walks up from each pending leaf.
+__global__ __launch_bounds__(64) void enlargePendingKernel(BatchView b) {
+  const int world = blockIdx.x;
+  WorldHeader* h = b.worlds + world;
+  Shape* shapes = b.shapes + h->shapeOffset;
+  const int ns = h->shapeCount;
+  bool any = false;
+  for (int k = threadIdx.x; k < ns; k += 64) any |= (shapes[k].flags & SF_TREE_PENDING) != 0;
+  if (!__syncthreads_or(any) || threadIdx.x != 0) return;
+  for (int k = 0; k < ns; ++k) {
+    Shape& sh = shapes[k];
+    if (!(sh.flags & SF_TREE_PENDING)) continue;
+    const int t = sh.proxyKey & 3;
+    if (sh.proxyKey < 0 || t == kStatic) continue;
+    TreeHeader* hd = b.treeHdrs + 3L * world + t;
+    TreeNode* n = b.treeNodes + hd->offset;
+    const int F = hd->fresh;
+    const int leaf = sh.proxyKey >> 2;
+    if (leaf < 0 || leaf >= F) continue;  // left to the tree launch, which flags the tree
+    sh.flags &= ~SF_TREE_PENDING;
+    const AABB box = sh.fat;
+    n[leaf].box = box;
+    int p = n[leaf].parent;
+    for (int g = 0; p >= 0 && p < F && g < F; ++g) {
+      const bool changed = aabbEnlarge(&n[p].box, box);
+      n[p].flags |= TN_ENLARGED;
+      p = n[p].parent;
+      if (!changed) break;
+    }
+    for (int g = 0; p >= 0 && p < F && g < F; ++g) {
+      if (n[p].flags & TN_ENLARGED) break;
+      n[p].flags |= TN_ENLARGED;
+      p = n[p].parent;
+    }
+  }
 }
 
-__device__ __forceinline__ bool boxOverlaps(float4 f, AABB b) {
-  AABB a = {{f.x, f.y}, {f.z, f.w}};
-  return aabbOverlaps(a, b);
-}
-
-// The leaf test of b2DynamicTree_RayCast (S/dynamic_tree.c:1172-1245): the segment's
-// box overlaps the fat AABB and the box is not separated from the ray's line,
-// |dot(v, p1 - c)| <= dot(|v|, h) with v perpendicular to the ray
-struct RaySat {
-  Vec2 p1, v, absV;
-  AABB segment;
+// Per-lane traversal stack in LDS, entries interleaved across the wavefront's lanes.
+struct LaneStack {
+  int* base;
+  int count;
+  bool overflow;
+  __device__ __forceinline__ void push(int v) {
+    if (count < kQueryStack) {
+      base[count * kQueryLanes] = v;
+      ++count;
+    } else {
+      overflow = true;
+    }
+  }
+  __device__ __forceinline__ int pop() { return base[--count * kQueryLanes]; }
 };
-__device__ __forceinline__ RaySat raySat(Vec2 p1, Vec2 d) {
-  RaySat r;
-  r.p1 = p1;
-  const Vec2 n = normalize(d);
-  r.v = crossSV(1.0f, n);
-  r.absV = {absf(r.v.x), absf(r.v.y)};
-  r.segment = rayBox(p1, d);
-  return r;
-}
-__device__ __forceinline__ bool rayMeetsBox(const RaySat& r, float4 f) {
-  if (!boxOverlaps(f, r.segment)) return false;
-  const Vec2 c = {0.5f * (f.x + f.z), 0.5f * (f.y + f.w)};
-  const Vec2 h = {0.5f * (f.z - f.x), 0.5f * (f.w - f.y)};
-  const float term1 = absf(dot(r.v, sub(r.p1, c)));
-  const float term2 = dot(r.absV, h);
-  return !(term2 < term1);
-}
 
-// Ray casts and AABB overlaps over the worlds of a batch, one wavefront per query,
-// the lanes over the world's shapes (the shapes' fat AABBs are the reference's tree
-// leaves, so the candidate sets are the trees' without a tree):
-//   QK_RAY_CLOSEST  b2World_CastRayClosest (S/world.c:2277-2308): the hit of
-//                   smallest fraction > 0 (initial overlaps are ignored, as
-//                   b2RayCastClosestFcn does); ties go to the lower shape id
-//   QK_RAY_ALL      b2World_CastRay with a callback that returns 1 (S/world.c:2185-2254):
-//                   every hit, initial overlaps included, in shape-id order
-//   QK_OVERLAP_AABB b2World_OverlapAABB (S/world.c:2050-2105): every shape whose fat
-//                   AABB overlaps the box, in shape-id order
+// One query of the batch query ABI per lane, walking the world's trees the way
+// the reference's world queries do, so hits come out in the reference's order:
+//   QK_RAY_CLOSEST  b2World_CastRayClosest (S/world.c:2277-2308): each tree in turn
+//                   (static, kinematic, dynamic) by b2DynamicTree_RayCast
+//                   (S/dynamic_tree.c:1172-1290: the nearer child first, the segment
+//                   shortened by every accepted hit) with b2RayCastClosestFcn (initial
+//                   overlaps ignored; the last hit at the smallest fraction is kept)
+//   QK_RAY_ALL      b2World_CastRay (S/world.c:2222-2254) with a callback that returns
+//                   1: every hit, initial overlaps included, in traversal order
+//   QK_OVERLAP_AABB b2World_OverlapAABB (S/world.c:2071-2105): each tree's
+//                   b2DynamicTree_Query (S/dynamic_tree.c:1114-1170, child2 popped
+//                   first), every shape whose fat AABB overlaps the box
 // counts[q] receives the number of results (which may exceed cap: only the first cap
-// are written to hits[q * cap ...]), or -1 for a world index outside the batch.
-__device__ void runQuery(const BatchView& b, const float4* packed, const Query& q, QueryHit* out, int* count,
-                         int cap, unsigned long long* best) {
-  const int lane = threadIdx.x;
+// are written to hits[q * cap ...]); -1 for a world index outside the batch; -2 if a
+// tree was deeper than the traversal stack (never for trees of realistic height).
+__device__ void runQuery(const BatchView& b, const Query& q, QueryHit* out, int* count, int cap, LaneStack& st) {
   if (q.world < 0 || q.world >= b.worldCount) {
-    if (lane == 0) *count = -1;
+    *count = -1;
     return;
   }
   const WorldHeader* h = b.worlds + q.world;
   const Shape* shapes = b.shapes + h->shapeOffset;
   const ShapeGeom* geoms = b.geoms + h->shapeOffset;
   const BodySim* sims = b.sims + h->bodyOffset;
-  const float4* boxes = packed + h->shapeOffset;
-  const int ns = h->shapeCount;
-  const RayInput in = {{q.p[0], q.p[1]}, {q.p[2], q.p[3]}, 1.0f};
-  const RaySat sat = raySat(in.origin, in.translation);
-  if (q.kind == QK_RAY_CLOSEST) {
-    unsigned long long key = ~0ull;
-    CastOutput mine = noHit();
-    for (int s = lane; s < ns; s += kQueryLanes) {
-      if (!rayMeetsBox(sat, boxes[s])) continue;
-      const Shape& sh = shapes[s];
-      if (!shouldQuery(sh, q.catLo, q.catHi, q.maskLo, q.maskHi)) continue;
-      const CastOutput o = rayCastShape(in, geoms[s], sims[sh.body].xf);
-      if (!o.hit || !(o.fraction > 0.0f)) continue;
-      // positive fractions order as their bit patterns
-      const unsigned long long k = ((unsigned long long)__float_as_uint(o.fraction) << 32) | (unsigned)s;
-      if (k < key) {
-        key = k;
-        mine = o;
-      }
-    }
-    atomicMin(best, key);
-    __syncthreads();
-    const unsigned long long winner = *best;
-    if (winner == ~0ull) {
-      if (lane == 0) *count = 0;
-    } else if (key == winner) {
-      QueryHit r;
-      r.shape = (int)(key & 0xFFFFFFFFull);
-      r.px = mine.point.x;
-      r.py = mine.point.y;
-      r.nx = mine.normal.x;
-      r.ny = mine.normal.y;
-      r.fraction = mine.fraction;
-      if (cap > 0) out[0] = r;
-      *count = 1;
-    }
-    return;
-  }
-  if (q.kind != QK_RAY_ALL && q.kind != QK_OVERLAP_AABB) {
-    if (lane == 0) *count = 0;
-    return;
-  }
-  const AABB box = {{q.p[0], q.p[1]}, {q.p[2], q.p[3]}};
+  const TreeHeader* th = b.treeHdrs + 3L * q.world;
   int total = 0;
-  for (int base = 0; base < ns; base += kQueryLanes) {
-    const int s = base + lane;
-    bool hit = false;
-    CastOutput o = noHit();
-    if (s < ns && (q.kind == QK_RAY_ALL ? rayMeetsBox(sat, boxes[s]) : boxOverlaps(boxes[s], box))) {
-      const Shape& sh = shapes[s];
-      if (shouldQuery(sh, q.catLo, q.catHi, q.maskLo, q.maskHi)) {
-        if (q.kind == QK_RAY_ALL) {
-          o = rayCastShape(in, geoms[s], sims[sh.body].xf);
-          hit = o.hit;
+  if (q.kind == QK_OVERLAP_AABB) {
+    const AABB box = {{q.p[0], q.p[1]}, {q.p[2], q.p[3]}};
+    for (int t = kStatic; t <= kDynamic; ++t) {
+      const TreeHeader& hd = th[t];
+      if (hd.nodeCount == 0 || hd.root < 0) continue;
+      const TreeNode* n = b.treeNodes + hd.offset;
+      st.count = 0;
+      st.push(hd.root);
+      while (st.count > 0) {
+        const TreeNode& node = n[st.pop()];
+        if (!aabbOverlaps(node.box, box)) continue;
+        if (node.flags & TN_LEAF) {
+          const int s = node.child1;
+          if (!shouldQuery(shapes[s], q.catLo, q.catHi, q.maskLo, q.maskHi)) continue;
+          if (total < cap) {
+            QueryHit r;
+            r.shape = s;
+            r.px = r.py = r.nx = r.ny = r.fraction = 0.0f;
+            out[total] = r;
+          }
+          ++total;
         } else {
-          hit = true;
+          st.push(node.child1);
+          st.push(node.child2);
         }
       }
     }
-    const unsigned long long m = __ballot(hit);
-    if (hit) {
-      const int pos = total + __popcll(m & ((1ull << lane) - 1ull));
-      if (pos < cap) {
-        QueryHit r;
-        r.shape = s;
-        r.px = o.point.x;
-        r.py = o.point.y;
-        r.nx = o.normal.x;
-        r.ny = o.normal.y;
-        r.fraction = o.fraction;
-        out[pos] = r;
+    *count = st.overflow ? -2 : total;
+    return;
+  }
+  if (q.kind != QK_RAY_CLOSEST && q.kind != QK_RAY_ALL) {
+    *count = 0;
+    return;
+  }
+  const bool closest = q.kind == QK_RAY_CLOSEST;
+  const Vec2 p1 = {q.p[0], q.p[1]}, d = {q.p[2], q.p[3]};
+  const Vec2 r = normalize(d);
+  const Vec2 v = crossSV(1.0f, r);
+  const Vec2 absV = {absf(v.x), absf(v.y)};
+  float inputMax = 1.0f;     // the world query's input.maxFraction, carried from tree to tree
+  float worldFraction = 1.0f;
+  QueryHit best;
+  bool haveBest = false;
+  for (int t = kStatic; t <= kDynamic; ++t) {
+    const TreeHeader& hd = th[t];
+    if (hd.nodeCount != 0 && hd.root >= 0) {
+      const TreeNode* n = b.treeNodes + hd.offset;
+      float maxFraction = inputMax;
+      Vec2 p2 = mulAdd(p1, maxFraction, d);
+      AABB seg = {vmin(p1, p2), vmax(p1, p2)};
+      st.count = 0;
+      st.push(hd.root);
+      bool terminated = false;
+      while (st.count > 0 && !terminated) {
+        const TreeNode& node = n[st.pop()];
+        if (!aabbOverlaps(node.box, seg)) continue;
+        const Vec2 c = boxCenter(node.box), e = boxExtents(node.box);
+        const float term1 = absf(dot(v, sub(p1, c)));
+        const float term2 = dot(absV, e);
+        if (term2 < term1) continue;
+        if (node.flags & TN_LEAF) {
+          // RayCastCallback, S/world.c:2191-2220
+          const int s = node.child1;
+          const Shape& sh = shapes[s];
+          float value = maxFraction;
+          if (shouldQuery(sh, q.catLo, q.catHi, q.maskLo, q.maskHi)) {
+            const RayInput in = {p1, d, maxFraction};
+            const CastOutput o = rayCastShape(in, geoms[s], sims[sh.body].xf);
+            if (o.hit) {
+              float fraction;
+              QueryHit rec;
+              rec.shape = s;
+              rec.px = o.point.x;
+              rec.py = o.point.y;
+              rec.nx = o.normal.x;
+              rec.ny = o.normal.y;
+              rec.fraction = o.fraction;
+              if (closest) {
+                // b2RayCastClosestFcn, S/world.c:2258-2275
+                if (o.fraction == 0.0f) {
+                  fraction = -1.0f;
+                } else {
+                  best = rec;
+                  haveBest = true;
+                  fraction = o.fraction;
+                }
+              } else {
+                if (total < cap) out[total] = rec;
+                ++total;
+                fraction = 1.0f;
+              }
+              if (0.0f <= fraction && fraction <= 1.0f) worldFraction = fraction;
+              value = fraction;
+            }
+          }
+          if (value == 0.0f) {
+            terminated = true;
+          } else if (0.0f < value && value <= maxFraction) {
+            maxFraction = value;
+            p2 = mulAdd(p1, maxFraction, d);
+            seg = {vmin(p1, p2), vmax(p1, p2)};
+          }
+        } else {
+          const Vec2 c1 = boxCenter(n[node.child1].box), c2 = boxCenter(n[node.child2].box);
+          if (distanceSq(c1, p1) < distanceSq(c2, p1)) {
+            st.push(node.child2);
+            st.push(node.child1);
+          } else {
+            st.push(node.child1);
+            st.push(node.child2);
+          }
+        }
       }
     }
-    total += __popcll(m);
+    if (worldFraction == 0.0f) break;
+    inputMax = worldFraction;
   }
-  if (lane == 0) *count = total;
+  if (closest) {
+    if (haveBest && cap > 0) out[0] = best;
+    total = haveBest ? 1 : 0;
+  }
+  *count = st.overflow ? -2 : total;
 }
 
-// kQueriesPerGroup consecutive queries per wavefront, one after another (a million
-// one-query workgroups would be bound by workgroup dispatch)
-constexpr int kQueriesPerGroup = 8;
-
-__global__ __launch_bounds__(kQueryLanes) void queryKernel(BatchView b, const float4* packed, const Query* qs, int n,
+__global__ __launch_bounds__(kQueryLanes) void queryKernel(BatchView b, const Query* qs, const int* order, int n,
                                                            QueryHit* hits, int* counts, int cap) {
-  __shared__ unsigned long long best;
-  const int first = blockIdx.x * kQueriesPerGroup;
-  const int last = min(first + kQueriesPerGroup, n);
-  for (int qi = first; qi < last; ++qi) {
-    if (threadIdx.x == 0) best = ~0ull;
+  __shared__ int stackMem[kQueryStack * kQueryLanes];
+  const int k = blockIdx.x * kQueryLanes + threadIdx.x;
+  if (k >= n) return;
+  const int qi = order ? order[k] : k;
+  LaneStack st = {stackMem + threadIdx.x, 0, false};
+  runQuery(b, qs[qi], hits + (long)qi * cap, counts + qi, cap, st);
+}
+
+// Queries grouped by world (a counting sort into `order`), so that the lanes of a
+// wavefront walk the same trees: the count of each world's queries, the worlds'
+// first slots (one workgroup scan), then each query's slot.
+__global__ void queryCountKernel(const Query* qs, int n, int* worldCount, int W) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int w = qs[i].world;
+  atomicAdd(worldCount + ((w >= 0 && w < W) ? w : W), 1);
+}
+
+__global__ __launch_bounds__(1024) void queryScanKernel(int* worldCount, int W) {
+  __shared__ int sums[1024 / 64];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base <= W; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int v = i <= W ? worldCount[i] : 0;
+    int total;
+    const int incl = blockInclusiveAdd<1024>(v, sums, &total);
+    if (i <= W) worldCount[i] = carry + incl - v;
     __syncthreads();
-    runQuery(b, packed, qs[qi], hits + (long)qi * cap, counts + qi, cap, &best);
+    if (threadIdx.x == 0) carry += total;
     __syncthreads();
   }
 }
 
-// b2World_Explode / ExplosionCallback, S/world.c:2640-2747: one explosion per
-// workgroup, at most one per world per launch. Lanes test 64 shapes at a time
-// (dynamic proxies whose category meets the mask and whose fat AABB meets the blast
-// box), compute the distance to the blast centre and the impulse; then thread 0
-// wakes and pushes their bodies in shape-id order, so a body with several shapes
-// sums its impulses in a fixed order.
+__global__ void queryScatterKernel(const Query* qs, int n, int* worldStart, int W, int* order) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int w = qs[i].world;
+  order[atomicAdd(worldStart + ((w >= 0 && w < W) ? w : W), 1)] = i;
+}
+
+// b2World_Explode, S/world.c:2722-2747: the dynamic tree's b2DynamicTree_Query over
+// the explosion's box in depth-first order, ExplosionCallback (S/world.c:2640-2720)
+// on each leaf in turn: the distance from the shape to the explosion, the body woken
+// (its sleeping set, S/body.c:332-344), and the impulse added to an awake body's
+// velocities. Thread 0 walks; one workgroup per explosion.
 __global__ __launch_bounds__(kQueryLanes) void explodeKernel(BatchView b, const Query* qs) {
-  __shared__ int listBody[kQueryLanes];
-  __shared__ Vec2 listImpulse[kQueryLanes];
-  __shared__ Vec2 listPoint[kQueryLanes];
+  __shared__ int stackMem[kQueryStack * kQueryLanes];
   const Query q = qs[blockIdx.x];
-  if (q.kind != QK_EXPLODE || q.world < 0 || q.world >= b.worldCount) return;
+  if (threadIdx.x != 0 || q.kind != QK_EXPLODE || q.world < 0 || q.world >= b.worldCount) return;
   WorldHeader* h = b.worlds + q.world;
   WorldPtrs w = worldPtrs(b, h);
+  const TreeHeader& hd = b.treeHdrs[3L * q.world + kDynamic];
+  if (hd.nodeCount == 0 || hd.root < 0) return;
+  const TreeNode* n = b.treeNodes + hd.offset;
   const Vec2 position = {q.p[0], q.p[1]};
   const float radius = q.p[2], falloff = q.p[3], impulsePerLength = q.p[4];
   AABB box;
@@ -2105,68 +2200,54 @@ __global__ __launch_bounds__(kQueryLanes) void explodeKernel(BatchView b, const 
   box.hi.x = position.x + (radius + falloff);
   box.hi.y = position.y + (radius + falloff);
   const Xf identity = {{0.0f, 0.0f}, {1.0f, 0.0f}};
-  const int ns = h->shapeCount;
-  for (int base = 0; base < ns; base += kQueryLanes) {
-    const int s = base + threadIdx.x;
-    bool hit = false;
-    int body = -1;
-    Vec2 impulse = {0.0f, 0.0f}, closest = {0.0f, 0.0f};
-    if (s < ns) {
-      const Shape& sh = w.shapes[s];
-      if ((sh.proxyKey & 3) == kDynamic && ((sh.catLo & q.maskLo) | (sh.catHi & q.maskHi)) != 0u &&
-          aabbOverlaps(sh.fat, box)) {
-        body = sh.body;
-        const Xf xf = w.sims[body].xf;
-        const ShapeGeom& g = w.geoms[s];
-        Proxy pa = makeShapeProxy(g);
-        Proxy pb;
-        pb.count = 1;
-        pb.p[0] = position;
-        pb.radius = 0.0f;
-        SimplexCache cache;
-        cache.count = 0;
-        const DistanceOut d = shapeDistanceOut(cache, pa, pb, xf, identity, true);
-        if (!(d.distance > radius + falloff)) {
-          hit = true;
-          closest = d.pointA;
-          if (d.distance == 0.0f) closest = transformPoint(xf, shapeCentroid(g));
-          Vec2 direction = sub(closest, position);
-          if (lengthSq(direction) > 100.0f * kFltEpsilon * kFltEpsilon) {
-            direction = normalize(direction);
-          } else {
-            direction = {1.0f, 0.0f};
-          }
-          const Vec2 localLine = invRotate(xf.q, leftPerp(direction));
-          const float perimeter = projectedPerimeter(g, localLine);
-          float scale = 1.0f;
-          if (d.distance > radius && falloff > 0.0f) scale = clampf((radius + falloff - d.distance) / falloff, 0.0f, 1.0f);
-          const float magnitude = impulsePerLength * perimeter * scale;
-          impulse = mulSV(magnitude, direction);
-        }
-      }
+  LaneStack st = {stackMem, 0, false};
+  st.push(hd.root);
+  while (st.count > 0) {
+    const TreeNode& node = n[st.pop()];
+    if (!aabbOverlaps(node.box, box)) continue;
+    if (!(node.flags & TN_LEAF)) {
+      st.push(node.child1);
+      st.push(node.child2);
+      continue;
     }
-    const unsigned long long m = __ballot(hit);
-    if (hit) {
-      const int k = __popcll(m & ((1ull << threadIdx.x) - 1ull));
-      listBody[k] = body;
-      listImpulse[k] = impulse;
-      listPoint[k] = closest;
+    const int s = node.child1;
+    const Shape& sh = w.shapes[s];
+    // the tree's category test against the explosion's mask
+    if (((sh.catLo & q.maskLo) | (sh.catHi & q.maskHi)) == 0u) continue;
+    const int body = sh.body;
+    const Xf xf = w.sims[body].xf;
+    const ShapeGeom& g = w.geoms[s];
+    Proxy pa = makeShapeProxy(g);
+    Proxy pb;
+    pb.count = 1;
+    pb.p[0] = position;
+    pb.radius = 0.0f;
+    SimplexCache cache;
+    cache.count = 0;
+    const DistanceOut dist = shapeDistanceOut(cache, pa, pb, xf, identity, true);
+    if (dist.distance > radius + falloff) continue;
+    if (w.sims[body].setIndex >= kFirstSleepingSet && w.sims[body].islandId >= 0) wakeIslandSet(w, w.sims[body].islandId);
+    if (w.sims[body].setIndex != kAwakeSet) continue;
+    Vec2 closestPoint = dist.pointA;
+    if (dist.distance == 0.0f) closestPoint = transformPoint(xf, shapeCentroid(g));
+    Vec2 direction = sub(closestPoint, position);
+    if (lengthSq(direction) > 100.0f * kFltEpsilon * kFltEpsilon) {
+      direction = normalize(direction);
+    } else {
+      direction = {1.0f, 0.0f};
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const int count = __popcll(m);
-      for (int k = 0; k < count; ++k) {
-        const int bi = listBody[k];
-        if (w.sims[bi].setIndex >= kFirstSleepingSet && w.sims[bi].islandId >= 0) wakeIslandSet(w, w.sims[bi].islandId);
-        if (w.sims[bi].setIndex != kAwakeSet) continue;
-        const BodySim& sim = w.sims[bi];
-        BodyState& st = w.states[bi];
-        st.v = mulAdd(st.v, sim.invMass, listImpulse[k]);
-        st.w += sim.invI * cross(sub(listPoint[k], sim.center), listImpulse[k]);
-      }
-    }
-    __syncthreads();
+    const Vec2 localLine = invRotate(xf.q, leftPerp(direction));
+    const float perimeter = projectedPerimeter(g, localLine);
+    float scale = 1.0f;
+    if (dist.distance > radius && falloff > 0.0f) scale = clampf((radius + falloff - dist.distance) / falloff, 0.0f, 1.0f);
+    const float magnitude = impulsePerLength * perimeter * scale;
+    const Vec2 impulse = mulSV(magnitude, direction);
+    const BodySim& sim = w.sims[body];
+    BodyState& bs = w.states[body];
+    bs.v = mulAdd(bs.v, sim.invMass, impulse);
+    bs.w += sim.invI * cross(sub(closestPoint, sim.center), impulse);
   }
+  if (st.overflow) h->error |= ERR_TREE;
 }
 
 // ---------------------------------------------------------------------------- broad-phase trees
@@ -2246,16 +2327,41 @@ static int launchBlock(const BatchView* b, const StepParams* sp, const int* worl
 
 }  // namespace f2d
 
-// boxes: scratch of 16 B per shape of the batch (nShapes), packed here before the queries
-extern "C" int f2d_launch_queries(const f2d::BatchView* b, float4* boxes, long nShapes, const f2d::Query* q, int n,
+// Zero n ints (a fill kernel, not hipMemsetAsync, so that callers may capture the
+// launches in a graph).
+__global__ void zeroIntsKernel(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// The pending leaf enlargements of every world applied to its trees (before queries
+// walk them).
+extern "C" int f2d_launch_enlarge_pending(const f2d::BatchView* b, hipStream_t stream) {
+  if (b->worldCount <= 0) return 0;
+  hipLaunchKernelGGL(f2d::enlargePendingKernel, dim3((unsigned)b->worldCount), dim3(64), 0, stream, *b);
+  return (int)hipGetLastError();
+}
+
+// Queries, one per lane. group (optional): n + worldCount + 1 ints of scratch for
+// grouping the queries by world first; null runs them in the given order. The
+// results do not depend on the grouping.
+extern "C" int f2d_launch_queries(const f2d::BatchView* b, int* group, const f2d::Query* q, int n,
                                   f2d::QueryHit* hits, int* counts, int cap, hipStream_t stream) {
   if (n <= 0) return 0;
-  if (nShapes > 0) {
-    hipLaunchKernelGGL(f2d::packBoxesKernel, dim3((unsigned)((nShapes + 255) / 256)), dim3(256), 0, stream, b->shapes,
-                       boxes, nShapes);
+  if (f2d_launch_enlarge_pending(b, stream) != 0) return -1;
+  const int W = b->worldCount;
+  int* order = nullptr;
+  if (group) {
+    order = group;
+    int* worldCount = group + n;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(zeroIntsKernel, dim3((unsigned)((W + 1 + 255) / 256)), dim3(256), 0, stream, worldCount, W + 1);
+    hipLaunchKernelGGL(f2d::queryCountKernel, dim3(nb), dim3(256), 0, stream, q, n, worldCount, W);
+    hipLaunchKernelGGL(f2d::queryScanKernel, dim3(1), dim3(1024), 0, stream, worldCount, W);
+    hipLaunchKernelGGL(f2d::queryScatterKernel, dim3(nb), dim3(256), 0, stream, q, n, worldCount, W, order);
   }
-  const unsigned groups = (unsigned)((n + f2d::kQueriesPerGroup - 1) / f2d::kQueriesPerGroup);
-  hipLaunchKernelGGL(f2d::queryKernel, dim3(groups), dim3(f2d::kQueryLanes), 0, stream, *b, boxes, q, n, hits, counts,
+  const unsigned groups = (unsigned)((n + f2d::kQueryLanes - 1) / f2d::kQueryLanes);
+  hipLaunchKernelGGL(f2d::queryKernel, dim3(groups), dim3(f2d::kQueryLanes), 0, stream, *b, q, order, n, hits, counts,
                      cap);
   return (int)hipGetLastError();
 }
@@ -2296,19 +2402,20 @@ extern "C" int f2d_launch_step(const f2d::BatchView* b, const f2d::StepParams* s
 
 namespace f2d {
 
-// Packed fat boxes of the allocation-free query entry point: file-scope device
-// memory, one copy per GPU, held until the process ends. A batch with more shapes
-// than this is refused.
-constexpr long kStaticBoxCapacity = 1L << 18;  // 262,144 shapes (4 MiB)
-static __device__ __align__(256) float4 scratch[kStaticBoxCapacity];
+// The query grouping scratch of the allocation-free query entry point (the query
+// order, then a count and first slot per world): file-scope device memory, one copy
+// per GPU, held until the process ends. Calls with more queries plus worlds than
+// it holds are refused.
+constexpr long kStaticGroupCapacity = 1L << 20;  // ints (4 MiB)
+static __device__ __align__(256) int scratch[kStaticGroupCapacity];
 
 constexpr int kMaxDevices = 64;
 static std::mutex g_scratchLock;
-static float4* g_scratchAt[kMaxDevices];  // the array's device address, per device
+static int* g_scratchAt[kMaxDevices];  // the array's device address, per device
 
 // The array's address on the current device, looked up once per device. In host
 // code `scratch` names a host-side stand-in, never the device memory.
-static float4* staticBoxes(std::string& err) {
+static int* staticScratch(std::string& err) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
     err = "no current HIP device";
@@ -2322,16 +2429,18 @@ static float4* staticBoxes(std::string& err) {
       err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
       return nullptr;
     }
-    g_scratchAt[dev] = static_cast<float4*>(p);
+    g_scratchAt[dev] = static_cast<int*>(p);
   }
   return g_scratchAt[dev];
 }
 
 }  // namespace f2d
 
-// f2dg_query_device with the packed boxes in the static array instead of a buffer
+// f2dg_query_device with the query grouping in the static array instead of a buffer
 // the library allocates: no allocation of any kind on this path. The batch must be
-// uploaded already (f2dg_upload or a step) with no host edits pending.
+// uploaded already (f2dg_upload or a step) with no host edits pending, and hits and
+// counts must be device memory. Calls on one GPU share the array: calls that
+// overlap on one GPU race.
 extern "C" int f2dg_query_device_static(void* h, const void* queries, int n, void* hits, void* counts, int cap) {
   f2d::Batch* b = static_cast<f2d::Batch*>(h);
   if (!b || n < 0 || cap < 1) return -1;
@@ -2339,14 +2448,18 @@ extern "C" int f2dg_query_device_static(void* h, const void* queries, int n, voi
     b->lastError = "f2dg_query_device_static: upload the batch first (it allocates nothing itself)";
     return -1;
   }
-  if (b->nShapes > f2d::kStaticBoxCapacity) {
-    b->lastError = "f2dg_query_device_static: the batch has " + std::to_string(b->nShapes) +
-                   " shapes, more than the static box array holds (" + std::to_string(f2d::kStaticBoxCapacity) + ")";
+  const long need = (long)n + (long)b->worlds.size() + 1;
+  if (need > f2d::kStaticGroupCapacity) {
+    b->lastError = "f2dg_query_device_static: " + std::to_string(n) + " queries over " +
+                   std::to_string(b->worlds.size()) + " worlds need " + std::to_string(need) +
+                   " ints of grouping scratch, more than the static array holds (" +
+                   std::to_string(f2d::kStaticGroupCapacity) + ")";
     return -1;
   }
-  float4* boxes = f2d::staticBoxes(b->lastError);
-  if (!boxes) return -1;
+  int* group = f2d::staticScratch(b->lastError);
+  if (!group) return -1;
   if (n == 0) return 0;
-  return f2d_launch_queries(&b->dev.view, boxes, b->nShapes, static_cast<const f2d::Query*>(queries), n,
+  b->deviceNewer = true;  // pending tree enlargements applied
+  return f2d_launch_queries(&b->dev.view, group, static_cast<const f2d::Query*>(queries), n,
                             static_cast<f2d::QueryHit*>(hits), static_cast<int*>(counts), cap, b->stream);
 }

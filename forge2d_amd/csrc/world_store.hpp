@@ -110,7 +110,8 @@ struct Batch {
   QueryHit* dHits = nullptr;
   long dHitCap = 0;
   int* dCounts = nullptr;
-  float4* dBoxes = nullptr;  // packed fat AABBs of every shape (nShapes)
+  int* dGroup = nullptr;     // query grouping by world: the order, then per world a count and first slot
+  long dGroupCap = 0;
   int forcedBlock = 0;       // f2dg_set_block before the batch is laid out: every class's size
   int tunePeriod = 0;        // online lane tuning: a trial round every tunePeriod steps (0: off)
   long tuneClock = 0;        // steps taken through f2dg_step since tuning was switched on

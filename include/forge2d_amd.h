@@ -264,13 +264,18 @@ int f2dg_body_events(void* batch, int world, double* out);
 /* [bodies, shapes, live contacts, touching, overflow, active colours, step, error bits] */
 int f2dg_world_counters(void* batch, int world, int32_t* out);
 
-/* World queries over the worlds of a batch, one wavefront per query (the reference
- * queries one world per call). kind 1: closest ray hit (b2World_CastRayClosest,
- * S/world.c:2277-2308; p = origin x/y, translation x/y); 2: every ray hit
- * (b2World_CastRay with a callback returning 1, :2185-2254), shape-id order; 3: every
- * shape whose fat AABB overlaps the box p = lower x/y, upper x/y (b2World_OverlapAABB,
- * :2050-2105), shape-id order; 4: explosion (b2World_Explode, :2640-2747; p = position
- * x/y, radius, falloff, impulse per length; mask_lo/hi = the explosion's mask). */
+/* World queries over the worlds of a batch, one lane per query walking the world's
+ * trees as the reference's world queries do (the reference queries one world per
+ * call); results come in the reference's order. kind 1: closest ray hit
+ * (b2World_CastRayClosest, S/world.c:2277-2308; p = origin x/y, translation x/y);
+ * 2: every ray hit (b2World_CastRay with a callback returning 1, :2222-2254), in the
+ * order of b2DynamicTree_RayCast's walks (static, kinematic, dynamic tree; nearer child
+ * first); 3: every shape whose fat AABB overlaps the box p = lower x/y, upper x/y
+ * (b2World_OverlapAABB, :2071-2105), in b2DynamicTree_Query's depth-first order;
+ * 4: explosion (b2World_Explode, :2640-2747; p = position x/y, radius, falloff,
+ * impulse per length; mask_lo/hi = the explosion's mask), the dynamic tree's shapes in
+ * depth-first order. A count of -2 reports a tree deeper than the 64-entry walk
+ * stack (a height above 63). */
 typedef struct f2dQuery {
   int32_t world;
   int32_t kind;
@@ -286,16 +291,18 @@ typedef struct f2dQueryHit {
  * number of results of query i (0 for explosions, -1 for a world outside the batch);
  * the first min(count, cap) are written to hits[i * cap ...]. cap >= 1. */
 int f2dg_query(void* batch, const f2dQuery* queries, int n, f2dQueryHit* hits, int32_t* counts, int cap);
-/* Device arrays (queries, hits, counts), kinds 1-3 only: a packing pass over the shapes'
- * fat AABBs and the query launch on the batch's stream, no synchronization. */
+/* Device arrays (queries, hits, counts), kinds 1-3 only, on the batch's stream with no
+ * synchronization: the trees' pending leaf enlargements applied, the queries grouped by
+ * world (a counting sort into a library buffer of n + worlds + 1 ints, grown on
+ * demand), then one query per lane. Results do not depend on the grouping. */
 int f2dg_query_device(void* batch, const void* queries, int n, void* hits, void* counts, int cap);
 /* No counterpart in the reference: f2dg_query_device for callers that cannot allocate
  * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
- * The packed boxes go to a static device array of 262,144 boxes (4 MiB per GPU, held
+ * The grouping scratch is a static device array of 1,048,576 ints (4 MiB per GPU, held
  * until the process ends) instead of a library buffer; nothing is allocated. The
- * batch must be uploaded with no host edits pending, and hold at most 262,144
- * shapes; otherwise -1 and f2dg_last_error says why. Calls on one GPU share the
- * array: calls whose launches overlap on one GPU (other streams or threads) race. */
+ * batch must be uploaded with no host edits pending, and n + worlds + 1 must be at
+ * most 1,048,576; otherwise -1 and f2dg_last_error says why. Calls on one GPU share
+ * the array: calls whose launches overlap on one GPU (other streams or threads) race. */
 int f2dg_query_device_static(void* batch, const void* queries, int n, void* hits, void* counts, int cap);
 
 #ifdef __cplusplus

@@ -4543,19 +4543,19 @@ static CastOut ray_polygon(V2 origin, V2 d, real maxFraction, const GEO* g) {
   return shape_cast(&pa, id, &pb, id, d, maxFraction);
 }
 
-/* b2RayCastShape, S/shape.c:792-820 */
-static CastOut ray_shape(V2 origin, V2 translation, const GEO* g, XF xf) {
+/* b2RayCastShape, S/shape.c:792-820, with the input's maxFraction */
+static CastOut ray_shape(V2 origin, V2 translation, real maxFraction, const GEO* g, XF xf) {
   V2 lo = xfinvpt(xf, origin);
   V2 ld = invrot2(xf.q, translation);
   CastOut o;
   if (g->type == SHAPE_CAPSULE) {
-    o = ray_capsule(lo, ld, 1, g->v[0], g->v[1], g->radius);
+    o = ray_capsule(lo, ld, maxFraction, g->v[0], g->v[1], g->radius);
   } else if (g->type == SHAPE_CIRCLE) {
-    o = ray_circle(lo, ld, 1, g->v[0], g->radius);
+    o = ray_circle(lo, ld, maxFraction, g->v[0], g->radius);
   } else if (g->type == SHAPE_POLYGON) {
-    o = ray_polygon(lo, ld, 1, g);
+    o = ray_polygon(lo, ld, maxFraction, g);
   } else {
-    o = ray_segment(lo, ld, 1, g->v[0], g->v[1]);
+    o = ray_segment(lo, ld, maxFraction, g->v[0], g->v[1]);
   }
   o.point = xfpt(xf, o.point);
   o.normal = rot2(xf.q, o.normal);
@@ -4579,7 +4579,23 @@ static real projected_perimeter(const GEO* g, V2 line) {
   return absr(dot2(g->v[1], line) - dot2(g->v[0], line));
 }
 
-/* ExplosionCallback / b2World_Explode, S/world.c:2640-2747, shapes in shape-id order */
+/* A tree walk's stack (b2DynamicTree_Query and _RayCast keep B2_TREE_STACK_SIZE
+   entries; this one grows) */
+typedef struct {
+  int* v;
+  int count, cap;
+} WalkStack;
+static void wpush(WalkStack* st, int id) {
+  if (st->count == st->cap) {
+    st->cap = 2 * st->cap + 64;
+    st->v = (int*)realloc(st->v, (size_t)st->cap * sizeof(int));
+  }
+  st->v[st->count++] = id;
+}
+
+/* ExplosionCallback / b2World_Explode, S/world.c:2640-2747: the dynamic tree queried
+   over the explosion's box (b2DynamicTree_Query, S/dynamic_tree.c:1114-1170, child2
+   popped first), the callback on each leaf in turn */
 static void explode(OWorld* w, uint64_t mask, V2 position, real radius, real falloff, real impulsePerLength) {
   AB box;
   box.lo = v2(position.x - (radius + falloff), position.y - (radius + falloff));
@@ -4588,10 +4604,19 @@ static void explode(OWorld* w, uint64_t mask, V2 position, real radius, real fal
   id.p = v2(0, 0);
   id.q.c = 1;
   id.q.s = 0;
-  for (int s = 0; s < w->shapeCount; ++s) {
-    OShape* sh = w->shapes + s;
-    if (sh->proxyKey < 0 || (sh->proxyKey & 3) != DYNAMIC_BODY || (sh->cat & mask) == 0) continue;
-    if (!aabb_overlaps(sh->fat, box)) continue;
+  const OTree* t = w->trees + DYNAMIC_BODY;
+  if (t->nodeCount == 0) return;
+  WalkStack st = {NULL, 0, 0};
+  wpush(&st, t->root);
+  while (st.count > 0) {
+    const TNode* node = t->n + st.v[--st.count];
+    if (!aabb_overlaps(node->box, box) || (node->cat & mask) == 0) continue;
+    if (!(node->flags & TN_LEAF)) {
+      wpush(&st, node->c1);
+      wpush(&st, node->c2);
+      continue;
+    }
+    OShape* sh = w->shapes + node->c1;
     OBody* body = w->bodies + sh->body;
     XF xf = body->xf;
     Proxy pa = make_proxy(&sh->g), pb;
@@ -4621,10 +4646,30 @@ static void explode(OWorld* w, uint64_t mask, V2 position, real radius, real fal
     body->v = madd(body->v, body->invMass, impulse);
     body->w += body->invI * cross2(sub2(closest, body->center), impulse);
   }
+  free(st.v);
 }
 
+static void put_hit(B2OHit* out, int cap, int k, int shape, const CastOut* o) {
+  if (k >= cap) return;
+  out[k].shape = shape;
+  out[k].px = o ? (float)o->point.x : 0.0f;
+  out[k].py = o ? (float)o->point.y : 0.0f;
+  out[k].nx = o ? (float)o->normal.x : 0.0f;
+  out[k].ny = o ? (float)o->normal.y : 0.0f;
+  out[k].fraction = o ? (float)o->fraction : 0.0f;
+}
+
+/* World queries (kinds as in the device ABI: 1 closest ray, 2 every ray hit, 3 AABB
+   overlap, 4 explosion), each as the reference's world query walks its trees:
+   b2World_CastRayClosest / b2World_CastRay (S/world.c:2222-2308) over
+   b2DynamicTree_RayCast (S/dynamic_tree.c:1172-1290: nearer child first, the segment
+   shortened by accepted hits), b2World_OverlapAABB (S/world.c:2071-2105) over
+   b2DynamicTree_Query; results in the order the callbacks see them. Queries run in
+   order; an explosion changes the world for the queries after it. */
 int b2o_query(void* h, const B2OQuery* qs, int n, B2OHit* hits, int32_t* counts, int cap) {
   OWorld* w = (OWorld*)h;
+  if (!w->hasTrees) return -1;
+  WalkStack st = {NULL, 0, 0};
   for (int qi = 0; qi < n; ++qi) {
     const B2OQuery* q = qs + qi;
     uint64_t qcat = (uint64_t)q->catLo | ((uint64_t)q->catHi << 32);
@@ -4636,64 +4681,121 @@ int b2o_query(void* h, const B2OQuery* qs, int n, B2OHit* hits, int32_t* counts,
       counts[qi] = 0;
       continue;
     }
-    V2 origin = v2(q->p[0], q->p[1]), d = v2(q->p[2], q->p[3]);
-    V2 p2 = madd(origin, 1, d);
-    AB seg;
-    seg.lo = v2(minr(origin.x, p2.x), minr(origin.y, p2.y));
-    seg.hi = v2(maxr(origin.x, p2.x), maxr(origin.y, p2.y));
-    AB box;
-    box.lo = v2(q->p[0], q->p[1]);
-    box.hi = v2(q->p[2], q->p[3]);
+    if (q->kind == 3) {
+      AB box;
+      box.lo = v2(q->p[0], q->p[1]);
+      box.hi = v2(q->p[2], q->p[3]);
+      for (int tt = STATIC_BODY; tt <= DYNAMIC_BODY; ++tt) {
+        const OTree* t = w->trees + tt;
+        if (t->nodeCount == 0) continue;
+        st.count = 0;
+        wpush(&st, t->root);
+        while (st.count > 0) {
+          const TNode* node = t->n + st.v[--st.count];
+          if (!aabb_overlaps(node->box, box) || (node->cat & qmask) == 0) continue;
+          if (node->flags & TN_LEAF) {
+            const OShape* sh = w->shapes + node->c1;
+            /* b2ShouldQueryCollide, S/shape.h:132-135 */
+            if ((sh->cat & qmask) == 0 || (sh->mask & qcat) == 0) continue;
+            put_hit(out, cap, total++, node->c1, NULL);
+          } else {
+            wpush(&st, node->c1);
+            wpush(&st, node->c2);
+          }
+        }
+      }
+      counts[qi] = total;
+      continue;
+    }
+    if (q->kind != 1 && q->kind != 2) {
+      counts[qi] = 0;
+      continue;
+    }
+    const int closest = q->kind == 1;
+    V2 p1 = v2(q->p[0], q->p[1]), d = v2(q->p[2], q->p[3]);
+    V2 r = norm2(d);
+    V2 v = v2(-r.y, r.x); /* b2CrossSV(1, r) */
+    V2 absV = v2(absr(v.x), absr(v.y));
+    real inputMax = 1, worldFraction = 1;
     int best = -1;
     CastOut bestOut = cast_none();
-    for (int s = 0; s < w->shapeCount; ++s) {
-      const OShape* sh = w->shapes + s;
-      if (sh->proxyKey < 0) continue;
-      /* b2ShouldQueryCollide, S/shape.h:132-135 */
-      if ((sh->cat & qmask) == 0 || (sh->mask & qcat) == 0) continue;
-      if (q->kind == 3) {
-        if (!aabb_overlaps(sh->fat, box)) continue;
-        if (total < cap) {
-          memset(out + total, 0, sizeof(B2OHit));
-          out[total].shape = s;
+    for (int tt = STATIC_BODY; tt <= DYNAMIC_BODY; ++tt) {
+      const OTree* t = w->trees + tt;
+      if (t->nodeCount > 0) {
+        real maxFraction = inputMax;
+        V2 p2 = madd(p1, maxFraction, d);
+        AB seg;
+        seg.lo = v2(minr(p1.x, p2.x), minr(p1.y, p2.y));
+        seg.hi = v2(maxr(p1.x, p2.x), maxr(p1.y, p2.y));
+        st.count = 0;
+        wpush(&st, t->root);
+        int terminated = 0;
+        while (st.count > 0 && !terminated) {
+          const TNode* node = t->n + st.v[--st.count];
+          if ((node->cat & qmask) == 0 || !aabb_overlaps(node->box, seg)) continue;
+          V2 c = v2((real)0.5f * (node->box.lo.x + node->box.hi.x), (real)0.5f * (node->box.lo.y + node->box.hi.y));
+          V2 e = v2((real)0.5f * (node->box.hi.x - node->box.lo.x), (real)0.5f * (node->box.hi.y - node->box.lo.y));
+          real term1 = absr(dot2(v, sub2(p1, c)));
+          real term2 = dot2(absV, e);
+          if (term2 < term1) continue;
+          if (node->flags & TN_LEAF) {
+            /* RayCastCallback, S/world.c:2191-2220 */
+            const OShape* sh = w->shapes + node->c1;
+            real value = maxFraction;
+            if ((sh->cat & qmask) != 0 && (sh->mask & qcat) != 0) {
+              CastOut o = ray_shape(p1, d, maxFraction, &sh->g, w->bodies[sh->body].xf);
+              if (o.hit) {
+                real fraction;
+                if (closest) {
+                  /* b2RayCastClosestFcn, S/world.c:2258-2275 */
+                  if (o.fraction == 0) {
+                    fraction = -1;
+                  } else {
+                    best = node->c1;
+                    bestOut = o;
+                    fraction = o.fraction;
+                  }
+                } else {
+                  put_hit(out, cap, total++, node->c1, &o);
+                  fraction = 1;
+                }
+                if (0 <= fraction && fraction <= 1) worldFraction = fraction;
+                value = fraction;
+              }
+            }
+            if (value == 0) {
+              terminated = 1;
+            } else if (0 < value && value <= maxFraction) {
+              maxFraction = value;
+              p2 = madd(p1, maxFraction, d);
+              seg.lo = v2(minr(p1.x, p2.x), minr(p1.y, p2.y));
+              seg.hi = v2(maxr(p1.x, p2.x), maxr(p1.y, p2.y));
+            }
+          } else {
+            const AB* b1 = &t->n[node->c1].box;
+            const AB* b2 = &t->n[node->c2].box;
+            V2 c1 = v2((real)0.5f * (b1->lo.x + b1->hi.x), (real)0.5f * (b1->lo.y + b1->hi.y));
+            V2 c2 = v2((real)0.5f * (b2->lo.x + b2->hi.x), (real)0.5f * (b2->lo.y + b2->hi.y));
+            V2 e1 = sub2(p1, c1), e2 = sub2(p1, c2);
+            if (dot2(e1, e1) < dot2(e2, e2)) {
+              wpush(&st, node->c2);
+              wpush(&st, node->c1);
+            } else {
+              wpush(&st, node->c1);
+              wpush(&st, node->c2);
+            }
+          }
         }
-        total++;
-        continue;
       }
-      if (!aabb_overlaps(sh->fat, seg)) continue;
-      CastOut o = ray_shape(origin, d, &sh->g, w->bodies[sh->body].xf);
-      if (!o.hit) continue;
-      if (q->kind == 1) {
-        if (!(o.fraction > 0)) continue; /* b2RayCastClosestFcn ignores initial overlaps */
-        if (best < 0 || o.fraction < bestOut.fraction) {
-          best = s;
-          bestOut = o;
-        }
-        continue;
-      }
-      if (total < cap) {
-        B2OHit* r = out + total;
-        r->shape = s;
-        r->px = (float)o.point.x;
-        r->py = (float)o.point.y;
-        r->nx = (float)o.normal.x;
-        r->ny = (float)o.normal.y;
-        r->fraction = (float)o.fraction;
-      }
-      total++;
+      if (worldFraction == 0) break;
+      inputMax = worldFraction;
     }
-    if (q->kind == 1 && best >= 0) {
-      if (cap > 0) {
-        out[0].shape = best;
-        out[0].px = (float)bestOut.point.x;
-        out[0].py = (float)bestOut.point.y;
-        out[0].nx = (float)bestOut.normal.x;
-        out[0].ny = (float)bestOut.normal.y;
-        out[0].fraction = (float)bestOut.fraction;
-      }
-      total = 1;
+    if (closest) {
+      if (best >= 0) put_hit(out, cap, 0, best, &bestOut);
+      total = best >= 0 ? 1 : 0;
     }
     counts[qi] = total;
   }
+  free(st.v);
   return 0;
 }

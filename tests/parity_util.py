@@ -146,39 +146,12 @@ def tree_differences(mine: S.Snapshot, ref: S.Snapshot, box_tol: float = 0.0) ->
 
 
 QUERY_FIXTURES = ["queries", "queries_sleep", "queries_wake2"]
-# fixtures whose explosion wakes several sleeping sets: the reference wakes them in its
-# dynamic tree's order, the restatements in shape-id order
-WAKE_ORDER_FIXTURES = {"queries_wake2"}
-
-
-def wake_order_differences(mine: S.Snapshot, ref: S.Snapshot) -> list:
-    """island_differences, where the awake orders (bodies' and islands' places in the
-    awake set, the free solver-set ids) may be permutations of the reference's: the
-    same bodies and islands are awake and the same set ids are free."""
-    out = []
-    for d in island_differences(mine, ref):
-        a, b = mine.islands, ref.islands
-        if d in ("awake_islands", "free_sets", "awake_bodies"):
-            if np.array_equal(np.sort(getattr(a, d)), np.sort(getattr(b, d))):
-                continue
-        elif d == "body.local_index":
-            awake = ref.bodies["set_index"] == 2
-            if np.array_equal(mine.bodies["local_index"][~awake], ref.bodies["local_index"][~awake]) and \
-                    np.array_equal(np.sort(mine.bodies["local_index"][awake]), np.sort(ref.bodies["local_index"][awake])):
-                continue
-        elif d == "islands":
-            keep = [f for f in a.islands.dtype.names if f != "local_index"]
-            if all(np.array_equal(a.islands[f], b.islands[f]) for f in keep):
-                continue
-        out.append(d)
-    return out
 
 
 def query_mismatches(queries, expected, counts, hits) -> list:
     """Indices of the queries whose results differ from the expected ones (a list of
-    hit arrays per query): closest ray hits field by field; every-hit rays and AABB
-    overlaps as sets (sorted by shape id: the reference reports them in its tree
-    walk's order), each hit's point, normal and fraction bit for bit."""
+    hit arrays per query), in order: every hit's shape, point, normal and fraction
+    bit for bit, in the order the reference's tree walks report them."""
     bad = []
     for i in range(len(queries)):
         ref = np.asarray(expected[i])
@@ -186,9 +159,6 @@ def query_mismatches(queries, expected, counts, hits) -> list:
         if counts[i] != len(ref):
             bad.append(i)
             continue
-        if int(queries[i]["kind"]) in (2, 3):
-            ref = np.sort(ref, order="shape")
-            mine = np.sort(mine, order="shape")
         for f in ref.dtype.names:
             if not np.array_equal(ref[f].view(np.uint32), mine[f].view(np.uint32)):
                 bad.append(i)

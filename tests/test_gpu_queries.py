@@ -5,8 +5,9 @@ f2d_world_explode) against the reference and the oracle:
 * the reference's own results on the query fixtures (tests/golden, written by the
   reference build: b2World_CastRayClosest, b2World_CastRay, b2World_OverlapAABB,
   b2World_Explode, S/world.c:2050-2308, 2640-2747) from the reference's snapshot:
-  closest hits bit for bit, every-hit rays and overlaps as sets, and after the
-  explosions every body, the island graph, the solver sets and the colouring;
+  every hit bit for bit and in the reference's tree-walk order, and after the
+  explosions every body, the island graph, the solver sets (in the order the
+  explosions woke them) and the colouring;
 * the oracle (oracle/b2oracle.c, pinned by tests/test_oracle_queries.py) on larger
   scenes the device has stepped, and on a batch of 64 mixed worlds queried in one
   launch (worlds interleaved, two explosions in one world run in order);

@@ -1,10 +1,12 @@
 """The allocation-free query entry point f2dg_query_device_static against the
 library's device query entry point f2dg_query_device: both read the same device
 arrays and must write the same counts and hit records bit for bit (the hit slots
-beyond each count keep their sentinel bytes in both). Two calls in a row, the
-second on a smaller batch with fewer queries, so stale boxes of the first call in
-the static array cannot leak into the second. A batch that is not uploaded, or
-holds more shapes than the static array, is refused with an error."""
+beyond each count keep their sentinel bytes in both). The queries come shuffled
+across worlds, so both group them by world in their scratch (the static array, or a
+buffer the library allocates). Two calls in a row, the second on a smaller batch
+with fewer queries, so stale grouping of the first call in the static array cannot
+leak into the second. A batch that is not uploaded, or a call with more queries and
+worlds than the static array holds, is refused with an error."""
 import numpy as np
 import pytest
 
@@ -60,6 +62,7 @@ def test_static_entry_point_matches_device_entry_point(gpu):
         b.step(steps=steps)
         assert b.sync() == 0
         q = world_queries(b, worlds, per_world, seed)
+        q = q[np.random.default_rng(seed).permutation(len(q))]
         ref_counts, ref_hits = run(L.f2dg_query_device, b, q)
         counts, hits = run(L.f2dg_query_device_static, b, q)
         assert (ref_counts > 0).sum() > len(q) // 4  # the queries do hit shapes
@@ -74,9 +77,8 @@ def test_static_entry_point_refuses_what_it_cannot_hold(gpu):
     assert L.f2dg_query_device_static(b.h, None, 1, None, None, 1) == -1  # not uploaded: it allocates nothing
     assert b"upload" in L.f2dg_last_error(b.h)
     b.close()
-    big = F.Batch(scenes.config4(worlds=2400))
-    assert big.shape_count(0) > 0
-    big.upload()
-    assert L.f2dg_query_device_static(big.h, None, 1, None, None, 1) == -1
-    assert b"262144" in L.f2dg_last_error(big.h)
-    big.close()
+    small = F.Batch(scenes.config4(worlds=4, lo=10, hi=50))
+    small.upload()
+    assert L.f2dg_query_device_static(small.h, None, 1 << 20, None, None, 1) == -1
+    assert b"1048576" in L.f2dg_last_error(small.h)
+    small.close()
