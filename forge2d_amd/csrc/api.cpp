@@ -58,7 +58,7 @@ void HostWorld::ensureColorWords() {
 
 // b2CreateBody, S/body.c:177-330
 int HostWorld::createBody(int type, Vec2 p, Rot q, Vec2 v, float w, float linDamp, float angDamp, float gravityScale,
-                          float sleepThreshold, int flags) {
+                          float sleepThreshold, int flags, bool isAwake) {
   BodySim s = {};
   s.xf.p = p;
   s.xf.q = q;
@@ -79,27 +79,56 @@ int HostWorld::createBody(int type, Vec2 p, Rot q, Vec2 v, float w, float linDam
   s.jointCount = 0;
   s.sleepThreshold = sleepThreshold;
   s.sleepTime = 0.0f;
+  // solver set and island (b2CreateBody, S/body.c:176-324): static bodies join the
+  // static set; a body created awake (or that may not sleep) joins the awake set, a
+  // body created asleep a new sleeping set of its own (its velocities are not kept:
+  // a sleeping set holds no body states); each non-static body gets an island
+  const bool awake = isAwake || (flags & BF_ENABLE_SLEEP) == 0;
   BodyState st = {};
   st.v = v;
   st.w = w;
   st.dq = {1.0f, 0.0f};
-  if (type == kStatic) {
+  if (type == kStatic || !awake) {
     st.v = {0.0f, 0.0f};
     st.w = 0.0f;
   }
-  // solver set and island (b2CreateBody, S/body.c:198-324): static bodies join the
-  // static set; every other body joins the awake set with an island of its own
   s.islandId = s.islandPrev = s.islandNext = -1;
   int id = (int)sims.size();
+  int sleepSet = -1;
   if (type == kStatic) {
     s.setIndex = kStaticSet;
     s.localIndex = staticBodies++;
-  } else {
+  } else if (awake) {
     s.setIndex = kAwakeSet;
     s.localIndex = hdr.awakeBodyCount;
+  } else {
+    setFree.resize(std::max<size_t>(setFree.size(), (size_t)hdr.setCount + 4));
+    sleepSet = poolAlloc(setFree.data(), hdr.setFreeCount, hdr.setCount);
+    s.setIndex = sleepSet;
+    s.localIndex = 0;
   }
   sims.push_back(s);
-  if (type != kStatic) {
+  if (type != kStatic && !awake) {
+    // b2CreateIslandForBody with the new set (S/body.c:84-97, S/island.c:18-55): the
+    // island is the set's first and only island
+    islands.resize(islands.size() + 1);
+    islandFree.resize(islands.size());
+    awakeIslands.resize(islands.size());
+    setFree.resize(std::max(islands.size() + 4, setFree.size()));
+    hdr.islandCap = (int)islands.size();
+    const int isl = poolAlloc(islandFree.data(), hdr.islandFreeCount, hdr.islandCount);
+    Island& is = islands[isl];
+    is = Island{};
+    is.id = isl;
+    is.setIndex = sleepSet;
+    is.localIndex = 0;
+    is.headBody = is.tailBody = id;
+    is.bodyCount = 1;
+    is.headContact = is.tailContact = -1;
+    is.headJoint = is.tailJoint = -1;
+    is.parent = -1;
+    sims[id].islandId = isl;
+  } else if (type != kStatic) {
     awakeBodies.resize(hdr.awakeBodyCount);  // the mirror may hold a committed batch's full-length vector
     awakeBodies.push_back(id);
     hdr.awakeBodyCount = (int)awakeBodies.size();
@@ -531,7 +560,7 @@ static int parseSpec(Batch& b, const unsigned char* buf, long bytes) {
       if (sb.flags & 4) flags |= BF_ALLOW_FAST_ROTATION;
       if (sb.flags & 8) flags |= BF_ENABLE_SLEEP;
       hw.createBody(sb.type, {sb.px, sb.py}, {sb.qc, sb.qs}, {sb.vx, sb.vy}, sb.w, sb.lin_damp, sb.ang_damp,
-                    sb.gravity_scale, sb.sleep_threshold, flags);
+                    sb.gravity_scale, sb.sleep_threshold, flags, (sb.flags & 16) != 0);
     }
     for (int k = 0; k < sw.shape_count; ++k) {
       SpecShape ss;
@@ -2627,11 +2656,17 @@ void f2d_create_body(uint32_t w, int type, float px, float py, float qc, float q
                      float sleep_threshold, const char* name, int enable_sleep, int is_awake, int fixed_rotation,
                      int is_bullet, int is_enabled, int allow_fast_rotation, int32_t* out) {
   (void)name;
-  (void)is_awake;
   out[0] = 0;
   out[1] = 0;
   Batch* b = batchOf(w);
-  if (!b || !is_enabled) return;
+  if (!b) return;
+  if (!is_enabled) {
+    // the reference puts the body in the disabled set (no proxies, no simulation until
+    // enabled); that set is not kept here: refused with a null id and an error
+    b->lastError = "f2d_create_body: bodies created disabled (is_enabled = 0) are not supported";
+    std::fprintf(stderr, "forge2d_amd: %s\n", b->lastError.c_str());
+    return;
+  }
   download(*b);
   if (b->committed) freeDevice(*b);  // new bodies change the batch layout
   int flags = 0;
@@ -2641,7 +2676,7 @@ void f2d_create_body(uint32_t w, int type, float px, float py, float qc, float q
   if (enable_sleep) flags |= BF_ENABLE_SLEEP;
   HostWorld& hw = b->worlds[0];
   int id = hw.createBody(type, {px, py}, {qc, qs}, {lvx, lvy}, angular_velocity, linear_damping, angular_damping,
-                         gravity_scale, sleep_threshold, flags);
+                         gravity_scale, sleep_threshold, flags, is_awake != 0);
   b->hostDirty = true;
   out[0] = id + 1;
   out[1] = (int32_t)(((w & 0xFFFF) - 1) | ((uint32_t)hw.bodyGen[id] << 16));

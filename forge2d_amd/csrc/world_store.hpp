@@ -56,7 +56,7 @@ struct HostWorld {
 
   void ensureColorWords();
   int createBody(int type, Vec2 p, Rot q, Vec2 v, float w, float linDamp, float angDamp, float gravityScale,
-                 float sleepThreshold, int flags);
+                 float sleepThreshold, int flags, bool isAwake = true);
   int createShape(int body, const ShapeGeom& g, float density, float friction, float restitution, float rolling,
                   float tangentSpeed, uint64_t cat, uint64_t mask, int group, int flags, bool invokeContact,
                   bool updateMass);

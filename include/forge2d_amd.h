@@ -78,7 +78,11 @@ int f2d_world_is_sleeping_enabled(uint32_t w);
 void f2d_world_enable_continuous(uint32_t w, int enabled);
 int f2d_world_is_continuous_enabled(uint32_t w);
 
-/* F/native/wasm/f2d_shim.c:192-218 (b2CreateBody); out = [index1, world_and_generation] */
+/* F/native/wasm/f2d_shim.c:192-218 (b2CreateBody); out = [index1, world_and_generation].
+ * is_awake = 0 with enable_sleep = 1 creates the body asleep in a solver set and island of
+ * its own, its velocities dropped, as the reference does (S/body.c:176-324).
+ * is_enabled = 0 (the reference's disabled set) is not supported: out stays {0, 0} and
+ * f2dg_last_error / stderr say so. */
 void f2d_create_body(uint32_t w, int type, float px, float py, float qc, float qs, float lvx, float lvy,
                      float angular_velocity, float linear_damping, float angular_damping, float gravity_scale,
                      float sleep_threshold, const char* name, int enable_sleep, int is_awake, int fixed_rotation,

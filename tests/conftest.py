@@ -10,7 +10,7 @@ if ROOT not in sys.path:
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 FIXTURE_NAMES = ["pyramid12", "pyramid40", "mixed", "overflow", "bounce", "ccd", "mixed_ccd", "events", "sleep",
-                 "features"]
+                 "features", "asleep"]
 
 
 def pytest_configure(config):

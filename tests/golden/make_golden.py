@@ -255,6 +255,30 @@ def features_world():
     return w
 
 
+def asleep_world():
+    """Bodies created asleep (b2CreateBody with isAwake false, S/body.c:176-324): each
+    gets a sleeping solver set and island of its own, keeps its place in the air and
+    loses its initial velocity; a ball dropped on one wakes its set (b2LinkContact),
+    another falls past them; a body created "asleep" that may not sleep, and a static
+    body created asleep, are created awake."""
+    w = scenes.World(scenes.world_header(sleep=True, continuous=True))
+    g = w.add_body(scenes.STATIC, 0.0, -1.0)
+    w.add_box(g, 30.0, 1.0)
+    asleep = scenes.BF_ENABLE_SLEEP
+    for i in range(3):
+        b = w.add_body(scenes.DYNAMIC, -6.0 + 3.0 * i, 4.0, vx=2.0 * i, w=0.5, flags=asleep)
+        w.add_box(b, 0.5, 0.5)
+    b = w.add_body(scenes.DYNAMIC, -3.0, 9.0)
+    w.add_circle(b, 0.4)
+    b = w.add_body(scenes.DYNAMIC, 8.0, 6.0, vx=-1.0, flags=0)  # may not sleep: created awake
+    w.add_capsule(b, (-0.4, 0.0), (0.4, 0.0), 0.2)
+    s = w.add_body(scenes.STATIC, 12.0, 1.0, flags=asleep)
+    w.add_box(s, 1.0, 1.0)
+    b = w.add_body(scenes.DYNAMIC, 12.2, 4.0)
+    w.add_box(b, 0.4, 0.4)
+    return w
+
+
 def query_world():
     """World queries (S/world.c:2050-2308, 2640-2747): every shape type on static,
     kinematic and dynamic bodies, rounded boxes (ray cast by shape cast), a body with
@@ -407,6 +431,7 @@ FIXTURES = {
     "events": (events_world, list(range(0, 121))),
     "sleep": (sleep_world, list(range(0, 261))),
     "features": (features_world, list(range(0, 7)) + [30, 31, 32, 60, 61, 62, 120, 121, 122]),
+    "asleep": (asleep_world, list(range(0, 61)) + [120, 121, 122]),
     "mixed_ccd": (lambda: scenes.container_world(200, 15.0, 40.0, 20261202, continuous=True, chains=3,
                                                  drop_band=(0.2, 1.0)),
                   [0, 1, 2, 3, 40, 41, 42, 43, 44, 45, 46, 47, 48, 49, 50, 60, 61, 62]),
