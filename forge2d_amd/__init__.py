@@ -111,6 +111,66 @@ SIGNATURES = [
     ("f2d_world_get_contact_hit_events", _I, [_U, _PD]),
     ("f2d_world_get_body_event_count", _I, [_U]),
     ("f2d_world_get_body_events", _I, [_U, _PD]),
+    ("f2d_body_set_transform", None, [ctypes.c_int32, _U, _F, _F, _F, _F]),
+    ("f2d_body_set_angular_velocity", None, [ctypes.c_int32, _U, _F]),
+    ("f2d_body_apply_force", None, [ctypes.c_int32, _U, _F, _F, _F, _F, _I]),
+    ("f2d_body_apply_torque", None, [ctypes.c_int32, _U, _F, _I]),
+    ("f2d_body_apply_linear_impulse", None, [ctypes.c_int32, _U, _F, _F, _F, _F, _I]),
+    ("f2d_body_apply_angular_impulse", None, [ctypes.c_int32, _U, _F, _I]),
+    ("f2d_body_set_mass_data", None, [ctypes.c_int32, _U, _F, _F, _F, _F]),
+    ("f2d_body_apply_mass_from_shapes", None, [ctypes.c_int32, _U]),
+    ("f2d_body_get_world_point", None, [ctypes.c_int32, _U, _F, _F, _PF]),
+    ("f2d_body_get_local_point", None, [ctypes.c_int32, _U, _F, _F, _PF]),
+    ("f2d_body_get_shape_count", _I, [ctypes.c_int32, _U]),
+    ("f2d_body_get_shapes", _I, [ctypes.c_int32, _U, _PI, _I]),
+    ("f2d_body_get_joint_count", _I, [ctypes.c_int32, _U]),
+    ("f2d_body_get_joints", _I, [ctypes.c_int32, _U, _PI, _I]),
+    ("f2d_body_get_name", ctypes.c_char_p, [ctypes.c_int32, _U]),
+    ("f2d_body_set_name", None, [ctypes.c_int32, _U, ctypes.c_char_p]),
+    ("f2d_body_is_enabled", _I, [ctypes.c_int32, _U]),
+    ("f2d_body_get_gravity_scale", _F, [ctypes.c_int32, _U]),
+    ("f2d_body_set_gravity_scale", None, [ctypes.c_int32, _U, _F]),
+    ("f2d_body_get_linear_damping", _F, [ctypes.c_int32, _U]),
+    ("f2d_body_set_linear_damping", None, [ctypes.c_int32, _U, _F]),
+    ("f2d_body_get_angular_damping", _F, [ctypes.c_int32, _U]),
+    ("f2d_body_set_angular_damping", None, [ctypes.c_int32, _U, _F]),
+    ("f2d_body_is_bullet", _I, [ctypes.c_int32, _U]),
+    ("f2d_body_set_bullet", None, [ctypes.c_int32, _U, _I]),
+    ("f2d_body_is_fixed_rotation", _I, [ctypes.c_int32, _U]),
+    ("f2d_body_set_fixed_rotation", None, [ctypes.c_int32, _U, _I]),
+    ("f2d_shape_is_valid", _I, [ctypes.c_int32, _U]),
+    ("f2d_shape_get_body", None, [ctypes.c_int32, _U, _PI]),
+    ("f2d_shape_get_type", _I, [ctypes.c_int32, _U]),
+    ("f2d_shape_is_sensor", _I, [ctypes.c_int32, _U]),
+    ("f2d_shape_get_density", _F, [ctypes.c_int32, _U]),
+    ("f2d_shape_set_density", None, [ctypes.c_int32, _U, _F, _I]),
+    ("f2d_shape_get_friction", _F, [ctypes.c_int32, _U]),
+    ("f2d_shape_set_friction", None, [ctypes.c_int32, _U, _F]),
+    ("f2d_shape_get_restitution", _F, [ctypes.c_int32, _U]),
+    ("f2d_shape_set_restitution", None, [ctypes.c_int32, _U, _F]),
+    ("f2d_shape_get_filter", None, [ctypes.c_int32, _U, _PI]),
+    ("f2d_shape_are_contact_events_enabled", _I, [ctypes.c_int32, _U]),
+    ("f2d_shape_enable_contact_events", None, [ctypes.c_int32, _U, _I]),
+    ("f2d_shape_are_hit_events_enabled", _I, [ctypes.c_int32, _U]),
+    ("f2d_shape_enable_hit_events", None, [ctypes.c_int32, _U, _I]),
+    ("f2d_shape_are_pre_solve_events_enabled", _I, [ctypes.c_int32, _U]),
+    ("f2d_shape_enable_pre_solve_events", None, [ctypes.c_int32, _U, _I]),
+    ("f2d_shape_test_point", _I, [ctypes.c_int32, _U, _F, _F]),
+    ("f2d_shape_get_circle", None, [ctypes.c_int32, _U, _PF]),
+    ("f2d_shape_get_capsule", None, [ctypes.c_int32, _U, _PF]),
+    ("f2d_shape_get_segment", None, [ctypes.c_int32, _U, _PF]),
+    ("f2d_shape_get_polygon", _I, [ctypes.c_int32, _U, _PF]),
+    ("f2d_shape_get_aabb", None, [ctypes.c_int32, _U, _PF]),
+    ("f2d_joint_is_valid", _I, [ctypes.c_int32, _U]),
+    ("f2d_joint_get_type", _I, [ctypes.c_int32, _U]),
+    ("f2d_joint_get_body_a", None, [ctypes.c_int32, _U, _PI]),
+    ("f2d_joint_get_body_b", None, [ctypes.c_int32, _U, _PI]),
+    ("f2d_joint_get_local_anchor_a", None, [ctypes.c_int32, _U, _PF]),
+    ("f2d_joint_get_local_anchor_b", None, [ctypes.c_int32, _U, _PF]),
+    ("f2d_joint_get_collide_connected", _I, [ctypes.c_int32, _U]),
+    ("f2d_joint_wake_bodies", None, [ctypes.c_int32, _U]),
+    ("f2d_joint_get_constraint_force", None, [ctypes.c_int32, _U, _PF]),
+    ("f2d_joint_get_constraint_torque", _F, [ctypes.c_int32, _U]),
     ("f2d_world_batch", _VP, [_U]),
     ("f2d_set_query_callbacks", None, [_VP, _VP]),
     ("f2d_world_cast_ray_closest", _I, [_U, _F, _F, _F, _F, _U, _U, _U, _U, _PD]),

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "manifold.hpp"
+#include "query.hpp"
 #include "solver.hpp"
 #include "world_store.hpp"
 
@@ -2742,27 +2743,31 @@ void f2d_body_set_linear_velocity(int32_t index1, uint32_t wg, float x, float y)
 void f2d_body_apply_force_to_center(int32_t index1, uint32_t wg, float fx, float fy, int wake) {
   int id;
   HostWorld* hw = bodyWorld(index1, wg, &id);
-  if (!hw || hw->sims[id].type != kDynamic) return;
-  if (wake) {
+  if (!hw) return;
+  if (wake && hw->sims[id].setIndex >= kFirstSleepingSet) {
     wakeBody(wg, id);
     hw = bodyWorld(index1, wg, &id);
   }
-  if (hw->sims[id].setIndex != kAwakeSet) return;
+  if (!hw || hw->sims[id].setIndex != kAwakeSet) return;
   hw->sims[id].force = add(hw->sims[id].force, {fx, fy});
   bodyBatch(wg)->hostDirty = true;
 }
 
-// b2Body_ApplyLinearImpulseToCenter, S/body.c:975-995: v += invMass * impulse
+// b2Body_ApplyLinearImpulseToCenter, S/body.c:975-995: v += invMass * impulse, then
+// b2LimitVelocity (S/body.c:28-35)
 void f2d_body_apply_linear_impulse_to_center(int32_t index1, uint32_t wg, float ix, float iy, int wake) {
   int id;
   HostWorld* hw = bodyWorld(index1, wg, &id);
-  if (!hw || hw->sims[id].type != kDynamic) return;
-  if (wake) {
+  if (!hw) return;
+  if (wake && hw->sims[id].setIndex >= kFirstSleepingSet) {
     wakeBody(wg, id);
     hw = bodyWorld(index1, wg, &id);
   }
-  if (hw->sims[id].setIndex != kAwakeSet) return;
-  hw->states[id].v = mulAdd(hw->states[id].v, hw->sims[id].invMass, {ix, iy});
+  if (!hw || hw->sims[id].setIndex != kAwakeSet) return;
+  BodyState& st = hw->states[id];
+  st.v = mulAdd(st.v, hw->sims[id].invMass, {ix, iy});
+  const float v2 = lengthSq(st.v), maxSpeed = hw->hdr.maxLinearSpeed;
+  if (v2 > maxSpeed * maxSpeed) st.v = mulSV(maxSpeed / sqrtf(v2), st.v);
   bodyBatch(wg)->hostDirty = true;
 }
 
@@ -2834,6 +2839,636 @@ void f2d_body_get_local_center(int32_t index1, uint32_t wg, float* out) {
   if (!hw) return;
   out[0] = hw->sims[id].localCenter.x;
   out[1] = hw->sims[id].localCenter.y;
+}
+
+// ---- more of the body API (S/body.c), on the host mirror ---------------------
+
+// b2Body_SetTransform, S/body.c:688-735: the shapes' boxes follow the new transform;
+// a fat box that no longer holds its shape's box grows and its proxy moves in its
+// tree (b2BroadPhase_MoveProxy, S/broad_phase.c:116-123: removed, reinserted without
+// rotations, and buffered as moved)
+void f2d_body_set_transform(int32_t index1, uint32_t wg, float px, float py, float qc, float qs) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  BodySim& sim = hw->sims[id];
+  sim.xf.p = {px, py};
+  sim.xf.q = {qc, qs};
+  sim.center = transformPoint(sim.xf, sim.localCenter);
+  sim.rotation0 = sim.xf.q;
+  sim.center0 = sim.center;
+  const Xf xf = sim.xf;
+  for (int s = sim.headShape, g = 0; s >= 0 && g < (int)hw->shapes.size(); s = hw->shapes[s].next, ++g) {
+    Shape& shape = hw->shapes[s];
+    AABB aabb = computeShapeAABB(hw->geoms[s], xf);
+    aabb.lo.x -= kSpeculativeDistance;
+    aabb.lo.y -= kSpeculativeDistance;
+    aabb.hi.x += kSpeculativeDistance;
+    aabb.hi.y += kSpeculativeDistance;
+    shape.aabb = aabb;
+    if (!aabbContains(shape.fat, aabb)) {
+      AABB fat;
+      fat.lo.x = aabb.lo.x - kAabbMargin;
+      fat.lo.y = aabb.lo.y - kAabbMargin;
+      fat.hi.x = aabb.hi.x + kAabbMargin;
+      fat.hi.y = aabb.hi.y + kAabbMargin;
+      shape.fat = fat;
+      if (shape.proxyKey >= 0) {
+        shape.flags &= ~SF_TREE_PENDING;
+        treeMoveProxy(hw->treeRef(shape.proxyKey & 3), shape.proxyKey >> 2, fat);
+        if (!(shape.flags & SF_MOVED)) {  // b2BufferMove: a proxy enters the move array once
+          shape.flags |= SF_MOVED;
+          shape.moveRank = (1 << 30) + hw->createdSeq++;
+        }
+      }
+    }
+  }
+  bodyBatch(wg)->hostDirty = true;
+}
+
+// b2Body_SetAngularVelocity, S/body.c:791-812
+void f2d_body_set_angular_velocity(int32_t index1, uint32_t wg, float w) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw || hw->sims[id].type == kStatic || hw->bodyFixedRotation[id]) return;
+  if (w != 0.0f) {
+    wakeBody(wg, id);
+    hw = bodyWorld(index1, wg, &id);
+  }
+  if (hw->sims[id].setIndex != kAwakeSet) return;
+  hw->states[id].w = w;
+  bodyBatch(wg)->hostDirty = true;
+}
+
+// the awake body's sim and state after an optional wake (the force and impulse calls)
+static HostWorld* awakeTarget(int32_t index1, uint32_t wg, int wake, int* id) {
+  HostWorld* hw = bodyWorld(index1, wg, id);
+  if (!hw) return nullptr;
+  if (wake && hw->sims[*id].setIndex >= kFirstSleepingSet) {
+    wakeBody(wg, *id);
+    hw = bodyWorld(index1, wg, id);
+  }
+  if (!hw || hw->sims[*id].setIndex != kAwakeSet) return nullptr;
+  bodyBatch(wg)->hostDirty = true;
+  return hw;
+}
+
+// b2LimitVelocity, S/body.c:28-35
+static void limitVelocity(BodyState& st, float maxLinearSpeed) {
+  const float v2 = lengthSq(st.v);
+  if (v2 > maxLinearSpeed * maxLinearSpeed) st.v = mulSV(maxLinearSpeed / sqrtf(v2), st.v);
+}
+
+// b2Body_ApplyForce, S/body.c:900-916
+void f2d_body_apply_force(int32_t index1, uint32_t wg, float fx, float fy, float px, float py, int wake) {
+  int id;
+  HostWorld* hw = awakeTarget(index1, wg, wake, &id);
+  if (!hw) return;
+  BodySim& sim = hw->sims[id];
+  sim.force = add(sim.force, {fx, fy});
+  sim.torque += cross(sub({px, py}, sim.center), {fx, fy});
+}
+
+// b2Body_ApplyTorque, S/body.c:935-950
+void f2d_body_apply_torque(int32_t index1, uint32_t wg, float torque, int wake) {
+  int id;
+  HostWorld* hw = awakeTarget(index1, wg, wake, &id);
+  if (!hw) return;
+  hw->sims[id].torque += torque;
+}
+
+// b2Body_ApplyLinearImpulse, S/body.c:952-973
+void f2d_body_apply_linear_impulse(int32_t index1, uint32_t wg, float ix, float iy, float px, float py, int wake) {
+  int id;
+  HostWorld* hw = awakeTarget(index1, wg, wake, &id);
+  if (!hw) return;
+  const BodySim& sim = hw->sims[id];
+  BodyState& st = hw->states[id];
+  st.v = mulAdd(st.v, sim.invMass, {ix, iy});
+  st.w += sim.invI * cross(sub({px, py}, sim.center), {ix, iy});
+  limitVelocity(st, hw->hdr.maxLinearSpeed);
+}
+
+// b2Body_ApplyAngularImpulse, S/body.c:997-1020
+void f2d_body_apply_angular_impulse(int32_t index1, uint32_t wg, float impulse, int wake) {
+  int id;
+  HostWorld* hw = awakeTarget(index1, wg, wake, &id);
+  if (!hw) return;
+  hw->states[id].w += hw->sims[id].invI * impulse;
+}
+
+// b2Body_SetMassData, S/body.c:1357-1381
+void f2d_body_set_mass_data(int32_t index1, uint32_t wg, float mass, float rotational_inertia, float cx, float cy) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  BodySim& sim = hw->sims[id];
+  hw->bodyMass[id] = mass;
+  hw->bodyInertia[id] = rotational_inertia;
+  sim.localCenter = {cx, cy};
+  const Vec2 center = transformPoint(sim.xf, sim.localCenter);
+  sim.center = center;
+  sim.center0 = center;
+  sim.invMass = mass > 0.0f ? 1.0f / mass : 0.0f;
+  sim.invI = rotational_inertia > 0.0f ? 1.0f / rotational_inertia : 0.0f;
+  bodyBatch(wg)->hostDirty = true;
+}
+
+// b2Body_ApplyMassFromShapes, S/body.c:1393-1403 (b2UpdateBodyMassData)
+void f2d_body_apply_mass_from_shapes(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  hw->updateBodyMass(id);
+  bodyBatch(wg)->hostDirty = true;
+}
+
+// b2Body_GetWorldPoint / GetLocalPoint, S/body.c:656-670
+void f2d_body_get_world_point(int32_t index1, uint32_t wg, float x, float y, float* out) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  const Vec2 p = transformPoint(hw->sims[id].xf, {x, y});
+  out[0] = p.x;
+  out[1] = p.y;
+}
+
+void f2d_body_get_local_point(int32_t index1, uint32_t wg, float x, float y, float* out) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  const Vec2 p = invTransformPoint(hw->sims[id].xf, {x, y});
+  out[0] = p.x;
+  out[1] = p.y;
+}
+
+// b2Body_GetShapeCount / GetShapes, S/body.c:1797-1827: the body's shape list, newest first
+int f2d_body_get_shape_count(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return 0;
+  int n = 0;
+  for (int s = hw->sims[id].headShape; s >= 0 && n <= (int)hw->shapes.size(); s = hw->shapes[s].next) ++n;
+  return n;
+}
+
+int f2d_body_get_shapes(int32_t index1, uint32_t wg, int32_t* out, int capacity) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return 0;
+  int n = 0;
+  for (int s = hw->sims[id].headShape; s >= 0 && n < capacity; s = hw->shapes[s].next) {
+    out[2 * n] = s + 1;
+    out[2 * n + 1] = (int32_t)((wg & 0xFFFF) | (1u << 16));
+    ++n;
+  }
+  return n;
+}
+
+// b2Body_GetJointCount / GetJoints, S/body.c:1829-1856: the body's joint list by edge keys
+int f2d_body_get_joint_count(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  return hw ? hw->sims[id].jointCount : 0;
+}
+
+int f2d_body_get_joints(int32_t index1, uint32_t wg, int32_t* out, int capacity) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return 0;
+  int n = 0;
+  for (int key = hw->sims[id].headJointKey; key >= 0 && n < capacity;) {
+    const int j = key >> 1;
+    out[2 * n] = j + 1;
+    out[2 * n + 1] = (int32_t)((wg & 0xFFFF) | (1u << 16));
+    ++n;
+    key = (key & 1) ? hw->joints[j].nextB : hw->joints[j].nextA;
+  }
+  return n;
+}
+
+// b2Body_GetName / SetName, S/body.c (31 characters kept)
+const char* f2d_body_get_name(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return "";
+  if ((int)hw->bodyNames.size() <= id) hw->bodyNames.resize(hw->sims.size());
+  return hw->bodyNames[id].c_str();
+}
+
+void f2d_body_set_name(int32_t index1, uint32_t wg, const char* name) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  if ((int)hw->bodyNames.size() <= id) hw->bodyNames.resize(hw->sims.size());
+  hw->bodyNames[id] = name ? std::string(name).substr(0, 31) : std::string();
+}
+
+// Scalar body accessors of the shim (S/body.c): gravity scale, damping, bullet, fixed
+// rotation; every body here is enabled (disabled bodies are not supported)
+int f2d_body_is_enabled(int32_t index1, uint32_t wg) {
+  int id;
+  return bodyWorld(index1, wg, &id) ? 1 : 0;
+}
+
+float f2d_body_get_gravity_scale(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  return hw ? hw->sims[id].gravityScale : 0.0f;
+}
+
+void f2d_body_set_gravity_scale(int32_t index1, uint32_t wg, float v) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  hw->sims[id].gravityScale = v;
+  bodyBatch(wg)->hostDirty = true;
+}
+
+float f2d_body_get_linear_damping(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  return hw ? hw->sims[id].linearDamping : 0.0f;
+}
+
+void f2d_body_set_linear_damping(int32_t index1, uint32_t wg, float v) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  hw->sims[id].linearDamping = v;
+  bodyBatch(wg)->hostDirty = true;
+}
+
+float f2d_body_get_angular_damping(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  return hw ? hw->sims[id].angularDamping : 0.0f;
+}
+
+void f2d_body_set_angular_damping(int32_t index1, uint32_t wg, float v) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  hw->sims[id].angularDamping = v;
+  bodyBatch(wg)->hostDirty = true;
+}
+
+int f2d_body_is_bullet(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  return hw ? ((hw->sims[id].flags & BF_BULLET) ? 1 : 0) : 0;
+}
+
+// b2Body_SetBullet, S/body.c:1751-1762
+void f2d_body_set_bullet(int32_t index1, uint32_t wg, int flag) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw) return;
+  if (flag) {
+    hw->sims[id].flags |= BF_BULLET;
+  } else {
+    hw->sims[id].flags &= ~BF_BULLET;
+  }
+  bodyBatch(wg)->hostDirty = true;
+}
+
+int f2d_body_is_fixed_rotation(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  return hw ? hw->bodyFixedRotation[id] : 0;
+}
+
+// b2Body_SetFixedRotation, S/body.c:1722-1742: a change zeroes an awake body's angular
+// velocity and recomputes its mass data
+void f2d_body_set_fixed_rotation(int32_t index1, uint32_t wg, int flag) {
+  int id;
+  HostWorld* hw = bodyWorld(index1, wg, &id);
+  if (!hw || (hw->bodyFixedRotation[id] != 0) == (flag != 0)) return;
+  hw->bodyFixedRotation[id] = flag ? 1 : 0;
+  if (flag) {
+    hw->sims[id].flags |= BF_FIXED_ROTATION;
+  } else {
+    hw->sims[id].flags &= ~BF_FIXED_ROTATION;
+  }
+  if (hw->sims[id].setIndex == kAwakeSet) hw->states[id].w = 0.0f;
+  hw->updateBodyMass(id);
+  bodyBatch(wg)->hostDirty = true;
+}
+
+// ---- shape accessors (S/shape.c) ---------------------------------------------
+
+static HostWorld* shapeWorld(int32_t index1, uint32_t wg, int* shapeId) {
+  uint32_t world0 = wg & 0xFFFF;
+  if (world0 >= g_worlds.size() || (wg >> 16) != 1u) return nullptr;
+  WorldSlot& s = g_worlds[world0];
+  if (!s.batch) return nullptr;
+  if (s.batch->deviceNewer && download(*s.batch) != 0) return nullptr;
+  HostWorld& hw = s.batch->worlds[0];
+  const int id = index1 - 1;
+  if (id < 0 || id >= (int)hw.shapes.size()) return nullptr;
+  *shapeId = id;
+  return &hw;
+}
+
+int f2d_shape_is_valid(int32_t index1, uint32_t wg) {
+  int id;
+  return shapeWorld(index1, wg, &id) ? 1 : 0;
+}
+
+void f2d_shape_get_body(int32_t index1, uint32_t wg, int32_t* out) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  if (!hw) return;
+  const int b = hw->shapes[id].body;
+  out[0] = b + 1;
+  out[1] = (int32_t)((wg & 0xFFFF) | ((uint32_t)hw->bodyGen[b] << 16));
+}
+
+// shape type in the reference's b2ShapeType numbering (circle 0, capsule 1, segment 2, polygon 3)
+int f2d_shape_get_type(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  return hw ? hw->geoms[id].type : -1;
+}
+
+int f2d_shape_is_sensor(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  return hw ? ((hw->shapes[id].flags & SF_SENSOR) ? 1 : 0) : 0;
+}
+
+float f2d_shape_get_density(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  return hw ? hw->shapeDensity[id] : 0.0f;
+}
+
+// b2Shape_SetDensity, S/shape.c:1055-1086
+void f2d_shape_set_density(int32_t index1, uint32_t wg, float density, int update_body_mass) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  if (!hw || density == hw->shapeDensity[id]) return;
+  hw->shapeDensity[id] = density;
+  if (update_body_mass) hw->updateBodyMass(hw->shapes[id].body);
+  g_worlds[wg & 0xFFFF].batch->hostDirty = true;
+}
+
+float f2d_shape_get_friction(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  return hw ? hw->shapes[id].friction : 0.0f;
+}
+
+// b2Shape_SetFriction / SetRestitution, S/shape.c:1088-1130: touching contacts take the
+// new values at their next update
+void f2d_shape_set_friction(int32_t index1, uint32_t wg, float v) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  if (!hw) return;
+  hw->shapes[id].friction = v;
+  g_worlds[wg & 0xFFFF].batch->hostDirty = true;
+}
+
+float f2d_shape_get_restitution(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  return hw ? hw->shapes[id].restitution : 0.0f;
+}
+
+void f2d_shape_set_restitution(int32_t index1, uint32_t wg, float v) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  if (!hw) return;
+  hw->shapes[id].restitution = v;
+  g_worlds[wg & 0xFFFF].batch->hostDirty = true;
+}
+
+// b2Shape_GetFilter: [category lo, hi, mask lo, hi, group]
+void f2d_shape_get_filter(int32_t index1, uint32_t wg, int32_t* out) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  if (!hw) return;
+  const Shape& s = hw->shapes[id];
+  out[0] = (int32_t)s.catLo;
+  out[1] = (int32_t)s.catHi;
+  out[2] = (int32_t)s.maskLo;
+  out[3] = (int32_t)s.maskHi;
+  out[4] = s.group;
+}
+
+// event flags of the shape (S/shape.c:1282-1340); sensor events are not kept
+#define F2D_SHAPE_FLAG(getter, setter, bit)                                   \
+  int getter(int32_t index1, uint32_t wg) {                                   \
+    int id;                                                                   \
+    HostWorld* hw = shapeWorld(index1, wg, &id);                              \
+    return hw ? ((hw->shapes[id].flags & (bit)) ? 1 : 0) : 0;                 \
+  }                                                                           \
+  void setter(int32_t index1, uint32_t wg, int flag) {                        \
+    int id;                                                                   \
+    HostWorld* hw = shapeWorld(index1, wg, &id);                              \
+    if (!hw) return;                                                          \
+    if (flag) {                                                               \
+      hw->shapes[id].flags |= (bit);                                          \
+    } else {                                                                  \
+      hw->shapes[id].flags &= ~(bit);                                         \
+    }                                                                         \
+    g_worlds[wg & 0xFFFF].batch->hostDirty = true;                            \
+  }
+F2D_SHAPE_FLAG(f2d_shape_are_contact_events_enabled, f2d_shape_enable_contact_events, SF_CONTACT_EVENTS)
+F2D_SHAPE_FLAG(f2d_shape_are_hit_events_enabled, f2d_shape_enable_hit_events, SF_HIT_EVENTS)
+F2D_SHAPE_FLAG(f2d_shape_are_pre_solve_events_enabled, f2d_shape_enable_pre_solve_events, SF_PRESOLVE)
+#undef F2D_SHAPE_FLAG
+
+// b2Shape_TestPoint, S/shape.c:979-1000 with b2PointInCircle / Capsule / Polygon,
+// S/geometry.c:456-501
+int f2d_shape_test_point(int32_t index1, uint32_t wg, float x, float y) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  if (!hw) return 0;
+  const ShapeGeom& g = hw->geoms[id];
+  const Vec2 p = invTransformPoint(hw->sims[hw->shapes[id].body].xf, {x, y});
+  if (g.type == kCircle) return distanceSq(p, g.v[0]) <= g.radius * g.radius ? 1 : 0;
+  if (g.type == kCapsule) {
+    const float rr = g.radius * g.radius;
+    const Vec2 p1 = g.v[0], p2 = g.v[1];
+    const Vec2 d = sub(p2, p1);
+    const float dd = dot(d, d);
+    if (dd == 0.0f) return distanceSq(p, p1) <= rr ? 1 : 0;
+    float t = dot(sub(p, p1), d) / dd;
+    t = clampf(t, 0.0f, 1.0f);
+    const Vec2 c = mulAdd(p1, t, d);
+    return distanceSq(p, c) <= rr ? 1 : 0;
+  }
+  if (g.type == kPolygon) {
+    Proxy pa;
+    pa.count = g.count;
+    for (int i = 0; i < g.count; ++i) pa.p[i] = g.v[i];
+    pa.radius = 0.0f;
+    Proxy pb;
+    pb.count = 1;
+    pb.p[0] = p;
+    pb.radius = 0.0f;
+    SimplexCache cache;
+    cache.count = 0;
+    const Xf identity = {{0.0f, 0.0f}, {1.0f, 0.0f}};
+    const DistanceOut o = shapeDistanceOut(cache, pa, pb, identity, identity, false);
+    return o.distance <= g.radius ? 1 : 0;
+  }
+  return 0;
+}
+
+void f2d_shape_get_circle(int32_t index1, uint32_t wg, float* out) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  if (!hw) return;
+  out[0] = hw->geoms[id].v[0].x;
+  out[1] = hw->geoms[id].v[0].y;
+  out[2] = hw->geoms[id].radius;
+}
+
+void f2d_shape_get_capsule(int32_t index1, uint32_t wg, float* out) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  if (!hw) return;
+  const ShapeGeom& g = hw->geoms[id];
+  out[0] = g.v[0].x;
+  out[1] = g.v[0].y;
+  out[2] = g.v[1].x;
+  out[3] = g.v[1].y;
+  out[4] = g.radius;
+}
+
+void f2d_shape_get_segment(int32_t index1, uint32_t wg, float* out) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  if (!hw) return;
+  const ShapeGeom& g = hw->geoms[id];
+  out[0] = g.v[0].x;
+  out[1] = g.v[0].y;
+  out[2] = g.v[1].x;
+  out[3] = g.v[1].y;
+}
+
+// the radius, then the vertex pairs; returns the vertex count
+int f2d_shape_get_polygon(int32_t index1, uint32_t wg, float* out) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  if (!hw) return 0;
+  const ShapeGeom& g = hw->geoms[id];
+  out[0] = g.radius;
+  for (int i = 0; i < g.count; ++i) {
+    out[1 + 2 * i] = g.v[i].x;
+    out[2 + 2 * i] = g.v[i].y;
+  }
+  return g.count;
+}
+
+// b2Shape_GetAABB: the shape's box as the last step (or call) left it
+void f2d_shape_get_aabb(int32_t index1, uint32_t wg, float* out) {
+  int id;
+  HostWorld* hw = shapeWorld(index1, wg, &id);
+  if (!hw) return;
+  const AABB a = hw->shapes[id].aabb;
+  out[0] = a.lo.x;
+  out[1] = a.lo.y;
+  out[2] = a.hi.x;
+  out[3] = a.hi.y;
+}
+
+// ---- joint accessors (S/joint.c, S/revolute_joint.c) -------------------------
+
+static HostWorld* jointWorld(int32_t index1, uint32_t wg, int* jointId) {
+  uint32_t world0 = wg & 0xFFFF;
+  if (world0 >= g_worlds.size() || (wg >> 16) != 1u) return nullptr;
+  WorldSlot& s = g_worlds[world0];
+  if (!s.batch) return nullptr;
+  if (s.batch->deviceNewer && download(*s.batch) != 0) return nullptr;
+  HostWorld& hw = s.batch->worlds[0];
+  const int id = index1 - 1;
+  if (id < 0 || id >= (int)hw.joints.size()) return nullptr;
+  *jointId = id;
+  return &hw;
+}
+
+int f2d_joint_is_valid(int32_t index1, uint32_t wg) {
+  int id;
+  return jointWorld(index1, wg, &id) ? 1 : 0;
+}
+
+// b2JointType numbering (I/types.h): every joint here is revolute (5)
+int f2d_joint_get_type(int32_t index1, uint32_t wg) {
+  int id;
+  return jointWorld(index1, wg, &id) ? 5 : -1;
+}
+
+static void jointBody(const HostWorld& hw, int body, uint32_t wg, int32_t* out) {
+  out[0] = body + 1;
+  out[1] = (int32_t)((wg & 0xFFFF) | ((uint32_t)hw.bodyGen[body] << 16));
+}
+
+void f2d_joint_get_body_a(int32_t index1, uint32_t wg, int32_t* out) {
+  int id;
+  HostWorld* hw = jointWorld(index1, wg, &id);
+  if (hw) jointBody(*hw, hw->joints[id].bodyA, wg, out);
+}
+
+void f2d_joint_get_body_b(int32_t index1, uint32_t wg, int32_t* out) {
+  int id;
+  HostWorld* hw = jointWorld(index1, wg, &id);
+  if (hw) jointBody(*hw, hw->joints[id].bodyB, wg, out);
+}
+
+// b2Joint_GetLocalAnchorA / B: the anchors in the bodies' origin frames
+void f2d_joint_get_local_anchor_a(int32_t index1, uint32_t wg, float* out) {
+  int id;
+  HostWorld* hw = jointWorld(index1, wg, &id);
+  if (!hw) return;
+  out[0] = hw->joints[id].localOriginAnchorA.x;
+  out[1] = hw->joints[id].localOriginAnchorA.y;
+}
+
+void f2d_joint_get_local_anchor_b(int32_t index1, uint32_t wg, float* out) {
+  int id;
+  HostWorld* hw = jointWorld(index1, wg, &id);
+  if (!hw) return;
+  out[0] = hw->joints[id].localOriginAnchorB.x;
+  out[1] = hw->joints[id].localOriginAnchorB.y;
+}
+
+int f2d_joint_get_collide_connected(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = jointWorld(index1, wg, &id);
+  return hw ? hw->joints[id].collideConnected : 0;
+}
+
+// b2Joint_WakeBodies, S/joint.c: both bodies' sleeping sets wake
+void f2d_joint_wake_bodies(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = jointWorld(index1, wg, &id);
+  if (!hw) return;
+  const int a = hw->joints[id].bodyA, b = hw->joints[id].bodyB;
+  const uint32_t bwg = (wg & 0xFFFF);
+  wakeBody(bwg | ((uint32_t)hw->bodyGen[a] << 16), a);
+  hw = jointWorld(index1, wg, &id);
+  if (hw) wakeBody(bwg | ((uint32_t)hw->bodyGen[b] << 16), b);
+}
+
+// b2GetRevoluteJointForce / Torque, S/revolute_joint.c:176-187 (world inv_h of the last step)
+void f2d_joint_get_constraint_force(int32_t index1, uint32_t wg, float* out) {
+  int id;
+  HostWorld* hw = jointWorld(index1, wg, &id);
+  if (!hw) return;
+  const Vec2 f = mulSV(hw->inv_h, hw->joints[id].linearImpulse);
+  out[0] = f.x;
+  out[1] = f.y;
+}
+
+float f2d_joint_get_constraint_torque(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = jointWorld(index1, wg, &id);
+  if (!hw) return 0.0f;
+  const Joint& j = hw->joints[id];
+  return hw->inv_h * (j.motorImpulse + j.lowerImpulse - j.upperImpulse);
 }
 
 float f2d_body_get_mass(int32_t index1, uint32_t wg) {

@@ -28,6 +28,7 @@ struct HostWorld {
   std::vector<float> bodyMass, bodyInertia, shapeDensity;
   std::vector<uint16_t> bodyGen;
   std::vector<int> bodyFixedRotation;
+  std::vector<std::string> bodyNames;  // b2Body name (31 characters), set through the flat ABI
   // broad-phase trees (static, kinematic, dynamic): headers, nodes, freed-id stacks
   TreeHeader tree[3] = {{-1, 0, 0, 0, 0, 0, 0, 0}, {-1, 0, 0, 0, 0, 0, 0, 0}, {-1, 0, 0, 0, 0, 0, 0, 0}};
   std::vector<TreeNode> tnodes[3];

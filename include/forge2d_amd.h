@@ -117,6 +117,50 @@ void f2d_body_set_sleep_threshold(int32_t index1, uint32_t world_and_generation,
 float f2d_body_get_mass(int32_t index1, uint32_t world_and_generation);
 float f2d_body_get_rotational_inertia(int32_t index1, uint32_t world_and_generation);
 int f2d_body_get_type(int32_t index1, uint32_t world_and_generation);
+/* F/native/wasm/f2d_shim.c:240-243 (b2Body_SetTransform, S/body.c:688-735): the shapes'
+ * boxes follow; a proxy whose fat box no longer holds its box moves in its tree */
+void f2d_body_set_transform(int32_t index1, uint32_t world_and_generation, float px, float py, float qc, float qs);
+/* F/native/wasm/f2d_shim.c:381 (b2Body_SetAngularVelocity, S/body.c:791-812) */
+void f2d_body_set_angular_velocity(int32_t index1, uint32_t world_and_generation, float w);
+/* F/native/wasm/f2d_shim.c:257-289 (b2Body_ApplyForce / ApplyTorque / ApplyLinearImpulse /
+ * ApplyAngularImpulse, S/body.c:900-1020): wake on request, awake bodies only; a linear
+ * impulse is followed by the world's speed limit (b2LimitVelocity) */
+void f2d_body_apply_force(int32_t index1, uint32_t world_and_generation, float fx, float fy, float px, float py,
+                          int wake);
+void f2d_body_apply_torque(int32_t index1, uint32_t world_and_generation, float torque, int wake);
+void f2d_body_apply_linear_impulse(int32_t index1, uint32_t world_and_generation, float ix, float iy, float px,
+                                   float py, int wake);
+void f2d_body_apply_angular_impulse(int32_t index1, uint32_t world_and_generation, float impulse, int wake);
+/* F/native/wasm/f2d_shim.c:305-310 (b2Body_SetMassData, S/body.c:1357-1381) and
+ * F2D_BODY_CALL b2Body_ApplyMassFromShapes (S/body.c:1393-1403) */
+void f2d_body_set_mass_data(int32_t index1, uint32_t world_and_generation, float mass, float rotational_inertia,
+                            float cx, float cy);
+void f2d_body_apply_mass_from_shapes(int32_t index1, uint32_t world_and_generation);
+/* F/native/wasm/f2d_shim.c:320-332 (b2Body_GetWorldPoint / GetLocalPoint) */
+void f2d_body_get_world_point(int32_t index1, uint32_t world_and_generation, float x, float y, float* out);
+void f2d_body_get_local_point(int32_t index1, uint32_t world_and_generation, float x, float y, float* out);
+/* F/native/wasm/f2d_shim.c:334-365 (b2Body_GetShapeCount / GetShapes / GetJointCount /
+ * GetJoints): out = [index1, world_and_generation] per id, list order */
+int f2d_body_get_shape_count(int32_t index1, uint32_t world_and_generation);
+int f2d_body_get_shapes(int32_t index1, uint32_t world_and_generation, int32_t* out, int capacity);
+int f2d_body_get_joint_count(int32_t index1, uint32_t world_and_generation);
+int f2d_body_get_joints(int32_t index1, uint32_t world_and_generation, int32_t* out, int capacity);
+/* F/native/wasm/f2d_shim.c:312-318 (b2Body_GetName / SetName, 31 characters) */
+const char* f2d_body_get_name(int32_t index1, uint32_t world_and_generation);
+void f2d_body_set_name(int32_t index1, uint32_t world_and_generation, const char* name);
+/* F/native/wasm/f2d_shim.c:395-407 (F2D_BODY_* accessors: enabled, gravity scale, damping,
+ * bullet, fixed rotation; S/body.c:1405-1762). Every body here is enabled. */
+int f2d_body_is_enabled(int32_t index1, uint32_t world_and_generation);
+float f2d_body_get_gravity_scale(int32_t index1, uint32_t world_and_generation);
+void f2d_body_set_gravity_scale(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_body_get_linear_damping(int32_t index1, uint32_t world_and_generation);
+void f2d_body_set_linear_damping(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_body_get_angular_damping(int32_t index1, uint32_t world_and_generation);
+void f2d_body_set_angular_damping(int32_t index1, uint32_t world_and_generation, float v);
+int f2d_body_is_bullet(int32_t index1, uint32_t world_and_generation);
+void f2d_body_set_bullet(int32_t index1, uint32_t world_and_generation, int flag);
+int f2d_body_is_fixed_rotation(int32_t index1, uint32_t world_and_generation);
+void f2d_body_set_fixed_rotation(int32_t index1, uint32_t world_and_generation, int flag);
 
 /* F/native/wasm/f2d_shim.c:459-466 (b2CreateCircleShape) */
 void f2d_create_circle_shape(int32_t index1, uint32_t world_and_generation, const f2dShapeDef* def, float cx,
@@ -133,6 +177,34 @@ void f2d_create_box_shape(int32_t index1, uint32_t world_and_generation, const f
 /* F/native/wasm/f2d_shim.c:502-515 (b2ComputeHull + b2MakePolygon + b2CreatePolygonShape); 0 = degenerate */
 int f2d_create_polygon_shape(int32_t index1, uint32_t world_and_generation, const f2dShapeDef* def,
                              const float* points, int count, float radius, int32_t* out);
+/* F/native/wasm/f2d_shim.c:522-617 (shape accessors, S/shape.c:979-1340): body, type (b2ShapeType),
+ * sensor, density (set: mass update on request), friction, restitution, filter
+ * [category lo, hi, mask lo, hi, group], event flags, point test, geometry (polygon: radius,
+ * then vertex pairs; returns the count), the shape's box */
+int f2d_shape_is_valid(int32_t index1, uint32_t world_and_generation);
+void f2d_shape_get_body(int32_t index1, uint32_t world_and_generation, int32_t* out);
+int f2d_shape_get_type(int32_t index1, uint32_t world_and_generation);
+int f2d_shape_is_sensor(int32_t index1, uint32_t world_and_generation);
+float f2d_shape_get_density(int32_t index1, uint32_t world_and_generation);
+void f2d_shape_set_density(int32_t index1, uint32_t world_and_generation, float density, int update_body_mass);
+float f2d_shape_get_friction(int32_t index1, uint32_t world_and_generation);
+void f2d_shape_set_friction(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_shape_get_restitution(int32_t index1, uint32_t world_and_generation);
+void f2d_shape_set_restitution(int32_t index1, uint32_t world_and_generation, float v);
+void f2d_shape_get_filter(int32_t index1, uint32_t world_and_generation, int32_t* out);
+int f2d_shape_are_contact_events_enabled(int32_t index1, uint32_t world_and_generation);
+void f2d_shape_enable_contact_events(int32_t index1, uint32_t world_and_generation, int flag);
+int f2d_shape_are_hit_events_enabled(int32_t index1, uint32_t world_and_generation);
+void f2d_shape_enable_hit_events(int32_t index1, uint32_t world_and_generation, int flag);
+int f2d_shape_are_pre_solve_events_enabled(int32_t index1, uint32_t world_and_generation);
+void f2d_shape_enable_pre_solve_events(int32_t index1, uint32_t world_and_generation, int flag);
+int f2d_shape_test_point(int32_t index1, uint32_t world_and_generation, float x, float y);
+void f2d_shape_get_circle(int32_t index1, uint32_t world_and_generation, float* out);
+void f2d_shape_get_capsule(int32_t index1, uint32_t world_and_generation, float* out);
+void f2d_shape_get_segment(int32_t index1, uint32_t world_and_generation, float* out);
+int f2d_shape_get_polygon(int32_t index1, uint32_t world_and_generation, float* out);
+void f2d_shape_get_aabb(int32_t index1, uint32_t world_and_generation, float* out);
+
 /* F/native/wasm/f2d_shim.c:816-843 (b2CreateRevoluteJoint) */
 void f2d_create_revolute_joint(uint32_t w, int32_t a_index1, uint32_t a_world_and_generation, int32_t b_index1,
                                uint32_t b_world_and_generation, float anchor_ax, float anchor_ay, float anchor_bx,
@@ -140,6 +212,20 @@ void f2d_create_revolute_joint(uint32_t w, int32_t a_index1, uint32_t a_world_an
                                float hertz, float damping_ratio, int enable_limit, float lower_angle,
                                float upper_angle, int enable_motor, float max_motor_torque, float motor_speed,
                                float draw_size, int collide_connected, int32_t* out);
+
+/* F/native/wasm/f2d_shim.c:892-950 (joint accessors, S/joint.c; revolute joints only: type 5
+ * = b2_revoluteJoint): bodies, local anchors, collideConnected, waking both bodies, and the
+ * last step's constraint force and torque (S/revolute_joint.c:176-187) */
+int f2d_joint_is_valid(int32_t index1, uint32_t world_and_generation);
+int f2d_joint_get_type(int32_t index1, uint32_t world_and_generation);
+void f2d_joint_get_body_a(int32_t index1, uint32_t world_and_generation, int32_t* out);
+void f2d_joint_get_body_b(int32_t index1, uint32_t world_and_generation, int32_t* out);
+void f2d_joint_get_local_anchor_a(int32_t index1, uint32_t world_and_generation, float* out);
+void f2d_joint_get_local_anchor_b(int32_t index1, uint32_t world_and_generation, float* out);
+int f2d_joint_get_collide_connected(int32_t index1, uint32_t world_and_generation);
+void f2d_joint_wake_bodies(int32_t index1, uint32_t world_and_generation);
+void f2d_joint_get_constraint_force(int32_t index1, uint32_t world_and_generation, float* out);
+float f2d_joint_get_constraint_torque(int32_t index1, uint32_t world_and_generation);
 
 /* Events of the last step, F/native/wasm/f2d_shim.c:1186-1300 (counts first, then the
  * fill functions write float64 records and return the record count):

@@ -185,6 +185,9 @@ static Scene load_scene( const char* path )
 
 static pthread_mutex_t g_world_lock = PTHREAD_MUTEX_INITIALIZER;
 
+static b2ShapeId* g_shapeIds; // when set, build_world records the shapes' ids here
+static b2JointId* g_jointIds; // and the joints' ids here
+
 static b2WorldId build_world( const SceneWorld* sw, b2BodyId* bodyIds )
 {
 	b2WorldDef def = b2DefaultWorldDef();
@@ -246,19 +249,22 @@ static b2WorldId build_world( const SceneWorld* sw, b2BodyId* bodyIds )
 			case 0:
 			{
 				b2Circle c = { { d[0], d[1] }, ss->radius };
-				b2CreateCircleShape( body, &sd, &c );
+				b2ShapeId sid = b2CreateCircleShape( body, &sd, &c );
+				if ( g_shapeIds ) g_shapeIds[i] = sid;
 			}
 			break;
 			case 1:
 			{
 				b2Capsule c = { { d[0], d[1] }, { d[2], d[3] }, ss->radius };
-				b2CreateCapsuleShape( body, &sd, &c );
+				b2ShapeId sid = b2CreateCapsuleShape( body, &sd, &c );
+				if ( g_shapeIds ) g_shapeIds[i] = sid;
 			}
 			break;
 			case 2:
 			{
 				b2Segment s = { { d[0], d[1] }, { d[2], d[3] } };
-				b2CreateSegmentShape( body, &sd, &s );
+				b2ShapeId sid = b2CreateSegmentShape( body, &sd, &s );
+				if ( g_shapeIds ) g_shapeIds[i] = sid;
 			}
 			break;
 			case 3:
@@ -270,14 +276,16 @@ static b2WorldId build_world( const SceneWorld* sw, b2BodyId* bodyIds )
 					exit( 3 );
 				}
 				b2Polygon p = b2MakePolygon( &hull, ss->radius );
-				b2CreatePolygonShape( body, &sd, &p );
+				b2ShapeId sid = b2CreatePolygonShape( body, &sd, &p );
+				if ( g_shapeIds ) g_shapeIds[i] = sid;
 			}
 			break;
 			case 4:
 			{
 				b2Polygon p = b2MakeOffsetRoundedBox( d[0], d[1], ( b2Vec2 ){ d[2], d[3] }, ( b2Rot ){ d[4], d[5] },
 													  ss->radius );
-				b2CreatePolygonShape( body, &sd, &p );
+				b2ShapeId sid = b2CreatePolygonShape( body, &sd, &p );
+				if ( g_shapeIds ) g_shapeIds[i] = sid;
 			}
 			break;
 			default:
@@ -305,7 +313,8 @@ static b2WorldId build_world( const SceneWorld* sw, b2BodyId* bodyIds )
 		jd.dampingRatio = sj->damping;
 		jd.collideConnected = sj->collide_connected != 0;
 		jd.drawSize = sj->draw_size;
-		b2CreateRevoluteJoint( b2Body_GetWorld( jd.bodyIdA ), &jd );
+		b2JointId jid = b2CreateRevoluteJoint( b2Body_GetWorld( jd.bodyIdA ), &jd );
+		if ( g_jointIds ) g_jointIds[i] = jid;
 	}
 	return wid;
 }
@@ -785,6 +794,112 @@ static void* qtime_worker( void* arg )
 	return NULL;
 }
 
+// ---- API call fixtures ("ops" mode): one body, shape or joint call of the reference's API
+// per record, applied before step `step` (tests/golden/make_golden.py OPS)
+typedef struct
+{
+	int32_t step, op, target, flag;
+	float f[6];
+} OpRec; // 40 bytes
+
+static uint32_t fbits( float v )
+{
+	uint32_t u;
+	memcpy( &u, &v, 4 );
+	return u;
+}
+
+static void apply_op( const OpRec* o, const b2BodyId* bodies, const b2ShapeId* shapes, const b2JointId* joints )
+{
+	const float* f = o->f;
+	b2BodyId b = bodies[o->target];
+	switch ( o->op )
+	{
+		case 1:
+			b2Body_ApplyForce( b, ( b2Vec2 ){ f[0], f[1] }, ( b2Vec2 ){ f[2], f[3] }, o->flag != 0 );
+			break;
+		case 2:
+			b2Body_ApplyForceToCenter( b, ( b2Vec2 ){ f[0], f[1] }, o->flag != 0 );
+			break;
+		case 3:
+			b2Body_ApplyTorque( b, f[0], o->flag != 0 );
+			break;
+		case 4:
+			b2Body_ApplyLinearImpulse( b, ( b2Vec2 ){ f[0], f[1] }, ( b2Vec2 ){ f[2], f[3] }, o->flag != 0 );
+			break;
+		case 5:
+			b2Body_ApplyLinearImpulseToCenter( b, ( b2Vec2 ){ f[0], f[1] }, o->flag != 0 );
+			break;
+		case 6:
+			b2Body_ApplyAngularImpulse( b, f[0], o->flag != 0 );
+			break;
+		case 7:
+			b2Body_SetTransform( b, ( b2Vec2 ){ f[0], f[1] }, ( b2Rot ){ f[2], f[3] } );
+			break;
+		case 8:
+			b2Body_SetLinearVelocity( b, ( b2Vec2 ){ f[0], f[1] } );
+			break;
+		case 9:
+			b2Body_SetAngularVelocity( b, f[0] );
+			break;
+		case 10:
+		{
+			b2MassData md = { f[0], { f[2], f[3] }, f[1] };
+			b2Body_SetMassData( b, md );
+		}
+		break;
+		case 11:
+			b2Body_ApplyMassFromShapes( b );
+			break;
+		case 12:
+			b2Shape_SetDensity( shapes[o->target], f[0], o->flag != 0 );
+			break;
+		case 13:
+			b2Shape_SetFriction( shapes[o->target], f[0] );
+			break;
+		case 14:
+			b2Shape_SetRestitution( shapes[o->target], f[0] );
+			break;
+		case 15:
+			b2Body_SetGravityScale( b, f[0] );
+			break;
+		case 16:
+			b2Body_SetLinearDamping( b, f[0] );
+			break;
+		case 17:
+			b2Body_SetAngularDamping( b, f[0] );
+			break;
+		case 18:
+			b2Body_SetFixedRotation( b, o->flag != 0 );
+			break;
+		case 19:
+			b2Body_SetBullet( b, o->flag != 0 );
+			break;
+		case 20:
+			b2Body_SetAwake( b, o->flag != 0 );
+			break;
+		case 21:
+		{
+			b2Filter filter = { (uint64_t)fbits( f[0] ) | ( (uint64_t)fbits( f[1] ) << 32 ),
+								(uint64_t)fbits( f[2] ) | ( (uint64_t)fbits( f[3] ) << 32 ), (int)fbits( f[4] ) };
+			b2Shape_SetFilter( shapes[o->target], filter );
+		}
+		break;
+		case 22:
+			b2DestroyJoint( joints[o->target] );
+			break;
+		case 23:
+			b2Joint_WakeBodies( joints[o->target] );
+			break;
+		case 24:
+			b2Joint_SetCollideConnected( joints[o->target], o->flag != 0 );
+			break;
+		default:
+			fprintf( stderr, "bad op %d\n", o->op );
+			exit( 3 );
+	}
+}
+
 int main( int argc, char** argv )
 {
 	if ( argc < 3 )
@@ -814,6 +929,58 @@ int main( int argc, char** argv )
 		for ( int s = 1; s <= steps; ++s )
 		{
 			b2World_Step( wid, 1.0f / 60.0f, substeps );
+			while ( di < nd && dumps[di] == s )
+			{
+				write_snapshot( f, wid, s );
+				di++;
+			}
+		}
+		fclose( f );
+		b2DestroyWorld( wid );
+		return 0;
+	}
+	if ( strcmp( argv[1], "ops" ) == 0 && argc >= 8 )
+	{
+		// ref_harness ops <scene> <ops> <out> <world> <steps> <dumps>: the world stepped
+		// like "dump", with body, shape and joint API calls applied before the steps
+		// the ops file names (tests/golden/make_golden.py writes it)
+		Scene sc = load_scene( argv[2] );
+		FILE* of = fopen( argv[3], "rb" );
+		if ( !of )
+			return 2;
+		fseek( of, 0, SEEK_END );
+		long ob = ftell( of );
+		fseek( of, 0, SEEK_SET );
+		int nops = (int)( ob / sizeof( OpRec ) );
+		OpRec* ops = malloc( sizeof( OpRec ) * ( nops + 1 ) );
+		if ( fread( ops, sizeof( OpRec ), nops, of ) != (size_t)nops )
+			return 2;
+		fclose( of );
+		const char* out = argv[4];
+		int wi = atoi( argv[5] );
+		int steps = atoi( argv[6] );
+		int dumps[4096];
+		int nd = parse_list( argv[7], dumps, 4096 );
+		const SceneWorld* sw = sc.worlds + wi;
+		b2BodyId* ids = malloc( sizeof( b2BodyId ) * ( sw->hdr.body_count + 1 ) );
+		g_shapeIds = malloc( sizeof( b2ShapeId ) * ( sw->hdr.shape_count + 1 ) );
+		g_jointIds = malloc( sizeof( b2JointId ) * ( sw->hdr.joint_count + 1 ) );
+		b2WorldId wid = build_world( sw, ids );
+		FILE* f = fopen( out, "wb" );
+		int di = 0, oi = 0;
+		if ( di < nd && dumps[di] == 0 )
+		{
+			write_snapshot( f, wid, 0 );
+			di++;
+		}
+		for ( int s = 1; s <= steps; ++s )
+		{
+			while ( oi < nops && ops[oi].step == s )
+			{
+				apply_op( ops + oi, ids, g_shapeIds, g_jointIds );
+				oi++;
+			}
+			b2World_Step( wid, 1.0f / 60.0f, 4 );
 			while ( di < nd && dumps[di] == s )
 			{
 				write_snapshot( f, wid, s );

@@ -279,6 +279,117 @@ def asleep_world():
     return w
 
 
+OP_DT = np.dtype([("step", "<i4"), ("op", "<i4"), ("target", "<i4"), ("flag", "<i4"), ("f", "<f4", (6,))])
+# op codes of the harness's "ops" mode (oracle/ref_harness.c apply_op) and of
+# tests/flat_util.py, which makes the same calls through the flat ABI
+OPS = {"apply_force": 1, "apply_force_to_center": 2, "apply_torque": 3, "apply_linear_impulse": 4,
+       "apply_linear_impulse_to_center": 5, "apply_angular_impulse": 6, "set_transform": 7,
+       "set_linear_velocity": 8, "set_angular_velocity": 9, "set_mass_data": 10, "apply_mass_from_shapes": 11,
+       "shape_set_density": 12, "shape_set_friction": 13, "shape_set_restitution": 14, "set_gravity_scale": 15,
+       "set_linear_damping": 16, "set_angular_damping": 17, "set_fixed_rotation": 18, "set_bullet": 19,
+       "set_awake": 20, "joint_wake_bodies": 23}
+
+
+def ops_array(rows):
+    """(step, name, target, flag, floats) rows -> OP_DT records in step order."""
+    a = np.zeros(len(rows), OP_DT)
+    for i, (step, name, target, flag, f) in enumerate(rows):
+        a[i]["step"], a[i]["op"], a[i]["target"], a[i]["flag"] = step, OPS[name], target, flag
+        a[i]["f"][:len(f)] = f
+    return a[np.argsort(a["step"], kind="stable")]
+
+
+def body_ops_world():
+    """A scene for the body, shape and joint calls of the flat ABI: boxes, circles and a
+    capsule on a ground box, a static post, a body that may not rotate, a hanging
+    revolute chain, and (sleep on) a box created asleep."""
+    w = scenes.World(scenes.world_header(sleep=True, continuous=True))
+    g = w.add_body(scenes.STATIC, 0.0, -1.0)                      # 0
+    w.add_box(g, 30.0, 1.0)                                       # shape 0
+    post = w.add_body(scenes.STATIC, 10.0, 1.0)                   # 1
+    w.add_box(post, 0.3, 1.0)                                     # 1
+    for i in range(4):                                            # bodies 2..5, shapes 2..5
+        b = w.add_body(scenes.DYNAMIC, -6.0 + 2.0 * i, 0.5 + 0.01 * i)
+        w.add_box(b, 0.5, 0.5, density=1.0 + 0.5 * i)
+    b = w.add_body(scenes.DYNAMIC, 3.0, 2.0)                      # 6, shape 6, 7
+    w.add_circle(b, 0.4)
+    w.add_capsule(b, (0.0, 0.0), (0.8, 0.0), 0.2)
+    b = w.add_body(scenes.DYNAMIC, 5.0, 0.5, flags=scenes.BF_FIXED_ROT | scenes.BF_AWAKE | scenes.BF_ENABLE_SLEEP)
+    w.add_box(b, 0.5, 0.5)                                        # 7, shape 8
+    b = w.add_body(scenes.DYNAMIC, -10.0, 3.0, flags=scenes.BF_ENABLE_SLEEP)  # 8: created asleep, shape 9
+    w.add_box(b, 0.5, 0.5)
+    anchor = w.add_body(scenes.STATIC, 15.0, 6.0)                 # 9, no shape
+    prev, pa = anchor, (0.0, 0.0)
+    for k in range(3):                                            # 10..12, shapes 10..12, joints 0..2
+        b = w.add_body(scenes.DYNAMIC, 15.0, 6.0 - 0.2 - 0.4 * k)
+        w.add_box(b, 0.05, 0.2)
+        w.add_revolute(prev, b, pa, (0.0, 0.2), enable_limit=True, lower=-0.5, upper=0.5)
+        prev, pa = b, (0.0, -0.2)
+    return w
+
+
+BODY_OPS = [
+    (3, "apply_force", 2, 1, [50.0, 20.0, -6.2, 0.9]),
+    (3, "apply_torque", 3, 1, [5.0]),
+    (4, "apply_force_to_center", 4, 0, [-30.0, 0.0]),
+    (5, "apply_linear_impulse", 5, 1, [2.0, 3.0, 0.2, 0.7]),
+    (5, "apply_angular_impulse", 6, 1, [1.5]),
+    (6, "apply_linear_impulse_to_center", 4, 1, [0.0, 1.0]),
+    (8, "set_transform", 6, 0, [0.0, 4.0, 0.8660254, 0.5]),       # moves the proxy
+    (8, "set_transform", 1, 0, [11.0, 1.0, 1.0, 0.0]),            # a static body's proxy moves
+    (9, "set_transform", 3, 0, [-4.0, 0.52, 1.0, 0.0]),           # within its fat box: no move
+    (10, "set_linear_velocity", 5, 0, [-1.0, 2.0]),
+    (10, "set_angular_velocity", 2, 0, [3.0]),
+    (10, "set_angular_velocity", 7, 0, [3.0]),                    # fixed rotation: ignored
+    (12, "set_mass_data", 4, 0, [3.0, 0.7, 0.1, -0.05]),
+    (14, "shape_set_density", 2, 1, [4.0]),
+    (14, "shape_set_friction", 3, 0, [0.1]),
+    (14, "shape_set_restitution", 4, 0, [0.5]),
+    (15, "set_gravity_scale", 5, 0, [0.4]),
+    (15, "set_linear_damping", 6, 0, [0.5]),
+    (15, "set_angular_damping", 6, 0, [0.9]),
+    (16, "set_fixed_rotation", 2, 1, []),
+    (16, "set_bullet", 6, 1, []),
+    (18, "apply_mass_from_shapes", 4, 0, []),
+    (19, "apply_force", 8, 0, [0.0, 100.0, -10.0, 3.0]),           # asleep, no wake: nothing
+    (20, "apply_linear_impulse_to_center", 8, 1, [30.0, 0.0]),     # wakes it
+    (21, "apply_linear_impulse_to_center", 5, 0, [1000.0, 0.0]),   # beyond the speed limit
+    (22, "set_fixed_rotation", 2, 0, []),
+    (24, "set_awake", 7, 0, []),
+    (26, "joint_wake_bodies", 1, 0, []),
+    (28, "set_transform", 10, 0, [15.5, 5.8, 1.0, 0.0]),
+]
+
+OPS_FIXTURES = {
+    # name: (world factory, ops, dumped steps)
+    "ops_bodies": (body_ops_world, BODY_OPS, list(range(0, 41))),
+}
+
+
+def make_ops(name: str):
+    factory, rows, steps = OPS_FIXTURES[name]
+    spec = scenes.to_bytes([factory()])
+    ops = ops_array(rows)
+    with tempfile.TemporaryDirectory() as td:
+        sp, op, dp = os.path.join(td, "scene.bin"), os.path.join(td, "ops.bin"), os.path.join(td, "snaps.bin")
+        with open(sp, "wb") as f:
+            f.write(spec)
+        with open(op, "wb") as f:
+            f.write(ops.tobytes())
+        subprocess.run([HARNESS, "ops", sp, op, dp, "0", str(max(steps)), ",".join(map(str, steps))], check=True)
+        with open(dp, "rb") as f:
+            snaps = f.read()
+        gp = os.path.join(td, "geoms.bin")
+        subprocess.run([HARNESS, "geom", sp, gp, "0"], check=True)
+        with open(gp, "rb") as f:
+            geoms = f.read()
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, spec=np.frombuffer(spec, np.uint8), snaps=np.frombuffer(snaps, np.uint8),
+                        steps=np.array(steps, np.int32), geoms=np.frombuffer(geoms, np.uint8),
+                        ops=np.frombuffer(ops.tobytes(), np.uint8))
+    print(f"{name}: {len(ops)} calls, {len(snaps)} B snapshots -> {os.path.getsize(path)} B")
+
+
 def query_world():
     """World queries (S/world.c:2050-2308, 2640-2747): every shape type on static,
     kinematic and dynamic bodies, rounded boxes (ray cast by shape cast), a body with
@@ -515,9 +626,11 @@ def make(name: str):
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
-    for n in sys.argv[1:] or list(FIXTURES) + list(QUERY_FIXTURES) + list(DIGEST_FIXTURES):
+    for n in sys.argv[1:] or list(FIXTURES) + list(QUERY_FIXTURES) + list(DIGEST_FIXTURES) + list(OPS_FIXTURES):
         if n in DIGEST_FIXTURES:
             make_digest(n)
+        elif n in OPS_FIXTURES:
+            make_ops(n)
         elif n in QUERY_FIXTURES:
             make_queries_fixture(n)
         else:
