@@ -241,18 +241,17 @@ int f2d_world_get_body_event_count(uint32_t w);
 int f2d_world_get_body_events(uint32_t w, double* out);
 
 /* Queries, F/native/wasm/f2d_shim.c:1308-1367 (rays, AABB overlap) and :177-189
- * (explosion). A device kernel tests the world's shapes against the ray or box (their
- * fat AABBs are the reference's tree leaves); hit shapes, points, normals and
- * fractions are the reference's bit for bit. The reference calls the host imports
+ * (explosion). A device kernel walks the world's trees as the reference's world queries
+ * do; hit shapes, points, normals and fractions are the reference's bit for bit and
+ * come in the reference's walk order. The reference calls the host imports
  * f2d_host_cast_ray / f2d_host_overlap (f2d_shim.c:27-31) during its tree walks; here
  * the host registers them with f2d_set_query_callbacks and they are called after the
- * device pass, tree by tree (static, kinematic, dynamic bodies' shapes) as the
- * reference walks them: ray hits nearest first within a tree with the reference's
- * clipping rules (a return of 0 ends the cast, -1 skips the hit, a fraction clips the
- * ray for the hits after it), overlaps in shape-id order (false ends that tree).
+ * device pass, in the walk order (static, kinematic, then dynamic tree), with the
+ * reference's clipping rules (a return of 0 ends the cast, -1 skips the hit, a fraction
+ * clips the ray for the hits after it; a false overlap return ends that tree).
  * f2d_world_cast_ray_closest writes [index1, world_and_generation, px, py, nx, ny,
  * fraction] and returns 1 on a hit. f2d_world_explode wakes and pushes the dynamic
- * bodies in reach in shape-id order. */
+ * bodies in reach in the dynamic tree's depth-first order. */
 typedef float (*f2dHostCastRayFn)(int32_t shape_index1, uint32_t shape_wg, float px, float py, float nx, float ny,
                                   float fraction);
 typedef int (*f2dHostOverlapFn)(int32_t shape_index1, uint32_t shape_wg);
