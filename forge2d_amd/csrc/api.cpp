@@ -639,6 +639,24 @@ static void freeDevice(Batch& b) {
   b.committed = false;
 }
 
+// Dynamic LDS bytes a step launch may ask for on the current device: the device's
+// LDS per workgroup (160 KiB on gfx950, 64 KiB when the query fails) less 4 KiB
+// for the step kernels' own static LDS (at most 1.3 KiB), since a dispatch whose
+// static plus dynamic LDS exceeds the device limit is refused.
+static int deviceLdsLimit() {
+  static int limit = 0;
+  if (limit == 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v >= 65536) {
+      limit = v - 4096;
+    } else {
+      limit = 65536 - 4096;
+    }
+  }
+  return limit;
+}
+
 // Lay out all worlds in flat arrays and allocate the device batch.
 // Dynamic LDS of a solve launch: the body states and per-step body terms (64 B per
 // body) of the class's largest world, its joints, plus room for `lanesPerBody`
@@ -2425,10 +2443,10 @@ int f2dg_autotune(void* h, float dt, int subSteps, int steps, float* msOut) {
 }
 
 // Force the solve launch's dynamic LDS bytes of one size class (at least its body
-// terms; at most 64 KiB).
+// terms; at most the device's LDS per workgroup).
 int f2dg_set_class_solve_lds(void* h, int cls, int bytes) {
   Batch* b = static_cast<Batch*>(h);
-  if (!b || cls < 0 || cls >= (int)b->classes.size() || bytes < 0 || bytes > 65536) return -1;
+  if (!b || cls < 0 || cls >= (int)b->classes.size() || bytes < 0 || bytes > deviceLdsLimit()) return -1;
   SizeClass& c = b->classes[cls];
   const int bodies = std::max(c.maxBodies, 1) * (int)(sizeof(BodyState) + sizeof(BodyConst));
   if (bytes < bodies && bytes != 0) return -1;

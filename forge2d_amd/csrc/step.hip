@@ -688,6 +688,13 @@ __device__ void updateContact(const WorldPtrs& w, Contact& c) {
     c.simFlags &= ~SIM_HIT_EVENT;
   }
   if (pointCount > 0) m.rollingImpulse = old.rollingImpulse;
+  // The old points' impulses as scalars: a matched old point hands its impulses
+  // over once and keeps its id, so a second new point with the same id matches it
+  // again and takes zeros (S/contact.c:568-583).
+  const int oldCount = old.pointCount;
+  const int oldId0 = old.points[0].id & 0xFFFF, oldId1 = old.points[1].id & 0xFFFF;
+  float oldN0 = old.points[0].normalImpulse, oldT0 = old.points[0].tangentImpulse;
+  float oldN1 = old.points[1].normalImpulse, oldT1 = old.points[1].tangentImpulse;
   F2D_UNROLL
   for (int i = 0; i < 2; ++i) {
     if (i >= pointCount) break;
@@ -700,18 +707,18 @@ __device__ void updateContact(const WorldPtrs& w, Contact& c) {
     mp2.normalVelocity = 0.0f;
     int id2 = mp2.id & 0xFFFF;
     mp2.id = id2;  // persisted = false
-    F2D_UNROLL
-    for (int j = 0; j < 2; ++j) {
-      if (j >= old.pointCount) break;
-      ManifoldPoint& mp1 = old.points[j];
-      if ((mp1.id & 0xFFFF) == id2) {
-        mp2.normalImpulse = mp1.normalImpulse;
-        mp2.tangentImpulse = mp1.tangentImpulse;
-        mp2.id = id2 | 0x10000;
-        mp1.normalImpulse = 0.0f;
-        mp1.tangentImpulse = 0.0f;
-        break;
-      }
+    if (oldCount > 0 && oldId0 == id2) {
+      mp2.normalImpulse = oldN0;
+      mp2.tangentImpulse = oldT0;
+      mp2.id = id2 | 0x10000;
+      oldN0 = 0.0f;
+      oldT0 = 0.0f;
+    } else if (oldCount > 1 && oldId1 == id2) {
+      mp2.normalImpulse = oldN1;
+      mp2.tangentImpulse = oldT1;
+      mp2.id = id2 | 0x10000;
+      oldN1 = 0.0f;
+      oldT1 = 0.0f;
     }
   }
   if (touching) {
@@ -1085,8 +1092,11 @@ __device__ __noinline__ void sleepPhase(const WorldPtrs& w) {
 }
 
 // Serial contact state loop, S/world.c:587-690
-__device__ void statePhase(const WorldPtrs& w, SharedState& sh) {
+// serialW: a workgroup-shared copy of w for the out-of-line island calls, so the
+// world pointers stay in registers elsewhere instead of a per-lane stack copy.
+__device__ void statePhase(const WorldPtrs& w, SharedState& sh, WorldPtrs& serialW) {
   if (threadIdx.x == 0) {
+    serialW = w;
     const int* list = w.scratch;
     const int n = sh.changedCount;
     WorldHeader* h = w.hdr;
@@ -1112,7 +1122,7 @@ __device__ void statePhase(const WorldPtrs& w, SharedState& sh) {
       } else if (simFlags & SIM_STARTED) {
         if ((c.flags & CF_CONTACT_EVENTS) && nBegin < h->contactCap) beginIds[nBegin++] = id;
         c.flags |= CF_TOUCHING;
-        linkContact(w, id);
+        linkContact(serialW, id);
         c.simFlags &= ~SIM_STARTED;
         addContactToGraph(w, id);
       } else if (simFlags & SIM_STOPPED) {
@@ -1161,6 +1171,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
   // same for the joints
   Cons cons = w.cons;
   Joint* jts = w.joints;
+  int* jointLanes = w.scratch;  // joint ids by colour
 
   // b2Solve, S/solver.c:1190-1220 and :1479-1706: merge islands linked since the last
   // step, stop when no body is awake, split the last step's candidate island
@@ -1244,6 +1255,11 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       if (lanes > 0 && off + 4L * CN_FIELD_COUNT * lanes <= ldsBytes) {
         cons.f = reinterpret_cast<float*>(reinterpret_cast<char*>(ldsStates) + off);
         cons.stride = lanes;
+        off += 4L * CN_FIELD_COUNT * lanes;
+      }
+      // the joint ids by colour, read at every colour stage
+      if (h->jointCount > 0 && off + 4L * h->jointCount <= ldsBytes) {
+        jointLanes = reinterpret_cast<int*>(reinterpret_cast<char*>(ldsStates) + off);
       }
     }
     for (int i = threadIdx.x; i < nc; i += BLOCK) {
@@ -1256,7 +1272,6 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
     for (int i = threadIdx.x; i < h->overflowCount; i += BLOCK) {
       cons.seti(CN_SLOT, sh.colorStart[kOverflowIndex] + i, w.overflow[i]);
     }
-    int* jointLanes = w.scratch;  // joint ids by colour
     for (int i = threadIdx.x; i < h->jointCount; i += BLOCK) {
       const Joint& j = w.joints[i];
       if (j.color >= 0 && j.color < kOverflowIndex) {
@@ -1604,6 +1619,7 @@ __device__ void stepContext(const WorldHeader* h, const StepParams& sp, SolveCtx
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void collideKernel(BatchView b, StepParams sp, const int* worldList, int ldsBytes) {
   __shared__ SharedState sh;
+  __shared__ WorldPtrs serialW;
   extern __shared__ __align__(16) char lds[];
   const int wi = worldList ? worldList[blockIdx.x] : blockIdx.x;
   WorldHeader* h = b.worlds + wi;
@@ -1629,7 +1645,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     pairsPhase<BLOCK>(w, sh, lds, ldsShapes, ldsPairs, ldsBytes, 2);
   }
   if (sp.phases & PH_COLLIDE) collidePhase<BLOCK>(w, sh);
-  if (sp.phases & PH_STATE) statePhase(w, sh);
+  if (sp.phases & PH_STATE) statePhase(w, sh, serialW);
   if (threadIdx.x == 0 && (sp.phases & (PH_SOLVE | PH_FINALIZE | PH_NO_COUNT)) == 0) h->stepIndex += 1;
 }
 
