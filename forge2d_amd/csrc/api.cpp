@@ -681,7 +681,9 @@ static int treeLdsBytes(const HostWorld& hw, int limit) {
 }
 
 // lanes of the tree launch: about two proxies per lane per pass
-static int treeBlockFor(int proxies) { return proxies <= 96 ? 64 : (proxies <= 192 ? 128 : 256); }
+static int treeBlockFor(int proxies) {
+  return proxies <= 96 ? 64 : (proxies <= 192 ? 128 : (proxies <= 4096 ? 256 : 1024));
+}
 
 static int commit(Batch& b) {
   freeDevice(b);
@@ -852,6 +854,9 @@ static int commit(Batch& b) {
       int cc = collideBytes(cnp);
       c.launch.collideLds = std::max(cc, 12 * 256);
       c.launch.sweepChunks = 28L * std::max(cnp, 4) > (long)c.launch.collideLds ? (c.maxShapes + 255) / 256 : 0;
+      // worlds too large for LDS sorts get 64 KiB for the new-pair sort and the
+      // contact state records
+      if (c.launch.sweepChunks > 0) c.launch.collideLds = 65536;
       c.launch.solveLds = solveLdsBytes(c.maxBodies, c.maxJoints, 1.0f, kLdsLimit);
       c.launch.treeLds = 0;
       int maxProxies = 0;

@@ -142,10 +142,8 @@ F2D_HD int islandRoot(IslandCtx& c, int id) {
 }
 
 // The island part of b2LinkContact, S/island.c:148-218 (the caller wakes sleeping
-// sets first)
-F2D_HD_COLD void linkContactIslands(IslandCtx& c, int cid) {
-  const Contact& ct = c.contacts[cid];
-  int iA = c.sims[ct.bodyA].islandId, iB = c.sims[ct.bodyB].islandId;
+// sets first); iA, iB: the islands of the contact's bodies
+F2D_HD void linkContactIslandIds(IslandCtx& c, int cid, int iA, int iB) {
   if (iA == iB) {
     addContactToIsland(c, iA, cid);
     return;
@@ -157,6 +155,11 @@ F2D_HD_COLD void linkContactIslands(IslandCtx& c, int cid) {
     c.h->mergePending = 1;
   }
   addContactToIsland(c, rA != -1 ? rA : rB, cid);
+}
+
+F2D_HD_COLD void linkContactIslands(IslandCtx& c, int cid) {
+  const Contact& ct = c.contacts[cid];
+  linkContactIslandIds(c, cid, c.sims[ct.bodyA].islandId, c.sims[ct.bodyB].islandId);
 }
 
 // b2UnlinkContact, S/island.c:221-262

@@ -315,7 +315,7 @@ struct LaunchConfig {
   int sweepChunks; // > 0: worlds whose sort arrays exceed LDS sweep in their own launch, this
                    // many 256-shape chunks per world
   int treeLds;     // tree rebuild work arrays (worlds whose trees need more use HBM)
-  int treeBlock;   // lanes of the tree launch (64, 128 or 256)
+  int treeBlock;   // lanes of the tree launch (64, 128, 256 or 1024)
 };
 
 // Contact constraint lanes (one constraint per lane), S/contact_solver.c:1034-1064.

@@ -780,37 +780,32 @@ __device__ __forceinline__ bool getBit(const uint32_t* set, int i) { return (set
 __device__ __forceinline__ void setBit(uint32_t* set, int i) { set[i >> 5] |= 1u << (i & 31); }
 __device__ __forceinline__ void clearBit(uint32_t* set, int i) { set[i >> 5] &= ~(1u << (i & 31)); }
 
-// b2AddContactToGraph colour choice, S/constraint_graph.c:68-135
-__device__ void addContactToGraph(const WorldPtrs& w, int id) {
+// b2AddContactToGraph colour choice, S/constraint_graph.c:68-135: the first colour
+// (from colour 1 when a body is static) that holds neither non-static body, or the
+// overflow. The bodies' bits of every colour are read at once (one memory round
+// trip) into a mask of the colours that hold either.
+__device__ void addContactToGraphBodies(const WorldPtrs& w, int id, int bodyA, int bodyB, bool staticA,
+                                        bool staticB) {
   Contact& c = w.contacts[id];
   int color = kOverflowIndex;
-  int bodyA = c.bodyA, bodyB = c.bodyB;
-  bool staticA = w.sims[bodyA].type == kStatic;
-  bool staticB = w.sims[bodyB].type == kStatic;
-  if (!staticA && !staticB) {
+  if (!staticA || !staticB) {
+    const long stride = w.hdr->colorWords;
+    const uint32_t* wordA = w.colorBits + (bodyA >> 5);
+    const uint32_t* wordB = w.colorBits + (bodyB >> 5);
+    uint32_t busy = 0u;
+    F2D_UNROLL
     for (int i = 0; i < kOverflowIndex; ++i) {
-      uint32_t* set = colorSet(w, i);
-      if (getBit(set, bodyA) || getBit(set, bodyB)) continue;
-      setBit(set, bodyA);
-      setBit(set, bodyB);
-      color = i;
-      break;
+      const uint32_t a = staticA ? 0u : (wordA[i * stride] >> (bodyA & 31));
+      const uint32_t b = staticB ? 0u : (wordB[i * stride] >> (bodyB & 31));
+      busy |= ((a | b) & 1u) << i;
     }
-  } else if (!staticA) {
-    for (int i = 1; i < kOverflowIndex; ++i) {
-      uint32_t* set = colorSet(w, i);
-      if (getBit(set, bodyA)) continue;
-      setBit(set, bodyA);
-      color = i;
-      break;
-    }
-  } else if (!staticB) {
-    for (int i = 1; i < kOverflowIndex; ++i) {
-      uint32_t* set = colorSet(w, i);
-      if (getBit(set, bodyB)) continue;
-      setBit(set, bodyB);
-      color = i;
-      break;
+    if (staticA || staticB) busy |= 1u;  // colour 0 is for dynamic pairs only
+    const uint32_t freeColors = ~busy & ((1u << kOverflowIndex) - 1u);
+    if (freeColors != 0u) {
+      color = __builtin_ctz(freeColors);
+      uint32_t* set = colorSet(w, color);
+      if (!staticA) setBit(set, bodyA);
+      if (!staticB) setBit(set, bodyB);
     }
   }
   c.color = color;
@@ -830,6 +825,12 @@ __device__ void addContactToGraph(const WorldPtrs& w, int id) {
     w.colorLists[kColorListHeader + (long)color * w.hdr->bodyCount + n] = id;
     c.localIndex = n;
   }
+}
+
+__device__ void addContactToGraph(const WorldPtrs& w, int id) {
+  const Contact& c = w.contacts[id];
+  const int bodyA = c.bodyA, bodyB = c.bodyB;
+  addContactToGraphBodies(w, id, bodyA, bodyB, w.sims[bodyA].type == kStatic, w.sims[bodyB].type == kStatic);
 }
 
 // b2RemoveContactFromGraph, S/constraint_graph.c:184-212
@@ -1092,54 +1093,111 @@ __device__ __noinline__ void sleepPhase(const WorldPtrs& w) {
 }
 
 // Serial contact state loop, S/world.c:587-690
-// serialW: a workgroup-shared copy of w for the out-of-line island calls, so the
-// world pointers stay in registers elsewhere instead of a per-lane stack copy.
-__device__ void statePhase(const WorldPtrs& w, SharedState& sh, WorldPtrs& serialW) {
-  if (threadIdx.x == 0) {
-    serialW = w;
-    const int* list = w.scratch;
-    const int n = sh.changedCount;
-    WorldHeader* h = w.hdr;
-    int* beginIds = w.events;
-    int* endPairs = w.events + h->contactCap;
-    int nBegin = 0, nEnd = 0;
-    for (int k = 0; k < n; ++k) {
-      int id = list[k];
-      Contact& c = w.contacts[id];
-      int simFlags = c.simFlags;
-      if (simFlags & SIM_DISJOINT) {
-        // b2DestroyContact reports the end of a touching contact, S/contact.c:354-372
-        if ((c.flags & CF_TOUCHING) && (c.flags & CF_CONTACT_EVENTS) && nEnd < h->contactCap) {
-          endPairs[2 * nEnd] = c.shapeA;
-          endPairs[2 * nEnd + 1] = c.shapeB;
-          ++nEnd;
+// What the serial state loop reads of one changed contact, gathered by all lanes
+// before the loop. None of it changes during the loop except the bodies' solver
+// sets, which a woken sleeping set moves (then they are read again).
+struct ChangeRec {
+  int id, flags, simFlags, shapeA, shapeB, bodyA, bodyB, islandA, islandB, setA, setB, bits;
+};
+enum : int { CR_STATIC_A = 1, CR_STATIC_B = 2, CR_LINKED = 4 };
+
+// Contact state changes in ascending contact id (S/world.c:553-628): begin and end
+// events, island links, graph colours, destroyed contacts. The changed contacts go
+// through LDS in chunks: all lanes gather a chunk's records, then one lane applies
+// them in order. serialW: a workgroup-shared copy of w for the out-of-line island
+// calls, so the world pointers stay in registers elsewhere instead of a per-lane
+// stack copy.
+template <int BLOCK>
+__device__ void statePhase(const WorldPtrs& w, SharedState& sh, WorldPtrs& serialW, char* lds, int ldsBytes) {
+  const int n = sh.changedCount;
+  const int* list = w.scratch;
+  WorldHeader* h = w.hdr;
+  ChangeRec* recs = reinterpret_cast<ChangeRec*>(lds);
+  const int cap = max(ldsBytes / (int)sizeof(ChangeRec), 1);
+  if (threadIdx.x == 0) serialW = w;
+  int nBegin = 0, nEnd = 0;  // thread 0's
+  for (int base = 0; base < n; base += cap) {
+    const int m = min(cap, n - base);
+    for (int k = threadIdx.x; k < m; k += BLOCK) {
+      const int id = list[base + k];
+      const Contact& c = w.contacts[id];
+      ChangeRec r;
+      r.id = id;
+      r.flags = c.flags;
+      r.simFlags = c.simFlags;
+      r.shapeA = c.shapeA;
+      r.shapeB = c.shapeB;
+      r.bodyA = c.bodyA;
+      r.bodyB = c.bodyB;
+      const BodySim& A = w.sims[r.bodyA];
+      const BodySim& B = w.sims[r.bodyB];
+      r.islandA = A.islandId;
+      r.islandB = B.islandId;
+      r.setA = A.setIndex;
+      r.setB = B.setIndex;
+      r.bits = (A.type == kStatic ? CR_STATIC_A : 0) | (B.type == kStatic ? CR_STATIC_B : 0) |
+               (w.links[id].x != -1 ? CR_LINKED : 0);
+      recs[k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int* beginIds = w.events;
+      int* endPairs = w.events + h->contactCap;
+      bool woke = false;
+      for (int k = 0; k < m; ++k) {
+        const ChangeRec r = recs[k];
+        const int id = r.id;
+        Contact& c = w.contacts[id];
+        if (r.simFlags & SIM_DISJOINT) {
+          // b2DestroyContact reports the end of a touching contact, S/contact.c:354-372
+          if ((r.flags & CF_TOUCHING) && (r.flags & CF_CONTACT_EVENTS) && nEnd < h->contactCap) {
+            endPairs[2 * nEnd] = r.shapeA;
+            endPairs[2 * nEnd + 1] = r.shapeB;
+            ++nEnd;
+          }
+          if (r.bits & CR_LINKED) {
+            IslandCtx ic = w.isl;
+            unlinkContact(ic, id);
+          }
+          destroyContact(w, id);
+        } else if (r.simFlags & SIM_STARTED) {
+          if ((r.flags & CF_CONTACT_EVENTS) && nBegin < h->contactCap) beginIds[nBegin++] = id;
+          c.flags = r.flags | CF_TOUCHING;
+          // b2LinkContact (S/island.c:117-218): a sleeping set touched from the awake
+          // set wakes first
+          int setA = r.setA, setB = r.setB;
+          if (woke) {
+            setA = w.sims[r.bodyA].setIndex;
+            setB = w.sims[r.bodyB].setIndex;
+          }
+          if ((setA == kAwakeSet && setB >= kFirstSleepingSet) || (setB == kAwakeSet && setA >= kFirstSleepingSet)) {
+            linkContact(serialW, id);
+            woke = true;
+          } else {
+            IslandCtx ic = w.isl;
+            linkContactIslandIds(ic, id, r.islandA, r.islandB);
+          }
+          c.simFlags = r.simFlags & ~SIM_STARTED;
+          addContactToGraphBodies(w, id, r.bodyA, r.bodyB, (r.bits & CR_STATIC_A) != 0, (r.bits & CR_STATIC_B) != 0);
+        } else if (r.simFlags & SIM_STOPPED) {
+          c.simFlags = r.simFlags & ~SIM_STOPPED;
+          c.flags = r.flags & ~CF_TOUCHING;
+          if ((r.flags & CF_CONTACT_EVENTS) && nEnd < h->contactCap) {
+            endPairs[2 * nEnd] = r.shapeA;
+            endPairs[2 * nEnd + 1] = r.shapeB;
+            ++nEnd;
+          }
+          if (r.bits & CR_LINKED) {
+            IslandCtx ic = w.isl;
+            unlinkContact(ic, id);
+          }
+          removeContactFromGraph(w, id);
         }
-        if (w.links[id].x != -1) {
-          IslandCtx ic = w.isl;
-          unlinkContact(ic, id);
-        }
-        destroyContact(w, id);
-      } else if (simFlags & SIM_STARTED) {
-        if ((c.flags & CF_CONTACT_EVENTS) && nBegin < h->contactCap) beginIds[nBegin++] = id;
-        c.flags |= CF_TOUCHING;
-        linkContact(serialW, id);
-        c.simFlags &= ~SIM_STARTED;
-        addContactToGraph(w, id);
-      } else if (simFlags & SIM_STOPPED) {
-        c.simFlags &= ~SIM_STOPPED;
-        c.flags &= ~CF_TOUCHING;
-        if ((c.flags & CF_CONTACT_EVENTS) && nEnd < h->contactCap) {
-          endPairs[2 * nEnd] = c.shapeA;
-          endPairs[2 * nEnd + 1] = c.shapeB;
-          ++nEnd;
-        }
-        if (w.links[id].x != -1) {
-          IslandCtx ic = w.isl;
-          unlinkContact(ic, id);
-        }
-        removeContactFromGraph(w, id);
       }
     }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
     h->beginCount = nBegin;
     h->endCount = nEnd;
   }
@@ -1645,7 +1703,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     pairsPhase<BLOCK>(w, sh, lds, ldsShapes, ldsPairs, ldsBytes, 2);
   }
   if (sp.phases & PH_COLLIDE) collidePhase<BLOCK>(w, sh);
-  if (sp.phases & PH_STATE) statePhase(w, sh, serialW);
+  if (sp.phases & PH_STATE) statePhase<BLOCK>(w, sh, serialW, lds, ldsBytes);
   if (threadIdx.x == 0 && (sp.phases & (PH_SOLVE | PH_FINALIZE | PH_NO_COUNT)) == 0) h->stepIndex += 1;
 }
 
@@ -2299,7 +2357,9 @@ __global__ void __launch_bounds__(BLOCK) treeKernel(BatchView b, const int* worl
 
 static void launchTree(const BatchView* b, const int* worldList, int nWorlds, hipStream_t stream,
                        const LaunchConfig* cfg) {
-  if (cfg->treeBlock >= 256) {
+  if (cfg->treeBlock >= 1024) {
+    hipLaunchKernelGGL(treeKernel<1024>, dim3(nWorlds), dim3(1024), cfg->treeLds, stream, *b, worldList, cfg->treeLds);
+  } else if (cfg->treeBlock >= 256) {
     hipLaunchKernelGGL(treeKernel<256>, dim3(nWorlds), dim3(256), cfg->treeLds, stream, *b, worldList, cfg->treeLds);
   } else if (cfg->treeBlock >= 128) {
     hipLaunchKernelGGL(treeKernel<128>, dim3(nWorlds), dim3(128), cfg->treeLds, stream, *b, worldList, cfg->treeLds);
