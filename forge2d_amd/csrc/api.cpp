@@ -857,6 +857,9 @@ static int commit(Batch& b) {
       // worlds too large for LDS sorts get 64 KiB for the new-pair sort and the
       // contact state records
       if (c.launch.sweepChunks > 0) c.launch.collideLds = 65536;
+      // large worlds spread their non-bullet continuous sweeps over workgroups of
+      // 64 fast bodies
+      c.launch.continuousChunks = c.maxBodies > 1024 ? (c.maxBodies + 63) / 64 : 0;
       c.launch.solveLds = solveLdsBytes(c.maxBodies, c.maxJoints, 1.0f, kLdsLimit);
       c.launch.treeLds = 0;
       int maxProxies = 0;

@@ -283,6 +283,7 @@ enum : int {
   PH_PAIRS_FINISH = 512,   // pair update of a large world: create the pairs the sweep launch found
   PH_NO_TREE = 1024,       // collide without the tree launch (timing splits)
   PH_TREE = 2048,          // the tree launch on its own (timing splits)
+  PH_CONTACTS_DONE = 4096, // collide launch of a large world: the narrow-phase launch updated the contacts
 };
 
 // A world query of the batch query ABI (f2dQuery in the public header). Rays:
@@ -313,9 +314,12 @@ struct LaunchConfig {
   int solveLds;    // body-state LDS
   int continuous;  // some world of the batch has continuous collision enabled
   int sweepChunks; // > 0: worlds whose sort arrays exceed LDS sweep in their own launch, this
-                   // many 256-shape chunks per world
+                   // many 256-shape chunks per world, and update their contacts in a
+                   // narrow-phase launch over many workgroups
   int treeLds;     // tree rebuild work arrays (worlds whose trees need more use HBM)
   int treeBlock;   // lanes of the tree launch (64, 128, 256 or 1024)
+  int continuousChunks = 0;  // > 0: the non-bullet sweeps of each world spread over this many
+                             // workgroups, then bullets and the sleep phase in a second launch
 };
 
 // Contact constraint lanes (one constraint per lane), S/contact_solver.c:1034-1064.

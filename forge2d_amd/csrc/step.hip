@@ -734,20 +734,27 @@ __device__ void updateContact(const WorldPtrs& w, Contact& c) {
   c.m = m;
 }
 
+// The narrow phase of contact slot i: awake contacts whose fat boxes stopped
+// overlapping are flagged disjoint, the others get a new manifold.
+__device__ __forceinline__ void collideContact(const WorldPtrs& w, int i) {
+  Contact& c = w.contacts[i];
+  if (c.shapeA < 0 || (c.flags & CF_OFF_AWAKE)) return;  // only the awake set collides (S/world.c:496-521)
+  AABB fa = w.shapes[c.shapeA].fat, fb = w.shapes[c.shapeB].fat;
+  if (!aabbOverlaps(fa, fb)) {
+    c.simFlags |= SIM_DISJOINT;
+    c.simFlags &= ~SIM_TOUCHING;
+  } else {
+    updateContact(w, c);
+  }
+}
+
+// contactsDone: a narrow-phase launch already updated the contacts
 template <int BLOCK>
-__device__ void collidePhase(const WorldPtrs& w, SharedState& sh) {
+__device__ void collidePhase(const WorldPtrs& w, SharedState& sh, bool contactsDone) {
   WorldHeader* h = w.hdr;
   const int n = h->contactCount;
-  for (int i = threadIdx.x; i < n; i += BLOCK) {
-    Contact& c = w.contacts[i];
-    if (c.shapeA < 0 || (c.flags & CF_OFF_AWAKE)) continue;  // only the awake set collides (S/world.c:496-521)
-    AABB fa = w.shapes[c.shapeA].fat, fb = w.shapes[c.shapeB].fat;
-    if (!aabbOverlaps(fa, fb)) {
-      c.simFlags |= SIM_DISJOINT;
-      c.simFlags &= ~SIM_TOUCHING;
-    } else {
-      updateContact(w, c);
-    }
+  if (!contactsDone) {
+    for (int i = threadIdx.x; i < n; i += BLOCK) collideContact(w, i);
   }
   __syncthreads();
   // ordered list of contacts whose state changed (ascending contact id)
@@ -1702,7 +1709,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   } else if (sp.phases & PH_PAIRS_FINISH) {
     pairsPhase<BLOCK>(w, sh, lds, ldsShapes, ldsPairs, ldsBytes, 2);
   }
-  if (sp.phases & PH_COLLIDE) collidePhase<BLOCK>(w, sh);
+  if (sp.phases & PH_COLLIDE) collidePhase<BLOCK>(w, sh, (sp.phases & PH_CONTACTS_DONE) != 0);
   if (sp.phases & PH_STATE) statePhase<BLOCK>(w, sh, serialW, lds, ldsBytes);
   if (threadIdx.x == 0 && (sp.phases & (PH_SOLVE | PH_FINALIZE | PH_NO_COUNT)) == 0) h->stepIndex += 1;
 }
@@ -1725,6 +1732,19 @@ __global__ __launch_bounds__(256) void sweepKernel(BatchView b, const int* world
 
 // Solver stages and finalize of one world per workgroup; body states and the
 // per-step body terms live in ldsBytes of dynamic LDS (64 B per body) when they fit.
+// The narrow phase of the worlds of a split pair update: workgroup blockIdx.x of
+// gridDim.x takes every gridDim.x-th run of 256 contact slots of world blockIdx.y
+// (contacts update independently; the collide launch after it lists the changes).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void narrowKernel(BatchView b,
+                                                                                            const int* worldList) {
+  const int wi = worldList ? worldList[blockIdx.y] : blockIdx.y;
+  WorldHeader* h = b.worlds + wi;
+  const int n = h->contactCount;
+  if ((int)blockIdx.x * 256 >= n) return;
+  WorldPtrs w = worldPtrs(b, h);
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) collideContact(w, i);
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void solveKernel(BatchView b, StepParams sp, const int* worldList, int ldsBytes) {
   __shared__ SharedState sh;
@@ -1888,7 +1908,7 @@ __device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList
         fat.hi.x = aabb.hi.x + kAabbMargin;
         fat.hi.y = aabb.hi.y + kAabbMargin;
         shape.fat = fat;
-          shape.flags |= SF_TREE_PENDING;
+        shape.flags |= SF_TREE_PENDING;
         shape.flags |= SF_MOVED;
         shape.moveRank = (sim.localIndex << 8) | (rank < 255 ? rank : 255);
         flags |= BF_ENLARGED;
@@ -1907,7 +1927,7 @@ __device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList
         fat.hi.x = shape.aabb.hi.x + kAabbMargin;
         fat.hi.y = shape.aabb.hi.y + kAabbMargin;
         shape.fat = fat;
-          shape.flags |= SF_TREE_PENDING;
+        shape.flags |= SF_TREE_PENDING;
         shape.flags |= SF_MOVED;
         shape.moveRank = (sim.localIndex << 8) | (rank < 255 ? rank : 255);
         flags |= BF_ENLARGED;
@@ -1917,16 +1937,22 @@ __device__ void solveContinuous(const WorldPtrs& w, int i, const int* staticList
   sim.flags = flags;
 }
 
-// Continuous collision of the fast bodies finalize listed, one world per
-// workgroup, one fast body per lane, then island sleep for worlds with sleep
-// enabled. Worlds with neither return at once.
+// Continuous collision of the fast bodies finalize listed, one fast body per lane,
+// then island sleep for worlds with sleep enabled. Worlds with neither return at
+// once. part 0: the whole pass, one workgroup per world (blockIdx.x). part 1: the
+// non-bullet sweeps only, workgroup blockIdx.x of gridDim.x taking every
+// gridDim.x-th stride of the fast list of world blockIdx.y (non-bullet fast bodies
+// sweep against static shapes only, so they are independent). part 2: the bullets
+// and the sleep phase after a part-1 launch, one workgroup per world.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void continuousKernel(BatchView b, const int* worldList) {
-  const int wi = worldList ? worldList[blockIdx.x] : blockIdx.x;
+__global__ __launch_bounds__(BLOCK) void continuousKernel(BatchView b, const int* worldList, int part) {
+  const int wi = part == 1 ? (worldList ? worldList[blockIdx.y] : blockIdx.y)
+                           : (worldList ? worldList[blockIdx.x] : blockIdx.x);
   WorldHeader* h = b.worlds + wi;
   const int nFast = h->fastCount;
   const bool sleep = h->sleepPending != 0;
-  if (nFast == 0 && !sleep) return;
+  if (nFast == 0 && (!sleep || part == 1)) return;
+  if (part == 1 && (int)blockIdx.x * BLOCK >= nFast) return;
   WorldPtrs w = worldPtrs(b, h);
   const int ns = h->shapeCount;
   const int* fastList = w.scratch;  // written by finalize, nFast <= shapeCount
@@ -1941,9 +1967,14 @@ __global__ __launch_bounds__(BLOCK) void continuousKernel(BatchView b, const int
     statics.visible[k] = ((t.catLo | t.catHi) != 0 && t.proxyKey >= 0) ? 1 : 0;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < nFast; k += BLOCK) {
-    int i = fastList[k];
-    if ((w.sims[i].flags & BF_BULLET) == 0) solveContinuous(w, i, staticList, staticCount, false, statics);
+  if (part != 2) {
+    const int first = part == 1 ? (int)blockIdx.x * BLOCK : 0;
+    const int stride = part == 1 ? (int)gridDim.x * BLOCK : BLOCK;
+    for (int k = first + threadIdx.x; k < nFast; k += stride) {
+      int i = fastList[k];
+      if ((w.sims[i].flags & BF_BULLET) == 0) solveContinuous(w, i, staticList, staticCount, false, statics);
+    }
+    if (part == 1) return;
   }
   __syncthreads();
   for (int k = threadIdx.x; k < nFast; k += BLOCK) {
@@ -2379,8 +2410,15 @@ static int launchBlock(const BatchView* b, const StepParams* sp, const int* worl
                        cfg->collideLds);
     hipLaunchKernelGGL(sweepKernel, dim3(cfg->sweepChunks, nWorlds), dim3(256), 0, stream, *b, worldList);
     StepParams c = *sp;
-    c.phases = (sp->phases & ~PH_PAIRS) | PH_PAIRS_FINISH;
+    c.phases = (sp->phases & ~(PH_PAIRS | PH_COLLIDE | PH_STATE)) | PH_PAIRS_FINISH | PH_NO_COUNT;
     hipLaunchKernelGGL(collideKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->collideLds, stream, *b, c, worldList,
+                       cfg->collideLds);
+    if (sp->phases & PH_COLLIDE) {
+      hipLaunchKernelGGL(narrowKernel, dim3(cfg->sweepChunks * 2, nWorlds), dim3(256), 0, stream, *b, worldList);
+    }
+    StepParams d = *sp;
+    d.phases = (sp->phases & ~PH_PAIRS) | PH_CONTACTS_DONE;
+    hipLaunchKernelGGL(collideKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->collideLds, stream, *b, d, worldList,
                        cfg->collideLds);
   } else if (sp->phases & (PH_PAIRS | PH_COLLIDE | PH_STATE)) {
     hipLaunchKernelGGL(collideKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->collideLds, stream, *b, *sp, worldList,
@@ -2395,8 +2433,15 @@ static int launchBlock(const BatchView* b, const StepParams* sp, const int* worl
   }
   bool ccd = (sp->phases & PH_CONTINUOUS) || ((sp->phases & PH_FINALIZE) && !(sp->phases & PH_NO_CONTINUOUS));
   if (ccd && cfg->continuous) {
-    hipLaunchKernelGGL(continuousKernel<kContinuousLanes>, dim3(nWorlds), dim3(kContinuousLanes), 0, stream, *b,
-                       worldList);
+    if (cfg->continuousChunks > 0 && nWorlds <= 65535) {
+      hipLaunchKernelGGL(continuousKernel<kContinuousLanes>, dim3(cfg->continuousChunks, nWorlds),
+                         dim3(kContinuousLanes), 0, stream, *b, worldList, 1);
+      hipLaunchKernelGGL(continuousKernel<kContinuousLanes>, dim3(nWorlds), dim3(kContinuousLanes), 0, stream, *b,
+                         worldList, 2);
+    } else {
+      hipLaunchKernelGGL(continuousKernel<kContinuousLanes>, dim3(nWorlds), dim3(kContinuousLanes), 0, stream, *b,
+                         worldList, 0);
+    }
   }
   return (int)hipGetLastError();
 }
