@@ -32,7 +32,7 @@
 namespace f2d {
 
 constexpr int kMaxWaves = 16;  // largest workgroup: 1024 lanes
-constexpr int kLongSweeps = 64;   // long sweeps a workgroup continues together
+constexpr int kLongSweeps = 256;  // long sweeps a workgroup continues together (all of a 256-shape chunk)
 constexpr int kLongRun = 32;      // candidates one lane tests before it hands a sweep over
 
 struct SharedState {
