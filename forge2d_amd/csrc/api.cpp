@@ -731,7 +731,7 @@ static int commit(Batch& b) {
     h.scratchOffset = (int)nscratch;
     int np = nextPow2(std::max(h.shapeCount, 1));
     int pp = nextPow2(cap);
-    long sc = std::max<long>({4L * np + 16, 3L * pp + 16, (long)cap + 16, (long)h.jointCount + 16});
+    long sc = std::max<long>({4L * np + 16, 3L * pp + 16, (long)cap + 16, (long)h.jointCount + 80});
     sc = (sc + 3) & ~3L;  // 16-byte aligned world regions (64-bit sort keys live here)
     h.scratchCap = (int)sc;
     h.contactCount = hw.contactCount;
@@ -860,6 +860,8 @@ static int commit(Batch& b) {
       // large worlds spread their non-bullet continuous sweeps over workgroups of
       // 64 fast bodies
       c.launch.continuousChunks = c.maxBodies > 1024 ? (c.maxBodies + 63) / 64 : 0;
+      // worlds of many thousand bodies run their solve stages over many workgroups
+      c.launch.wideChunks = c.maxBodies > 4096 ? (c.maxBodies + 255) / 256 : 0;
       c.launch.solveLds = solveLdsBytes(c.maxBodies, c.maxJoints, 1.0f, kLdsLimit);
       c.launch.treeLds = 0;
       int maxProxies = 0;

@@ -284,6 +284,8 @@ enum : int {
   PH_NO_TREE = 1024,       // collide without the tree launch (timing splits)
   PH_TREE = 2048,          // the tree launch on its own (timing splits)
   PH_CONTACTS_DONE = 4096, // collide launch of a large world: the narrow-phase launch updated the contacts
+  PH_SOLVE_PREP = 8192,    // solve launch of a large world: islands and colour buckets only, the stages
+                           // run as launches over many workgroups
 };
 
 // A world query of the batch query ABI (f2dQuery in the public header). Rays:
@@ -320,6 +322,8 @@ struct LaunchConfig {
   int treeBlock;   // lanes of the tree launch (64, 128, 256 or 1024)
   int continuousChunks = 0;  // > 0: the non-bullet sweeps of each world spread over this many
                              // workgroups, then bullets and the sleep phase in a second launch
+  int wideChunks = 0;        // > 0: the solve stages of each world as launches of this many
+                             // 256-lane workgroups
 };
 
 // Contact constraint lanes (one constraint per lane), S/contact_solver.c:1034-1064.

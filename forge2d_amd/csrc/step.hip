@@ -34,6 +34,10 @@ namespace f2d {
 constexpr int kMaxWaves = 16;  // largest workgroup: 1024 lanes
 constexpr int kLongSweeps = 256;  // long sweeps a workgroup continues together (all of a 256-shape chunk)
 constexpr int kLongRun = 32;      // candidates one lane tests before it hands a sweep over
+// Colour table of a split solve at the front of the world's scratch: constraint
+// counts and starts by colour, joint counts and starts by colour
+constexpr int kWideColorCount = 0, kWideColorStart = 12, kWideJointCount = 25, kWideJointStart = 37;
+constexpr int kWideTableInts = 64;
 
 struct SharedState {
   int colorCount[kGraphColorCount];
@@ -1236,7 +1240,8 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
   // same for the joints
   Cons cons = w.cons;
   Joint* jts = w.joints;
-  int* jointLanes = w.scratch;  // joint ids by colour
+  // joint ids by colour (behind the colour table of the stage launches in a split solve)
+  int* jointLanes = (phases & PH_SOLVE_PREP) ? w.scratch + kWideTableInts : w.scratch;
 
   // b2Solve, S/solver.c:1190-1220 and :1479-1706: merge islands linked since the last
   // step, stop when no body is awake, split the last step's candidate island
@@ -1369,6 +1374,17 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       }
     }
     __syncthreads();
+    if (phases & PH_SOLVE_PREP) {
+      // the colour table of the stage launches (wideStageKernel)
+      int* table = w.scratch;
+      if (threadIdx.x < kGraphColorCount) {
+        table[kWideColorCount + threadIdx.x] = sh.colorCount[threadIdx.x];
+        table[kWideJointCount + threadIdx.x] = sh.jointColorCount[threadIdx.x];
+        table[kWideJointStart + threadIdx.x] = sh.jointColorStart[threadIdx.x];
+      }
+      if (threadIdx.x <= kGraphColorCount) table[kWideColorStart + threadIdx.x] = sh.colorStart[threadIdx.x];
+      return;
+    }
     const int totalLanes = sh.colorStart[kGraphColorCount];
     const int overflowStart = sh.colorStart[kOverflowIndex];
     const int overflowCount = sh.colorCount[kOverflowIndex];
@@ -1759,7 +1775,119 @@ __global__ __launch_bounds__(BLOCK) void solveKernel(BatchView b, StepParams sp,
   } else {
     solveAndFinalize<BLOCK, false>(w, sh, nullptr, ctx, sp.phases, ldsBytes);
   }
-  if (threadIdx.x == 0) h->stepIndex += 1;
+  if (threadIdx.x == 0 && (sp.phases & PH_NO_COUNT) == 0) h->stepIndex += 1;
+}
+
+// The solve stages of large worlds, one launch per stage (and colour), workgroup
+// blockIdx.x of gridDim.x taking every gridDim.x-th run of 256 items of world
+// blockIdx.y: the per-item work of solveAndFinalize's HBM path, in the same stage
+// order (prepare; per substep: integrate velocities, warm start, solve, integrate
+// positions, relax, each constraint stage the overflow first, one lane, then the
+// colours; restitution; store). The bodies a colour's constraints touch are
+// distinct, so a colour's items run in any order. After a PH_SOLVE_PREP launch.
+enum WideStage : int { WS_PREPARE, WS_INTEGRATE_VEL, WS_WARM, WS_SOLVE, WS_INTEGRATE_POS, WS_RELAX, WS_RESTITUTION,
+                       WS_STORE };
+
+__global__ __launch_bounds__(256) void wideStageKernel(BatchView b, StepParams sp, const int* worldList, int stage,
+                                                       int color) {
+  const int wi = worldList ? worldList[blockIdx.y] : blockIdx.y;
+  WorldHeader* h = b.worlds + wi;
+  if (h->awakeBodyCount == 0) return;  // b2Solve stopped after merging islands
+  const int* table = b.scratch + h->scratchOffset;
+  const int gtid = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
+  const bool constraintStage = stage == WS_WARM || stage == WS_SOLVE || stage == WS_RELAX || stage == WS_RESTITUTION;
+  if (constraintStage) {
+    const int cnt = table[kWideColorCount + color];
+    const int jcnt = stage == WS_RESTITUTION ? 0 : table[kWideJointCount + color];
+    if (color == kOverflowIndex ? (gtid != 0 || cnt + jcnt == 0) : gtid >= max(cnt, jcnt)) return;
+  }
+  WorldPtrs w = worldPtrs(b, h);
+  SolveCtx ctx;
+  stepContext(h, sp, ctx);
+  const Cons cons = w.cons;
+  BodyState* states = w.states;
+  Joint* jts = w.joints;
+  const int* jointLanes = w.scratch + kWideTableInts;
+  const int nb = h->bodyCount;
+  if (stage == WS_PREPARE) {
+    // S/solver.c:967-987, S/contact_solver.c:1438-1669
+    for (int i = gtid; i < h->jointCount; i += stride)
+      if (jts[i].color >= 0) prepareRevolute(jts[i], w.sims, ctx.h, ctx.inv_h, ctx.warmStartScale);
+    const int totalLanes = table[kWideColorStart + kGraphColorCount];
+    for (int lane = gtid; lane < totalLanes; lane += stride) {
+      const int slot = cons.geti(CN_SLOT, lane);
+      const Contact& c = w.contacts[slot];
+      const BodySim& sa = w.sims[c.bodyA];
+      const BodySim& sb = w.sims[c.bodyB];
+      const bool staticA = sa.type == kStatic, staticB = sb.type == kStatic;
+      float mA = sa.invMass, iA = sa.invI, mB = sb.invMass, iB = sb.invI;
+      const int ia = staticA ? -1 : c.bodyA;
+      const int ib = staticB ? -1 : c.bodyB;
+      if (staticA) mA = 0.0f, iA = 0.0f;
+      if (staticB) mB = 0.0f, iB = 0.0f;
+      prepareContactLane(cons, lane, c, slot, ia, ib, mA, iA, mB, iB, states, ctx.contactSoftness,
+                         ctx.staticSoftness, ctx.warmStartScale);
+    }
+  } else if (stage == WS_INTEGRATE_VEL) {
+    for (int i = gtid; i < nb; i += stride) {
+      BodySim& sim = w.sims[i];
+      if (sim.setIndex != kAwakeSet) continue;
+      BodyState st = states[i];
+      integrateVelocity(sim, st, ctx.gravity, ctx.h, ctx.maxLinearSpeed, ctx.maxAngularSpeed);
+      states[i] = st;
+    }
+  } else if (stage == WS_INTEGRATE_POS) {
+    for (int i = gtid; i < nb; i += stride) {
+      if (w.sims[i].setIndex != kAwakeSet) continue;
+      BodyState st = states[i];
+      integratePosition(st, ctx.h);
+      states[i] = st;
+    }
+  } else if (stage == WS_STORE) {
+    const int totalLanes = table[kWideColorStart + kGraphColorCount];
+    const int overflowStart = table[kWideColorStart + kOverflowIndex];
+    for (int lane = gtid; lane < totalLanes; lane += stride) {
+      const int slot = cons.geti(CN_SLOT, lane);
+      if (lane >= overflowStart) {
+        overflowStore(cons, lane, w.contacts[slot]);
+      } else {
+        storeLane(cons, lane, w.contacts[slot]);
+      }
+    }
+  } else if (color == kOverflowIndex) {
+    // one lane, the overflow joints then the overflow contacts in list order
+    const int start = table[kWideColorStart + kOverflowIndex], cnt = table[kWideColorCount + kOverflowIndex];
+    const int* jointList = jointLanes + table[kWideJointStart + kOverflowIndex];
+    const int jcnt = table[kWideJointCount + kOverflowIndex];
+    if (stage == WS_RESTITUTION) {
+      for (int k = 0; k < cnt; ++k) overflowRestitution(cons, start + k, states, ctx.restitutionThreshold);
+    } else if (stage == WS_WARM) {
+      for (int k = 0; k < jcnt; ++k)
+        if (jointList[k] >= 0) warmStartRevolute(jts[jointList[k]], states);
+      for (int k = 0; k < cnt; ++k) overflowWarmStart(cons, start + k, states);
+    } else {
+      const bool useBias = stage == WS_SOLVE;
+      for (int k = 0; k < jcnt; ++k)
+        if (jointList[k] >= 0) solveRevolute(jts[jointList[k]], states, ctx.h, ctx.inv_h, useBias);
+      for (int k = 0; k < cnt; ++k)
+        overflowSolve(cons, start + k, states, ctx.inv_h, ctx.maxContactPushSpeed, useBias);
+    }
+  } else {
+    const int start = table[kWideColorStart + color], cnt = table[kWideColorCount + color];
+    const int jstart = table[kWideJointStart + color], jcnt = table[kWideJointCount + color];
+    if (stage == WS_RESTITUTION) {
+      for (int k = gtid; k < cnt; k += stride) restitutionLane(cons, start + k, states, ctx.restitutionThreshold);
+    } else if (stage == WS_WARM) {
+      for (int k = gtid; k < jcnt; k += stride) warmStartRevolute(jts[jointLanes[jstart + k]], states);
+      for (int k = gtid; k < cnt; k += stride) warmStartLane(cons, start + k, states);
+    } else {
+      const bool useBias = stage == WS_SOLVE;
+      for (int k = gtid; k < jcnt; k += stride)
+        solveRevolute(jts[jointLanes[jstart + k]], states, ctx.h, ctx.inv_h, useBias);
+      for (int k = gtid; k < cnt; k += stride)
+        solveLane(cons, start + k, states, ctx.inv_h, -ctx.contactSpeed, useBias);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------- phase: continuous
@@ -2427,7 +2555,37 @@ static int launchBlock(const BatchView* b, const StepParams* sp, const int* worl
   if (((sp->phases & PH_COLLIDE) && !(sp->phases & PH_NO_TREE)) || (sp->phases & PH_TREE)) {
     launchTree(b, worldList, nWorlds, stream, cfg);
   }
-  if (sp->phases & (PH_SOLVE | PH_FINALIZE)) {
+  if (cfg->wideChunks > 0 && (sp->phases & PH_SOLVE) && nWorlds <= 65535) {
+    // large worlds: islands and colour buckets in one workgroup per world, then
+    // every stage over many workgroups, then finalize in one workgroup per world
+    StepParams p = *sp;
+    p.phases = PH_SOLVE | PH_SOLVE_PREP | ((sp->phases & PH_FINALIZE) ? PH_NO_COUNT : 0);
+    hipLaunchKernelGGL(solveKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), 0, stream, *b, p, worldList, 0);
+    const dim3 grid(cfg->wideChunks, nWorlds);
+    auto stage = [&](int st, int color) {
+      hipLaunchKernelGGL(wideStageKernel, grid, dim3(256), 0, stream, *b, *sp, worldList, st, color);
+    };
+    auto colors = [&](int st) {
+      stage(st, kOverflowIndex);
+      for (int c = 0; c < kOverflowIndex; ++c) stage(st, c);
+    };
+    stage(WS_PREPARE, 0);
+    for (int sub = 0; sub < sp->subSteps; ++sub) {
+      stage(WS_INTEGRATE_VEL, 0);
+      colors(WS_WARM);
+      colors(WS_SOLVE);
+      stage(WS_INTEGRATE_POS, 0);
+      colors(WS_RELAX);
+    }
+    colors(WS_RESTITUTION);
+    stage(WS_STORE, 0);
+    if (sp->phases & PH_FINALIZE) {
+      StepParams f = *sp;
+      f.phases = sp->phases & ~PH_SOLVE;
+      hipLaunchKernelGGL(solveKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->solveLds, stream, *b, f, worldList,
+                         cfg->solveLds);
+    }
+  } else if (sp->phases & (PH_SOLVE | PH_FINALIZE)) {
     hipLaunchKernelGGL(solveKernel<BLOCK>, dim3(nWorlds), dim3(BLOCK), cfg->solveLds, stream, *b, *sp, worldList,
                        cfg->solveLds);
   }
