@@ -194,17 +194,20 @@ F2D_HD_COLD void linkJointIslands(IslandCtx& c, int jid) {
 }
 
 // b2MergeIsland, S/island.c:446-531: relabel the child's bodies, contacts and
-// joints and append its lists to the root's
-F2D_HD void mergeIsland(IslandCtx& c, int id) {
+// joints and append its lists to the root's. relabel false: the caller has
+// relabelled them already (the labels do not take part in the appends).
+F2D_HD void mergeIsland(IslandCtx& c, int id, bool relabel = true) {
   Island& is = c.islands[id];
   int rootId = is.parent;
   Island& root = c.islands[rootId];
-  int guard = c.h->bodyCount;
-  for (int b = is.headBody; b != -1 && guard-- > 0; b = c.sims[b].islandNext) c.sims[b].islandId = rootId;
-  guard = c.h->contactCap;
-  for (int k = is.headContact; k != -1 && guard-- > 0; k = c.links[k].z) c.links[k].x = rootId;
-  guard = c.h->jointCount;
-  for (int j = is.headJoint; j != -1 && guard-- > 0; j = c.joints[j].islandNext) c.joints[j].islandId = rootId;
+  if (relabel) {
+    int guard = c.h->bodyCount;
+    for (int b = is.headBody; b != -1 && guard-- > 0; b = c.sims[b].islandNext) c.sims[b].islandId = rootId;
+    guard = c.h->contactCap;
+    for (int k = is.headContact; k != -1 && guard-- > 0; k = c.links[k].z) c.links[k].x = rootId;
+    guard = c.h->jointCount;
+    for (int j = is.headJoint; j != -1 && guard-- > 0; j = c.joints[j].islandNext) c.joints[j].islandId = rootId;
+  }
   c.sims[root.tailBody].islandNext = is.headBody;
   c.sims[is.headBody].islandPrev = root.tailBody;
   root.tailBody = is.tailBody;
@@ -252,16 +255,17 @@ F2D_HD_COLD void mergeAwakeIslands(IslandCtx& c) {
   }
 }
 
-// The merges of mergeAwakeIslands once every awake island points at its root (the
-// solve kernel finds the roots on all lanes): children merge into their roots in
-// reverse awake order.
+// The merges of mergeAwakeIslands once every awake island points at its root and
+// the children's bodies, contacts and joints carry their roots' ids (the solve
+// kernel does both on all lanes): children merge into their roots in reverse
+// awake order.
 F2D_HD_COLD void mergeAwakeChildren(IslandCtx& c) {
   WorldHeader* h = c.h;
   h->mergePending = 0;
   for (int i = h->awakeIslandCount - 1; i >= 0; --i) {
     int id = c.awakeIslands[i];
     if (c.islands[id].parent == -1) continue;
-    mergeIsland(c, id);
+    mergeIsland(c, id, false);
     destroyIsland(c, id);
   }
 }

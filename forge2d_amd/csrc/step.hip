@@ -1262,6 +1262,28 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         if (root != id) ic.islands[id].parent = root;
       }
       __syncthreads();
+      // the relabelling of b2MergeIsland on all lanes: whatever belongs to an awake
+      // island with a parent (a child to merge) takes the parent's (its root's) id
+      const int ni = h->islandCap;
+      for (int i = threadIdx.x; i < nb; i += BLOCK) {
+        const int id = w.sims[i].islandId;
+        if (id < 0 || id >= ni) continue;
+        const Island& is = ic.islands[id];
+        if (is.parent != -1 && is.setIndex == kAwakeSet) w.sims[i].islandId = is.parent;
+      }
+      for (int k = threadIdx.x; k < nc; k += BLOCK) {
+        const int id = w.links[k].x;
+        if (id < 0 || id >= ni) continue;
+        const Island& is = ic.islands[id];
+        if (is.parent != -1 && is.setIndex == kAwakeSet) w.links[k].x = is.parent;
+      }
+      for (int j = threadIdx.x; j < h->jointCount; j += BLOCK) {
+        const int id = w.joints[j].islandId;
+        if (id < 0 || id >= ni) continue;
+        const Island& is = ic.islands[id];
+        if (is.parent != -1 && is.setIndex == kAwakeSet) w.joints[j].islandId = is.parent;
+      }
+      __syncthreads();
     }
     if (threadIdx.x == 0) {
       IslandCtx ic = w.isl;
