@@ -3,8 +3,10 @@
 Requests arrive over time; each names a world of a population and a number of
 steps k. A world that is not resident is created on the device (admission is
 bounded by an HBM budget; when the budget is full, idle worlds are evicted in
-least-recently-used order, a whole device batch at a time); a resident world is
-continued from its device state. Every scheduler iteration launches one step for
+least-recently-used order, a whole device batch at a time, each world's state
+exported to host memory first); a resident world is continued from its device
+state, and a re-admitted world from its exported state. A request for a world
+that could never fit the budget is refused. Every scheduler iteration launches one step for
 every world that still owes steps — across all resident batches, each on its
 own HIP stream so their kernels overlap — and requests that arrive meanwhile
 join at the next iteration. A request completes when its k-th step has
@@ -22,7 +24,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Batch, scenes
+from . import Batch, scenes, snapshot
 
 
 @dataclass
@@ -48,9 +50,13 @@ class _Resident:
 class Generation:
     """One device batch holding the worlds admitted together."""
 
-    def __init__(self, members, specs, stream=None):
+    def __init__(self, members, specs, stream=None, states=None):
         self.members = list(members)                      # population indices
         self.batch = Batch(b"".join([scenes.MAGIC, np.array([len(specs), 0], "<i4").tobytes()] + specs))
+        # worlds evicted earlier continue from their exported states
+        for i, snap in enumerate(states or []):
+            if snap is not None:
+                self.batch.import_state(i, snap)
         self.stream = stream
         if stream is not None:
             self.batch.set_stream(stream.cuda_stream)
@@ -105,6 +111,8 @@ class ServingEngine:
         self.iterations = 0
         self.evicted = 0
         self.refused = 0
+        self.saved: dict[int, bytes] = {}        # exported states of evicted worlds
+        self.total_steps: dict[int, int] = collections.defaultdict(int)  # steps run per world, all admissions
 
     # -- admission -------------------------------------------------------------------------------------------
     def _estimate(self, world: int) -> int:
@@ -122,7 +130,9 @@ class ServingEngine:
             if any(m in self.resident and self.resident[m].queue for m in g.members):
                 continue
             for m in g.members:
-                self.resident.pop(m, None)
+                res = self.resident.pop(m, None)
+                if res is not None:
+                    self.saved[m] = g.batch.export(res.index)
             self.used -= g.bytes
             self.generations.remove(g)
             g.close()
@@ -138,7 +148,7 @@ class ServingEngine:
             import torch
 
             stream = torch.cuda.Stream()
-        g = Generation(worlds, [self.pop.spec(w) for w in worlds], stream)
+        g = Generation(worlds, [self.pop.spec(w) for w in worlds], stream, [self.saved.pop(w, None) for w in worlds])
         self.generations.append(g)
         self.used += g.bytes
         for i, w in enumerate(worlds):
@@ -154,7 +164,11 @@ class ServingEngine:
         while True:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
-                waiting.append(pending.popleft())
+                r = pending.popleft()
+                if self._estimate(r.world) > self.budget:
+                    self.refused += 1  # the world can never be resident
+                    continue
+                waiting.append(r)
             # admit new worlds (grouped into one batch per iteration), queue steps on resident ones
             new_worlds = []
             still = collections.deque()
@@ -179,8 +193,13 @@ class ServingEngine:
             if not active:
                 if not pending and not waiting:
                     break
-                if pending and not waiting:
-                    time.sleep(max(0.0, min(pending[0].arrival - now, 0.001)))
+                if waiting:
+                    # nothing runs and the waiting worlds could not be admitted even
+                    # with every idle batch evicted: refuse them rather than spin
+                    self.refused += len(waiting)
+                    waiting = collections.deque()
+                    continue
+                time.sleep(max(0.0, min(pending[0].arrival - now, 0.001)))
                 continue
             # one step for every world that owes steps: one launch set per batch, batches overlap
             per_gen: dict[int, list] = collections.defaultdict(list)
@@ -198,6 +217,7 @@ class ServingEngine:
             for w, res in active:
                 res.owed -= 1
                 res.steps_done += 1
+                self.total_steps[w] += 1
                 res.last_used = now
                 self.body_substeps += res.batch.bodies[res.index] * self.substeps
                 # the head request completes when its own steps are done
@@ -225,8 +245,17 @@ class ServingEngine:
                            "mean": float(lat.mean()) if lat.size else None},
             "body_substeps_per_s": self.body_substeps / wall, "iterations": self.iterations,
             "resident_worlds": len(self.resident), "batches": len(self.generations),
-            "hbm_used_gib": self.used / 2**30, "evicted_batches": self.evicted,
+            "hbm_used_gib": self.used / 2**30, "evicted_batches": self.evicted, "refused": self.refused,
         }
+
+    def world_step(self, world: int) -> int:
+        """The world's step counter: from its device batch when resident, else from
+        its exported state (0 for a world never admitted)."""
+        res = self.resident.get(world)
+        if res is not None:
+            return res.batch.batch.counters(res.index)["step"]
+        snap = self.saved.get(world)
+        return snapshot.parse_one(snap).step if snap is not None else 0
 
     def close(self):
         for g in self.generations:
