@@ -245,6 +245,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-worlds", type=int, default=2000)
     ap.add_argument("--serve-budget-gib", type=float, default=200.0)
+    ap.add_argument("--serve-max-backlog", type=float, default=2.0, metavar="SECONDS",
+                    help="admission control: refuse requests that would queue more than this many seconds "
+                         "of work at the measured throughput (0: admit everything)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain elementwise copy of MIB MiB (HBM counter calibration)")
     ap.add_argument("--kernel-profile", action="store_true",
@@ -630,8 +633,13 @@ def serve(args):
         pop.spec(int(np.nonzero(pop.template_of == t)[0][0]))
     log(f"[serve] {len(reqs)} requests over {args.serve:.0f}s, population {len(pop.sizes)} worlds, templates "
         f"built in {time.time() - t0:.1f}s")
-    eng = serving.ServingEngine(pop, int(args.serve_budget_gib * 2**30))
+    eng = serving.ServingEngine(pop, int(args.serve_budget_gib * 2**30),
+                                max_backlog_s=args.serve_max_backlog if args.serve_max_backlog > 0 else None)
+    t1 = time.time()
+    preloaded = eng.preload()
+    log(f"[serve] population resident before the first request: {preloaded} ({time.time() - t1:.1f}s)")
     st = eng.run(reqs, horizon=args.serve, log=log)
+    st["preloaded"] = preloaded
     eng.close()
     out = {
         "metric": METRIC, "value": st["body_substeps_per_s"], "unit": "body-substeps/s", "n_gpus": 1,
@@ -640,7 +648,8 @@ def serve(args):
         "data": "synthetic: seeded container worlds (circles, capsules, polygons), Poisson arrivals",
         "config": {"workload": f"config5 serving: Poisson arrivals at {args.serve_rate:g} req/s for {args.serve:g}s, "
                                f"{args.serve_worlds} worlds log-uniform in [50, 20000] bodies, k in {{1, 8, 60}} "
-                               f"steps per request, HBM budget {args.serve_budget_gib:g} GiB",
+                               f"steps per request, HBM budget {args.serve_budget_gib:g} GiB, admission control "
+                               f"at {args.serve_max_backlog:g} s of queued work",
                    "parallelism": "continuous batching: one step launch set per resident batch per iteration, "
                                   "batches on separate HIP streams"},
         "latency_ms": st["latency_ms"], "serving": st,

@@ -830,9 +830,12 @@ static int commit(Batch& b) {
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
     F2D_HIP(hipMalloc(&b.dOrder, sizeof(int) * std::max(W, 1)));
     F2D_HIP(hipMemcpy(b.dOrder, order.data(), sizeof(int) * W, hipMemcpyHostToDevice));
-    // size classes by max(bodies, shapes): 64, 128, ..., 1024, and everything larger
-    const int limits[6] = {64, 128, 256, 512, 1024, 1 << 30};
-    std::vector<std::vector<int>> members(6);
+    // size classes by max(bodies, shapes): 64, 128, ..., 8192, and everything larger
+    // (the classes of large worlds stay narrow too: one class's launches size their
+    // grids and LDS for its largest world)
+    constexpr int kClassLimits = 9;
+    const int limits[kClassLimits] = {64, 128, 256, 512, 1024, 2048, 4096, 8192, 1 << 30};
+    std::vector<std::vector<int>> members(kClassLimits);
     for (int i : order) {
       const HostWorld& hw = b.worlds[i];
       int size = std::max((int)hw.sims.size(), (int)hw.shapes.size());
@@ -840,8 +843,10 @@ static int commit(Batch& b) {
       while (size > limits[k]) ++k;
       members[k].push_back(i);
     }
-    for (int k = 0; k < 6; ++k) {
+    b.classOf.assign(W, 0);
+    for (int k = 0; k < kClassLimits; ++k) {
       if (members[k].empty()) continue;
+      for (int i : members[k]) b.classOf[i] = (int)b.classes.size();
       SizeClass c;
       c.limit = limits[k];
       c.count = (int)members[k].size();
@@ -1990,10 +1995,39 @@ int f2dg_step_worlds(void* h, const int* worldIds, int n, float dt, int subSteps
     if (worldIds[i] < 0 || worldIds[i] >= (int)b->worlds.size()) return -1;
   }
   if (ensureWorldList(*b, n) != 0) return -1;
-  F2D_HIP(hipMemcpyAsync(b->dWorldList, worldIds, sizeof(int) * n, hipMemcpyHostToDevice, b->stream));
   StepParams sp = makeParams(dt, subSteps, PH_ALL);
+  const int K = (int)b->classes.size();
+  if (K == 0 || (int)b->classOf.size() != (int)b->worlds.size()) {
+    F2D_HIP(hipMemcpyAsync(b->dWorldList, worldIds, sizeof(int) * n, hipMemcpyHostToDevice, b->stream));
+    for (int s = 0; s < steps; ++s) {
+      if (f2d_launch_step(&b->dev.view, &sp, b->dWorldList, n, b->stream, &b->launch) != 0) return -1;
+    }
+    b->deviceNewer = true;
+    return 0;
+  }
+  // the listed worlds grouped by size class, each group with its class's launch
+  // configuration on its class's stream (forked from and joined into the batch's)
+  std::vector<int> start(K + 1, 0), fill(K, 0);
+  for (int i = 0; i < n; ++i) start[b->classOf[worldIds[i]] + 1] += 1;
+  for (int k = 0; k < K; ++k) start[k + 1] += start[k];
+  std::vector<int> grouped(n);
+  for (int i = 0; i < n; ++i) {
+    const int k = b->classOf[worldIds[i]];
+    grouped[start[k] + fill[k]++] = worldIds[i];
+  }
+  F2D_HIP(hipMemcpyAsync(b->dWorldList, grouped.data(), sizeof(int) * n, hipMemcpyHostToDevice, b->stream));
   for (int s = 0; s < steps; ++s) {
-    if (f2d_launch_step(&b->dev.view, &sp, b->dWorldList, n, b->stream, &b->launch) != 0) return -1;
+    F2D_HIP(hipEventRecord(b->forkEvent, b->stream));
+    for (int k = 0; k < K; ++k) {
+      const int cnt = start[k + 1] - start[k];
+      if (cnt == 0) continue;
+      SizeClass& c = b->classes[k];
+      F2D_HIP(hipStreamWaitEvent(c.stream, b->forkEvent, 0));
+      if (f2d_launch_step(&b->dev.view, &sp, b->dWorldList + start[k], cnt, c.stream, &c.launch) != 0) return -1;
+      F2D_HIP(hipEventRecord(c.done, c.stream));
+    }
+    for (int k = 0; k < K; ++k)
+      if (start[k + 1] > start[k]) F2D_HIP(hipStreamWaitEvent(b->stream, b->classes[k].done, 0));
   }
   b->deviceNewer = true;
   return 0;

@@ -99,6 +99,7 @@ struct Batch {
   int dWorldListCap = 0;
   int* dOrder = nullptr;     // every world, largest first: the launch order of full-batch steps
   std::vector<SizeClass> classes;
+  std::vector<int> classOf;  // per world: its index in classes
   hipEvent_t forkEvent = nullptr;
   std::string lastError;
   float contactCapFactor = 6.0f;

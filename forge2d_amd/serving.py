@@ -6,7 +6,11 @@ bounded by an HBM budget; when the budget is full, idle worlds are evicted in
 least-recently-used order, a whole device batch at a time, each world's state
 exported to host memory first); a resident world is continued from its device
 state, and a re-admitted world from its exported state. A request for a world
-that could never fit the budget is refused. Every scheduler iteration launches one step for
+that could never fit the budget is refused, and so is one whose world would then
+owe more steps than `max_backlog_s` seconds of iterations at the iteration time
+measured so far (admission control: latency stays bounded at the rate the engine
+admits). `preload` makes the whole population resident before the first
+request when it fits the budget (HBM is large enough for most populations). Every scheduler iteration launches one step for
 every world that still owes steps — across all resident batches, each on its
 own HIP stream so their kernels overlap — and requests that arrive meanwhile
 join at the next iteration. A request completes when its k-th step has
@@ -99,8 +103,10 @@ class Population:
 
 class ServingEngine:
     def __init__(self, population: Population, hbm_budget: int, dt: float = 1.0 / 60.0, substeps: int = 4,
-                 use_streams: bool = True):
+                 use_streams: bool = True, max_backlog_s: float | None = None, iteration_s: float = 0.02):
         self.pop = population
+        self.max_backlog_s = max_backlog_s
+        self.iteration_s = iteration_s  # mean wall time of a scheduler iteration, measured as it runs
         self.budget = hbm_budget
         self.dt, self.substeps = dt, substeps
         self.resident: dict[int, _Resident] = {}
@@ -168,6 +174,12 @@ class ServingEngine:
                 if self._estimate(r.world) > self.budget:
                     self.refused += 1  # the world can never be resident
                     continue
+                if self.max_backlog_s is not None:
+                    res = self.resident.get(r.world)
+                    owed = (res.owed if res is not None else 0) + sum(q.steps for q in waiting if q.world == r.world)
+                    if (owed + r.steps) * self.iteration_s > self.max_backlog_s:
+                        self.refused += 1  # over capacity: refused rather than queued
+                        continue
                 waiting.append(r)
             # admit new worlds (grouped into one batch per iteration), queue steps on resident ones
             new_worlds = []
@@ -212,8 +224,10 @@ class ServingEngine:
                 launched.append(g)
             for g in launched:
                 g.batch.synchronize()
+            t_prev = now
             now = time.perf_counter() - t0
             self.iterations += 1
+            self.iteration_s = 0.9 * self.iteration_s + 0.1 * (now - t_prev)
             for w, res in active:
                 res.owed -= 1
                 res.steps_done += 1
@@ -246,7 +260,17 @@ class ServingEngine:
             "body_substeps_per_s": self.body_substeps / wall, "iterations": self.iterations,
             "resident_worlds": len(self.resident), "batches": len(self.generations),
             "hbm_used_gib": self.used / 2**30, "evicted_batches": self.evicted, "refused": self.refused,
+            "admitted_per_s": (len(requests) - self.refused) / max(horizon, 1e-9),
+            "iteration_ms": 1e3 * self.iteration_s,
         }
+
+    def preload(self) -> bool:
+        """Make every world of the population resident (one device batch) when the
+        whole population fits the budget; returns whether it did."""
+        worlds = list(range(len(self.pop.sizes)))
+        if sum(self._estimate(w) for w in worlds) > self.budget:
+            return False
+        return self._admit(worlds, 0.0)
 
     def world_step(self, world: int) -> int:
         """The world's step counter: from its device batch when resident, else from

@@ -288,7 +288,8 @@ long f2dg_device_bytes(void* batch);
 /* step every world `steps` times; phases bitmask: 1 pairs, 2 collide, 4 contact state,
  * 8 solve, 16 finalize (0 = all) */
 int f2dg_step(void* batch, float dt, int sub_steps, int steps, int phases);
-/* step only the listed worlds (continuous-batching serving) */
+/* step only the listed worlds (continuous-batching serving); the worlds of each
+ * size class launch with that class's configuration on the class's stream */
 int f2dg_step_worlds(void* batch, const int32_t* world_ids, int n, float dt, int sub_steps, int steps);
 /* the one-world-at-a-time GPU path: a launch per world per step */
 int f2dg_step_each(void* batch, float dt, int sub_steps, int steps);

@@ -117,3 +117,22 @@ def test_oversize_world_is_refused(gpu):
     stats = eng.run(reqs, horizon=0.1)
     assert stats["refused"] == 1 and stats["completed"] == 1
     eng.close()
+
+
+def test_admission_control_bounds_queued_steps(gpu):
+    pop = small_population()
+    reqs = serving.poisson_requests(400.0, 0.5, pop, seed=6, steps_choices=(8, 60))
+    eng = serving.ServingEngine(pop, hbm_budget=8 << 30, max_backlog_s=0.05)
+    assert eng.preload()
+    stats = eng.run(reqs, horizon=0.5)
+    # every request is either refused on arrival or completed; a world never owes
+    # more than the bound allows at the iteration time measured when it was admitted
+    assert stats["refused"] > 0
+    assert stats["completed"] + stats["refused"] == len(reqs)
+    asked = {}
+    for r in reqs:
+        if r.done >= 0:
+            asked[r.world] = asked.get(r.world, 0) + r.steps
+    for w, n in asked.items():
+        assert eng.total_steps[w] == n, w
+    eng.close()
