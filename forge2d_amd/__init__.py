@@ -171,6 +171,30 @@ SIGNATURES = [
     ("f2d_joint_wake_bodies", None, [ctypes.c_int32, _U]),
     ("f2d_joint_get_constraint_force", None, [ctypes.c_int32, _U, _PF]),
     ("f2d_joint_get_constraint_torque", _F, [ctypes.c_int32, _U]),
+    ("f2d_create_prismatic_joint", None, [_U, ctypes.c_int32, _U, ctypes.c_int32, _U, _F, _F, _F, _F, _F, _F, _F, _F,
+                                          _I, _F, _F, _I, _F, _F, _I, _F, _F, _I, _PI]),
+    ("f2d_prismatic_joint_is_spring_enabled", _I, [ctypes.c_int32, _U]),
+    ("f2d_prismatic_joint_enable_spring", None, [ctypes.c_int32, _U, _I]),
+    ("f2d_prismatic_joint_get_spring_hertz", _F, [ctypes.c_int32, _U]),
+    ("f2d_prismatic_joint_set_spring_hertz", None, [ctypes.c_int32, _U, _F]),
+    ("f2d_prismatic_joint_get_spring_damping_ratio", _F, [ctypes.c_int32, _U]),
+    ("f2d_prismatic_joint_set_spring_damping_ratio", None, [ctypes.c_int32, _U, _F]),
+    ("f2d_prismatic_joint_get_target_translation", _F, [ctypes.c_int32, _U]),
+    ("f2d_prismatic_joint_set_target_translation", None, [ctypes.c_int32, _U, _F]),
+    ("f2d_prismatic_joint_is_limit_enabled", _I, [ctypes.c_int32, _U]),
+    ("f2d_prismatic_joint_enable_limit", None, [ctypes.c_int32, _U, _I]),
+    ("f2d_prismatic_joint_get_lower_limit", _F, [ctypes.c_int32, _U]),
+    ("f2d_prismatic_joint_get_upper_limit", _F, [ctypes.c_int32, _U]),
+    ("f2d_prismatic_joint_set_limits", None, [ctypes.c_int32, _U, _F, _F]),
+    ("f2d_prismatic_joint_is_motor_enabled", _I, [ctypes.c_int32, _U]),
+    ("f2d_prismatic_joint_enable_motor", None, [ctypes.c_int32, _U, _I]),
+    ("f2d_prismatic_joint_get_motor_speed", _F, [ctypes.c_int32, _U]),
+    ("f2d_prismatic_joint_set_motor_speed", None, [ctypes.c_int32, _U, _F]),
+    ("f2d_prismatic_joint_get_max_motor_force", _F, [ctypes.c_int32, _U]),
+    ("f2d_prismatic_joint_set_max_motor_force", None, [ctypes.c_int32, _U, _F]),
+    ("f2d_prismatic_joint_get_motor_force", _F, [ctypes.c_int32, _U]),
+    ("f2d_prismatic_joint_get_translation", _F, [ctypes.c_int32, _U]),
+    ("f2d_prismatic_joint_get_speed", _F, [ctypes.c_int32, _U]),
     ("f2d_world_batch", _VP, [_U]),
     ("f2d_set_query_callbacks", None, [_VP, _VP]),
     ("f2d_world_cast_ray_closest", _I, [_U, _F, _F, _F, _F, _U, _U, _U, _U, _PD]),

@@ -333,6 +333,7 @@ int HostWorld::createRevolute(int bodyA, int bodyB, Vec2 la, Vec2 lb, float refA
   j.enableSpring = enableSpring;
   j.enableLimit = enableLimit;
   j.enableMotor = enableMotor;
+  j.type = kRevoluteJoint;
   j.constraintHertz = 60.0f;        // B2_JOINT_CONSTRAINT_HERTZ
   j.constraintDampingRatio = 2.0f;  // B2_JOINT_CONSTRAINT_DAMPING_RATIO
   j.constraintSoftness = {0.0f, 1.0f, 0.0f};
@@ -385,6 +386,33 @@ int HostWorld::createRevolute(int bodyA, int bodyB, Vec2 la, Vec2 lb, float refA
     linkJointIslands(c, id);
     mergeAwakeIslands(c);
   }
+  return id;
+}
+
+// b2CreatePrismaticJoint, S/joint.c:559-610: the same body lists, colour and islands
+// as a revolute joint (they do not depend on the joint's parameters), then the
+// prismatic parameters; the axis is normalized and the reference angle is not
+// clamped.
+int HostWorld::createPrismatic(int bodyA, int bodyB, Vec2 la, Vec2 lb, Vec2 axis, float refAngle,
+                               float targetTranslation, bool enableSpring, float hertz, float damping,
+                               bool enableLimit, float lower, float upper, bool enableMotor, float maxMotorForce,
+                               float motorSpeed, bool collideConnected) {
+  const int id = createRevolute(bodyA, bodyB, la, lb, 0.0f, 0.0f, false, 0.0f, 0.0f, false, 0.0f, 0.0f, false, 0.0f,
+                                0.0f, collideConnected);
+  Joint& j = joints[id];
+  j.type = kPrismaticJoint;
+  j.localAxisA = normalize(axis);
+  j.referenceAngle = refAngle;
+  j.targetAngle = targetTranslation;
+  j.hertz = hertz;
+  j.dampingRatio = damping;
+  j.lowerAngle = lower;
+  j.upperAngle = upper;
+  j.maxMotorTorque = maxMotorForce;
+  j.motorSpeed = motorSpeed;
+  j.enableSpring = enableSpring;
+  j.enableLimit = enableLimit;
+  j.enableMotor = enableMotor;
   return id;
 }
 
@@ -1359,6 +1387,7 @@ static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
     sj.lower_impulse = j.lowerImpulse;
     sj.upper_impulse = j.upperImpulse;
     sj.ref_angle = j.referenceAngle;
+    sj.reserved = j.type == kPrismaticJoint ? 1.0f : 0.0f;  // joint kind: 0 revolute, 1 prismatic
     put(out, sj);
   }
   for (int c = 0; c < kGraphColorCount; ++c) put(out, (int32_t)colorLists[c].size());
@@ -3460,7 +3489,8 @@ int f2d_joint_is_valid(int32_t index1, uint32_t wg) {
 // b2JointType numbering (I/types.h): every joint here is revolute (5)
 int f2d_joint_get_type(int32_t index1, uint32_t wg) {
   int id;
-  return jointWorld(index1, wg, &id) ? 5 : -1;
+  HostWorld* hw = jointWorld(index1, wg, &id);
+  return hw ? hw->joints[id].type : -1;
 }
 
 static void jointBody(const HostWorld& hw, int body, uint32_t wg, int32_t* out) {
@@ -3516,21 +3546,195 @@ void f2d_joint_wake_bodies(int32_t index1, uint32_t wg) {
 }
 
 // b2GetRevoluteJointForce / Torque, S/revolute_joint.c:176-187 (world inv_h of the last step)
+// b2Joint_GetConstraintForce: revolute S/revolute_joint.c (inv_h x linear impulse),
+// prismatic S/prismatic_joint.c:211-227 (perpendicular and axial parts in body A's
+// current frame)
 void f2d_joint_get_constraint_force(int32_t index1, uint32_t wg, float* out) {
   int id;
   HostWorld* hw = jointWorld(index1, wg, &id);
   if (!hw) return;
-  const Vec2 f = mulSV(hw->inv_h, hw->joints[id].linearImpulse);
+  const Joint& j = hw->joints[id];
+  Vec2 f;
+  if (j.type == kPrismaticJoint) {
+    const Vec2 axisA = rotate(hw->sims[j.bodyA].xf.q, j.localAxisA);
+    const Vec2 perpA = leftPerp(axisA);
+    const float perpForce = hw->inv_h * j.linearImpulse.x;
+    const float axialForce = hw->inv_h * (j.motorImpulse + j.lowerImpulse - j.upperImpulse);
+    f = add(mulSV(perpForce, perpA), mulSV(axialForce, axisA));
+  } else {
+    f = mulSV(hw->inv_h, j.linearImpulse);
+  }
   out[0] = f.x;
   out[1] = f.y;
 }
 
+// b2Joint_GetConstraintTorque: prismatic S/prismatic_joint.c:229-232
 float f2d_joint_get_constraint_torque(int32_t index1, uint32_t wg) {
   int id;
   HostWorld* hw = jointWorld(index1, wg, &id);
   if (!hw) return 0.0f;
   const Joint& j = hw->joints[id];
+  if (j.type == kPrismaticJoint) return hw->inv_h * j.linearImpulse.y;
   return hw->inv_h * (j.motorImpulse + j.lowerImpulse - j.upperImpulse);
+}
+
+// ---- prismatic joint parameters (f2d_shim.c:1054-1094 -> S/prismatic_joint.c:12-209).
+// A call on a joint of another type is ignored (getters return 0).
+static Joint* prismaticOf(int32_t index1, uint32_t wg, HostWorld** world = nullptr) {
+  int id;
+  HostWorld* hw = jointWorld(index1, wg, &id);
+  if (!hw || hw->joints[id].type != kPrismaticJoint) return nullptr;
+  if (world) *world = hw;
+  return &hw->joints[id];
+}
+static void prismaticChanged(uint32_t wg) {
+  Batch* b = bodyBatch(wg);
+  if (b) b->hostDirty = true;
+}
+
+int f2d_prismatic_joint_is_spring_enabled(int32_t index1, uint32_t wg) {
+  const Joint* j = prismaticOf(index1, wg);
+  return j ? j->enableSpring : 0;
+}
+void f2d_prismatic_joint_enable_spring(int32_t index1, uint32_t wg, int v) {
+  Joint* j = prismaticOf(index1, wg);
+  if (!j) return;
+  if ((v != 0) != (j->enableSpring != 0)) {
+    j->enableSpring = v != 0;
+    j->springImpulse = 0.0f;
+  }
+  prismaticChanged(wg);
+}
+float f2d_prismatic_joint_get_spring_hertz(int32_t index1, uint32_t wg) {
+  const Joint* j = prismaticOf(index1, wg);
+  return j ? j->hertz : 0.0f;
+}
+void f2d_prismatic_joint_set_spring_hertz(int32_t index1, uint32_t wg, float v) {
+  Joint* j = prismaticOf(index1, wg);
+  if (!j) return;
+  j->hertz = v;
+  prismaticChanged(wg);
+}
+float f2d_prismatic_joint_get_spring_damping_ratio(int32_t index1, uint32_t wg) {
+  const Joint* j = prismaticOf(index1, wg);
+  return j ? j->dampingRatio : 0.0f;
+}
+void f2d_prismatic_joint_set_spring_damping_ratio(int32_t index1, uint32_t wg, float v) {
+  Joint* j = prismaticOf(index1, wg);
+  if (!j) return;
+  j->dampingRatio = v;
+  prismaticChanged(wg);
+}
+float f2d_prismatic_joint_get_target_translation(int32_t index1, uint32_t wg) {
+  const Joint* j = prismaticOf(index1, wg);
+  return j ? j->targetAngle : 0.0f;
+}
+void f2d_prismatic_joint_set_target_translation(int32_t index1, uint32_t wg, float v) {
+  Joint* j = prismaticOf(index1, wg);
+  if (!j) return;
+  j->targetAngle = v;
+  prismaticChanged(wg);
+}
+int f2d_prismatic_joint_is_limit_enabled(int32_t index1, uint32_t wg) {
+  const Joint* j = prismaticOf(index1, wg);
+  return j ? j->enableLimit : 0;
+}
+void f2d_prismatic_joint_enable_limit(int32_t index1, uint32_t wg, int v) {
+  Joint* j = prismaticOf(index1, wg);
+  if (!j) return;
+  if ((v != 0) != (j->enableLimit != 0)) {
+    j->enableLimit = v != 0;
+    j->lowerImpulse = 0.0f;
+    j->upperImpulse = 0.0f;
+  }
+  prismaticChanged(wg);
+}
+float f2d_prismatic_joint_get_lower_limit(int32_t index1, uint32_t wg) {
+  const Joint* j = prismaticOf(index1, wg);
+  return j ? j->lowerAngle : 0.0f;
+}
+float f2d_prismatic_joint_get_upper_limit(int32_t index1, uint32_t wg) {
+  const Joint* j = prismaticOf(index1, wg);
+  return j ? j->upperAngle : 0.0f;
+}
+void f2d_prismatic_joint_set_limits(int32_t index1, uint32_t wg, float lower, float upper) {
+  Joint* j = prismaticOf(index1, wg);
+  if (!j) return;
+  if (lower != j->lowerAngle || upper != j->upperAngle) {
+    j->lowerAngle = minf(lower, upper);
+    j->upperAngle = maxf(lower, upper);
+    j->lowerImpulse = 0.0f;
+    j->upperImpulse = 0.0f;
+  }
+  prismaticChanged(wg);
+}
+int f2d_prismatic_joint_is_motor_enabled(int32_t index1, uint32_t wg) {
+  const Joint* j = prismaticOf(index1, wg);
+  return j ? j->enableMotor : 0;
+}
+void f2d_prismatic_joint_enable_motor(int32_t index1, uint32_t wg, int v) {
+  Joint* j = prismaticOf(index1, wg);
+  if (!j) return;
+  if ((v != 0) != (j->enableMotor != 0)) {
+    j->enableMotor = v != 0;
+    j->motorImpulse = 0.0f;
+  }
+  prismaticChanged(wg);
+}
+float f2d_prismatic_joint_get_motor_speed(int32_t index1, uint32_t wg) {
+  const Joint* j = prismaticOf(index1, wg);
+  return j ? j->motorSpeed : 0.0f;
+}
+void f2d_prismatic_joint_set_motor_speed(int32_t index1, uint32_t wg, float v) {
+  Joint* j = prismaticOf(index1, wg);
+  if (!j) return;
+  j->motorSpeed = v;
+  prismaticChanged(wg);
+}
+float f2d_prismatic_joint_get_max_motor_force(int32_t index1, uint32_t wg) {
+  const Joint* j = prismaticOf(index1, wg);
+  return j ? j->maxMotorTorque : 0.0f;
+}
+void f2d_prismatic_joint_set_max_motor_force(int32_t index1, uint32_t wg, float v) {
+  Joint* j = prismaticOf(index1, wg);
+  if (!j) return;
+  j->maxMotorTorque = v;
+  prismaticChanged(wg);
+}
+float f2d_prismatic_joint_get_motor_force(int32_t index1, uint32_t wg) {
+  HostWorld* hw = nullptr;
+  const Joint* j = prismaticOf(index1, wg, &hw);
+  return j ? hw->inv_h * j->motorImpulse : 0.0f;
+}
+// S/prismatic_joint.c:158-172
+float f2d_prismatic_joint_get_translation(int32_t index1, uint32_t wg) {
+  HostWorld* hw = nullptr;
+  const Joint* j = prismaticOf(index1, wg, &hw);
+  if (!j) return 0.0f;
+  const Xf xfA = hw->sims[j->bodyA].xf, xfB = hw->sims[j->bodyB].xf;
+  const Vec2 axisA = rotate(xfA.q, j->localAxisA);
+  const Vec2 pA = transformPoint(xfA, j->localOriginAnchorA);
+  const Vec2 pB = transformPoint(xfB, j->localOriginAnchorB);
+  return dot(sub(pB, pA), axisA);
+}
+// S/prismatic_joint.c:174-209: bodies outside the awake set move with zero velocity
+float f2d_prismatic_joint_get_speed(int32_t index1, uint32_t wg) {
+  HostWorld* hw = nullptr;
+  const Joint* j = prismaticOf(index1, wg, &hw);
+  if (!j) return 0.0f;
+  const BodySim& a = hw->sims[j->bodyA];
+  const BodySim& b = hw->sims[j->bodyB];
+  const Vec2 axisA = rotate(a.xf.q, j->localAxisA);
+  const Vec2 rA = rotate(a.xf.q, sub(j->localOriginAnchorA, a.localCenter));
+  const Vec2 rB = rotate(b.xf.q, sub(j->localOriginAnchorB, b.localCenter));
+  const Vec2 d = add(sub(b.center, a.center), sub(rB, rA));
+  const bool awakeA = a.setIndex == kAwakeSet, awakeB = b.setIndex == kAwakeSet;
+  const Vec2 vA = awakeA ? hw->states[j->bodyA].v : Vec2{0.0f, 0.0f};
+  const Vec2 vB = awakeB ? hw->states[j->bodyB].v : Vec2{0.0f, 0.0f};
+  const float wA = awakeA ? hw->states[j->bodyA].w : 0.0f;
+  const float wB = awakeB ? hw->states[j->bodyB].w : 0.0f;
+  const Vec2 vRel = sub(add(vB, crossSV(wB, rB)), add(vA, crossSV(wA, rA)));
+  return dot(d, crossSV(wA, axisA)) + dot(axisA, vRel);
 }
 
 float f2d_body_get_mass(int32_t index1, uint32_t wg) {
@@ -3673,6 +3877,30 @@ void f2d_create_revolute_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int3
   int id = hw->createRevolute(ia, ib, {lax, lay}, {lbx, lby}, reference_angle, target_angle, enable_spring != 0, hertz,
                               damping_ratio, enable_limit != 0, lower_angle, upper_angle, enable_motor != 0,
                               max_motor_torque, motor_speed, collide_connected != 0);
+  b->hostDirty = true;
+  out[0] = id + 1;
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+}
+
+// f2d_shim.c:787-814
+void f2d_create_prismatic_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t b_index1, uint32_t b_wg,
+                                float lax, float lay, float lbx, float lby, float axis_x, float axis_y,
+                                float reference_angle, float target_translation, int enable_spring, float hertz,
+                                float damping_ratio, int enable_limit, float lower_translation,
+                                float upper_translation, int enable_motor, float max_motor_force, float motor_speed,
+                                int collide_connected, int32_t* out) {
+  (void)b_wg;
+  out[0] = 0;
+  out[1] = 0;
+  Batch* b = batchOf(w);
+  int ia, ib;
+  HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
+  if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (b->committed) freeDevice(*b);
+  int id = hw->createPrismatic(ia, ib, {lax, lay}, {lbx, lby}, {axis_x, axis_y}, reference_angle, target_translation,
+                               enable_spring != 0, hertz, damping_ratio, enable_limit != 0, lower_translation,
+                               upper_translation, enable_motor != 0, max_motor_force, motor_speed,
+                               collide_connected != 0);
   b->hostDirty = true;
   out[0] = id + 1;
   out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));

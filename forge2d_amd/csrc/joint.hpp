@@ -1,7 +1,8 @@
 // Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
-// Revolute joint stages, one joint per lane, in the reference's operation order:
-// prepare S/joint.c:1348-1355 + S/revolute_joint.c:202-259, warm start :261-287,
-// solve :289-465. Static bodies read the identity body state as in the reference.
+// Revolute and prismatic joint stages, one joint per lane, in the reference's
+// operation order: revolute prepare S/joint.c:1348-1355 + S/revolute_joint.c:202-259,
+// warm start :261-287, solve :289-465; prismatic below. Static bodies read the
+// identity body state as in the reference.
 #pragma once
 
 #include "solver.hpp"
@@ -198,6 +199,236 @@ F2D_HD void solveRevolute(Joint& j, BodyState* states, float h, float inv_h, boo
   }
   storeVel(states, ia, vA, wA);
   storeVel(states, ib, vB, wB);
+}
+
+// Prismatic joint stages, S/prismatic_joint.c: prepare :279-347 (with the common
+// part of S/joint.c:1348-1355), warm start :349-392, solve :394-631.
+F2D_HD void preparePrismatic(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+  float hertz = minf(j.constraintHertz, 0.25f * inv_h);
+  j.constraintSoftness = makeSoft(hertz, j.constraintDampingRatio, h);
+  const BodySim& a = sims[j.bodyA];
+  const BodySim& b = sims[j.bodyB];
+  const float mA = a.invMass, iA = a.invI, mB = b.invMass, iB = b.invI;
+  j.invMassA = mA;
+  j.invMassB = mB;
+  j.invIA = iA;
+  j.invIB = iB;
+  j.simA = a.setIndex == kAwakeSet ? j.bodyA : -1;
+  j.simB = b.setIndex == kAwakeSet ? j.bodyB : -1;
+  const Rot qA = a.xf.q, qB = b.xf.q;
+  j.anchorA = rotate(qA, sub(j.localOriginAnchorA, a.localCenter));
+  j.anchorB = rotate(qB, sub(j.localOriginAnchorB, b.localCenter));
+  j.axisA = rotate(qA, j.localAxisA);
+  j.deltaCenter = sub(b.center, a.center);
+  j.deltaAngle = relativeAngle(qB, qA) - j.referenceAngle;
+  j.deltaAngle = unwindAngle(j.deltaAngle);
+  const Vec2 rA = j.anchorA, rB = j.anchorB;
+  const Vec2 d = add(j.deltaCenter, sub(rB, rA));
+  const float a1 = cross(add(d, rA), j.axisA);
+  const float a2 = cross(rB, j.axisA);
+  const float k = mA + mB + iA * a1 * a1 + iB * a2 * a2;
+  j.axialMass = k > 0.0f ? 1.0f / k : 0.0f;
+  j.springSoftness = makeSoft(j.hertz, j.dampingRatio, h);
+  if (warmStartScale == 0.0f) {
+    j.linearImpulse = {0.0f, 0.0f};
+    j.springImpulse = 0.0f;
+    j.motorImpulse = 0.0f;
+    j.lowerImpulse = 0.0f;
+    j.upperImpulse = 0.0f;
+  }
+}
+
+F2D_HD void warmStartPrismatic(Joint& j, BodyState* states) {
+  const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+  const int ia = j.simA, ib = j.simB;
+  BodyState sA = loadState(states, ia);
+  BodyState sB = loadState(states, ib);
+  const Vec2 rA = rotate(sA.dq, j.anchorA);
+  const Vec2 rB = rotate(sB.dq, j.anchorB);
+  const Vec2 d = add(add(sub(sB.dp, sA.dp), j.deltaCenter), sub(rB, rA));
+  const Vec2 axisA = rotate(sA.dq, j.axisA);
+  // the impulse acts at body B's anchor
+  const float a1 = cross(add(d, rA), axisA);
+  const float a2 = cross(rB, axisA);
+  const float axialImpulse = j.springImpulse + j.motorImpulse + j.lowerImpulse - j.upperImpulse;
+  const Vec2 perpA = leftPerp(axisA);
+  const float s1 = cross(add(d, rA), perpA);
+  const float s2 = cross(rB, perpA);
+  const float perpImpulse = j.linearImpulse.x;
+  const float angleImpulse = j.linearImpulse.y;
+  const Vec2 P = add(mulSV(axialImpulse, axisA), mulSV(perpImpulse, perpA));
+  const float LA = axialImpulse * a1 + perpImpulse * s1 + angleImpulse;
+  const float LB = axialImpulse * a2 + perpImpulse * s2 + angleImpulse;
+  sA.v = mulSub(sA.v, mA, P);
+  sA.w -= iA * LA;
+  sB.v = mulAdd(sB.v, mB, P);
+  sB.w += iB * LB;
+  storeVel(states, ia, sA.v, sA.w);
+  storeVel(states, ib, sB.v, sB.w);
+}
+
+F2D_HD void solvePrismatic(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
+  const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+  const int ia = j.simA, ib = j.simB;
+  const BodyState sA = loadState(states, ia);
+  const BodyState sB = loadState(states, ib);
+  Vec2 vA = sA.v;
+  float wA = sA.w;
+  Vec2 vB = sB.v;
+  float wB = sB.w;
+  const Vec2 rA = rotate(sA.dq, j.anchorA);
+  const Vec2 rB = rotate(sB.dq, j.anchorB);
+  const Vec2 d = add(add(sub(sB.dp, sA.dp), j.deltaCenter), sub(rB, rA));
+  const Vec2 axisA = rotate(sA.dq, j.axisA);
+  const float translation = dot(axisA, d);
+  // torques of axial forces
+  const float a1 = cross(add(d, rA), axisA);
+  const float a2 = cross(rB, axisA);
+  if (j.enableSpring) {
+    // a real spring: applied in the relax iterations too
+    const float C = translation - j.targetAngle;
+    const float bias = j.springSoftness.biasRate * C;
+    const float massScale = j.springSoftness.massScale;
+    const float impulseScale = j.springSoftness.impulseScale;
+    const float Cdot = dot(axisA, sub(vB, vA)) + a2 * wB - a1 * wA;
+    const float deltaImpulse = -massScale * j.axialMass * (Cdot + bias) - impulseScale * j.springImpulse;
+    j.springImpulse += deltaImpulse;
+    const Vec2 P = mulSV(deltaImpulse, axisA);
+    const float LA = deltaImpulse * a1;
+    const float LB = deltaImpulse * a2;
+    vA = mulSub(vA, mA, P);
+    wA -= iA * LA;
+    vB = mulAdd(vB, mB, P);
+    wB += iB * LB;
+  }
+  if (j.enableMotor) {
+    const float Cdot = dot(axisA, sub(vB, vA)) + a2 * wB - a1 * wA;
+    float impulse = j.axialMass * (j.motorSpeed - Cdot);
+    const float oldImpulse = j.motorImpulse;
+    const float maxImpulse = h * j.maxMotorTorque;
+    j.motorImpulse = clampf(j.motorImpulse + impulse, -maxImpulse, maxImpulse);
+    impulse = j.motorImpulse - oldImpulse;
+    const Vec2 P = mulSV(impulse, axisA);
+    const float LA = impulse * a1;
+    const float LB = impulse * a2;
+    vA = mulSub(vA, mA, P);
+    wA -= iA * LA;
+    vB = mulAdd(vB, mB, P);
+    wB += iB * LB;
+  }
+  if (j.enableLimit) {
+    {
+      // lower limit
+      const float C = translation - j.lowerAngle;
+      float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
+      if (C > 0.0f) {
+        bias = C * inv_h;  // speculative
+      } else if (useBias) {
+        bias = j.constraintSoftness.biasRate * C;
+        massScale = j.constraintSoftness.massScale;
+        impulseScale = j.constraintSoftness.impulseScale;
+      }
+      const float oldImpulse = j.lowerImpulse;
+      const float Cdot = dot(axisA, sub(vB, vA)) + a2 * wB - a1 * wA;
+      float impulse = -j.axialMass * massScale * (Cdot + bias) - impulseScale * oldImpulse;
+      j.lowerImpulse = maxf(oldImpulse + impulse, 0.0f);
+      impulse = j.lowerImpulse - oldImpulse;
+      const Vec2 P = mulSV(impulse, axisA);
+      const float LA = impulse * a1;
+      const float LB = impulse * a2;
+      vA = mulSub(vA, mA, P);
+      wA -= iA * LA;
+      vB = mulAdd(vB, mB, P);
+      wB += iB * LB;
+    }
+    {
+      // upper limit, signs flipped so that C and the impulse stay positive
+      const float C = j.upperAngle - translation;
+      float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
+      if (C > 0.0f) {
+        bias = C * inv_h;
+      } else if (useBias) {
+        bias = j.constraintSoftness.biasRate * C;
+        massScale = j.constraintSoftness.massScale;
+        impulseScale = j.constraintSoftness.impulseScale;
+      }
+      const float oldImpulse = j.upperImpulse;
+      const float Cdot = dot(axisA, sub(vA, vB)) + a1 * wA - a2 * wB;
+      float impulse = -j.axialMass * massScale * (Cdot + bias) - impulseScale * oldImpulse;
+      j.upperImpulse = maxf(oldImpulse + impulse, 0.0f);
+      impulse = j.upperImpulse - oldImpulse;
+      const Vec2 P = mulSV(impulse, axisA);
+      const float LA = impulse * a1;
+      const float LB = impulse * a2;
+      vA = mulAdd(vA, mA, P);
+      wA += iA * LA;
+      vB = mulSub(vB, mB, P);
+      wB -= iB * LB;
+    }
+  }
+  {
+    // the perpendicular and angular constraints as one 2x2 block
+    const Vec2 perpA = leftPerp(axisA);
+    const float s1 = cross(add(d, rA), perpA);
+    const float s2 = cross(rB, perpA);
+    Vec2 Cdot;
+    Cdot.x = dot(perpA, sub(vB, vA)) + s2 * wB - s1 * wA;
+    Cdot.y = wB - wA;
+    Vec2 bias = {0.0f, 0.0f};
+    float massScale = 1.0f, impulseScale = 0.0f;
+    if (useBias) {
+      Vec2 C;
+      C.x = dot(perpA, d);
+      C.y = relativeAngle(sB.dq, sA.dq) + j.deltaAngle;
+      bias = mulSV(j.constraintSoftness.biasRate, C);
+      massScale = j.constraintSoftness.massScale;
+      impulseScale = j.constraintSoftness.impulseScale;
+    }
+    const float k11 = mA + mB + iA * s1 * s1 + iB * s2 * s2;
+    const float k12 = iA * s1 + iB * s2;
+    float k22 = iA + iB;
+    if (k22 == 0.0f) k22 = 1.0f;  // bodies with fixed rotation
+    // b2Mat22 {cx = (k11, k12), cy = (k12, k22)}
+    const Vec2 b = solve22(k11, k12, k12, k22, add(Cdot, bias));
+    Vec2 impulse;
+    impulse.x = -massScale * b.x - impulseScale * j.linearImpulse.x;
+    impulse.y = -massScale * b.y - impulseScale * j.linearImpulse.y;
+    j.linearImpulse.x += impulse.x;
+    j.linearImpulse.y += impulse.y;
+    const Vec2 P = mulSV(impulse.x, perpA);
+    const float LA = impulse.x * s1 + impulse.y;
+    const float LB = impulse.x * s2 + impulse.y;
+    vA = mulSub(vA, mA, P);
+    wA -= iA * LA;
+    vB = mulAdd(vB, mB, P);
+    wB += iB * LB;
+  }
+  storeVel(states, ia, vA, wA);
+  storeVel(states, ib, vB, wB);
+}
+
+// The joint stages by joint type (b2PrepareJointsTask / b2WarmStartJoint /
+// b2SolveJoint, S/joint.c:1348-1450)
+F2D_HD void prepareJoint(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+  if (j.type == kPrismaticJoint) {
+    preparePrismatic(j, sims, h, inv_h, warmStartScale);
+  } else {
+    prepareRevolute(j, sims, h, inv_h, warmStartScale);
+  }
+}
+F2D_HD void warmStartJoint(Joint& j, BodyState* states) {
+  if (j.type == kPrismaticJoint) {
+    warmStartPrismatic(j, states);
+  } else {
+    warmStartRevolute(j, states);
+  }
+}
+F2D_HD void solveJoint(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
+  if (j.type == kPrismaticJoint) {
+    solvePrismatic(j, states, h, inv_h, useBias);
+  } else {
+    solveRevolute(j, states, h, inv_h, useBias);
+  }
 }
 
 }  // namespace f2d

@@ -152,6 +152,14 @@ struct alignas(16) Contact {
 };
 
 // Revolute joint slot (b2Joint + b2JointSim + b2RevoluteJoint, S/joint.h:27-60, 162-190, 247-281)
+// b2JointType values of the joints built here (I/types.h)
+enum JointType : int { kPrismaticJoint = 4, kRevoluteJoint = 5 };
+
+// One joint (b2Joint + b2JointSim, S/joint.h). The revolute and prismatic joints
+// share the angle fields: for a prismatic joint targetAngle holds the target
+// translation, lowerAngle/upperAngle the translation limits, maxMotorTorque the
+// maximum motor force and linearImpulse the (perpendicular, angular) impulse of
+// its block constraint (S/joint.h b2PrismaticJoint).
 struct alignas(16) Joint {
   int bodyA, bodyB;
   int color, localIndex;
@@ -171,7 +179,9 @@ struct alignas(16) Joint {
   int islandId, islandPrev, islandNext;  // island graph (b2Joint, S/joint.h:44-48)
   int setIndex;
   int simA, simB;  // solver state index of each body, -1 outside the awake set (set by prepare)
-  int pad[2];
+  int type;        // JointType
+  int pad;
+  Vec2 localAxisA, axisA;  // prismatic: unit axis in body A's frame; rotated by prepare
 };
 
 // Per-world header. Offsets index the batch's flat arrays.

@@ -1418,7 +1418,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
 
     // -- prepare joints (all), S/solver.c:967-987 -> S/joint.c:1348-1402
     for (int i = threadIdx.x; i < h->jointCount; i += BLOCK) {
-      if (jts[i].color >= 0) prepareRevolute(jts[i], w.sims, ctx.h, ctx.inv_h, ctx.warmStartScale);
+      if (jts[i].color >= 0) prepareJoint(jts[i], w.sims, ctx.h, ctx.inv_h, ctx.warmStartScale);
     }
     // -- prepare contacts (all lanes), S/contact_solver.c:1438-1669
     for (int lane = threadIdx.x; lane < totalLanes; lane += BLOCK) {
@@ -1471,7 +1471,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
       if (anyOverflow) {
         if (threadIdx.x == 0) {
           for (int k = 0; k < overflowJoints; ++k)
-            if (overflowJointList[k] >= 0) warmStartRevolute(jts[overflowJointList[k]], states);
+            if (overflowJointList[k] >= 0) warmStartJoint(jts[overflowJointList[k]], states);
           for (int k = 0; k < overflowCount; ++k) overflowWarmStart(cons, overflowStart + k, states);
         }
         __syncthreads();
@@ -1480,7 +1480,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
         if (cnt + jcnt == 0) continue;
         for (int k = threadIdx.x; k < jcnt; k += BLOCK)
-          warmStartRevolute(jts[jointLanes[sh.jointColorStart[col] + k]], states);
+          warmStartJoint(jts[jointLanes[sh.jointColorStart[col] + k]], states);
         for (int k = threadIdx.x; k < cnt; k += BLOCK) warmStartLane(cons, sh.colorStart[col] + k, states);
         __syncthreads();
       }
@@ -1489,7 +1489,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         if (threadIdx.x == 0) {
           for (int k = 0; k < overflowJoints; ++k)
             if (overflowJointList[k] >= 0)
-              solveRevolute(jts[overflowJointList[k]], states, ctx.h, ctx.inv_h, true);
+              solveJoint(jts[overflowJointList[k]], states, ctx.h, ctx.inv_h, true);
           for (int k = 0; k < overflowCount; ++k)
             overflowSolve(cons, overflowStart + k, states, ctx.inv_h, ctx.maxContactPushSpeed, true);
         }
@@ -1499,7 +1499,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
         if (cnt + jcnt == 0) continue;
         for (int k = threadIdx.x; k < jcnt; k += BLOCK)
-          solveRevolute(jts[jointLanes[sh.jointColorStart[col] + k]], states, ctx.h, ctx.inv_h, true);
+          solveJoint(jts[jointLanes[sh.jointColorStart[col] + k]], states, ctx.h, ctx.inv_h, true);
         for (int k = threadIdx.x; k < cnt; k += BLOCK)
           solveLane(cons, sh.colorStart[col] + k, states, ctx.inv_h, -ctx.contactSpeed, true);
         __syncthreads();
@@ -1517,7 +1517,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         if (threadIdx.x == 0) {
           for (int k = 0; k < overflowJoints; ++k)
             if (overflowJointList[k] >= 0)
-              solveRevolute(jts[overflowJointList[k]], states, ctx.h, ctx.inv_h, false);
+              solveJoint(jts[overflowJointList[k]], states, ctx.h, ctx.inv_h, false);
           for (int k = 0; k < overflowCount; ++k)
             overflowSolve(cons, overflowStart + k, states, ctx.inv_h, ctx.maxContactPushSpeed, false);
         }
@@ -1527,7 +1527,7 @@ __device__ void solveAndFinalize(const WorldPtrs& w, SharedState& sh, BodyState*
         int cnt = sh.colorCount[col], jcnt = sh.jointColorCount[col];
         if (cnt + jcnt == 0) continue;
         for (int k = threadIdx.x; k < jcnt; k += BLOCK)
-          solveRevolute(jts[jointLanes[sh.jointColorStart[col] + k]], states, ctx.h, ctx.inv_h, false);
+          solveJoint(jts[jointLanes[sh.jointColorStart[col] + k]], states, ctx.h, ctx.inv_h, false);
         for (int k = threadIdx.x; k < cnt; k += BLOCK)
           solveLane(cons, sh.colorStart[col] + k, states, ctx.inv_h, -ctx.contactSpeed, false);
         __syncthreads();
@@ -1834,7 +1834,7 @@ __global__ __launch_bounds__(256) void wideStageKernel(BatchView b, StepParams s
   if (stage == WS_PREPARE) {
     // S/solver.c:967-987, S/contact_solver.c:1438-1669
     for (int i = gtid; i < h->jointCount; i += stride)
-      if (jts[i].color >= 0) prepareRevolute(jts[i], w.sims, ctx.h, ctx.inv_h, ctx.warmStartScale);
+      if (jts[i].color >= 0) prepareJoint(jts[i], w.sims, ctx.h, ctx.inv_h, ctx.warmStartScale);
     const int totalLanes = table[kWideColorStart + kGraphColorCount];
     for (int lane = gtid; lane < totalLanes; lane += stride) {
       const int slot = cons.geti(CN_SLOT, lane);
@@ -1885,12 +1885,12 @@ __global__ __launch_bounds__(256) void wideStageKernel(BatchView b, StepParams s
       for (int k = 0; k < cnt; ++k) overflowRestitution(cons, start + k, states, ctx.restitutionThreshold);
     } else if (stage == WS_WARM) {
       for (int k = 0; k < jcnt; ++k)
-        if (jointList[k] >= 0) warmStartRevolute(jts[jointList[k]], states);
+        if (jointList[k] >= 0) warmStartJoint(jts[jointList[k]], states);
       for (int k = 0; k < cnt; ++k) overflowWarmStart(cons, start + k, states);
     } else {
       const bool useBias = stage == WS_SOLVE;
       for (int k = 0; k < jcnt; ++k)
-        if (jointList[k] >= 0) solveRevolute(jts[jointList[k]], states, ctx.h, ctx.inv_h, useBias);
+        if (jointList[k] >= 0) solveJoint(jts[jointList[k]], states, ctx.h, ctx.inv_h, useBias);
       for (int k = 0; k < cnt; ++k)
         overflowSolve(cons, start + k, states, ctx.inv_h, ctx.maxContactPushSpeed, useBias);
     }
@@ -1900,12 +1900,12 @@ __global__ __launch_bounds__(256) void wideStageKernel(BatchView b, StepParams s
     if (stage == WS_RESTITUTION) {
       for (int k = gtid; k < cnt; k += stride) restitutionLane(cons, start + k, states, ctx.restitutionThreshold);
     } else if (stage == WS_WARM) {
-      for (int k = gtid; k < jcnt; k += stride) warmStartRevolute(jts[jointLanes[jstart + k]], states);
+      for (int k = gtid; k < jcnt; k += stride) warmStartJoint(jts[jointLanes[jstart + k]], states);
       for (int k = gtid; k < cnt; k += stride) warmStartLane(cons, start + k, states);
     } else {
       const bool useBias = stage == WS_SOLVE;
       for (int k = gtid; k < jcnt; k += stride)
-        solveRevolute(jts[jointLanes[jstart + k]], states, ctx.h, ctx.inv_h, useBias);
+        solveJoint(jts[jointLanes[jstart + k]], states, ctx.h, ctx.inv_h, useBias);
       for (int k = gtid; k < cnt; k += stride)
         solveLane(cons, start + k, states, ctx.inv_h, -ctx.contactSpeed, useBias);
     }

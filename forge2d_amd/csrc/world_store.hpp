@@ -64,6 +64,9 @@ struct HostWorld {
   int createRevolute(int bodyA, int bodyB, Vec2 la, Vec2 lb, float refAngle, float targetAngle, bool enableSpring,
                      float hertz, float damping, bool enableLimit, float lower, float upper, bool enableMotor,
                      float maxMotorTorque, float motorSpeed, bool collideConnected);
+  int createPrismatic(int bodyA, int bodyB, Vec2 la, Vec2 lb, Vec2 axis, float refAngle, float targetTranslation,
+                      bool enableSpring, float hertz, float damping, bool enableLimit, float lower, float upper,
+                      bool enableMotor, float maxMotorForce, float motorSpeed, bool collideConnected);
   void updateBodyMass(int body);
 };
 

@@ -593,17 +593,36 @@ static void write_snapshot( FILE* f, b2WorldId wid, int step )
 			sj.local_index = joint->localIndex;
 			sj.body_a = joint->edges[0].bodyId;
 			sj.body_b = joint->edges[1].bodyId;
-			const b2RevoluteJoint* rj = &js->revoluteJoint;
-			sj.enable_limit = rj->enableLimit;
-			sj.enable_motor = rj->enableMotor;
-			sj.enable_spring = rj->enableSpring;
-			sj.linear_impulse_x = rj->linearImpulse.x;
-			sj.linear_impulse_y = rj->linearImpulse.y;
-			sj.spring_impulse = rj->springImpulse;
-			sj.motor_impulse = rj->motorImpulse;
-			sj.lower_impulse = rj->lowerImpulse;
-			sj.upper_impulse = rj->upperImpulse;
-			sj.ref_angle = rj->referenceAngle;
+			if ( js->type == b2_prismaticJoint )
+			{
+				// kind 1 in the reserved slot; the block impulse in linear_impulse
+				const b2PrismaticJoint* pj = &js->prismaticJoint;
+				sj.enable_limit = pj->enableLimit;
+				sj.enable_motor = pj->enableMotor;
+				sj.enable_spring = pj->enableSpring;
+				sj.linear_impulse_x = pj->impulse.x;
+				sj.linear_impulse_y = pj->impulse.y;
+				sj.spring_impulse = pj->springImpulse;
+				sj.motor_impulse = pj->motorImpulse;
+				sj.lower_impulse = pj->lowerImpulse;
+				sj.upper_impulse = pj->upperImpulse;
+				sj.ref_angle = pj->referenceAngle;
+				sj.reserved = 1.0f;
+			}
+			else
+			{
+				const b2RevoluteJoint* rj = &js->revoluteJoint;
+				sj.enable_limit = rj->enableLimit;
+				sj.enable_motor = rj->enableMotor;
+				sj.enable_spring = rj->enableSpring;
+				sj.linear_impulse_x = rj->linearImpulse.x;
+				sj.linear_impulse_y = rj->linearImpulse.y;
+				sj.spring_impulse = rj->springImpulse;
+				sj.motor_impulse = rj->motorImpulse;
+				sj.lower_impulse = rj->lowerImpulse;
+				sj.upper_impulse = rj->upperImpulse;
+				sj.ref_angle = rj->referenceAngle;
+			}
 		}
 		fwrite( &sj, sizeof( sj ), 1, f );
 	}
@@ -809,7 +828,9 @@ static uint32_t fbits( float v )
 	return u;
 }
 
-static void apply_op( const OpRec* o, const b2BodyId* bodies, const b2ShapeId* shapes, const b2JointId* joints )
+static int g_jointCount; // joints of the scene plus the ones ops 25 created
+
+static void apply_op( const OpRec* o, const b2BodyId* bodies, const b2ShapeId* shapes, b2JointId* joints )
 {
 	const float* f = o->f;
 	b2BodyId b = bodies[o->target];
@@ -894,6 +915,38 @@ static void apply_op( const OpRec* o, const b2BodyId* bodies, const b2ShapeId* s
 		case 24:
 			b2Joint_SetCollideConnected( joints[o->target], o->flag != 0 );
 			break;
+		case 25:
+		{
+			// a prismatic joint between bodies target and (flag & 0xFFFF), anchors f0..f3,
+			// axis f4, f5, collideConnected flag >> 16, everything else the default def
+			b2PrismaticJointDef def = b2DefaultPrismaticJointDef();
+			def.bodyIdA = b;
+			def.bodyIdB = bodies[o->flag & 0xFFFF];
+			def.localAnchorA = ( b2Vec2 ){ f[0], f[1] };
+			def.localAnchorB = ( b2Vec2 ){ f[2], f[3] };
+			def.localAxisA = ( b2Vec2 ){ f[4], f[5] };
+			def.collideConnected = ( o->flag >> 16 ) != 0;
+			joints[g_jointCount++] = b2CreatePrismaticJoint( b2Body_GetWorld( b ), &def );
+		}
+		break;
+		case 26:
+		{
+			// prismatic joint target: limits f0, f1, motor force f2 and speed f3, spring
+			// hertz f4 and damping f5, then limit / motor / spring on per flag bits 1, 2, 4
+			b2JointId j = joints[o->target];
+			b2PrismaticJoint_SetLimits( j, f[0], f[1] );
+			b2PrismaticJoint_EnableLimit( j, ( o->flag & 1 ) != 0 );
+			b2PrismaticJoint_SetMaxMotorForce( j, f[2] );
+			b2PrismaticJoint_SetMotorSpeed( j, f[3] );
+			b2PrismaticJoint_EnableMotor( j, ( o->flag & 2 ) != 0 );
+			b2PrismaticJoint_SetSpringHertz( j, f[4] );
+			b2PrismaticJoint_SetSpringDampingRatio( j, f[5] );
+			b2PrismaticJoint_EnableSpring( j, ( o->flag & 4 ) != 0 );
+		}
+		break;
+		case 27:
+			b2PrismaticJoint_SetTargetTranslation( joints[o->target], f[0] );
+			break;
 		default:
 			fprintf( stderr, "bad op %d\n", o->op );
 			exit( 3 );
@@ -964,7 +1017,8 @@ int main( int argc, char** argv )
 		const SceneWorld* sw = sc.worlds + wi;
 		b2BodyId* ids = malloc( sizeof( b2BodyId ) * ( sw->hdr.body_count + 1 ) );
 		g_shapeIds = malloc( sizeof( b2ShapeId ) * ( sw->hdr.shape_count + 1 ) );
-		g_jointIds = malloc( sizeof( b2JointId ) * ( sw->hdr.joint_count + 1 ) );
+		g_jointIds = malloc( sizeof( b2JointId ) * ( sw->hdr.joint_count + nops + 1 ) );
+		g_jointCount = sw->hdr.joint_count;
 		b2WorldId wid = build_world( sw, ids );
 		FILE* f = fopen( out, "wb" );
 		int di = 0, oi = 0;

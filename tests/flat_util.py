@@ -1,8 +1,8 @@
 """Scene specs through the flat per-world ABI (test infrastructure): a world built
 with f2d_create_world / f2d_create_body / f2d_create_*_shape /
 f2d_create_revolute_joint call by call, as a forge2d binding would build it, and the
-API calls of an ops fixture (tests/golden/make_golden.py OPS) replayed through the
-f2d_* functions."""
+API calls of an ops fixture (tests/golden/make_golden.py OPS, prismatic joints
+included) replayed through the f2d_* functions."""
 from __future__ import annotations
 
 import ctypes
@@ -115,7 +115,7 @@ class FlatWorld:
         L = F.lib()
         code, t, flag = int(op["op"]), int(op["target"]), int(op["flag"])
         f = [float(x) for x in op["f"]]
-        bi, bwg = self.bodies[t] if code not in (12, 13, 14, 23) else (0, 0)
+        bi, bwg = self.bodies[t] if code not in (12, 13, 14, 23, 26, 27) else (0, 0)
         if code == 1:
             L.f2d_body_apply_force(bi, bwg, f[0], f[1], f[2], f[3], flag)
         elif code == 2:
@@ -158,6 +158,26 @@ class FlatWorld:
             L.f2d_body_set_awake(bi, bwg, flag)
         elif code == 23:
             L.f2d_joint_wake_bodies(*self.joints[t])
+        elif code == 25:
+            # a prismatic joint from the default def with anchors, axis and collideConnected
+            bb = self.bodies[flag & 0xFFFF]
+            out = (ctypes.c_int32 * 2)()
+            L.f2d_create_prismatic_joint(self.w, bi, bwg, bb[0], bb[1], f[0], f[1], f[2], f[3], f[4], f[5], 0.0,
+                                         0.0, 0, 0.0, 0.0, 0, 0.0, 0.0, 0, 0.0, 0.0, int(flag >> 16 != 0), out)
+            assert out[0] > 0
+            self.joints.append((out[0], ctypes.c_uint32(out[1]).value))
+        elif code == 26:
+            j = self.joints[t]
+            L.f2d_prismatic_joint_set_limits(*j, f[0], f[1])
+            L.f2d_prismatic_joint_enable_limit(*j, int(flag & 1 != 0))
+            L.f2d_prismatic_joint_set_max_motor_force(*j, f[2])
+            L.f2d_prismatic_joint_set_motor_speed(*j, f[3])
+            L.f2d_prismatic_joint_enable_motor(*j, int(flag & 2 != 0))
+            L.f2d_prismatic_joint_set_spring_hertz(*j, f[4])
+            L.f2d_prismatic_joint_set_spring_damping_ratio(*j, f[5])
+            L.f2d_prismatic_joint_enable_spring(*j, int(flag & 4 != 0))
+        elif code == 27:
+            L.f2d_prismatic_joint_set_target_translation(*self.joints[t], f[0])
         else:
             raise ValueError(f"op {code}")
 

@@ -287,7 +287,8 @@ OPS = {"apply_force": 1, "apply_force_to_center": 2, "apply_torque": 3, "apply_l
        "set_linear_velocity": 8, "set_angular_velocity": 9, "set_mass_data": 10, "apply_mass_from_shapes": 11,
        "shape_set_density": 12, "shape_set_friction": 13, "shape_set_restitution": 14, "set_gravity_scale": 15,
        "set_linear_damping": 16, "set_angular_damping": 17, "set_fixed_rotation": 18, "set_bullet": 19,
-       "set_awake": 20, "joint_wake_bodies": 23}
+       "set_awake": 20, "joint_wake_bodies": 23, "create_prismatic": 25, "prismatic_config": 26,
+       "prismatic_target": 27}
 
 
 def ops_array(rows):
@@ -360,9 +361,50 @@ BODY_OPS = [
     (28, "set_transform", 10, 0, [15.5, 5.8, 1.0, 0.0]),
 ]
 
+def prismatic_world():
+    """A scene for prismatic joints (S/prismatic_joint.c), created by API calls: boxes,
+    a circle and a fixed-rotation box on a ground box, two boxes to be joined to each
+    other, and loose boxes that fall on the jointed ones (sleep off)."""
+    w = scenes.World(scenes.world_header(sleep=False, continuous=True))
+    g = w.add_body(scenes.STATIC, 0.0, -1.0)                      # 0
+    w.add_box(g, 30.0, 1.0)                                       # shape 0
+    w.add_box(w.add_body(scenes.DYNAMIC, -6.0, 3.0), 0.5, 0.5)    # 1
+    w.add_box(w.add_body(scenes.DYNAMIC, -2.0, 2.0, angle=0.3), 0.4, 0.6)  # 2
+    w.add_box(w.add_body(scenes.DYNAMIC, 3.0, 1.0), 0.5, 0.3)     # 3
+    w.add_box(w.add_body(scenes.DYNAMIC, 4.5, 1.0), 0.5, 0.3)     # 4
+    w.add_circle(w.add_body(scenes.DYNAMIC, 8.0, 4.0), 0.5)       # 5
+    fixed = scenes.BF_FIXED_ROT | scenes.BF_AWAKE | scenes.BF_ENABLE_SLEEP
+    w.add_box(w.add_body(scenes.DYNAMIC, 11.0, 2.0, flags=fixed), 0.5, 0.5)  # 6
+    for k in range(4):                                            # 7..10: fall onto the jointed bodies
+        w.add_box(w.add_body(scenes.DYNAMIC, -6.2 + 4.7 * k, 6.0 + 0.5 * k, angle=0.2 * k), 0.3, 0.3)
+    return w
+
+
+PRISMATIC_OPS = [
+    # joints 0..5 created before step 1 (flag: body B | collideConnected << 16)
+    (1, "create_prismatic", 0, 1, [-6.0, 0.0, 0.0, -1.0, 0.0, 1.0]),             # 0: vertical slider
+    (1, "prismatic_config", 0, 3, [-1.0, 2.0, 50.0, 1.0, 0.0, 0.0]),             # limit + motor
+    (1, "create_prismatic", 0, 2, [-2.0, 1.0, 0.0, 0.0, 1.0, 1.0]),              # 1: axis (1, 1) normalized
+    (1, "prismatic_config", 1, 4, [0.0, 0.0, 0.0, 0.0, 2.0, 0.5]),               # spring
+    (1, "prismatic_target", 1, 0, [0.5]),
+    (1, "create_prismatic", 3, 4, [0.5, 0.0, -0.5, 0.0, 1.0, 0.0]),              # 2: box to box
+    (1, "prismatic_config", 2, 3, [0.5, 2.0, 20.0, -2.0, 0.0, 0.0]),             # limit + motor
+    (1, "create_prismatic", 0, 5 | (1 << 16), [8.0, 1.0, 0.0, 0.0, 0.3, -1.0]),  # 3: collide connected
+    (1, "prismatic_config", 3, 7, [-3.0, 1.5, 30.0, 0.5, 4.0, 0.7]),             # limit + motor + spring
+    (1, "create_prismatic", 6, 0, [0.0, 0.0, 11.0, 1.0, 1.0, 0.0]),              # 4: fixed rotation on A
+    (1, "prismatic_config", 4, 1, [-2.0, 2.0, 0.0, 0.0, 0.0, 0.0]),
+    # later changes: target, motor reversal, limits moved, spring off, limit off
+    (20, "prismatic_target", 1, 0, [-0.5]),
+    (25, "prismatic_config", 0, 3, [-1.0, 2.0, 50.0, -1.5, 0.0, 0.0]),
+    (30, "prismatic_config", 2, 3, [0.2, 1.0, 20.0, -2.0, 0.0, 0.0]),
+    (35, "prismatic_config", 3, 3, [-3.0, 1.5, 30.0, 0.5, 4.0, 0.7]),
+    (40, "prismatic_config", 4, 0, [-2.0, 2.0, 0.0, 0.0, 0.0, 0.0]),
+]
+
 OPS_FIXTURES = {
     # name: (world factory, ops, dumped steps)
     "ops_bodies": (body_ops_world, BODY_OPS, list(range(0, 41))),
+    "ops_prismatic": (prismatic_world, PRISMATIC_OPS, list(range(0, 61))),
 }
 
 

@@ -261,3 +261,49 @@ def test_ops_fixture_exercises_every_call():
     assert not np.array_equal(by[7].shapes["fat"][1], by[8].shapes["fat"][1])
     # the impulse beyond the speed limit is clipped to the world's 400 m/s (before step 21)
     assert np.hypot(by[21].bodies["vx"][5], by[21].bodies["vy"][5]) < 400.0
+
+
+def test_prismatic_joint_accessors_on_the_host():
+    """f2d_create_prismatic_joint and the f2d_prismatic_joint_* calls on the host
+    (S/prismatic_joint.c:12-209): the step-1 calls of the ops_prismatic fixture made
+    through the flat ABI read back as set (limits ordered by SetLimits, enable flags,
+    target, motor and spring parameters), the joint type is b2_prismaticJoint (4), a
+    call on a revolute joint is ignored, and the translation along the normalized
+    axis matches the bodies' placement."""
+    spec, snaps, ops = load_ops_fixture("ops_prismatic", GOLDEN)
+    fw = FlatWorld(spec)
+    for op in ops[ops["step"] == 1]:
+        fw.apply(op)
+    L = F.lib()
+    assert len(fw.joints) == 5
+    assert [L.f2d_joint_get_type(*j) for j in fw.joints] == [4] * 5
+    j0, j1, j2, j3 = fw.joints[:4]
+    assert (L.f2d_prismatic_joint_is_limit_enabled(*j0), L.f2d_prismatic_joint_is_motor_enabled(*j0),
+            L.f2d_prismatic_joint_is_spring_enabled(*j0)) == (1, 1, 0)
+    assert (L.f2d_prismatic_joint_get_lower_limit(*j0), L.f2d_prismatic_joint_get_upper_limit(*j0)) == (-1.0, 2.0)
+    assert L.f2d_prismatic_joint_get_max_motor_force(*j0) == 50.0 and L.f2d_prismatic_joint_get_motor_speed(*j0) == 1.0
+    assert L.f2d_prismatic_joint_get_spring_hertz(*j1) == 2.0
+    assert L.f2d_prismatic_joint_get_spring_damping_ratio(*j1) == 0.5
+    assert L.f2d_prismatic_joint_get_target_translation(*j1) == 0.5
+    assert L.f2d_joint_get_collide_connected(*j3) == 1 and L.f2d_joint_get_collide_connected(*j2) == 0
+    L.f2d_prismatic_joint_set_limits(*j2, 3.0, -1.0)  # SetLimits orders them
+    assert (L.f2d_prismatic_joint_get_lower_limit(*j2), L.f2d_prismatic_joint_get_upper_limit(*j2)) == (-1.0, 3.0)
+    # joint 0: ground (0, -1) anchor (-6, 0) -> (-6, -1); body 1 at (-6, 3) anchor (0, -1) -> (-6, 2): 3 along +y
+    assert math.isclose(L.f2d_prismatic_joint_get_translation(*j0), 3.0, abs_tol=1e-6)
+    # joint 1's axis (1, 1) is normalized: ground anchor (-2, 1) -> (-2, 0), body 2's centre (-2, 2): 2 / sqrt(2)
+    assert math.isclose(L.f2d_prismatic_joint_get_translation(*j1), math.sqrt(2.0), rel_tol=1e-6)
+    fw.close()
+
+
+def test_prismatic_fixture_exercises_every_branch():
+    """Over the 60 reference steps of ops_prismatic the joints hold lower and upper limit
+    impulses, motor impulses of both signs, spring impulses, an angular block impulse,
+    and the fixed-rotation joint (k22 = 0 replaced by 1) a perpendicular impulse."""
+    spec, snaps, ops = load_ops_fixture("ops_prismatic", GOLDEN)
+    J = np.stack([s.joints for s in snaps[1:]])
+    assert (J["reserved"] == 1.0).all()
+    assert (J["lower_impulse"] > 0).any() and (J["upper_impulse"] > 0).any()
+    assert (J["motor_impulse"] > 0).any() and (J["motor_impulse"] < 0).any()
+    assert (J["spring_impulse"] != 0).any()
+    assert (J["linear_impulse_y"] != 0).any()
+    assert (J["linear_impulse_x"][:, 4] != 0).any()
