@@ -195,6 +195,26 @@ SIGNATURES = [
     ("f2d_prismatic_joint_get_motor_force", _F, [ctypes.c_int32, _U]),
     ("f2d_prismatic_joint_get_translation", _F, [ctypes.c_int32, _U]),
     ("f2d_prismatic_joint_get_speed", _F, [ctypes.c_int32, _U]),
+    ("f2d_create_weld_joint", None, [_U, ctypes.c_int32, _U, ctypes.c_int32, _U, _F, _F, _F, _F, _F, _F, _F, _F, _F,
+                                     _I, _PI]),
+    ("f2d_create_wheel_joint", None, [_U, ctypes.c_int32, _U, ctypes.c_int32, _U, _F, _F, _F, _F, _F, _F, _I, _F, _F,
+                                      _I, _F, _F, _I, _F, _F, _I, _PI]),
+] + [(f"f2d_{k}_joint_get_{n}", _F, [ctypes.c_int32, _U]) for k, n in
+     [("weld", "linear_hertz"), ("weld", "angular_hertz"), ("weld", "linear_damping_ratio"),
+      ("weld", "angular_damping_ratio"), ("wheel", "spring_hertz"), ("wheel", "spring_damping_ratio"),
+      ("wheel", "lower_limit"), ("wheel", "upper_limit"), ("wheel", "motor_speed"), ("wheel", "max_motor_torque"),
+      ("wheel", "motor_torque")]] + [
+    (f"f2d_{k}_joint_set_{n}", None, [ctypes.c_int32, _U, _F]) for k, n in
+    [("weld", "linear_hertz"), ("weld", "angular_hertz"), ("weld", "linear_damping_ratio"),
+     ("weld", "angular_damping_ratio"), ("wheel", "spring_hertz"), ("wheel", "spring_damping_ratio"),
+     ("wheel", "motor_speed"), ("wheel", "max_motor_torque")]] + [
+    ("f2d_wheel_joint_is_spring_enabled", _I, [ctypes.c_int32, _U]),
+    ("f2d_wheel_joint_enable_spring", None, [ctypes.c_int32, _U, _I]),
+    ("f2d_wheel_joint_is_limit_enabled", _I, [ctypes.c_int32, _U]),
+    ("f2d_wheel_joint_enable_limit", None, [ctypes.c_int32, _U, _I]),
+    ("f2d_wheel_joint_set_limits", None, [ctypes.c_int32, _U, _F, _F]),
+    ("f2d_wheel_joint_is_motor_enabled", _I, [ctypes.c_int32, _U]),
+    ("f2d_wheel_joint_enable_motor", None, [ctypes.c_int32, _U, _I]),
     ("f2d_world_batch", _VP, [_U]),
     ("f2d_set_query_callbacks", None, [_VP, _VP]),
     ("f2d_world_cast_ray_closest", _I, [_U, _F, _F, _F, _F, _U, _U, _U, _U, _PD]),

@@ -416,6 +416,42 @@ int HostWorld::createPrismatic(int bodyA, int bodyB, Vec2 la, Vec2 lb, Vec2 axis
   return id;
 }
 
+// b2CreateWeldJoint, S/joint.c:612-649 (reference angle not clamped)
+int HostWorld::createWeld(int bodyA, int bodyB, Vec2 la, Vec2 lb, float refAngle, float linearHertz,
+                          float angularHertz, float linearDamping, float angularDamping, bool collideConnected) {
+  const int id = createRevolute(bodyA, bodyB, la, lb, 0.0f, 0.0f, false, 0.0f, 0.0f, false, 0.0f, 0.0f, false, 0.0f,
+                                0.0f, collideConnected);
+  Joint& j = joints[id];
+  j.type = kWeldJoint;
+  j.referenceAngle = refAngle;
+  j.hertz = linearHertz;
+  j.dampingRatio = linearDamping;
+  j.angularHertz = angularHertz;
+  j.angularDampingRatio = angularDamping;
+  return id;
+}
+
+// b2CreateWheelJoint, S/joint.c:651-700 (axis normalized)
+int HostWorld::createWheel(int bodyA, int bodyB, Vec2 la, Vec2 lb, Vec2 axis, bool enableSpring, float hertz,
+                           float damping, bool enableLimit, float lower, float upper, bool enableMotor,
+                           float maxMotorTorque, float motorSpeed, bool collideConnected) {
+  const int id = createRevolute(bodyA, bodyB, la, lb, 0.0f, 0.0f, false, 0.0f, 0.0f, false, 0.0f, 0.0f, false, 0.0f,
+                                0.0f, collideConnected);
+  Joint& j = joints[id];
+  j.type = kWheelJoint;
+  j.localAxisA = normalize(axis);
+  j.lowerAngle = lower;
+  j.upperAngle = upper;
+  j.maxMotorTorque = maxMotorTorque;
+  j.motorSpeed = motorSpeed;
+  j.hertz = hertz;
+  j.dampingRatio = damping;
+  j.enableSpring = enableSpring;
+  j.enableLimit = enableLimit;
+  j.enableMotor = enableMotor;
+  return id;
+}
+
 IslandCtx HostWorld::islandCtx() {
   // the list-walk bounds read the world's current sizes
   hdr.bodyCount = (int)sims.size();
@@ -1387,7 +1423,8 @@ static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
     sj.lower_impulse = j.lowerImpulse;
     sj.upper_impulse = j.upperImpulse;
     sj.ref_angle = j.referenceAngle;
-    sj.reserved = j.type == kPrismaticJoint ? 1.0f : 0.0f;  // joint kind: 0 revolute, 1 prismatic
+    // joint kind: 0 revolute, 1 prismatic, 2 weld, 3 wheel
+    sj.reserved = j.type == kPrismaticJoint ? 1.0f : (j.type == kWeldJoint ? 2.0f : (j.type == kWheelJoint ? 3.0f : 0.0f));
     put(out, sj);
   }
   for (int c = 0; c < kGraphColorCount; ++c) put(out, (int32_t)colorLists[c].size());
@@ -3561,8 +3598,14 @@ void f2d_joint_get_constraint_force(int32_t index1, uint32_t wg, float* out) {
     const float perpForce = hw->inv_h * j.linearImpulse.x;
     const float axialForce = hw->inv_h * (j.motorImpulse + j.lowerImpulse - j.upperImpulse);
     f = add(mulSV(perpForce, perpA), mulSV(axialForce, axisA));
+  } else if (j.type == kWheelJoint) {
+    // S/wheel_joint.c:147-160: the axis of the last prepare (a frame behind)
+    const Vec2 perpA = leftPerp(j.axisA);
+    const float perpForce = hw->inv_h * j.linearImpulse.x;
+    const float axialForce = hw->inv_h * (j.springImpulse + j.lowerImpulse - j.upperImpulse);
+    f = add(mulSV(perpForce, perpA), mulSV(axialForce, j.axisA));
   } else {
-    f = mulSV(hw->inv_h, j.linearImpulse);
+    f = mulSV(hw->inv_h, j.linearImpulse);  // revolute, weld (S/weld_joint.c:66-70)
   }
   out[0] = f.x;
   out[1] = f.y;
@@ -3575,7 +3618,108 @@ float f2d_joint_get_constraint_torque(int32_t index1, uint32_t wg) {
   if (!hw) return 0.0f;
   const Joint& j = hw->joints[id];
   if (j.type == kPrismaticJoint) return hw->inv_h * j.linearImpulse.y;
+  if (j.type == kWeldJoint) return hw->inv_h * j.springImpulse;  // angular impulse, S/weld_joint.c:72-75
+  if (j.type == kWheelJoint) return hw->inv_h * j.motorImpulse;  // S/wheel_joint.c:162-165
   return hw->inv_h * (j.motorImpulse + j.lowerImpulse - j.upperImpulse);
+}
+
+// ---- weld and wheel joint parameters (f2d_shim.c:1138-1185 -> S/weld_joint.c:14-64,
+// S/wheel_joint.c:12-145). A call on a joint of another type is ignored (getters return 0).
+static Joint* jointOfType(int32_t index1, uint32_t wg, int type, HostWorld** world = nullptr) {
+  int id;
+  HostWorld* hw = jointWorld(index1, wg, &id);
+  if (!hw || hw->joints[id].type != type) return nullptr;
+  if (world) *world = hw;
+  return &hw->joints[id];
+}
+#define F2D_JOINT_FLOAT(getter, setter, type, field)                      \
+  float getter(int32_t index1, uint32_t wg) {                              \
+    const Joint* j = jointOfType(index1, wg, type);                        \
+    return j ? j->field : 0.0f;                                            \
+  }                                                                        \
+  void setter(int32_t index1, uint32_t wg, float v) {                      \
+    Joint* j = jointOfType(index1, wg, type);                              \
+    if (!j) return;                                                        \
+    j->field = v;                                                          \
+    bodyBatch(wg)->hostDirty = true;                                       \
+  }
+F2D_JOINT_FLOAT(f2d_weld_joint_get_linear_hertz, f2d_weld_joint_set_linear_hertz, kWeldJoint, hertz)
+F2D_JOINT_FLOAT(f2d_weld_joint_get_angular_hertz, f2d_weld_joint_set_angular_hertz, kWeldJoint, angularHertz)
+F2D_JOINT_FLOAT(f2d_weld_joint_get_linear_damping_ratio, f2d_weld_joint_set_linear_damping_ratio, kWeldJoint,
+                dampingRatio)
+F2D_JOINT_FLOAT(f2d_weld_joint_get_angular_damping_ratio, f2d_weld_joint_set_angular_damping_ratio, kWeldJoint,
+                angularDampingRatio)
+F2D_JOINT_FLOAT(f2d_wheel_joint_get_spring_hertz, f2d_wheel_joint_set_spring_hertz, kWheelJoint, hertz)
+F2D_JOINT_FLOAT(f2d_wheel_joint_get_spring_damping_ratio, f2d_wheel_joint_set_spring_damping_ratio, kWheelJoint,
+                dampingRatio)
+F2D_JOINT_FLOAT(f2d_wheel_joint_get_motor_speed, f2d_wheel_joint_set_motor_speed, kWheelJoint, motorSpeed)
+F2D_JOINT_FLOAT(f2d_wheel_joint_get_max_motor_torque, f2d_wheel_joint_set_max_motor_torque, kWheelJoint,
+                maxMotorTorque)
+#undef F2D_JOINT_FLOAT
+
+int f2d_wheel_joint_is_spring_enabled(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kWheelJoint);
+  return j ? j->enableSpring : 0;
+}
+void f2d_wheel_joint_enable_spring(int32_t index1, uint32_t wg, int v) {
+  Joint* j = jointOfType(index1, wg, kWheelJoint);
+  if (!j) return;
+  if ((v != 0) != (j->enableSpring != 0)) {
+    j->enableSpring = v != 0;
+    j->springImpulse = 0.0f;
+  }
+  bodyBatch(wg)->hostDirty = true;
+}
+int f2d_wheel_joint_is_limit_enabled(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kWheelJoint);
+  return j ? j->enableLimit : 0;
+}
+void f2d_wheel_joint_enable_limit(int32_t index1, uint32_t wg, int v) {
+  Joint* j = jointOfType(index1, wg, kWheelJoint);
+  if (!j) return;
+  if ((v != 0) != (j->enableLimit != 0)) {
+    j->enableLimit = v != 0;
+    j->lowerImpulse = 0.0f;
+    j->upperImpulse = 0.0f;
+  }
+  bodyBatch(wg)->hostDirty = true;
+}
+float f2d_wheel_joint_get_lower_limit(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kWheelJoint);
+  return j ? j->lowerAngle : 0.0f;
+}
+float f2d_wheel_joint_get_upper_limit(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kWheelJoint);
+  return j ? j->upperAngle : 0.0f;
+}
+void f2d_wheel_joint_set_limits(int32_t index1, uint32_t wg, float lower, float upper) {
+  Joint* j = jointOfType(index1, wg, kWheelJoint);
+  if (!j) return;
+  if (lower != j->lowerAngle || upper != j->upperAngle) {
+    j->lowerAngle = minf(lower, upper);
+    j->upperAngle = maxf(lower, upper);
+    j->lowerImpulse = 0.0f;
+    j->upperImpulse = 0.0f;
+  }
+  bodyBatch(wg)->hostDirty = true;
+}
+int f2d_wheel_joint_is_motor_enabled(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kWheelJoint);
+  return j ? j->enableMotor : 0;
+}
+void f2d_wheel_joint_enable_motor(int32_t index1, uint32_t wg, int v) {
+  Joint* j = jointOfType(index1, wg, kWheelJoint);
+  if (!j) return;
+  if ((v != 0) != (j->enableMotor != 0)) {
+    j->enableMotor = v != 0;
+    j->motorImpulse = 0.0f;
+  }
+  bodyBatch(wg)->hostDirty = true;
+}
+float f2d_wheel_joint_get_motor_torque(int32_t index1, uint32_t wg) {
+  HostWorld* hw = nullptr;
+  const Joint* j = jointOfType(index1, wg, kWheelJoint, &hw);
+  return j ? hw->inv_h * j->motorImpulse : 0.0f;
 }
 
 // ---- prismatic joint parameters (f2d_shim.c:1054-1094 -> S/prismatic_joint.c:12-209).
@@ -3901,6 +4045,46 @@ void f2d_create_prismatic_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int
                                enable_spring != 0, hertz, damping_ratio, enable_limit != 0, lower_translation,
                                upper_translation, enable_motor != 0, max_motor_force, motor_speed,
                                collide_connected != 0);
+  b->hostDirty = true;
+  out[0] = id + 1;
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+}
+
+// f2d_shim.c:844-862
+void f2d_create_weld_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t b_index1, uint32_t b_wg, float lax,
+                           float lay, float lbx, float lby, float reference_angle, float linear_hertz,
+                           float angular_hertz, float linear_damping_ratio, float angular_damping_ratio,
+                           int collide_connected, int32_t* out) {
+  out[0] = 0;
+  out[1] = 0;
+  Batch* b = batchOf(w);
+  int ia, ib;
+  HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
+  if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (b->committed) freeDevice(*b);
+  int id = hw->createWeld(ia, ib, {lax, lay}, {lbx, lby}, reference_angle, linear_hertz, angular_hertz,
+                          linear_damping_ratio, angular_damping_ratio, collide_connected != 0);
+  b->hostDirty = true;
+  out[0] = id + 1;
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+}
+
+// f2d_shim.c:864-888
+void f2d_create_wheel_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t b_index1, uint32_t b_wg, float lax,
+                            float lay, float lbx, float lby, float axis_x, float axis_y, int enable_spring, float hertz,
+                            float damping_ratio, int enable_limit, float lower_translation, float upper_translation,
+                            int enable_motor, float max_motor_torque, float motor_speed, int collide_connected,
+                            int32_t* out) {
+  out[0] = 0;
+  out[1] = 0;
+  Batch* b = batchOf(w);
+  int ia, ib;
+  HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
+  if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (b->committed) freeDevice(*b);
+  int id = hw->createWheel(ia, ib, {lax, lay}, {lbx, lby}, {axis_x, axis_y}, enable_spring != 0, hertz, damping_ratio,
+                           enable_limit != 0, lower_translation, upper_translation, enable_motor != 0,
+                           max_motor_torque, motor_speed, collide_connected != 0);
   b->hostDirty = true;
   out[0] = id + 1;
   out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));

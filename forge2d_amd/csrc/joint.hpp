@@ -1,8 +1,8 @@
 // Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
-// Revolute and prismatic joint stages, one joint per lane, in the reference's
-// operation order: revolute prepare S/joint.c:1348-1355 + S/revolute_joint.c:202-259,
-// warm start :261-287, solve :289-465; prismatic below. Static bodies read the
-// identity body state as in the reference.
+// Revolute, prismatic, weld and wheel joint stages, one joint per lane, in the
+// reference's operation order: revolute prepare S/joint.c:1348-1355 +
+// S/revolute_joint.c:202-259, warm start :261-287, solve :289-465; the others below.
+// Static bodies read the identity body state as in the reference.
 #pragma once
 
 #include "solver.hpp"
@@ -407,27 +407,321 @@ F2D_HD void solvePrismatic(Joint& j, BodyState* states, float h, float inv_h, bo
   storeVel(states, ib, vB, wB);
 }
 
+// Weld joint stages, S/weld_joint.c: prepare :91-163, warm start :165-188, solve :190-275.
+F2D_HD void prepareWeld(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+  float hertz = minf(j.constraintHertz, 0.25f * inv_h);
+  j.constraintSoftness = makeSoft(hertz, j.constraintDampingRatio, h);
+  const BodySim& a = sims[j.bodyA];
+  const BodySim& b = sims[j.bodyB];
+  const float iA = a.invI, iB = b.invI;
+  j.invMassA = a.invMass;
+  j.invMassB = b.invMass;
+  j.invIA = iA;
+  j.invIB = iB;
+  j.simA = a.setIndex == kAwakeSet ? j.bodyA : -1;
+  j.simB = b.setIndex == kAwakeSet ? j.bodyB : -1;
+  const Rot qA = a.xf.q, qB = b.xf.q;
+  j.anchorA = rotate(qA, sub(j.localOriginAnchorA, a.localCenter));
+  j.anchorB = rotate(qB, sub(j.localOriginAnchorB, b.localCenter));
+  j.deltaCenter = sub(b.center, a.center);
+  j.deltaAngle = relativeAngle(qB, qA) - j.referenceAngle;
+  j.deltaAngle = unwindAngle(j.deltaAngle);
+  const float ka = iA + iB;
+  j.axialMass = ka > 0.0f ? 1.0f / ka : 0.0f;
+  j.springSoftness = j.hertz == 0.0f ? j.constraintSoftness : makeSoft(j.hertz, j.dampingRatio, h);
+  j.angularSoftness =
+      j.angularHertz == 0.0f ? j.constraintSoftness : makeSoft(j.angularHertz, j.angularDampingRatio, h);
+  if (warmStartScale == 0.0f) {
+    j.linearImpulse = {0.0f, 0.0f};
+    j.springImpulse = 0.0f;
+  }
+}
+
+F2D_HD void warmStartWeld(Joint& j, BodyState* states) {
+  const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+  const int ia = j.simA, ib = j.simB;
+  BodyState sA = loadState(states, ia);
+  BodyState sB = loadState(states, ib);
+  const Vec2 rA = rotate(sA.dq, j.anchorA);
+  const Vec2 rB = rotate(sB.dq, j.anchorB);
+  sA.v = mulSub(sA.v, mA, j.linearImpulse);
+  sA.w -= iA * (cross(rA, j.linearImpulse) + j.springImpulse);
+  sB.v = mulAdd(sB.v, mB, j.linearImpulse);
+  sB.w += iB * (cross(rB, j.linearImpulse) + j.springImpulse);
+  storeVel(states, ia, sA.v, sA.w);
+  storeVel(states, ib, sB.v, sB.w);
+}
+
+F2D_HD void solveWeld(Joint& j, BodyState* states, bool useBias) {
+  const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+  const int ia = j.simA, ib = j.simB;
+  const BodyState sA = loadState(states, ia);
+  const BodyState sB = loadState(states, ib);
+  Vec2 vA = sA.v;
+  float wA = sA.w;
+  Vec2 vB = sB.v;
+  float wB = sB.w;
+  {
+    // angular constraint
+    float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
+    if (useBias || j.angularHertz > 0.0f) {
+      const float C = relativeAngle(sB.dq, sA.dq) + j.deltaAngle;
+      bias = j.angularSoftness.biasRate * C;
+      massScale = j.angularSoftness.massScale;
+      impulseScale = j.angularSoftness.impulseScale;
+    }
+    const float Cdot = wB - wA;
+    const float impulse = -massScale * j.axialMass * (Cdot + bias) - impulseScale * j.springImpulse;
+    j.springImpulse += impulse;
+    wA -= iA * impulse;
+    wB += iB * impulse;
+  }
+  {
+    // linear constraint
+    const Vec2 rA = rotate(sA.dq, j.anchorA);
+    const Vec2 rB = rotate(sB.dq, j.anchorB);
+    Vec2 bias = {0.0f, 0.0f};
+    float massScale = 1.0f, impulseScale = 0.0f;
+    if (useBias || j.hertz > 0.0f) {
+      const Vec2 C = add(add(sub(sB.dp, sA.dp), sub(rB, rA)), j.deltaCenter);
+      bias = mulSV(j.springSoftness.biasRate, C);
+      massScale = j.springSoftness.massScale;
+      impulseScale = j.springSoftness.impulseScale;
+    }
+    const Vec2 Cdot = sub(add(vB, crossSV(wB, rB)), add(vA, crossSV(wA, rA)));
+    const float k11 = mA + mB + rA.y * rA.y * iA + rB.y * rB.y * iB;
+    const float k12 = -rA.y * rA.x * iA - rB.y * rB.x * iB;
+    const float k22 = mA + mB + rA.x * rA.x * iA + rB.x * rB.x * iB;
+    // b2Mat22 {cx = (k11, k12), cy = (k12, k22)}
+    const Vec2 b = solve22(k11, k12, k12, k22, add(Cdot, bias));
+    const Vec2 impulse = {-massScale * b.x - impulseScale * j.linearImpulse.x,
+                          -massScale * b.y - impulseScale * j.linearImpulse.y};
+    j.linearImpulse = add(j.linearImpulse, impulse);
+    vA = mulSub(vA, mA, impulse);
+    wA -= iA * cross(rA, impulse);
+    vB = mulAdd(vB, mB, impulse);
+    wB += iB * cross(rB, impulse);
+  }
+  storeVel(states, ia, vA, wA);
+  storeVel(states, ib, vB, wB);
+}
+
+// Wheel joint stages, S/wheel_joint.c: prepare :183-263, warm start :265-304, solve :306-491.
+F2D_HD void prepareWheel(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+  float hertz = minf(j.constraintHertz, 0.25f * inv_h);
+  j.constraintSoftness = makeSoft(hertz, j.constraintDampingRatio, h);
+  const BodySim& a = sims[j.bodyA];
+  const BodySim& b = sims[j.bodyB];
+  const float mA = a.invMass, iA = a.invI, mB = b.invMass, iB = b.invI;
+  j.invMassA = mA;
+  j.invMassB = mB;
+  j.invIA = iA;
+  j.invIB = iB;
+  j.simA = a.setIndex == kAwakeSet ? j.bodyA : -1;
+  j.simB = b.setIndex == kAwakeSet ? j.bodyB : -1;
+  const Rot qA = a.xf.q, qB = b.xf.q;
+  j.anchorA = rotate(qA, sub(j.localOriginAnchorA, a.localCenter));
+  j.anchorB = rotate(qB, sub(j.localOriginAnchorB, b.localCenter));
+  j.axisA = rotate(qA, j.localAxisA);
+  j.deltaCenter = sub(b.center, a.center);
+  const Vec2 rA = j.anchorA, rB = j.anchorB;
+  const Vec2 d = add(j.deltaCenter, sub(rB, rA));
+  const Vec2 axisA = j.axisA;
+  const Vec2 perpA = leftPerp(axisA);
+  // perpendicular constraint (keeps the wheel on its line)
+  const float s1 = cross(add(d, rA), perpA);
+  const float s2 = cross(rB, perpA);
+  const float kp = mA + mB + iA * s1 * s1 + iB * s2 * s2;
+  j.perpMass = kp > 0.0f ? 1.0f / kp : 0.0f;
+  // spring constraint
+  const float a1 = cross(add(d, rA), axisA);
+  const float a2 = cross(rB, axisA);
+  const float ka = mA + mB + iA * a1 * a1 + iB * a2 * a2;
+  j.axialMass = ka > 0.0f ? 1.0f / ka : 0.0f;
+  j.springSoftness = makeSoft(j.hertz, j.dampingRatio, h);
+  const float km = iA + iB;
+  j.motorMass = km > 0.0f ? 1.0f / km : 0.0f;
+  if (warmStartScale == 0.0f) {
+    j.linearImpulse.x = 0.0f;
+    j.springImpulse = 0.0f;
+    j.motorImpulse = 0.0f;
+    j.lowerImpulse = 0.0f;
+    j.upperImpulse = 0.0f;
+  }
+}
+
+F2D_HD void warmStartWheel(Joint& j, BodyState* states) {
+  const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+  const int ia = j.simA, ib = j.simB;
+  BodyState sA = loadState(states, ia);
+  BodyState sB = loadState(states, ib);
+  const Vec2 rA = rotate(sA.dq, j.anchorA);
+  const Vec2 rB = rotate(sB.dq, j.anchorB);
+  const Vec2 d = add(add(sub(sB.dp, sA.dp), j.deltaCenter), sub(rB, rA));
+  const Vec2 axisA = rotate(sA.dq, j.axisA);
+  const Vec2 perpA = leftPerp(axisA);
+  const float a1 = cross(add(d, rA), axisA);
+  const float a2 = cross(rB, axisA);
+  const float s1 = cross(add(d, rA), perpA);
+  const float s2 = cross(rB, perpA);
+  const float axialImpulse = j.springImpulse + j.lowerImpulse - j.upperImpulse;
+  const float perpImpulse = j.linearImpulse.x;
+  const Vec2 P = add(mulSV(axialImpulse, axisA), mulSV(perpImpulse, perpA));
+  const float LA = axialImpulse * a1 + perpImpulse * s1 + j.motorImpulse;
+  const float LB = axialImpulse * a2 + perpImpulse * s2 + j.motorImpulse;
+  sA.v = mulSub(sA.v, mA, P);
+  sA.w -= iA * LA;
+  sB.v = mulAdd(sB.v, mB, P);
+  sB.w += iB * LB;
+  storeVel(states, ia, sA.v, sA.w);
+  storeVel(states, ib, sB.v, sB.w);
+}
+
+F2D_HD void solveWheel(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
+  const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+  const int ia = j.simA, ib = j.simB;
+  const BodyState sA = loadState(states, ia);
+  const BodyState sB = loadState(states, ib);
+  Vec2 vA = sA.v;
+  float wA = sA.w;
+  Vec2 vB = sB.v;
+  float wB = sB.w;
+  const bool fixedRotation = (iA + iB == 0.0f);
+  const Vec2 rA = rotate(sA.dq, j.anchorA);
+  const Vec2 rB = rotate(sB.dq, j.anchorB);
+  const Vec2 d = add(add(sub(sB.dp, sA.dp), j.deltaCenter), sub(rB, rA));
+  const Vec2 axisA = rotate(sA.dq, j.axisA);
+  const float translation = dot(axisA, d);
+  const float a1 = cross(add(d, rA), axisA);
+  const float a2 = cross(rB, axisA);
+  if (j.enableMotor && fixedRotation == false) {
+    const float Cdot = wB - wA - j.motorSpeed;
+    float impulse = -j.motorMass * Cdot;
+    const float oldImpulse = j.motorImpulse;
+    const float maxImpulse = h * j.maxMotorTorque;
+    j.motorImpulse = clampf(j.motorImpulse + impulse, -maxImpulse, maxImpulse);
+    impulse = j.motorImpulse - oldImpulse;
+    wA -= iA * impulse;
+    wB += iB * impulse;
+  }
+  if (j.enableSpring) {
+    // a real spring: applied in the relax iterations too
+    const float C = translation;
+    const float bias = j.springSoftness.biasRate * C;
+    const float massScale = j.springSoftness.massScale;
+    const float impulseScale = j.springSoftness.impulseScale;
+    const float Cdot = dot(axisA, sub(vB, vA)) + a2 * wB - a1 * wA;
+    const float impulse = -massScale * j.axialMass * (Cdot + bias) - impulseScale * j.springImpulse;
+    j.springImpulse += impulse;
+    const Vec2 P = mulSV(impulse, axisA);
+    const float LA = impulse * a1;
+    const float LB = impulse * a2;
+    vA = mulSub(vA, mA, P);
+    wA -= iA * LA;
+    vB = mulAdd(vB, mB, P);
+    wB += iB * LB;
+  }
+  if (j.enableLimit) {
+    {
+      // lower limit
+      const float C = translation - j.lowerAngle;
+      float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
+      if (C > 0.0f) {
+        bias = C * inv_h;  // speculative
+      } else if (useBias) {
+        bias = j.constraintSoftness.biasRate * C;
+        massScale = j.constraintSoftness.massScale;
+        impulseScale = j.constraintSoftness.impulseScale;
+      }
+      const float Cdot = dot(axisA, sub(vB, vA)) + a2 * wB - a1 * wA;
+      float impulse = -massScale * j.axialMass * (Cdot + bias) - impulseScale * j.lowerImpulse;
+      const float oldImpulse = j.lowerImpulse;
+      j.lowerImpulse = maxf(oldImpulse + impulse, 0.0f);
+      impulse = j.lowerImpulse - oldImpulse;
+      const Vec2 P = mulSV(impulse, axisA);
+      const float LA = impulse * a1;
+      const float LB = impulse * a2;
+      vA = mulSub(vA, mA, P);
+      wA -= iA * LA;
+      vB = mulAdd(vB, mB, P);
+      wB += iB * LB;
+    }
+    {
+      // upper limit, signs flipped so that C and the impulse stay positive
+      const float C = j.upperAngle - translation;
+      float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
+      if (C > 0.0f) {
+        bias = C * inv_h;
+      } else if (useBias) {
+        bias = j.constraintSoftness.biasRate * C;
+        massScale = j.constraintSoftness.massScale;
+        impulseScale = j.constraintSoftness.impulseScale;
+      }
+      const float Cdot = dot(axisA, sub(vA, vB)) + a1 * wA - a2 * wB;
+      float impulse = -massScale * j.axialMass * (Cdot + bias) - impulseScale * j.upperImpulse;
+      const float oldImpulse = j.upperImpulse;
+      j.upperImpulse = maxf(oldImpulse + impulse, 0.0f);
+      impulse = j.upperImpulse - oldImpulse;
+      const Vec2 P = mulSV(impulse, axisA);
+      const float LA = impulse * a1;
+      const float LB = impulse * a2;
+      vA = mulAdd(vA, mA, P);
+      wA += iA * LA;
+      vB = mulSub(vB, mB, P);
+      wB -= iB * LB;
+    }
+  }
+  {
+    // point to line
+    const Vec2 perpA = leftPerp(axisA);
+    float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
+    if (useBias) {
+      const float C = dot(perpA, d);
+      bias = j.constraintSoftness.biasRate * C;
+      massScale = j.constraintSoftness.massScale;
+      impulseScale = j.constraintSoftness.impulseScale;
+    }
+    const float s1 = cross(add(d, rA), perpA);
+    const float s2 = cross(rB, perpA);
+    const float Cdot = dot(perpA, sub(vB, vA)) + s2 * wB - s1 * wA;
+    const float impulse = -massScale * j.perpMass * (Cdot + bias) - impulseScale * j.linearImpulse.x;
+    j.linearImpulse.x += impulse;
+    const Vec2 P = mulSV(impulse, perpA);
+    const float LA = impulse * s1;
+    const float LB = impulse * s2;
+    vA = mulSub(vA, mA, P);
+    wA -= iA * LA;
+    vB = mulAdd(vB, mB, P);
+    wB += iB * LB;
+  }
+  storeVel(states, ia, vA, wA);
+  storeVel(states, ib, vB, wB);
+}
+
 // The joint stages by joint type (b2PrepareJointsTask / b2WarmStartJoint /
 // b2SolveJoint, S/joint.c:1348-1450)
 F2D_HD void prepareJoint(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
-  if (j.type == kPrismaticJoint) {
-    preparePrismatic(j, sims, h, inv_h, warmStartScale);
-  } else {
-    prepareRevolute(j, sims, h, inv_h, warmStartScale);
+  switch (j.type) {
+    case kPrismaticJoint: preparePrismatic(j, sims, h, inv_h, warmStartScale); break;
+    case kWeldJoint: prepareWeld(j, sims, h, inv_h, warmStartScale); break;
+    case kWheelJoint: prepareWheel(j, sims, h, inv_h, warmStartScale); break;
+    default: prepareRevolute(j, sims, h, inv_h, warmStartScale); break;
   }
 }
 F2D_HD void warmStartJoint(Joint& j, BodyState* states) {
-  if (j.type == kPrismaticJoint) {
-    warmStartPrismatic(j, states);
-  } else {
-    warmStartRevolute(j, states);
+  switch (j.type) {
+    case kPrismaticJoint: warmStartPrismatic(j, states); break;
+    case kWeldJoint: warmStartWeld(j, states); break;
+    case kWheelJoint: warmStartWheel(j, states); break;
+    default: warmStartRevolute(j, states); break;
   }
 }
 F2D_HD void solveJoint(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
-  if (j.type == kPrismaticJoint) {
-    solvePrismatic(j, states, h, inv_h, useBias);
-  } else {
-    solveRevolute(j, states, h, inv_h, useBias);
+  switch (j.type) {
+    case kPrismaticJoint: solvePrismatic(j, states, h, inv_h, useBias); break;
+    case kWeldJoint: solveWeld(j, states, useBias); break;
+    case kWheelJoint: solveWheel(j, states, h, inv_h, useBias); break;
+    default: solveRevolute(j, states, h, inv_h, useBias); break;
   }
 }
 

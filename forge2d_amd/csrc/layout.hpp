@@ -153,13 +153,17 @@ struct alignas(16) Contact {
 
 // Revolute joint slot (b2Joint + b2JointSim + b2RevoluteJoint, S/joint.h:27-60, 162-190, 247-281)
 // b2JointType values of the joints built here (I/types.h)
-enum JointType : int { kPrismaticJoint = 4, kRevoluteJoint = 5 };
+enum JointType : int { kPrismaticJoint = 4, kRevoluteJoint = 5, kWeldJoint = 6, kWheelJoint = 7 };
 
-// One joint (b2Joint + b2JointSim, S/joint.h). The revolute and prismatic joints
-// share the angle fields: for a prismatic joint targetAngle holds the target
-// translation, lowerAngle/upperAngle the translation limits, maxMotorTorque the
-// maximum motor force and linearImpulse the (perpendicular, angular) impulse of
-// its block constraint (S/joint.h b2PrismaticJoint).
+// One joint (b2Joint + b2JointSim, S/joint.h). The joint types share fields
+// (S/joint.h b2PrismaticJoint, b2WeldJoint, b2WheelJoint):
+//   prismatic: targetAngle = target translation, lowerAngle/upperAngle = translation
+//     limits, maxMotorTorque = maximum motor force, linearImpulse = (perpendicular,
+//     angular) impulse of its block constraint;
+//   weld: hertz/dampingRatio = linear stiffness, springSoftness = linear softness,
+//     springImpulse = angular impulse;
+//   wheel: lowerAngle/upperAngle = translation limits, linearImpulse.x =
+//     perpendicular impulse.
 struct alignas(16) Joint {
   int bodyA, bodyB;
   int color, localIndex;
@@ -181,7 +185,11 @@ struct alignas(16) Joint {
   int simA, simB;  // solver state index of each body, -1 outside the awake set (set by prepare)
   int type;        // JointType
   int pad;
-  Vec2 localAxisA, axisA;  // prismatic: unit axis in body A's frame; rotated by prepare
+  Vec2 localAxisA, axisA;  // prismatic, wheel: unit axis in body A's frame; rotated by prepare
+  float angularHertz, angularDampingRatio;  // weld
+  float perpMass, motorMass;                // wheel (prepared)
+  Softness angularSoftness;                 // weld (prepared)
+  int pad2;
 };
 
 // Per-world header. Offsets index the batch's flat arrays.

@@ -67,6 +67,11 @@ struct HostWorld {
   int createPrismatic(int bodyA, int bodyB, Vec2 la, Vec2 lb, Vec2 axis, float refAngle, float targetTranslation,
                       bool enableSpring, float hertz, float damping, bool enableLimit, float lower, float upper,
                       bool enableMotor, float maxMotorForce, float motorSpeed, bool collideConnected);
+  int createWeld(int bodyA, int bodyB, Vec2 la, Vec2 lb, float refAngle, float linearHertz, float angularHertz,
+                 float linearDamping, float angularDamping, bool collideConnected);
+  int createWheel(int bodyA, int bodyB, Vec2 la, Vec2 lb, Vec2 axis, bool enableSpring, float hertz, float damping,
+                  bool enableLimit, float lower, float upper, bool enableMotor, float maxMotorTorque,
+                  float motorSpeed, bool collideConnected);
   void updateBodyMass(int body);
 };
 

@@ -609,6 +609,30 @@ static void write_snapshot( FILE* f, b2WorldId wid, int step )
 				sj.ref_angle = pj->referenceAngle;
 				sj.reserved = 1.0f;
 			}
+			else if ( js->type == b2_weldJoint )
+			{
+				// kind 2; the angular impulse in spring_impulse
+				const b2WeldJoint* wj = &js->weldJoint;
+				sj.linear_impulse_x = wj->linearImpulse.x;
+				sj.linear_impulse_y = wj->linearImpulse.y;
+				sj.spring_impulse = wj->angularImpulse;
+				sj.ref_angle = wj->referenceAngle;
+				sj.reserved = 2.0f;
+			}
+			else if ( js->type == b2_wheelJoint )
+			{
+				// kind 3; the perpendicular impulse in linear_impulse_x
+				const b2WheelJoint* hj = &js->wheelJoint;
+				sj.enable_limit = hj->enableLimit;
+				sj.enable_motor = hj->enableMotor;
+				sj.enable_spring = hj->enableSpring;
+				sj.linear_impulse_x = hj->perpImpulse;
+				sj.spring_impulse = hj->springImpulse;
+				sj.motor_impulse = hj->motorImpulse;
+				sj.lower_impulse = hj->lowerImpulse;
+				sj.upper_impulse = hj->upperImpulse;
+				sj.reserved = 3.0f;
+			}
 			else
 			{
 				const b2RevoluteJoint* rj = &js->revoluteJoint;
@@ -947,6 +971,59 @@ static void apply_op( const OpRec* o, const b2BodyId* bodies, const b2ShapeId* s
 		case 27:
 			b2PrismaticJoint_SetTargetTranslation( joints[o->target], f[0] );
 			break;
+		case 28:
+		{
+			// a weld joint between bodies target and (flag & 0xFFFF), anchors f0..f3,
+			// reference angle f4, collideConnected flag >> 16, stiffness from the default def
+			b2WeldJointDef def = b2DefaultWeldJointDef();
+			def.bodyIdA = b;
+			def.bodyIdB = bodies[o->flag & 0xFFFF];
+			def.localAnchorA = ( b2Vec2 ){ f[0], f[1] };
+			def.localAnchorB = ( b2Vec2 ){ f[2], f[3] };
+			def.referenceAngle = f[4];
+			def.collideConnected = ( o->flag >> 16 ) != 0;
+			joints[g_jointCount++] = b2CreateWeldJoint( b2Body_GetWorld( b ), &def );
+		}
+		break;
+		case 29:
+		{
+			// weld joint target: linear hertz f0, angular hertz f1, linear damping f2, angular damping f3
+			b2JointId j = joints[o->target];
+			b2WeldJoint_SetLinearHertz( j, f[0] );
+			b2WeldJoint_SetAngularHertz( j, f[1] );
+			b2WeldJoint_SetLinearDampingRatio( j, f[2] );
+			b2WeldJoint_SetAngularDampingRatio( j, f[3] );
+		}
+		break;
+		case 30:
+		{
+			// a wheel joint from the default def (spring on, 1 Hz, damping 0.7), anchors
+			// f0..f3, axis f4, f5, collideConnected flag >> 16
+			b2WheelJointDef def = b2DefaultWheelJointDef();
+			def.bodyIdA = b;
+			def.bodyIdB = bodies[o->flag & 0xFFFF];
+			def.localAnchorA = ( b2Vec2 ){ f[0], f[1] };
+			def.localAnchorB = ( b2Vec2 ){ f[2], f[3] };
+			def.localAxisA = ( b2Vec2 ){ f[4], f[5] };
+			def.collideConnected = ( o->flag >> 16 ) != 0;
+			joints[g_jointCount++] = b2CreateWheelJoint( b2Body_GetWorld( b ), &def );
+		}
+		break;
+		case 31:
+		{
+			// wheel joint target: limits f0, f1, motor torque f2 and speed f3, spring hertz
+			// f4 and damping f5, then limit / motor / spring on per flag bits 1, 2, 4
+			b2JointId j = joints[o->target];
+			b2WheelJoint_SetLimits( j, f[0], f[1] );
+			b2WheelJoint_EnableLimit( j, ( o->flag & 1 ) != 0 );
+			b2WheelJoint_SetMaxMotorTorque( j, f[2] );
+			b2WheelJoint_SetMotorSpeed( j, f[3] );
+			b2WheelJoint_EnableMotor( j, ( o->flag & 2 ) != 0 );
+			b2WheelJoint_SetSpringHertz( j, f[4] );
+			b2WheelJoint_SetSpringDampingRatio( j, f[5] );
+			b2WheelJoint_EnableSpring( j, ( o->flag & 4 ) != 0 );
+		}
+		break;
 		default:
 			fprintf( stderr, "bad op %d\n", o->op );
 			exit( 3 );

@@ -288,7 +288,7 @@ OPS = {"apply_force": 1, "apply_force_to_center": 2, "apply_torque": 3, "apply_l
        "shape_set_density": 12, "shape_set_friction": 13, "shape_set_restitution": 14, "set_gravity_scale": 15,
        "set_linear_damping": 16, "set_angular_damping": 17, "set_fixed_rotation": 18, "set_bullet": 19,
        "set_awake": 20, "joint_wake_bodies": 23, "create_prismatic": 25, "prismatic_config": 26,
-       "prismatic_target": 27}
+       "prismatic_target": 27, "create_weld": 28, "weld_config": 29, "create_wheel": 30, "wheel_config": 31}
 
 
 def ops_array(rows):
@@ -401,10 +401,49 @@ PRISMATIC_OPS = [
     (40, "prismatic_config", 4, 0, [-2.0, 2.0, 0.0, 0.0, 0.0, 0.0]),
 ]
 
+def weld_wheel_world():
+    """A scene for weld and wheel joints (S/weld_joint.c, S/wheel_joint.c), created by API
+    calls: a box to weld to the ground, two boxes to weld to each other (one soft
+    weld), a chassis box with two wheels, a box and a circle on a slope of boxes, and
+    loose boxes that fall on them (sleep off)."""
+    w = scenes.World(scenes.world_header(sleep=False, continuous=True))
+    g = w.add_body(scenes.STATIC, 0.0, -1.0)                      # 0
+    w.add_box(g, 30.0, 1.0)                                       # shape 0
+    w.add_box(w.add_body(scenes.DYNAMIC, -8.0, 2.0), 0.5, 0.5)    # 1: welded to the ground
+    w.add_box(w.add_body(scenes.DYNAMIC, -4.0, 3.0), 0.6, 0.2)    # 2
+    w.add_box(w.add_body(scenes.DYNAMIC, -2.6, 3.2, angle=0.4), 0.6, 0.2)  # 3: soft weld to 2
+    w.add_box(w.add_body(scenes.DYNAMIC, 4.0, 1.5), 1.5, 0.3)     # 4: chassis
+    w.add_circle(w.add_body(scenes.DYNAMIC, 3.0, 0.9), 0.4)       # 5: rear wheel
+    w.add_circle(w.add_body(scenes.DYNAMIC, 5.0, 0.9), 0.4)       # 6: front wheel
+    w.add_box(w.add_body(scenes.DYNAMIC, 10.0, 1.0, angle=0.7), 0.4, 0.4)  # 7: welded with a reference angle
+    for k in range(3):                                            # 8..10: fall onto the jointed bodies
+        w.add_box(w.add_body(scenes.DYNAMIC, -7.5 + 6.0 * k, 6.0 + 0.5 * k, angle=0.3 * k), 0.3, 0.3)
+    return w
+
+
+WELD_WHEEL_OPS = [
+    (1, "create_weld", 0, 1, [-8.0, 1.5, 0.0, -0.5, 0.0]),                   # 0: rigid weld to the ground
+    (1, "create_weld", 2, 3, [0.7, 0.0, -0.7, 0.0, 0.4]),                    # 1: box to box
+    (1, "weld_config", 1, 0, [5.0, 3.0, 0.7, 0.5]),                          #    soft in both senses
+    (1, "create_wheel", 4, 5, [-1.0, -0.6, 0.0, 0.0, 0.0, 1.0]),             # 2: rear wheel, default spring
+    (1, "wheel_config", 2, 7, [-0.02, 0.3, 30.0, -6.0, 4.0, 0.7]),           #    limit + motor + spring
+    (1, "create_wheel", 4, 6 | (1 << 16), [1.0, -0.6, 0.0, 0.0, 0.3, 2.0]),  # 3: front wheel, axis normalized
+    (1, "create_weld", 0, 7, [10.0, 2.0, 0.0, 0.0, 0.7]),                    # 4: reference angle
+    (1, "weld_config", 4, 0, [0.0, 2.0, 0.0, 0.3]),                          #    angular spring only
+    # later changes: motor reversed, spring off, a stiffer weld, the limits moved so
+    # that the upper one acts, limit off
+    (25, "wheel_config", 2, 7, [0.05, 0.3, 30.0, 4.0, 4.0, 0.7]),
+    (35, "wheel_config", 3, 0, [0.0, 0.0, 0.0, 0.0, 1.0, 0.7]),
+    (40, "weld_config", 1, 0, [20.0, 10.0, 1.0, 1.0]),
+    (45, "wheel_config", 2, 7, [-0.3, -0.25, 30.0, 4.0, 4.0, 0.7]),
+    (55, "wheel_config", 2, 6, [-0.3, -0.25, 30.0, 4.0, 4.0, 0.7]),
+]
+
 OPS_FIXTURES = {
     # name: (world factory, ops, dumped steps)
     "ops_bodies": (body_ops_world, BODY_OPS, list(range(0, 41))),
     "ops_prismatic": (prismatic_world, PRISMATIC_OPS, list(range(0, 61))),
+    "ops_weld_wheel": (weld_wheel_world, WELD_WHEEL_OPS, list(range(0, 61))),
 }
 
 

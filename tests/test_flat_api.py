@@ -307,3 +307,16 @@ def test_prismatic_fixture_exercises_every_branch():
     assert (J["spring_impulse"] != 0).any()
     assert (J["linear_impulse_y"] != 0).any()
     assert (J["linear_impulse_x"][:, 4] != 0).any()
+
+
+def test_weld_wheel_fixture_exercises_every_branch():
+    """Over the 60 reference steps of ops_weld_wheel: weld linear and angular impulses
+    (rigid and soft), and on the rear wheel motor impulses of both signs, lower and
+    upper limit impulses and spring impulses; the front wheel's perpendicular impulse."""
+    spec, snaps, ops = load_ops_fixture("ops_weld_wheel", GOLDEN)
+    J = np.stack([s.joints for s in snaps[1:]])
+    assert J["reserved"][0].tolist() == [2.0, 2.0, 3.0, 3.0, 2.0]
+    assert (J["linear_impulse_y"][:, 0] != 0).any() and (J["spring_impulse"][:, 1] != 0).any()
+    assert (J["motor_impulse"][:, 2] > 0).any() and (J["motor_impulse"][:, 2] < 0).any()
+    assert (J["lower_impulse"][:, 2] > 0).any() and (J["upper_impulse"][:, 2] > 0).any()
+    assert (J["spring_impulse"][:, 2] != 0).any() and (J["linear_impulse_x"][:, 3] != 0).any()

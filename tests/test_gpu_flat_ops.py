@@ -64,6 +64,54 @@ JOINT_FIELDS = ["color", "local_index", "body_a", "body_b", "enable_limit", "ena
                 "upper_impulse", "ref_angle", "reserved"]
 
 
+def replay_joint_fixture(name, joints):
+    spec, snaps, ops = load_ops_fixture(name, GOLDEN)
+    fw = FlatWorld(spec)
+    k = 0
+    for ref in snaps[1:]:
+        s = ref.step
+        while k < len(ops) and int(ops[k]["step"]) == s:
+            fw.apply(ops[k])
+            k += 1
+        print(f"[{name}] step {s}", flush=True)
+        fw.step()
+        mine = fw.snapshot()
+        for f in P.BODY_FIELDS + ["set_index", "local_index"]:
+            assert np.array_equal(mine.bodies[f], ref.bodies[f]), (s, f)
+        assert len(mine.joints) == len(ref.joints) == joints, s
+        for f in JOINT_FIELDS:
+            assert np.array_equal(mine.joints[f], ref.joints[f]), (s, f)
+        assert contacts_by_id(mine) == contacts_by_id(ref), s
+        rep = P.manifold_report(mine, ref, with_impulses=True)
+        assert rep["point_count"] == rep["ids"] == rep["normals"] == rep["values"] == rep["impulses"] == 0, (s, rep)
+        assert all(np.array_equal(a, c) for a, c in zip(mine.colors, ref.colors)), s
+        assert P.island_differences(mine, ref) == [], s
+    assert k == len(ops)
+    return fw, snaps
+
+
+def test_weld_and_wheel_joints_match_reference(gpu):
+    """Weld and wheel joints (S/weld_joint.c, S/wheel_joint.c) made through
+    f2d_create_weld_joint / f2d_create_wheel_joint and their setters (fixture
+    ops_weld_wheel: a rigid weld to the ground, a weld soft in both senses between
+    two boxes, a weld with a reference angle and an angular spring only, a two-wheel
+    chassis whose rear wheel has limit, motor and spring and whose front wheel takes
+    an unnormalized axis and collideConnected; later a reversed motor, limits moved
+    so that the lower and then the upper one acts, a spring and a limit turned off
+    and a stiffer weld): after every one of 60 steps bodies, contacts, colours,
+    islands and every joint's impulses equal the reference's bit for bit."""
+    fw, snaps = replay_joint_fixture("ops_weld_wheel", 5)
+    import forge2d_amd as F
+    L = F.lib()
+    assert [L.f2d_joint_get_type(*j) for j in fw.joints] == [6, 6, 7, 7, 6]
+    assert L.f2d_weld_joint_get_linear_hertz(*fw.joints[1]) == 20.0
+    assert L.f2d_wheel_joint_is_limit_enabled(*fw.joints[2]) == 0
+    inv_h = np.float32(4.0) * (np.float32(1.0) / np.float32(1.0 / 60.0))
+    assert L.f2d_wheel_joint_get_motor_torque(*fw.joints[2]) == inv_h * snaps[-1].joints["motor_impulse"][2]
+    assert L.f2d_joint_get_constraint_torque(*fw.joints[0]) == inv_h * snaps[-1].joints["spring_impulse"][0]
+    fw.close()
+
+
 def test_prismatic_joints_match_reference(gpu):
     """Prismatic joints (S/prismatic_joint.c) made through f2d_create_prismatic_joint and
     the f2d_prismatic_joint_* setters (fixture ops_prismatic: a vertical slider with
