@@ -208,6 +208,18 @@ SIGNATURES = [
     [("weld", "linear_hertz"), ("weld", "angular_hertz"), ("weld", "linear_damping_ratio"),
      ("weld", "angular_damping_ratio"), ("wheel", "spring_hertz"), ("wheel", "spring_damping_ratio"),
      ("wheel", "motor_speed"), ("wheel", "max_motor_torque")]] + [
+    ("f2d_create_distance_joint", None, [_U, ctypes.c_int32, _U, ctypes.c_int32, _U, _F, _F, _F, _F, _F, _I, _F,
+                                         _F, _I, _F, _F, _I, _F, _F, _I, _PI]),
+] + [(f"f2d_distance_joint_get_{n}", _F, [ctypes.c_int32, _U]) for n in
+     ["length", "current_length", "spring_hertz", "spring_damping_ratio", "min_length", "max_length", "motor_speed",
+      "max_motor_force", "motor_force"]] + [
+    (f"f2d_distance_joint_set_{n}", None, [ctypes.c_int32, _U, _F]) for n in
+    ["length", "spring_hertz", "spring_damping_ratio", "motor_speed", "max_motor_force"]] + [
+    (f"f2d_distance_joint_{n}", _I, [ctypes.c_int32, _U]) for n in
+    ["is_spring_enabled", "is_limit_enabled", "is_motor_enabled"]] + [
+    (f"f2d_distance_joint_{n}", None, [ctypes.c_int32, _U, _I]) for n in
+    ["enable_spring", "enable_limit", "enable_motor"]] + [
+    ("f2d_distance_joint_set_length_range", None, [ctypes.c_int32, _U, _F, _F]),
     ("f2d_wheel_joint_is_spring_enabled", _I, [ctypes.c_int32, _U]),
     ("f2d_wheel_joint_enable_spring", None, [ctypes.c_int32, _U, _I]),
     ("f2d_wheel_joint_is_limit_enabled", _I, [ctypes.c_int32, _U]),

@@ -452,6 +452,28 @@ int HostWorld::createWheel(int bodyA, int bodyB, Vec2 la, Vec2 lb, Vec2 axis, bo
   return id;
 }
 
+// b2CreateDistanceJoint, S/joint.c:353-405: lengths clamped below by the linear slop
+// (B2_LINEAR_SLOP, 0.005 m), the upper bound of the range at least the given minimum
+int HostWorld::createDistance(int bodyA, int bodyB, Vec2 la, Vec2 lb, float length, bool enableSpring, float hertz,
+                              float damping, bool enableLimit, float minLength, float maxLength, bool enableMotor,
+                              float maxMotorForce, float motorSpeed, bool collideConnected) {
+  const int id = createRevolute(bodyA, bodyB, la, lb, 0.0f, 0.0f, false, 0.0f, 0.0f, false, 0.0f, 0.0f, false, 0.0f,
+                                0.0f, collideConnected);
+  Joint& j = joints[id];
+  j.type = kDistanceJoint;
+  j.targetAngle = maxf(length, kLinearSlop);
+  j.hertz = hertz;
+  j.dampingRatio = damping;
+  j.lowerAngle = maxf(minLength, kLinearSlop);
+  j.upperAngle = maxf(minLength, maxLength);
+  j.maxMotorTorque = maxMotorForce;
+  j.motorSpeed = motorSpeed;
+  j.enableSpring = enableSpring;
+  j.enableLimit = enableLimit;
+  j.enableMotor = enableMotor;
+  return id;
+}
+
 IslandCtx HostWorld::islandCtx() {
   // the list-walk bounds read the world's current sizes
   hdr.bodyCount = (int)sims.size();
@@ -1423,8 +1445,12 @@ static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
     sj.lower_impulse = j.lowerImpulse;
     sj.upper_impulse = j.upperImpulse;
     sj.ref_angle = j.referenceAngle;
-    // joint kind: 0 revolute, 1 prismatic, 2 weld, 3 wheel
-    sj.reserved = j.type == kPrismaticJoint ? 1.0f : (j.type == kWeldJoint ? 2.0f : (j.type == kWheelJoint ? 3.0f : 0.0f));
+    // joint kind: 0 revolute, 1 prismatic, 2 weld, 3 wheel, 4 distance
+    sj.reserved = j.type == kPrismaticJoint ? 1.0f
+                  : j.type == kWeldJoint    ? 2.0f
+                  : j.type == kWheelJoint   ? 3.0f
+                  : j.type == kDistanceJoint ? 4.0f
+                                             : 0.0f;
     put(out, sj);
   }
   for (int c = 0; c < kGraphColorCount; ++c) put(out, (int32_t)colorLists[c].size());
@@ -3604,6 +3630,13 @@ void f2d_joint_get_constraint_force(int32_t index1, uint32_t wg, float* out) {
     const float perpForce = hw->inv_h * j.linearImpulse.x;
     const float axialForce = hw->inv_h * (j.springImpulse + j.lowerImpulse - j.upperImpulse);
     f = add(mulSV(perpForce, perpA), mulSV(axialForce, j.axisA));
+  } else if (j.type == kDistanceJoint) {
+    // S/distance_joint.c:185-198: along the current anchor-to-anchor axis
+    const Vec2 pA = transformPoint(hw->sims[j.bodyA].xf, j.localOriginAnchorA);
+    const Vec2 pB = transformPoint(hw->sims[j.bodyB].xf, j.localOriginAnchorB);
+    const Vec2 axis = normalize(sub(pB, pA));
+    const float force = (j.linearImpulse.x + j.lowerImpulse - j.upperImpulse + j.motorImpulse) * hw->inv_h;
+    f = mulSV(force, axis);
   } else {
     f = mulSV(hw->inv_h, j.linearImpulse);  // revolute, weld (S/weld_joint.c:66-70)
   }
@@ -3620,6 +3653,7 @@ float f2d_joint_get_constraint_torque(int32_t index1, uint32_t wg) {
   if (j.type == kPrismaticJoint) return hw->inv_h * j.linearImpulse.y;
   if (j.type == kWeldJoint) return hw->inv_h * j.springImpulse;  // angular impulse, S/weld_joint.c:72-75
   if (j.type == kWheelJoint) return hw->inv_h * j.motorImpulse;  // S/wheel_joint.c:162-165
+  if (j.type == kDistanceJoint) return 0.0f;  // S/joint.c b2GetJointTorque: no torque
   return hw->inv_h * (j.motorImpulse + j.lowerImpulse - j.upperImpulse);
 }
 
@@ -3655,7 +3689,94 @@ F2D_JOINT_FLOAT(f2d_wheel_joint_get_spring_damping_ratio, f2d_wheel_joint_set_sp
 F2D_JOINT_FLOAT(f2d_wheel_joint_get_motor_speed, f2d_wheel_joint_set_motor_speed, kWheelJoint, motorSpeed)
 F2D_JOINT_FLOAT(f2d_wheel_joint_get_max_motor_torque, f2d_wheel_joint_set_max_motor_torque, kWheelJoint,
                 maxMotorTorque)
+F2D_JOINT_FLOAT(f2d_distance_joint_get_spring_hertz, f2d_distance_joint_set_spring_hertz, kDistanceJoint, hertz)
+F2D_JOINT_FLOAT(f2d_distance_joint_get_spring_damping_ratio, f2d_distance_joint_set_spring_damping_ratio,
+                kDistanceJoint, dampingRatio)
+F2D_JOINT_FLOAT(f2d_distance_joint_get_motor_speed, f2d_distance_joint_set_motor_speed, kDistanceJoint, motorSpeed)
+F2D_JOINT_FLOAT(f2d_distance_joint_get_max_motor_force, f2d_distance_joint_set_max_motor_force, kDistanceJoint,
+                maxMotorTorque)
 #undef F2D_JOINT_FLOAT
+
+// ---- distance joint (S/distance_joint.c:19-183)
+float f2d_distance_joint_get_length(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kDistanceJoint);
+  return j ? j->targetAngle : 0.0f;
+}
+void f2d_distance_joint_set_length(int32_t index1, uint32_t wg, float v) {
+  Joint* j = jointOfType(index1, wg, kDistanceJoint);
+  if (!j) return;
+  j->targetAngle = clampf(v, kLinearSlop, kHuge);
+  j->linearImpulse.x = 0.0f;
+  j->lowerImpulse = 0.0f;
+  j->upperImpulse = 0.0f;
+  bodyBatch(wg)->hostDirty = true;
+}
+float f2d_distance_joint_get_current_length(int32_t index1, uint32_t wg) {
+  HostWorld* hw = nullptr;
+  const Joint* j = jointOfType(index1, wg, kDistanceJoint, &hw);
+  if (!j) return 0.0f;
+  const Vec2 pA = transformPoint(hw->sims[j->bodyA].xf, j->localOriginAnchorA);
+  const Vec2 pB = transformPoint(hw->sims[j->bodyB].xf, j->localOriginAnchorB);
+  return length(sub(pB, pA));
+}
+int f2d_distance_joint_is_spring_enabled(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kDistanceJoint);
+  return j ? j->enableSpring : 0;
+}
+void f2d_distance_joint_enable_spring(int32_t index1, uint32_t wg, int v) {
+  Joint* j = jointOfType(index1, wg, kDistanceJoint);
+  if (!j) return;
+  j->enableSpring = v != 0;
+  bodyBatch(wg)->hostDirty = true;
+}
+int f2d_distance_joint_is_limit_enabled(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kDistanceJoint);
+  return j ? j->enableLimit : 0;
+}
+void f2d_distance_joint_enable_limit(int32_t index1, uint32_t wg, int v) {
+  Joint* j = jointOfType(index1, wg, kDistanceJoint);
+  if (!j) return;
+  j->enableLimit = v != 0;
+  bodyBatch(wg)->hostDirty = true;
+}
+float f2d_distance_joint_get_min_length(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kDistanceJoint);
+  return j ? j->lowerAngle : 0.0f;
+}
+float f2d_distance_joint_get_max_length(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kDistanceJoint);
+  return j ? j->upperAngle : 0.0f;
+}
+void f2d_distance_joint_set_length_range(int32_t index1, uint32_t wg, float minLength, float maxLength) {
+  Joint* j = jointOfType(index1, wg, kDistanceJoint);
+  if (!j) return;
+  minLength = clampf(minLength, kLinearSlop, kHuge);
+  maxLength = clampf(maxLength, kLinearSlop, kHuge);
+  j->lowerAngle = minf(minLength, maxLength);
+  j->upperAngle = maxf(minLength, maxLength);
+  j->linearImpulse.x = 0.0f;
+  j->lowerImpulse = 0.0f;
+  j->upperImpulse = 0.0f;
+  bodyBatch(wg)->hostDirty = true;
+}
+int f2d_distance_joint_is_motor_enabled(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kDistanceJoint);
+  return j ? j->enableMotor : 0;
+}
+void f2d_distance_joint_enable_motor(int32_t index1, uint32_t wg, int v) {
+  Joint* j = jointOfType(index1, wg, kDistanceJoint);
+  if (!j) return;
+  if ((v != 0) != (j->enableMotor != 0)) {
+    j->enableMotor = v != 0;
+    j->motorImpulse = 0.0f;
+  }
+  bodyBatch(wg)->hostDirty = true;
+}
+float f2d_distance_joint_get_motor_force(int32_t index1, uint32_t wg) {
+  HostWorld* hw = nullptr;
+  const Joint* j = jointOfType(index1, wg, kDistanceJoint, &hw);
+  return j ? hw->inv_h * j->motorImpulse : 0.0f;
+}
 
 int f2d_wheel_joint_is_spring_enabled(int32_t index1, uint32_t wg) {
   const Joint* j = jointOfType(index1, wg, kWheelJoint);
@@ -4045,6 +4166,27 @@ void f2d_create_prismatic_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int
                                enable_spring != 0, hertz, damping_ratio, enable_limit != 0, lower_translation,
                                upper_translation, enable_motor != 0, max_motor_force, motor_speed,
                                collide_connected != 0);
+  b->hostDirty = true;
+  out[0] = id + 1;
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+}
+
+// f2d_shim.c:716-740
+void f2d_create_distance_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t b_index1, uint32_t b_wg,
+                               float lax, float lay, float lbx, float lby, float length, int enable_spring, float hertz,
+                               float damping_ratio, int enable_limit, float min_length, float max_length,
+                               int enable_motor, float max_motor_force, float motor_speed, int collide_connected,
+                               int32_t* out) {
+  out[0] = 0;
+  out[1] = 0;
+  Batch* b = batchOf(w);
+  int ia, ib;
+  HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
+  if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (b->committed) freeDevice(*b);
+  int id = hw->createDistance(ia, ib, {lax, lay}, {lbx, lby}, length, enable_spring != 0, hertz, damping_ratio,
+                              enable_limit != 0, min_length, max_length, enable_motor != 0, max_motor_force,
+                              motor_speed, collide_connected != 0);
   b->hostDirty = true;
   out[0] = id + 1;
   out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));

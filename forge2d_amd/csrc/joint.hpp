@@ -1,5 +1,5 @@
 // Restates algorithms of Box2D v3.1.1, MIT License, Copyright (c) 2022 Erin Catto (THIRD_PARTY_NOTICES).
-// Revolute, prismatic, weld and wheel joint stages, one joint per lane, in the
+// Revolute, prismatic, weld, wheel and distance joint stages, one joint per lane, in the
 // reference's operation order: revolute prepare S/joint.c:1348-1355 +
 // S/revolute_joint.c:202-259, warm start :261-287, solve :289-465; the others below.
 // Static bodies read the identity body state as in the reference.
@@ -698,6 +698,163 @@ F2D_HD void solveWheel(Joint& j, BodyState* states, float h, float inv_h, bool u
   storeVel(states, ib, vB, wB);
 }
 
+// Distance joint stages, S/distance_joint.c: prepare :215-278, warm start :280-310,
+// solve :312-490.
+F2D_HD void prepareDistance(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+  float hertz = minf(j.constraintHertz, 0.25f * inv_h);
+  j.constraintSoftness = makeSoft(hertz, j.constraintDampingRatio, h);
+  const BodySim& a = sims[j.bodyA];
+  const BodySim& b = sims[j.bodyB];
+  const float mA = a.invMass, iA = a.invI, mB = b.invMass, iB = b.invI;
+  j.invMassA = mA;
+  j.invMassB = mB;
+  j.invIA = iA;
+  j.invIB = iB;
+  j.simA = a.setIndex == kAwakeSet ? j.bodyA : -1;
+  j.simB = b.setIndex == kAwakeSet ? j.bodyB : -1;
+  j.anchorA = rotate(a.xf.q, sub(j.localOriginAnchorA, a.localCenter));
+  j.anchorB = rotate(b.xf.q, sub(j.localOriginAnchorB, b.localCenter));
+  j.deltaCenter = sub(b.center, a.center);
+  const Vec2 rA = j.anchorA, rB = j.anchorB;
+  const Vec2 separation = add(sub(rB, rA), j.deltaCenter);
+  const Vec2 axis = normalize(separation);
+  const float crA = cross(rA, axis);
+  const float crB = cross(rB, axis);
+  const float k = mA + mB + iA * crA * crA + iB * crB * crB;
+  j.axialMass = k > 0.0f ? 1.0f / k : 0.0f;
+  j.springSoftness = makeSoft(j.hertz, j.dampingRatio, h);
+  if (warmStartScale == 0.0f) {
+    j.linearImpulse.x = 0.0f;
+    j.lowerImpulse = 0.0f;
+    j.upperImpulse = 0.0f;
+    j.motorImpulse = 0.0f;
+  }
+}
+
+F2D_HD void warmStartDistance(Joint& j, BodyState* states) {
+  const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+  const int ia = j.simA, ib = j.simB;
+  BodyState sA = loadState(states, ia);
+  BodyState sB = loadState(states, ib);
+  const Vec2 rA = rotate(sA.dq, j.anchorA);
+  const Vec2 rB = rotate(sB.dq, j.anchorB);
+  const Vec2 ds = add(sub(sB.dp, sA.dp), sub(rB, rA));
+  const Vec2 separation = add(j.deltaCenter, ds);
+  const Vec2 axis = normalize(separation);
+  const float axialImpulse = j.linearImpulse.x + j.lowerImpulse - j.upperImpulse + j.motorImpulse;
+  const Vec2 P = mulSV(axialImpulse, axis);
+  sA.v = mulSub(sA.v, mA, P);
+  sA.w -= iA * cross(rA, P);
+  sB.v = mulAdd(sB.v, mB, P);
+  sB.w += iB * cross(rB, P);
+  storeVel(states, ia, sA.v, sA.w);
+  storeVel(states, ib, sB.v, sB.w);
+}
+
+F2D_HD void solveDistance(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
+  const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+  const int ia = j.simA, ib = j.simB;
+  const BodyState sA = loadState(states, ia);
+  const BodyState sB = loadState(states, ib);
+  Vec2 vA = sA.v;
+  float wA = sA.w;
+  Vec2 vB = sB.v;
+  float wB = sB.w;
+  const Vec2 rA = rotate(sA.dq, j.anchorA);
+  const Vec2 rB = rotate(sB.dq, j.anchorB);
+  const Vec2 ds = add(sub(sB.dp, sA.dp), sub(rB, rA));
+  const Vec2 separation = add(j.deltaCenter, ds);
+  const float len = length(separation);
+  const Vec2 axis = normalize(separation);
+  const float minLength = j.lowerAngle, maxLength = j.upperAngle;
+  auto apply = [&](float impulse) {
+    const Vec2 P = mulSV(impulse, axis);
+    vA = mulSub(vA, mA, P);
+    wA -= iA * cross(rA, P);
+    vB = mulAdd(vB, mB, P);
+    wB += iB * cross(rB, P);
+  };
+  if (j.enableSpring && (minLength < maxLength || j.enableLimit == false)) {
+    // spring
+    if (j.hertz > 0.0f) {
+      const Vec2 vr = add(sub(vB, vA), sub(crossSV(wB, rB), crossSV(wA, rA)));
+      const float Cdot = dot(axis, vr);
+      const float C = len - j.targetAngle;
+      const float bias = j.springSoftness.biasRate * C;
+      const float m = j.springSoftness.massScale * j.axialMass;
+      const float impulse = -m * (Cdot + bias) - j.springSoftness.impulseScale * j.linearImpulse.x;
+      j.linearImpulse.x += impulse;
+      apply(impulse);
+    }
+    if (j.enableLimit) {
+      {
+        // lower
+        const Vec2 vr = add(sub(vB, vA), sub(crossSV(wB, rB), crossSV(wA, rA)));
+        const float Cdot = dot(axis, vr);
+        const float C = len - minLength;
+        float bias = 0.0f, massCoeff = 1.0f, impulseCoeff = 0.0f;
+        if (C > 0.0f) {
+          bias = C * inv_h;  // speculative
+        } else if (useBias) {
+          bias = j.constraintSoftness.biasRate * C;
+          massCoeff = j.constraintSoftness.massScale;
+          impulseCoeff = j.constraintSoftness.impulseScale;
+        }
+        float impulse = -massCoeff * j.axialMass * (Cdot + bias) - impulseCoeff * j.lowerImpulse;
+        const float newImpulse = maxf(0.0f, j.lowerImpulse + impulse);
+        impulse = newImpulse - j.lowerImpulse;
+        j.lowerImpulse = newImpulse;
+        apply(impulse);
+      }
+      {
+        // upper
+        const Vec2 vr = add(sub(vA, vB), sub(crossSV(wA, rA), crossSV(wB, rB)));
+        const float Cdot = dot(axis, vr);
+        const float C = maxLength - len;
+        float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
+        if (C > 0.0f) {
+          bias = C * inv_h;
+        } else if (useBias) {
+          bias = j.constraintSoftness.biasRate * C;
+          massScale = j.constraintSoftness.massScale;
+          impulseScale = j.constraintSoftness.impulseScale;
+        }
+        float impulse = -massScale * j.axialMass * (Cdot + bias) - impulseScale * j.upperImpulse;
+        const float newImpulse = maxf(0.0f, j.upperImpulse + impulse);
+        impulse = newImpulse - j.upperImpulse;
+        j.upperImpulse = newImpulse;
+        apply(-impulse);
+      }
+    }
+    if (j.enableMotor) {
+      const Vec2 vr = add(sub(vB, vA), sub(crossSV(wB, rB), crossSV(wA, rA)));
+      const float Cdot = dot(axis, vr);
+      float impulse = j.axialMass * (j.motorSpeed - Cdot);
+      const float oldImpulse = j.motorImpulse;
+      const float maxImpulse = h * j.maxMotorTorque;
+      j.motorImpulse = clampf(j.motorImpulse + impulse, -maxImpulse, maxImpulse);
+      impulse = j.motorImpulse - oldImpulse;
+      apply(impulse);
+    }
+  } else {
+    // rigid constraint
+    const Vec2 vr = add(sub(vB, vA), sub(crossSV(wB, rB), crossSV(wA, rA)));
+    const float Cdot = dot(axis, vr);
+    const float C = len - j.targetAngle;
+    float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
+    if (useBias) {
+      bias = j.constraintSoftness.biasRate * C;
+      massScale = j.constraintSoftness.massScale;
+      impulseScale = j.constraintSoftness.impulseScale;
+    }
+    const float impulse = -massScale * j.axialMass * (Cdot + bias) - impulseScale * j.linearImpulse.x;
+    j.linearImpulse.x += impulse;
+    apply(impulse);
+  }
+  storeVel(states, ia, vA, wA);
+  storeVel(states, ib, vB, wB);
+}
+
 // The joint stages by joint type (b2PrepareJointsTask / b2WarmStartJoint /
 // b2SolveJoint, S/joint.c:1348-1450)
 F2D_HD void prepareJoint(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
@@ -705,6 +862,7 @@ F2D_HD void prepareJoint(Joint& j, const BodySim* sims, float h, float inv_h, fl
     case kPrismaticJoint: preparePrismatic(j, sims, h, inv_h, warmStartScale); break;
     case kWeldJoint: prepareWeld(j, sims, h, inv_h, warmStartScale); break;
     case kWheelJoint: prepareWheel(j, sims, h, inv_h, warmStartScale); break;
+    case kDistanceJoint: prepareDistance(j, sims, h, inv_h, warmStartScale); break;
     default: prepareRevolute(j, sims, h, inv_h, warmStartScale); break;
   }
 }
@@ -713,6 +871,7 @@ F2D_HD void warmStartJoint(Joint& j, BodyState* states) {
     case kPrismaticJoint: warmStartPrismatic(j, states); break;
     case kWeldJoint: warmStartWeld(j, states); break;
     case kWheelJoint: warmStartWheel(j, states); break;
+    case kDistanceJoint: warmStartDistance(j, states); break;
     default: warmStartRevolute(j, states); break;
   }
 }
@@ -721,6 +880,7 @@ F2D_HD void solveJoint(Joint& j, BodyState* states, float h, float inv_h, bool u
     case kPrismaticJoint: solvePrismatic(j, states, h, inv_h, useBias); break;
     case kWeldJoint: solveWeld(j, states, useBias); break;
     case kWheelJoint: solveWheel(j, states, h, inv_h, useBias); break;
+    case kDistanceJoint: solveDistance(j, states, h, inv_h, useBias); break;
     default: solveRevolute(j, states, h, inv_h, useBias); break;
   }
 }

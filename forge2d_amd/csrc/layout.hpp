@@ -153,7 +153,7 @@ struct alignas(16) Contact {
 
 // Revolute joint slot (b2Joint + b2JointSim + b2RevoluteJoint, S/joint.h:27-60, 162-190, 247-281)
 // b2JointType values of the joints built here (I/types.h)
-enum JointType : int { kPrismaticJoint = 4, kRevoluteJoint = 5, kWeldJoint = 6, kWheelJoint = 7 };
+enum JointType : int { kDistanceJoint = 0, kPrismaticJoint = 4, kRevoluteJoint = 5, kWeldJoint = 6, kWheelJoint = 7 };
 
 // One joint (b2Joint + b2JointSim, S/joint.h). The joint types share fields
 // (S/joint.h b2PrismaticJoint, b2WeldJoint, b2WheelJoint):
@@ -163,7 +163,10 @@ enum JointType : int { kPrismaticJoint = 4, kRevoluteJoint = 5, kWeldJoint = 6, 
 //   weld: hertz/dampingRatio = linear stiffness, springSoftness = linear softness,
 //     springImpulse = angular impulse;
 //   wheel: lowerAngle/upperAngle = translation limits, linearImpulse.x =
-//     perpendicular impulse.
+//     perpendicular impulse;
+//   distance: targetAngle = rest length, lowerAngle/upperAngle = length range,
+//     maxMotorTorque = maximum motor force, springSoftness = distance softness,
+//     linearImpulse.x = the length impulse.
 struct alignas(16) Joint {
   int bodyA, bodyB;
   int color, localIndex;

@@ -72,6 +72,9 @@ struct HostWorld {
   int createWheel(int bodyA, int bodyB, Vec2 la, Vec2 lb, Vec2 axis, bool enableSpring, float hertz, float damping,
                   bool enableLimit, float lower, float upper, bool enableMotor, float maxMotorTorque,
                   float motorSpeed, bool collideConnected);
+  int createDistance(int bodyA, int bodyB, Vec2 la, Vec2 lb, float length, bool enableSpring, float hertz,
+                     float damping, bool enableLimit, float minLength, float maxLength, bool enableMotor,
+                     float maxMotorForce, float motorSpeed, bool collideConnected);
   void updateBodyMass(int body);
 };
 

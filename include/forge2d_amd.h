@@ -247,6 +247,38 @@ float f2d_prismatic_joint_get_motor_force(int32_t index1, uint32_t world_and_gen
 float f2d_prismatic_joint_get_translation(int32_t index1, uint32_t world_and_generation);
 float f2d_prismatic_joint_get_speed(int32_t index1, uint32_t world_and_generation);
 
+/* F/native/wasm/f2d_shim.c:716-740 (b2CreateDistanceJoint, S/joint.c:353-405) */
+void f2d_create_distance_joint(uint32_t w, int32_t a_index1, uint32_t a_world_and_generation, int32_t b_index1,
+                               uint32_t b_world_and_generation, float anchor_ax, float anchor_ay, float anchor_bx,
+                               float anchor_by, float length, int enable_spring, float hertz, float damping_ratio,
+                               int enable_limit, float min_length, float max_length, int enable_motor,
+                               float max_motor_force, float motor_speed, int collide_connected, int32_t* out);
+
+/* F/native/wasm/f2d_shim.c:965-1004 (distance joint parameters, S/distance_joint.c:19-183);
+ * a call on a joint of another type is ignored and getters return 0 */
+float f2d_distance_joint_get_length(int32_t index1, uint32_t world_and_generation);
+void f2d_distance_joint_set_length(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_distance_joint_get_current_length(int32_t index1, uint32_t world_and_generation);
+int f2d_distance_joint_is_spring_enabled(int32_t index1, uint32_t world_and_generation);
+void f2d_distance_joint_enable_spring(int32_t index1, uint32_t world_and_generation, int v);
+float f2d_distance_joint_get_spring_hertz(int32_t index1, uint32_t world_and_generation);
+void f2d_distance_joint_set_spring_hertz(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_distance_joint_get_spring_damping_ratio(int32_t index1, uint32_t world_and_generation);
+void f2d_distance_joint_set_spring_damping_ratio(int32_t index1, uint32_t world_and_generation, float v);
+int f2d_distance_joint_is_limit_enabled(int32_t index1, uint32_t world_and_generation);
+void f2d_distance_joint_enable_limit(int32_t index1, uint32_t world_and_generation, int v);
+float f2d_distance_joint_get_min_length(int32_t index1, uint32_t world_and_generation);
+float f2d_distance_joint_get_max_length(int32_t index1, uint32_t world_and_generation);
+void f2d_distance_joint_set_length_range(int32_t index1, uint32_t world_and_generation, float min_length,
+                                         float max_length);
+int f2d_distance_joint_is_motor_enabled(int32_t index1, uint32_t world_and_generation);
+void f2d_distance_joint_enable_motor(int32_t index1, uint32_t world_and_generation, int v);
+float f2d_distance_joint_get_motor_speed(int32_t index1, uint32_t world_and_generation);
+void f2d_distance_joint_set_motor_speed(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_distance_joint_get_max_motor_force(int32_t index1, uint32_t world_and_generation);
+void f2d_distance_joint_set_max_motor_force(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_distance_joint_get_motor_force(int32_t index1, uint32_t world_and_generation);
+
 /* F/native/wasm/f2d_shim.c:844-862 (b2CreateWeldJoint, S/joint.c:612-649) */
 void f2d_create_weld_joint(uint32_t w, int32_t a_index1, uint32_t a_world_and_generation, int32_t b_index1,
                            uint32_t b_world_and_generation, float anchor_ax, float anchor_ay, float anchor_bx,
@@ -292,11 +324,11 @@ float f2d_wheel_joint_get_max_motor_torque(int32_t index1, uint32_t world_and_ge
 void f2d_wheel_joint_set_max_motor_torque(int32_t index1, uint32_t world_and_generation, float v);
 float f2d_wheel_joint_get_motor_torque(int32_t index1, uint32_t world_and_generation);
 
-/* F/native/wasm/f2d_shim.c:892-950 (joint accessors, S/joint.c): type (4 b2_prismaticJoint,
- * 5 b2_revoluteJoint, 6 b2_weldJoint, 7 b2_wheelJoint), bodies, local anchors,
- * collideConnected, waking both bodies, and the last step's constraint force and torque
- * (S/revolute_joint.c:176-187, S/prismatic_joint.c:211-232, S/weld_joint.c:66-75,
- * S/wheel_joint.c:147-165) */
+/* F/native/wasm/f2d_shim.c:892-950 (joint accessors, S/joint.c): type (0 b2_distanceJoint,
+ * 4 b2_prismaticJoint, 5 b2_revoluteJoint, 6 b2_weldJoint, 7 b2_wheelJoint), bodies, local
+ * anchors, collideConnected, waking both bodies, and the last step's constraint force and
+ * torque (S/revolute_joint.c:176-187, S/prismatic_joint.c:211-232, S/weld_joint.c:66-75,
+ * S/wheel_joint.c:147-165, S/distance_joint.c:185-198) */
 int f2d_joint_is_valid(int32_t index1, uint32_t world_and_generation);
 int f2d_joint_get_type(int32_t index1, uint32_t world_and_generation);
 void f2d_joint_get_body_a(int32_t index1, uint32_t world_and_generation, int32_t* out);

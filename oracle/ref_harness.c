@@ -619,6 +619,19 @@ static void write_snapshot( FILE* f, b2WorldId wid, int step )
 				sj.ref_angle = wj->referenceAngle;
 				sj.reserved = 2.0f;
 			}
+			else if ( js->type == b2_distanceJoint )
+			{
+				// kind 4; the length impulse in linear_impulse_x
+				const b2DistanceJoint* dj = &js->distanceJoint;
+				sj.enable_limit = dj->enableLimit;
+				sj.enable_motor = dj->enableMotor;
+				sj.enable_spring = dj->enableSpring;
+				sj.linear_impulse_x = dj->impulse;
+				sj.motor_impulse = dj->motorImpulse;
+				sj.lower_impulse = dj->lowerImpulse;
+				sj.upper_impulse = dj->upperImpulse;
+				sj.reserved = 4.0f;
+			}
 			else if ( js->type == b2_wheelJoint )
 			{
 				// kind 3; the perpendicular impulse in linear_impulse_x
@@ -1024,6 +1037,38 @@ static void apply_op( const OpRec* o, const b2BodyId* bodies, const b2ShapeId* s
 			b2WheelJoint_EnableSpring( j, ( o->flag & 4 ) != 0 );
 		}
 		break;
+		case 32:
+		{
+			// a distance joint from the default def, anchors f0..f3, length f4,
+			// collideConnected flag >> 16
+			b2DistanceJointDef def = b2DefaultDistanceJointDef();
+			def.bodyIdA = b;
+			def.bodyIdB = bodies[o->flag & 0xFFFF];
+			def.localAnchorA = ( b2Vec2 ){ f[0], f[1] };
+			def.localAnchorB = ( b2Vec2 ){ f[2], f[3] };
+			def.length = f[4];
+			def.collideConnected = ( o->flag >> 16 ) != 0;
+			joints[g_jointCount++] = b2CreateDistanceJoint( b2Body_GetWorld( b ), &def );
+		}
+		break;
+		case 33:
+		{
+			// distance joint target: length range f0, f1, motor force f2 and speed f3,
+			// spring hertz f4 and damping f5, then limit / motor / spring per flag bits 1, 2, 4
+			b2JointId j = joints[o->target];
+			b2DistanceJoint_SetLengthRange( j, f[0], f[1] );
+			b2DistanceJoint_EnableLimit( j, ( o->flag & 1 ) != 0 );
+			b2DistanceJoint_SetMaxMotorForce( j, f[2] );
+			b2DistanceJoint_SetMotorSpeed( j, f[3] );
+			b2DistanceJoint_EnableMotor( j, ( o->flag & 2 ) != 0 );
+			b2DistanceJoint_SetSpringHertz( j, f[4] );
+			b2DistanceJoint_SetSpringDampingRatio( j, f[5] );
+			b2DistanceJoint_EnableSpring( j, ( o->flag & 4 ) != 0 );
+		}
+		break;
+		case 34:
+			b2DistanceJoint_SetLength( joints[o->target], f[0] );
+			break;
 		default:
 			fprintf( stderr, "bad op %d\n", o->op );
 			exit( 3 );

@@ -288,7 +288,8 @@ OPS = {"apply_force": 1, "apply_force_to_center": 2, "apply_torque": 3, "apply_l
        "shape_set_density": 12, "shape_set_friction": 13, "shape_set_restitution": 14, "set_gravity_scale": 15,
        "set_linear_damping": 16, "set_angular_damping": 17, "set_fixed_rotation": 18, "set_bullet": 19,
        "set_awake": 20, "joint_wake_bodies": 23, "create_prismatic": 25, "prismatic_config": 26,
-       "prismatic_target": 27, "create_weld": 28, "weld_config": 29, "create_wheel": 30, "wheel_config": 31}
+       "prismatic_target": 27, "create_weld": 28, "weld_config": 29, "create_wheel": 30, "wheel_config": 31,
+       "create_distance": 32, "distance_config": 33, "distance_length": 34}
 
 
 def ops_array(rows):
@@ -439,11 +440,46 @@ WELD_WHEEL_OPS = [
     (55, "wheel_config", 2, 6, [-0.3, -0.25, 30.0, 4.0, 4.0, 0.7]),
 ]
 
+def distance_world():
+    """A scene for distance joints (S/distance_joint.c), created by API calls: a pendulum
+    box and a circle hanging from a static anchor, two boxes on the ground, a box
+    hanging from a box, and loose boxes that fall on them (sleep off)."""
+    w = scenes.World(scenes.world_header(sleep=False, continuous=True))
+    g = w.add_body(scenes.STATIC, 0.0, -1.0)                      # 0
+    w.add_box(g, 30.0, 1.0)                                       # shape 0
+    anchor = w.add_body(scenes.STATIC, 0.0, 8.0)                  # 1, no shape
+    w.add_box(w.add_body(scenes.DYNAMIC, 3.0, 7.0), 0.3, 0.3)     # 2: rigid pendulum
+    w.add_circle(w.add_body(scenes.DYNAMIC, -2.0, 5.0), 0.4)      # 3: spring with a length range
+    w.add_box(w.add_body(scenes.DYNAMIC, 6.0, 0.5), 0.5, 0.5)     # 4
+    w.add_box(w.add_body(scenes.DYNAMIC, 8.0, 0.5), 0.5, 0.5)     # 5: motor along 4-5
+    w.add_box(w.add_body(scenes.DYNAMIC, -8.0, 4.0), 0.6, 0.3)    # 6
+    w.add_box(w.add_body(scenes.DYNAMIC, -8.5, 2.0, angle=0.5), 0.3, 0.3)  # 7: hangs from 6
+    for k in range(3):                                            # 8..10
+        w.add_box(w.add_body(scenes.DYNAMIC, -7.8 + 7.0 * k, 9.0 + 0.4 * k, angle=0.2 * k), 0.3, 0.3)
+    return w
+
+
+DISTANCE_OPS = [
+    (1, "create_distance", 1, 2, [0.0, 0.0, 0.0, 0.2, 3.0]),                 # 0: rigid rope
+    (1, "create_distance", 1, 3, [0.0, -0.5, 0.0, 0.0, 2.5]),                # 1
+    (1, "distance_config", 1, 5, [1.5, 3.0, 0.0, 0.0, 3.0, 0.3]),            #    spring + range
+    (1, "create_distance", 4, 5, [0.5, 0.0, -0.5, 0.0, 1.0]),                # 2
+    (1, "distance_config", 2, 7, [0.8, 2.0, 40.0, 3.0, 0.0, 0.0]),           #    range + motor, spring without force
+    (1, "create_distance", 6, 7 | (1 << 16), [-0.5, 0.0, 0.0, 0.3, 1.2]),    # 3: collide connected
+    (1, "distance_config", 3, 5, [0.8, 1.6, 0.0, 0.0, 5.0, 0.9]),
+    # later changes: new length, motor reversed, spring off (rigid), range tightened
+    (20, "distance_length", 0, 0, [2.0]),
+    (25, "distance_config", 2, 7, [0.8, 2.0, 40.0, -4.0, 0.0, 0.0]),
+    (40, "distance_config", 1, 1, [1.5, 3.0, 0.0, 0.0, 3.0, 0.3]),
+    (45, "distance_config", 3, 5, [1.0, 1.1, 0.0, 0.0, 5.0, 0.9]),
+]
+
 OPS_FIXTURES = {
     # name: (world factory, ops, dumped steps)
     "ops_bodies": (body_ops_world, BODY_OPS, list(range(0, 41))),
     "ops_prismatic": (prismatic_world, PRISMATIC_OPS, list(range(0, 61))),
     "ops_weld_wheel": (weld_wheel_world, WELD_WHEEL_OPS, list(range(0, 61))),
+    "ops_distance": (distance_world, DISTANCE_OPS, list(range(0, 61))),
 }
 
 

@@ -320,3 +320,33 @@ def test_weld_wheel_fixture_exercises_every_branch():
     assert (J["motor_impulse"][:, 2] > 0).any() and (J["motor_impulse"][:, 2] < 0).any()
     assert (J["lower_impulse"][:, 2] > 0).any() and (J["upper_impulse"][:, 2] > 0).any()
     assert (J["spring_impulse"][:, 2] != 0).any() and (J["linear_impulse_x"][:, 3] != 0).any()
+
+
+def test_distance_fixture_exercises_every_branch():
+    """Over the 60 reference steps of ops_distance: the rigid branch's impulse (rope, and
+    the spring joint after its spring is turned off), the spring branch, motor impulses
+    of both signs and lower and upper range impulses."""
+    spec, snaps, ops = load_ops_fixture("ops_distance", GOLDEN)
+    J = np.stack([s.joints for s in snaps[1:]])
+    assert J["reserved"][0].tolist() == [4.0] * 4
+    assert (J["linear_impulse_x"][:, 0] != 0).any() and (J["linear_impulse_x"][:, 1] != 0).any()
+    assert (J["motor_impulse"][:, 2] > 0).any() and (J["motor_impulse"][:, 2] < 0).any()
+    assert (J["lower_impulse"][:, 2] > 0).any() and (J["upper_impulse"][:, 2] > 0).any()
+
+
+def test_joint_calls_on_other_types_are_ignored():
+    """A joint parameter call of the wrong joint type leaves the joint alone and its
+    getter returns 0, as the reference's type check would refuse it."""
+    spec, snaps, ops = load_ops_fixture("ops_distance", GOLDEN)
+    fw = FlatWorld(spec)
+    for op in ops[ops["step"] == 1]:
+        fw.apply(op)
+    L = F.lib()
+    j = fw.joints[0]
+    assert L.f2d_prismatic_joint_get_lower_limit(*j) == 0.0
+    L.f2d_wheel_joint_set_motor_speed(*j, 3.0)
+    L.f2d_prismatic_joint_set_limits(*j, -1.0, 1.0)
+    assert L.f2d_distance_joint_get_min_length(*j) == np.float32(0.005)
+    assert L.f2d_distance_joint_get_length(*j) == 3.0
+    assert L.f2d_distance_joint_get_motor_speed(*j) == 0.0
+    fw.close()

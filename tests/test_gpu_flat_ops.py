@@ -112,6 +112,26 @@ def test_weld_and_wheel_joints_match_reference(gpu):
     fw.close()
 
 
+def test_distance_joints_match_reference(gpu):
+    """Distance joints (S/distance_joint.c) made through f2d_create_distance_joint and the
+    f2d_distance_joint_* calls (fixture ops_distance: a rigid rope, a spring with a
+    length range, a motor that drives two boxes into the upper and then, reversed, the
+    lower end of their range with the spring branch on at zero stiffness, a spring with
+    collideConnected; later a new rest length, the spring turned off (rigid branch) and a
+    tightened range): after every one of 60 steps bodies, contacts, colours, islands and
+    every joint's impulses equal the reference's bit for bit."""
+    fw, snaps = replay_joint_fixture("ops_distance", 4)
+    import forge2d_amd as F
+    L = F.lib()
+    assert [L.f2d_joint_get_type(*j) for j in fw.joints] == [0, 0, 0, 0]
+    assert L.f2d_distance_joint_get_length(*fw.joints[0]) == 2.0
+    assert (L.f2d_distance_joint_get_min_length(*fw.joints[3]), L.f2d_distance_joint_get_max_length(*fw.joints[3])) \
+        == (np.float32(1.0), np.float32(1.1))
+    inv_h = np.float32(4.0) * (np.float32(1.0) / np.float32(1.0 / 60.0))
+    assert L.f2d_distance_joint_get_motor_force(*fw.joints[2]) == inv_h * snaps[-1].joints["motor_impulse"][2]
+    fw.close()
+
+
 def test_prismatic_joints_match_reference(gpu):
     """Prismatic joints (S/prismatic_joint.c) made through f2d_create_prismatic_joint and
     the f2d_prismatic_joint_* setters (fixture ops_prismatic: a vertical slider with
