@@ -220,6 +220,16 @@ SIGNATURES = [
     (f"f2d_distance_joint_{n}", None, [ctypes.c_int32, _U, _I]) for n in
     ["enable_spring", "enable_limit", "enable_motor"]] + [
     ("f2d_distance_joint_set_length_range", None, [ctypes.c_int32, _U, _F, _F]),
+] + [(f"f2d_revolute_joint_get_{n}", _F, [ctypes.c_int32, _U]) for n in
+     ["spring_hertz", "spring_damping_ratio", "target_angle", "angle", "lower_limit", "upper_limit", "motor_speed",
+      "max_motor_torque", "motor_torque"]] + [
+    (f"f2d_revolute_joint_set_{n}", None, [ctypes.c_int32, _U, _F]) for n in
+    ["spring_hertz", "spring_damping_ratio", "target_angle", "motor_speed", "max_motor_torque"]] + [
+    (f"f2d_revolute_joint_{n}", _I, [ctypes.c_int32, _U]) for n in
+    ["is_spring_enabled", "is_limit_enabled", "is_motor_enabled"]] + [
+    (f"f2d_revolute_joint_{n}", None, [ctypes.c_int32, _U, _I]) for n in
+    ["enable_spring", "enable_limit", "enable_motor"]] + [
+    ("f2d_revolute_joint_set_limits", None, [ctypes.c_int32, _U, _F, _F]),
     ("f2d_wheel_joint_is_spring_enabled", _I, [ctypes.c_int32, _U]),
     ("f2d_wheel_joint_enable_spring", None, [ctypes.c_int32, _U, _I]),
     ("f2d_wheel_joint_is_limit_enabled", _I, [ctypes.c_int32, _U]),

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "manifold.hpp"
+#include "joint.hpp"
 #include "query.hpp"
 #include "solver.hpp"
 #include "world_store.hpp"
@@ -3695,7 +3696,86 @@ F2D_JOINT_FLOAT(f2d_distance_joint_get_spring_damping_ratio, f2d_distance_joint_
 F2D_JOINT_FLOAT(f2d_distance_joint_get_motor_speed, f2d_distance_joint_set_motor_speed, kDistanceJoint, motorSpeed)
 F2D_JOINT_FLOAT(f2d_distance_joint_get_max_motor_force, f2d_distance_joint_set_max_motor_force, kDistanceJoint,
                 maxMotorTorque)
+
+// ---- revolute joint (S/revolute_joint.c:20-176): the same rules as the prismatic calls
+static void enableWithReset(Joint* j, int& flag, int v, float* a, float* b = nullptr) {
+  if ((v != 0) != (flag != 0)) {
+    flag = v != 0;
+    *a = 0.0f;
+    if (b) *b = 0.0f;
+  }
+}
+F2D_JOINT_FLOAT(f2d_revolute_joint_get_spring_hertz, f2d_revolute_joint_set_spring_hertz, kRevoluteJoint, hertz)
+F2D_JOINT_FLOAT(f2d_revolute_joint_get_spring_damping_ratio, f2d_revolute_joint_set_spring_damping_ratio,
+                kRevoluteJoint, dampingRatio)
+F2D_JOINT_FLOAT(f2d_revolute_joint_get_target_angle, f2d_revolute_joint_set_target_angle, kRevoluteJoint,
+                targetAngle)
+F2D_JOINT_FLOAT(f2d_revolute_joint_get_motor_speed, f2d_revolute_joint_set_motor_speed, kRevoluteJoint, motorSpeed)
+F2D_JOINT_FLOAT(f2d_revolute_joint_get_max_motor_torque, f2d_revolute_joint_set_max_motor_torque, kRevoluteJoint,
+                maxMotorTorque)
+int f2d_revolute_joint_is_spring_enabled(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kRevoluteJoint);
+  return j ? j->enableSpring : 0;
+}
+void f2d_revolute_joint_enable_spring(int32_t index1, uint32_t wg, int v) {
+  Joint* j = jointOfType(index1, wg, kRevoluteJoint);
+  if (!j) return;
+  enableWithReset(j, j->enableSpring, v, &j->springImpulse);
+  bodyBatch(wg)->hostDirty = true;
+}
+int f2d_revolute_joint_is_limit_enabled(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kRevoluteJoint);
+  return j ? j->enableLimit : 0;
+}
+void f2d_revolute_joint_enable_limit(int32_t index1, uint32_t wg, int v) {
+  Joint* j = jointOfType(index1, wg, kRevoluteJoint);
+  if (!j) return;
+  enableWithReset(j, j->enableLimit, v, &j->lowerImpulse, &j->upperImpulse);
+  bodyBatch(wg)->hostDirty = true;
+}
+float f2d_revolute_joint_get_lower_limit(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kRevoluteJoint);
+  return j ? j->lowerAngle : 0.0f;
+}
+float f2d_revolute_joint_get_upper_limit(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kRevoluteJoint);
+  return j ? j->upperAngle : 0.0f;
+}
+void f2d_revolute_joint_set_limits(int32_t index1, uint32_t wg, float lower, float upper) {
+  Joint* j = jointOfType(index1, wg, kRevoluteJoint);
+  if (!j) return;
+  if (lower != j->lowerAngle || upper != j->upperAngle) {
+    j->lowerAngle = minf(lower, upper);
+    j->upperAngle = maxf(lower, upper);
+    j->lowerImpulse = 0.0f;
+    j->upperImpulse = 0.0f;
+  }
+  bodyBatch(wg)->hostDirty = true;
+}
+int f2d_revolute_joint_is_motor_enabled(int32_t index1, uint32_t wg) {
+  const Joint* j = jointOfType(index1, wg, kRevoluteJoint);
+  return j ? j->enableMotor : 0;
+}
+void f2d_revolute_joint_enable_motor(int32_t index1, uint32_t wg, int v) {
+  Joint* j = jointOfType(index1, wg, kRevoluteJoint);
+  if (!j) return;
+  enableWithReset(j, j->enableMotor, v, &j->motorImpulse);
+  bodyBatch(wg)->hostDirty = true;
+}
 #undef F2D_JOINT_FLOAT
+float f2d_revolute_joint_get_motor_torque(int32_t index1, uint32_t wg) {
+  HostWorld* hw = nullptr;
+  const Joint* j = jointOfType(index1, wg, kRevoluteJoint, &hw);
+  return j ? hw->inv_h * j->motorImpulse : 0.0f;
+}
+// S/revolute_joint.c:72-82
+float f2d_revolute_joint_get_angle(int32_t index1, uint32_t wg) {
+  HostWorld* hw = nullptr;
+  const Joint* j = jointOfType(index1, wg, kRevoluteJoint, &hw);
+  if (!j) return 0.0f;
+  const float angle = relativeAngle(hw->sims[j->bodyB].xf.q, hw->sims[j->bodyA].xf.q) - j->referenceAngle;
+  return unwindAngle(angle);
+}
 
 // ---- distance joint (S/distance_joint.c:19-183)
 float f2d_distance_joint_get_length(int32_t index1, uint32_t wg) {

@@ -247,6 +247,30 @@ float f2d_prismatic_joint_get_motor_force(int32_t index1, uint32_t world_and_gen
 float f2d_prismatic_joint_get_translation(int32_t index1, uint32_t world_and_generation);
 float f2d_prismatic_joint_get_speed(int32_t index1, uint32_t world_and_generation);
 
+/* F/native/wasm/f2d_shim.c:1097-1136 (revolute joint parameters, S/revolute_joint.c:20-176);
+ * a call on a joint of another type is ignored and getters return 0 */
+int f2d_revolute_joint_is_spring_enabled(int32_t index1, uint32_t world_and_generation);
+void f2d_revolute_joint_enable_spring(int32_t index1, uint32_t world_and_generation, int v);
+float f2d_revolute_joint_get_spring_hertz(int32_t index1, uint32_t world_and_generation);
+void f2d_revolute_joint_set_spring_hertz(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_revolute_joint_get_spring_damping_ratio(int32_t index1, uint32_t world_and_generation);
+void f2d_revolute_joint_set_spring_damping_ratio(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_revolute_joint_get_target_angle(int32_t index1, uint32_t world_and_generation);
+void f2d_revolute_joint_set_target_angle(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_revolute_joint_get_angle(int32_t index1, uint32_t world_and_generation);
+int f2d_revolute_joint_is_limit_enabled(int32_t index1, uint32_t world_and_generation);
+void f2d_revolute_joint_enable_limit(int32_t index1, uint32_t world_and_generation, int v);
+float f2d_revolute_joint_get_lower_limit(int32_t index1, uint32_t world_and_generation);
+float f2d_revolute_joint_get_upper_limit(int32_t index1, uint32_t world_and_generation);
+void f2d_revolute_joint_set_limits(int32_t index1, uint32_t world_and_generation, float lower, float upper);
+int f2d_revolute_joint_is_motor_enabled(int32_t index1, uint32_t world_and_generation);
+void f2d_revolute_joint_enable_motor(int32_t index1, uint32_t world_and_generation, int v);
+float f2d_revolute_joint_get_motor_speed(int32_t index1, uint32_t world_and_generation);
+void f2d_revolute_joint_set_motor_speed(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_revolute_joint_get_max_motor_torque(int32_t index1, uint32_t world_and_generation);
+void f2d_revolute_joint_set_max_motor_torque(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_revolute_joint_get_motor_torque(int32_t index1, uint32_t world_and_generation);
+
 /* F/native/wasm/f2d_shim.c:716-740 (b2CreateDistanceJoint, S/joint.c:353-405) */
 void f2d_create_distance_joint(uint32_t w, int32_t a_index1, uint32_t a_world_and_generation, int32_t b_index1,
                                uint32_t b_world_and_generation, float anchor_ax, float anchor_ay, float anchor_bx,

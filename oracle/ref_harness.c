@@ -1069,6 +1069,24 @@ static void apply_op( const OpRec* o, const b2BodyId* bodies, const b2ShapeId* s
 		case 34:
 			b2DistanceJoint_SetLength( joints[o->target], f[0] );
 			break;
+		case 35:
+		{
+			// revolute joint target: limits f0, f1, motor torque f2 and speed f3, spring
+			// hertz f4 and damping f5, then limit / motor / spring per flag bits 1, 2, 4
+			b2JointId j = joints[o->target];
+			b2RevoluteJoint_SetLimits( j, f[0], f[1] );
+			b2RevoluteJoint_EnableLimit( j, ( o->flag & 1 ) != 0 );
+			b2RevoluteJoint_SetMaxMotorTorque( j, f[2] );
+			b2RevoluteJoint_SetMotorSpeed( j, f[3] );
+			b2RevoluteJoint_EnableMotor( j, ( o->flag & 2 ) != 0 );
+			b2RevoluteJoint_SetSpringHertz( j, f[4] );
+			b2RevoluteJoint_SetSpringDampingRatio( j, f[5] );
+			b2RevoluteJoint_EnableSpring( j, ( o->flag & 4 ) != 0 );
+		}
+		break;
+		case 36:
+			b2RevoluteJoint_SetTargetAngle( joints[o->target], f[0] );
+			break;
 		default:
 			fprintf( stderr, "bad op %d\n", o->op );
 			exit( 3 );

@@ -115,7 +115,7 @@ class FlatWorld:
         L = F.lib()
         code, t, flag = int(op["op"]), int(op["target"]), int(op["flag"])
         f = [float(x) for x in op["f"]]
-        bi, bwg = self.bodies[t] if code not in (12, 13, 14, 23, 26, 27, 29, 31, 33, 34) else (0, 0)
+        bi, bwg = self.bodies[t] if code not in (12, 13, 14, 23, 26, 27, 29, 31, 33, 34, 35, 36) else (0, 0)
         if code == 1:
             L.f2d_body_apply_force(bi, bwg, f[0], f[1], f[2], f[3], flag)
         elif code == 2:
@@ -230,6 +230,18 @@ class FlatWorld:
             L.f2d_distance_joint_enable_spring(*j, int(flag & 4 != 0))
         elif code == 34:
             L.f2d_distance_joint_set_length(*self.joints[t], f[0])
+        elif code == 35:
+            j = self.joints[t]
+            L.f2d_revolute_joint_set_limits(*j, f[0], f[1])
+            L.f2d_revolute_joint_enable_limit(*j, int(flag & 1 != 0))
+            L.f2d_revolute_joint_set_max_motor_torque(*j, f[2])
+            L.f2d_revolute_joint_set_motor_speed(*j, f[3])
+            L.f2d_revolute_joint_enable_motor(*j, int(flag & 2 != 0))
+            L.f2d_revolute_joint_set_spring_hertz(*j, f[4])
+            L.f2d_revolute_joint_set_spring_damping_ratio(*j, f[5])
+            L.f2d_revolute_joint_enable_spring(*j, int(flag & 4 != 0))
+        elif code == 36:
+            L.f2d_revolute_joint_set_target_angle(*self.joints[t], f[0])
         else:
             raise ValueError(f"op {code}")
 

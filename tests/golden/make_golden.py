@@ -289,7 +289,8 @@ OPS = {"apply_force": 1, "apply_force_to_center": 2, "apply_torque": 3, "apply_l
        "set_linear_damping": 16, "set_angular_damping": 17, "set_fixed_rotation": 18, "set_bullet": 19,
        "set_awake": 20, "joint_wake_bodies": 23, "create_prismatic": 25, "prismatic_config": 26,
        "prismatic_target": 27, "create_weld": 28, "weld_config": 29, "create_wheel": 30, "wheel_config": 31,
-       "create_distance": 32, "distance_config": 33, "distance_length": 34}
+       "create_distance": 32, "distance_config": 33, "distance_length": 34, "revolute_config": 35,
+       "revolute_target": 36}
 
 
 def ops_array(rows):
@@ -474,12 +475,24 @@ DISTANCE_OPS = [
     (45, "distance_config", 3, 5, [1.0, 1.1, 0.0, 0.0, 5.0, 0.9]),
 ]
 
+# the revolute chain of body_ops_world (joints 0..2): motor, limits, spring and target
+# changed through the revolute joint calls
+REVOLUTE_OPS = [
+    (2, "revolute_config", 0, 2, [-0.5, 0.5, 5.0, 2.0, 0.0, 0.0]),    # motor on, limit off
+    (10, "revolute_config", 1, 5, [-0.2, 0.3, 0.0, 0.0, 2.0, 0.5]),   # limit + spring
+    (10, "revolute_target", 1, 0, [0.25]),
+    (20, "revolute_config", 0, 3, [-0.3, 0.3, 5.0, -3.0, 0.0, 0.0]),  # motor reversed, limits moved
+    (30, "revolute_config", 2, 4, [-0.5, 0.5, 0.0, 0.0, 4.0, 0.1]),   # limit off, spring on
+    (35, "revolute_config", 1, 1, [-0.2, 0.3, 0.0, 0.0, 2.0, 0.5]),   # spring off
+]
+
 OPS_FIXTURES = {
     # name: (world factory, ops, dumped steps)
     "ops_bodies": (body_ops_world, BODY_OPS, list(range(0, 41))),
     "ops_prismatic": (prismatic_world, PRISMATIC_OPS, list(range(0, 61))),
     "ops_weld_wheel": (weld_wheel_world, WELD_WHEEL_OPS, list(range(0, 61))),
     "ops_distance": (distance_world, DISTANCE_OPS, list(range(0, 61))),
+    "ops_revolute": (body_ops_world, REVOLUTE_OPS, list(range(0, 41))),
 }
 
 

@@ -334,6 +334,49 @@ def test_distance_fixture_exercises_every_branch():
     assert (J["lower_impulse"][:, 2] > 0).any() and (J["upper_impulse"][:, 2] > 0).any()
 
 
+def test_revolute_fixture_exercises_every_branch():
+    """Over the 40 reference steps of ops_revolute: motor impulses of both signs on
+    joint 0, lower and upper limit impulses on every joint, and spring impulses of both
+    signs on joints 1 and 2."""
+    spec, snaps, ops = load_ops_fixture("ops_revolute", GOLDEN)
+    J = np.stack([s.joints for s in snaps[1:]])
+    assert (J["reserved"] == 0.0).all()
+    assert (J["motor_impulse"][:, 0] > 0).any() and (J["motor_impulse"][:, 0] < 0).any()
+    assert ((J["lower_impulse"] > 0).any(axis=0)).all() and ((J["upper_impulse"] > 0).any(axis=0)).all()
+    for k in (1, 2):
+        assert (J["spring_impulse"][:, k] > 0).any() and (J["spring_impulse"][:, k] < 0).any()
+
+
+def test_revolute_joint_calls():
+    """The revolute parameter calls of f2d_shim.c:1097-1136 store what they are given
+    (limits ordered as S/revolute_joint.c:113-127 orders them) and read it back; the
+    joint angle of a fresh chain link is 0."""
+    spec, snaps, ops = load_ops_fixture("ops_revolute", GOLDEN)
+    fw = FlatWorld(spec)
+    L = F.lib()
+    j = fw.joints[1]
+    assert L.f2d_revolute_joint_is_limit_enabled(*j) == 1 and L.f2d_revolute_joint_is_motor_enabled(*j) == 0
+    assert (L.f2d_revolute_joint_get_lower_limit(*j), L.f2d_revolute_joint_get_upper_limit(*j)) == (-0.5, 0.5)
+    L.f2d_revolute_joint_set_limits(*j, 0.4, -0.2)
+    assert (L.f2d_revolute_joint_get_lower_limit(*j), L.f2d_revolute_joint_get_upper_limit(*j)) \
+        == (np.float32(-0.2), np.float32(0.4))
+    L.f2d_revolute_joint_enable_motor(*j, 1)
+    L.f2d_revolute_joint_set_motor_speed(*j, 1.5)
+    L.f2d_revolute_joint_set_max_motor_torque(*j, 7.0)
+    L.f2d_revolute_joint_enable_spring(*j, 1)
+    L.f2d_revolute_joint_set_spring_hertz(*j, 3.0)
+    L.f2d_revolute_joint_set_spring_damping_ratio(*j, 0.25)
+    L.f2d_revolute_joint_set_target_angle(*j, 0.125)
+    assert L.f2d_revolute_joint_is_motor_enabled(*j) == 1 and L.f2d_revolute_joint_is_spring_enabled(*j) == 1
+    assert (L.f2d_revolute_joint_get_motor_speed(*j), L.f2d_revolute_joint_get_max_motor_torque(*j)) == (1.5, 7.0)
+    assert (L.f2d_revolute_joint_get_spring_hertz(*j), L.f2d_revolute_joint_get_spring_damping_ratio(*j)) \
+        == (3.0, 0.25)
+    assert L.f2d_revolute_joint_get_target_angle(*j) == 0.125
+    assert L.f2d_revolute_joint_get_motor_torque(*j) == 0.0
+    assert L.f2d_revolute_joint_get_angle(*j) == 0.0
+    fw.close()
+
+
 def test_joint_calls_on_other_types_are_ignored():
     """A joint parameter call of the wrong joint type leaves the joint alone and its
     getter returns 0, as the reference's type check would refuse it."""

@@ -112,6 +112,24 @@ def test_weld_and_wheel_joints_match_reference(gpu):
     fw.close()
 
 
+def test_revolute_joint_calls_match_reference(gpu):
+    """The f2d_revolute_joint_* calls (fixture ops_revolute on the revolute chain of
+    ops_bodies: a motor turned on and later reversed, limits moved and turned off, a
+    spring with a target angle turned on and off): after every one of 40 steps the
+    state and every joint's impulses equal the reference's bit for bit."""
+    fw, snaps = replay_joint_fixture("ops_revolute", 3)
+    import forge2d_amd as F
+    L = F.lib()
+    j0 = fw.joints[0]
+    assert L.f2d_revolute_joint_get_motor_speed(*j0) == -3.0
+    assert (L.f2d_revolute_joint_get_lower_limit(*j0), L.f2d_revolute_joint_get_upper_limit(*j0)) \
+        == (np.float32(-0.3), np.float32(0.3))
+    assert L.f2d_revolute_joint_get_target_angle(*fw.joints[1]) == np.float32(0.25)
+    assert L.f2d_revolute_joint_is_spring_enabled(*fw.joints[1]) == 0
+    assert L.f2d_revolute_joint_is_spring_enabled(*fw.joints[2]) == 1
+    fw.close()
+
+
 def test_distance_joints_match_reference(gpu):
     """Distance joints (S/distance_joint.c) made through f2d_create_distance_joint and the
     f2d_distance_joint_* calls (fixture ops_distance: a rigid rope, a spring with a
