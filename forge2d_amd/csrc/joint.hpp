@@ -203,7 +203,7 @@ F2D_HD void solveRevolute(Joint& j, BodyState* states, float h, float inv_h, boo
 
 // Prismatic joint stages, S/prismatic_joint.c: prepare :279-347 (with the common
 // part of S/joint.c:1348-1355), warm start :349-392, solve :394-631.
-F2D_HD void preparePrismatic(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+F2D_HD_COLD void preparePrismatic(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
   float hertz = minf(j.constraintHertz, 0.25f * inv_h);
   j.constraintSoftness = makeSoft(hertz, j.constraintDampingRatio, h);
   const BodySim& a = sims[j.bodyA];
@@ -238,7 +238,7 @@ F2D_HD void preparePrismatic(Joint& j, const BodySim* sims, float h, float inv_h
   }
 }
 
-F2D_HD void warmStartPrismatic(Joint& j, BodyState* states) {
+F2D_HD_COLD void warmStartPrismatic(Joint& j, BodyState* states) {
   const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
   const int ia = j.simA, ib = j.simB;
   BodyState sA = loadState(states, ia);
@@ -267,7 +267,7 @@ F2D_HD void warmStartPrismatic(Joint& j, BodyState* states) {
   storeVel(states, ib, sB.v, sB.w);
 }
 
-F2D_HD void solvePrismatic(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
+F2D_HD_COLD void solvePrismatic(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
   const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
   const int ia = j.simA, ib = j.simB;
   const BodyState sA = loadState(states, ia);
@@ -408,7 +408,7 @@ F2D_HD void solvePrismatic(Joint& j, BodyState* states, float h, float inv_h, bo
 }
 
 // Weld joint stages, S/weld_joint.c: prepare :91-163, warm start :165-188, solve :190-275.
-F2D_HD void prepareWeld(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+F2D_HD_COLD void prepareWeld(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
   float hertz = minf(j.constraintHertz, 0.25f * inv_h);
   j.constraintSoftness = makeSoft(hertz, j.constraintDampingRatio, h);
   const BodySim& a = sims[j.bodyA];
@@ -437,7 +437,7 @@ F2D_HD void prepareWeld(Joint& j, const BodySim* sims, float h, float inv_h, flo
   }
 }
 
-F2D_HD void warmStartWeld(Joint& j, BodyState* states) {
+F2D_HD_COLD void warmStartWeld(Joint& j, BodyState* states) {
   const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
   const int ia = j.simA, ib = j.simB;
   BodyState sA = loadState(states, ia);
@@ -452,7 +452,7 @@ F2D_HD void warmStartWeld(Joint& j, BodyState* states) {
   storeVel(states, ib, sB.v, sB.w);
 }
 
-F2D_HD void solveWeld(Joint& j, BodyState* states, bool useBias) {
+F2D_HD_COLD void solveWeld(Joint& j, BodyState* states, bool useBias) {
   const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
   const int ia = j.simA, ib = j.simB;
   const BodyState sA = loadState(states, ia);
@@ -507,7 +507,7 @@ F2D_HD void solveWeld(Joint& j, BodyState* states, bool useBias) {
 }
 
 // Wheel joint stages, S/wheel_joint.c: prepare :183-263, warm start :265-304, solve :306-491.
-F2D_HD void prepareWheel(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+F2D_HD_COLD void prepareWheel(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
   float hertz = minf(j.constraintHertz, 0.25f * inv_h);
   j.constraintSoftness = makeSoft(hertz, j.constraintDampingRatio, h);
   const BodySim& a = sims[j.bodyA];
@@ -550,7 +550,7 @@ F2D_HD void prepareWheel(Joint& j, const BodySim* sims, float h, float inv_h, fl
   }
 }
 
-F2D_HD void warmStartWheel(Joint& j, BodyState* states) {
+F2D_HD_COLD void warmStartWheel(Joint& j, BodyState* states) {
   const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
   const int ia = j.simA, ib = j.simB;
   BodyState sA = loadState(states, ia);
@@ -577,7 +577,7 @@ F2D_HD void warmStartWheel(Joint& j, BodyState* states) {
   storeVel(states, ib, sB.v, sB.w);
 }
 
-F2D_HD void solveWheel(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
+F2D_HD_COLD void solveWheel(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
   const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
   const int ia = j.simA, ib = j.simB;
   const BodyState sA = loadState(states, ia);
@@ -700,7 +700,7 @@ F2D_HD void solveWheel(Joint& j, BodyState* states, float h, float inv_h, bool u
 
 // Distance joint stages, S/distance_joint.c: prepare :215-278, warm start :280-310,
 // solve :312-490.
-F2D_HD void prepareDistance(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+F2D_HD_COLD void prepareDistance(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
   float hertz = minf(j.constraintHertz, 0.25f * inv_h);
   j.constraintSoftness = makeSoft(hertz, j.constraintDampingRatio, h);
   const BodySim& a = sims[j.bodyA];
@@ -731,7 +731,7 @@ F2D_HD void prepareDistance(Joint& j, const BodySim* sims, float h, float inv_h,
   }
 }
 
-F2D_HD void warmStartDistance(Joint& j, BodyState* states) {
+F2D_HD_COLD void warmStartDistance(Joint& j, BodyState* states) {
   const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
   const int ia = j.simA, ib = j.simB;
   BodyState sA = loadState(states, ia);
@@ -751,7 +751,7 @@ F2D_HD void warmStartDistance(Joint& j, BodyState* states) {
   storeVel(states, ib, sB.v, sB.w);
 }
 
-F2D_HD void solveDistance(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
+F2D_HD_COLD void solveDistance(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
   const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
   const int ia = j.simA, ib = j.simB;
   const BodyState sA = loadState(states, ia);
@@ -856,8 +856,11 @@ F2D_HD void solveDistance(Joint& j, BodyState* states, float h, float inv_h, boo
 }
 
 // The joint stages by joint type (b2PrepareJointsTask / b2WarmStartJoint /
-// b2SolveJoint, S/joint.c:1348-1450)
-F2D_HD void prepareJoint(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+// b2SolveJoint, S/joint.c:1348-1450). Revolute joints take an inlined path; the other
+// types are reached through out-of-line calls, which keeps the solve kernel at the
+// register count of the revolute code (inlined together, the five types raise it from
+// 77 to 248 VGPRs and the occupancy from 6 to 2 waves per SIMD).
+F2D_HD_COLD void prepareJointOther(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
   switch (j.type) {
     case kPrismaticJoint: preparePrismatic(j, sims, h, inv_h, warmStartScale); break;
     case kWeldJoint: prepareWeld(j, sims, h, inv_h, warmStartScale); break;
@@ -866,7 +869,7 @@ F2D_HD void prepareJoint(Joint& j, const BodySim* sims, float h, float inv_h, fl
     default: prepareRevolute(j, sims, h, inv_h, warmStartScale); break;
   }
 }
-F2D_HD void warmStartJoint(Joint& j, BodyState* states) {
+F2D_HD_COLD void warmStartJointOther(Joint& j, BodyState* states) {
   switch (j.type) {
     case kPrismaticJoint: warmStartPrismatic(j, states); break;
     case kWeldJoint: warmStartWeld(j, states); break;
@@ -875,7 +878,7 @@ F2D_HD void warmStartJoint(Joint& j, BodyState* states) {
     default: warmStartRevolute(j, states); break;
   }
 }
-F2D_HD void solveJoint(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
+F2D_HD_COLD void solveJointOther(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
   switch (j.type) {
     case kPrismaticJoint: solvePrismatic(j, states, h, inv_h, useBias); break;
     case kWeldJoint: solveWeld(j, states, useBias); break;
@@ -883,6 +886,18 @@ F2D_HD void solveJoint(Joint& j, BodyState* states, float h, float inv_h, bool u
     case kDistanceJoint: solveDistance(j, states, h, inv_h, useBias); break;
     default: solveRevolute(j, states, h, inv_h, useBias); break;
   }
+}
+F2D_HD void prepareJoint(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+  if (j.type == kRevoluteJoint) prepareRevolute(j, sims, h, inv_h, warmStartScale);
+  else prepareJointOther(j, sims, h, inv_h, warmStartScale);
+}
+F2D_HD void warmStartJoint(Joint& j, BodyState* states) {
+  if (j.type == kRevoluteJoint) warmStartRevolute(j, states);
+  else warmStartJointOther(j, states);
+}
+F2D_HD void solveJoint(Joint& j, BodyState* states, float h, float inv_h, bool useBias) {
+  if (j.type == kRevoluteJoint) solveRevolute(j, states, h, inv_h, useBias);
+  else solveJointOther(j, states, h, inv_h, useBias);
 }
 
 }  // namespace f2d
