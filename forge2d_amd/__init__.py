@@ -220,9 +220,24 @@ SIGNATURES = [
     (f"f2d_distance_joint_{n}", None, [ctypes.c_int32, _U, _I]) for n in
     ["enable_spring", "enable_limit", "enable_motor"]] + [
     ("f2d_distance_joint_set_length_range", None, [ctypes.c_int32, _U, _F, _F]),
-] + [(f"f2d_revolute_joint_get_{n}", _F, [ctypes.c_int32, _U]) for n in
-     ["spring_hertz", "spring_damping_ratio", "target_angle", "angle", "lower_limit", "upper_limit", "motor_speed",
-      "max_motor_torque", "motor_torque"]] + [
+    ("f2d_create_motor_joint", None, [_U, ctypes.c_int32, _U, ctypes.c_int32, _U, _F, _F, _F, _F, _F, _F, _I,
+                                      ctypes.POINTER(ctypes.c_int32)]),
+    ("f2d_create_mouse_joint", None, [_U, ctypes.c_int32, _U, ctypes.c_int32, _U, _F, _F, _F, _F, _F, _I,
+                                      ctypes.POINTER(ctypes.c_int32)]),
+    ("f2d_create_filter_joint", None, [_U, ctypes.c_int32, _U, ctypes.c_int32, _U, ctypes.POINTER(ctypes.c_int32)]),
+    ("f2d_motor_joint_get_linear_offset", None, [ctypes.c_int32, _U, ctypes.POINTER(ctypes.c_float)]),
+    ("f2d_motor_joint_set_linear_offset", None, [ctypes.c_int32, _U, _F, _F]),
+    ("f2d_mouse_joint_get_target", None, [ctypes.c_int32, _U, ctypes.POINTER(ctypes.c_float)]),
+    ("f2d_mouse_joint_set_target", None, [ctypes.c_int32, _U, _F, _F]),
+] + [(f"f2d_{t}_joint_get_{n}", _F, [ctypes.c_int32, _U]) for t, n in
+     [("motor", "angular_offset"), ("motor", "max_force"), ("motor", "max_torque"), ("motor", "correction_factor"),
+      ("mouse", "spring_hertz"), ("mouse", "spring_damping_ratio"), ("mouse", "max_force")]] + [
+    (f"f2d_{t}_joint_set_{n}", None, [ctypes.c_int32, _U, _F]) for t, n in
+    [("motor", "angular_offset"), ("motor", "max_force"), ("motor", "max_torque"), ("motor", "correction_factor"),
+     ("mouse", "spring_hertz"), ("mouse", "spring_damping_ratio"), ("mouse", "max_force")]] + [
+    (f"f2d_revolute_joint_get_{n}", _F, [ctypes.c_int32, _U]) for n in
+    ["spring_hertz", "spring_damping_ratio", "target_angle", "angle", "lower_limit", "upper_limit", "motor_speed",
+     "max_motor_torque", "motor_torque"]] + [
     (f"f2d_revolute_joint_set_{n}", None, [ctypes.c_int32, _U, _F]) for n in
     ["spring_hertz", "spring_damping_ratio", "target_angle", "motor_speed", "max_motor_torque"]] + [
     (f"f2d_revolute_joint_{n}", _I, [ctypes.c_int32, _U]) for n in

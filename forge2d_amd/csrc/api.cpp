@@ -475,6 +475,47 @@ int HostWorld::createDistance(int bodyA, int bodyB, Vec2 la, Vec2 lb, float leng
   return id;
 }
 
+// b2CreateMotorJoint, S/joint.c:406-442: anchors at the body origins; maximum force and
+// torque as given (the setters clamp them at 0), correction factor clamped to [0, 1]
+int HostWorld::createMotor(int bodyA, int bodyB, Vec2 linearOffset, float angularOffset, float maxForce,
+                           float maxTorque, float correctionFactor, bool collideConnected) {
+  const int id = createRevolute(bodyA, bodyB, {0.0f, 0.0f}, {0.0f, 0.0f}, 0.0f, 0.0f, false, 0.0f, 0.0f, false, 0.0f,
+                                0.0f, false, 0.0f, 0.0f, collideConnected);
+  Joint& j = joints[id];
+  j.type = kMotorJoint;
+  j.localAxisA = linearOffset;
+  j.referenceAngle = angularOffset;
+  j.maxMotorTorque = maxForce;
+  j.angularHertz = maxTorque;
+  j.angularDampingRatio = clampf(correctionFactor, 0.0f, 1.0f);
+  return id;
+}
+
+// b2CreateMouseJoint, S/joint.c:444-478: the target in each body's frame as the local
+// anchors (transforms of the moment of creation)
+int HostWorld::createMouse(int bodyA, int bodyB, Vec2 target, float hertz, float damping, float maxForce,
+                           bool collideConnected) {
+  const Vec2 la = invTransformPoint(sims[bodyA].xf, target);
+  const Vec2 lb = invTransformPoint(sims[bodyB].xf, target);
+  const int id = createRevolute(bodyA, bodyB, la, lb, 0.0f, 0.0f, false, 0.0f, 0.0f, false, 0.0f, 0.0f, false, 0.0f,
+                                0.0f, collideConnected);
+  Joint& j = joints[id];
+  j.type = kMouseJoint;
+  j.localAxisA = target;
+  j.hertz = hertz;
+  j.dampingRatio = damping;
+  j.maxMotorTorque = maxForce;
+  return id;
+}
+
+// b2CreateFilterJoint, S/joint.c:480-505: no constraint, collideConnected always false
+int HostWorld::createFilter(int bodyA, int bodyB) {
+  const int id = createRevolute(bodyA, bodyB, {0.0f, 0.0f}, {0.0f, 0.0f}, 0.0f, 0.0f, false, 0.0f, 0.0f, false, 0.0f,
+                                0.0f, false, 0.0f, 0.0f, false);
+  joints[id].type = kFilterJoint;
+  return id;
+}
+
 IslandCtx HostWorld::islandCtx() {
   // the list-walk bounds read the world's current sizes
   hdr.bodyCount = (int)sims.size();
@@ -1446,11 +1487,14 @@ static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
     sj.lower_impulse = j.lowerImpulse;
     sj.upper_impulse = j.upperImpulse;
     sj.ref_angle = j.referenceAngle;
-    // joint kind: 0 revolute, 1 prismatic, 2 weld, 3 wheel, 4 distance
+    // joint kind: 0 revolute, 1 prismatic, 2 weld, 3 wheel, 4 distance, 5 motor, 6 mouse, 7 filter
     sj.reserved = j.type == kPrismaticJoint ? 1.0f
                   : j.type == kWeldJoint    ? 2.0f
                   : j.type == kWheelJoint   ? 3.0f
                   : j.type == kDistanceJoint ? 4.0f
+                  : j.type == kMotorJoint   ? 5.0f
+                  : j.type == kMouseJoint   ? 6.0f
+                  : j.type == kFilterJoint  ? 7.0f
                                              : 0.0f;
     put(out, sj);
   }
@@ -3638,8 +3682,12 @@ void f2d_joint_get_constraint_force(int32_t index1, uint32_t wg, float* out) {
     const Vec2 axis = normalize(sub(pB, pA));
     const float force = (j.linearImpulse.x + j.lowerImpulse - j.upperImpulse + j.motorImpulse) * hw->inv_h;
     f = mulSV(force, axis);
+  } else if (j.type == kFilterJoint) {
+    f = {0.0f, 0.0f};
   } else {
-    f = mulSV(hw->inv_h, j.linearImpulse);  // revolute, weld (S/weld_joint.c:66-70)
+    // revolute, weld (S/weld_joint.c:66-70), motor (S/motor_joint.c:73-77), mouse
+    // (S/mouse_joint.c:65-69)
+    f = mulSV(hw->inv_h, j.linearImpulse);
   }
   out[0] = f.x;
   out[1] = f.y;
@@ -3654,7 +3702,8 @@ float f2d_joint_get_constraint_torque(int32_t index1, uint32_t wg) {
   if (j.type == kPrismaticJoint) return hw->inv_h * j.linearImpulse.y;
   if (j.type == kWeldJoint) return hw->inv_h * j.springImpulse;  // angular impulse, S/weld_joint.c:72-75
   if (j.type == kWheelJoint) return hw->inv_h * j.motorImpulse;  // S/wheel_joint.c:162-165
-  if (j.type == kDistanceJoint) return 0.0f;  // S/joint.c b2GetJointTorque: no torque
+  if (j.type == kDistanceJoint || j.type == kFilterJoint) return 0.0f;  // S/joint.c:1100-1120
+  if (j.type == kMotorJoint || j.type == kMouseJoint) return hw->inv_h * j.springImpulse;  // angular impulse
   return hw->inv_h * (j.motorImpulse + j.lowerImpulse - j.upperImpulse);
 }
 
@@ -3696,6 +3745,60 @@ F2D_JOINT_FLOAT(f2d_distance_joint_get_spring_damping_ratio, f2d_distance_joint_
 F2D_JOINT_FLOAT(f2d_distance_joint_get_motor_speed, f2d_distance_joint_set_motor_speed, kDistanceJoint, motorSpeed)
 F2D_JOINT_FLOAT(f2d_distance_joint_get_max_motor_force, f2d_distance_joint_set_max_motor_force, kDistanceJoint,
                 maxMotorTorque)
+
+// ---- motor and mouse joint parameters (f2d_shim.c:1006-1050 -> S/motor_joint.c:14-71,
+// S/mouse_joint.c:14-63): stored as given, no impulse reset and no wake; the motor's
+// maximum force and torque are clamped at 0 and its correction factor to [0, 1]
+F2D_JOINT_FLOAT(f2d_motor_joint_get_angular_offset, f2d_motor_joint_set_angular_offset, kMotorJoint, referenceAngle)
+F2D_JOINT_FLOAT(f2d_mouse_joint_get_spring_hertz, f2d_mouse_joint_set_spring_hertz, kMouseJoint, hertz)
+F2D_JOINT_FLOAT(f2d_mouse_joint_get_spring_damping_ratio, f2d_mouse_joint_set_spring_damping_ratio, kMouseJoint,
+                dampingRatio)
+F2D_JOINT_FLOAT(f2d_mouse_joint_get_max_force, f2d_mouse_joint_set_max_force, kMouseJoint, maxMotorTorque)
+static float motorGet(int32_t index1, uint32_t wg, float Joint::*field) {
+  const Joint* j = jointOfType(index1, wg, kMotorJoint);
+  return j ? j->*field : 0.0f;
+}
+static void motorSet(int32_t index1, uint32_t wg, float Joint::*field, float v) {
+  Joint* j = jointOfType(index1, wg, kMotorJoint);
+  if (!j) return;
+  j->*field = v;
+  bodyBatch(wg)->hostDirty = true;
+}
+float f2d_motor_joint_get_max_force(int32_t index1, uint32_t wg) { return motorGet(index1, wg, &Joint::maxMotorTorque); }
+void f2d_motor_joint_set_max_force(int32_t index1, uint32_t wg, float v) {
+  motorSet(index1, wg, &Joint::maxMotorTorque, maxf(0.0f, v));
+}
+float f2d_motor_joint_get_max_torque(int32_t index1, uint32_t wg) { return motorGet(index1, wg, &Joint::angularHertz); }
+void f2d_motor_joint_set_max_torque(int32_t index1, uint32_t wg, float v) {
+  motorSet(index1, wg, &Joint::angularHertz, maxf(0.0f, v));
+}
+float f2d_motor_joint_get_correction_factor(int32_t index1, uint32_t wg) {
+  return motorGet(index1, wg, &Joint::angularDampingRatio);
+}
+void f2d_motor_joint_set_correction_factor(int32_t index1, uint32_t wg, float v) {
+  motorSet(index1, wg, &Joint::angularDampingRatio, clampf(v, 0.0f, 1.0f));
+}
+static void jointVec2(int32_t index1, uint32_t wg, int type, float* out) {
+  const Joint* j = jointOfType(index1, wg, type);
+  out[0] = j ? j->localAxisA.x : 0.0f;
+  out[1] = j ? j->localAxisA.y : 0.0f;
+}
+static void setJointVec2(int32_t index1, uint32_t wg, int type, float x, float y) {
+  Joint* j = jointOfType(index1, wg, type);
+  if (!j) return;
+  j->localAxisA = {x, y};
+  bodyBatch(wg)->hostDirty = true;
+}
+void f2d_motor_joint_get_linear_offset(int32_t index1, uint32_t wg, float* out) {
+  jointVec2(index1, wg, kMotorJoint, out);
+}
+void f2d_motor_joint_set_linear_offset(int32_t index1, uint32_t wg, float x, float y) {
+  setJointVec2(index1, wg, kMotorJoint, x, y);
+}
+void f2d_mouse_joint_get_target(int32_t index1, uint32_t wg, float* out) { jointVec2(index1, wg, kMouseJoint, out); }
+void f2d_mouse_joint_set_target(int32_t index1, uint32_t wg, float x, float y) {
+  setJointVec2(index1, wg, kMouseJoint, x, y);
+}
 
 // ---- revolute joint (S/revolute_joint.c:20-176): the same rules as the prismatic calls
 static void enableWithReset(Joint* j, int& flag, int v, float* a, float* b = nullptr) {
@@ -4267,6 +4370,57 @@ void f2d_create_distance_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int3
   int id = hw->createDistance(ia, ib, {lax, lay}, {lbx, lby}, length, enable_spring != 0, hertz, damping_ratio,
                               enable_limit != 0, min_length, max_length, enable_motor != 0, max_motor_force,
                               motor_speed, collide_connected != 0);
+  b->hostDirty = true;
+  out[0] = id + 1;
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+}
+
+// f2d_shim.c:751-768
+void f2d_create_motor_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t b_index1, uint32_t b_wg,
+                            float offset_x, float offset_y, float angular_offset, float max_force, float max_torque,
+                            float correction_factor, int collide_connected, int32_t* out) {
+  out[0] = 0;
+  out[1] = 0;
+  Batch* b = batchOf(w);
+  int ia, ib;
+  HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
+  if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (b->committed) freeDevice(*b);
+  int id = hw->createMotor(ia, ib, {offset_x, offset_y}, angular_offset, max_force, max_torque, correction_factor,
+                           collide_connected != 0);
+  b->hostDirty = true;
+  out[0] = id + 1;
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+}
+
+// f2d_shim.c:770-785
+void f2d_create_mouse_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t b_index1, uint32_t b_wg,
+                            float target_x, float target_y, float hertz, float damping_ratio, float max_force,
+                            int collide_connected, int32_t* out) {
+  out[0] = 0;
+  out[1] = 0;
+  Batch* b = batchOf(w);
+  int ia, ib;
+  HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
+  if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (b->committed) freeDevice(*b);
+  int id = hw->createMouse(ia, ib, {target_x, target_y}, hertz, damping_ratio, max_force, collide_connected != 0);
+  b->hostDirty = true;
+  out[0] = id + 1;
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+}
+
+// f2d_shim.c:742-749
+void f2d_create_filter_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t b_index1, uint32_t b_wg,
+                             int32_t* out) {
+  out[0] = 0;
+  out[1] = 0;
+  Batch* b = batchOf(w);
+  int ia, ib;
+  HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
+  if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (b->committed) freeDevice(*b);
+  int id = hw->createFilter(ia, ib);
   b->hostDirty = true;
   out[0] = id + 1;
   out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));

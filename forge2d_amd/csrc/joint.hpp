@@ -855,6 +855,203 @@ F2D_HD_COLD void solveDistance(Joint& j, BodyState* states, float h, float inv_h
   storeVel(states, ib, vB, wB);
 }
 
+// b2GetInverse22 (I/math_functions.h:551-564) of K = [[k11, k12], [k12, k22]], written
+// into the two columns (cx, cy) that b2MulMV reads (I/math_functions.h:541-548)
+F2D_HD void inverse22(float k11, float k12, float k22, Vec2& cx, Vec2& cy) {
+  const float a = k11, b = k12, c = k12, d = k22;
+  float det = a * d - b * c;
+  if (det != 0.0f) det = 1.0f / det;
+  cx = {det * d, -det * c};
+  cy = {-det * b, det * a};
+}
+F2D_HD Vec2 mulMV(Vec2 cx, Vec2 cy, Vec2 v) { return {cx.x * v.x + cy.x * v.y, cx.y * v.x + cy.y * v.y}; }
+
+// Motor joint, S/motor_joint.c:99-254. Fields: localAxisA = linear offset,
+// referenceAngle = angular offset, maxMotorTorque = maximum force, angularHertz =
+// maximum torque, angularDampingRatio = correction factor; prepared linearMass in
+// (axisA, {perpMass, motorMass}) as columns, angularMass in axialMass; the angular
+// impulse in springImpulse.
+F2D_HD_COLD void prepareMotor(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+  float hertz = minf(j.constraintHertz, 0.25f * inv_h);
+  j.constraintSoftness = makeSoft(hertz, j.constraintDampingRatio, h);
+  const BodySim& a = sims[j.bodyA];
+  const BodySim& b = sims[j.bodyB];
+  const float mA = a.invMass, iA = a.invI, mB = b.invMass, iB = b.invI;
+  j.invMassA = mA;
+  j.invMassB = mB;
+  j.invIA = iA;
+  j.invIB = iB;
+  j.simA = a.setIndex == kAwakeSet ? j.bodyA : -1;
+  j.simB = b.setIndex == kAwakeSet ? j.bodyB : -1;
+  j.anchorA = rotate(a.xf.q, sub(j.localOriginAnchorA, a.localCenter));
+  j.anchorB = rotate(b.xf.q, sub(j.localOriginAnchorB, b.localCenter));
+  j.deltaCenter = sub(sub(b.center, a.center), j.localAxisA);
+  j.deltaAngle = relativeAngle(b.xf.q, a.xf.q) - j.referenceAngle;
+  const Vec2 rA = j.anchorA, rB = j.anchorB;
+  const float k11 = mA + mB + rA.y * rA.y * iA + rB.y * rB.y * iB;
+  const float k12 = -rA.y * rA.x * iA - rB.y * rB.x * iB;
+  const float k22 = mA + mB + rA.x * rA.x * iA + rB.x * rB.x * iB;
+  Vec2 cy;
+  inverse22(k11, k12, k22, j.axisA, cy);
+  j.perpMass = cy.x;
+  j.motorMass = cy.y;
+  const float ka = iA + iB;
+  j.axialMass = ka > 0.0f ? 1.0f / ka : 0.0f;
+  if (warmStartScale == 0.0f) {
+    j.linearImpulse = {0.0f, 0.0f};
+    j.springImpulse = 0.0f;
+  }
+}
+
+F2D_HD_COLD void warmStartMotor(Joint& j, BodyState* states) {
+  const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+  const int ia = j.simA, ib = j.simB;
+  BodyState sA = loadState(states, ia);
+  BodyState sB = loadState(states, ib);
+  const Vec2 rA = rotate(sA.dq, j.anchorA);
+  const Vec2 rB = rotate(sB.dq, j.anchorB);
+  sA.v = mulSub(sA.v, mA, j.linearImpulse);
+  sA.w -= iA * (cross(rA, j.linearImpulse) + j.springImpulse);
+  sB.v = mulAdd(sB.v, mB, j.linearImpulse);
+  sB.w += iB * (cross(rB, j.linearImpulse) + j.springImpulse);
+  storeVel(states, ia, sA.v, sA.w);
+  storeVel(states, ib, sB.v, sB.w);
+}
+
+F2D_HD_COLD void solveMotor(Joint& j, BodyState* states, float h, float inv_h) {
+  const float mA = j.invMassA, mB = j.invMassB, iA = j.invIA, iB = j.invIB;
+  const int ia = j.simA, ib = j.simB;
+  const BodyState sA = loadState(states, ia);
+  const BodyState sB = loadState(states, ib);
+  Vec2 vA = sA.v, vB = sB.v;
+  float wA = sA.w, wB = sB.w;
+  const float correction = j.angularDampingRatio;
+  {
+    float angularSeparation = relativeAngle(sB.dq, sA.dq) + j.deltaAngle;
+    angularSeparation = unwindAngle(angularSeparation);
+    const float angularBias = inv_h * correction * angularSeparation;
+    const float Cdot = wB - wA;
+    float impulse = -j.axialMass * (Cdot + angularBias);
+    const float oldImpulse = j.springImpulse;
+    const float maxImpulse = h * j.angularHertz;
+    j.springImpulse = clampf(j.springImpulse + impulse, -maxImpulse, maxImpulse);
+    impulse = j.springImpulse - oldImpulse;
+    wA -= iA * impulse;
+    wB += iB * impulse;
+  }
+  {
+    const Vec2 rA = rotate(sA.dq, j.anchorA);
+    const Vec2 rB = rotate(sB.dq, j.anchorB);
+    const Vec2 ds = add(sub(sB.dp, sA.dp), sub(rB, rA));
+    const Vec2 linearSeparation = add(j.deltaCenter, ds);
+    const Vec2 linearBias = mulSV(inv_h * correction, linearSeparation);
+    const Vec2 Cdot = sub(add(vB, crossSV(wB, rB)), add(vA, crossSV(wA, rA)));
+    const Vec2 b = mulMV(j.axisA, {j.perpMass, j.motorMass}, add(Cdot, linearBias));
+    Vec2 impulse = {-b.x, -b.y};
+    const Vec2 oldImpulse = j.linearImpulse;
+    const float maxImpulse = h * j.maxMotorTorque;
+    j.linearImpulse = add(j.linearImpulse, impulse);
+    if (lengthSq(j.linearImpulse) > maxImpulse * maxImpulse) {
+      j.linearImpulse = normalize(j.linearImpulse);
+      j.linearImpulse.x *= maxImpulse;
+      j.linearImpulse.y *= maxImpulse;
+    }
+    impulse = sub(j.linearImpulse, oldImpulse);
+    vA = mulSub(vA, mA, impulse);
+    wA -= iA * cross(rA, impulse);
+    vB = mulAdd(vB, mB, impulse);
+    wB += iB * cross(rB, impulse);
+  }
+  storeVel(states, ia, vA, wA);
+  storeVel(states, ib, vB, wB);
+}
+
+// Mouse joint, S/mouse_joint.c:77-214: body B only. Fields: localAxisA = target,
+// maxMotorTorque = maximum force, springSoftness = linear softness, angularSoftness
+// (0.5 Hz, damping 0.1); prepared linearMass in (axisA, {perpMass, motorMass}); the
+// angular impulse in springImpulse.
+F2D_HD_COLD void prepareMouse(Joint& j, const BodySim* sims, float h, float inv_h, float warmStartScale) {
+  float hertz = minf(j.constraintHertz, 0.25f * inv_h);
+  j.constraintSoftness = makeSoft(hertz, j.constraintDampingRatio, h);
+  const BodySim& b = sims[j.bodyB];
+  j.invMassB = b.invMass;
+  j.invIB = b.invI;
+  j.simB = b.setIndex == kAwakeSet ? j.bodyB : -1;
+  j.anchorB = rotate(b.xf.q, sub(j.localOriginAnchorB, b.localCenter));
+  j.springSoftness = makeSoft(j.hertz, j.dampingRatio, h);
+  j.angularSoftness = makeSoft(0.5f, 0.1f, h);
+  const Vec2 rB = j.anchorB;
+  const float mB = b.invMass, iB = b.invI;
+  const float k11 = mB + iB * rB.y * rB.y;
+  const float k12 = -iB * rB.x * rB.y;
+  const float k22 = mB + iB * rB.x * rB.x;
+  Vec2 cy;
+  inverse22(k11, k12, k22, j.axisA, cy);
+  j.perpMass = cy.x;
+  j.motorMass = cy.y;
+  j.deltaCenter = sub(b.center, j.localAxisA);
+  if (warmStartScale == 0.0f) {
+    j.linearImpulse = {0.0f, 0.0f};
+    j.springImpulse = 0.0f;
+  }
+}
+
+F2D_HD_COLD void warmStartMouse(Joint& j, BodyState* states) {
+  const float mB = j.invMassB, iB = j.invIB;
+  const int ib = j.simB;
+  BodyState sB = loadState(states, ib);
+  const Vec2 rB = rotate(sB.dq, j.anchorB);
+  Vec2 vB = mulAdd(sB.v, mB, j.linearImpulse);
+  float wB = sB.w + iB * (cross(rB, j.linearImpulse) + j.springImpulse);
+  storeVel(states, ib, vB, wB);
+}
+
+F2D_HD_COLD void solveMouse(Joint& j, BodyState* states, float h) {
+  const float mB = j.invMassB, iB = j.invIB;
+  const int ib = j.simB;
+  const BodyState sB = loadState(states, ib);
+  Vec2 vB = sB.v;
+  float wB = sB.w;
+  {
+    const float massScale = j.angularSoftness.massScale;
+    const float impulseScale = j.angularSoftness.impulseScale;
+    float impulse = iB > 0.0f ? -wB / iB : 0.0f;
+    impulse = massScale * impulse - impulseScale * j.springImpulse;
+    j.springImpulse += impulse;
+    wB += iB * impulse;
+  }
+  const float maxImpulse = j.maxMotorTorque * h;
+  {
+    const Vec2 rB = rotate(sB.dq, j.anchorB);
+    const Vec2 Cdot = add(vB, crossSV(wB, rB));
+    const Vec2 separation = add(add(sB.dp, rB), j.deltaCenter);
+    const Vec2 bias = mulSV(j.springSoftness.biasRate, separation);
+    const float massScale = j.springSoftness.massScale;
+    const float impulseScale = j.springSoftness.impulseScale;
+    const Vec2 b = mulMV(j.axisA, {j.perpMass, j.motorMass}, add(Cdot, bias));
+    Vec2 impulse;
+    impulse.x = -massScale * b.x - impulseScale * j.linearImpulse.x;
+    impulse.y = -massScale * b.y - impulseScale * j.linearImpulse.y;
+    const Vec2 oldImpulse = j.linearImpulse;
+    j.linearImpulse.x += impulse.x;
+    j.linearImpulse.y += impulse.y;
+    const float mag = length(j.linearImpulse);
+    if (mag > maxImpulse) j.linearImpulse = mulSV(maxImpulse, normalize(j.linearImpulse));
+    impulse.x = j.linearImpulse.x - oldImpulse.x;
+    impulse.y = j.linearImpulse.y - oldImpulse.y;
+    vB = mulAdd(vB, mB, impulse);
+    wB += iB * cross(rB, impulse);
+  }
+  storeVel(states, ib, vB, wB);
+}
+
+// Filter joint: only the common constraint softness of b2PrepareJoint
+// (S/joint.c:1348-1352); no warm start or solve.
+F2D_HD_COLD void prepareFilter(Joint& j, float h, float inv_h) {
+  float hertz = minf(j.constraintHertz, 0.25f * inv_h);
+  j.constraintSoftness = makeSoft(hertz, j.constraintDampingRatio, h);
+}
+
 // The joint stages by joint type (b2PrepareJointsTask / b2WarmStartJoint /
 // b2SolveJoint, S/joint.c:1348-1450). Revolute joints take an inlined path; the other
 // types are reached through out-of-line calls, which keeps the solve kernel at the
@@ -866,6 +1063,9 @@ F2D_HD_COLD void prepareJointOther(Joint& j, const BodySim* sims, float h, float
     case kWeldJoint: prepareWeld(j, sims, h, inv_h, warmStartScale); break;
     case kWheelJoint: prepareWheel(j, sims, h, inv_h, warmStartScale); break;
     case kDistanceJoint: prepareDistance(j, sims, h, inv_h, warmStartScale); break;
+    case kMotorJoint: prepareMotor(j, sims, h, inv_h, warmStartScale); break;
+    case kMouseJoint: prepareMouse(j, sims, h, inv_h, warmStartScale); break;
+    case kFilterJoint: prepareFilter(j, h, inv_h); break;
     default: prepareRevolute(j, sims, h, inv_h, warmStartScale); break;
   }
 }
@@ -875,6 +1075,9 @@ F2D_HD_COLD void warmStartJointOther(Joint& j, BodyState* states) {
     case kWeldJoint: warmStartWeld(j, states); break;
     case kWheelJoint: warmStartWheel(j, states); break;
     case kDistanceJoint: warmStartDistance(j, states); break;
+    case kMotorJoint: warmStartMotor(j, states); break;
+    case kMouseJoint: warmStartMouse(j, states); break;
+    case kFilterJoint: break;
     default: warmStartRevolute(j, states); break;
   }
 }
@@ -884,6 +1087,9 @@ F2D_HD_COLD void solveJointOther(Joint& j, BodyState* states, float h, float inv
     case kWeldJoint: solveWeld(j, states, useBias); break;
     case kWheelJoint: solveWheel(j, states, h, inv_h, useBias); break;
     case kDistanceJoint: solveDistance(j, states, h, inv_h, useBias); break;
+    case kMotorJoint: solveMotor(j, states, h, inv_h); break;
+    case kMouseJoint: solveMouse(j, states, h); break;
+    case kFilterJoint: break;
     default: solveRevolute(j, states, h, inv_h, useBias); break;
   }
 }

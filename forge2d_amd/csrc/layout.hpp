@@ -153,7 +153,16 @@ struct alignas(16) Contact {
 
 // Revolute joint slot (b2Joint + b2JointSim + b2RevoluteJoint, S/joint.h:27-60, 162-190, 247-281)
 // b2JointType values of the joints built here (I/types.h)
-enum JointType : int { kDistanceJoint = 0, kPrismaticJoint = 4, kRevoluteJoint = 5, kWeldJoint = 6, kWheelJoint = 7 };
+enum JointType : int {
+  kDistanceJoint = 0,
+  kFilterJoint = 1,
+  kMotorJoint = 2,
+  kMouseJoint = 3,
+  kPrismaticJoint = 4,
+  kRevoluteJoint = 5,
+  kWeldJoint = 6,
+  kWheelJoint = 7
+};
 
 // One joint (b2Joint + b2JointSim, S/joint.h). The joint types share fields
 // (S/joint.h b2PrismaticJoint, b2WeldJoint, b2WheelJoint):
@@ -166,7 +175,14 @@ enum JointType : int { kDistanceJoint = 0, kPrismaticJoint = 4, kRevoluteJoint =
 //     perpendicular impulse;
 //   distance: targetAngle = rest length, lowerAngle/upperAngle = length range,
 //     maxMotorTorque = maximum motor force, springSoftness = distance softness,
-//     linearImpulse.x = the length impulse.
+//     linearImpulse.x = the length impulse;
+//   motor: localAxisA = linear offset, referenceAngle = angular offset,
+//     maxMotorTorque = maximum force, angularHertz = maximum torque,
+//     angularDampingRatio = correction factor, springImpulse = angular impulse;
+//   mouse: localAxisA = target, maxMotorTorque = maximum force, springImpulse =
+//     angular impulse;
+//   motor and mouse keep their prepared 2x2 linear mass in axisA (first column) and
+//     (perpMass, motorMass) (second column).
 struct alignas(16) Joint {
   int bodyA, bodyB;
   int color, localIndex;

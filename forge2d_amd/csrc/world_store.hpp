@@ -75,6 +75,11 @@ struct HostWorld {
   int createDistance(int bodyA, int bodyB, Vec2 la, Vec2 lb, float length, bool enableSpring, float hertz,
                      float damping, bool enableLimit, float minLength, float maxLength, bool enableMotor,
                      float maxMotorForce, float motorSpeed, bool collideConnected);
+  int createMotor(int bodyA, int bodyB, Vec2 linearOffset, float angularOffset, float maxForce, float maxTorque,
+                  float correctionFactor, bool collideConnected);
+  int createMouse(int bodyA, int bodyB, Vec2 target, float hertz, float damping, float maxForce,
+                  bool collideConnected);
+  int createFilter(int bodyA, int bodyB);
   void updateBodyMass(int body);
 };
 

@@ -278,6 +278,41 @@ void f2d_create_distance_joint(uint32_t w, int32_t a_index1, uint32_t a_world_an
                                int enable_limit, float min_length, float max_length, int enable_motor,
                                float max_motor_force, float motor_speed, int collide_connected, int32_t* out);
 
+/* F/native/wasm/f2d_shim.c:751-768 (b2CreateMotorJoint, S/joint.c:406-442) */
+void f2d_create_motor_joint(uint32_t w, int32_t a_index1, uint32_t a_world_and_generation, int32_t b_index1,
+                            uint32_t b_world_and_generation, float offset_x, float offset_y, float angular_offset,
+                            float max_force, float max_torque, float correction_factor, int collide_connected,
+                            int32_t* out);
+/* F/native/wasm/f2d_shim.c:770-785 (b2CreateMouseJoint, S/joint.c:444-478) */
+void f2d_create_mouse_joint(uint32_t w, int32_t a_index1, uint32_t a_world_and_generation, int32_t b_index1,
+                            uint32_t b_world_and_generation, float target_x, float target_y, float hertz,
+                            float damping_ratio, float max_force, int collide_connected, int32_t* out);
+/* F/native/wasm/f2d_shim.c:742-749 (b2CreateFilterJoint, S/joint.c:480-505) */
+void f2d_create_filter_joint(uint32_t w, int32_t a_index1, uint32_t a_world_and_generation, int32_t b_index1,
+                             uint32_t b_world_and_generation, int32_t* out);
+
+/* F/native/wasm/f2d_shim.c:1006-1050 (motor and mouse joint parameters,
+ * S/motor_joint.c:14-71, S/mouse_joint.c:14-63); a call on a joint of another type is
+ * ignored and getters return 0 */
+void f2d_motor_joint_get_linear_offset(int32_t index1, uint32_t world_and_generation, float* out);
+void f2d_motor_joint_set_linear_offset(int32_t index1, uint32_t world_and_generation, float x, float y);
+float f2d_motor_joint_get_angular_offset(int32_t index1, uint32_t world_and_generation);
+void f2d_motor_joint_set_angular_offset(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_motor_joint_get_max_force(int32_t index1, uint32_t world_and_generation);
+void f2d_motor_joint_set_max_force(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_motor_joint_get_max_torque(int32_t index1, uint32_t world_and_generation);
+void f2d_motor_joint_set_max_torque(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_motor_joint_get_correction_factor(int32_t index1, uint32_t world_and_generation);
+void f2d_motor_joint_set_correction_factor(int32_t index1, uint32_t world_and_generation, float v);
+void f2d_mouse_joint_get_target(int32_t index1, uint32_t world_and_generation, float* out);
+void f2d_mouse_joint_set_target(int32_t index1, uint32_t world_and_generation, float x, float y);
+float f2d_mouse_joint_get_spring_hertz(int32_t index1, uint32_t world_and_generation);
+void f2d_mouse_joint_set_spring_hertz(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_mouse_joint_get_spring_damping_ratio(int32_t index1, uint32_t world_and_generation);
+void f2d_mouse_joint_set_spring_damping_ratio(int32_t index1, uint32_t world_and_generation, float v);
+float f2d_mouse_joint_get_max_force(int32_t index1, uint32_t world_and_generation);
+void f2d_mouse_joint_set_max_force(int32_t index1, uint32_t world_and_generation, float v);
+
 /* F/native/wasm/f2d_shim.c:965-1004 (distance joint parameters, S/distance_joint.c:19-183);
  * a call on a joint of another type is ignored and getters return 0 */
 float f2d_distance_joint_get_length(int32_t index1, uint32_t world_and_generation);

@@ -646,6 +646,29 @@ static void write_snapshot( FILE* f, b2WorldId wid, int step )
 				sj.upper_impulse = hj->upperImpulse;
 				sj.reserved = 3.0f;
 			}
+			else if ( js->type == b2_motorJoint )
+			{
+				// kind 5; the angular impulse in spring_impulse, the angular offset in ref_angle
+				const b2MotorJoint* mj = &js->motorJoint;
+				sj.linear_impulse_x = mj->linearImpulse.x;
+				sj.linear_impulse_y = mj->linearImpulse.y;
+				sj.spring_impulse = mj->angularImpulse;
+				sj.ref_angle = mj->angularOffset;
+				sj.reserved = 5.0f;
+			}
+			else if ( js->type == b2_mouseJoint )
+			{
+				// kind 6; the angular impulse in spring_impulse
+				const b2MouseJoint* mj = &js->mouseJoint;
+				sj.linear_impulse_x = mj->linearImpulse.x;
+				sj.linear_impulse_y = mj->linearImpulse.y;
+				sj.spring_impulse = mj->angularImpulse;
+				sj.reserved = 6.0f;
+			}
+			else if ( js->type == b2_filterJoint )
+			{
+				sj.reserved = 7.0f;
+			}
 			else
 			{
 				const b2RevoluteJoint* rj = &js->revoluteJoint;
@@ -1087,6 +1110,64 @@ static void apply_op( const OpRec* o, const b2BodyId* bodies, const b2ShapeId* s
 		case 36:
 			b2RevoluteJoint_SetTargetAngle( joints[o->target], f[0] );
 			break;
+		case 37:
+		{
+			// a motor joint: linear offset f0, f1, angular offset f2, maximum force f3 and
+			// torque f4, correction factor f5, collideConnected flag >> 16
+			b2MotorJointDef def = b2DefaultMotorJointDef();
+			def.bodyIdA = b;
+			def.bodyIdB = bodies[o->flag & 0xFFFF];
+			def.linearOffset = ( b2Vec2 ){ f[0], f[1] };
+			def.angularOffset = f[2];
+			def.maxForce = f[3];
+			def.maxTorque = f[4];
+			def.correctionFactor = f[5];
+			def.collideConnected = ( o->flag >> 16 ) != 0;
+			joints[g_jointCount++] = b2CreateMotorJoint( b2Body_GetWorld( b ), &def );
+		}
+		break;
+		case 38:
+		{
+			b2JointId j = joints[o->target];
+			b2MotorJoint_SetLinearOffset( j, ( b2Vec2 ){ f[0], f[1] } );
+			b2MotorJoint_SetAngularOffset( j, f[2] );
+			b2MotorJoint_SetMaxForce( j, f[3] );
+			b2MotorJoint_SetMaxTorque( j, f[4] );
+			b2MotorJoint_SetCorrectionFactor( j, f[5] );
+		}
+		break;
+		case 39:
+		{
+			// a mouse joint: target f0, f1, hertz f2, damping f3, maximum force f4,
+			// collideConnected flag >> 16
+			b2MouseJointDef def = b2DefaultMouseJointDef();
+			def.bodyIdA = b;
+			def.bodyIdB = bodies[o->flag & 0xFFFF];
+			def.target = ( b2Vec2 ){ f[0], f[1] };
+			def.hertz = f[2];
+			def.dampingRatio = f[3];
+			def.maxForce = f[4];
+			def.collideConnected = ( o->flag >> 16 ) != 0;
+			joints[g_jointCount++] = b2CreateMouseJoint( b2Body_GetWorld( b ), &def );
+		}
+		break;
+		case 40:
+		{
+			b2JointId j = joints[o->target];
+			b2MouseJoint_SetTarget( j, ( b2Vec2 ){ f[0], f[1] } );
+			b2MouseJoint_SetSpringHertz( j, f[2] );
+			b2MouseJoint_SetSpringDampingRatio( j, f[3] );
+			b2MouseJoint_SetMaxForce( j, f[4] );
+		}
+		break;
+		case 41:
+		{
+			b2FilterJointDef def = b2DefaultFilterJointDef();
+			def.bodyIdA = b;
+			def.bodyIdB = bodies[o->flag & 0xFFFF];
+			joints[g_jointCount++] = b2CreateFilterJoint( b2Body_GetWorld( b ), &def );
+		}
+		break;
 		default:
 			fprintf( stderr, "bad op %d\n", o->op );
 			exit( 3 );

@@ -115,7 +115,7 @@ class FlatWorld:
         L = F.lib()
         code, t, flag = int(op["op"]), int(op["target"]), int(op["flag"])
         f = [float(x) for x in op["f"]]
-        bi, bwg = self.bodies[t] if code not in (12, 13, 14, 23, 26, 27, 29, 31, 33, 34, 35, 36) else (0, 0)
+        bi, bwg = self.bodies[t] if code not in (12, 13, 14, 23, 26, 27, 29, 31, 33, 34, 35, 36, 38, 40) else (0, 0)
         if code == 1:
             L.f2d_body_apply_force(bi, bwg, f[0], f[1], f[2], f[3], flag)
         elif code == 2:
@@ -242,6 +242,32 @@ class FlatWorld:
             L.f2d_revolute_joint_enable_spring(*j, int(flag & 4 != 0))
         elif code == 36:
             L.f2d_revolute_joint_set_target_angle(*self.joints[t], f[0])
+        elif code in (37, 39, 41):
+            bb = self.bodies[flag & 0xFFFF]
+            out = (ctypes.c_int32 * 2)()
+            if code == 37:
+                L.f2d_create_motor_joint(self.w, bi, bwg, bb[0], bb[1], f[0], f[1], f[2], f[3], f[4], f[5],
+                                         int(flag >> 16 != 0), out)
+            elif code == 39:
+                L.f2d_create_mouse_joint(self.w, bi, bwg, bb[0], bb[1], f[0], f[1], f[2], f[3], f[4],
+                                         int(flag >> 16 != 0), out)
+            else:
+                L.f2d_create_filter_joint(self.w, bi, bwg, bb[0], bb[1], out)
+            assert out[0] > 0
+            self.joints.append((out[0], ctypes.c_uint32(out[1]).value))
+        elif code == 38:
+            j = self.joints[t]
+            L.f2d_motor_joint_set_linear_offset(*j, f[0], f[1])
+            L.f2d_motor_joint_set_angular_offset(*j, f[2])
+            L.f2d_motor_joint_set_max_force(*j, f[3])
+            L.f2d_motor_joint_set_max_torque(*j, f[4])
+            L.f2d_motor_joint_set_correction_factor(*j, f[5])
+        elif code == 40:
+            j = self.joints[t]
+            L.f2d_mouse_joint_set_target(*j, f[0], f[1])
+            L.f2d_mouse_joint_set_spring_hertz(*j, f[2])
+            L.f2d_mouse_joint_set_spring_damping_ratio(*j, f[3])
+            L.f2d_mouse_joint_set_max_force(*j, f[4])
         else:
             raise ValueError(f"op {code}")
 

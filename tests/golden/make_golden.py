@@ -290,7 +290,8 @@ OPS = {"apply_force": 1, "apply_force_to_center": 2, "apply_torque": 3, "apply_l
        "set_awake": 20, "joint_wake_bodies": 23, "create_prismatic": 25, "prismatic_config": 26,
        "prismatic_target": 27, "create_weld": 28, "weld_config": 29, "create_wheel": 30, "wheel_config": 31,
        "create_distance": 32, "distance_config": 33, "distance_length": 34, "revolute_config": 35,
-       "revolute_target": 36}
+       "revolute_target": 36, "create_motor": 37, "motor_config": 38, "create_mouse": 39, "mouse_config": 40,
+       "create_filter": 41}
 
 
 def ops_array(rows):
@@ -475,6 +476,42 @@ DISTANCE_OPS = [
     (45, "distance_config", 3, 5, [1.0, 1.1, 0.0, 0.0, 5.0, 0.9]),
 ]
 
+def motor_mouse_world():
+    """A scene for motor, mouse and filter joints (S/motor_joint.c, S/mouse_joint.c,
+    S/joint.c:480-505), created by API calls: a box driven by a motor joint towards an
+    offset from the ground, a box carried by a motor joint on another box, a box and a
+    circle dragged by mouse joints, two overlapping boxes kept from colliding by a filter
+    joint, and loose boxes that fall on them (sleep off)."""
+    w = scenes.World(scenes.world_header(sleep=False, continuous=True))
+    g = w.add_body(scenes.STATIC, 0.0, -1.0)                      # 0
+    w.add_box(g, 30.0, 1.0)                                       # shape 0
+    w.add_box(w.add_body(scenes.DYNAMIC, -6.0, 0.5), 0.5, 0.5)    # 1: motor from the ground
+    w.add_box(w.add_body(scenes.DYNAMIC, -2.0, 0.6), 1.0, 0.4)    # 2: carrier
+    w.add_box(w.add_body(scenes.DYNAMIC, -2.0, 1.6, angle=0.3), 0.3, 0.3)  # 3: motor on 2
+    w.add_box(w.add_body(scenes.DYNAMIC, 3.0, 0.5), 0.5, 0.5)     # 4: mouse
+    w.add_circle(w.add_body(scenes.DYNAMIC, 6.0, 3.0), 0.4)       # 5: mouse, weak force
+    w.add_box(w.add_body(scenes.DYNAMIC, 9.0, 0.5), 0.5, 0.5)     # 6
+    w.add_box(w.add_body(scenes.DYNAMIC, 9.4, 1.2), 0.5, 0.5)     # 7: filtered against 6
+    for k in range(3):                                            # 8..10
+        w.add_box(w.add_body(scenes.DYNAMIC, -5.5 + 6.0 * k, 6.0 + 0.5 * k, angle=0.2 * k), 0.3, 0.3)
+    return w
+
+
+MOTOR_MOUSE_OPS = [
+    (1, "create_motor", 0, 1, [0.5, 2.0, 0.8, 30.0, 5.0, 0.3]),               # 0
+    (1, "create_motor", 2, 3, [0.0, 1.0, 0.0, 8.0, 1.0, 0.5]),                # 1
+    (1, "create_mouse", 0, 4, [3.0, 0.5, 5.0, 0.7, 500.0]),                   # 2
+    (1, "create_mouse", 0, 5 | (1 << 16), [6.0, 3.0, 2.0, 0.2, 3.0]),         # 3: weak force, clamped
+    (1, "create_filter", 6, 7, []),                                           # 4
+    # later: targets dragged, offsets and limits changed (negative force clamped to 0,
+    # correction factor clamped to 1)
+    (15, "mouse_config", 2, 0, [5.0, 3.0, 5.0, 0.7, 500.0]),
+    (20, "motor_config", 0, 0, [-1.0, 3.0, -0.5, 30.0, 5.0, 0.3]),
+    (25, "mouse_config", 3, 0, [2.0, 6.0, 8.0, 1.0, 40.0]),
+    (30, "motor_config", 1, 0, [0.5, 1.0, 0.4, -2.0, 1.0, 1.5]),
+    (40, "mouse_config", 2, 0, [1.0, 4.0, 1.0, 0.0, 50.0]),
+]
+
 # the revolute chain of body_ops_world (joints 0..2): motor, limits, spring and target
 # changed through the revolute joint calls
 REVOLUTE_OPS = [
@@ -493,6 +530,7 @@ OPS_FIXTURES = {
     "ops_weld_wheel": (weld_wheel_world, WELD_WHEEL_OPS, list(range(0, 61))),
     "ops_distance": (distance_world, DISTANCE_OPS, list(range(0, 61))),
     "ops_revolute": (body_ops_world, REVOLUTE_OPS, list(range(0, 41))),
+    "ops_motor_mouse": (motor_mouse_world, MOTOR_MOUSE_OPS, list(range(0, 61))),
 }
 
 

@@ -377,6 +377,67 @@ def test_revolute_joint_calls():
     fw.close()
 
 
+def test_motor_mouse_fixture_exercises_every_branch():
+    """Over the 60 reference steps of ops_motor_mouse: the ground motor's linear impulse
+    held on its force limit (|impulse| = h x max force), the carried motor's angular
+    impulse on both torque limits and its linear impulse 0 once its force is set below 0
+    (clamped to 0), the weak mouse joint on its force limit, and the filter joint with
+    no impulse."""
+    spec, snaps, ops = load_ops_fixture("ops_motor_mouse", GOLDEN)
+    J = np.stack([s.joints for s in snaps[1:]])
+    assert J["reserved"][0].tolist() == [5.0, 5.0, 6.0, 6.0, 7.0]
+    h = np.float32(np.float32(1.0 / 60.0) / np.float32(4.0))
+    mag0 = np.hypot(J["linear_impulse_x"][:, 0], J["linear_impulse_y"][:, 0])
+    assert np.allclose(mag0[5:], np.float32(30.0) * h, rtol=1e-6)
+    ang1 = J["spring_impulse"][:, 1]
+    assert np.isclose(ang1, h).any() and np.isclose(ang1, -h).any()
+    assert (J["linear_impulse_x"][31:, 1] == 0).all() and (J["linear_impulse_y"][31:, 1] == 0).all()
+    mag3 = np.hypot(J["linear_impulse_x"][:, 3], J["linear_impulse_y"][:, 3])
+    assert np.isclose(mag3[5:20], np.float32(3.0) * h, rtol=1e-6).all()
+    assert (J["linear_impulse_x"][:, 4] == 0).all() and (J["spring_impulse"][:, 4] == 0).all()
+    assert J["ref_angle"][-1].tolist()[:2] == [np.float32(-0.5), np.float32(0.4)]
+
+
+def test_motor_mouse_joint_calls():
+    """The motor and mouse calls of f2d_shim.c:1006-1050: values read back as stored,
+    the motor's maximum force and torque clamped at 0 and its correction factor to
+    [0, 1] (S/motor_joint.c:36-71); the mouse joint's local anchors are the target in
+    each body's frame (S/joint.c:467-468); calls on the other type are ignored."""
+    spec, snaps, ops = load_ops_fixture("ops_motor_mouse", GOLDEN)
+    fw = FlatWorld(spec)
+    for op in ops[ops["step"] == 1]:
+        fw.apply(op)
+    L = F.lib()
+    motor, mouse = fw.joints[0], fw.joints[2]
+    assert [L.f2d_joint_get_type(*j) for j in fw.joints] == [2, 2, 3, 3, 1]
+    out = (ctypes.c_float * 2)()
+    L.f2d_motor_joint_get_linear_offset(*motor, out)
+    assert (out[0], out[1]) == (0.5, 2.0)
+    assert L.f2d_motor_joint_get_angular_offset(*motor) == np.float32(0.8)
+    L.f2d_motor_joint_set_max_force(*motor, -3.0)
+    L.f2d_motor_joint_set_max_torque(*motor, -1.0)
+    L.f2d_motor_joint_set_correction_factor(*motor, -0.5)
+    assert (L.f2d_motor_joint_get_max_force(*motor), L.f2d_motor_joint_get_max_torque(*motor),
+            L.f2d_motor_joint_get_correction_factor(*motor)) == (0.0, 0.0, 0.0)
+    L.f2d_motor_joint_set_correction_factor(*motor, 2.0)
+    assert L.f2d_motor_joint_get_correction_factor(*motor) == 1.0
+    L.f2d_mouse_joint_get_target(*mouse, out)
+    assert (out[0], out[1]) == (3.0, 0.5)
+    L.f2d_mouse_joint_set_target(*mouse, -1.0, 2.5)
+    L.f2d_mouse_joint_get_target(*mouse, out)
+    assert (out[0], out[1]) == (-1.0, 2.5)
+    assert (L.f2d_mouse_joint_get_spring_hertz(*mouse), L.f2d_mouse_joint_get_spring_damping_ratio(*mouse),
+            L.f2d_mouse_joint_get_max_force(*mouse)) == (5.0, np.float32(0.7), 500.0)
+    # the target (3, 0.5) in the frames of the ground (0, -1) and of body 4 at (3, 0.5)
+    L.f2d_joint_get_local_anchor_a(*mouse, out)
+    assert (out[0], out[1]) == (3.0, 1.5)
+    L.f2d_joint_get_local_anchor_b(*mouse, out)
+    assert (out[0], out[1]) == (0.0, 0.0)
+    L.f2d_mouse_joint_set_max_force(*motor, 9.0)
+    assert L.f2d_mouse_joint_get_max_force(*motor) == 0.0 and L.f2d_motor_joint_get_max_force(*mouse) == 0.0
+    fw.close()
+
+
 def test_joint_calls_on_other_types_are_ignored():
     """A joint parameter call of the wrong joint type leaves the joint alone and its
     getter returns 0, as the reference's type check would refuse it."""

@@ -15,6 +15,8 @@ state must equal the reference's bit for bit: bodies (and their solver sets and
 sleep times), contacts by id with manifolds and impulses, colour arrays, trees,
 islands and the move array.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -127,6 +129,32 @@ def test_revolute_joint_calls_match_reference(gpu):
     assert L.f2d_revolute_joint_get_target_angle(*fw.joints[1]) == np.float32(0.25)
     assert L.f2d_revolute_joint_is_spring_enabled(*fw.joints[1]) == 0
     assert L.f2d_revolute_joint_is_spring_enabled(*fw.joints[2]) == 1
+    fw.close()
+
+
+def test_motor_mouse_filter_joints_match_reference(gpu):
+    """Motor, mouse and filter joints (S/motor_joint.c, S/mouse_joint.c, S/joint.c:480-505)
+    made through f2d_create_motor_joint / f2d_create_mouse_joint / f2d_create_filter_joint
+    and the f2d_motor_joint_* / f2d_mouse_joint_* calls (fixture ops_motor_mouse: a motor
+    from the ground held at its force limit, a motor on a moving carrier at its torque
+    limit and later with zero force, a stiff mouse joint dragged to new targets, a weak
+    one held at its force limit, a filter joint between two overlapping boxes): after
+    every one of 60 steps bodies, contacts, colours, islands and every joint's impulses
+    equal the reference's bit for bit."""
+    fw, snaps = replay_joint_fixture("ops_motor_mouse", 5)
+    import forge2d_amd as F
+    L = F.lib()
+    assert [L.f2d_joint_get_type(*j) for j in fw.joints] == [2, 2, 3, 3, 1]
+    inv_h = np.float32(4.0) * (np.float32(1.0) / np.float32(1.0 / 60.0))
+    out = (ctypes.c_float * 2)()
+    L.f2d_joint_get_constraint_force(*fw.joints[3], out)
+    last = snaps[-1].joints
+    assert (out[0], out[1]) == (inv_h * last["linear_impulse_x"][3], inv_h * last["linear_impulse_y"][3])
+    assert L.f2d_joint_get_constraint_torque(*fw.joints[0]) == inv_h * last["spring_impulse"][0]
+    L.f2d_joint_get_constraint_force(*fw.joints[4], out)
+    assert (out[0], out[1]) == (0.0, 0.0)
+    assert L.f2d_motor_joint_get_max_force(*fw.joints[1]) == 0.0
+    assert L.f2d_motor_joint_get_correction_factor(*fw.joints[1]) == 1.0
     fw.close()
 
 
