@@ -4307,6 +4307,26 @@ int f2d_create_polygon_shape(int32_t index1, uint32_t wg, const f2dShapeDef* def
   return 1;
 }
 
+// A joint with collideConnected off destroys the contacts between its bodies when it is
+// created (b2DestroyContactsBetweenBodies, S/joint.c:313-345; not for mouse and filter
+// joints). That edit of a live world's contact, island and colour state is not restated
+// on the host, so such a creation is refused while the bodies share a contact.
+static bool refuseContactsBetween(const HostWorld& hw, int a, int b, int collideConnected) {
+  if (collideConnected) return false;
+  int key = hw.sims[a].headContactKey;
+  while (key >= 0) {
+    const Contact& c = hw.contacts[key >> 1];
+    const int other = (key & 1) == 0 ? c.bodyB : c.bodyA;
+    if (other == b) {
+      fprintf(stderr, "forge2d_amd: a joint without collideConnected between bodies %d and %d, which share "
+                      "contact %d, is not supported (the reference destroys that contact)\n", a, b, key >> 1);
+      return true;
+    }
+    key = (key & 1) == 0 ? c.nextA : c.nextB;
+  }
+  return false;
+}
+
 // f2d_shim.c:816-843 (revolute subset of the def)
 void f2d_create_revolute_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t b_index1, uint32_t b_wg,
                                float lax, float lay, float lbx, float lby, float reference_angle, float target_angle,
@@ -4321,6 +4341,7 @@ void f2d_create_revolute_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int3
   int ia, ib;
   HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
   if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (refuseContactsBetween(*hw, ia, ib, collide_connected)) return;
   if (b->committed) freeDevice(*b);
   int id = hw->createRevolute(ia, ib, {lax, lay}, {lbx, lby}, reference_angle, target_angle, enable_spring != 0, hertz,
                               damping_ratio, enable_limit != 0, lower_angle, upper_angle, enable_motor != 0,
@@ -4344,6 +4365,7 @@ void f2d_create_prismatic_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int
   int ia, ib;
   HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
   if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (refuseContactsBetween(*hw, ia, ib, collide_connected)) return;
   if (b->committed) freeDevice(*b);
   int id = hw->createPrismatic(ia, ib, {lax, lay}, {lbx, lby}, {axis_x, axis_y}, reference_angle, target_translation,
                                enable_spring != 0, hertz, damping_ratio, enable_limit != 0, lower_translation,
@@ -4366,6 +4388,7 @@ void f2d_create_distance_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int3
   int ia, ib;
   HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
   if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (refuseContactsBetween(*hw, ia, ib, collide_connected)) return;
   if (b->committed) freeDevice(*b);
   int id = hw->createDistance(ia, ib, {lax, lay}, {lbx, lby}, length, enable_spring != 0, hertz, damping_ratio,
                               enable_limit != 0, min_length, max_length, enable_motor != 0, max_motor_force,
@@ -4385,6 +4408,7 @@ void f2d_create_motor_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t
   int ia, ib;
   HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
   if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (refuseContactsBetween(*hw, ia, ib, collide_connected)) return;
   if (b->committed) freeDevice(*b);
   int id = hw->createMotor(ia, ib, {offset_x, offset_y}, angular_offset, max_force, max_torque, correction_factor,
                            collide_connected != 0);
@@ -4437,6 +4461,7 @@ void f2d_create_weld_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t 
   int ia, ib;
   HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
   if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (refuseContactsBetween(*hw, ia, ib, collide_connected)) return;
   if (b->committed) freeDevice(*b);
   int id = hw->createWeld(ia, ib, {lax, lay}, {lbx, lby}, reference_angle, linear_hertz, angular_hertz,
                           linear_damping_ratio, angular_damping_ratio, collide_connected != 0);
@@ -4457,6 +4482,7 @@ void f2d_create_wheel_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t
   int ia, ib;
   HostWorld* hw = bodyWorld(a_index1, a_wg, &ia);
   if (!b || !hw || !bodyWorld(b_index1, b_wg, &ib)) return;
+  if (refuseContactsBetween(*hw, ia, ib, collide_connected)) return;
   if (b->committed) freeDevice(*b);
   int id = hw->createWheel(ia, ib, {lax, lay}, {lbx, lby}, {axis_x, axis_y}, enable_spring != 0, hertz, damping_ratio,
                            enable_limit != 0, lower_translation, upper_translation, enable_motor != 0,

@@ -205,7 +205,12 @@ void f2d_shape_get_segment(int32_t index1, uint32_t world_and_generation, float*
 int f2d_shape_get_polygon(int32_t index1, uint32_t world_and_generation, float* out);
 void f2d_shape_get_aabb(int32_t index1, uint32_t world_and_generation, float* out);
 
-/* F/native/wasm/f2d_shim.c:816-843 (b2CreateRevoluteJoint) */
+/* F/native/wasm/f2d_shim.c:816-843 (b2CreateRevoluteJoint).
+ * Every joint creation call with collide_connected = 0 (revolute, prismatic, weld,
+ * wheel, distance, motor) between bodies that already share a contact is refused: out
+ * receives a null id and a message goes to stderr, since the reference would destroy
+ * that contact (S/joint.c:313-345) and that edit is not restated here. Mouse and filter
+ * joints destroy no contacts and are always made. */
 void f2d_create_revolute_joint(uint32_t w, int32_t a_index1, uint32_t a_world_and_generation, int32_t b_index1,
                                uint32_t b_world_and_generation, float anchor_ax, float anchor_ay, float anchor_bx,
                                float anchor_by, float reference_angle, float target_angle, int enable_spring,

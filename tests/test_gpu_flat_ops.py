@@ -158,6 +158,35 @@ def test_motor_mouse_filter_joints_match_reference(gpu):
     fw.close()
 
 
+def test_joint_between_touching_bodies_refused(gpu):
+    """A joint with collideConnected off between bodies that share a contact would have
+    the reference destroy that contact (S/joint.c:313-345); that edit is not restated,
+    so the creation is refused with a null id. With collideConnected on, and for a mouse
+    joint (whose creation destroys nothing), the joint is made and the world steps on."""
+    spec, snaps, ops = load_ops_fixture("ops_motor_mouse", GOLDEN)
+    fw = FlatWorld(spec)
+    for _ in range(30):
+        fw.step()
+    import forge2d_amd as F
+    L = F.lib()
+    mine = fw.snapshot()
+    touching = [(int(c["shape_a"]), int(c["shape_b"])) for c in mine.live_contacts()]
+    assert (0, 6) in touching or (6, 0) in touching, touching  # box 6 (shape 6) rests on the ground
+    ground, box = fw.bodies[0], fw.bodies[6]
+    out = (ctypes.c_int32 * 2)()
+    L.f2d_create_revolute_joint(fw.w, *ground, *box, 9.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0, 0.0, 0.0, 0, 0.0, 0.0, 0,
+                                0.0, 0.0, 1.0, 0, out)
+    assert out[0] == 0
+    L.f2d_create_revolute_joint(fw.w, *ground, *box, 9.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0, 0.0, 0.0, 0, 0.0, 0.0, 0,
+                                0.0, 0.0, 1.0, 1, out)
+    assert out[0] > 0
+    L.f2d_create_mouse_joint(fw.w, *ground, *box, 9.0, 1.0, 5.0, 0.7, 100.0, 0, out)
+    assert out[0] > 0
+    fw.step()
+    assert L.f2d_body_get_joint_count(*box) == 2
+    fw.close()
+
+
 def test_distance_joints_match_reference(gpu):
     """Distance joints (S/distance_joint.c) made through f2d_create_distance_joint and the
     f2d_distance_joint_* calls (fixture ops_distance: a rigid rope, a spring with a
