@@ -299,7 +299,12 @@ int HostWorld::createRevolute(int bodyA, int bodyB, Vec2 la, Vec2 lb, float refA
                               bool enableSpring, float hertz, float damping, bool enableLimit, float lower, float upper,
                               bool enableMotor, float maxMotorTorque, float motorSpeed, bool collideConnected) {
   Joint j = {};
+  // b2AllocId: the last freed id first, else a fresh one (S/id_pool.c)
   int id = (int)joints.size();
+  if (!jointFree.empty()) {
+    id = jointFree.back();
+    jointFree.pop_back();
+  }
   j.bodyA = bodyA;
   j.bodyB = bodyB;
   j.collideConnected = collideConnected ? 1 : 0;
@@ -381,7 +386,10 @@ int HostWorld::createRevolute(int bodyA, int bodyB, Vec2 la, Vec2 lb, float refA
   // S/joint.c:204-309; b2LinkJoint with mergeIslands, S/island.c:289-385)
   j.islandId = j.islandPrev = j.islandNext = -1;
   j.setIndex = (staticA && staticB) ? kStaticSet : kAwakeSet;
-  joints.push_back(j);
+  if (id == (int)joints.size()) joints.push_back(j);
+  else joints[id] = j;
+  if ((int)jointGen.size() < (int)joints.size()) jointGen.resize(joints.size(), 0);
+  jointGen[id] = (uint16_t)(jointGen[id] + 1);
   if (j.setIndex == kAwakeSet) {
     IslandCtx c = islandCtx();
     linkJointIslands(c, id);
@@ -514,6 +522,65 @@ int HostWorld::createFilter(int bodyA, int bodyB) {
                                 0.0f, false, 0.0f, 0.0f, false);
   joints[id].type = kFilterJoint;
   return id;
+}
+
+// b2DestroyJointInternal, S/joint.c:702-805, for a joint of the awake set: out of both
+// bodies' joint lists and its island (b2UnlinkJoint, S/island.c:384-425, which counts
+// the removal for the island split), out of its colour (b2RemoveJointFromGraph,
+// S/constraint_graph.c:292-322: the colour's last joint takes its index), its id back
+// to the pool. Both bodies are awake, so waking them changes nothing. Joints of a
+// sleeping or the static set are refused (-1): their set arrays are not restated.
+int HostWorld::destroyJoint(int id) {
+  Joint& j = joints[id];
+  if (j.setIndex != kAwakeSet) return -1;
+  auto unlinkEdge = [&](int body, int edge, int prevKey, int nextKey) {
+    if (prevKey >= 0) {
+      Joint& p = joints[prevKey >> 1];
+      if ((prevKey & 1) == 0) p.nextA = nextKey; else p.nextB = nextKey;
+    }
+    if (nextKey >= 0) {
+      Joint& n = joints[nextKey >> 1];
+      if ((nextKey & 1) == 0) n.prevA = prevKey; else n.prevB = prevKey;
+    }
+    if (sims[body].headJointKey == ((id << 1) | edge)) sims[body].headJointKey = nextKey;
+    sims[body].jointCount -= 1;
+  };
+  unlinkEdge(j.bodyA, 0, j.prevA, j.nextA);
+  unlinkEdge(j.bodyB, 1, j.prevB, j.nextB);
+  if (j.islandId >= 0) {
+    Island& is = islands[j.islandId];
+    if (j.islandPrev >= 0) joints[j.islandPrev].islandNext = j.islandNext;
+    if (j.islandNext >= 0) joints[j.islandNext].islandPrev = j.islandPrev;
+    if (is.headJoint == id) is.headJoint = j.islandNext;
+    if (is.tailJoint == id) is.tailJoint = j.islandPrev;
+    is.jointCount -= 1;
+    is.removeCount += 1;
+    j.islandId = j.islandPrev = j.islandNext = -1;
+  }
+  if (j.color >= 0) {
+    const int color = j.color;
+    if (color != kOverflowIndex) {
+      ensureColorWords();
+      colorBits[(size_t)color * colorWords + (j.bodyA >> 5)] &= ~(1u << (j.bodyA & 31));
+      colorBits[(size_t)color * colorWords + (j.bodyB >> 5)] &= ~(1u << (j.bodyB & 31));
+    }
+    int last = -1;
+    for (const Joint& o : joints) last += o.color == color ? 1 : 0;
+    if (j.localIndex != last) {
+      for (Joint& o : joints) {
+        if (&o != &j && o.color == color && o.localIndex == last) {
+          o.localIndex = j.localIndex;
+          break;
+        }
+      }
+    }
+  }
+  j.setIndex = -1;
+  j.localIndex = -1;
+  j.color = -1;
+  j.prevA = j.nextA = j.prevB = j.nextB = -1;
+  jointFree.push_back(id);
+  return 0;
 }
 
 IslandCtx HostWorld::islandCtx() {
@@ -1472,6 +1539,11 @@ static void exportWorld(const HostWorld& hw, std::vector<unsigned char>& out) {
   for (size_t i = 0; i < hw.joints.size(); ++i) {
     const Joint& j = hw.joints[i];
     SnapJoint sj = {};
+    if (j.setIndex < 0) {  // a destroyed joint's slot (jointId null in the reference)
+      sj.id = -1;
+      put(out, sj);
+      continue;
+    }
     sj.id = (int)i;
     sj.color = j.color;
     sj.local_index = j.localIndex;
@@ -3239,7 +3311,7 @@ int f2d_body_get_joints(int32_t index1, uint32_t wg, int32_t* out, int capacity)
   for (int key = hw->sims[id].headJointKey; key >= 0 && n < capacity;) {
     const int j = key >> 1;
     out[2 * n] = j + 1;
-    out[2 * n + 1] = (int32_t)((wg & 0xFFFF) | (1u << 16));
+    out[2 * n + 1] = (int32_t)((wg & 0xFFFF) | ((uint32_t)hw->jointGen[j] << 16));
     ++n;
     key = (key & 1) ? hw->joints[j].nextB : hw->joints[j].nextA;
   }
@@ -3578,15 +3650,29 @@ void f2d_shape_get_aabb(int32_t index1, uint32_t wg, float* out) {
 
 static HostWorld* jointWorld(int32_t index1, uint32_t wg, int* jointId) {
   uint32_t world0 = wg & 0xFFFF;
-  if (world0 >= g_worlds.size() || (wg >> 16) != 1u) return nullptr;
+  if (world0 >= g_worlds.size()) return nullptr;
   WorldSlot& s = g_worlds[world0];
   if (!s.batch) return nullptr;
   if (s.batch->deviceNewer && download(*s.batch) != 0) return nullptr;
   HostWorld& hw = s.batch->worlds[0];
   const int id = index1 - 1;
   if (id < 0 || id >= (int)hw.joints.size()) return nullptr;
+  if (hw.joints[id].setIndex < 0 || (id < (int)hw.jointGen.size() && (wg >> 16) != hw.jointGen[id])) return nullptr;
   *jointId = id;
   return &hw;
+}
+
+// f2d_shim.c:890-893 (b2DestroyJoint, S/joint.c:807-819). A joint of a sleeping or the
+// static set is left alone with a message (its set arrays are not restated).
+void f2d_destroy_joint(int32_t index1, uint32_t wg) {
+  int id;
+  HostWorld* hw = jointWorld(index1, wg, &id);
+  if (!hw) return;
+  if (hw->destroyJoint(id) != 0) {
+    fprintf(stderr, "forge2d_amd: destroying joint %d outside the awake set is not supported\n", id);
+    return;
+  }
+  bodyBatch(wg)->hostDirty = true;
 }
 
 int f2d_joint_is_valid(int32_t index1, uint32_t wg) {
@@ -4348,7 +4434,7 @@ void f2d_create_revolute_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int3
                               max_motor_torque, motor_speed, collide_connected != 0);
   b->hostDirty = true;
   out[0] = id + 1;
-  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | ((uint32_t)hw->jointGen[id] << 16));
 }
 
 // f2d_shim.c:787-814
@@ -4373,7 +4459,7 @@ void f2d_create_prismatic_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int
                                collide_connected != 0);
   b->hostDirty = true;
   out[0] = id + 1;
-  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | ((uint32_t)hw->jointGen[id] << 16));
 }
 
 // f2d_shim.c:716-740
@@ -4395,7 +4481,7 @@ void f2d_create_distance_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int3
                               motor_speed, collide_connected != 0);
   b->hostDirty = true;
   out[0] = id + 1;
-  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | ((uint32_t)hw->jointGen[id] << 16));
 }
 
 // f2d_shim.c:751-768
@@ -4414,7 +4500,7 @@ void f2d_create_motor_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t
                            collide_connected != 0);
   b->hostDirty = true;
   out[0] = id + 1;
-  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | ((uint32_t)hw->jointGen[id] << 16));
 }
 
 // f2d_shim.c:770-785
@@ -4431,7 +4517,7 @@ void f2d_create_mouse_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t
   int id = hw->createMouse(ia, ib, {target_x, target_y}, hertz, damping_ratio, max_force, collide_connected != 0);
   b->hostDirty = true;
   out[0] = id + 1;
-  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | ((uint32_t)hw->jointGen[id] << 16));
 }
 
 // f2d_shim.c:742-749
@@ -4447,7 +4533,7 @@ void f2d_create_filter_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_
   int id = hw->createFilter(ia, ib);
   b->hostDirty = true;
   out[0] = id + 1;
-  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | ((uint32_t)hw->jointGen[id] << 16));
 }
 
 // f2d_shim.c:844-862
@@ -4467,7 +4553,7 @@ void f2d_create_weld_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t 
                           linear_damping_ratio, angular_damping_ratio, collide_connected != 0);
   b->hostDirty = true;
   out[0] = id + 1;
-  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | ((uint32_t)hw->jointGen[id] << 16));
 }
 
 // f2d_shim.c:864-888
@@ -4489,7 +4575,7 @@ void f2d_create_wheel_joint(uint32_t w, int32_t a_index1, uint32_t a_wg, int32_t
                            max_motor_torque, motor_speed, collide_connected != 0);
   b->hostDirty = true;
   out[0] = id + 1;
-  out[1] = (int32_t)(((w & 0xFFFF) - 1) | (1u << 16));
+  out[1] = (int32_t)(((w & 0xFFFF) - 1) | ((uint32_t)hw->jointGen[id] << 16));
 }
 
 // Batch handle behind a single world, for the batch ABI (export, counters).

@@ -20,6 +20,8 @@ struct HostWorld {
   std::vector<Shape> shapes;
   std::vector<ShapeGeom> geoms;
   std::vector<Joint> joints;
+  std::vector<uint16_t> jointGen;  // b2Joint::generation, raised at each creation in the slot
+  std::vector<int> jointFree;      // the joint id pool's free list (b2IdPool, LIFO)
   std::vector<Contact> contacts;   // size = contact capacity after commit
   std::vector<int> freeIds;        // LIFO stack
   std::vector<int> overflow;
@@ -80,6 +82,7 @@ struct HostWorld {
   int createMouse(int bodyA, int bodyB, Vec2 target, float hertz, float damping, float maxForce,
                   bool collideConnected);
   int createFilter(int bodyA, int bodyB);
+  int destroyJoint(int id);
   void updateBodyMass(int body);
 };
 

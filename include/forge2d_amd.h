@@ -205,6 +205,12 @@ void f2d_shape_get_segment(int32_t index1, uint32_t world_and_generation, float*
 int f2d_shape_get_polygon(int32_t index1, uint32_t world_and_generation, float* out);
 void f2d_shape_get_aabb(int32_t index1, uint32_t world_and_generation, float* out);
 
+/* F/native/wasm/f2d_shim.c:890-893 (b2DestroyJoint, S/joint.c:702-819): the joint
+ * leaves its bodies, island and colour, and its id returns to the pool (the next joint
+ * created takes it, with a new generation, so old handles become invalid). A joint of a
+ * sleeping set or between two static bodies is left alone with a message on stderr. */
+void f2d_destroy_joint(int32_t index1, uint32_t world_and_generation);
+
 /* F/native/wasm/f2d_shim.c:816-843 (b2CreateRevoluteJoint).
  * Every joint creation call with collide_connected = 0 (revolute, prismatic, weld,
  * wheel, distance, motor) between bodies that already share a contact is refused: out

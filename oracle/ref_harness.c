@@ -1160,6 +1160,9 @@ static void apply_op( const OpRec* o, const b2BodyId* bodies, const b2ShapeId* s
 			b2MouseJoint_SetMaxForce( j, f[4] );
 		}
 		break;
+		case 42:
+			b2DestroyJoint( joints[o->target] );
+			break;
 		case 41:
 		{
 			b2FilterJointDef def = b2DefaultFilterJointDef();

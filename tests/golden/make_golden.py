@@ -291,7 +291,7 @@ OPS = {"apply_force": 1, "apply_force_to_center": 2, "apply_torque": 3, "apply_l
        "prismatic_target": 27, "create_weld": 28, "weld_config": 29, "create_wheel": 30, "wheel_config": 31,
        "create_distance": 32, "distance_config": 33, "distance_length": 34, "revolute_config": 35,
        "revolute_target": 36, "create_motor": 37, "motor_config": 38, "create_mouse": 39, "mouse_config": 40,
-       "create_filter": 41}
+       "create_filter": 41, "destroy_joint": 42}
 
 
 def ops_array(rows):
@@ -512,6 +512,24 @@ MOTOR_MOUSE_OPS = [
     (40, "mouse_config", 2, 0, [1.0, 4.0, 1.0, 0.0, 50.0]),
 ]
 
+# motor_mouse_world with joints destroyed: a mouse joint released and another one
+# made (it takes the freed id), a motor and the filter joint destroyed (the filtered
+# boxes then collide), a joint made after them
+DESTROY_OPS = [
+    (1, "create_motor", 0, 1, [0.5, 2.0, 0.8, 30.0, 5.0, 0.3]),               # 0
+    (1, "create_motor", 2, 3, [0.0, 1.0, 0.0, 8.0, 1.0, 0.5]),                # 1
+    (1, "create_mouse", 0, 4, [3.0, 0.5, 5.0, 0.7, 500.0]),                   # 2
+    (1, "create_filter", 6, 7, []),                                           # 3
+    (1, "create_mouse", 0, 5, [6.0, 3.0, 2.0, 0.2, 30.0]),                    # 4
+    (10, "mouse_config", 2, 0, [5.0, 3.0, 5.0, 0.7, 500.0]),
+    (20, "destroy_joint", 2, 0, []),                                          # mouse released
+    (25, "create_mouse", 0, 4, [2.0, 4.0, 5.0, 0.7, 500.0]),                  # 5: takes id 2
+    (30, "destroy_joint", 1, 0, []),                                          # motor on the carrier
+    (35, "destroy_joint", 3, 0, []),                                          # filter: 6 and 7 collide
+    (40, "create_motor", 2, 3, [0.0, 1.2, 0.0, 8.0, 1.0, 0.5]),              # 6: takes id 3
+    (45, "destroy_joint", 0, 0, []),
+]
+
 # the revolute chain of body_ops_world (joints 0..2): motor, limits, spring and target
 # changed through the revolute joint calls
 REVOLUTE_OPS = [
@@ -531,6 +549,7 @@ OPS_FIXTURES = {
     "ops_distance": (distance_world, DISTANCE_OPS, list(range(0, 61))),
     "ops_revolute": (body_ops_world, REVOLUTE_OPS, list(range(0, 41))),
     "ops_motor_mouse": (motor_mouse_world, MOTOR_MOUSE_OPS, list(range(0, 61))),
+    "ops_destroy_joint": (motor_mouse_world, DESTROY_OPS, list(range(0, 61))),
 }
 
 

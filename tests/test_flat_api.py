@@ -438,6 +438,32 @@ def test_motor_mouse_joint_calls():
     fw.close()
 
 
+def test_destroy_joint_on_the_host():
+    """f2d_destroy_joint before any step: the joint's slot is freed (id -1 in the
+    snapshot), the colour's last joint takes its local index (S/constraint_graph.c:
+    292-322), its bodies' joint lists drop it, the next joint created takes the freed id
+    with a new generation (the old handle is invalid), as in the reference's fixture
+    ops_destroy_joint (step 21: the joints of colour 0 at local indices 0, 1, -, 3, 2)."""
+    spec, snaps, ops = load_ops_fixture("ops_destroy_joint", GOLDEN)
+    fw = FlatWorld(spec)
+    for op in ops[ops["step"] == 1]:
+        fw.apply(op)
+    L = F.lib()
+    mouse = fw.joints[2]
+    L.f2d_destroy_joint(*mouse)
+    assert L.f2d_joint_is_valid(*mouse) == 0
+    assert L.f2d_body_get_joint_count(*fw.bodies[4]) == 0
+    snap = fw.snapshot()
+    ref = snaps[21].joints
+    assert snap.joints["id"].tolist() == ref["id"].tolist() == [0, 1, -1, 3, 4]
+    assert snap.joints["local_index"].tolist() == ref["local_index"].tolist()
+    out = (ctypes.c_int32 * 2)()
+    L.f2d_create_mouse_joint(fw.w, *fw.bodies[0], *fw.bodies[4], 2.0, 4.0, 5.0, 0.7, 500.0, 0, out)
+    assert out[0] == 3 and (out[1] >> 16) == 2
+    assert L.f2d_joint_is_valid(*mouse) == 0 and L.f2d_joint_is_valid(out[0], out[1]) == 1
+    fw.close()
+
+
 def test_joint_calls_on_other_types_are_ignored():
     """A joint parameter call of the wrong joint type leaves the joint alone and its
     getter returns 0, as the reference's type check would refuse it."""

@@ -158,6 +158,21 @@ def test_motor_mouse_filter_joints_match_reference(gpu):
     fw.close()
 
 
+def test_destroy_joint_matches_reference(gpu):
+    """f2d_destroy_joint (S/joint.c:702-819; fixture ops_destroy_joint: a mouse joint
+    released and a new one made in its freed id, a motor and the filter joint destroyed,
+    so the filtered boxes collide, a motor made in the next freed id, the first motor
+    destroyed): after every one of 60 steps bodies, contacts, colours, islands and every
+    joint slot (freed ones included) equal the reference's bit for bit; a handle to a
+    destroyed joint is invalid once its id is reused."""
+    fw, snaps = replay_joint_fixture("ops_destroy_joint", 5)
+    import forge2d_amd as F
+    L = F.lib()
+    assert [L.f2d_joint_is_valid(*j) for j in fw.joints] == [0, 0, 0, 0, 1, 1, 1]
+    assert [L.f2d_joint_get_type(*j) for j in fw.joints[4:]] == [3, 3, 2]
+    fw.close()
+
+
 def test_joint_between_touching_bodies_refused(gpu):
     """A joint with collideConnected off between bodies that share a contact would have
     the reference destroy that contact (S/joint.c:313-345); that edit is not restated,
