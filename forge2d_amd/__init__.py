@@ -225,6 +225,7 @@ SIGNATURES = [
     ("f2d_create_mouse_joint", None, [_U, ctypes.c_int32, _U, ctypes.c_int32, _U, _F, _F, _F, _F, _F, _I,
                                       ctypes.POINTER(ctypes.c_int32)]),
     ("f2d_destroy_joint", None, [ctypes.c_int32, _U]),
+    ("f2d_joint_set_collide_connected", None, [ctypes.c_int32, _U, _I]),
     ("f2d_create_filter_joint", None, [_U, ctypes.c_int32, _U, ctypes.c_int32, _U, ctypes.POINTER(ctypes.c_int32)]),
     ("f2d_motor_joint_get_linear_offset", None, [ctypes.c_int32, _U, ctypes.POINTER(ctypes.c_float)]),
     ("f2d_motor_joint_set_linear_offset", None, [ctypes.c_int32, _U, _F, _F]),

@@ -3675,6 +3675,42 @@ void f2d_destroy_joint(int32_t index1, uint32_t wg) {
   bodyBatch(wg)->hostDirty = true;
 }
 
+static bool refuseContactsBetween(const HostWorld& hw, int a, int b, int collideConnected);
+
+// f2d_shim.c:932-935 (b2Joint_SetCollideConnected, S/joint.c:979-1022): turned on, the
+// shapes of the body with fewer shapes (body B on a tie) enter the move buffer so the
+// next step looks for their pairs; turned off, the contacts between the bodies are
+// destroyed in the reference, which is not restated: refused while they share one.
+void f2d_joint_set_collide_connected(int32_t index1, uint32_t wg, int v) {
+  int id;
+  HostWorld* hw = jointWorld(index1, wg, &id);
+  if (!hw) return;
+  Joint& j = hw->joints[id];
+  const int should = v != 0 ? 1 : 0;
+  if (j.collideConnected == should) return;
+  if (!should) {
+    if (refuseContactsBetween(*hw, j.bodyA, j.bodyB, 0)) return;
+    j.collideConnected = 0;
+    bodyBatch(wg)->hostDirty = true;
+    return;
+  }
+  j.collideConnected = 1;
+  auto shapeCount = [&](int body) {
+    int n = 0;
+    for (int s = hw->sims[body].headShape, g = 0; s >= 0 && g < (int)hw->shapes.size(); s = hw->shapes[s].next, ++g) ++n;
+    return n;
+  };
+  const int body = shapeCount(j.bodyA) < shapeCount(j.bodyB) ? j.bodyA : j.bodyB;
+  for (int s = hw->sims[body].headShape, g = 0; s >= 0 && g < (int)hw->shapes.size(); s = hw->shapes[s].next, ++g) {
+    Shape& shape = hw->shapes[s];
+    if (shape.proxyKey >= 0 && !(shape.flags & SF_MOVED)) {  // b2BufferMove
+      shape.flags |= SF_MOVED;
+      shape.moveRank = (1 << 30) + hw->createdSeq++;
+    }
+  }
+  bodyBatch(wg)->hostDirty = true;
+}
+
 int f2d_joint_is_valid(int32_t index1, uint32_t wg) {
   int id;
   return jointWorld(index1, wg, &id) ? 1 : 0;

@@ -211,6 +211,12 @@ void f2d_shape_get_aabb(int32_t index1, uint32_t world_and_generation, float* ou
  * sleeping set or between two static bodies is left alone with a message on stderr. */
 void f2d_destroy_joint(int32_t index1, uint32_t world_and_generation);
 
+/* F/native/wasm/f2d_shim.c:932-935 (b2Joint_SetCollideConnected, S/joint.c:979-1022):
+ * turning it on buffers the shapes of the body with fewer shapes as moved; turning it
+ * off between bodies that share a contact is refused with a message on stderr (the
+ * reference destroys those contacts, which is not restated here). */
+void f2d_joint_set_collide_connected(int32_t index1, uint32_t world_and_generation, int v);
+
 /* F/native/wasm/f2d_shim.c:816-843 (b2CreateRevoluteJoint).
  * Every joint creation call with collide_connected = 0 (revolute, prismatic, weld,
  * wheel, distance, motor) between bodies that already share a contact is refused: out

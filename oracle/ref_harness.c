@@ -1163,6 +1163,9 @@ static void apply_op( const OpRec* o, const b2BodyId* bodies, const b2ShapeId* s
 		case 42:
 			b2DestroyJoint( joints[o->target] );
 			break;
+		case 43:
+			b2Joint_SetCollideConnected( joints[o->target], o->flag != 0 );
+			break;
 		case 41:
 		{
 			b2FilterJointDef def = b2DefaultFilterJointDef();

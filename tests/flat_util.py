@@ -115,7 +115,7 @@ class FlatWorld:
         L = F.lib()
         code, t, flag = int(op["op"]), int(op["target"]), int(op["flag"])
         f = [float(x) for x in op["f"]]
-        bi, bwg = self.bodies[t] if code not in (12, 13, 14, 23, 26, 27, 29, 31, 33, 34, 35, 36, 38, 40, 42) \
+        bi, bwg = self.bodies[t] if code not in (12, 13, 14, 23, 26, 27, 29, 31, 33, 34, 35, 36, 38, 40, 42, 43) \
             else (0, 0)
         if code == 1:
             L.f2d_body_apply_force(bi, bwg, f[0], f[1], f[2], f[3], flag)
@@ -258,6 +258,8 @@ class FlatWorld:
             self.joints.append((out[0], ctypes.c_uint32(out[1]).value))
         elif code == 42:
             L.f2d_destroy_joint(*self.joints[t])
+        elif code == 43:
+            L.f2d_joint_set_collide_connected(*self.joints[t], int(flag))
         elif code == 38:
             j = self.joints[t]
             L.f2d_motor_joint_set_linear_offset(*j, f[0], f[1])

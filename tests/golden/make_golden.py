@@ -291,7 +291,7 @@ OPS = {"apply_force": 1, "apply_force_to_center": 2, "apply_torque": 3, "apply_l
        "prismatic_target": 27, "create_weld": 28, "weld_config": 29, "create_wheel": 30, "wheel_config": 31,
        "create_distance": 32, "distance_config": 33, "distance_length": 34, "revolute_config": 35,
        "revolute_target": 36, "create_motor": 37, "motor_config": 38, "create_mouse": 39, "mouse_config": 40,
-       "create_filter": 41, "destroy_joint": 42}
+       "create_filter": 41, "destroy_joint": 42, "collide_connected": 43}
 
 
 def ops_array(rows):
@@ -530,6 +530,18 @@ DESTROY_OPS = [
     (45, "destroy_joint", 0, 0, []),
 ]
 
+# motor_mouse_world: a box pulled into its carrier by a motor joint without collision,
+# then collideConnected turned on (the box's shape enters the move buffer and the two
+# collide); a joint between distant boxes turned off and on again
+COLLIDE_CONNECTED_OPS = [
+    (1, "create_motor", 2, 3, [0.0, 1.0, 0.0, 8.0, 1.0, 0.5]),                # 0
+    (1, "create_motor", 1, 4 | (1 << 16), [9.0, 0.0, 0.0, 0.0, 0.0, 0.0]),   # 1: no force
+    (10, "motor_config", 0, 0, [0.0, 0.55, 0.0, 8.0, 1.0, 0.5]),
+    (20, "collide_connected", 0, 1, []),
+    (30, "collide_connected", 1, 0, []),
+    (40, "collide_connected", 1, 1, []),
+]
+
 # the revolute chain of body_ops_world (joints 0..2): motor, limits, spring and target
 # changed through the revolute joint calls
 REVOLUTE_OPS = [
@@ -550,6 +562,7 @@ OPS_FIXTURES = {
     "ops_revolute": (body_ops_world, REVOLUTE_OPS, list(range(0, 41))),
     "ops_motor_mouse": (motor_mouse_world, MOTOR_MOUSE_OPS, list(range(0, 61))),
     "ops_destroy_joint": (motor_mouse_world, DESTROY_OPS, list(range(0, 61))),
+    "ops_collide_connected": (motor_mouse_world, COLLIDE_CONNECTED_OPS, list(range(0, 61))),
 }
 
 

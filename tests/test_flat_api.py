@@ -464,6 +464,18 @@ def test_destroy_joint_on_the_host():
     fw.close()
 
 
+def test_collide_connected_fixture_creates_the_pair():
+    """In ops_collide_connected the carrier and the box its motor joint pulls in (shapes
+    2 and 3) share no contact until collideConnected is turned on before step 20, and
+    share one from then on."""
+    spec, snaps, ops = load_ops_fixture("ops_collide_connected", GOLDEN)
+    def touching(st):
+        pairs = {(int(c["shape_a"]), int(c["shape_b"])) for c in snaps[st].live_contacts()}
+        return (2, 3) in pairs or (3, 2) in pairs
+    assert not any(touching(st) for st in range(1, 20))
+    assert all(touching(st) for st in range(20, 61))
+
+
 def test_joint_calls_on_other_types_are_ignored():
     """A joint parameter call of the wrong joint type leaves the joint alone and its
     getter returns 0, as the reference's type check would refuse it."""

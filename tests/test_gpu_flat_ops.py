@@ -173,6 +173,19 @@ def test_destroy_joint_matches_reference(gpu):
     fw.close()
 
 
+def test_collide_connected_matches_reference(gpu):
+    """f2d_joint_set_collide_connected (S/joint.c:979-1022; fixture ops_collide_connected:
+    a box pulled into its carrier by a motor joint while they do not collide, then
+    collideConnected turned on, so the box's shape enters the move buffer and the pair
+    is created in the next step; a joint between distant boxes turned off and on):
+    after every one of 60 steps the world equals the reference's bit for bit."""
+    fw, snaps = replay_joint_fixture("ops_collide_connected", 2)
+    import forge2d_amd as F
+    L = F.lib()
+    assert [L.f2d_joint_get_collide_connected(*j) for j in fw.joints] == [1, 1]
+    fw.close()
+
+
 def test_joint_between_touching_bodies_refused(gpu):
     """A joint with collideConnected off between bodies that share a contact would have
     the reference destroy that contact (S/joint.c:313-345); that edit is not restated,
